@@ -1,0 +1,211 @@
+/*
+ * nairnmpm_gpu.h -- C ABI of the MI355X explicit MPM time step.
+ *
+ * This is the drop-in boundary for NairnMPM's explicit step. The reference runs the step as a
+ * linked list of MPMTask objects whose `virtual bool Execute(int)` works on process globals
+ * (NairnMPM/src/NairnMPM_Class/MPMTask.hpp:20-47, Common/System/CommonTask.hpp:26,
+ * NairnMPM/src/NairnMPM_Class/NairnMPM.cpp:284-335). Every entry point below replaces one piece
+ * of that interface and takes plain pointers and sizes only: no C++ types, no exceptions.
+ *
+ * Status convention: 0 = success; non-zero = failure, with a message from mpmgpu_last_error()
+ * (per context) or mpmgpu_global_error() (calls that have no context yet). This replaces the
+ * reference's CommonException throw/catch (NairnMPM.cpp:202-225).
+ *
+ * Units are the reference's Legacy units after input scaling (mm, g, s; stress in Pa, stored
+ * "specific", i.e. divided by rho0, as NairnMPM does).
+ */
+#ifndef NAIRNMPM_GPU_H
+#define NAIRNMPM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* analysis types: same numbers as the reference (NairnMPM/src/System/MPMPrefix.hpp:114-115) */
+enum { MPMGPU_PLANE_STRAIN = 10, MPMGPU_PLANE_STRESS = 11, MPMGPU_THREED = 12 };
+
+/* shape functions: same numbers as ElementBase::useGimp (MPMPrefix.hpp:127-137) */
+enum { MPMGPU_SHAPE_LINEAR = 0, MPMGPU_SHAPE_UGIMP = 1, MPMGPU_SHAPE_LCPDI = 10 };
+
+/* material types: the reference's XML Type= numbers (Read_MPM/MaterialController.cpp:105-190) */
+enum {
+    MPMGPU_MAT_ISOTROPIC = 1,
+    MPMGPU_MAT_MOONEY = 8,
+    MPMGPU_MAT_ISOPLASTICITY = 9,
+    MPMGPU_MAT_NEOHOOKEAN = 28,
+    MPMGPU_MAT_CONTACT = 60,
+    MPMGPU_MAT_COULOMB = 61
+};
+
+/* per-particle FP64 fields of the structure-of-arrays layout; a host buffer holds
+   MPF_NUM_FIELDS consecutive arrays of n doubles. Tensor order is the reference's
+   Tensor (xx,yy,zz,yz,xz,xy) and TensorAntisym (yz,xz,xy) (Common/System/DataTypes.hpp:41-56). */
+enum {
+    MPF_POS_X = 0, MPF_POS_Y, MPF_POS_Z,
+    MPF_VEL_X, MPF_VEL_Y, MPF_VEL_Z,
+    MPF_MASS,
+    MPF_LP_X, MPF_LP_Y, MPF_LP_Z,                    /* dimensionless half size (mpm_lp) */
+    MPF_EP_XX, MPF_EP_YY, MPF_EP_ZZ, MPF_EP_YZ, MPF_EP_XZ, MPF_EP_XY,   /* ep: encodes F */
+    MPF_WR_YZ, MPF_WR_XZ, MPF_WR_XY,                 /* wrot: encodes F */
+    MPF_SP_XX, MPF_SP_YY, MPF_SP_ZZ, MPF_SP_YZ, MPF_SP_XZ, MPF_SP_XY,   /* specific stress */
+    MPF_PRESSURE,
+    MPF_ALT_XX, MPF_ALT_YY, MPF_ALT_ZZ, MPF_ALT_YZ, MPF_ALT_XZ, MPF_ALT_XY, /* eplast or B */
+    MPF_HIST0, MPF_HIST1,                            /* J2: alpha; hyperelastic: J, Jres */
+    MPF_ORIG_X, MPF_ORIG_Y, MPF_ORIG_Z,              /* origpos */
+    MPF_NUM_FIELDS
+};
+
+/* per-particle int32 fields */
+enum {
+    MPI_ELEM = 0,      /* zero-based element (reference inElem-1) */
+    MPI_MAT,           /* zero-based material (reference matnum-1) */
+    MPI_ID,            /* creation index (reference mpm[] order) */
+    MPI_LEFT,          /* 1 once the particle has left the grid (HasLeftTheGridBefore) */
+    MPI_NUM_FIELDS
+};
+
+/* Regular structured grid, after the one-cell GIMP pad per side
+   (Read_MPM/Generators.cpp:1760-1833). Node i on x is at xmin + i*dx. */
+typedef struct MpmGridDesc {
+    int32_t analysis;          /* MPMGPU_PLANE_STRAIN or MPMGPU_THREED */
+    int32_t nx, ny, nz;        /* cells per axis (nz = 0 in 2D) */
+    double xmin, ymin, zmin;   /* padded origin (mm) */
+    double dx, dy, dz;         /* (max-min)/n per axis (mm) */
+    double thickness;          /* 2D out-of-plane thickness (mm) */
+} MpmGridDesc;
+
+#define MPMGPU_MAX_MATERIALS 16
+
+/* Material constants after VerifyAndLoadProperties, in the reference's specific units. */
+typedef struct MpmMaterial {
+    int32_t type;              /* MPMGPU_MAT_* */
+    int32_t field;             /* velocity field (multimaterial mode), 0 otherwise */
+    double rho;                /* g/mm^3 */
+    double E, nu, G, K;        /* Pa (as entered, after unit scaling and completion) */
+    double C[6][6];            /* ElasticProperties::C, specific (Common/Materials/Elastic.cpp:43-150) */
+    double Gred, Kred;         /* IsoPlasticity reduced moduli (IsoPlasticity.cpp:60-66) */
+    double yield, Ep;          /* Pa */
+    double yldred, yldredMin, Epred, alphaMax;   /* LinearHardening (LinearHardening.cpp:56-76) */
+    double Gsp, Lamesp, Ksp;   /* Neohookean specific (Neohookean.cpp:135-138) */
+    double G1sp, G2sp;         /* Mooney specific */
+    double waveSpeed;          /* mm/s, used for the CFL time step */
+    double friction;           /* CoulombFriction coefficient (contact laws only) */
+    int32_t ujOption;          /* HyperElastic UofJOption */
+    int32_t pad;
+} MpmMaterial;
+
+/* Step parameters (NairnMPM.hpp:23 globals timestep, strainTimestepFirst/Last) */
+typedef struct MpmRunParams {
+    double timestep;           /* s */
+    double strainFirst;        /* USAVG first half: fractionUSF*timestep */
+    double strainLast;         /* timestep - strainFirst */
+    int32_t shape;             /* MPMGPU_SHAPE_* */
+    int32_t mpmMethod;         /* 2 = USAVG (only method of the hot path) */
+    int32_t multiMaterial;     /* 0 single-material mode */
+    int32_t contactLaw;        /* material index of the default contact law, -1 none */
+    int32_t normalsMethod;     /* MultiMaterialMode Normals= */
+    int32_t deleteLeaving;     /* fmobj->deleteLeavingParticles */
+    int32_t maxLeaving;        /* warnings limit before abort (0 = never abort) */
+    int32_t pad;
+} MpmRunParams;
+
+/* Counters reported back from the device (ResetElementsTask.cpp:71-151 warnings) */
+typedef struct MpmStats {
+    int64_t steps;             /* steps run so far */
+    int64_t nparticles;
+    int64_t leftGrid;          /* particles that left the grid (first time) */
+    int64_t nanParticles;      /* particles with a NaN position */
+    double time;               /* simulation time (s) */
+    double lastStepMs;         /* device time of the last timed step (ms) */
+} MpmStats;
+
+/* Task ids for mpmgpu_run_task: one per hot-path MPMTask (NairnMPM.cpp:952-1108) */
+enum {
+    MPMGPU_TASK_INITIALIZATION = 0,     /* InitializationTask.cpp:43 */
+    MPMGPU_TASK_MASS_MOMENTUM,          /* MassAndMomentumTask.cpp:48 */
+    MPMGPU_TASK_POST_EXTRAPOLATION,     /* PostExtrapolationTask.cpp:46 */
+    MPMGPU_TASK_UPDATE_STRAINS_FIRST,   /* UpdateStrainsFirstTask.cpp:52 */
+    MPMGPU_TASK_GRID_FORCES,            /* GridForcesTask.cpp:40 */
+    MPMGPU_TASK_POST_FORCES,            /* PostForcesTask.cpp:43 */
+    MPMGPU_TASK_UPDATE_MOMENTA,         /* UpdateMomentaTask.cpp:44 */
+    MPMGPU_TASK_UPDATE_PARTICLES,       /* UpdateParticlesTask.cpp:44 */
+    MPMGPU_TASK_UPDATE_STRAINS_LAST,    /* UpdateStrainsLastContactTask.cpp:60 */
+    MPMGPU_TASK_RESET_ELEMENTS,         /* ResetElementsTask.cpp:44 */
+    MPMGPU_NUM_TASKS
+};
+
+typedef struct MpmProblem MpmProblem;   /* host-side problem (input + generated particles) */
+typedef struct MpmCtx MpmCtx;           /* device context: all device buffers of one simulation */
+
+/* ---- input boundary: the reference's XML input (MPMReadHandler) ---------------------- */
+
+/* Build a problem in code, element by element, as the XML reader does:
+   <Header><Analysis> (MPMGPU_PLANE_STRAIN or MPMGPU_THREED) ...
+   Replaces MPMReadHandler::myStartElement for the hot-path elements
+   (Read_MPM/MPMReadHandler.cpp:83-700). */
+int mpmgpu_problem_create(int32_t analysis, MpmProblem **out);
+/* <Grid xmin= xmax= ymin= ymax= zmin= zmax=> with <Horiz cellsize=> etc.; cell3[a] <= 0 means
+   the count n3[a] is used instead. Pads one cell per side (Read_MPM/Generators.cpp:1675-1833). */
+int mpmgpu_problem_set_grid(MpmProblem *prob, const double *ext6, const double *cell3,
+                            const int32_t *n3, double thickness);
+/* <MPMHeader> settings: "MatlPtsPerElement", "CFL", "TimeStep" (s), "MaxTime" (s),
+   "GIMP" (MPMGPU_SHAPE_*) */
+int mpmgpu_problem_set(MpmProblem *prob, const char *key, double value);
+/* <Material Type= Name=> (Read_MPM/MaterialController.cpp:105-190); id is 1-based */
+int mpmgpu_problem_add_material(MpmProblem *prob, int32_t type, const char *name, int32_t *id);
+/* one property element of a material (<E>, <nu>, <G>, <K>, <rho>, <yield>, <Ep>, ...), XML units */
+int mpmgpu_problem_material_property(MpmProblem *prob, int32_t id, const char *prop, double value);
+/* <Body mat= vx= vy= vz= thick=> holding one shape: "Box", "Cylinder", "Rect", "Oval", "Sphere"
+   (MPMReadHandler::MPMPts, Read_MPM/Generators.cpp:1423-1545) */
+int mpmgpu_problem_add_body(MpmProblem *prob, int32_t mat, const char *shape, const double *ext6,
+                            int32_t axis, const double *vel3, double thick);
+/* explicit <mp matl= lpx= lpy= lpz=><pt/><vel/><mass/></mp> (MPMReadHandler.cpp:1159-1290);
+   mass < 0 means rho*volume */
+int mpmgpu_problem_add_point(MpmProblem *prob, int32_t mat, const double *pos3, const double *vel3,
+                             const double *lp3, double thick, double mass);
+
+/* Parse an XML input file's text (the NairnMPM.dtd subset of the hot path) and generate the
+   grid and particles, then do PreliminaryParticleCalcs and CFLTimeStep.
+   Replaces CommonAnalysis::ReadFile (Common/System/CommonAnalysis.cpp:184-273) +
+   MPMReadHandler::grid/MPMPts (Read_MPM/Generators.cpp:1423,1675) +
+   NairnMPM::PreliminaryParticleCalcs/CFLTimeStep (NairnMPM.cpp:589-730,1207-1240). */
+int mpmgpu_problem_from_xml(const char *xml_text, MpmProblem **out);
+/* Grid, materials and step parameters of a problem */
+int mpmgpu_problem_describe(const MpmProblem *prob, MpmGridDesc *grid, MpmMaterial *mats,
+                            int32_t *nmat, MpmRunParams *run, int64_t *nparticles,
+                            double *maxTime);
+/* Initial particle state in mpm[] order: dfields is MPF_NUM_FIELDS*n doubles, ifields is
+   MPI_NUM_FIELDS*n int32 */
+int mpmgpu_problem_particles(const MpmProblem *prob, double *dfields, int32_t *ifields);
+void mpmgpu_problem_destroy(MpmProblem *prob);
+
+/* ---- device context (NairnMPM::CMPreparations/CMAnalysis state, NairnMPM.cpp:135-280) -- */
+
+/* Create a device context for one simulation on the current HIP device. */
+int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat,
+                  const MpmRunParams *run, MpmCtx **out);
+/* Upload particles (layout as mpmgpu_problem_particles); replaces the mpm[] globals
+   (MPM_Classes/MPMBase.hpp:269-270). */
+int mpmgpu_upload_particles(MpmCtx *ctx, int64_t n, const double *dfields, const int32_t *ifields);
+/* Run one hot-path MPMTask's Execute() on the device (MPMGPU_TASK_*), in reference order. */
+int mpmgpu_run_task(MpmCtx *ctx, int32_t task);
+/* Run nsteps whole steps (NairnMPM::MPMStep, NairnMPM.cpp:284-335), fused and graph-replayed. */
+int mpmgpu_step(MpmCtx *ctx, int32_t nsteps);
+/* Wait for all queued device work of this context. */
+int mpmgpu_sync(MpmCtx *ctx);
+/* Download particles in creation (mpm[]) order. */
+int mpmgpu_download_particles(MpmCtx *ctx, double *dfields, int32_t *ifields);
+/* Counters and timings (MPMTask::TrackTimes, MPMTask.cpp:51-72; warnings). */
+int mpmgpu_query(MpmCtx *ctx, MpmStats *stats);
+/* Grid node values (mass, momentum, force) after the last task, nodes numbered x fastest. */
+int mpmgpu_download_grid(MpmCtx *ctx, double *mass, double *pk, double *ftot);
+const char *mpmgpu_last_error(const MpmCtx *ctx);
+const char *mpmgpu_global_error(void);
+void mpmgpu_destroy(MpmCtx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,515 @@
+// mpm_ctx.hip -- the C ABI: problem building (input boundary) and the device context that runs
+// the step. The context owns every device buffer of one simulation; all work goes on its own
+// stream (default priority), and whole steps are replayed from a captured HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpm_kernels.h"
+#include "mpm_problem.h"
+
+namespace {
+thread_local std::string g_err;
+
+int fail_global(const std::string &m) {
+    g_err = m;
+    return 1;
+}
+}  // namespace
+
+struct MpmCtx {
+    int dim = 3;
+    MpmGridDesc gd{};
+    MpmRunParams run{};
+    std::vector<MpmMaterial> mats;
+    KGrid kg{};
+    KPart kp{};
+    KBins kb{};
+    KNodes kn{};
+    KStats ks{};
+    MpmMaterial *dmats = nullptr;
+    double *pbuf = nullptr;           // particle double fields + second position buffer
+    int *ibuf = nullptr;              // particle int fields
+    int *bbuf = nullptr;              // binning arrays
+    unsigned long long *sbuf = nullptr;
+    long long ncap = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // two-step graphs, by position parity
+    int parity = 0;                  // which position buffer holds the current positions
+    double *posA[3] = {nullptr, nullptr, nullptr}, *posB[3] = {nullptr, nullptr, nullptr};
+    int nextTask = 0;                // task sequence check for mpmgpu_run_task
+    int64_t steps = 0;
+    double mtime = 0.;
+    float lastStepMs = 0.f;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+
+    int fail(const std::string &m) {
+        err = m;
+        return 1;
+    }
+};
+
+#define HIPCHECK(ctx, call)                                                               \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return (ctx)->fail(std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+static void set_positions(MpmCtx *c) {
+    double **cur = c->parity == 0 ? c->posA : c->posB;
+    double **nxt = c->parity == 0 ? c->posB : c->posA;
+    for (int a = 0; a < 3; a++) {
+        c->kp.f[MPF_POS_X + a] = cur[a];
+        c->kp.posn[a] = nxt[a];
+    }
+}
+
+static void destroy_graphs(MpmCtx *c) {
+    for (int i = 0; i < 2; i++)
+        if (c->graph[i]) { (void)hipGraphExecDestroy(c->graph[i]); c->graph[i] = nullptr; }
+}
+
+// the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108)
+static void enqueue_step(MpmCtx *c) {
+    hipStream_t s = c->stream;
+    set_positions(c);
+    launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+    launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
+    launch_usf_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+    launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
+    launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s);
+    launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
+    launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
+    launch_bin_finish(c->dim, c->kg, c->kp, c->kb, s);
+    c->parity ^= 1;
+}
+
+extern "C" {
+
+const char *mpmgpu_global_error(void) { return g_err.c_str(); }
+const char *mpmgpu_last_error(const MpmCtx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+// ---- problem building ------------------------------------------------------------------
+
+int mpmgpu_problem_create(int32_t analysis, MpmProblem **out) {
+    if (analysis != MPMGPU_PLANE_STRAIN && analysis != MPMGPU_THREED)
+        return fail_global("analysis type " + std::to_string(analysis) + " is not supported (10 or 12)");
+    MpmProblem *p = new MpmProblem();
+    p->analysis = analysis;
+    p->dim = analysis == MPMGPU_THREED ? 3 : 2;
+    if (p->dim == 3) p->ptsPerSide = 2;
+    p->run.shape = MPMGPU_SHAPE_UGIMP;
+    p->run.mpmMethod = 2;
+    p->run.contactLaw = -1;
+    *out = p;
+    return 0;
+}
+
+// <Grid xmin= xmax= ...> with <Horiz cellsize=>/<Vert>/<Depth> (or nx,ny,nz counts when the
+// cell size is <= 0)
+int mpmgpu_problem_set_grid(MpmProblem *p, const double *ext6, const double *cell3, const int32_t *n3,
+                            double thickness) {
+    p->haveGrid = true;
+    p->gxmin = ext6[0]; p->gxmax = ext6[1]; p->gymin = ext6[2]; p->gymax = ext6[3];
+    p->gzmin = ext6[4]; p->gzmax = ext6[5];
+    p->cellx = cell3[0]; p->celly = cell3[1]; p->cellz = cell3[2];
+    p->nhoriz = n3 ? n3[0] : 0; p->nvert = n3 ? n3[1] : 0; p->ndepth = n3 ? n3[2] : 0;
+    if (thickness > 0.) p->thickness = thickness;
+    p->gridBuilt = false;
+    if (!mpmsetup::build_grid(*p)) return fail_global(p->err);
+    return 0;
+}
+
+// MPMHeader settings by element name: MatlPtsPerElement, CFL (Timing CFL=), TimeStep, MaxTime,
+// GIMP (1 uGIMP, 0 classic, 10 lCPDI)
+int mpmgpu_problem_set(MpmProblem *p, const char *key, double value) {
+    std::string k(key);
+    if (k == "MatlPtsPerElement") {
+        int n = (int)(value + 0.5);
+        int side = p->dim == 3 ? (int)(cbrt((double)n) + 0.5) : (int)(sqrt((double)n) + 0.5);
+        if ((p->dim == 3 && side * side * side != n) || (p->dim == 2 && side * side != n))
+            return fail_global("MatlPtsPerElement must be a square (2D) or cube (3D)");
+        p->ptsPerSide = side;
+    } else if (k == "CFL") p->cfl = value;
+    else if (k == "TimeStep") p->timestepIn = value;
+    else if (k == "MaxTime") p->maxTime = value;
+    else if (k == "GIMP") p->run.shape = (int)value;
+    else return fail_global("unknown setting " + k);
+    return 0;
+}
+
+// <Material Type= Name=> (Read_MPM/MaterialController.cpp:105-190); returns the 1-based id
+int mpmgpu_problem_add_material(MpmProblem *p, int32_t type, const char *name, int32_t *id) {
+    MaterialInput m;
+    m.type = type;
+    m.name = name ? name : "";
+    mpmsetup::add_material(*p, m);
+    *id = (int32_t)p->matIn.size();
+    p->finalized = false;
+    return 0;
+}
+
+// one property element of a <Material> (<E>, <nu>, <G>, <K>, <rho>, <yield>, <Ep>, ...), in
+// the XML's units
+int mpmgpu_problem_material_property(MpmProblem *p, int32_t id, const char *prop, double value) {
+    if (id < 1 || id > (int)p->matIn.size()) return fail_global("bad material id");
+    MaterialInput &m = p->matIn[id - 1];
+    std::string k(prop);
+    if (k == "rho") m.rho = value;
+    else if (k == "E" || k == "Etens") { m.E = value; m.readE = true; }
+    else if (k == "nu") { m.nu = value; m.readNu = true; }
+    else if (k == "G" || k == "G1") { m.G = value; m.readG = true; }
+    else if (k == "K" || k == "Kbulk") m.K = value;
+    else if (k == "Lame") m.Lame = value;
+    else if (k == "yield") m.yield = value;
+    else if (k == "yieldMin") m.yieldMin = value;
+    else if (k == "Ep") m.Ep = value;
+    else if (k == "Khard") m.Khard = value;
+    else if (k == "coeff") m.friction = value;
+    else if (k == "UJOption") m.ujOption = (int)value;
+    else if (k == "alpha" || k == "beta" || k == "Cv" || k == "kCond" || k == "D" || k == "csat") {
+        if (value != 0. && (k == "alpha" || k == "beta")) {
+            // thermal and moisture expansion only act with transport tasks, which are not on
+            // this path; the expansion coefficients are accepted and unused
+        }
+    } else return fail_global("material property " + k + " is not supported on this path");
+    p->finalized = false;
+    return 0;
+}
+
+// <Body mat= vx= vy= vz=> with one shape (<Box>/<Cylinder axis=>, <Rect>, <Oval>, <Sphere>)
+int mpmgpu_problem_add_body(MpmProblem *p, int32_t mat, const char *shape, const double *ext6,
+                            int32_t axis, const double *vel3, double thick) {
+    std::string s(shape);
+    int kind;
+    if (s == "Box" || s == "Cylinder") kind = 0;
+    else if (s == "Rect") kind = 1;
+    else if (s == "Oval") kind = 2;
+    else if (s == "Sphere") kind = 3;
+    else return fail_global("shape " + s + " is not supported on this path");
+    if (mat < 1) return fail_global("Positive material ID needed in <Body> element.");
+    if (!mpmsetup::add_body(*p, mat, kind, ext6, s == "Cylinder" && axis == 0 ? 3 : axis, vel3, thick))
+        return fail_global(p->err);
+    p->finalized = false;
+    return 0;
+}
+
+// explicit <mp> particle with <pt>, <vel> (and optional mass, <0 for rho*volume)
+int mpmgpu_problem_add_point(MpmProblem *p, int32_t mat, const double *pos3, const double *vel3,
+                             const double *lp3, double thick, double mass) {
+    if (!mpmsetup::add_point(*p, mat, pos3, vel3, lp3, thick, mass)) return fail_global(p->err);
+    p->finalized = false;
+    return 0;
+}
+
+int mpmgpu_problem_from_xml(const char *xml_text, MpmProblem **out) {
+    MpmProblem *p = new MpmProblem();
+    if (!mpm_read_xml(xml_text, *p)) {
+        g_err = p->err;
+        delete p;
+        return 1;
+    }
+    *out = p;
+    return 0;
+}
+
+int mpmgpu_problem_describe(const MpmProblem *pc, MpmGridDesc *grid, MpmMaterial *mats, int32_t *nmat,
+                            MpmRunParams *run, int64_t *nparticles, double *maxTime) {
+    MpmProblem *p = const_cast<MpmProblem *>(pc);
+    if (!mpmsetup::finalize(*p)) return fail_global(p->err);
+    if (grid) *grid = p->grid;
+    if (mats) for (size_t i = 0; i < p->mats.size(); i++) mats[i] = p->mats[i];
+    if (nmat) *nmat = (int32_t)p->mats.size();
+    if (run) *run = p->run;
+    if (nparticles) *nparticles = p->nparticles();
+    if (maxTime) *maxTime = p->maxTime;
+    return 0;
+}
+
+int mpmgpu_problem_particles(const MpmProblem *pc, double *dfields, int32_t *ifields) {
+    MpmProblem *p = const_cast<MpmProblem *>(pc);
+    if (!mpmsetup::finalize(*p)) return fail_global(p->err);
+    size_t n = (size_t)p->nparticles();
+    for (int f = 0; f < MPF_NUM_FIELDS; f++)
+        if (n) memcpy(dfields + f * n, p->d[f].data(), n * sizeof(double));
+    if (ifields)
+        for (int f = 0; f < MPI_NUM_FIELDS; f++)
+            if (n) memcpy(ifields + f * n, p->iv[f].data(), n * sizeof(int32_t));
+    return 0;
+}
+
+void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
+
+// ---- device context --------------------------------------------------------------------
+
+int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat, const MpmRunParams *run,
+                  MpmCtx **out) {
+    *out = nullptr;
+    if (grid->analysis != MPMGPU_PLANE_STRAIN && grid->analysis != MPMGPU_THREED)
+        return fail_global("analysis type must be 10 (plane strain) or 12 (3D)");
+    if (run->shape != MPMGPU_SHAPE_UGIMP)
+        return fail_global("only uGIMP shape functions are implemented on the GPU path");
+    if (run->mpmMethod != 2) return fail_global("only the USAVG method is implemented");
+    if (run->multiMaterial) return fail_global("multimaterial mode is not implemented on the GPU path yet");
+    if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return fail_global("bad material count");
+    for (int i = 0; i < nmat; i++) {
+        int t = mats[i].type;
+        if (t != MPMGPU_MAT_ISOTROPIC && t != MPMGPU_MAT_ISOPLASTICITY && t != MPMGPU_MAT_NEOHOOKEAN &&
+            t != MPMGPU_MAT_COULOMB && t != MPMGPU_MAT_CONTACT)
+            return fail_global("material type " + std::to_string(t) + " is not implemented on the GPU path");
+    }
+    MpmCtx *c = new MpmCtx();
+    c->dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
+    c->gd = *grid;
+    c->run = *run;
+    c->mats.assign(mats, mats + nmat);
+    KGrid &g = c->kg;
+    g.dim = c->dim;
+    g.nx = grid->nx; g.ny = grid->ny; g.nz = c->dim == 3 ? grid->nz : 1;
+    g.nnx = g.nx + 1; g.nny = g.ny + 1; g.nnz = c->dim == 3 ? g.nz + 1 : 1;
+    int tx, ty, tz, nc;
+    if (c->dim == 3) { tx = TileGeom<3>::TX; ty = TileGeom<3>::TY; tz = TileGeom<3>::TZ; nc = TileGeom<3>::NC; }
+    else { tx = TileGeom<2>::TX; ty = TileGeom<2>::TY; tz = 1; nc = TileGeom<2>::NC; }
+    g.ntx = (g.nx + tx - 1) / tx; g.nty = (g.ny + ty - 1) / ty; g.ntz = (g.nz + tz - 1) / tz;
+    g.ntiles = g.ntx * g.nty * g.ntz;
+    g.nnodes = (long long)g.nnx * g.nny * g.nnz;
+    g.xmin = grid->xmin; g.ymin = grid->ymin; g.zmin = grid->zmin;
+    g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
+    g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
+    int foot = c->dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
+    long long nn = g.nnodes;
+    c->kb.nkeys = g.ntiles * nc;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+        std::string m = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete c;
+        return fail_global(m);
+    }
+    size_t nodeBytes = (size_t)nn * (1 + 3 + 3 + 3) * sizeof(double);
+    double *nodeMem = nullptr;
+    int *cntMem = nullptr;
+    double *partMem = nullptr;
+    if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nn * sizeof(int)) != hipSuccess ||
+        hipMalloc(&partMem, (size_t)g.ntiles * NQMAX * foot * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dmats, nmat * sizeof(MpmMaterial)) != hipSuccess ||
+        hipMalloc(&c->sbuf, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return fail_global("out of device memory allocating the grid");
+    }
+    c->kn.mass = nodeMem;
+    c->kn.pk = nodeMem + nn;
+    c->kn.vk = nodeMem + 4 * nn;
+    c->kn.ftot = nodeMem + 7 * nn;
+    c->kn.cnt = cntMem;
+    c->kn.part = partMem;
+    c->ks.leftGrid = c->sbuf;
+    c->ks.nanCount = c->sbuf + 1;
+    (void)hipMemsetAsync(nodeMem, 0, nodeBytes, c->stream);
+    (void)hipMemsetAsync(cntMem, 0, (size_t)nn * sizeof(int), c->stream);
+    (void)hipMemsetAsync(c->sbuf, 0, 2 * sizeof(unsigned long long), c->stream);
+    (void)hipMemcpyAsync(c->dmats, mats, nmat * sizeof(MpmMaterial), hipMemcpyHostToDevice, c->stream);
+    // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums
+    long long nk = c->kb.nkeys;
+    long long nblk = (nk + 1 + 1023) / 1024 + 1;
+    if (hipMalloc(&c->bbuf, (size_t)(3 * nk + 2 + nblk) * sizeof(int)) != hipSuccess) {
+        delete c;
+        return fail_global("out of device memory allocating the bins");
+    }
+    c->kb.cellStart = c->bbuf;
+    c->kb.count = c->bbuf + nk + 1;
+    c->kb.cursor = c->bbuf + 2 * nk + 2;
+    c->kb.blockSums = c->bbuf + 3 * nk + 2;
+    (void)hipMemsetAsync(c->bbuf, 0, (size_t)(3 * nk + 2 + nblk) * sizeof(int), c->stream);
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        delete c;
+        return fail_global("device initialisation failed");
+    }
+    *out = c;
+    return 0;
+}
+
+int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const int32_t *ifields) {
+    destroy_graphs(c);
+    if (n > 0x7fffffffLL) return c->fail("too many particles for one context");
+    if (n > c->ncap) {
+        if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
+        if (c->ibuf) { (void)hipFree(c->ibuf); c->ibuf = nullptr; }
+        HIPCHECK(c, hipMalloc(&c->pbuf, (size_t)(MPF_NUM_FIELDS + 3) * n * sizeof(double)));
+        HIPCHECK(c, hipMalloc(&c->ibuf, (size_t)(MPI_NUM_FIELDS + 2) * n * sizeof(int)));
+        c->ncap = n;
+    }
+    size_t nn = (size_t)n;
+    for (int f = 0; f < MPF_NUM_FIELDS; f++) c->kp.f[f] = c->pbuf + f * nn;
+    for (int a = 0; a < 3; a++) {
+        c->posA[a] = c->pbuf + (MPF_POS_X + a) * nn;
+        c->posB[a] = c->pbuf + (MPF_NUM_FIELDS + a) * nn;
+    }
+    c->parity = 0;
+    c->kp.elem = c->ibuf;
+    c->kp.mat = c->ibuf + nn;
+    c->kp.id = c->ibuf + 2 * nn;
+    c->kp.left = c->ibuf + 3 * nn;
+    c->kb.key = c->ibuf + 4 * nn;
+    c->kb.order = c->ibuf + 5 * nn;
+    c->kp.n = n;
+    // validate materials and elements on the host before anything reaches the device
+    int nmat = (int)c->mats.size();
+    long long nelem = (long long)c->gd.nx * c->gd.ny * (c->dim == 3 ? c->gd.nz : 1);
+    for (int64_t i = 0; i < n; i++) {
+        int m = ifields[MPI_MAT * nn + i];
+        long long e = ifields[MPI_ELEM * nn + i];
+        if (m < 0 || m >= nmat) return c->fail("particle with an undefined material");
+        int t = c->mats[m].type;
+        if (t == MPMGPU_MAT_COULOMB || t == MPMGPU_MAT_CONTACT) return c->fail("particle with a contact-law material");
+        if (e < 0 || e >= nelem) return c->fail("particle element outside the grid");
+    }
+    // uniform particle size lets the kernels skip the lp arrays
+    c->kg.uniformLp = 1;
+    if (n > 0) {
+        for (int a = 0; a < 3; a++) c->kg.lp[a] = dfields[(MPF_LP_X + a) * nn];
+        for (int64_t i = 0; i < n && c->kg.uniformLp; i++)
+            for (int a = 0; a < 3; a++)
+                if (dfields[(MPF_LP_X + a) * nn + i] != c->kg.lp[a]) { c->kg.uniformLp = 0; break; }
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 2 * sizeof(unsigned long long), c->stream));
+    set_positions(c);
+    launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
+    launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->nextTask = 0;
+    c->steps = 0;
+    c->mtime = 0.;
+    return 0;
+}
+
+int mpmgpu_run_task(MpmCtx *c, int32_t task) {
+    if (task != c->nextTask)
+        return c->fail("tasks must run in the step order (expected task " + std::to_string(c->nextTask) + ")");
+    hipStream_t s = c->stream;
+    set_positions(c);
+    switch (task) {
+        case MPMGPU_TASK_INITIALIZATION: break;   // natural coordinates are found in each pass
+        case MPMGPU_TASK_MASS_MOMENTUM:
+            launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+            launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
+            break;
+        case MPMGPU_TASK_POST_EXTRAPOLATION: break;   // done by the node pass of the scatter
+        case MPMGPU_TASK_UPDATE_STRAINS_FIRST: break;  // fused with the force scatter
+        case MPMGPU_TASK_GRID_FORCES: launch_usf_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
+        case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
+        case MPMGPU_TASK_UPDATE_MOMENTA: launch_node_force(c->dim, c->kg, c->kb, c->kn, s); break;
+        case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
+        case MPMGPU_TASK_UPDATE_STRAINS_LAST:
+            launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
+            launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
+            break;
+        case MPMGPU_TASK_RESET_ELEMENTS:
+            launch_bin_finish(c->dim, c->kg, c->kp, c->kb, s);
+            c->parity ^= 1;
+            c->steps++;
+            c->mtime += c->run.timestep;
+            break;
+        default: return c->fail("unknown task id");
+    }
+    HIPCHECK(c, hipGetLastError());
+    c->nextTask = (task + 1) % MPMGPU_NUM_TASKS;
+    return 0;
+}
+
+int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
+    if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
+    if (c->kp.n == 0) { c->steps += nsteps; c->mtime += nsteps * c->run.timestep; return 0; }
+    HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    int done = 0;
+    if (nsteps >= 2) {
+        int par = c->parity;
+        if (!c->graph[par]) {
+            hipGraph_t gr;
+            HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(c);
+            enqueue_step(c);
+            HIPCHECK(c, hipStreamEndCapture(c->stream, &gr));
+            hipError_t e = hipGraphInstantiate(&c->graph[par], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) return c->fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            c->parity = par;      // capture does not run the work
+        }
+        for (; done + 2 <= nsteps; done += 2) HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
+    }
+    for (; done < nsteps; done++) enqueue_step(c);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    c->steps += nsteps;
+    c->mtime += nsteps * c->run.timestep;
+    return 0;
+}
+
+int mpmgpu_sync(MpmCtx *c) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->lastStepMs = ms;
+    return 0;
+}
+
+int mpmgpu_download_particles(MpmCtx *c, double *dfields, int32_t *ifields) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    size_t nn = (size_t)c->kp.n;
+    if (nn == 0) return 0;
+    set_positions(c);
+    for (int f = 0; f < MPF_NUM_FIELDS; f++)
+        HIPCHECK(c, hipMemcpy(dfields + f * nn, c->kp.f[f], nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (ifields) HIPCHECK(c, hipMemcpy(ifields, c->ibuf, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mpmgpu_download_grid(MpmCtx *c, double *mass, double *pk, double *ftot) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    size_t nn = (size_t)c->kg.nnodes;
+    if (mass) HIPCHECK(c, hipMemcpy(mass, c->kn.mass, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (pk) HIPCHECK(c, hipMemcpy(pk, c->kn.pk, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (ftot) HIPCHECK(c, hipMemcpy(ftot, c->kn.ftot, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mpmgpu_query(MpmCtx *c, MpmStats *st) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    unsigned long long h[2] = {0, 0};
+    HIPCHECK(c, hipMemcpy(h, c->sbuf, sizeof(h), hipMemcpyDeviceToHost));
+    st->steps = c->steps;
+    st->nparticles = c->kp.n;
+    st->leftGrid = (int64_t)h[0];
+    st->nanParticles = (int64_t)h[1];
+    st->time = c->mtime;
+    st->lastStepMs = c->lastStepMs;
+    return 0;
+}
+
+void mpmgpu_destroy(MpmCtx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    destroy_graphs(c);
+    if (c->pbuf) (void)hipFree(c->pbuf);
+    if (c->ibuf) (void)hipFree(c->ibuf);
+    if (c->bbuf) (void)hipFree(c->bbuf);
+    if (c->kn.mass) (void)hipFree(c->kn.mass);
+    if (c->kn.cnt) (void)hipFree(c->kn.cnt);
+    if (c->kn.part) (void)hipFree(c->kn.part);
+    if (c->dmats) (void)hipFree(c->dmats);
+    if (c->sbuf) (void)hipFree(c->sbuf);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// mpm_kernels.h -- launch interface of the step kernels (mpm_kernels.hip) used by the context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/nairnmpm_gpu.h"
+
+// Tile geometry: cells per tile on each axis. A particle in a cell touches nodes at offsets
+// -1..2, so a tile's node footprint is (T+3) nodes per axis.
+template <int DIM> struct TileGeom;
+template <> struct TileGeom<3> {
+    static constexpr int TX = 4, TY = 4, TZ = 4;
+    static constexpr int NC = TX * TY * TZ;
+    static constexpr int FX = TX + 3, FY = TY + 3, FZ = TZ + 3;
+    static constexpr int FOOT = FX * FY * FZ;
+};
+template <> struct TileGeom<2> {
+    static constexpr int TX = 8, TY = 8, TZ = 1;
+    static constexpr int NC = TX * TY;
+    static constexpr int FX = TX + 3, FY = TY + 3, FZ = 1;
+    static constexpr int FOOT = FX * FY;
+};
+
+// kernel arguments (passed by value)
+struct KGrid {
+    int dim;
+    int nx, ny, nz;            // cells (nz = 1 in 2D)
+    int nnx, nny, nnz;         // nodes (nnz = 1 in 2D)
+    int ntx, nty, ntz;         // tiles
+    int ntiles;
+    long long nnodes;
+    double xmin, ymin, zmin, dx, dy, dz;
+    double timestep, strainFirst, strainLast;
+    int uniformLp;             // 1 when every particle has the same lp (then lp[] is used)
+    double lp[3];
+};
+
+struct KPart {
+    double *f[MPF_NUM_FIELDS]; // f[MPF_POS_*] is the start-of-step position
+    double *posn[3];           // end-of-step position (written by the particle update)
+    int *elem, *mat, *id, *left;
+    long long n;
+};
+
+struct KBins {
+    int *key;                  // tile-major cell key per particle
+    int *order;                // sorted slot -> particle
+    int *cellStart;            // nkeys + 1
+    int *count;                // nkeys histogram
+    int *cursor;               // nkeys scatter cursor
+    int *blockSums;            // scan scratch
+    int nkeys;
+};
+
+struct KNodes {
+    double *mass;              // nnodes
+    double *pk;                // 3*nnodes (component-major)
+    double *vk;                // 3*nnodes
+    double *ftot;              // 3*nnodes
+    int *cnt;                  // nnodes: particle-node pairs (numberPoints)
+    double *part;              // per-tile partial sums, ntiles * NQMAX * FOOT
+};
+
+struct KStats {
+    unsigned long long *leftGrid;
+    unsigned long long *nanCount;
+};
+
+enum { NQ_MASS = 5, NQ_FORCE = 3, NQ_MOM = 3, NQMAX = 5 };
+
+// host-side launchers (mpm_kernels.hip); all take the context's stream
+void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
+void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
+void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                     const MpmMaterial *mats, hipStream_t s);
+void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
+void launch_usf_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                      const MpmMaterial *mats, hipStream_t s);
+void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
+void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                    hipStream_t s);
+void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
+void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                      const MpmMaterial *mats, const KStats &st, hipStream_t s);

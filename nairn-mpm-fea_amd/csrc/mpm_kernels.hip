@@ -1,0 +1,917 @@
+// mpm_kernels.hip -- gfx950 kernels of NairnMPM's explicit step (USAVG+, FLIP, uGIMP).
+//
+// Layout: particles stay in structure-of-arrays form; every step they are binned by cell with a
+// stable counting sort into tiles of TileGeom<DIM> cells (tile-major cell keys), which gives a
+// sorted slot -> particle map (order[]). Particle-to-grid sums never use floating-point atomics:
+// a tile kernel stages its particles' per-axis GIMP factors in LDS, and every node of the
+// tile's footprint is owned by one thread that adds the contributions in a fixed order (cells
+// ascending, particles ascending in the sorted order). The footprint sums of each tile go to a
+// partial buffer that a node kernel adds in fixed tile order. Results are therefore
+// deterministic; they differ from the reference's single-patch sums by summation order only.
+//
+// Reference tasks (paths relative to the reference root, NairnMPM/src/NairnMPM_Class/):
+//   k_p2g_mass    MassAndMomentumTask.cpp:48-127      (+ InitializationTask xi, :43-85)
+//   k_node_mass   PostExtrapolationTask.cpp:46-165 + GridValueCalculation (UpdateStrainsFirstTask.cpp:120)
+//   k_usf_force   UpdateStrainsFirstTask.cpp:101-168 + GridForcesTask.cpp:40-153
+//   k_node_force  PostForcesTask.cpp:43-111 + UpdateMomentaTask.cpp:44-64 + GridValueCalculation
+//   k_g2p_p2g     UpdateParticlesTask.cpp:44-298 + UpdateStrainsLastContactTask.cpp:60-142
+//   k_node_mom2   UpdateStrainsLastContactTask.cpp:139-149 + GridValueCalculation
+//   k_usl_reset   UpdateStrainsFirstTask::FullStrainUpdate(last) + ResetElementsTask.cpp:44-265
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.h"
+#include "mpm_kernels.h"
+
+#define NT 256
+
+template <int DIM>
+__device__ __forceinline__ void tile_origin(const KGrid &g, int t, int &ox, int &oy, int &oz) {
+    using G = TileGeom<DIM>;
+    int tx = t % g.ntx;
+    int r = t / g.ntx;
+    int ty = r % g.nty;
+    int tz = r / g.nty;
+    ox = tx * G::TX; oy = ty * G::TY; oz = tz * G::TZ;
+}
+
+template <int DIM>
+__device__ __forceinline__ int cell_key(const KGrid &g, int ci, int cj, int ck) {
+    using G = TileGeom<DIM>;
+    int tx = ci / G::TX, ty = cj / G::TY, tz = ck / G::TZ;
+    int t = (tz * g.nty + ty) * g.ntx + tx;
+    int l = ((ck - tz * G::TZ) * G::TY + (cj - ty * G::TY)) * G::TX + (ci - tx * G::TX);
+    return t * G::NC + l;
+}
+
+__device__ __forceinline__ void elem_ijk(const KGrid &g, int e, int &i, int &j, int &k) {
+    i = e % g.nx;
+    int r = e / g.nx;
+    j = r % g.ny;
+    k = r / g.ny;
+}
+
+// natural coordinates in the particle's element (EightNodeIsoparamBrick.cpp:276-281,
+// FourNodeIsoparam.cpp:351-356), node coordinates xmin + i*dx (Generators.cpp:1823-1833)
+template <int DIM>
+__device__ __forceinline__ void particle_xi(const KGrid &g, int ei, int ej, int ek, double x, double y,
+                                            double z, double xi[3], double invd[3]) {
+    double xmn = g.xmin + (double)ei * g.dx, xmx = g.xmin + (double)(ei + 1) * g.dx;
+    double ymn = g.ymin + (double)ej * g.dy, ymx = g.ymin + (double)(ej + 1) * g.dy;
+    xi[0] = (2. * x - xmn - xmx) / (xmx - xmn);
+    xi[1] = (2. * y - ymn - ymx) / (ymx - ymn);
+    invd[0] = 2.0 / (xmx - xmn);
+    invd[1] = 2.0 / (ymx - ymn);
+    if (DIM == 3) {
+        double zmn = g.zmin + (double)ek * g.dz, zmx = g.zmin + (double)(ek + 1) * g.dz;
+        xi[2] = (2. * z - zmn - zmx) / (zmx - zmn);
+        invd[2] = 2.0 / (zmx - zmn);
+    } else {
+        xi[2] = 0.;
+        invd[2] = 0.;
+    }
+}
+
+// per-axis GIMP factors of one particle
+template <int DIM>
+__device__ __forceinline__ void particle_factors(const KGrid &g, const KPart &P, int p, const double xi[3],
+                                                 double S[3][4], double dS[3][4]) {
+    double lpx, lpy, lpz;
+    if (g.uniformLp) { lpx = g.lp[0]; lpy = g.lp[1]; lpz = g.lp[2]; }
+    else { lpx = P.f[MPF_LP_X][p]; lpy = P.f[MPF_LP_Y][p]; lpz = P.f[MPF_LP_Z][p]; }
+    if (DIM == 3) {
+        gimp_axis3(xi[0], lpx, 1. / (4. * lpx), S[0], dS[0]);
+        gimp_axis3(xi[1], lpy, 1. / (4. * lpy), S[1], dS[1]);
+        gimp_axis3(xi[2], lpz, 1. / (4. * lpy), S[2], dS[2]);   // EightNodeIsoparamBrick.cpp:303
+    } else {
+        gimp_axis2(xi[0], lpx, S[0], dS[0]);
+        gimp_axis2(xi[1], lpy, S[1], dS[1]);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ int foot_slot(int lx, int ly, int lz) {
+    using G = TileGeom<DIM>;
+    if (DIM == 3) return ((lz + 1) * G::FY + (ly + 1)) * G::FX + (lx + 1);
+    return (ly + 1) * G::FX + (lx + 1);
+}
+
+// global node index of footprint slot s of a tile with cell origin (ox,oy,oz); -1 outside grid
+template <int DIM>
+__device__ __forceinline__ long long foot_node(const KGrid &g, int s, int ox, int oy, int oz) {
+    using G = TileGeom<DIM>;
+    int fx = s % G::FX;
+    int fy = (s / G::FX) % G::FY;
+    int fz = DIM == 3 ? s / (G::FX * G::FY) : 1;
+    int i = ox + fx - 1, j = oy + fy - 1, k = DIM == 3 ? oz + fz - 1 : 0;
+    if (i < 0 || i >= g.nnx || j < 0 || j >= g.nny || k < 0 || k >= g.nnz) return -1;
+    return ((long long)k * g.nny + j) * g.nnx + i;
+}
+
+// ---------------------------------------------------------------------------------------
+// binning: key + histogram, exclusive scan, scatter, per-cell sort (stable by particle index)
+
+template <int DIM>
+__global__ void k_bin_keys(KGrid g, KPart P, KBins B) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    int i, j, k;
+    elem_ijk(g, P.elem[p], i, j, k);
+    int key = cell_key<DIM>(g, i, j, DIM == 3 ? k : 0);
+    B.key[p] = key;
+    atomicAdd(&B.count[key], 1);
+}
+
+#define SCAN_PER_THREAD 4
+#define SCAN_BLOCK (NT * SCAN_PER_THREAD)
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *s_tmp, int &total) {
+    // Hillis-Steele over the block in LDS
+    int tid = threadIdx.x;
+    s_tmp[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < NT; off <<= 1) {
+        int add = tid >= off ? s_tmp[tid - off] : 0;
+        __syncthreads();
+        s_tmp[tid] += add;
+        __syncthreads();
+    }
+    total = s_tmp[NT - 1];
+    int incl = s_tmp[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(NT) void k_scan_blocks(const int *in, int *out, int *blockSums, int n) {
+    __shared__ int s_tmp[NT];
+    int base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_PER_THREAD;
+    int v[SCAN_PER_THREAD];
+    int sum = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; i++) {
+        v[i] = base + i < n ? in[base + i] : 0;
+        sum += v[i];
+    }
+    int total;
+    int ex = block_exclusive_scan(sum, s_tmp, total);
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; i++) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(NT) void k_scan_sums(int *blockSums, int nb) {
+    __shared__ int s_tmp[NT];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += NT) {
+        int i = b0 + threadIdx.x;
+        int v = i < nb ? blockSums[i] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, s_tmp, total);
+        if (i < nb) blockSums[i] = ex + carry;
+        carry += total;
+    }
+}
+
+__global__ void k_scan_add(int *out, const int *blockSums, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += blockSums[i / SCAN_BLOCK];
+}
+
+__global__ void k_bin_scatter(KPart P, KBins B) {
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.n) return;
+    int key = B.key[p];
+    int slot = B.cellStart[key] + atomicAdd(&B.cursor[key], 1);
+    B.order[slot] = (int)p;
+}
+
+__global__ void k_bin_sort_cells(KBins B) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= B.nkeys) return;
+    int b = B.cellStart[c], e = B.cellStart[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        int v = B.order[i];
+        int j = i - 1;
+        while (j >= b && B.order[j] > v) {
+            B.order[j + 1] = B.order[j];
+            j--;
+        }
+        B.order[j + 1] = v;
+    }
+    B.cursor[c] = 0;
+    B.count[c] = 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// node-owner accumulation over the tile's staged chunk. Calls body(slotIndex j, lq, ix, iy, iz)
+// for every (owned footprint slot, staged particle) pair whose cell is within reach, in the
+// fixed order cells ascending, particles ascending.
+template <int DIM, int SPT, typename Body>
+__device__ __forceinline__ void node_phase(const int *s_cs, int c0, int c1, Body body) {
+    using G = TileGeom<DIM>;
+    int tid = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+        int s = tid + j * NT;
+        if (s >= G::FOOT) break;
+        int fx = s % G::FX;
+        int fy = (s / G::FX) % G::FY;
+        int fz = DIM == 3 ? s / (G::FX * G::FY) : 1;
+        int nlx = fx - 1, nly = fy - 1, nlz = fz - 1;
+        int cx0 = max(0, nlx - 2), cx1 = min(G::TX - 1, nlx + 1);
+        int cy0 = max(0, nly - 2), cy1 = min(G::TY - 1, nly + 1);
+        int cz0 = DIM == 3 ? max(0, nlz - 2) : 0, cz1 = DIM == 3 ? min(G::TZ - 1, nlz + 1) : 0;
+        for (int cz = cz0; cz <= cz1; cz++)
+            for (int cy = cy0; cy <= cy1; cy++)
+                for (int cx = cx0; cx <= cx1; cx++) {
+                    int c = (cz * G::TY + cy) * G::TX + cx;
+                    int b = max(s_cs[c], c0), e = min(s_cs[c + 1], c1);
+                    int ix = nlx - cx + 1, iy = nly - cy + 1, iz = DIM == 3 ? nlz - cz + 1 : 0;
+                    for (int q = b; q < e; q++) body(j, q - c0, ix, iy, iz);
+                }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_tile_starts(const KBins &B, int t, int *s_cs) {
+    using G = TileGeom<DIM>;
+    for (int i = threadIdx.x; i <= G::NC; i += NT) s_cs[i] = B.cellStart[t * G::NC + i];
+    __syncthreads();
+}
+
+template <int DIM, int NQ, int SPT>
+__device__ __forceinline__ void write_partials(const KNodes &N, int t, double acc[SPT][NQ]) {
+    using G = TileGeom<DIM>;
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+        int s = threadIdx.x + j * NT;
+        if (s >= G::FOOT) break;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) N.part[((size_t)t * NQMAX + q) * G::FOOT + s] = acc[j][q];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// P2G #1: mass, momentum, point count (MassAndMomentumTask -> NodalPoint::AddMassMomentum,
+// NodalPointMPM.cpp:419-453)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNodes N) {
+    using G = TileGeom<DIM>;
+    constexpr int SPT = (G::FOOT + NT - 1) / NT;
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_S[DIM][4][NT];
+    __shared__ double s_q[4][NT];
+    const int t = blockIdx.x;
+    load_tile_starts<DIM>(B, t, s_cs);
+    const int tb = s_cs[0], te = s_cs[G::NC];
+    if (tb == te) return;
+    double acc[SPT][NQ_MASS];
+#pragma unroll
+    for (int j = 0; j < SPT; j++)
+#pragma unroll
+        for (int q = 0; q < NQ_MASS; q++) acc[j][q] = 0.;
+    for (int c0 = tb; c0 < te; c0 += NT) {
+        int k = c0 + threadIdx.x;
+        if (k < te) {
+            int p = B.order[k];
+            int ei, ej, ek;
+            elem_ijk(g, P.elem[p], ei, ej, ek);
+            double xi[3], invd[3], S[3][4], dS[3][4];
+            particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                             DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+            particle_factors<DIM>(g, P, p, xi, S, dS);
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int o = 0; o < 4; o++) s_S[a][o][threadIdx.x] = S[a][o];
+            s_q[0][threadIdx.x] = P.f[MPF_MASS][p];
+            s_q[1][threadIdx.x] = P.f[MPF_VEL_X][p];
+            s_q[2][threadIdx.x] = P.f[MPF_VEL_Y][p];
+            s_q[3][threadIdx.x] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+        }
+        __syncthreads();
+        int c1 = min(c0 + NT, te);
+        node_phase<DIM, SPT>(s_cs, c0, c1, [&](int j, int lq, int ix, int iy, int iz) {
+            double wx = s_S[0][ix][lq], wy = s_S[1][iy][lq];
+            if (wx < 0. || wy < 0.) return;
+            double w;
+            if (DIM == 3) {
+                double wz = s_S[DIM - 1][iz][lq];
+                if (wz < 0.) return;
+                w = wx * wy * wz;
+            } else w = wx * wy;
+            double fnmp = w * s_q[0][lq];
+            acc[j][1] += fnmp * s_q[1][lq];
+            acc[j][2] += fnmp * s_q[2][lq];
+            acc[j][3] += fnmp * s_q[3][lq];
+            acc[j][0] += fnmp;
+            acc[j][4] += 1.;
+        });
+        __syncthreads();
+    }
+    write_partials<DIM, NQ_MASS, SPT>(N, t, acc);
+}
+
+// sum the partials of the tiles that cover node (i,j,k), in fixed tile order
+template <int DIM, int NQ>
+__device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, const KNodes &N, int i,
+                                                int j, int k, double out[NQ]) {
+    using G = TileGeom<DIM>;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) out[q] = 0.;
+    int txh = min(g.ntx - 1, (i + 1) / G::TX), txl = i >= G::TX + 1 ? (i - 2) / G::TX : 0;
+    int tyh = min(g.nty - 1, (j + 1) / G::TY), tyl = j >= G::TY + 1 ? (j - 2) / G::TY : 0;
+    int tzh = 0, tzl = 0;
+    if (DIM == 3) { tzh = min(g.ntz - 1, (k + 1) / G::TZ); tzl = k >= G::TZ + 1 ? (k - 2) / G::TZ : 0; }
+    for (int tz = tzl; tz <= tzh; tz++)
+        for (int ty = tyl; ty <= tyh; ty++)
+            for (int tx = txl; tx <= txh; tx++) {
+                int t = (tz * g.nty + ty) * g.ntx + tx;
+                if (B.cellStart[(t + 1) * G::NC] == B.cellStart[t * G::NC]) continue;
+                int fx = i - (tx * G::TX - 1), fy = j - (ty * G::TY - 1);
+                int s = fy * G::FX + fx;
+                if (DIM == 3) s += (k - (tz * G::TZ - 1)) * G::FX * G::FY;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) out[q] += N.part[((size_t)t * NQMAX + q) * G::FOOT + s];
+            }
+}
+
+// node pass after P2G #1: mass, momentum, count and v = p/m (MatVelocityField.cpp:158-164,239-251)
+template <int DIM>
+__global__ void k_node_mass(KGrid g, KBins B, KNodes N) {
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nnodes) return;
+    int i = (int)(n % g.nnx);
+    long long r = n / g.nnx;
+    int j = (int)(r % g.nny);
+    int k = (int)(r / g.nny);
+    double a[NQ_MASS];
+    gather_partials<DIM, NQ_MASS>(g, B, N, i, j, k, a);
+    long long nn = g.nnodes;
+    N.mass[n] = a[0];
+    N.pk[n] = a[1]; N.pk[nn + n] = a[2]; N.pk[2 * nn + n] = a[3];
+    int cnt = (int)a[4];
+    N.cnt[n] = cnt;
+    double vx = 0., vy = 0., vz = 0.;
+    if (cnt > 0 && a[0] != 0.) {
+        double s = 1. / a[0];
+        vx = s * a[1]; vy = s * a[2]; vz = s * a[3];
+    }
+    N.vk[n] = vx; N.vk[nn + n] = vy; N.vk[2 * nn + n] = vz;
+    N.ftot[n] = 0.; N.ftot[nn + n] = 0.; N.ftot[2 * nn + n] = 0.;
+}
+
+// ---------------------------------------------------------------------------------------
+// load the footprint node vectors of a tile into LDS (zero outside the grid)
+template <int DIM, int NV>
+__device__ __forceinline__ void load_foot(const KGrid &g, int ox, int oy, int oz, const double *src[NV],
+                                          const long long stride[NV], double (*dst)[TileGeom<DIM>::FOOT]) {
+    using G = TileGeom<DIM>;
+    for (int s = threadIdx.x; s < G::FOOT; s += NT) {
+        long long n = foot_node<DIM>(g, s, ox, oy, oz);
+#pragma unroll
+        for (int v = 0; v < NV; v++) dst[v][s] = n >= 0 ? src[v][n + stride[v]] : 0.;
+    }
+}
+
+// velocity gradient of one particle from footprint velocities, candidates in reference order
+// (MatPoint3D::ExtrapVelocityGradient, MatPoint3D.cpp:62-94; MatPoint2D.cpp:57-89)
+template <int DIM>
+__device__ __forceinline__ void velocity_gradient(const double S[3][4], const double dS[3][4],
+                                                  const double invd[3], int lx, int ly, int lz,
+                                                  const double (*s_v)[TileGeom<DIM>::FOOT], double L[3][3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) L[a][b] = 0.;
+    if (DIM == 3) {
+        for (int id = 0; id < 64; id++) {
+            int ox = c_off3x[id] + 1, oy = c_off3y[id] + 1, oz = c_off3z[id] + 1;
+            double Sx = S[0][ox], Sy = S[1][oy], Sz = S[2][oz];
+            if (Sx < 0. || Sy < 0. || Sz < 0.) continue;
+            double gx = dS[0][ox] * Sy * Sz * invd[0];
+            double gy = Sx * dS[1][oy] * Sz * invd[1];
+            double gz = Sx * Sy * dS[2][oz] * invd[2];
+            int s = foot_slot<DIM>(lx + ox - 1, ly + oy - 1, lz + oz - 1);
+            double vx = s_v[0][s], vy = s_v[1][s], vz = s_v[2][s];
+            L[0][0] += vx * gx; L[0][1] += vx * gy; L[0][2] += vx * gz;
+            L[1][0] += vy * gx; L[1][1] += vy * gy; L[1][2] += vy * gz;
+            L[2][0] += vz * gx; L[2][1] += vz * gy; L[2][2] += vz * gz;
+        }
+    } else {
+        for (int id = 0; id < 16; id++) {
+            int ox = c_off2x[id] + 1, oy = c_off2y[id] + 1;
+            double Sx = S[0][ox], Sy = S[1][oy];
+            if (Sx < 0. || Sy < 0.) continue;
+            double gx = dS[0][ox] * Sy * invd[0];
+            double gy = Sx * dS[1][oy] * invd[1];
+            int s = foot_slot<DIM>(lx + ox - 1, ly + oy - 1, 0);
+            double vx = s_v[0][s], vy = s_v[1][s];
+            L[0][0] += vx * gx; L[0][1] += vx * gy;
+            L[1][0] += vy * gx; L[1][1] += vy * gy;
+            L[2][2] += 0.;
+        }
+    }
+}
+
+// Matrix3::Scale (Matrix3.cpp:410-424)
+template <int DIM>
+__device__ __forceinline__ void scale_gradient(double L[3][3], double f) {
+    L[0][0] *= f; L[0][1] *= f; L[1][0] *= f; L[1][1] *= f; L[2][2] *= f;
+    if (DIM == 3) { L[0][2] *= f; L[1][2] *= f; L[2][0] *= f; L[2][1] *= f; }
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_state(const KPart &P, int p, int type, PState &s) {
+    s.ep[0] = P.f[MPF_EP_XX][p]; s.ep[1] = P.f[MPF_EP_YY][p]; s.ep[2] = P.f[MPF_EP_ZZ][p];
+    s.ep[5] = P.f[MPF_EP_XY][p]; s.wr[2] = P.f[MPF_WR_XY][p];
+    s.sp[0] = P.f[MPF_SP_XX][p]; s.sp[1] = P.f[MPF_SP_YY][p]; s.sp[2] = P.f[MPF_SP_ZZ][p];
+    s.sp[5] = P.f[MPF_SP_XY][p];
+    if (DIM == 3) {
+        s.ep[3] = P.f[MPF_EP_YZ][p]; s.ep[4] = P.f[MPF_EP_XZ][p];
+        s.wr[0] = P.f[MPF_WR_YZ][p]; s.wr[1] = P.f[MPF_WR_XZ][p];
+        s.sp[3] = P.f[MPF_SP_YZ][p]; s.sp[4] = P.f[MPF_SP_XZ][p];
+    } else {
+        s.ep[3] = s.ep[4] = 0.; s.wr[0] = s.wr[1] = 0.; s.sp[3] = s.sp[4] = 0.;
+    }
+    s.P = 0.; s.h0 = 0.; s.h1 = 0.;
+#pragma unroll
+    for (int c = 0; c < 6; c++) s.alt[c] = 0.;
+    if (type != MPMGPU_MAT_ISOTROPIC) {
+        s.P = P.f[MPF_PRESSURE][p];
+        s.alt[0] = P.f[MPF_ALT_XX][p]; s.alt[1] = P.f[MPF_ALT_YY][p]; s.alt[2] = P.f[MPF_ALT_ZZ][p];
+        s.alt[5] = P.f[MPF_ALT_XY][p];
+        if (DIM == 3) { s.alt[3] = P.f[MPF_ALT_YZ][p]; s.alt[4] = P.f[MPF_ALT_XZ][p]; }
+        s.h0 = P.f[MPF_HIST0][p];
+        if (type == MPMGPU_MAT_NEOHOOKEAN) s.h1 = P.f[MPF_HIST1][p];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_state(const KPart &P, int p, int type, const PState &s) {
+    P.f[MPF_EP_XX][p] = s.ep[0]; P.f[MPF_EP_YY][p] = s.ep[1]; P.f[MPF_EP_ZZ][p] = s.ep[2];
+    P.f[MPF_EP_XY][p] = s.ep[5]; P.f[MPF_WR_XY][p] = s.wr[2];
+    P.f[MPF_SP_XX][p] = s.sp[0]; P.f[MPF_SP_YY][p] = s.sp[1]; P.f[MPF_SP_ZZ][p] = s.sp[2];
+    P.f[MPF_SP_XY][p] = s.sp[5];
+    if (DIM == 3) {
+        P.f[MPF_EP_YZ][p] = s.ep[3]; P.f[MPF_EP_XZ][p] = s.ep[4];
+        P.f[MPF_WR_YZ][p] = s.wr[0]; P.f[MPF_WR_XZ][p] = s.wr[1];
+        P.f[MPF_SP_YZ][p] = s.sp[3]; P.f[MPF_SP_XZ][p] = s.sp[4];
+    }
+    if (type != MPMGPU_MAT_ISOTROPIC) {
+        P.f[MPF_PRESSURE][p] = s.P;
+        P.f[MPF_ALT_XX][p] = s.alt[0]; P.f[MPF_ALT_YY][p] = s.alt[1]; P.f[MPF_ALT_ZZ][p] = s.alt[2];
+        P.f[MPF_ALT_XY][p] = s.alt[5];
+        if (DIM == 3) { P.f[MPF_ALT_YZ][p] = s.alt[3]; P.f[MPF_ALT_XZ][p] = s.alt[4]; }
+        P.f[MPF_HIST0][p] = s.h0;
+        if (type == MPMGPU_MAT_NEOHOOKEAN) P.f[MPF_HIST1][p] = s.h1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// USF strain update (first half step) fused with the internal-force scatter
+// (UpdateStrainsFirstTask::FullStrainUpdate, GridForcesTask + MatPoint3D::GetFintPlusFext,
+// MatPoint3D.cpp:248-270, MatPoint2D.cpp:242-263). pFext is zero (no particle loads).
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_usf_force(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
+    using G = TileGeom<DIM>;
+    constexpr int SPT = (G::FOOT + NT - 1) / NT;
+    constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_v[3][G::FOOT];
+    __shared__ double s_S[DIM][4][NT];
+    __shared__ double s_dS[DIM][4][NT];
+    __shared__ double s_id[DIM][NT];
+    __shared__ double s_st[NS][NT];
+    const int t = blockIdx.x;
+    load_tile_starts<DIM>(B, t, s_cs);
+    const int tb = s_cs[0], te = s_cs[G::NC];
+    if (tb == te) return;
+    int ox, oy, oz;
+    tile_origin<DIM>(g, t, ox, oy, oz);
+    {
+        const double *src[3] = {N.vk, N.vk, N.vk};
+        const long long st[3] = {0, g.nnodes, 2 * g.nnodes};
+        load_foot<DIM, 3>(g, ox, oy, oz, src, st, s_v);
+    }
+    __syncthreads();
+    double acc[SPT][NQ_FORCE];
+#pragma unroll
+    for (int j = 0; j < SPT; j++)
+#pragma unroll
+        for (int q = 0; q < NQ_FORCE; q++) acc[j][q] = 0.;
+    for (int c0 = tb; c0 < te; c0 += NT) {
+        int k = c0 + threadIdx.x;
+        if (k < te) {
+            int p = B.order[k];
+            int ei, ej, ek;
+            elem_ijk(g, P.elem[p], ei, ej, ek);
+            double xi[3], invd[3], S[3][4], dS[3][4];
+            particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                             DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+            particle_factors<DIM>(g, P, p, xi, S, dS);
+            double L[3][3];
+            velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+            scale_gradient<DIM>(L, g.strainFirst);
+            const MpmMaterial &m = mats[P.mat[p]];
+            PState s;
+            load_state<DIM>(P, p, m.type, s);
+            constitutive<DIM>(L, m, s);
+            store_state<DIM>(P, p, m.type, s);
+#pragma unroll
+            for (int a = 0; a < DIM; a++) {
+#pragma unroll
+                for (int o = 0; o < 4; o++) {
+                    s_S[a][o][threadIdx.x] = S[a][o];
+                    s_dS[a][o][threadIdx.x] = dS[a][o];
+                }
+                s_id[a][threadIdx.x] = invd[a];
+            }
+            double Pr = s.P;
+            s_st[0][threadIdx.x] = -P.f[MPF_MASS][p];
+            s_st[1][threadIdx.x] = s.sp[0] - Pr;
+            s_st[2][threadIdx.x] = s.sp[1] - Pr;
+            s_st[3][threadIdx.x] = s.sp[5];
+            if (DIM == 3) {
+                s_st[NS - 3][threadIdx.x] = s.sp[2] - Pr;
+                s_st[NS - 2][threadIdx.x] = s.sp[4];
+                s_st[NS - 1][threadIdx.x] = s.sp[3];
+            }
+        }
+        __syncthreads();
+        int c1 = min(c0 + NT, te);
+        node_phase<DIM, SPT>(s_cs, c0, c1, [&](int j, int lq, int ix, int iy, int iz) {
+            double wx = s_S[0][ix][lq], wy = s_S[1][iy][lq];
+            if (wx < 0. || wy < 0.) return;
+            double negmp = s_st[0][lq];
+            if (DIM == 3) {
+                double wz = s_S[DIM - 1][iz][lq];
+                if (wz < 0.) return;
+                double gx = s_dS[0][ix][lq] * wy * wz * s_id[0][lq];
+                double gy = wx * s_dS[1][iy][lq] * wz * s_id[1][lq];
+                double gz = wx * wy * s_dS[DIM - 1][iz][lq] * s_id[DIM - 1][lq];
+                double sxxP = s_st[1][lq], syyP = s_st[2][lq], sxy = s_st[3][lq];
+                double szzP = s_st[NS - 3][lq], sxz = s_st[NS - 2][lq], syz = s_st[NS - 1][lq];
+                acc[j][0] += negmp * (sxxP * gx + sxy * gy + sxz * gz);
+                acc[j][1] += negmp * (sxy * gx + syyP * gy + syz * gz);
+                acc[j][2] += negmp * (sxz * gx + syz * gy + szzP * gz);
+            } else {
+                double gx = s_dS[0][ix][lq] * wy * s_id[0][lq];
+                double gy = wx * s_dS[1][iy][lq] * s_id[1][lq];
+                double sxxP = s_st[1][lq], syyP = s_st[2][lq], sxy = s_st[3][lq];
+                acc[j][0] += negmp * (sxxP * gx + sxy * gy);
+                acc[j][1] += negmp * (sxy * gx + syyP * gy);
+            }
+        });
+        __syncthreads();
+    }
+    write_partials<DIM, NQ_FORCE, SPT>(N, t, acc);
+}
+
+// node pass after the forces: ftot, restore p (identity without BCs), p += f dt, v = p/m
+// (PostForcesTask.cpp:46-48, MatVelocityField.cpp:301-314, :239-251)
+template <int DIM>
+__global__ void k_node_force(KGrid g, KBins B, KNodes N) {
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nnodes) return;
+    int i = (int)(n % g.nnx);
+    long long r = n / g.nnx;
+    int j = (int)(r % g.nny);
+    int k = (int)(r / g.nny);
+    double f[NQ_FORCE];
+    gather_partials<DIM, NQ_FORCE>(g, B, N, i, j, k, f);
+    long long nn = g.nnodes;
+    if (DIM == 2) f[2] = 0.;
+    N.ftot[n] = f[0]; N.ftot[nn + n] = f[1]; N.ftot[2 * nn + n] = f[2];
+    int cnt = N.cnt[n];
+    if (cnt <= 0) return;
+    double dt = g.timestep;
+    double px = N.pk[n] + f[0] * dt;
+    double py = N.pk[nn + n] + f[1] * dt;
+    double pz = N.pk[2 * nn + n] + f[2] * dt;
+    N.pk[n] = px; N.pk[nn + n] = py; N.pk[2 * nn + n] = pz;
+    double m = N.mass[n];
+    if (m != 0.) {
+        double s = 1. / m;
+        N.vk[n] = s * px; N.vk[nn + n] = s * py; N.vk[2 * nn + n] = s * pz;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Particle update (FLIP, no damping) fused with the momentum re-extrapolation of the USL pass
+// with start-of-step shape functions (MatVelocityField::IncrementNodalVelAcc,
+// MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
+// NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNodes N) {
+    using G = TileGeom<DIM>;
+    constexpr int SPT = (G::FOOT + NT - 1) / NT;
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_n[7][G::FOOT];         // v(3), f(3), mass
+    __shared__ double s_S[DIM][4][NT];
+    __shared__ double s_q[4][NT];
+    const int t = blockIdx.x;
+    load_tile_starts<DIM>(B, t, s_cs);
+    const int tb = s_cs[0], te = s_cs[G::NC];
+    if (tb == te) return;
+    int ox, oy, oz;
+    tile_origin<DIM>(g, t, ox, oy, oz);
+    {
+        long long nn = g.nnodes;
+        const double *src[7] = {N.vk, N.vk, N.vk, N.ftot, N.ftot, N.ftot, N.mass};
+        const long long st[7] = {0, nn, 2 * nn, 0, nn, 2 * nn, 0};
+        load_foot<DIM, 7>(g, ox, oy, oz, src, st, s_n);
+    }
+    __syncthreads();
+    double acc[SPT][NQ_MOM];
+#pragma unroll
+    for (int j = 0; j < SPT; j++)
+#pragma unroll
+        for (int q = 0; q < NQ_MOM; q++) acc[j][q] = 0.;
+    const double dt = g.timestep;
+    for (int c0 = tb; c0 < te; c0 += NT) {
+        int k = c0 + threadIdx.x;
+        if (k < te) {
+            int p = B.order[k];
+            int ei, ej, ek;
+            elem_ijk(g, P.elem[p], ei, ej, ek);
+            double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+            double xi[3], invd[3], S[3][4], dS[3][4];
+            particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+            particle_factors<DIM>(g, P, p, xi, S, dS);
+            int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
+            double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
+            const int ncand = DIM == 3 ? 64 : 16;
+            for (int id = 0; id < ncand; id++) {
+                int ax, ay, az;
+                if (DIM == 3) { ax = c_off3x[id] + 1; ay = c_off3y[id] + 1; az = c_off3z[id] + 1; }
+                else { ax = c_off2x[id] + 1; ay = c_off2y[id] + 1; az = 1; }
+                double Sx = S[0][ax], Sy = S[1][ay];
+                if (Sx < 0. || Sy < 0.) continue;
+                double fi;
+                if (DIM == 3) {
+                    double Sz = S[2][az];
+                    if (Sz < 0.) continue;
+                    fi = Sx * Sy * Sz;
+                } else fi = Sx * Sy;
+                int s = foot_slot<DIM>(lx + ax - 1, ly + ay - 1, lz + az - 1);
+                Svk[0] += s_n[0][s] * fi;
+                Svk[1] += s_n[1][s] * fi;
+                Svk[2] += s_n[2][s] * fi;
+                double mnode = fi / s_n[6][s];
+                Sacc[0] += s_n[3][s] * mnode;
+                Sacc[1] += s_n[4][s] * mnode;
+                Sacc[2] += s_n[5][s] * mnode;
+            }
+            double vnew[3] = {0., 0., 0.};
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+                double vm = Svk[c] - Sacc[c] * dt;
+                double delV = Sacc[c] * dt;
+                vnew[c] = P.f[MPF_VEL_X + c][p] + delV;
+                P.f[MPF_VEL_X + c][p] = vnew[c];
+                double delXRate = vm + 0.5 * delV;
+                P.posn[c][p] = pos[c] + delXRate * dt;
+            }
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int o = 0; o < 4; o++) s_S[a][o][threadIdx.x] = S[a][o];
+            s_q[0][threadIdx.x] = P.f[MPF_MASS][p];
+            s_q[1][threadIdx.x] = vnew[0];
+            s_q[2][threadIdx.x] = vnew[1];
+            s_q[3][threadIdx.x] = vnew[2];
+        }
+        __syncthreads();
+        int c1 = min(c0 + NT, te);
+        node_phase<DIM, SPT>(s_cs, c0, c1, [&](int j, int lq, int ix, int iy, int iz) {
+            double wx = s_S[0][ix][lq], wy = s_S[1][iy][lq];
+            if (wx < 0. || wy < 0.) return;
+            double w;
+            if (DIM == 3) {
+                double wz = s_S[DIM - 1][iz][lq];
+                if (wz < 0.) return;
+                w = wx * wy * wz;
+            } else w = wx * wy;
+            double fnmp = w * s_q[0][lq];
+            acc[j][0] += s_q[1][lq] * fnmp;
+            acc[j][1] += s_q[2][lq] * fnmp;
+            acc[j][2] += s_q[3][lq] * fnmp;
+        });
+        __syncthreads();
+    }
+    write_partials<DIM, NQ_MOM, SPT>(N, t, acc);
+}
+
+// node pass of the USL re-extrapolation: p (rezeroed, RezeroNodeTask6) and v = p/m
+template <int DIM>
+__global__ void k_node_mom2(KGrid g, KBins B, KNodes N) {
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nnodes) return;
+    int cnt = N.cnt[n];
+    if (cnt <= 0) return;
+    int i = (int)(n % g.nnx);
+    long long r = n / g.nnx;
+    int j = (int)(r % g.nny);
+    int k = (int)(r / g.nny);
+    double a[NQ_MOM];
+    gather_partials<DIM, NQ_MOM>(g, B, N, i, j, k, a);
+    long long nn = g.nnodes;
+    N.pk[n] = a[0]; N.pk[nn + n] = a[1]; N.pk[2 * nn + n] = a[2];
+    double m = N.mass[n];
+    if (m != 0.) {
+        double s = 1. / m;
+        N.vk[n] = s * a[0]; N.vk[nn + n] = s * a[1]; N.vk[2 * nn + n] = s * a[2];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// element test of a rectangle (EightNodeIsoparamBrick.cpp:207-213, FourNodeIsoparam.cpp:413-423)
+template <int DIM>
+__device__ __forceinline__ bool pt_in_elem(const KGrid &g, int ei, int ej, int ek, const double p[3]) {
+    double xmn = g.xmin + (double)ei * g.dx, xmx = g.xmin + (double)(ei + 1) * g.dx;
+    if (p[0] < xmn || p[0] >= xmx) return false;
+    double ymn = g.ymin + (double)ej * g.dy, ymx = g.ymin + (double)(ej + 1) * g.dy;
+    if (p[1] < ymn || p[1] >= ymx) return false;
+    if (DIM == 3) {
+        double zmn = g.zmin + (double)ek * g.dz, zmx = g.zmin + (double)(ek + 1) * g.dz;
+        if (p[2] < zmn || p[2] >= zmx) return false;
+    }
+    return true;
+}
+
+// USL strain update (second half step) and ResetElementsTask; also the cell key and histogram
+// of the next binning
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+                                                  KStats st) {
+    using G = TileGeom<DIM>;
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_v[3][G::FOOT];
+    const int t = blockIdx.x;
+    load_tile_starts<DIM>(B, t, s_cs);
+    const int tb = s_cs[0], te = s_cs[G::NC];
+    if (tb == te) return;
+    int ox, oy, oz;
+    tile_origin<DIM>(g, t, ox, oy, oz);
+    {
+        const double *src[3] = {N.vk, N.vk, N.vk};
+        const long long stt[3] = {0, g.nnodes, 2 * g.nnodes};
+        load_foot<DIM, 3>(g, ox, oy, oz, src, stt, s_v);
+    }
+    __syncthreads();
+    for (int k = tb + threadIdx.x; k < te; k += NT) {
+        int p = B.order[k];
+        int e = P.elem[p];
+        int ei, ej, ek;
+        elem_ijk(g, e, ei, ej, ek);
+        double xi[3], invd[3], S[3][4], dS[3][4];
+        particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                         DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+        particle_factors<DIM>(g, P, p, xi, S, dS);
+        double L[3][3];
+        velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+        scale_gradient<DIM>(L, g.strainLast);
+        const MpmMaterial &m = mats[P.mat[p]];
+        PState s;
+        load_state<DIM>(P, p, m.type, s);
+        constitutive<DIM>(L, m, s);
+        store_state<DIM>(P, p, m.type, s);
+
+        // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
+        double pos[3] = {P.posn[0][p], P.posn[1][p], DIM == 3 ? P.posn[2][p] : 0.};
+        if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
+            atomicAdd(st.nanCount, 1ull);
+        } else if (!pt_in_elem<DIM>(g, ei, ej, ek, pos)) {
+            // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640) and EdgeElement2D/3D (:171-199)
+            bool left = false;
+            long long horiz = g.nx, vert = g.ny, depth = DIM == 3 ? g.nz : 1;
+            long long total = horiz * vert * depth;
+            long long col = (long long)(int)((pos[0] - g.xmin) / g.dx);
+            if (col < 0 || col >= horiz) {
+                if (pos[0] == g.xmin + horiz * g.dx) col = horiz - 1; else left = true;
+            }
+            long long row = 0, zrow = 0;
+            if (!left) {
+                row = (long long)(int)((pos[1] - g.ymin) / g.dy);
+                if (row < 0 || row >= vert) {
+                    if (pos[1] == g.ymin + vert * g.dy) row = vert - 1; else left = true;
+                }
+            }
+            if (!left && DIM == 3) {
+                zrow = (long long)(int)((pos[2] - g.zmin) / g.dz);
+                if (zrow < 0 || zrow >= depth) {
+                    if (pos[2] == g.zmin + depth * g.dz) zrow = depth - 1; else left = true;
+                }
+            }
+            long long num = 0;
+            if (!left) {
+                num = (DIM == 3 ? horiz * (zrow * vert + row) + col : row * horiz + col) + 1;
+                if (DIM == 3) {
+                    if (num <= horiz * vert || num > total - horiz * vert) left = true;
+                    else if (num % horiz <= 1) left = true;
+                    else {
+                        long long xz = num % (horiz * vert);
+                        if (xz <= horiz || xz > horiz * (vert - 1)) left = true;
+                    }
+                } else {
+                    if (num <= horiz || num > total - horiz) left = true;
+                    else if (num % horiz <= 1) left = true;
+                }
+            }
+            if (!left) {
+                e = (int)(num - 1);
+                P.elem[p] = e;
+                elem_ijk(g, e, ei, ej, ek);
+            } else {
+                if (!P.left[p]) { P.left[p] = 1; atomicAdd(st.leftGrid, 1ull); }
+                // ReturnToElement (ResetElementsTask.cpp:232-265)
+                double dtt = g.timestep;
+                double outside[3] = {pos[0], pos[1], pos[2]};
+                double inside[3];
+                inside[0] = outside[0] - dtt * P.f[MPF_VEL_X][p];
+                inside[1] = outside[1] - dtt * P.f[MPF_VEL_Y][p];
+                inside[2] = DIM == 3 ? outside[2] - dtt * P.f[MPF_VEL_Z][p] : 0.;
+                if (!pt_in_elem<DIM>(g, ei, ej, ek, inside)) {
+                    inside[0] = ((g.xmin + (double)ei * g.dx) + (g.xmin + (double)(ei + 1) * g.dx)) / 2.;
+                    inside[1] = ((g.ymin + (double)ej * g.dy) + (g.ymin + (double)(ej + 1) * g.dy)) / 2.;
+                    inside[2] = DIM == 3 ? ((g.zmin + (double)ek * g.dz) + (g.zmin + (double)(ek + 1) * g.dz)) / 2. : 0.;
+                }
+                for (int pass = 1; pass <= 10; pass++) {
+                    double mid[3] = {(outside[0] + inside[0]) / 2., (outside[1] + inside[1]) / 2.,
+                                     (outside[2] + inside[2]) / 2.};
+                    if (pt_in_elem<DIM>(g, ei, ej, ek, mid)) { inside[0] = mid[0]; inside[1] = mid[1]; inside[2] = mid[2]; }
+                    else { outside[0] = mid[0]; outside[1] = mid[1]; outside[2] = mid[2]; }
+                }
+                P.posn[0][p] = inside[0];
+                P.posn[1][p] = inside[1];
+                if (DIM == 3) P.posn[2][p] = inside[2];
+            }
+        }
+        int key = cell_key<DIM>(g, ei, ej, DIM == 3 ? ek : 0);
+        B.key[p] = key;
+        atomicAdd(&B.count[key], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers
+static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s) {
+    if (p.n == 0) return;
+    if (dim == 3) hipLaunchKernelGGL(k_bin_keys<3>, dim3(blocks_for(p.n, NT)), dim3(NT), 0, s, g, p, b);
+    else hipLaunchKernelGGL(k_bin_keys<2>, dim3(blocks_for(p.n, NT)), dim3(NT), 0, s, g, p, b);
+}
+
+void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s) {
+    (void)dim; (void)g;
+    int n = b.nkeys + 1;          // count[nkeys] stays zero, so cellStart[nkeys] = total
+    int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(NT), 0, s, b.count, b.cellStart, b.blockSums, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(NT), 0, s, b.blockSums, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, NT)), dim3(NT), 0, s, b.cellStart, b.blockSums, n);
+    if (p.n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(p.n, NT)), dim3(NT), 0, s, p, b);
+    hipLaunchKernelGGL(k_bin_sort_cells, dim3(blocks_for(b.nkeys, NT)), dim3(NT), 0, s, b);
+}
+
+void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                     const MpmMaterial *mats, hipStream_t s) {
+    (void)mats;
+    if (dim == 3) hipLaunchKernelGGL(k_p2g_mass<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+    else hipLaunchKernelGGL(k_p2g_mass<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+}
+
+void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_node_mass<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+}
+
+void launch_usf_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                      const MpmMaterial *mats, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_usf_force<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats);
+    else hipLaunchKernelGGL(k_usf_force<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats);
+}
+
+void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_node_force<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+}
+
+void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_g2p_p2g<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+    else hipLaunchKernelGGL(k_g2p_p2g<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+}
+
+void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_node_mom2<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+}
+
+void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                      const MpmMaterial *mats, const KStats &st, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_usl_reset<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st);
+    else hipLaunchKernelGGL(k_usl_reset<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st);
+}

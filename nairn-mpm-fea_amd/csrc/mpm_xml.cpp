@@ -1,0 +1,451 @@
+// mpm_xml.cpp -- reader for the NairnMPM XML input (NairnMPM/input/NairnMPM.dtd), limited to the
+// elements of the explicit-step path. It replaces the Xerces SAX2 pipeline of the reference
+// (Common/System/CommonAnalysis.cpp:184-273 -> CommonReadHandler.cpp:50-340 ->
+// Read_MPM/MPMReadHandler.cpp:83-1330, Read_MPM/Generators.cpp:89-1545) with a small in-memory
+// parser: elements, attributes, text, comments, processing instructions and a DOCTYPE internal
+// subset with <!ENTITY name "value"> declarations (as used by NairnMPM/input/XML_Input).
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mpm_problem.h"
+
+namespace {
+
+struct XNode {
+    std::string name;
+    std::vector<std::pair<std::string, std::string>> attrs;
+    std::string text;
+    std::vector<std::unique_ptr<XNode>> kids;
+    const char *attr(const char *k) const {
+        for (auto &a : attrs) if (a.first == k) return a.second.c_str();
+        return nullptr;
+    }
+};
+
+struct Parser {
+    const char *s;
+    size_t i = 0, n;
+    std::map<std::string, std::string> entities;
+    std::string err;
+    explicit Parser(const char *t) : s(t), n(strlen(t)) {}
+
+    bool starts(const char *p) const { return strncmp(s + i, p, strlen(p)) == 0; }
+    void ws() { while (i < n && isspace((unsigned char)s[i])) i++; }
+    bool fail(const std::string &m) {
+        if (err.empty()) {
+            int line = 1;
+            for (size_t k = 0; k < i && k < n; k++) if (s[k] == '\n') line++;
+            err = "XML input line " + std::to_string(line) + ": " + m;
+        }
+        return false;
+    }
+    std::string expand(const std::string &raw) {
+        std::string out;
+        for (size_t k = 0; k < raw.size(); k++) {
+            if (raw[k] != '&') { out += raw[k]; continue; }
+            size_t e = raw.find(';', k);
+            if (e == std::string::npos) { out += raw[k]; continue; }
+            std::string name = raw.substr(k + 1, e - k - 1);
+            if (name == "lt") out += '<';
+            else if (name == "gt") out += '>';
+            else if (name == "amp") out += '&';
+            else if (name == "quot") out += '"';
+            else if (name == "apos") out += '\'';
+            else if (!name.empty() && name[0] == '#') out += (char)strtol(name.c_str() + (name[1] == 'x' ? 2 : 1), nullptr, name[1] == 'x' ? 16 : 10);
+            else if (entities.count(name)) out += expand(entities[name]);
+            else { out += raw.substr(k, e - k + 1); }
+            k = e;
+        }
+        return out;
+    }
+    bool skip_misc() {
+        for (;;) {
+            ws();
+            if (starts("<?")) {
+                size_t e = std::string(s + i).find("?>");
+                if (e == std::string::npos) return fail("unterminated processing instruction");
+                i += e + 2;
+            } else if (starts("<!--")) {
+                size_t e = std::string(s + i).find("-->");
+                if (e == std::string::npos) return fail("unterminated comment");
+                i += e + 3;
+            } else if (starts("<!DOCTYPE")) {
+                if (!doctype()) return false;
+            } else return true;
+        }
+    }
+    bool doctype() {
+        i += 9;
+        while (i < n && s[i] != '[' && s[i] != '>') {
+            if (s[i] == '"' || s[i] == '\'') { char q = s[i++]; while (i < n && s[i] != q) i++; }
+            i++;
+        }
+        if (i < n && s[i] == '[') {
+            i++;
+            for (;;) {
+                ws();
+                if (i >= n) return fail("unterminated DOCTYPE");
+                if (s[i] == ']') { i++; break; }
+                if (starts("<!--")) {
+                    size_t e = std::string(s + i).find("-->");
+                    if (e == std::string::npos) return fail("unterminated comment");
+                    i += e + 3;
+                } else if (starts("<!ENTITY")) {
+                    i += 8; ws();
+                    size_t b = i;
+                    while (i < n && !isspace((unsigned char)s[i])) i++;
+                    std::string name(s + b, i - b);
+                    ws();
+                    if (i >= n || (s[i] != '"' && s[i] != '\'')) return fail("bad ENTITY declaration");
+                    char q = s[i++];
+                    b = i;
+                    while (i < n && s[i] != q) i++;
+                    entities[name] = std::string(s + b, i - b);
+                    i++;
+                    while (i < n && s[i] != '>') i++;
+                    i++;
+                } else {
+                    while (i < n && s[i] != '>') i++;
+                    i++;
+                }
+            }
+        }
+        ws();
+        if (i < n && s[i] == '>') i++;
+        return true;
+    }
+    bool element(XNode &node) {
+        if (s[i] != '<') return fail("expected an element");
+        i++;
+        size_t b = i;
+        while (i < n && (isalnum((unsigned char)s[i]) || s[i] == '_' || s[i] == ':' || s[i] == '-' || s[i] == '.')) i++;
+        node.name.assign(s + b, i - b);
+        for (;;) {
+            ws();
+            if (i >= n) return fail("unterminated element " + node.name);
+            if (starts("/>")) { i += 2; return true; }
+            if (s[i] == '>') { i++; break; }
+            b = i;
+            while (i < n && s[i] != '=' && !isspace((unsigned char)s[i])) i++;
+            std::string an(s + b, i - b);
+            ws();
+            if (s[i] != '=') return fail("attribute without value in " + node.name);
+            i++; ws();
+            if (s[i] != '"' && s[i] != '\'') return fail("unquoted attribute in " + node.name);
+            char q = s[i++];
+            b = i;
+            while (i < n && s[i] != q) i++;
+            node.attrs.emplace_back(an, expand(std::string(s + b, i - b)));
+            i++;
+        }
+        // content
+        for (;;) {
+            if (i >= n) return fail("unterminated element " + node.name);
+            if (starts("</")) {
+                i += 2;
+                size_t e = i;
+                while (e < n && s[e] != '>') e++;
+                std::string close(s + i, e - i);
+                while (!close.empty() && isspace((unsigned char)close.back())) close.pop_back();
+                if (close != node.name) return fail("mismatched </" + close + "> for <" + node.name + ">");
+                i = e + 1;
+                node.text = expand(node.text);
+                return true;
+            }
+            if (starts("<!--")) {
+                size_t e = std::string(s + i).find("-->");
+                if (e == std::string::npos) return fail("unterminated comment");
+                i += e + 3;
+                continue;
+            }
+            if (starts("<![CDATA[")) {
+                size_t e = std::string(s + i).find("]]>");
+                if (e == std::string::npos) return fail("unterminated CDATA");
+                node.text.append(s + i + 9, e - 9);
+                i += e + 3;
+                continue;
+            }
+            if (s[i] == '<') {
+                node.kids.emplace_back(new XNode());
+                if (!element(*node.kids.back())) return false;
+                continue;
+            }
+            node.text += s[i++];
+        }
+    }
+};
+
+double num(const char *v, double dflt = 0.) { return v ? strtod(v, nullptr) : dflt; }
+double textnum(const XNode &x) { return strtod(x.text.c_str(), nullptr); }
+
+// SEC_UNITS attribute scaling (Common/System/UnitsController.cpp:455-465)
+double time_units(const XNode &x) {
+    const char *u = x.attr("units");
+    if (!u) return 1.;
+    std::string s(u);
+    if (s == "msec" || s == "ms") return 1.e-3;
+    if (s == "microsec" || s == "us") return 1.e-6;
+    return 1.;
+}
+
+// LENGTH_UNITS and VELOCITY_UNITS attribute scaling to mm and mm/s
+double length_units(const XNode &x) {
+    const char *u = x.attr("units");
+    if (!u) return 1.;
+    std::string s(u);
+    if (s == "m") return 1.e3;
+    if (s == "cm") return 10.;
+    if (s == "microns") return 1.e-3;
+    if (s == "in") return 25.4;
+    return 1.;
+}
+double velocity_units(const XNode &x) {
+    const char *u = x.attr("units");
+    if (!u) return 1.;
+    std::string s(u);
+    if (s == "m/sec" || s == "m/s") return 1.e3;
+    if (s == "mm/msec" || s == "mm/ms") return 1.e3;
+    if (s == "cm/sec" || s == "cm/s") return 10.;
+    return 1.;
+}
+
+struct Reader {
+    MpmProblem &p;
+    std::map<std::string, int> matByName;
+    explicit Reader(MpmProblem &pp) : p(pp) {}
+    bool fail(const std::string &m) { if (p.err.empty()) p.err = m; return false; }
+
+    int mat_id(const XNode &x, const char *numAttr) {
+        const char *mn = x.attr("matname");
+        if (mn) {
+            auto it = matByName.find(mn);
+            return it == matByName.end() ? -1 : it->second;
+        }
+        return (int)num(x.attr(numAttr), 0.);
+    }
+
+    bool header(const XNode &h) {
+        for (auto &k : h.kids)
+            if (k->name == "Analysis") {
+                int a = (int)textnum(*k);
+                if (a != MPMGPU_PLANE_STRAIN && a != MPMGPU_THREED)
+                    return fail("<Analysis> " + std::to_string(a) + " is not on this path (10 plane strain or 12 3D)");
+                p.analysis = a;
+                p.dim = a == MPMGPU_THREED ? 3 : 2;
+            }
+        return true;
+    }
+
+    bool mpmheader(const XNode &h) {
+        for (auto &kp : h.kids) {
+            const XNode &k = *kp;
+            if (k.name == "MPMMethod") {
+                if ((int)textnum(k) != 2) return fail("only MPMMethod 2 (USAVG) is on this path");
+            } else if (k.name == "MatlPtsPerElement") {
+                int np = (int)textnum(k);
+                int side = p.dim == 3 ? (int)(cbrt((double)np) + 0.5) : (int)(sqrt((double)np) + 0.5);
+                if ((p.dim == 3 && side * side * side != np) || (p.dim == 2 && side * side != np))
+                    return fail("MatlPtsPerElement must be a square (2D) or a cube (3D)");
+                p.ptsPerSide = side;
+            } else if (k.name == "Timing") {
+                double sc = time_units(k);
+                if (k.attr("CFL")) p.cfl = num(k.attr("CFL"));
+                if (k.attr("step")) p.timestepIn = num(k.attr("step")) * sc;
+                if (k.attr("max")) p.maxTime = num(k.attr("max")) * sc;
+            } else if (k.name == "TimeFactor") {
+                p.cfl = textnum(k);
+            } else if (k.name == "MaxTime") {
+                p.maxTime = textnum(k) * time_units(k);
+            } else if (k.name == "TimeStep") {
+                p.timestepIn = textnum(k) * time_units(k);
+            } else if (k.name == "GIMP") {
+                const char *t = k.attr("type");
+                std::string v = t ? t : "uGIMP";
+                if (v == "uGIMP" || v == "GIMP" || v == "1") p.run.shape = MPMGPU_SHAPE_UGIMP;
+                else if (v == "Dirac" || v == "Classic" || v == "0") p.run.shape = MPMGPU_SHAPE_LINEAR;
+                else if (v == "lCPDI" || v == "CPDI" || v == "2") p.run.shape = MPMGPU_SHAPE_LCPDI;
+                else return fail("GIMP type " + v + " is not on this path");
+            } else if (k.name == "MultiMaterialMode") {
+                p.run.multiMaterial = 1;
+                if (k.attr("Normals")) p.run.normalsMethod = (int)num(k.attr("Normals"));
+                for (auto &c : k.kids)
+                    if (c->name == "Friction" && c->attr("mat")) p.run.contactLaw = (int)num(c->attr("mat")) - 1;
+            } else if (k.name == "ArchiveRoot") {
+                p.archiveRoot = k.text;
+            } else if (k.name == "LeaveLimit") {
+                p.run.maxLeaving = (int)textnum(k);
+            }
+            // archive, global-archive and output-only settings do not change the step
+        }
+        return true;
+    }
+
+    bool mesh(const XNode &m) {
+        for (auto &gp : m.kids) {
+            if (gp->name != "Grid") continue;
+            const XNode &g = *gp;
+            double ls = length_units(g);
+            double ext[6] = {num(g.attr("xmin")) * ls, num(g.attr("xmax")) * ls, num(g.attr("ymin")) * ls,
+                             num(g.attr("ymax")) * ls, num(g.attr("zmin")) * ls, num(g.attr("zmax")) * ls};
+            double cell[3] = {-1., -1., -1.};
+            int nc[3] = {0, 0, 0};
+            if (g.attr("thickness")) p.thickness = num(g.attr("thickness"));
+            const char *axes[3] = {"Horiz", "Vert", "Depth"};
+            for (auto &ap : g.kids)
+                for (int a = 0; a < 3; a++)
+                    if (ap->name == axes[a]) {
+                        if (ap->attr("cellsize")) cell[a] = num(ap->attr("cellsize")) * ls;
+                        if (ap->attr("nx")) nc[a] = (int)num(ap->attr("nx"));
+                    }
+            p.haveGrid = true;
+            p.gxmin = ext[0]; p.gxmax = ext[1]; p.gymin = ext[2]; p.gymax = ext[3]; p.gzmin = ext[4]; p.gzmax = ext[5];
+            p.cellx = cell[0]; p.celly = cell[1]; p.cellz = cell[2];
+            p.nhoriz = nc[0]; p.nvert = nc[1]; p.ndepth = nc[2];
+            if (!mpmsetup::build_grid(p)) return false;
+        }
+        return true;
+    }
+
+    bool body(const XNode &b) {
+        int mat = mat_id(b, "mat");
+        if (mat <= 0) return fail("Positive material ID needed in <Body> element.");
+        double vel[3] = {num(b.attr("vx")), num(b.attr("vy")), num(b.attr("vz"))};
+        double thick = b.attr("thick") ? num(b.attr("thick")) : -1.;
+        int savedSide = p.ptsPerSide;
+        for (auto &sp : b.kids) {
+            const XNode &s = *sp;
+            if (s.name == "MatlPtsPerElement") {
+                int np = (int)textnum(s);
+                p.ptsPerSide = p.dim == 3 ? (int)(cbrt((double)np) + 0.5) : (int)(sqrt((double)np) + 0.5);
+                continue;
+            }
+            double ls = length_units(s);
+            double ext[6] = {num(s.attr("xmin")) * ls, num(s.attr("xmax")) * ls, num(s.attr("ymin")) * ls,
+                             num(s.attr("ymax")) * ls, num(s.attr("zmin")) * ls, num(s.attr("zmax")) * ls};
+            int kind = -1, axis = 0;
+            if (s.name == "Box") {
+                kind = 0;
+                if (const char *ax = s.attr("axis")) {
+                    std::string a(ax);
+                    axis = (a == "x" || a == "X" || a == "1") ? 1 : (a == "y" || a == "Y" || a == "2") ? 2
+                         : (a == "z" || a == "Z" || a == "3") ? 3 : 0;
+                }
+            } else if (s.name == "Cylinder") {
+                kind = 0;
+                axis = 3;
+                if (const char *ax = s.attr("axis")) {
+                    std::string a(ax);
+                    axis = (a == "x" || a == "X" || a == "1") ? 1 : (a == "y" || a == "Y" || a == "2") ? 2 : 3;
+                }
+            } else if (s.name == "Rect") kind = 1;
+            else if (s.name == "Oval") kind = 2;
+            else if (s.name == "Sphere") kind = 3;
+            else return fail("<" + s.name + "> in <Body> is not on this path");
+            if (!s.kids.empty()) return fail("nested shapes (holes) in <Body> are not on this path");
+            if (!mpmsetup::add_body(p, mat, kind, ext, axis, vel, thick)) return false;
+        }
+        p.ptsPerSide = savedSide;
+        return true;
+    }
+
+    bool points(const XNode &mp) {
+        // <PointList><mp matl= lpx=><pt x= y= z=/><vel x= y=/><mass m=/></mp></PointList>
+        for (auto &xp : mp.kids) {
+            const XNode &x = *xp;
+            if (x.name == "Body") { if (!body(x)) return false; continue; }
+            if (x.name == "PointList") { if (!points(x)) return false; continue; }
+            if (x.name != "mp") return fail("<" + x.name + "> in <MaterialPoints> is not on this path");
+            int mat = x.attr("matname") ? mat_id(x, "matl") : (int)num(x.attr("matl") ? x.attr("matl") : x.attr("mat"));
+            double lp[3] = {num(x.attr("lpx"), 0.5), num(x.attr("lpy"), 0.5), num(x.attr("lpz"), 0.5)};
+            double thick = x.attr("thick") ? num(x.attr("thick")) : -1.;
+            double pos[3] = {0., 0., 0.}, vel[3] = {0., 0., 0.}, mass = -1.;
+            for (auto &cp : x.kids) {
+                const XNode &c = *cp;
+                if (c.name == "pt") {
+                    double sc = length_units(c);
+                    pos[0] = num(c.attr("x")) * sc; pos[1] = num(c.attr("y")) * sc; pos[2] = num(c.attr("z")) * sc;
+                } else if (c.name == "vel") {
+                    double sc = velocity_units(c);
+                    vel[0] = num(c.attr("x")) * sc; vel[1] = num(c.attr("y")) * sc; vel[2] = num(c.attr("z")) * sc;
+                } else if (c.name == "mass") {
+                    double m = num(c.attr("m"), -1.);
+                    if (m > 0.) mass = m;
+                }
+            }
+            if (!mpmsetup::add_point(p, mat, pos, vel, lp, thick, mass)) return false;
+        }
+        return true;
+    }
+
+    bool material(const XNode &m) {
+        MaterialInput in;
+        in.type = (int)num(m.attr("Type"));
+        in.name = m.attr("Name") ? m.attr("Name") : "";
+        for (auto &kp : m.kids) {
+            const XNode &k = *kp;
+            double v = textnum(k);
+            const std::string &n = k.name;
+            if (n == "rho") in.rho = v;
+            else if (n == "E" || n == "Etens") { in.E = v; in.readE = true; }
+            else if (n == "nu") { in.nu = v; in.readNu = true; }
+            else if (n == "G" || n == "G1") { in.G = v; in.readG = true; }
+            else if (n == "K" || n == "Kbulk") in.K = v;
+            else if (n == "Lame") in.Lame = v;
+            else if (n == "yield") in.yield = v;
+            else if (n == "yieldMin") in.yieldMin = v;
+            else if (n == "Ep") in.Ep = v;
+            else if (n == "Khard") in.Khard = v;
+            else if (n == "coeff") in.friction = v;
+            else if (n == "UJOption") in.ujOption = (int)v;
+            else if (n == "Hardening") {
+                if (k.text.find("Linear") == std::string::npos && (int)v != 1 && !k.text.empty() && k.text.find_first_not_of(" \t\r\n") != std::string::npos)
+                    return fail("only linear hardening is on this path");
+            }
+            // thermal, transport and output-only properties do not act on this path
+        }
+        int id = (int)p.matIn.size() + 1;
+        if (!in.name.empty()) matByName[in.name] = id;
+        mpmsetup::add_material(p, in);
+        return true;
+    }
+
+    bool run(const XNode &root) {
+        if (root.name != "JANFEAInput") return fail("root element must be <JANFEAInput>");
+        p.run.shape = MPMGPU_SHAPE_LINEAR;       // ElementBase::useGimp default until <GIMP>
+        p.run.mpmMethod = 2;
+        p.run.contactLaw = -1;
+        // headers and materials first (materials may follow the points in the file)
+        for (auto &k : root.kids) if (k->name == "Header" && !header(*k)) return false;
+        for (auto &k : root.kids) if (k->name == "MPMHeader" && !mpmheader(*k)) return false;
+        for (auto &k : root.kids) if (k->name == "Material" && !material(*k)) return false;
+        for (auto &k : root.kids) if (k->name == "Mesh" && !mesh(*k)) return false;
+        for (auto &k : root.kids) {
+            if (k->name == "MaterialPoints" && !points(*k)) return false;
+            if (k->name == "PointList" && !points(*k)) return false;
+        }
+        for (auto &k : root.kids) {
+            const std::string &n = k->name;
+            if (n == "GridBCs" || n == "ParticleBCs" || n == "CrackList" || n == "Thermal" || n == "CustomTasks" || n == "Gravity")
+                if (!k->kids.empty() || !k->text.empty())
+                    return fail("<" + n + "> is not on this path");
+        }
+        return mpmsetup::finalize(p);
+    }
+};
+
+}  // namespace
+
+bool mpm_read_xml(const char *text, MpmProblem &p) {
+    Parser ps(text);
+    if (!ps.skip_misc()) { p.err = ps.err; return false; }
+    XNode root;
+    if (!ps.element(root)) { p.err = ps.err; return false; }
+    Reader r(p);
+    return r.run(root);
+}

@@ -1,0 +1,284 @@
+"""Python binding of the MI355X explicit MPM step (libnairnmpm_gpu.so).
+
+The library's C ABI is declared in include/nairnmpm_gpu.h. This module mirrors the reference's
+structure: a :class:`Problem` is what NairnMPM holds after reading an XML input file
+(MPMReadHandler, Read_MPM/Generators.cpp) and preparing particles (NairnMPM::CMPreparations,
+NairnMPM_Class/NairnMPM.cpp:135-168); a :class:`Simulation` is the device state the MPMTask list
+works on (NairnMPM::MPMStep, NairnMPM.cpp:284-335). Errors are raised as :class:`MPMError`, the
+counterpart of the reference's CommonException.
+
+There is no CPU fallback: if the HIP library is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBDIR = os.path.dirname(_HERE)
+LIBRARY_PATH = os.path.join(_LIBDIR, "libnairnmpm_gpu.so")
+
+# analysis types, shape functions, material types (include/nairnmpm_gpu.h)
+PLANE_STRAIN, PLANE_STRESS, THREED = 10, 11, 12
+SHAPE_LINEAR, SHAPE_UGIMP, SHAPE_LCPDI = 0, 1, 10
+MAT_ISOTROPIC, MAT_MOONEY, MAT_ISOPLASTICITY, MAT_NEOHOOKEAN, MAT_CONTACT, MAT_COULOMB = 1, 8, 9, 28, 60, 61
+
+FIELDS = [
+    "pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "mass", "lp_x", "lp_y", "lp_z",
+    "ep_xx", "ep_yy", "ep_zz", "ep_yz", "ep_xz", "ep_xy", "wr_yz", "wr_xz", "wr_xy",
+    "sp_xx", "sp_yy", "sp_zz", "sp_yz", "sp_xz", "sp_xy", "pressure",
+    "alt_xx", "alt_yy", "alt_zz", "alt_yz", "alt_xz", "alt_xy", "hist0", "hist1",
+    "orig_x", "orig_y", "orig_z",
+]
+F = {name: i for i, name in enumerate(FIELDS)}
+NUM_FIELDS = len(FIELDS)
+IFIELDS = ["elem", "mat", "id", "left"]
+NUM_IFIELDS = len(IFIELDS)
+
+TASKS = [
+    "Initialize", "Extrapolate Mass and Momentum", "Post Extrapolation Tasks",
+    "Update Strains First", "Extrapolate Grid Forces", "Post Force Extrapolation Tasks",
+    "Update Momenta", "Update Particles", "Update Strains Last with Extrapolation", "Reset Elements",
+]
+
+
+class MPMError(RuntimeError):
+    """Failure reported by the library (the reference's CommonException)."""
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("analysis", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                ("nz", ctypes.c_int32), ("xmin", ctypes.c_double), ("ymin", ctypes.c_double),
+                ("zmin", ctypes.c_double), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("dz", ctypes.c_double), ("thickness", ctypes.c_double)]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("field", ctypes.c_int32), ("rho", ctypes.c_double),
+                ("E", ctypes.c_double), ("nu", ctypes.c_double), ("G", ctypes.c_double),
+                ("K", ctypes.c_double), ("C", (ctypes.c_double * 6) * 6), ("Gred", ctypes.c_double),
+                ("Kred", ctypes.c_double), ("yield_", ctypes.c_double), ("Ep", ctypes.c_double),
+                ("yldred", ctypes.c_double), ("yldredMin", ctypes.c_double), ("Epred", ctypes.c_double),
+                ("alphaMax", ctypes.c_double), ("Gsp", ctypes.c_double), ("Lamesp", ctypes.c_double),
+                ("Ksp", ctypes.c_double), ("G1sp", ctypes.c_double), ("G2sp", ctypes.c_double),
+                ("waveSpeed", ctypes.c_double), ("friction", ctypes.c_double),
+                ("ujOption", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class RunParams(ctypes.Structure):
+    _fields_ = [("timestep", ctypes.c_double), ("strainFirst", ctypes.c_double),
+                ("strainLast", ctypes.c_double), ("shape", ctypes.c_int32), ("mpmMethod", ctypes.c_int32),
+                ("multiMaterial", ctypes.c_int32), ("contactLaw", ctypes.c_int32),
+                ("normalsMethod", ctypes.c_int32), ("deleteLeaving", ctypes.c_int32),
+                ("maxLeaving", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("nparticles", ctypes.c_int64), ("leftGrid", ctypes.c_int64),
+                ("nanParticles", ctypes.c_int64), ("time", ctypes.c_double), ("lastStepMs", ctypes.c_double)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int32)
+
+_SIGNATURES = {
+    "mpmgpu_global_error": ([], ctypes.c_char_p),
+    "mpmgpu_last_error": ([_P], ctypes.c_char_p),
+    "mpmgpu_problem_create": ([ctypes.c_int32, ctypes.POINTER(_P)], ctypes.c_int),
+    "mpmgpu_problem_set_grid": ([_P, _PD, _PD, _PI, ctypes.c_double], ctypes.c_int),
+    "mpmgpu_problem_set": ([_P, ctypes.c_char_p, ctypes.c_double], ctypes.c_int),
+    "mpmgpu_problem_add_material": ([_P, ctypes.c_int32, ctypes.c_char_p, _PI], ctypes.c_int),
+    "mpmgpu_problem_material_property": ([_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_double], ctypes.c_int),
+    "mpmgpu_problem_add_body": ([_P, ctypes.c_int32, ctypes.c_char_p, _PD, ctypes.c_int32, _PD, ctypes.c_double], ctypes.c_int),
+    "mpmgpu_problem_add_point": ([_P, ctypes.c_int32, _PD, _PD, _PD, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "mpmgpu_problem_from_xml": ([ctypes.c_char_p, ctypes.POINTER(_P)], ctypes.c_int),
+    "mpmgpu_problem_describe": ([_P, ctypes.POINTER(GridDesc), ctypes.POINTER(Material), _PI,
+                                 ctypes.POINTER(RunParams), ctypes.POINTER(ctypes.c_int64), _PD], ctypes.c_int),
+    "mpmgpu_problem_particles": ([_P, _PD, _PI], ctypes.c_int),
+    "mpmgpu_problem_destroy": ([_P], None),
+    "mpmgpu_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Material), ctypes.c_int32,
+                       ctypes.POINTER(RunParams), ctypes.POINTER(_P)], ctypes.c_int),
+    "mpmgpu_upload_particles": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
+    "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_step": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_sync": ([_P], ctypes.c_int),
+    "mpmgpu_download_particles": ([_P, _PD, _PI], ctypes.c_int),
+    "mpmgpu_query": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "mpmgpu_download_grid": ([_P, _PD, _PD, _PD], ctypes.c_int),
+    "mpmgpu_destroy": ([_P], None),
+}
+
+
+def library() -> ctypes.CDLL:
+    """Load libnairnmpm_gpu.so from this package's directory (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise MPMError(f"{LIBRARY_PATH} is missing: build it with `make -C nairn-mpm-fea_amd` "
+                           "(or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIBRARY_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def _check(status: int, ctx=None):
+    if status != 0:
+        lib = library()
+        msg = lib.mpmgpu_last_error(ctx) if ctx else lib.mpmgpu_global_error()
+        raise MPMError(msg.decode() if msg else "unknown error")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_PD)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_PI)
+
+
+class Problem:
+    """Input and generated particles (MPMReadHandler + PreliminaryParticleCalcs)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_xml(cls, text: str) -> "Problem":
+        h = _P()
+        _check(library().mpmgpu_problem_from_xml(text.encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls, analysis: int) -> "Problem":
+        h = _P()
+        _check(library().mpmgpu_problem_create(analysis, ctypes.byref(h)))
+        return cls(h)
+
+    def set_grid(self, ext6, cell3=(-1., -1., -1.), n3=(0, 0, 0), thickness=1.):
+        e = np.ascontiguousarray(ext6, dtype=np.float64)
+        c = np.ascontiguousarray(cell3, dtype=np.float64)
+        n = np.ascontiguousarray(n3, dtype=np.int32)
+        _check(library().mpmgpu_problem_set_grid(self._h, _dptr(e), _dptr(c), _iptr(n), thickness))
+
+    def set(self, key: str, value: float):
+        _check(library().mpmgpu_problem_set(self._h, key.encode(), float(value)))
+
+    def add_material(self, mtype: int, name: str = "", **props) -> int:
+        mid = ctypes.c_int32()
+        _check(library().mpmgpu_problem_add_material(self._h, mtype, name.encode(), ctypes.byref(mid)))
+        for k, v in props.items():
+            _check(library().mpmgpu_problem_material_property(self._h, mid.value, k.encode(), float(v)))
+        return mid.value
+
+    def add_body(self, mat: int, shape: str, ext6, vel=(0., 0., 0.), axis: int = 0, thick: float = -1.):
+        e = np.ascontiguousarray(ext6, dtype=np.float64)
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        _check(library().mpmgpu_problem_add_body(self._h, mat, shape.encode(), _dptr(e), axis, _dptr(v), thick))
+
+    def add_point(self, mat: int, pos, vel=(0., 0., 0.), lp=(0.5, 0.5, 0.5), thick: float = -1., mass: float = -1.):
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        l = np.ascontiguousarray(lp, dtype=np.float64)
+        _check(library().mpmgpu_problem_add_point(self._h, mat, _dptr(p), _dptr(v), _dptr(l), thick, mass))
+
+    def describe(self):
+        g, mats, nm, run = GridDesc(), (Material * 16)(), ctypes.c_int32(), RunParams()
+        n, tmax = ctypes.c_int64(), ctypes.c_double()
+        _check(library().mpmgpu_problem_describe(self._h, ctypes.byref(g), mats, ctypes.byref(nm),
+                                                 ctypes.byref(run), ctypes.byref(n), ctypes.byref(tmax)))
+        return g, [mats[i] for i in range(nm.value)], run, n.value, tmax.value
+
+    def particles(self):
+        """Initial particle state in mpm[] order: (double fields [NUM_FIELDS, n], int fields [4, n])."""
+        _, _, _, n, _ = self.describe()
+        d = np.zeros((NUM_FIELDS, n), dtype=np.float64)
+        i = np.zeros((NUM_IFIELDS, n), dtype=np.int32)
+        _check(library().mpmgpu_problem_particles(self._h, _dptr(d), _iptr(i)))
+        return d, i
+
+    def close(self):
+        if self._h:
+            library().mpmgpu_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Simulation:
+    """Device state of one simulation; step() runs whole NairnMPM steps on the GPU."""
+
+    def __init__(self, grid: GridDesc, mats, run: RunParams):
+        arr = (Material * len(mats))(*mats)
+        h = _P()
+        _check(library().mpmgpu_create(ctypes.byref(grid), arr, len(mats), ctypes.byref(run), ctypes.byref(h)))
+        self._h = h
+        self.grid, self.run = grid, run
+        self.n = 0
+
+    @classmethod
+    def from_problem(cls, prob: Problem) -> "Simulation":
+        g, mats, run, _, _ = prob.describe()
+        sim = cls(g, mats, run)
+        d, i = prob.particles()
+        sim.upload(d, i)
+        return sim
+
+    def upload(self, dfields: np.ndarray, ifields: np.ndarray):
+        d = np.ascontiguousarray(dfields, dtype=np.float64)
+        i = np.ascontiguousarray(ifields, dtype=np.int32)
+        n = d.shape[1]
+        _check(library().mpmgpu_upload_particles(self._h, n, _dptr(d), _iptr(i)), self._h)
+        self.n = n
+
+    def step(self, nsteps: int = 1):
+        _check(library().mpmgpu_step(self._h, int(nsteps)), self._h)
+
+    def run_task(self, task: int):
+        _check(library().mpmgpu_run_task(self._h, int(task)), self._h)
+
+    def sync(self):
+        _check(library().mpmgpu_sync(self._h), self._h)
+
+    def particles(self):
+        d = np.zeros((NUM_FIELDS, self.n), dtype=np.float64)
+        i = np.zeros((NUM_IFIELDS, self.n), dtype=np.int32)
+        _check(library().mpmgpu_download_particles(self._h, _dptr(d), _iptr(i)), self._h)
+        return d, i
+
+    def grid_values(self):
+        g = self.grid
+        nn = (g.nx + 1) * (g.ny + 1) * ((g.nz + 1) if g.analysis == THREED else 1)
+        m = np.zeros(nn)
+        pk = np.zeros(3 * nn)
+        f = np.zeros(3 * nn)
+        _check(library().mpmgpu_download_grid(self._h, _dptr(m), _dptr(pk), _dptr(f)), self._h)
+        return m, pk.reshape(3, nn), f.reshape(3, nn)
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(library().mpmgpu_query(self._h, ctypes.byref(s)), self._h)
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mpmgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,159 @@
+"""NairnMPM XML inputs of the benchmark workloads, written as a user of the reference would.
+
+Units follow the reference's Legacy input units: lengths in mm, velocities in mm/s, moduli and
+yield stresses in MPa, densities in g/cm^3, times in s (MPMReadHandler.cpp:1163-1170,
+Common/Materials/IsotropicMat.cpp:40, MaterialBaseMPM.cpp:56-58).
+"""
+from __future__ import annotations
+
+import math
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014) -> uniform doubles in [0, 1) from the top 53 bits."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+
+def _header(analysis: int, cfl: float, max_time: float, extra: str = "") -> str:
+    return f"""<?xml version='1.0'?>
+<JANFEAInput version='3'>
+  <Header>
+    <Description>benchmark workload</Description>
+    <Analysis>{analysis}</Analysis>
+  </Header>
+  <MPMHeader>
+    <Timing CFL="{cfl!r}" max="{max_time!r}"/>
+    <GIMP type="uGIMP"/>
+{extra}  </MPMHeader>
+"""
+
+
+def elastic_block_2d(cells: int = 64, block: int = 32, seed: int = 20260101, E: float = 1000.,
+                     nu: float = 0.33, rho: float = 1.0, vmax: float = 10000., cfl: float = 0.2,
+                     jitter: float = 0.05, steps_hint: int = 1000, material_xml: str | None = None) -> str:
+    """Plain 2D plane-strain elastic block: cells x cells grid of 1 mm cells, 2x2 particles per
+    cell in the central block x block cells, each coordinate jittered by U(-jitter, jitter) mm
+    with a SplitMix64 stream, v_x = vmax*sin(pi*(y-y0)/L), v_y = 0 (mm/s)."""
+    rng = SplitMix64(seed)
+    x0 = (cells - block) / 2.0
+    pts = []
+    for j in range(block):
+        for i in range(block):
+            for b in range(2):
+                for a in range(2):
+                    x = x0 + i + 0.25 + 0.5 * a + jitter * (2.0 * rng.uniform() - 1.0)
+                    y = x0 + j + 0.25 + 0.5 * b + jitter * (2.0 * rng.uniform() - 1.0)
+                    vx = vmax * math.sin(math.pi * (y - x0) / block)
+                    pts.append(f'    <mp matl="1" lpx="0.5" lpy="0.5"><pt x="{x!r}" y="{y!r}"/>'
+                               f'<vel x="{vx!r}" y="0.0"/></mp>')
+    mat = material_xml or f"""  <Material Type="1" Name="block">
+    <rho>{rho!r}</rho>
+    <E>{E!r}</E>
+    <nu>{nu!r}</nu>
+  </Material>
+"""
+    return (_header(10, cfl, 1.0) +
+            f"""  <Mesh output="file">
+    <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <PointList>
+""" + "\n".join(pts) + """
+    </PointList>
+  </MaterialPoints>
+""" + mat + "</JANFEAInput>\n")
+
+
+def j2_block_3d(cells: int = 128, lo: int = 16, hi: int = 112, vx: float = 50000., E: float = 70000.,
+                nu: float = 0.33, yield_: float = 300., Ep: float = 1000., rho: float = 2.7,
+                cfl: float = 0.5, ppe: int = 8) -> str:
+    """3D elastic-plastic block: cells^3 grid of 1 mm cells; a box [lo, hi]^3 filled with ppe
+    particles per cell of IsoPlasticity (J2, linear hardening); the half with x < mid moves at
+    +vx, the other half at -vx (mm/s): two halves colliding."""
+    mid = (lo + hi) / 2.0
+    return (_header(12, cfl, 1.0, f"    <MatlPtsPerElement>{ppe}</MatlPtsPerElement>\n") +
+            f"""  <Mesh output="file">
+    <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}" zmin="0" zmax="{cells}">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+      <Depth cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <Body mat="1" vx="{vx!r}" vy="0" vz="0">
+      <Box xmin="{lo}" xmax="{mid!r}" ymin="{lo}" ymax="{hi}" zmin="{lo}" zmax="{hi}"/>
+    </Body>
+    <Body mat="1" vx="{-vx!r}" vy="0" vz="0">
+      <Box xmin="{mid!r}" xmax="{hi}" ymin="{lo}" ymax="{hi}" zmin="{lo}" zmax="{hi}"/>
+    </Body>
+  </MaterialPoints>
+  <Material Type="9" Name="J2 block">
+    <rho>{rho!r}</rho>
+    <E>{E!r}</E>
+    <nu>{nu!r}</nu>
+    <yield>{yield_!r}</yield>
+    <Ep>{Ep!r}</Ep>
+  </Material>
+</JANFEAInput>
+""")
+
+
+def neohookean_block_3d(cells: int = 32, lo: int = 8, hi: int = 24, vx: float = 20000., G: float = 10.,
+                        K: float = 50., rho: float = 1.1, cfl: float = 0.5) -> str:
+    """3D neo-Hookean block (C3/C5 material) with two colliding halves."""
+    mid = (lo + hi) / 2.0
+    return (_header(12, cfl, 1.0, "    <MatlPtsPerElement>8</MatlPtsPerElement>\n") +
+            f"""  <Mesh output="file">
+    <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}" zmin="0" zmax="{cells}">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+      <Depth cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <Body mat="1" vx="{vx!r}" vy="0" vz="0">
+      <Box xmin="{lo}" xmax="{mid!r}" ymin="{lo}" ymax="{hi}" zmin="{lo}" zmax="{hi}"/>
+    </Body>
+    <Body mat="1" vx="{-vx!r}" vy="0" vz="0">
+      <Box xmin="{mid!r}" xmax="{hi}" ymin="{lo}" ymax="{hi}" zmin="{lo}" zmax="{hi}"/>
+    </Body>
+  </MaterialPoints>
+  <Material Type="28" Name="rubber">
+    <rho>{rho!r}</rho>
+    <G>{G!r}</G>
+    <K>{K!r}</K>
+  </Material>
+</JANFEAInput>
+""")
+
+
+def j2_block_2d(cells: int = 48, block: int = 24, seed: int = 7, E: float = 2000., nu: float = 0.3,
+                yield_: float = 10., Ep: float = 20., rho: float = 1.0, vmax: float = 20000.,
+                cfl: float = 0.5) -> str:
+    """2D plane-strain J2 block (the C4 sweep's plastic member), C1 layout and velocity profile."""
+    mat = f"""  <Material Type="9" Name="J2 block">
+    <rho>{rho!r}</rho>
+    <E>{E!r}</E>
+    <nu>{nu!r}</nu>
+    <yield>{yield_!r}</yield>
+    <Ep>{Ep!r}</Ep>
+  </Material>
+"""
+    return elastic_block_2d(cells=cells, block=block, seed=seed, vmax=vmax, cfl=cfl, material_xml=mat)

@@ -1,0 +1,981 @@
+/*
+ * mpm_oracle.cpp -- CPU restatement of NairnMPM's explicit time step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker of the HIP path: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The product never links it.
+ *
+ * It restates, in the reference's order of operations, the hot path of one MPM step for the
+ * configurations of this project: single- or multi-material (multimaterial contact is not yet
+ * restated here), uGIMP or linear shape functions, USAVG+ with FLIP, no cracks, no transport,
+ * no boundary conditions. Every function cites the reference file:line it follows (paths are
+ * relative to the reference root). Grid sums are accumulated particle by particle in mpm[]
+ * order, which is the reference's summation order when it runs as a single patch (-np 1).
+ *
+ * Pinned by tests/test_oracle_*.py against closed forms, invariants and brute force (there are
+ * no reference golden vectors: the reference needs Xerces-C, absent from this image).
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../include/nairnmpm_gpu.h"
+
+namespace {
+
+const double SQRT_TWOTHIRDS = 0.8164965809277260;   // System/MPMPrefix.hpp:92
+
+struct Vec3 { double x, y, z; };
+
+// ---------------------------------------------------------------------------------------
+// 3x3 matrix with the reference's 2D flag (Common/System/Matrix3.cpp:66-160)
+struct Mat3 {
+    double m[3][3];
+    bool is2D;
+    Mat3() { zero(); }
+    void zero() { for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m[i][j] = 0.; is2D = true; }
+    static Mat3 full(double xx, double xy, double xz, double yx, double yy, double yz,
+                     double zx, double zy, double zz) {          // Matrix3.cpp:75-89
+        Mat3 a; a.m[0][0] = xx; a.m[0][1] = xy; a.m[0][2] = xz;
+        a.m[1][0] = yx; a.m[1][1] = yy; a.m[1][2] = yz;
+        a.m[2][0] = zx; a.m[2][1] = zy; a.m[2][2] = zz; a.is2D = false; return a;
+    }
+    static Mat3 two(double xx, double xy, double yx, double yy, double zz) {   // :92-104
+        Mat3 a; a.m[0][0] = xx; a.m[0][1] = xy; a.m[1][0] = yx; a.m[1][1] = yy; a.m[2][2] = zz;
+        a.is2D = true; return a;
+    }
+    void add(const Mat3 &r) {                                    // operator+= :884-892
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) m[i][j] += r.m[i][j];
+        if (!r.is2D) is2D = false;
+    }
+    void scale(double f) {                                       // Scale :410-424
+        m[0][0] *= f; m[0][1] *= f; m[1][0] *= f; m[1][1] *= f; m[2][2] *= f;
+        if (!is2D) { m[0][2] *= f; m[1][2] *= f; m[2][0] *= f; m[2][1] *= f; }
+    }
+    Mat3 mul(const Mat3 &r) const {                              // operator*= :920-966
+        Mat3 res;
+        if (is2D && r.is2D) {
+            res.m[0][0] = m[0][0] * r.m[0][0] + m[0][1] * r.m[1][0];
+            res.m[0][1] = m[0][0] * r.m[0][1] + m[0][1] * r.m[1][1];
+            res.m[1][0] = m[1][0] * r.m[0][0] + m[1][1] * r.m[1][0];
+            res.m[1][1] = m[1][0] * r.m[0][1] + m[1][1] * r.m[1][1];
+            res.m[2][2] = m[2][2] * r.m[2][2];
+            res.is2D = true;
+        } else {
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double s = 0.;
+                    for (int k = 0; k < 3; k++) s += m[i][k] * r.m[k][j];
+                    res.m[i][j] = s;
+                }
+            res.is2D = false;
+        }
+        return res;
+    }
+    double trace() const { return m[0][0] + m[1][1] + m[2][2]; }   // :1192
+    double determinant() const {                                    // :1195-1202
+        if (is2D) return m[2][2] * (m[0][0] * m[1][1] - m[1][0] * m[0][1]);
+        return m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2])
+             - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2])
+             + m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+    }
+    // Exponential(kmax) for kmax = 1 or 2 (Matrix3.cpp:312-394)
+    Mat3 exponential(int kmax) const {
+        if (is2D) {
+            if (kmax == 1) return two(1. + m[0][0], m[0][1], m[1][0], 1. + m[1][1], 1. + m[2][2]);
+            double c0 = m[0][1] * m[1][0] - m[0][0] * m[1][1];
+            double c1 = m[0][0] + m[1][1];
+            double beta0 = 0., beta1 = 1., alpha0 = 1., alpha1 = 1.;
+            double betaz = m[2][2], ezz = 1. + betaz;
+            for (int k = 2; k <= kmax; k++) {
+                double factor = 1 / (double)k;
+                double temp = beta1;
+                beta1 = factor * (c1 * temp + beta0);
+                beta0 = factor * c0 * temp;
+                betaz *= factor * m[2][2];
+                alpha0 += beta0;
+                alpha1 += beta1;
+                ezz += betaz;
+            }
+            return two(alpha0 + alpha1 * m[0][0], alpha1 * m[0][1], alpha1 * m[1][0],
+                       alpha0 + alpha1 * m[1][1], ezz);
+        }
+        if (kmax == 1)
+            return full(1. + m[0][0], m[0][1], m[0][2], m[1][0], 1. + m[1][1], m[1][2],
+                        m[2][0], m[2][1], 1. + m[2][2]);
+        Mat3 m2 = *this;
+        m2 = m2.mul(m2);
+        return full(1. + m[0][0] + 0.5 * m2.m[0][0], m[0][1] + 0.5 * m2.m[0][1], m[0][2] + 0.5 * m2.m[0][2],
+                    m[1][0] + 0.5 * m2.m[1][0], 1. + m[1][1] + 0.5 * m2.m[1][1], m[1][2] + 0.5 * m2.m[1][2],
+                    m[2][0] + 0.5 * m2.m[2][0], m[2][1] + 0.5 * m2.m[2][1], 1. + m[2][2] + 0.5 * m2.m[2][2]);
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// uGIMP node tables (Common/Elements/FourNodeIsoparam.cpp:27-31,
+// NairnMPM/src/Elements/EightNodeIsoparamBrick.cpp:23-53). Natural coordinate of a candidate
+// node is 2*off-1 on each axis.
+const int xoff2[16] = {0, 1, 1, 0, -1, 0, 1, 2, 2, 2, 2, 1, 0, -1, -1, -1};
+const int yoff2[16] = {0, 0, 1, 1, -1, -1, -1, -1, 0, 1, 2, 2, 2, 2, 1, 0};
+const int xoff3[64] = {0,1,1,0,0,1,1,0,
+    -1,0,1,2,2,2,2,1,0,-1,-1,-1,
+    -1,0,1,2,2,2,2,1,0,-1,-1,-1,
+    0,1,1,0,-1,0,1,2,2,2,2,1,0,-1,-1,-1,
+    0,1,1,0,-1,0,1,2,2,2,2,1,0,-1,-1,-1};
+const int yoff3[64] = {0,0,1,1,0,0,1,1,
+    -1,-1,-1,-1,0,1,2,2,2,2,1,0,
+    -1,-1,-1,-1,0,1,2,2,2,2,1,0,
+    0,0,1,1,-1,-1,-1,-1,0,1,2,2,2,2,1,0,
+    0,0,1,1,-1,-1,-1,-1,0,1,2,2,2,2,1,0};
+const int zoff3[64] = {0,0,0,0,1,1,1,1,
+    0,0,0,0,0,0,0,0,0,0,0,0,
+    1,1,1,1,1,1,1,1,1,1,1,1,
+    -1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,
+    2,2,2,2,2,2,2,2,2,2,2,2,2,2,2,2};
+// linear element node order (FourNodeIsoparam.cpp:21-22, EightNodeIsoparamBrick.cpp:18-20)
+const double xii4[4] = {-1., 1., 1., -1.};
+const double eti4[4] = {-1., -1., 1., 1.};
+const double xii8[8] = {-1., 1., 1., -1., -1., 1., 1., -1.};
+const double eti8[8] = {-1., -1., 1., 1., -1., -1., 1., 1.};
+const double zti8[8] = {-1., -1., -1., -1., 1., 1., 1., 1.};
+
+struct ShapeOut {
+    int n;
+    long node[64];
+    double fn[64], dx[64], dy[64], dz[64];
+};
+
+// 1D uGIMP factor of one candidate (shared by both dimensions' closed forms)
+inline bool gimp_in_support(double xp, double lp) { return xp < 2. + lp; }
+
+struct Grid {
+    MpmGridDesc d;
+    int dim;
+    long nnx, nny, nnz;     // nodes per axis
+    long nnodes, nelems;
+    std::vector<double> xpts, ypts, zpts;
+    long node_index(long i, long j, long k) const { return (k * nny + j) * nnx + i; }
+    void build(const MpmGridDesc &g) {
+        d = g;
+        dim = g.analysis == MPMGPU_THREED ? 3 : 2;
+        nnx = g.nx + 1; nny = g.ny + 1; nnz = dim == 3 ? g.nz + 1 : 1;
+        nnodes = nnx * nny * nnz;
+        nelems = (long)g.nx * g.ny * (dim == 3 ? g.nz : 1);
+        // node coordinates (Read_MPM/Generators.cpp:1823-1833)
+        xpts.resize(nnx); ypts.resize(nny); zpts.resize(nnz);
+        for (long i = 0; i < nnx; i++) xpts[i] = g.xmin + (double)i * g.dx;
+        for (long i = 0; i < nny; i++) ypts[i] = g.ymin + (double)i * g.dy;
+        for (long i = 0; i < nnz; i++) zpts[i] = dim == 3 ? g.zmin + (double)i * g.dz : 0.;
+    }
+    void elem_ijk(long e, long &i, long &j, long &k) const {
+        i = e % d.nx; long r = e / d.nx; j = r % d.ny; k = dim == 3 ? r / d.ny : 0;
+    }
+};
+
+// ElementBase::PtInElement for rectangles (EightNodeIsoparamBrick.cpp:207-213,
+// FourNodeIsoparam.cpp:413-423); element extents from node coordinates (ElementBase.cpp:62)
+bool pt_in_element(const Grid &g, long e, const double *p) {
+    long i, j, k; g.elem_ijk(e, i, j, k);
+    if (p[0] < g.xpts[i] || p[0] >= g.xpts[i + 1]) return false;
+    if (p[1] < g.ypts[j] || p[1] >= g.ypts[j + 1]) return false;
+    if (g.dim == 3 && (p[2] < g.zpts[k] || p[2] >= g.zpts[k + 1])) return false;
+    return true;
+}
+
+// GetXiPos (EightNodeIsoparamBrick.cpp:276-281, FourNodeIsoparam.cpp:351-356)
+void get_xi(const Grid &g, long e, const double *p, double *xi) {
+    long i, j, k; g.elem_ijk(e, i, j, k);
+    double xmn = g.xpts[i], xmx = g.xpts[i + 1], ymn = g.ypts[j], ymx = g.ypts[j + 1];
+    xi[0] = (2. * p[0] - xmn - xmx) / (xmx - xmn);
+    xi[1] = (2. * p[1] - ymn - ymx) / (ymx - ymn);
+    if (g.dim == 3) {
+        double zmn = g.zpts[k], zmx = g.zpts[k + 1];
+        xi[2] = (2. * p[2] - zmn - zmx) / (zmx - zmn);
+    } else xi[2] = 0.;
+}
+
+// 2D uGIMP (Common/Elements/FourNodeIsoparam.cpp:431-521)
+void gimp2d(const Grid &g, long e, const double *xi, const double *lp, bool getDeriv, ShapeOut &o) {
+    long ei, ej, ek; g.elem_ijk(e, ei, ej, ek);
+    double q1x = 2. - lp[0], q2x = 2. + lp[0];
+    double q1y = 2. - lp[1], q2y = 2. + lp[1];
+    double inv_size_x = 1. / (4. * lp[0]);
+    double inv_size_y = 1. / (4. * lp[1]);
+    double inv_dx = 0, inv_dy = 0;
+    if (getDeriv) {
+        inv_dx = 2.0 / (g.xpts[ei + 1] - g.xpts[ei]);
+        inv_dy = 2.0 / (g.ypts[ej + 1] - g.ypts[ej]);
+    }
+    int i = 0;
+    for (int id = 0; id < 16; id++) {
+        double gx = 2. * xoff2[id] - 1., gy = 2. * yoff2[id] - 1.;
+        double xp = fabs(xi[0] - gx);
+        if (xp >= q2x) continue;
+        double yp = fabs(xi[1] - gy);
+        if (yp >= q2y) continue;
+        double Svpx, Svpy, argx = 0., argy = 0.;
+        if (xp < lp[0]) Svpx = ((4. - lp[0]) * lp[0] - xp * xp) * inv_size_x;
+        else if (xp <= q1x) Svpx = (2. - xp) / 2.;
+        else { argx = (q2x - xp) * inv_size_x; Svpx = 2. * lp[0] * argx * argx; }
+        if (yp < lp[1]) Svpy = ((4. - lp[1]) * lp[1] - yp * yp) * inv_size_y;
+        else if (yp <= q1y) Svpy = (2. - yp) / 2.;
+        else { argy = (q2y - yp) * inv_size_y; Svpy = 2. * lp[1] * argy * argy; }
+        o.fn[i] = Svpx * Svpy;
+        if (getDeriv) {
+            double xsign = xi[0] > gx ? 1. : -1.;
+            double ysign = xi[1] > gy ? 1. : -1.;
+            double dSvpx, dSvpy;
+            if (xp < lp[0]) dSvpx = -xp * inv_size_x * 2.0;
+            else if (xp <= q1x) dSvpx = -0.5;
+            else dSvpx = -argx;
+            if (yp < lp[1]) dSvpy = -yp * inv_size_y * 2.0;
+            else if (yp <= q1y) dSvpy = -0.5;
+            else dSvpy = -argy;
+            o.dx[i] = xsign * dSvpx * Svpy * inv_dx;
+            o.dy[i] = ysign * Svpx * dSvpy * inv_dy;
+            o.dz[i] = 0.;
+        }
+        o.node[i] = g.node_index(ei + xoff2[id], ej + yoff2[id], 0);
+        i++;
+    }
+    o.n = i;
+}
+
+// 3D uGIMP (NairnMPM/src/Elements/EightNodeIsoparamBrick.cpp:289-400), including the
+// reference's inv_size_z = 1/(4*lp.y) (line 303)
+void gimp3d(const Grid &g, long e, const double *xi, const double *lp, bool getDeriv, ShapeOut &o) {
+    long ei, ej, ek; g.elem_ijk(e, ei, ej, ek);
+    double q1x = 2. - lp[0], q2x = 2. + lp[0];
+    double q1y = 2. - lp[1], q2y = 2. + lp[1];
+    double q1z = 2. - lp[2], q2z = 2. + lp[2];
+    double inv_size_x = 1. / (4. * lp[0]);
+    double inv_size_y = 1. / (4. * lp[1]);
+    double inv_size_z = 1. / (4. * lp[1]);
+    double inv_dx = 0, inv_dy = 0, inv_dz = 0;
+    if (getDeriv) {
+        inv_dx = 2.0 / (g.xpts[ei + 1] - g.xpts[ei]);
+        inv_dy = 2.0 / (g.ypts[ej + 1] - g.ypts[ej]);
+        inv_dz = 2.0 / (g.zpts[ek + 1] - g.zpts[ek]);
+    }
+    int i = 0;
+    for (int id = 0; id < 64; id++) {
+        double gx = 2. * xoff3[id] - 1., gy = 2. * yoff3[id] - 1., gz = 2. * zoff3[id] - 1.;
+        double xp = fabs(xi[0] - gx);
+        if (xp >= q2x) continue;
+        double yp = fabs(xi[1] - gy);
+        if (yp >= q2y) continue;
+        double zp = fabs(xi[2] - gz);
+        if (zp >= q2z) continue;
+        double Svpx, Svpy, Svpz, argx = 0., argy = 0., argz = 0.;
+        if (xp < lp[0]) Svpx = ((4. - lp[0]) * lp[0] - xp * xp) * inv_size_x;
+        else if (xp <= q1x) Svpx = 0.5 * (2. - xp);
+        else { argx = (q2x - xp) * inv_size_x; Svpx = 2. * lp[0] * argx * argx; }
+        if (yp < lp[1]) Svpy = ((4. - lp[1]) * lp[1] - yp * yp) * inv_size_y;
+        else if (yp <= q1y) Svpy = 0.5 * (2. - yp);
+        else { argy = (q2y - yp) * inv_size_y; Svpy = 2. * lp[1] * argy * argy; }
+        if (zp < lp[2]) Svpz = ((4. - lp[2]) * lp[2] - zp * zp) * inv_size_z;
+        else if (zp <= q1z) Svpz = 0.5 * (2. - zp);
+        else { argz = (q2z - zp) * inv_size_z; Svpz = 2. * lp[2] * argz * argz; }
+        o.fn[i] = Svpx * Svpy * Svpz;
+        if (getDeriv) {
+            double xsign = xi[0] > gx ? 1. : -1.;
+            double ysign = xi[1] > gy ? 1. : -1.;
+            double zsign = xi[2] > gz ? 1. : -1.;
+            double dSvpx, dSvpy, dSvpz;
+            if (xp < lp[0]) dSvpx = -xp / (2. * lp[0]);
+            else if (xp <= q1x) dSvpx = -0.5;
+            else dSvpx = -argx;
+            if (yp < lp[1]) dSvpy = -yp / (2. * lp[1]);
+            else if (yp <= q1y) dSvpy = -0.5;
+            else dSvpy = -argy;
+            if (zp < lp[2]) dSvpz = -zp / (2. * lp[2]);
+            else if (zp <= q1z) dSvpz = -0.5;
+            else dSvpz = -argz;
+            o.dx[i] = xsign * dSvpx * Svpy * Svpz * inv_dx;
+            o.dy[i] = ysign * Svpx * dSvpy * Svpz * inv_dy;
+            o.dz[i] = zsign * Svpx * Svpy * dSvpz * inv_dz;
+        }
+        o.node[i] = g.node_index(ei + xoff3[id], ej + yoff3[id], ek + zoff3[id]);
+        i++;
+    }
+    o.n = i;
+}
+
+// classic linear shape functions on the particle's element
+// (FourNodeIsoparam.cpp:189-214, EightNodeIsoparamBrick.cpp:87-108)
+void linear_shape(const Grid &g, long e, const double *xi, bool getDeriv, ShapeOut &o) {
+    long ei, ej, ek; g.elem_ijk(e, ei, ej, ek);
+    if (g.dim == 2) {
+        double dx = g.xpts[ei + 1] - g.xpts[ei], dy = g.ypts[ej + 1] - g.ypts[ej];
+        const int ox[4] = {0, 1, 1, 0}, oy[4] = {0, 0, 1, 1};
+        for (int i = 0; i < 4; i++) {
+            double temp1 = (1. + xii4[i] * xi[0]);
+            double temp2 = (1. + eti4[i] * xi[1]);
+            o.fn[i] = 0.25 * temp1 * temp2;
+            if (getDeriv) {
+                o.dx[i] = 0.5 * xii4[i] * temp2 / dx;
+                o.dy[i] = 0.5 * eti4[i] * temp1 / dy;
+                o.dz[i] = 0.;
+            }
+            o.node[i] = g.node_index(ei + ox[i], ej + oy[i], 0);
+        }
+        o.n = 4;
+    } else {
+        double dx = g.xpts[ei + 1] - g.xpts[ei], dy = g.ypts[ej + 1] - g.ypts[ej];
+        double dz = g.zpts[ek + 1] - g.zpts[ek];
+        const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0}, oy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+        const int oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+        for (int i = 0; i < 8; i++) {
+            double temp1 = (1. + xii8[i] * xi[0]);
+            double temp2 = (1. + eti8[i] * xi[1]);
+            double temp3 = (1. + zti8[i] * xi[2]);
+            o.fn[i] = 0.125 * temp1 * temp2 * temp3;
+            if (getDeriv) {
+                o.dx[i] = 0.25 * xii8[i] * temp2 * temp3 / dx;
+                o.dy[i] = 0.25 * eti8[i] * temp1 * temp3 / dy;
+                o.dz[i] = 0.25 * zti8[i] * temp1 * temp2 / dz;
+            }
+            o.node[i] = g.node_index(ei + ox[i], ej + oy[i], ek + oz[i]);
+        }
+        o.n = 8;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Particle state accessor over the SoA buffers
+struct PState {
+    double *f[MPF_NUM_FIELDS];
+    int32_t *iv[MPI_NUM_FIELDS];
+};
+
+struct Oracle {
+    Grid g;
+    std::vector<MpmMaterial> mats;
+    MpmRunParams run;
+    long np = 0;
+    std::vector<double> pd;     // MPF_NUM_FIELDS * np
+    std::vector<int32_t> pi;    // MPI_NUM_FIELDS * np
+    std::vector<double> xi;     // 3*np: ncpos of the step
+    // node fields (single velocity field, MatVelocityField.hpp:40-124)
+    std::vector<double> mass, pk, ftot, vk, pkc;
+    std::vector<int> npts;
+    int64_t steps = 0, leftGrid = 0, nanCount = 0;
+    double mtime = 0.;
+    std::string err;
+
+    double &F(int f, long p) { return pd[(size_t)f * np + p]; }
+    int32_t &I(int f, long p) { return pi[(size_t)f * np + p]; }
+
+    void shape(long p, bool deriv, ShapeOut &o) {
+        long e = I(MPI_ELEM, p);
+        const double *x = &xi[3 * p];
+        double lp[3] = {F(MPF_LP_X, p), F(MPF_LP_Y, p), F(MPF_LP_Z, p)};
+        if (run.shape == MPMGPU_SHAPE_UGIMP) {
+            if (g.dim == 3) gimp3d(g, e, x, lp, deriv, o); else gimp2d(g, e, x, lp, deriv, o);
+        } else linear_shape(g, e, x, deriv, o);
+    }
+
+    // InitializationTask::Execute (InitializationTask.cpp:43-85): zero the grid, find xi
+    void task_initialize() {
+        std::fill(mass.begin(), mass.end(), 0.);
+        std::fill(pk.begin(), pk.end(), 0.);
+        std::fill(ftot.begin(), ftot.end(), 0.);
+        std::fill(vk.begin(), vk.end(), 0.);
+        std::fill(pkc.begin(), pkc.end(), 0.);
+        std::fill(npts.begin(), npts.end(), 0);
+        for (long p = 0; p < np; p++) {
+            double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
+            get_xi(g, I(MPI_ELEM, p), pos, &xi[3 * p]);
+        }
+    }
+
+    // MassAndMomentumTask::Execute (MassAndMomentumTask.cpp:48-127) ->
+    // NodalPoint::AddMassMomentum (NodalPointMPM.cpp:419-453)
+    void task_mass_momentum() {
+        ShapeOut o;
+        for (long p = 0; p < np; p++) {
+            shape(p, false, o);
+            double mp = F(MPF_MASS, p);
+            double v[3] = {F(MPF_VEL_X, p), F(MPF_VEL_Y, p), F(MPF_VEL_Z, p)};
+            for (int i = 0; i < o.n; i++) {
+                long n = o.node[i];
+                double fnmp = o.fn[i] * mp;
+                for (int c = 0; c < 3; c++) pk[3 * n + c] += fnmp * v[c];   // AddMomentumTask1
+                npts[n] += 1;
+                mass[n] += fnmp;                                             // AddMass
+            }
+        }
+    }
+
+    // PostExtrapolationTask::Execute (PostExtrapolationTask.cpp:46-165):
+    // CalcTotalMassAndCount copies pk (MatVelocityField.cpp:158-164)
+    void task_post_extrapolation() {
+        for (long n = 0; n < g.nnodes; n++)
+            if (npts[n] >= 1) for (int c = 0; c < 3; c++) pkc[3 * n + c] = pk[3 * n + c];
+    }
+
+    // MatVelocityField::GridValueCalculation (MatVelocityField.cpp:239-251)
+    void grid_velocity() {
+        for (long n = 0; n < g.nnodes; n++) {
+            if (npts[n] == 0 || mass[n] == 0.) continue;
+            double s = 1. / mass[n];
+            for (int c = 0; c < 3; c++) vk[3 * n + c] = s * pk[3 * n + c];
+        }
+    }
+
+    // UpdateStrainsFirstTask::FullStrainUpdate (UpdateStrainsFirstTask.cpp:101-168)
+    void full_strain_update(double strainTime) {
+        grid_velocity();
+        ShapeOut o;
+        for (long p = 0; p < np; p++) {
+            shape(p, true, o);
+            Mat3 dv;
+            if (g.dim == 3) {       // MatPoint3D::ExtrapVelocityGradient (MatPoint3D.cpp:62-94)
+                for (int i = 0; i < o.n; i++) {
+                    const double *v = &vk[3 * o.node[i]];
+                    Mat3 t = Mat3::full(v[0] * o.dx[i], v[0] * o.dy[i], v[0] * o.dz[i],
+                                        v[1] * o.dx[i], v[1] * o.dy[i], v[1] * o.dz[i],
+                                        v[2] * o.dx[i], v[2] * o.dy[i], v[2] * o.dz[i]);
+                    dv.add(t);
+                }
+            } else {                // MatPoint2D::ExtrapVelocityGradient (MatPoint2D.cpp:57-89)
+                for (int i = 0; i < o.n; i++) {
+                    const double *v = &vk[3 * o.node[i]];
+                    dv.add(Mat3::two(v[0] * o.dx[i], v[0] * o.dy[i], v[1] * o.dx[i], v[1] * o.dy[i], 0.));
+                }
+            }
+            dv.scale(strainTime);
+            constitutive(p, dv);
+        }
+    }
+
+    // ---- constitutive laws ------------------------------------------------------------
+    Mat3 get_F(long p) {
+        // MatPoint3D::GetDeformationGradient (MatPoint3D.cpp:364-377), MatPoint2D.cpp:369-377
+        double exx = F(MPF_EP_XX, p), eyy = F(MPF_EP_YY, p), ezz = F(MPF_EP_ZZ, p);
+        double exy = F(MPF_EP_XY, p), wxy = F(MPF_WR_XY, p);
+        if (g.dim == 2)
+            return Mat3::two(1. + exx, 0.5 * (exy - wxy), 0.5 * (exy + wxy), 1. + eyy, 1. + ezz);
+        double exz = F(MPF_EP_XZ, p), eyz = F(MPF_EP_YZ, p), wxz = F(MPF_WR_XZ, p), wyz = F(MPF_WR_YZ, p);
+        return Mat3::full(1. + exx, 0.5 * (exy - wxy), 0.5 * (exz - wxz),
+                          0.5 * (exy + wxy), 1. + eyy, 0.5 * (eyz - wyz),
+                          0.5 * (exz + wxz), 0.5 * (eyz + wyz), 1. + ezz);
+    }
+    void set_F(long p, const Mat3 &Fm) {
+        // SetDeformationGradientMatrix (MatPoint3D.cpp:320-336, MatPoint2D.cpp:310-322)
+        F(MPF_EP_XX, p) = Fm.m[0][0] - 1.;
+        F(MPF_EP_YY, p) = Fm.m[1][1] - 1.;
+        F(MPF_EP_ZZ, p) = Fm.m[2][2] - 1.;
+        F(MPF_EP_XY, p) = Fm.m[1][0] + Fm.m[0][1];
+        F(MPF_WR_XY, p) = Fm.m[1][0] - Fm.m[0][1];
+        if (g.dim == 3) {
+            F(MPF_EP_XZ, p) = Fm.m[2][0] + Fm.m[0][2];
+            F(MPF_EP_YZ, p) = Fm.m[2][1] + Fm.m[1][2];
+            F(MPF_WR_XZ, p) = Fm.m[2][0] - Fm.m[0][2];
+            F(MPF_WR_YZ, p) = Fm.m[2][1] - Fm.m[1][2];
+        }
+    }
+    // Elastic::HypoIncrementDeformation (ElasticMPM.cpp:392-403)
+    void hypo_increment(long p, const Mat3 &du) {
+        Mat3 dF = du.exponential(1);
+        Mat3 pF = get_F(p);
+        set_F(p, dF.mul(pF));
+    }
+
+    void constitutive(long p, const Mat3 &du) {
+        const MpmMaterial &m = mats[I(MPI_MAT, p)];
+        switch (m.type) {
+            case MPMGPU_MAT_ISOTROPIC: isotropic(p, du, m); break;
+            case MPMGPU_MAT_ISOPLASTICITY: isoplastic(p, du, m); break;
+            case MPMGPU_MAT_NEOHOOKEAN: neohookean(p, du, m); break;
+            default: break;
+        }
+    }
+
+    // IsotropicMat::MPMConstitutiveLaw (MoreIsotropicMat.cpp:33-47) small-rotation branch
+    void isotropic(long p, const Mat3 &du, const MpmMaterial &m) {
+        hypo_increment(p, du);
+        double *sxx = &F(MPF_SP_XX, p), *syy = &F(MPF_SP_YY, p), *szz = &F(MPF_SP_ZZ, p);
+        double *syz = &F(MPF_SP_YZ, p), *sxz = &F(MPF_SP_XZ, p), *sxy = &F(MPF_SP_XY, p);
+        if (g.dim == 3) {
+            // SRConstitutiveLaw3D (MoreIsotropicMat.cpp:286-351) with no residual strains
+            double dvxx = du.m[0][0], dvyy = du.m[1][1], dvzz = du.m[2][2];
+            double dgamxy = du.m[0][1] + du.m[1][0];
+            double dgamxz = du.m[0][2] + du.m[2][0];
+            double dgamyz = du.m[1][2] + du.m[2][1];
+            double dwrotxy = du.m[1][0] - du.m[0][1];
+            double dwrotxz = du.m[2][0] - du.m[0][2];
+            double dwrotyz = du.m[2][1] - du.m[1][2];
+            double d[6];
+            d[0] = m.C[0][0] * dvxx + m.C[0][1] * dvyy + m.C[0][2] * dvzz;
+            d[1] = m.C[1][0] * dvxx + m.C[1][1] * dvyy + m.C[1][2] * dvzz;
+            d[2] = m.C[2][0] * dvxx + m.C[2][1] * dvyy + m.C[2][2] * dvzz;
+            d[3] = m.C[3][3] * dgamyz;
+            d[4] = m.C[4][4] * dgamxz;
+            d[5] = m.C[5][5] * dgamxy;
+            // MaterialBase::Hypo3DCalculations (MaterialBaseMPM.cpp:1026-1047)
+            double sp[6] = {*sxx, *syy, *szz, *syz, *sxz, *sxy};
+            double st[6];
+            st[0] = -dwrotxy * sp[5] - dwrotxz * sp[4];
+            st[1] = dwrotxy * sp[5] - dwrotyz * sp[3];
+            st[2] = dwrotxz * sp[4] + dwrotyz * sp[3];
+            st[3] = 0.5 * (dwrotxy * sp[4] + dwrotxz * sp[5] + dwrotyz * (sp[1] - sp[2]));
+            st[4] = 0.5 * (-dwrotxy * sp[3] + dwrotxz * (sp[0] - sp[2]) + dwrotyz * sp[5]);
+            st[5] = 0.5 * (dwrotxy * (sp[0] - sp[1]) - dwrotxz * sp[3] - dwrotyz * sp[4]);
+            *sxx += d[0] + st[0]; *syy += d[1] + st[1]; *szz += d[2] + st[2];
+            *syz += d[3] + st[3]; *sxz += d[4] + st[4]; *sxy += d[5] + st[5];
+        } else {
+            // SRConstitutiveLaw2D plane strain (MoreIsotropicMat.cpp:185-283)
+            double dvxx = du.m[0][0], dvyy = du.m[1][1];
+            double dgam = du.m[0][1] + du.m[1][0];
+            double dwrotxy = du.m[1][0] - du.m[0][1];
+            double c1 = m.C[1][1] * dvxx + m.C[1][2] * dvyy;
+            double c2 = m.C[1][2] * dvxx + m.C[2][2] * dvyy;
+            double c3 = m.C[3][3] * dgam;
+            // Hypo2DCalculations (MaterialBaseMPM.cpp:1012-1020)
+            double dnorm = dwrotxy * (*sxy);
+            double dshear = 0.5 * dwrotxy * (*sxx - *syy);
+            *sxx += c1 - dnorm;
+            *syy += c2 + dnorm;
+            *sxy += c3 + dshear;
+            *szz += m.C[4][1] * dvxx + m.C[4][2] * dvyy + m.C[4][4] * 0.;
+        }
+    }
+
+    // IsoPlasticity::MPMConstitutiveLaw + PlasticityConstLaw (IsoPlasticity.cpp:128-451) with
+    // LinearHardening (LinearHardening.cpp:93-147), small rotation, no residual strains,
+    // plane strain or 3D
+    void isoplastic(long p, const Mat3 &de, const MpmMaterial &m) {
+        hypo_increment(p, de);
+        bool is2D = g.dim == 2;
+        double delV = de.trace() - 3. * 0.;
+        double dgxy = de.m[0][1] + de.m[1][0], dgxz = 0., dgyz = 0.;
+        if (!is2D) { dgxz = de.m[0][2] + de.m[2][0]; dgyz = de.m[1][2] + de.m[2][1]; }
+        double dexxr = de.m[0][0], deyyr = de.m[1][1], dezzr = de.m[2][2];
+        // UpdatePressure (IsoPlasticity.cpp:462-492)
+        double dP = -m.Kred * delV;
+        F(MPF_PRESSURE, p) += dP;
+        // rotate plastic strain and stress (IsoPlasticity.cpp:224-277)
+        double exx = F(MPF_ALT_XX, p), eyy = F(MPF_ALT_YY, p), ezz = F(MPF_ALT_ZZ, p);
+        double eyz = F(MPF_ALT_YZ, p), exz = F(MPF_ALT_XZ, p), exy = F(MPF_ALT_XY, p);
+        double sxx = F(MPF_SP_XX, p), syy = F(MPF_SP_YY, p), szz = F(MPF_SP_ZZ, p);
+        double syz = F(MPF_SP_YZ, p), sxz = F(MPF_SP_XZ, p), sxy = F(MPF_SP_XY, p);
+        double dwrotxy = de.m[1][0] - de.m[0][1];
+        double st0[6] = {sxx, syy, szz, syz, sxz, sxy};
+        if (is2D) {
+            double dnorm = 0.5 * dwrotxy * exy;
+            F(MPF_ALT_XX, p) -= dnorm;
+            F(MPF_ALT_YY, p) += dnorm;
+            F(MPF_ALT_XY, p) += dwrotxy * (exx - eyy);
+            double dn = dwrotxy * sxy;
+            st0[0] -= dn;
+            st0[1] += dn;
+            st0[5] += 0.5 * dwrotxy * (exx - eyy);     // reference quirk: ep, not sp (:252)
+        } else {
+            double dwrotxz = de.m[2][0] - de.m[0][2];
+            double dwrotyz = de.m[2][1] - de.m[1][2];
+            double dxy = 0.5 * dwrotxy * exy;
+            double dxz = 0.5 * dwrotxz * exz;
+            double dyz = 0.5 * dwrotyz * eyz;
+            F(MPF_ALT_XX, p) += -dxy - dxz;
+            F(MPF_ALT_YY, p) += dxy - dyz;
+            F(MPF_ALT_ZZ, p) += dxz + dyz;
+            F(MPF_ALT_YZ, p) += dwrotyz * (eyy - ezz) + 0.5 * (dwrotxz * exy + dwrotxy * exz);
+            F(MPF_ALT_XZ, p) += dwrotxz * (exx - ezz) + 0.5 * (dwrotyz * exy - dwrotxy * eyz);
+            F(MPF_ALT_XY, p) += dwrotxy * (exx - eyy) - 0.5 * (dwrotyz * exz + dwrotxz * eyz);
+            double sdxy = dwrotxy * sxy;
+            double sdxz = dwrotxz * sxz;
+            double sdyz = dwrotyz * syz;
+            st0[0] += -sdxy - sdxz;
+            st0[1] += sdxy - sdyz;
+            st0[2] += sdxz + sdyz;
+            st0[3] += 0.5 * (dwrotyz * (syy - szz) + dwrotxz * sxy + dwrotxy * sxz);
+            st0[4] += 0.5 * (dwrotxz * (sxx - szz) + dwrotyz * sxy - dwrotxy * syz);
+            st0[5] += 0.5 * (dwrotxy * (sxx - syy) - dwrotyz * sxz - dwrotxz * syz);
+        }
+        // deviatoric trial stress (IsoPlasticity.cpp:279-301)
+        double thirdDelV = delV / 3.;
+        double dels[6] = {0., 0., 0., 0., 0., 0.};
+        dels[0] = 2. * m.Gred * (dexxr - thirdDelV);
+        dels[1] = 2. * m.Gred * (deyyr - thirdDelV);
+        dels[5] = m.Gred * dgxy;
+        dels[2] = 2. * m.Gred * (dezzr - thirdDelV);
+        if (!is2D) { dels[3] = m.Gred * dgyz; dels[4] = m.Gred * dgxz; }
+        double str[6];
+        for (int c = 0; c < 6; c++) str[c] = st0[c] + dels[c];
+        if (is2D) { str[3] = 0.; str[4] = 0.; }
+        // GetPlasticPotential: J2 (IsoPlasticity.cpp:513-517, MoreIsotropicMat.cpp:356-373)
+        double alpint = F(MPF_HIST0, p);
+        double s = str[0] * str[0] + str[1] * str[1] + str[2] * str[2];
+        double t = is2D ? str[5] * str[5] : str[5] * str[5] + str[4] * str[4] + str[3] * str[3];
+        double strialmag = sqrt(s + t + t);
+        double yld = alpint < m.alphaMax ? m.yldred + m.Epred * alpint : m.yldredMin;
+        double ftrial = strialmag - SQRT_TWOTHIRDS * yld;
+        if (ftrial < 0.) {
+            F(MPF_SP_XX, p) = str[0]; F(MPF_SP_YY, p) = str[1]; F(MPF_SP_ZZ, p) = str[2];
+            F(MPF_SP_YZ, p) = str[3]; F(MPF_SP_XZ, p) = str[4]; F(MPF_SP_XY, p) = str[5];
+            return;
+        }
+        // LinearHardening::SolveForLambdaBracketed closed form (LinearHardening.cpp:124-147)
+        double lambdak = (strialmag - SQRT_TWOTHIRDS * (m.yldred + m.Epred * alpint)) / (2. * (m.Gred + m.Epred / 3.));
+        if (alpint + SQRT_TWOTHIRDS * lambdak > m.alphaMax)
+            lambdak = (strialmag - SQRT_TWOTHIRDS * m.yldredMin) / (2. * m.Gred);
+        double dalpha = SQRT_TWOTHIRDS * lambdak;           // HardeningLawBase.cpp:131-134
+        alpint = F(MPF_HIST0, p) + dalpha;
+        // GetDfDsigma (IsoPlasticity.cpp:497-508) and plastic increment (:385-420)
+        double dfds[6];
+        for (int c = 0; c < 6; c++) dfds[c] = str[c] / strialmag;
+        double dep[6];
+        dep[0] = lambdak * dfds[0];
+        dep[1] = lambdak * dfds[1];
+        dep[2] = lambdak * dfds[2];
+        dep[5] = 2. * lambdak * dfds[5];
+        F(MPF_ALT_XX, p) += dep[0]; F(MPF_ALT_YY, p) += dep[1]; F(MPF_ALT_ZZ, p) += dep[2];
+        F(MPF_ALT_XY, p) += dep[5];
+        if (!is2D) {
+            dep[4] = 2. * lambdak * dfds[4];
+            dep[3] = 2. * lambdak * dfds[3];
+            F(MPF_ALT_XZ, p) += dep[4]; F(MPF_ALT_YZ, p) += dep[3];
+        }
+        F(MPF_SP_XX, p) = str[0] - 2. * m.Gred * dep[0];
+        F(MPF_SP_YY, p) = str[1] - 2. * m.Gred * dep[1];
+        F(MPF_SP_ZZ, p) = str[2] - 2. * m.Gred * dep[2];
+        F(MPF_SP_XY, p) = str[5] - m.Gred * dep[5];
+        if (!is2D) {
+            F(MPF_SP_YZ, p) = str[3] - m.Gred * dep[3];
+            F(MPF_SP_XZ, p) = str[4] - m.Gred * dep[4];
+        }
+        F(MPF_HIST0, p) = alpint;                            // UpdatePlasticInternal (:136-139)
+    }
+
+    // Neohookean::MPMConstitutiveLaw (Neohookean.cpp:177-335) with
+    // HyperElastic::IncrementDeformation (HyperElastic.cpp:104-141), default U(J),
+    // no residual strains, no artificial viscosity, plane strain or 3D
+    void neohookean(long p, const Mat3 &du, const MpmMaterial &m) {
+        bool is3 = g.dim == 3;
+        Mat3 dF = du.exponential(is3 ? 1 : 2);     // incrementalDefGradTerms (StartOutput.cpp:116-119)
+        Mat3 pF = get_F(p);
+        set_F(p, dF.mul(pF));
+        double bxx = F(MPF_ALT_XX, p), byy = F(MPF_ALT_YY, p), bzz = F(MPF_ALT_ZZ, p);
+        double bxy = F(MPF_ALT_XY, p), bxz = F(MPF_ALT_XZ, p), byz = F(MPF_ALT_YZ, p);
+        Mat3 pBold = is3 ? Mat3::full(bxx, bxy, bxz, bxy, byy, byz, bxz, byz, bzz)
+                         : Mat3::two(bxx, bxy, bxy, byy, bzz);
+        Mat3 dFoldB = dF.mul(pBold);
+        const double (*a)[3] = dFoldB.m;
+        const double (*f)[3] = dF.m;
+        double Bxx = a[0][0] * f[0][0] + a[0][1] * f[0][1] + a[0][2] * f[0][2];
+        double Bxy = a[0][0] * f[1][0] + a[0][1] * f[1][1] + a[0][2] * f[1][2];
+        double Byy = a[1][0] * f[1][0] + a[1][1] * f[1][1] + a[1][2] * f[1][2];
+        double Bzz = a[2][0] * f[2][0] + a[2][1] * f[2][1] + a[2][2] * f[2][2];
+        F(MPF_ALT_XX, p) = Bxx; F(MPF_ALT_XY, p) = Bxy; F(MPF_ALT_YY, p) = Byy; F(MPF_ALT_ZZ, p) = Bzz;
+        double Bxz = 0., Byz = 0.;
+        if (is3) {
+            Bxz = a[0][0] * f[2][0] + a[0][1] * f[2][1] + a[0][2] * f[2][2];
+            Byz = a[1][0] * f[2][0] + a[1][1] * f[2][1] + a[1][2] * f[2][2];
+            F(MPF_ALT_XZ, p) = Bxz; F(MPF_ALT_YZ, p) = Byz;
+        }
+        double detDf = dF.determinant();
+        double Jres = F(MPF_HIST1, p);
+        double dJres = exp(0.);                        // GetIncrementalResJ (HyperElastic.cpp:89-95)
+        Jres *= dJres;
+        F(MPF_HIST1, p) = Jres;
+        double resStretch = pow(Jres, 1. / 3.);
+        double Jres23 = resStretch * resStretch;
+        double J = detDf * F(MPF_HIST0, p);
+        F(MPF_HIST0, p) = J;
+        double Jeff = J / Jres;
+        // GetVolumetricTerms (HyperElastic.cpp:171-207); option numbers from HyperElastic.hpp:21
+        double Kterm;
+        if (m.ujOption == 1) Kterm = m.Lamesp * (Jeff - 1.);                        // J_MINUS_1_SQUARED
+        else if (m.ujOption == 2) { double lj = log(Jeff); Kterm = m.Lamesp * lj; Kterm /= Jeff; }
+        else Kterm = 0.5 * m.Lamesp * (Jeff - 1. / Jeff);                           // default (0)
+        double Pterm = J * Kterm + Jres * m.Gsp * ((Bxx + Byy + Bzz) / (3. * Jres23) - 1.);
+        F(MPF_PRESSURE, p) = -Pterm + 0.;
+        double GJeff = resStretch * m.Gsp;
+        double I1third = (Bxx + Byy + Bzz) / 3.;
+        F(MPF_SP_XX, p) = GJeff * (Bxx - I1third);
+        F(MPF_SP_YY, p) = GJeff * (Byy - I1third);
+        F(MPF_SP_ZZ, p) = GJeff * (Bzz - I1third);
+        F(MPF_SP_XY, p) = GJeff * Bxy;
+        if (is3) { F(MPF_SP_XZ, p) = GJeff * Bxz; F(MPF_SP_YZ, p) = GJeff * Byz; }
+    }
+
+    // GridForcesTask::Execute (GridForcesTask.cpp:40-153) -> GetFintPlusFext
+    // (MatPoint3D.cpp:248-270, MatPoint2D.cpp:242-263); pFext is zero (no particle loads)
+    void task_grid_forces() {
+        ShapeOut o;
+        for (long p = 0; p < np; p++) {
+            shape(p, true, o);
+            double mp = F(MPF_MASS, p), P = F(MPF_PRESSURE, p);
+            double sxx = F(MPF_SP_XX, p), syy = F(MPF_SP_YY, p), szz = F(MPF_SP_ZZ, p);
+            double syz = F(MPF_SP_YZ, p), sxz = F(MPF_SP_XZ, p), sxy = F(MPF_SP_XY, p);
+            for (int i = 0; i < o.n; i++) {
+                long n = o.node[i];
+                double fx, fy, fz;
+                if (g.dim == 3) {
+                    fx = -mp * ((sxx - P) * o.dx[i] + sxy * o.dy[i] + sxz * o.dz[i]);
+                    fy = -mp * (sxy * o.dx[i] + (syy - P) * o.dy[i] + syz * o.dz[i]);
+                    fz = -mp * (sxz * o.dx[i] + syz * o.dy[i] + (szz - P) * o.dz[i]);
+                } else {
+                    fx = -mp * ((sxx - P) * o.dx[i] + sxy * o.dy[i]);
+                    fy = -mp * (sxy * o.dx[i] + (syy - P) * o.dy[i]);
+                    fz = 0.0;
+                }
+                ftot[3 * n] += fx; ftot[3 * n + 1] += fy; ftot[3 * n + 2] += fz;   // AddFtotTask3
+            }
+        }
+    }
+
+    // PostForcesTask::Execute (PostForcesTask.cpp:43-111) -> RestoreMomenta (MatVelocityField.cpp:177)
+    void task_post_forces() {
+        for (long n = 0; n < g.nnodes; n++)
+            if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = pkc[3 * n + c];
+    }
+
+    // UpdateMomentaTask::Execute (UpdateMomentaTask.cpp:44-64) -> UpdateMomentum (MatVelocityField.cpp:301-314)
+    void task_update_momenta() {
+        double dt = run.timestep;
+        for (long n = 0; n < g.nnodes; n++)
+            if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] += ftot[3 * n + c] * dt;
+    }
+
+    // UpdateParticlesTask::Execute (UpdateParticlesTask.cpp:44-298), FLIP, no damping ->
+    // IncrementNodalVelAcc (MatVelocityField.cpp:410-426) -> MoveParticle (MatPoint3D.cpp:104-194)
+    void task_update_particles() {
+        grid_velocity();
+        double dt = run.timestep;
+        ShapeOut o;
+        for (long p = 0; p < np; p++) {
+            shape(p, false, o);
+            double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
+            for (int i = 0; i < o.n; i++) {
+                long n = o.node[i];
+                for (int c = 0; c < 3; c++) Svk[c] += vk[3 * n + c] * o.fn[i];
+                double mnode = o.fn[i] / mass[n];
+                for (int c = 0; c < 3; c++) Sacc[c] += ftot[3 * n + c] * mnode;
+            }
+            int nc = g.dim == 3 ? 3 : 2;
+            for (int c = 0; c < nc; c++) {
+                double vm = Svk[c] - Sacc[c] * dt;
+                double delV = Sacc[c] * dt;
+                F(MPF_VEL_X + c, p) += delV;
+                double delXRate = vm + 0.5 * delV;
+                F(MPF_POS_X + c, p) += delXRate * dt;
+            }
+        }
+    }
+
+    // UpdateStrainsLastContactTask::Execute (UpdateStrainsLastContactTask.cpp:60-152):
+    // RezeroNodeTask6 + AddMassMomentumLast (NodalPointMPM.cpp:470-486) with start-of-step xi
+    void task_update_strains_last() {
+        for (long n = 0; n < g.nnodes; n++)
+            if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = 0.;
+        ShapeOut o;
+        for (long p = 0; p < np; p++) {
+            shape(p, false, o);
+            double mp = F(MPF_MASS, p);
+            double v[3] = {F(MPF_VEL_X, p), F(MPF_VEL_Y, p), F(MPF_VEL_Z, p)};
+            for (int i = 0; i < o.n; i++) {
+                long n = o.node[i];
+                double fnmp = o.fn[i] * mp;
+                for (int c = 0; c < 3; c++) pk[3 * n + c] += v[c] * fnmp;   // AddMomentumTask6
+            }
+        }
+        full_strain_update(run.strainLast);
+    }
+
+    // ResetElementsTask::Execute / ResetElement / ReturnToElement (ResetElementsTask.cpp:44-265),
+    // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640), EdgeElement2D/3D (MeshInfo.cpp:171-199)
+    void task_reset_elements() {
+        const MpmGridDesc &d = g.d;
+        long horiz = d.nx, vert = d.ny, depth = g.dim == 3 ? d.nz : 1;
+        long totalElems = horiz * vert * depth;
+        for (long p = 0; p < np; p++) {
+            double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
+            if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) { nanCount++; continue; }
+            long e = I(MPI_ELEM, p);
+            if (pt_in_element(g, e, pos)) continue;
+            bool left = false;
+            long col = (long)(int)((pos[0] - d.xmin) / d.dx);
+            if (col < 0 || col >= horiz) {
+                if (pos[0] == d.xmin + horiz * d.dx) col = horiz - 1; else left = true;
+            }
+            long row = (long)(int)((pos[1] - d.ymin) / d.dy);
+            if (!left && (row < 0 || row >= vert)) {
+                if (pos[1] == d.ymin + vert * d.dy) row = vert - 1; else left = true;
+            }
+            long zrow = 0;
+            if (!left && g.dim == 3) {
+                zrow = (long)(int)((pos[2] - d.zmin) / d.dz);
+                if (zrow < 0 || zrow >= depth) {
+                    if (pos[2] == d.zmin + depth * d.dz) zrow = depth - 1; else left = true;
+                }
+            }
+            long num = 0;
+            if (!left) {
+                num = (g.dim == 3 ? horiz * (zrow * vert + row) + col : row * horiz + col) + 1;
+                if (g.dim == 3) {
+                    if (num <= horiz * vert || num > totalElems - horiz * vert) left = true;
+                    else if (num % horiz <= 1) left = true;
+                    else { long xz = num % (horiz * vert); if (xz <= horiz || xz > horiz * (vert - 1)) left = true; }
+                } else {
+                    if (num <= horiz || num > totalElems - horiz) left = true;
+                    else if (num % horiz <= 1) left = true;
+                }
+            }
+            if (!left) { I(MPI_ELEM, p) = (int32_t)(num - 1); continue; }
+            if (!I(MPI_LEFT, p)) { leftGrid++; I(MPI_LEFT, p) = 1; }
+            // ReturnToElement
+            double dt = run.timestep;
+            double outside[3] = {pos[0], pos[1], pos[2]};
+            double inside[3] = {outside[0] - dt * F(MPF_VEL_X, p), outside[1] - dt * F(MPF_VEL_Y, p),
+                                outside[2] - dt * F(MPF_VEL_Z, p)};
+            if (!pt_in_element(g, e, inside)) {
+                long i, j, k; g.elem_ijk(e, i, j, k);
+                inside[0] = (g.xpts[i] + g.xpts[i + 1]) / 2.;
+                inside[1] = (g.ypts[j] + g.ypts[j + 1]) / 2.;
+                inside[2] = g.dim == 3 ? (g.zpts[k] + g.zpts[k + 1]) / 2. : 0.;
+            }
+            for (int pass = 1; pass <= 10; pass++) {
+                double mid[3] = {(outside[0] + inside[0]) / 2., (outside[1] + inside[1]) / 2.,
+                                 (outside[2] + inside[2]) / 2.};
+                if (pt_in_element(g, e, mid)) { inside[0] = mid[0]; inside[1] = mid[1]; inside[2] = mid[2]; }
+                else { outside[0] = mid[0]; outside[1] = mid[1]; outside[2] = mid[2]; }
+            }
+            F(MPF_POS_X, p) = inside[0]; F(MPF_POS_Y, p) = inside[1];
+            if (g.dim == 3) F(MPF_POS_Z, p) = inside[2];
+        }
+    }
+
+    // NairnMPM::MPMStep (NairnMPM.cpp:284-335) with the task list of CreateTasks (:870-1110)
+    void step() {
+        task_initialize();
+        task_mass_momentum();
+        task_post_extrapolation();
+        full_strain_update(run.strainFirst);     // UpdateStrainsFirstTask (UpdateStrainsFirstTask.cpp:52)
+        task_grid_forces();
+        task_post_forces();
+        task_update_momenta();
+        task_update_particles();
+        task_update_strains_last();
+        task_reset_elements();
+        steps++;
+        mtime += run.timestep;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void *oracle_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat,
+                    const MpmRunParams *run, int64_t n, const double *dfields, const int32_t *ifields) {
+    Oracle *o = new Oracle();
+    o->g.build(*grid);
+    o->mats.assign(mats, mats + nmat);
+    o->run = *run;
+    o->np = n;
+    o->pd.assign(dfields, dfields + (size_t)MPF_NUM_FIELDS * n);
+    o->pi.assign(ifields, ifields + (size_t)MPI_NUM_FIELDS * n);
+    o->xi.assign(3 * (size_t)n, 0.);
+    size_t nn = (size_t)o->g.nnodes;
+    o->mass.assign(nn, 0.); o->npts.assign(nn, 0);
+    o->pk.assign(3 * nn, 0.); o->ftot.assign(3 * nn, 0.); o->vk.assign(3 * nn, 0.); o->pkc.assign(3 * nn, 0.);
+    return o;
+}
+
+void oracle_destroy(void *h) { delete (Oracle *)h; }
+
+int oracle_step(void *h, int32_t nsteps) {
+    Oracle *o = (Oracle *)h;
+    for (int s = 0; s < nsteps; s++) o->step();
+    return 0;
+}
+
+/* run one task of the step (same ids as MPMGPU_TASK_*) */
+int oracle_run_task(void *h, int32_t task) {
+    Oracle *o = (Oracle *)h;
+    switch (task) {
+        case MPMGPU_TASK_INITIALIZATION: o->task_initialize(); break;
+        case MPMGPU_TASK_MASS_MOMENTUM: o->task_mass_momentum(); break;
+        case MPMGPU_TASK_POST_EXTRAPOLATION: o->task_post_extrapolation(); break;
+        case MPMGPU_TASK_UPDATE_STRAINS_FIRST: o->full_strain_update(o->run.strainFirst); break;
+        case MPMGPU_TASK_GRID_FORCES: o->task_grid_forces(); break;
+        case MPMGPU_TASK_POST_FORCES: o->task_post_forces(); break;
+        case MPMGPU_TASK_UPDATE_MOMENTA: o->task_update_momenta(); break;
+        case MPMGPU_TASK_UPDATE_PARTICLES: o->task_update_particles(); break;
+        case MPMGPU_TASK_UPDATE_STRAINS_LAST: o->task_update_strains_last(); break;
+        case MPMGPU_TASK_RESET_ELEMENTS:
+            o->task_reset_elements(); o->steps++; o->mtime += o->run.timestep; break;
+        default: return 1;
+    }
+    return 0;
+}
+
+int oracle_get_particles(void *h, double *dfields, int32_t *ifields) {
+    Oracle *o = (Oracle *)h;
+    memcpy(dfields, o->pd.data(), o->pd.size() * sizeof(double));
+    if (ifields) memcpy(ifields, o->pi.data(), o->pi.size() * sizeof(int32_t));
+    return 0;
+}
+
+int64_t oracle_num_nodes(void *h) { return ((Oracle *)h)->g.nnodes; }
+
+int oracle_get_grid(void *h, double *mass, double *pk, double *ftot, int32_t *npts) {
+    Oracle *o = (Oracle *)h;
+    size_t nn = (size_t)o->g.nnodes;
+    if (mass) memcpy(mass, o->mass.data(), nn * sizeof(double));
+    if (pk) memcpy(pk, o->pk.data(), 3 * nn * sizeof(double));
+    if (ftot) memcpy(ftot, o->ftot.data(), 3 * nn * sizeof(double));
+    if (npts) memcpy(npts, o->npts.data(), nn * sizeof(int32_t));
+    return 0;
+}
+
+int oracle_counters(void *h, int64_t *out) {
+    Oracle *o = (Oracle *)h;
+    out[0] = o->steps; out[1] = o->leftGrid; out[2] = o->nanCount;
+    return 0;
+}
+
+/* Shape functions of one particle position in element `elem` (unit tests):
+   returns the node count; nodes/fn/dx/dy/dz get up to 64 entries. */
+int oracle_shape(const MpmGridDesc *grid, int32_t shape, int64_t elem, const double *pos,
+                 const double *lp, int32_t getDeriv, int64_t *nodes, double *fn, double *dx,
+                 double *dy, double *dz) {
+    Grid g; g.build(*grid);
+    double xi[3];
+    get_xi(g, elem, pos, xi);
+    ShapeOut o;
+    if (shape == MPMGPU_SHAPE_UGIMP) {
+        if (g.dim == 3) gimp3d(g, elem, xi, lp, getDeriv != 0, o); else gimp2d(g, elem, xi, lp, getDeriv != 0, o);
+    } else linear_shape(g, elem, xi, getDeriv != 0, o);
+    for (int i = 0; i < o.n; i++) {
+        nodes[i] = o.node[i]; fn[i] = o.fn[i];
+        if (getDeriv) { dx[i] = o.dx[i]; dy[i] = o.dy[i]; dz[i] = o.dz[i]; }
+    }
+    return o.n;
+}
+
+/* One constitutive update of a single particle (unit tests): du is the 3x3 displacement
+   gradient increment (row-major), state is MPF_NUM_FIELDS doubles updated in place. */
+int oracle_constitutive(int32_t analysis, const MpmMaterial *mat, const double *du, double *state) {
+    Oracle o;
+    MpmGridDesc gd; memset(&gd, 0, sizeof(gd));
+    gd.analysis = analysis; gd.nx = gd.ny = 1; gd.nz = analysis == MPMGPU_THREED ? 1 : 0;
+    gd.dx = gd.dy = gd.dz = 1.;
+    o.g.build(gd);
+    o.mats.assign(mat, mat + 1);
+    o.np = 1;
+    o.pd.assign(state, state + MPF_NUM_FIELDS);
+    o.pi.assign(MPI_NUM_FIELDS, 0);
+    Mat3 m;
+    if (analysis == MPMGPU_THREED) m = Mat3::full(du[0], du[1], du[2], du[3], du[4], du[5], du[6], du[7], du[8]);
+    else m = Mat3::two(du[0], du[1], du[3], du[4], 0.);
+    o.constitutive(0, m);
+    memcpy(state, o.pd.data(), MPF_NUM_FIELDS * sizeof(double));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker of the HIP path. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_P = ctypes.c_void_p
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int32)
+_PL = ctypes.POINTER(ctypes.c_int64)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        import nairnmpm as nm  # structures shared with the product's C ABI
+        _lib.oracle_create.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.POINTER(nm.Material), ctypes.c_int32,
+                                       ctypes.POINTER(nm.RunParams), ctypes.c_int64, _PD, _PI]
+        _lib.oracle_create.restype = _P
+        _lib.oracle_destroy.argtypes = [_P]
+        _lib.oracle_step.argtypes = [_P, ctypes.c_int32]
+        _lib.oracle_run_task.argtypes = [_P, ctypes.c_int32]
+        _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
+        _lib.oracle_num_nodes.argtypes = [_P]
+        _lib.oracle_num_nodes.restype = ctypes.c_int64
+        _lib.oracle_get_grid.argtypes = [_P, _PD, _PD, _PD, _PI]
+        _lib.oracle_counters.argtypes = [_P, _PL]
+        _lib.oracle_shape.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.c_int32, ctypes.c_int64, _PD, _PD,
+                                      ctypes.c_int32, _PL, _PD, _PD, _PD, _PD]
+        _lib.oracle_constitutive.argtypes = [ctypes.c_int32, ctypes.POINTER(nm.Material), _PD, _PD]
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_PD)
+
+
+class Oracle:
+    """CPU restatement of NairnMPM's step on one problem (particles in mpm[] order)."""
+
+    def __init__(self, grid, mats, run, dfields, ifields):
+        import nairnmpm as nm
+        self.dfields = np.ascontiguousarray(dfields, dtype=np.float64)
+        self.ifields = np.ascontiguousarray(ifields, dtype=np.int32)
+        self.n = self.dfields.shape[1]
+        arr = (nm.Material * len(mats))(*mats)
+        self.grid = grid
+        self._h = lib().oracle_create(ctypes.byref(grid), arr, len(mats), ctypes.byref(run), self.n,
+                                      _d(self.dfields), self.ifields.ctypes.data_as(_PI))
+
+    @classmethod
+    def from_problem(cls, prob):
+        g, mats, run, _, _ = prob.describe()
+        d, i = prob.particles()
+        return cls(g, mats, run, d, i)
+
+    def step(self, n=1):
+        lib().oracle_step(self._h, n)
+
+    def run_task(self, t):
+        lib().oracle_run_task(self._h, t)
+
+    def particles(self):
+        import nairnmpm as nm
+        d = np.zeros((nm.NUM_FIELDS, self.n))
+        i = np.zeros((nm.NUM_IFIELDS, self.n), dtype=np.int32)
+        lib().oracle_get_particles(self._h, _d(d), i.ctypes.data_as(_PI))
+        return d, i
+
+    def grid_values(self):
+        nn = lib().oracle_num_nodes(self._h)
+        m = np.zeros(nn)
+        pk = np.zeros(3 * nn)
+        f = np.zeros(3 * nn)
+        cnt = np.zeros(nn, dtype=np.int32)
+        lib().oracle_get_grid(self._h, _d(m), _d(pk), _d(f), cnt.ctypes.data_as(_PI))
+        return m, pk.reshape(nn, 3).T, f.reshape(nn, 3).T, cnt
+
+    def counters(self):
+        c = np.zeros(3, dtype=np.int64)
+        lib().oracle_counters(self._h, c.ctypes.data_as(_PL))
+        return {"steps": int(c[0]), "leftGrid": int(c[1]), "nan": int(c[2])}
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().oracle_destroy(self._h)
+        except Exception:
+            pass
+
+
+def shape(grid, shape_kind, elem, pos, lp, deriv=True):
+    """Nodes and shape functions of one particle: (nodes, N, dNdx, dNdy, dNdz)."""
+    nodes = np.zeros(64, dtype=np.int64)
+    fn, dx, dy, dz = (np.zeros(64) for _ in range(4))
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    l = np.ascontiguousarray(lp, dtype=np.float64)
+    k = lib().oracle_shape(ctypes.byref(grid), shape_kind, elem, _d(p), _d(l), 1 if deriv else 0,
+                           nodes.ctypes.data_as(_PL), _d(fn), _d(dx), _d(dy), _d(dz))
+    return nodes[:k], fn[:k], dx[:k], dy[:k], dz[:k]
+
+
+def constitutive(analysis, mat, du, state):
+    """One constitutive update of one particle: du is 3x3, state is a NUM_FIELDS vector."""
+    s = np.ascontiguousarray(state, dtype=np.float64).copy()
+    d = np.ascontiguousarray(du, dtype=np.float64).reshape(9)
+    lib().oracle_constitutive(analysis, ctypes.byref(mat), _d(d), _d(s))
+    return s

@@ -1,0 +1,105 @@
+"""HIP path against the CPU oracle on the same seeded inputs (tolerance 1e-9 relative L2, the
+north-star bound; the gap is summation order of the grid sums only)."""
+import numpy as np
+import pytest
+
+import nairnmpm as nm
+from nairnmpm import configs
+import oracle as orc
+from helpers import compare
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def _pair(xml):
+    prob = nm.Problem.from_xml(xml)
+    return prob, nm.Simulation.from_problem(prob), orc.Oracle.from_problem(prob)
+
+
+def _check(sim, ora, dim, label, tol=TOL):
+    g, _ = sim.particles()
+    r, _ = ora.particles()
+    errs = compare(g, r, dim)
+    print(label, {k: f"{v:.3e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= tol, f"{label}: {k} relative L2 {v:.3e} > {tol}"
+    return errs
+
+
+def test_c1_elastic_2d_1000_steps():
+    prob, sim, ora = _pair(configs.elastic_block_2d())
+    done = 0
+    for target in (1, 10, 100, 1000):
+        sim.step(target - done)
+        ora.step(target - done)
+        done = target
+        _check(sim, ora, 2, f"C1 step {target}")
+    st = sim.stats()
+    assert st.steps == 1000 and st.leftGrid == 0 and st.nanParticles == 0
+
+
+def test_j2_3d_reduced_c2():
+    xml = configs.j2_block_3d(cells=24, lo=4, hi=20)
+    prob, sim, ora = _pair(xml)
+    for target, n in ((1, 1), (20, 19), (60, 40)):
+        sim.step(n)
+        ora.step(n)
+        _check(sim, ora, 3, f"J2 3D step {target}")
+    d, _ = sim.particles()
+    assert np.any(d[nm.F["hist0"]] > 0.0), "the reduced C2 block should yield"
+
+
+def test_neohookean_3d_block():
+    prob, sim, ora = _pair(configs.neohookean_block_3d(cells=20, lo=4, hi=16))
+    for n in (1, 29):
+        sim.step(n)
+        ora.step(n)
+        _check(sim, ora, 3, "NH 3D")
+
+
+def test_j2_2d_block():
+    prob, sim, ora = _pair(configs.j2_block_2d())
+    for n in (1, 99, 200):
+        sim.step(n)
+        ora.step(n)
+        _check(sim, ora, 2, "J2 2D")
+    d, _ = sim.particles()
+    assert np.any(d[nm.F["hist0"]] > 0.0)
+
+
+def test_task_by_task_equals_fused_step():
+    xml = configs.j2_block_3d(cells=16, lo=4, hi=12)
+    prob = nm.Problem.from_xml(xml)
+    a = nm.Simulation.from_problem(prob)
+    b = nm.Simulation.from_problem(prob)
+    for _ in range(3):
+        a.step(1)
+        for t in range(len(nm.TASKS)):
+            b.run_task(t)
+    da, _ = a.particles()
+    db, _ = b.particles()
+    assert np.array_equal(da, db)
+
+
+def test_deterministic_rerun():
+    xml = configs.j2_block_3d(cells=20, lo=4, hi=16)
+    prob = nm.Problem.from_xml(xml)
+    runs = []
+    for _ in range(2):
+        s = nm.Simulation.from_problem(prob)
+        s.step(25)
+        runs.append(s.particles()[0])
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_graph_replay_matches_eager():
+    xml = configs.elastic_block_2d(cells=24, block=12)
+    prob = nm.Problem.from_xml(xml)
+    a = nm.Simulation.from_problem(prob)
+    b = nm.Simulation.from_problem(prob)
+    a.step(10)                       # graph replays of two-step graphs
+    for _ in range(10):
+        b.step(1)                    # eager launches
+    assert np.array_equal(a.particles()[0], b.particles()[0])
