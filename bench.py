@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""FP64 particle-steps/s of NairnMPM's explicit step on one MI355X.
+
+Workload: the 3D elastic-plastic block -- 128^3 grid of 1 mm cells, 2x2x2 particles per cell in
+the central 96^3 cells (7,077,888 particles), IsoPlasticity J2 with linear hardening (E=70 GPa,
+nu=0.33, yield 300 MPa, Ep 1 GPa, rho 2.7 g/cm^3), uGIMP, USAVG+ with FLIP, the two halves
+colliding at +-50 m/s. A step is one NairnMPM::MPMStep over all particles; inputs are resident in
+HBM before the timed region. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nairn-mpm-fea_amd"))
+
+import numpy as np  # noqa: E402
+
+import nairnmpm as nm  # noqa: E402
+from nairnmpm import configs  # noqa: E402
+
+METRIC = "FP64 particle-steps/s (3D J2 GIMP) + served p50/p99 latency vs serial"
+HBM_PEAK = 8.0e12
+
+# Algorithmic HBM bytes of each kernel group for the 3D J2 path: (bytes per particle, bytes per
+# active node). Particle terms count every particle word the kernel must read or write once; node
+# terms count the grid values it must read or write once.
+ALG_BYTES = {
+    "k_p2g_mass": (56 + 4, 32),        # pos, vel, mass, elem -> m, p
+    "k_node_mass": (0, 2 * 32 + 24),   # m, p (partials) -> m, p, v
+    "k_usf_force": (24 + 8 + 4 + 2 * 184, 24 + 24),   # pos, mass, elem, state r/w; v in, f out
+    "k_node_force": (0, 24 + 24 + 24 + 24),           # f in; p r/w; v out
+    "k_g2p_p2g": (24 + 24 + 8 + 4 + 48, 56 + 24),     # pos, vel, mass, elem; vel, pos out; v,f,m in; p out
+    "k_node_mom2": (0, 24 + 24),
+    "k_usl_reset": (24 + 24 + 4 + 2 * 184 + 8, 24),   # pos, new pos, elem, state r/w, elem+key out; v in
+    "binning": (16, 0),
+}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU slab "
+              "decomposition is a design, not a built path", file=sys.stderr)
+        sys.exit(2)
+
+    lo, hi = a.cells // 8, a.cells - a.cells // 8
+    t0 = time.time()
+    prob = nm.Problem.from_xml(configs.j2_block_3d(cells=a.cells, lo=lo, hi=hi))
+    grid, mats, run, npart, _ = prob.describe()
+    dfields, ifields = prob.particles()
+    setup_s = time.time() - t0
+    print(f"# setup {setup_s:.1f} s: {npart} particles, grid {grid.nx}x{grid.ny}x{grid.nz} cells, "
+          f"dt {run.timestep:.4e} s", file=sys.stderr, flush=True)
+
+    sim = nm.Simulation(grid, mats, run)
+    sim.upload(dfields, ifields)
+    sim.step(a.warmup)
+    sim.sync()
+    mass, _, _ = sim.grid_values()
+    active_nodes = int(np.count_nonzero(mass))
+
+    # timed region: K steps, HIP events between kernel groups on the simulation's stream
+    sim.set_profiling(True)
+    sim.sync()
+    t1 = time.perf_counter()
+    sim.step(a.steps)
+    sim.sync()
+    t2 = time.perf_counter()
+    times = sim.kernel_times()
+    sim.set_profiling(False)
+    elapsed = t2 - t1
+    value = npart * a.steps / elapsed
+
+    # the same number of steps replayed from the captured two-step graph
+    sim.sync()
+    t3 = time.perf_counter()
+    sim.step(a.steps)
+    sim.sync()
+    graph_value = npart * a.steps / (time.perf_counter() - t3)
+    st = sim.stats()
+    print(f"# eager+events {value:.4e} particle-steps/s, graph replay {graph_value:.4e}; "
+          f"left grid {st.leftGrid}, nan {st.nanParticles}", file=sys.stderr, flush=True)
+    for g, (ms, n) in times.items():
+        print(f"#   {g:14s} {ms / max(n, 1):9.4f} ms/launch ({n} launches)", file=sys.stderr)
+
+    dom = max(times, key=lambda g: times[g][0])
+    dom_ms = times[dom][0] / max(times[dom][1], 1)
+    bp, bn = ALG_BYTES[dom]
+    alg_bytes = bp * npart + bn * active_nodes
+    achieved = alg_bytes / (dom_ms * 1e-3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms}
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        o = orc.Oracle(grid, mats, run, dfields, ifields)
+        tc = time.perf_counter()
+        o.step(a.cpu_steps)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": npart * a.cpu_steps / cpu_s, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{a.cpu_steps} step(s) of the same {npart}-particle block on one host core "
+                         f"(oracle/mpm_oracle.cpp, serial), {cpu_s:.1f} s"}
+        del o
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D J2 GIMP block, {a.cells}^3 cells of 1 mm, 2x2x2 particles per cell "
+                               f"in [{lo},{hi}]^3 mm, halves colliding at +-50 m/s",
+                   "particles": npart, "grid_cells": [grid.nx, grid.ny, grid.nz], "active_nodes": active_nodes,
+                   "material": "IsoPlasticity J2, linear hardening", "shape": "uGIMP", "method": "USAVG+ FLIP",
+                   "graph_replay_value": graph_value},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -136,6 +136,19 @@ enum {
     MPMGPU_NUM_TASKS
 };
 
+/* kernel groups of one step, for mpmgpu_kernel_times */
+enum {
+    MPMGPU_KG_P2G_MASS = 0,   /* MassAndMomentumTask scatter */
+    MPMGPU_KG_NODE_MASS,      /* PostExtrapolationTask + grid velocity */
+    MPMGPU_KG_USF_FORCE,      /* UpdateStrainsFirstTask + GridForcesTask */
+    MPMGPU_KG_NODE_FORCE,     /* PostForcesTask + UpdateMomentaTask */
+    MPMGPU_KG_G2P_P2G,        /* UpdateParticlesTask + USL momentum re-extrapolation */
+    MPMGPU_KG_NODE_MOM2,      /* USL grid velocity */
+    MPMGPU_KG_USL_RESET,      /* USL strain update + ResetElementsTask */
+    MPMGPU_KG_BINNING,        /* cell sort of the particles for the next step */
+    MPMGPU_NUM_KERNEL_GROUPS
+};
+
 typedef struct MpmProblem MpmProblem;   /* host-side problem (input + generated particles) */
 typedef struct MpmCtx MpmCtx;           /* device context: all device buffers of one simulation */
 
@@ -201,6 +214,13 @@ int mpmgpu_download_particles(MpmCtx *ctx, double *dfields, int32_t *ifields);
 int mpmgpu_query(MpmCtx *ctx, MpmStats *stats);
 /* Grid node values (mass, momentum, force) after the last task, nodes numbered x fastest. */
 int mpmgpu_download_grid(MpmCtx *ctx, double *mass, double *pk, double *ftot);
+/* Per-kernel-group HIP event timing on the context's stream (MPMTask::TrackTimes,
+   MPMTask.cpp:51-72). Turning it on or off clears the totals; mpmgpu_step then launches
+   eagerly (no graph replay) with an event between kernel groups. */
+int mpmgpu_set_profiling(MpmCtx *ctx, int32_t on);
+/* totals (ms) and launch counts per MPMGPU_KG_* group since profiling was turned on */
+int mpmgpu_kernel_times(MpmCtx *ctx, double *totalMs, int64_t *launches);
+const char *mpmgpu_kernel_group_name(int32_t group);
 const char *mpmgpu_last_error(const MpmCtx *ctx);
 const char *mpmgpu_global_error(void);
 void mpmgpu_destroy(MpmCtx *ctx);

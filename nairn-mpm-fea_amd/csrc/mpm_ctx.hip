@@ -45,7 +45,25 @@ struct MpmCtx {
     double mtime = 0.;
     float lastStepMs = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // kernel timing: events around each kernel group of a step, on this context's stream
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::pair<int, std::pair<int, int>>> evMarks;   // (group, (start ev, end ev))
+    size_t evUsed = 0;
+    double groupMs[MPMGPU_NUM_KERNEL_GROUPS] = {0};
+    int64_t groupLaunches[MPMGPU_NUM_KERNEL_GROUPS] = {0};
     std::string err;
+
+    int mark() {
+        if (evUsed == evPool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            evPool.push_back(e);
+        }
+        int i = (int)evUsed++;
+        (void)hipEventRecord(evPool[i], stream);
+        return i;
+    }
 
     int fail(const std::string &m) {
         err = m;
@@ -74,19 +92,48 @@ static void destroy_graphs(MpmCtx *c) {
         if (c->graph[i]) { (void)hipGraphExecDestroy(c->graph[i]); c->graph[i] = nullptr; }
 }
 
-// the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108)
+// the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108); with
+// profiling on, an event is recorded between kernel groups
 static void enqueue_step(MpmCtx *c) {
     hipStream_t s = c->stream;
     set_positions(c);
+    int e = c->profiling ? c->mark() : -1;
+    auto done = [&](int group) {
+        if (!c->profiling) return;
+        int e2 = c->mark();
+        c->evMarks.push_back({group, {e, e2}});
+        e = e2;
+    };
     launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+    done(MPMGPU_KG_P2G_MASS);
     launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
+    done(MPMGPU_KG_NODE_MASS);
     launch_usf_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+    done(MPMGPU_KG_USF_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
+    done(MPMGPU_KG_NODE_FORCE);
     launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s);
+    done(MPMGPU_KG_G2P_P2G);
     launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
+    done(MPMGPU_KG_NODE_MOM2);
     launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
+    done(MPMGPU_KG_USL_RESET);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, s);
+    done(MPMGPU_KG_BINNING);
     c->parity ^= 1;
+}
+
+// fold the recorded kernel-group events into per-group totals (after a stream sync)
+static void collect_marks(MpmCtx *c) {
+    for (auto &m : c->evMarks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->evPool[m.second.first], c->evPool[m.second.second]) == hipSuccess) {
+            c->groupMs[m.first] += ms;
+            c->groupLaunches[m.first] += 1;
+        }
+    }
+    c->evMarks.clear();
+    c->evUsed = 0;
 }
 
 extern "C" {
@@ -431,7 +478,7 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
     if (c->kp.n == 0) { c->steps += nsteps; c->mtime += nsteps * c->run.timestep; return 0; }
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
     int done = 0;
-    if (nsteps >= 2) {
+    if (nsteps >= 2 && !c->profiling) {
         int par = c->parity;
         if (!c->graph[par]) {
             hipGraph_t gr;
@@ -456,6 +503,7 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
 
 int mpmgpu_sync(MpmCtx *c) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    collect_marks(c);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->lastStepMs = ms;
     return 0;
@@ -494,9 +542,35 @@ int mpmgpu_query(MpmCtx *c, MpmStats *st) {
     return 0;
 }
 
+int mpmgpu_set_profiling(MpmCtx *c, int32_t on) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    collect_marks(c);
+    c->profiling = on != 0;
+    for (int g = 0; g < MPMGPU_NUM_KERNEL_GROUPS; g++) { c->groupMs[g] = 0.; c->groupLaunches[g] = 0; }
+    return 0;
+}
+
+int mpmgpu_kernel_times(MpmCtx *c, double *totalMs, int64_t *launches) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    collect_marks(c);
+    for (int g = 0; g < MPMGPU_NUM_KERNEL_GROUPS; g++) {
+        if (totalMs) totalMs[g] = c->groupMs[g];
+        if (launches) launches[g] = c->groupLaunches[g];
+    }
+    return 0;
+}
+
+const char *mpmgpu_kernel_group_name(int32_t g) {
+    static const char *names[MPMGPU_NUM_KERNEL_GROUPS] = {
+        "k_p2g_mass", "k_node_mass", "k_usf_force", "k_node_force",
+        "k_g2p_p2g", "k_node_mom2", "k_usl_reset", "binning"};
+    return g >= 0 && g < MPMGPU_NUM_KERNEL_GROUPS ? names[g] : "";
+}
+
 void mpmgpu_destroy(MpmCtx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->evPool) (void)hipEventDestroy(e);
     destroy_graphs(c);
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);

@@ -111,7 +111,13 @@ _SIGNATURES = {
     "mpmgpu_query": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
     "mpmgpu_download_grid": ([_P, _PD, _PD, _PD], ctypes.c_int),
     "mpmgpu_destroy": ([_P], None),
+    "mpmgpu_set_profiling": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_kernel_times": ([_P, _PD, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "mpmgpu_kernel_group_name": ([ctypes.c_int32], ctypes.c_char_p),
 }
+
+KERNEL_GROUPS = ["k_p2g_mass", "k_node_mass", "k_usf_force", "k_node_force", "k_g2p_p2g",
+                 "k_node_mom2", "k_usl_reset", "binning"]
 
 
 def library() -> ctypes.CDLL:
@@ -266,6 +272,18 @@ class Simulation:
         f = np.zeros(3 * nn)
         _check(library().mpmgpu_download_grid(self._h, _dptr(m), _dptr(pk), _dptr(f)), self._h)
         return m, pk.reshape(3, nn), f.reshape(3, nn)
+
+    def set_profiling(self, on: bool):
+        """HIP-event timing of every kernel group (eager launches while on)."""
+        _check(library().mpmgpu_set_profiling(self._h, 1 if on else 0), self._h)
+
+    def kernel_times(self):
+        """{group: (total ms, launches)} since profiling was turned on"""
+        ms = np.zeros(len(KERNEL_GROUPS))
+        n = np.zeros(len(KERNEL_GROUPS), dtype=np.int64)
+        _check(library().mpmgpu_kernel_times(self._h, _dptr(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))),
+               self._h)
+        return {g: (float(ms[k]), int(n[k])) for k, g in enumerate(KERNEL_GROUPS)}
 
     def stats(self) -> Stats:
         s = Stats()
