@@ -30,7 +30,8 @@ HBM_PEAK = 8.0e12
 ALG_BYTES = {
     "k_p2g_mass": (56 + 4, 32),        # pos, vel, mass, elem -> m, p
     "k_node_mass": (0, 2 * 32 + 24),   # m, p (partials) -> m, p, v
-    "k_usf_force": (24 + 8 + 4 + 2 * 184, 24 + 24),   # pos, mass, elem, state r/w; v in, f out
+    "k_usf": (24 + 4 + 4 + 2 * 184, 24),              # pos, elem, mat, state r/w; v in
+    "k_p2g_force": (24 + 4 + 8 + 56, 24),             # pos, elem, mass, stress+P; f out
     "k_node_force": (0, 24 + 24 + 24 + 24),           # f in; p r/w; v out
     "k_g2p_p2g": (24 + 24 + 8 + 4 + 48, 56 + 24),     # pos, vel, mass, elem; vel, pos out; v,f,m in; p out
     "k_node_mom2": (0, 24 + 24),

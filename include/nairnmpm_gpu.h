@@ -140,7 +140,8 @@ enum {
 enum {
     MPMGPU_KG_P2G_MASS = 0,   /* MassAndMomentumTask scatter */
     MPMGPU_KG_NODE_MASS,      /* PostExtrapolationTask + grid velocity */
-    MPMGPU_KG_USF_FORCE,      /* UpdateStrainsFirstTask + GridForcesTask */
+    MPMGPU_KG_USF,            /* UpdateStrainsFirstTask */
+    MPMGPU_KG_P2G_FORCE,      /* GridForcesTask */
     MPMGPU_KG_NODE_FORCE,     /* PostForcesTask + UpdateMomentaTask */
     MPMGPU_KG_G2P_P2G,        /* UpdateParticlesTask + USL momentum re-extrapolation */
     MPMGPU_KG_NODE_MOM2,      /* USL grid velocity */

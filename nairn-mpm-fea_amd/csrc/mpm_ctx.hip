@@ -108,8 +108,10 @@ static void enqueue_step(MpmCtx *c) {
     done(MPMGPU_KG_P2G_MASS);
     launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
     done(MPMGPU_KG_NODE_MASS);
-    launch_usf_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
-    done(MPMGPU_KG_USF_FORCE);
+    launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+    done(MPMGPU_KG_USF);
+    launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s);
+    done(MPMGPU_KG_P2G_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
     done(MPMGPU_KG_NODE_FORCE);
     launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s);
@@ -416,6 +418,15 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         int t = c->mats[m].type;
         if (t == MPMGPU_MAT_COULOMB || t == MPMGPU_MAT_CONTACT) return c->fail("particle with a contact-law material");
         if (e < 0 || e >= nelem) return c->fail("particle element outside the grid");
+        if (!(dfields[MPF_MASS * nn + i] > 0.)) return c->fail("particle with a mass that is not positive");
+    }
+    // one material type for all particles lets the strain kernels compile a single law
+    c->kg.matType = 0;
+    if (n > 0) {
+        int t0 = c->mats[ifields[MPI_MAT * nn]].type;
+        bool same = true;
+        for (int64_t i = 1; i < n && same; i++) same = c->mats[ifields[MPI_MAT * nn + i]].type == t0;
+        c->kg.matType = same ? t0 : 0;
     }
     // uniform particle size lets the kernels skip the lp arrays
     c->kg.uniformLp = 1;
@@ -451,8 +462,8 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
             break;
         case MPMGPU_TASK_POST_EXTRAPOLATION: break;   // done by the node pass of the scatter
-        case MPMGPU_TASK_UPDATE_STRAINS_FIRST: break;  // fused with the force scatter
-        case MPMGPU_TASK_GRID_FORCES: launch_usf_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
+        case MPMGPU_TASK_UPDATE_STRAINS_FIRST: launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
+        case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
         case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
         case MPMGPU_TASK_UPDATE_MOMENTA: launch_node_force(c->dim, c->kg, c->kb, c->kn, s); break;
         case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
@@ -562,7 +573,7 @@ int mpmgpu_kernel_times(MpmCtx *c, double *totalMs, int64_t *launches) {
 
 const char *mpmgpu_kernel_group_name(int32_t g) {
     static const char *names[MPMGPU_NUM_KERNEL_GROUPS] = {
-        "k_p2g_mass", "k_node_mass", "k_usf_force", "k_node_force",
+        "k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_node_force",
         "k_g2p_p2g", "k_node_mom2", "k_usl_reset", "binning"};
     return g >= 0 && g < MPMGPU_NUM_KERNEL_GROUPS ? names[g] : "";
 }

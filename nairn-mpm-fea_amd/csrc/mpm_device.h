@@ -384,8 +384,12 @@ __device__ __forceinline__ void law_neohookean(const double du[3][3], const MpmM
     if (DIM == 3) { s.sp[4] = GJeff * Bxz; s.sp[3] = GJeff * Byz; }
 }
 
-template <int DIM>
+// MAT = 0 dispatches on the particle's material type; a nonzero MAT compiles one law only
+template <int DIM, int MAT = 0>
 __device__ __forceinline__ void constitutive(const double du[3][3], const MpmMaterial &m, PState &s) {
+    if (MAT == MPMGPU_MAT_ISOTROPIC) { law_isotropic<DIM>(du, m, s); return; }
+    if (MAT == MPMGPU_MAT_ISOPLASTICITY) { law_isoplastic<DIM>(du, m, s); return; }
+    if (MAT == MPMGPU_MAT_NEOHOOKEAN) { law_neohookean<DIM>(du, m, s); return; }
     switch (m.type) {
         case MPMGPU_MAT_ISOTROPIC: law_isotropic<DIM>(du, m, s); break;
         case MPMGPU_MAT_ISOPLASTICITY: law_isoplastic<DIM>(du, m, s); break;

@@ -32,6 +32,7 @@ struct KGrid {
     double xmin, ymin, zmin, dx, dy, dz;
     double timestep, strainFirst, strainLast;
     int uniformLp;             // 1 when every particle has the same lp (then lp[] is used)
+    int matType;               // the one material type of all particles, 0 when mixed
     double lp[3];
 };
 
@@ -66,7 +67,7 @@ struct KStats {
     unsigned long long *nanCount;
 };
 
-enum { NQ_MASS = 5, NQ_FORCE = 3, NQ_MOM = 3, NQMAX = 5 };
+enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQMAX = 4 };
 
 // host-side launchers (mpm_kernels.hip); all take the context's stream
 void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
@@ -74,8 +75,9 @@ void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, 
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                      const MpmMaterial *mats, hipStream_t s);
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
-void launch_usf_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                      const MpmMaterial *mats, hipStream_t s);
+void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                const MpmMaterial *mats, hipStream_t s);
+void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                     hipStream_t s);

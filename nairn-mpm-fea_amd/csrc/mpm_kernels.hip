@@ -3,11 +3,13 @@
 // Layout: particles stay in structure-of-arrays form; every step they are binned by cell with a
 // stable counting sort into tiles of TileGeom<DIM> cells (tile-major cell keys), which gives a
 // sorted slot -> particle map (order[]). Particle-to-grid sums never use floating-point atomics:
-// a tile kernel stages its particles' per-axis GIMP factors in LDS, and every node of the
-// tile's footprint is owned by one thread that adds the contributions in a fixed order (cells
-// ascending, particles ascending in the sorted order). The footprint sums of each tile go to a
-// partial buffer that a node kernel adds in fixed tile order. Results are therefore
-// deterministic; they differ from the reference's single-patch sums by summation order only.
+// a tile kernel stages its particles' per-axis GIMP factors in LDS; then one wave handles one
+// cell at a time with each lane owning one of the 64 candidate node offsets, sums the cell's
+// particles in sorted order in registers and adds the sum to the wave's own LDS copy of the
+// tile's node footprint (wave_cells). The copies are added in wave order and the footprint sums
+// of each tile go to a partial buffer that a node kernel adds in fixed tile order. Results are
+// therefore deterministic; they differ from the reference's single-patch sums by summation
+// order only.
 //
 // Reference tasks (paths relative to the reference root, NairnMPM/src/NairnMPM_Class/):
 //   k_p2g_mass    MassAndMomentumTask.cpp:48-127      (+ InitializationTask xi, :43-85)
@@ -205,32 +207,76 @@ __global__ void k_bin_sort_cells(KBins B) {
 }
 
 // ---------------------------------------------------------------------------------------
-// node-owner accumulation over the tile's staged chunk. Calls body(slotIndex j, lq, ix, iy, iz)
-// for every (owned footprint slot, staged particle) pair whose cell is within reach, in the
-// fixed order cells ascending, particles ascending.
-template <int DIM, int SPT, typename Body>
-__device__ __forceinline__ void node_phase(const int *s_cs, int c0, int c1, Body body) {
+// Deterministic per-wave scatter. Wave w of the block takes the tile's cells w, w+NW, ... in
+// order; for one cell, lane l owns the candidate node offset (l%4-1, (l/4)%4-1, l/16-1) (2D: the
+// first 16 lanes, (l%4-1, l/4-1)), so the 64 lanes write 64 different nodes. The lane sums the
+// cell's staged particles in sorted order in registers, then adds the sum to the wave's own
+// accumulator copy in LDS. The copies are added in wave order at the end. Every node value is
+// therefore the same fixed-order sum on every run.
+#define NW (NT / 64)
+
+template <int DIM, int NQ, typename Load, typename Add>
+__device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
+                                           double (*s_acc)[NQ][TileGeom<DIM>::FOOT], Load load, Add add) {
     using G = TileGeom<DIM>;
-    int tid = threadIdx.x;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int o1x = l & 3, o1y = (l >> 2) & 3, o1z = DIM == 3 ? (l >> 4) : 1;   // offset + 1
+    const bool lane_on = DIM == 3 || l < 16;
+    for (int c = w; c < G::NC; c += NW) {
+        int b = max(s_cs[c], r0), e = min(s_cs[c + 1], r1);
+        if (b >= e) continue;
+        int lx = c % G::TX, ly = (c / G::TX) % G::TY, lz = DIM == 3 ? c / (G::TX * G::TY) : 0;
+        double acc[NQ];
 #pragma unroll
-    for (int j = 0; j < SPT; j++) {
-        int s = tid + j * NT;
-        if (s >= G::FOOT) break;
-        int fx = s % G::FX;
-        int fy = (s / G::FX) % G::FY;
-        int fz = DIM == 3 ? s / (G::FX * G::FY) : 1;
-        int nlx = fx - 1, nly = fy - 1, nlz = fz - 1;
-        int cx0 = max(0, nlx - 2), cx1 = min(G::TX - 1, nlx + 1);
-        int cy0 = max(0, nly - 2), cy1 = min(G::TY - 1, nly + 1);
-        int cz0 = DIM == 3 ? max(0, nlz - 2) : 0, cz1 = DIM == 3 ? min(G::TZ - 1, nlz + 1) : 0;
-        for (int cz = cz0; cz <= cz1; cz++)
-            for (int cy = cy0; cy <= cy1; cy++)
-                for (int cx = cx0; cx <= cx1; cx++) {
-                    int c = (cz * G::TY + cy) * G::TX + cx;
-                    int b = max(s_cs[c], c0), e = min(s_cs[c + 1], c1);
-                    int ix = nlx - cx + 1, iy = nly - cy + 1, iz = DIM == 3 ? nlz - cz + 1 : 0;
-                    for (int q = b; q < e; q++) body(j, q - c0, ix, iy, iz);
-                }
+        for (int q = 0; q < NQ; q++) acc[q] = 0.;
+        if (lane_on) {
+            // records of four particles are read before they are added, in particle order
+            int k = b;
+            for (; k + 4 <= e; k += 4) {
+                auto a0 = load(k - r0, o1x, o1y, o1z);
+                auto a1 = load(k + 1 - r0, o1x, o1y, o1z);
+                auto a2 = load(k + 2 - r0, o1x, o1y, o1z);
+                auto a3 = load(k + 3 - r0, o1x, o1y, o1z);
+                add(a0, acc, lx, ly, lz);
+                add(a1, acc, lx, ly, lz);
+                add(a2, acc, lx, ly, lz);
+                add(a3, acc, lx, ly, lz);
+            }
+            for (; k < e; k++) {
+                auto a0 = load(k - r0, o1x, o1y, o1z);
+                add(a0, acc, lx, ly, lz);
+            }
+            int slot = foot_slot<DIM>(lx + o1x - 1, ly + o1y - 1, lz + o1z - 1);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) s_acc[w][q][slot] += acc[q];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// particle records of the scatters: per-axis factors of the lane's candidate and the payload
+struct RecMass { double wx, wy, wz, mp, vx, vy, vz; };
+struct RecForce { double wx, wy, wz, dx, dy, dz, negmp, s0, s1, s2, s3, s4, s5; };
+
+template <int DIM, int NQ>
+__device__ __forceinline__ void zero_acc(double (*s_acc)[NQ][TileGeom<DIM>::FOOT]) {
+    using G = TileGeom<DIM>;
+    double *a = &s_acc[0][0][0];
+    for (int i = threadIdx.x; i < NW * NQ * G::FOOT; i += NT) a[i] = 0.;
+}
+
+// add the wave copies in wave order and store the tile's footprint sums
+template <int DIM, int NQ>
+__device__ __forceinline__ void write_partials(const KNodes &N, int t, double (*s_acc)[NQ][TileGeom<DIM>::FOOT]) {
+    using G = TileGeom<DIM>;
+    for (int s = threadIdx.x; s < G::FOOT; s += NT) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double v = s_acc[0][q][s];
+#pragma unroll
+            for (int w = 1; w < NW; w++) v += s_acc[w][q][s];
+            N.part[((size_t)t * NQMAX + q) * G::FOOT + s] = v;
+        }
     }
 }
 
@@ -241,37 +287,22 @@ __device__ __forceinline__ void load_tile_starts(const KBins &B, int t, int *s_c
     __syncthreads();
 }
 
-template <int DIM, int NQ, int SPT>
-__device__ __forceinline__ void write_partials(const KNodes &N, int t, double acc[SPT][NQ]) {
-    using G = TileGeom<DIM>;
-#pragma unroll
-    for (int j = 0; j < SPT; j++) {
-        int s = threadIdx.x + j * NT;
-        if (s >= G::FOOT) break;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) N.part[((size_t)t * NQMAX + q) * G::FOOT + s] = acc[j][q];
-    }
-}
-
 // ---------------------------------------------------------------------------------------
-// P2G #1: mass, momentum, point count (MassAndMomentumTask -> NodalPoint::AddMassMomentum,
-// NodalPointMPM.cpp:419-453)
+// P2G #1: mass and momentum (MassAndMomentumTask -> NodalPoint::AddMassMomentum,
+// NodalPointMPM.cpp:419-453). The point count of a node is not summed: every particle has
+// mp > 0 and every in-support weight is > 0, so numberPoints > 0 exactly when the mass is > 0.
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
-    constexpr int SPT = (G::FOOT + NT - 1) / NT;
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_S[DIM][4][NT];
-    __shared__ double s_q[4][NT];
+    __shared__ double s_S[NT][DIM][4];
+    __shared__ double s_q[NT][4];
+    __shared__ double s_acc[NW][NQ_MASS][G::FOOT];
     const int t = blockIdx.x;
     load_tile_starts<DIM>(B, t, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
-    double acc[SPT][NQ_MASS];
-#pragma unroll
-    for (int j = 0; j < SPT; j++)
-#pragma unroll
-        for (int q = 0; q < NQ_MASS; q++) acc[j][q] = 0.;
+    zero_acc<DIM, NQ_MASS>(s_acc);
     for (int c0 = tb; c0 < te; c0 += NT) {
         int k = c0 + threadIdx.x;
         if (k < te) {
@@ -285,33 +316,32 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNod
 #pragma unroll
             for (int a = 0; a < DIM; a++)
 #pragma unroll
-                for (int o = 0; o < 4; o++) s_S[a][o][threadIdx.x] = S[a][o];
-            s_q[0][threadIdx.x] = P.f[MPF_MASS][p];
-            s_q[1][threadIdx.x] = P.f[MPF_VEL_X][p];
-            s_q[2][threadIdx.x] = P.f[MPF_VEL_Y][p];
-            s_q[3][threadIdx.x] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+                for (int o = 0; o < 4; o++) s_S[threadIdx.x][a][o] = S[a][o];
+            s_q[threadIdx.x][0] = P.f[MPF_MASS][p];
+            s_q[threadIdx.x][1] = P.f[MPF_VEL_X][p];
+            s_q[threadIdx.x][2] = P.f[MPF_VEL_Y][p];
+            s_q[threadIdx.x][3] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
         }
         __syncthreads();
-        int c1 = min(c0 + NT, te);
-        node_phase<DIM, SPT>(s_cs, c0, c1, [&](int j, int lq, int ix, int iy, int iz) {
-            double wx = s_S[0][ix][lq], wy = s_S[1][iy][lq];
-            if (wx < 0. || wy < 0.) return;
-            double w;
-            if (DIM == 3) {
-                double wz = s_S[DIM - 1][iz][lq];
-                if (wz < 0.) return;
-                w = wx * wy * wz;
-            } else w = wx * wy;
-            double fnmp = w * s_q[0][lq];
-            acc[j][1] += fnmp * s_q[1][lq];
-            acc[j][2] += fnmp * s_q[2][lq];
-            acc[j][3] += fnmp * s_q[3][lq];
-            acc[j][0] += fnmp;
-            acc[j][4] += 1.;
-        });
+        wave_cells<DIM, NQ_MASS>(s_cs, c0, min(c0 + NT, te), s_acc,
+            [&](int lq, int ix, int iy, int iz) {
+                RecMass r;
+                r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                r.mp = s_q[lq][0]; r.vx = s_q[lq][1]; r.vy = s_q[lq][2]; r.vz = s_q[lq][3];
+                return r;
+            },
+            [&](const RecMass &r, double *acc, int, int, int) {
+                bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
+                double w = DIM == 3 ? r.wx * r.wy * r.wz : r.wx * r.wy;
+                double fnmp = on ? w * r.mp : 0.;
+                acc[1] += fnmp * r.vx;
+                acc[2] += fnmp * r.vy;
+                acc[3] += fnmp * r.vz;
+                acc[0] += fnmp;
+            });
         __syncthreads();
     }
-    write_partials<DIM, NQ_MASS, SPT>(N, t, acc);
+    write_partials<DIM, NQ_MASS>(N, t, s_acc);
 }
 
 // sum the partials of the tiles that cover node (i,j,k), in fixed tile order
@@ -352,7 +382,7 @@ __global__ void k_node_mass(KGrid g, KBins B, KNodes N) {
     long long nn = g.nnodes;
     N.mass[n] = a[0];
     N.pk[n] = a[1]; N.pk[nn + n] = a[2]; N.pk[2 * nn + n] = a[3];
-    int cnt = (int)a[4];
+    int cnt = a[0] > 0. ? 1 : 0;       // numberPoints > 0 <=> mass > 0 (all mp > 0)
     N.cnt[n] = cnt;
     double vx = 0., vy = 0., vz = 0.;
     if (cnt > 0 && a[0] != 0.) {
@@ -387,6 +417,7 @@ __device__ __forceinline__ void velocity_gradient(const double S[3][4], const do
 #pragma unroll
         for (int b = 0; b < 3; b++) L[a][b] = 0.;
     if (DIM == 3) {
+#pragma unroll
         for (int id = 0; id < 64; id++) {
             int ox = c_off3x[id] + 1, oy = c_off3y[id] + 1, oz = c_off3z[id] + 1;
             double Sx = S[0][ox], Sy = S[1][oy], Sz = S[2][oz];
@@ -401,6 +432,7 @@ __device__ __forceinline__ void velocity_gradient(const double S[3][4], const do
             L[2][0] += vz * gx; L[2][1] += vz * gy; L[2][2] += vz * gz;
         }
     } else {
+#pragma unroll
         for (int id = 0; id < 16; id++) {
             int ox = c_off2x[id] + 1, oy = c_off2y[id] + 1;
             double Sx = S[0][ox], Sy = S[1][oy];
@@ -471,20 +503,13 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
 }
 
 // ---------------------------------------------------------------------------------------
-// USF strain update (first half step) fused with the internal-force scatter
-// (UpdateStrainsFirstTask::FullStrainUpdate, GridForcesTask + MatPoint3D::GetFintPlusFext,
-// MatPoint3D.cpp:248-270, MatPoint2D.cpp:242-263). pFext is zero (no particle loads).
-template <int DIM>
-__global__ __launch_bounds__(NT) void k_usf_force(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
+// USF strain update, first half step (UpdateStrainsFirstTask::FullStrainUpdate,
+// UpdateStrainsFirstTask.cpp:101-168 -> MatPoint3D::UpdateStrain, MatPoint3D.cpp:45-60)
+template <int DIM, int MAT>
+__global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
     using G = TileGeom<DIM>;
-    constexpr int SPT = (G::FOOT + NT - 1) / NT;
-    constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    __shared__ double s_S[DIM][4][NT];
-    __shared__ double s_dS[DIM][4][NT];
-    __shared__ double s_id[DIM][NT];
-    __shared__ double s_st[NS][NT];
     const int t = blockIdx.x;
     load_tile_starts<DIM>(B, t, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
@@ -497,14 +522,57 @@ __global__ __launch_bounds__(NT) void k_usf_force(KGrid g, KPart P, KBins B, KNo
         load_foot<DIM, 3>(g, ox, oy, oz, src, st, s_v);
     }
     __syncthreads();
-    double acc[SPT][NQ_FORCE];
-#pragma unroll
-    for (int j = 0; j < SPT; j++)
-#pragma unroll
-        for (int q = 0; q < NQ_FORCE; q++) acc[j][q] = 0.;
-    for (int c0 = tb; c0 < te; c0 += NT) {
+    for (int k = tb + threadIdx.x; k < te; k += NT) {
+        int p = B.order[k];
+        int ei, ej, ek;
+        elem_ijk(g, P.elem[p], ei, ej, ek);
+        double xi[3], invd[3], S[3][4], dS[3][4];
+        particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                         DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+        particle_factors<DIM>(g, P, p, xi, S, dS);
+        double L[3][3];
+        velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+        scale_gradient<DIM>(L, g.strainFirst);
+        const MpmMaterial &m = mats[P.mat[p]];
+        const int mtype = MAT ? MAT : m.type;
+        PState s;
+        load_state<DIM>(P, p, mtype, s);
+        constitutive<DIM, MAT>(L, m, s);
+        store_state<DIM>(P, p, mtype, s);
+    }
+}
+
+// internal-force scatter (GridForcesTask::Execute, GridForcesTask.cpp:40-153 ->
+// MatPoint3D::GetFintPlusFext, MatPoint3D.cpp:248-270; MatPoint2D.cpp:242-263) with the stress
+// just updated by k_usf; pFext is zero (no particle loads)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNodes N) {
+    using G = TileGeom<DIM>;
+    constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
+    constexpr int CH = NT / 2;                // particles staged per chunk (keeps LDS < 80 KB)
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_S[CH][DIM][4];
+    __shared__ double s_dS[CH][DIM][4];
+    __shared__ double s_st[CH][NS];
+    __shared__ double s_acc[NW][NQ_FORCE][G::FOOT];
+    __shared__ double s_id[DIM][G::TX];       // 2/dx of the tile's cell columns, rows, layers
+    const int t = blockIdx.x;
+    load_tile_starts<DIM>(B, t, s_cs);
+    const int tb = s_cs[0], te = s_cs[G::NC];
+    if (tb == te) return;
+    int ox, oy, oz;
+    tile_origin<DIM>(g, t, ox, oy, oz);
+    if (threadIdx.x < G::TX) {
+        int i = threadIdx.x;
+        s_id[0][i] = 2.0 / ((g.xmin + (double)(ox + i + 1) * g.dx) - (g.xmin + (double)(ox + i) * g.dx));
+        if (i < G::TY) s_id[1][i] = 2.0 / ((g.ymin + (double)(oy + i + 1) * g.dy) - (g.ymin + (double)(oy + i) * g.dy));
+        if (DIM == 3 && i < G::TZ)
+            s_id[DIM - 1][i] = 2.0 / ((g.zmin + (double)(oz + i + 1) * g.dz) - (g.zmin + (double)(oz + i) * g.dz));
+    }
+    zero_acc<DIM, NQ_FORCE>(s_acc);
+    for (int c0 = tb; c0 < te; c0 += CH) {
         int k = c0 + threadIdx.x;
-        if (k < te) {
+        if (threadIdx.x < CH && k < te) {
             int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
@@ -512,62 +580,58 @@ __global__ __launch_bounds__(NT) void k_usf_force(KGrid g, KPart P, KBins B, KNo
             particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
                              DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
             particle_factors<DIM>(g, P, p, xi, S, dS);
-            double L[3][3];
-            velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
-            scale_gradient<DIM>(L, g.strainFirst);
-            const MpmMaterial &m = mats[P.mat[p]];
-            PState s;
-            load_state<DIM>(P, p, m.type, s);
-            constitutive<DIM>(L, m, s);
-            store_state<DIM>(P, p, m.type, s);
 #pragma unroll
-            for (int a = 0; a < DIM; a++) {
+            for (int a = 0; a < DIM; a++)
 #pragma unroll
                 for (int o = 0; o < 4; o++) {
-                    s_S[a][o][threadIdx.x] = S[a][o];
-                    s_dS[a][o][threadIdx.x] = dS[a][o];
+                    s_S[threadIdx.x][a][o] = S[a][o];
+                    s_dS[threadIdx.x][a][o] = dS[a][o];
                 }
-                s_id[a][threadIdx.x] = invd[a];
-            }
-            double Pr = s.P;
-            s_st[0][threadIdx.x] = -P.f[MPF_MASS][p];
-            s_st[1][threadIdx.x] = s.sp[0] - Pr;
-            s_st[2][threadIdx.x] = s.sp[1] - Pr;
-            s_st[3][threadIdx.x] = s.sp[5];
+            double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
+            s_st[threadIdx.x][0] = -P.f[MPF_MASS][p];
+            s_st[threadIdx.x][1] = P.f[MPF_SP_XX][p] - Pr;
+            s_st[threadIdx.x][2] = P.f[MPF_SP_YY][p] - Pr;
+            s_st[threadIdx.x][3] = P.f[MPF_SP_XY][p];
             if (DIM == 3) {
-                s_st[NS - 3][threadIdx.x] = s.sp[2] - Pr;
-                s_st[NS - 2][threadIdx.x] = s.sp[4];
-                s_st[NS - 1][threadIdx.x] = s.sp[3];
+                s_st[threadIdx.x][NS - 3] = P.f[MPF_SP_ZZ][p] - Pr;
+                s_st[threadIdx.x][NS - 2] = P.f[MPF_SP_XZ][p];
+                s_st[threadIdx.x][NS - 1] = P.f[MPF_SP_YZ][p];
             }
         }
         __syncthreads();
-        int c1 = min(c0 + NT, te);
-        node_phase<DIM, SPT>(s_cs, c0, c1, [&](int j, int lq, int ix, int iy, int iz) {
-            double wx = s_S[0][ix][lq], wy = s_S[1][iy][lq];
-            if (wx < 0. || wy < 0.) return;
-            double negmp = s_st[0][lq];
-            if (DIM == 3) {
-                double wz = s_S[DIM - 1][iz][lq];
-                if (wz < 0.) return;
-                double gx = s_dS[0][ix][lq] * wy * wz * s_id[0][lq];
-                double gy = wx * s_dS[1][iy][lq] * wz * s_id[1][lq];
-                double gz = wx * wy * s_dS[DIM - 1][iz][lq] * s_id[DIM - 1][lq];
-                double sxxP = s_st[1][lq], syyP = s_st[2][lq], sxy = s_st[3][lq];
-                double szzP = s_st[NS - 3][lq], sxz = s_st[NS - 2][lq], syz = s_st[NS - 1][lq];
-                acc[j][0] += negmp * (sxxP * gx + sxy * gy + sxz * gz);
-                acc[j][1] += negmp * (sxy * gx + syyP * gy + syz * gz);
-                acc[j][2] += negmp * (sxz * gx + syz * gy + szzP * gz);
-            } else {
-                double gx = s_dS[0][ix][lq] * wy * s_id[0][lq];
-                double gy = wx * s_dS[1][iy][lq] * s_id[1][lq];
-                double sxxP = s_st[1][lq], syyP = s_st[2][lq], sxy = s_st[3][lq];
-                acc[j][0] += negmp * (sxxP * gx + sxy * gy);
-                acc[j][1] += negmp * (sxy * gx + syyP * gy);
-            }
-        });
+        wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
+            [&](int lq, int ix, int iy, int iz) {
+                RecForce r;
+                r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
+                r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
+                if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
+                else { r.wz = 1.; r.dz = 0.; }
+                r.negmp = s_st[lq][0]; r.s0 = s_st[lq][1]; r.s1 = s_st[lq][2]; r.s2 = s_st[lq][3];
+                if (DIM == 3) { r.s3 = s_st[lq][NS - 3]; r.s4 = s_st[lq][NS - 2]; r.s5 = s_st[lq][NS - 1]; }
+                else { r.s3 = r.s4 = r.s5 = 0.; }
+                return r;
+            },
+            [&](const RecForce &r, double *acc, int lx, int ly, int lz) {
+                bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
+                double negmp = on ? r.negmp : 0.;
+                if (DIM == 3) {
+                    double gx = r.dx * r.wy * r.wz * s_id[0][lx];
+                    double gy = r.wx * r.dy * r.wz * s_id[1][ly];
+                    double gz = r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
+                    // s0 = sxx-P, s1 = syy-P, s2 = sxy, s3 = szz-P, s4 = sxz, s5 = syz
+                    acc[0] += negmp * (r.s0 * gx + r.s2 * gy + r.s4 * gz);
+                    acc[1] += negmp * (r.s2 * gx + r.s1 * gy + r.s5 * gz);
+                    acc[2] += negmp * (r.s4 * gx + r.s5 * gy + r.s3 * gz);
+                } else {
+                    double gx = r.dx * r.wy * s_id[0][lx];
+                    double gy = r.wx * r.dy * s_id[1][ly];
+                    acc[0] += negmp * (r.s0 * gx + r.s2 * gy);
+                    acc[1] += negmp * (r.s2 * gx + r.s1 * gy);
+                }
+            });
         __syncthreads();
     }
-    write_partials<DIM, NQ_FORCE, SPT>(N, t, acc);
+    write_partials<DIM, NQ_FORCE>(N, t, s_acc);
 }
 
 // node pass after the forces: ftot, restore p (identity without BCs), p += f dt, v = p/m
@@ -607,11 +671,12 @@ __global__ void k_node_force(KGrid g, KBins B, KNodes N) {
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
-    constexpr int SPT = (G::FOOT + NT - 1) / NT;
+    constexpr int HALF = NT / 2;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_n[7][G::FOOT];         // v(3), f(3), mass
-    __shared__ double s_S[DIM][4][NT];
-    __shared__ double s_q[4][NT];
+    __shared__ double s_S[HALF][DIM][4];
+    __shared__ double s_q[HALF][4];
+    __shared__ double s_acc[NW][NQ_MOM][G::FOOT];
     const int t = blockIdx.x;
     load_tile_starts<DIM>(B, t, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
@@ -624,26 +689,25 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
         const long long st[7] = {0, nn, 2 * nn, 0, nn, 2 * nn, 0};
         load_foot<DIM, 7>(g, ox, oy, oz, src, st, s_n);
     }
+    zero_acc<DIM, NQ_MOM>(s_acc);
     __syncthreads();
-    double acc[SPT][NQ_MOM];
-#pragma unroll
-    for (int j = 0; j < SPT; j++)
-#pragma unroll
-        for (int q = 0; q < NQ_MOM; q++) acc[j][q] = 0.;
     const double dt = g.timestep;
     for (int c0 = tb; c0 < te; c0 += NT) {
-        int k = c0 + threadIdx.x;
-        if (k < te) {
+        const int c1 = min(c0 + NT, te);
+        const int k = c0 + threadIdx.x;
+        double S[3][4], q4[4];
+        if (k < c1) {
             int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
             double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-            double xi[3], invd[3], S[3][4], dS[3][4];
+            double xi[3], invd[3], dS[3][4];
             particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
             particle_factors<DIM>(g, P, p, xi, S, dS);
             int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
             double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
-            const int ncand = DIM == 3 ? 64 : 16;
+            constexpr int ncand = DIM == 3 ? 64 : 16;
+#pragma unroll
             for (int id = 0; id < ncand; id++) {
                 int ax, ay, az;
                 if (DIM == 3) { ax = c_off3x[id] + 1; ay = c_off3y[id] + 1; az = c_off3z[id] + 1; }
@@ -675,34 +739,41 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
                 double delXRate = vm + 0.5 * delV;
                 P.posn[c][p] = pos[c] + delXRate * dt;
             }
-#pragma unroll
-            for (int a = 0; a < DIM; a++)
-#pragma unroll
-                for (int o = 0; o < 4; o++) s_S[a][o][threadIdx.x] = S[a][o];
-            s_q[0][threadIdx.x] = P.f[MPF_MASS][p];
-            s_q[1][threadIdx.x] = vnew[0];
-            s_q[2][threadIdx.x] = vnew[1];
-            s_q[3][threadIdx.x] = vnew[2];
+            q4[0] = P.f[MPF_MASS][p];
+            q4[1] = vnew[0]; q4[2] = vnew[1]; q4[3] = vnew[2];
         }
-        __syncthreads();
-        int c1 = min(c0 + NT, te);
-        node_phase<DIM, SPT>(s_cs, c0, c1, [&](int j, int lq, int ix, int iy, int iz) {
-            double wx = s_S[0][ix][lq], wy = s_S[1][iy][lq];
-            if (wx < 0. || wy < 0.) return;
-            double w;
-            if (DIM == 3) {
-                double wz = s_S[DIM - 1][iz][lq];
-                if (wz < 0.) return;
-                w = wx * wy * wz;
-            } else w = wx * wy;
-            double fnmp = w * s_q[0][lq];
-            acc[j][0] += s_q[1][lq] * fnmp;
-            acc[j][1] += s_q[2][lq] * fnmp;
-            acc[j][2] += s_q[3][lq] * fnmp;
-        });
-        __syncthreads();
+        for (int h = 0; h < 2; h++) {
+            const int r0 = c0 + h * HALF, r1 = min(r0 + HALF, c1);
+            if (r0 >= r1) break;
+            if ((int)(threadIdx.x / HALF) == h && k < c1) {
+                const int i = threadIdx.x - h * HALF;
+#pragma unroll
+                for (int a = 0; a < DIM; a++)
+#pragma unroll
+                    for (int o = 0; o < 4; o++) s_S[i][a][o] = S[a][o];
+#pragma unroll
+                for (int q = 0; q < 4; q++) s_q[i][q] = q4[q];
+            }
+            __syncthreads();
+            wave_cells<DIM, NQ_MOM>(s_cs, r0, r1, s_acc,
+                [&](int lq, int ix, int iy, int iz) {
+                    RecMass r;
+                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                    r.mp = s_q[lq][0]; r.vx = s_q[lq][1]; r.vy = s_q[lq][2]; r.vz = s_q[lq][3];
+                    return r;
+                },
+                [&](const RecMass &r, double *acc, int, int, int) {
+                    bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
+                    double w = DIM == 3 ? r.wx * r.wy * r.wz : r.wx * r.wy;
+                    double fnmp = on ? w * r.mp : 0.;
+                    acc[0] += r.vx * fnmp;
+                    acc[1] += r.vy * fnmp;
+                    acc[2] += r.vz * fnmp;
+                });
+            __syncthreads();
+        }
     }
-    write_partials<DIM, NQ_MOM, SPT>(N, t, acc);
+    write_partials<DIM, NQ_MOM>(N, t, s_acc);
 }
 
 // node pass of the USL re-extrapolation: p (rezeroed, RezeroNodeTask6) and v = p/m
@@ -744,7 +815,7 @@ __device__ __forceinline__ bool pt_in_elem(const KGrid &g, int ei, int ej, int e
 
 // USL strain update (second half step) and ResetElementsTask; also the cell key and histogram
 // of the next binning
-template <int DIM>
+template <int DIM, int MAT>
 __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
                                                   KStats st) {
     using G = TileGeom<DIM>;
@@ -775,10 +846,11 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
         velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
         scale_gradient<DIM>(L, g.strainLast);
         const MpmMaterial &m = mats[P.mat[p]];
+        const int mtype = MAT ? MAT : m.type;
         PState s;
-        load_state<DIM>(P, p, m.type, s);
-        constitutive<DIM>(L, m, s);
-        store_state<DIM>(P, p, m.type, s);
+        load_state<DIM>(P, p, mtype, s);
+        constitutive<DIM, MAT>(L, m, s);
+        store_state<DIM>(P, p, mtype, s);
 
         // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
         double pos[3] = {P.posn[0][p], P.posn[1][p], DIM == 3 ? P.posn[2][p] : 0.};
@@ -889,10 +961,30 @@ void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
     else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
-void launch_usf_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                      const MpmMaterial *mats, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_usf_force<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats);
-    else hipLaunchKernelGGL(k_usf_force<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats);
+template <int DIM>
+static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                    const MpmMaterial *mats, hipStream_t s) {
+    switch (mat) {
+        case MPMGPU_MAT_ISOTROPIC:
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+        case MPMGPU_MAT_ISOPLASTICITY:
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+        case MPMGPU_MAT_NEOHOOKEAN:
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+        default:
+            hipLaunchKernelGGL((k_usf<DIM, 0>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+    }
+}
+
+void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                const MpmMaterial *mats, hipStream_t s) {
+    if (dim == 3) usf_mat<3>(g.matType, g, p, b, nd, mats, s);
+    else usf_mat<2>(g.matType, g, p, b, nd, mats, s);
+}
+
+void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_p2g_force<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+    else hipLaunchKernelGGL(k_p2g_force<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
 }
 
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
@@ -910,8 +1002,23 @@ void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
     else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
+template <int DIM>
+static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                          const MpmMaterial *mats, const KStats &st, hipStream_t s) {
+    switch (mat) {
+        case MPMGPU_MAT_ISOTROPIC:
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+        case MPMGPU_MAT_ISOPLASTICITY:
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+        case MPMGPU_MAT_NEOHOOKEAN:
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+        default:
+            hipLaunchKernelGGL((k_usl_reset<DIM, 0>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+    }
+}
+
 void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                       const MpmMaterial *mats, const KStats &st, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_usl_reset<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st);
-    else hipLaunchKernelGGL(k_usl_reset<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st);
+    if (dim == 3) usl_reset_mat<3>(g.matType, g, p, b, nd, mats, st, s);
+    else usl_reset_mat<2>(g.matType, g, p, b, nd, mats, st, s);
 }

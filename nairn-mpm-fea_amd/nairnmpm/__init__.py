@@ -116,7 +116,7 @@ _SIGNATURES = {
     "mpmgpu_kernel_group_name": ([ctypes.c_int32], ctypes.c_char_p),
 }
 
-KERNEL_GROUPS = ["k_p2g_mass", "k_node_mass", "k_usf_force", "k_node_force", "k_g2p_p2g",
+KERNEL_GROUPS = ["k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_node_force", "k_g2p_p2g",
                  "k_node_mom2", "k_usl_reset", "binning"]
 
 
