@@ -114,6 +114,13 @@ bool material_verify(int analysis, const MaterialInput &in, MpmMaterial &o, std:
             if (!(in.readE || in.readG) || !(in.readNu || (in.readE && in.readG))) {
                 err = "A required material property is missing"; return false;
             }
+            // legality of the elastic constants (Elastic::FillElasticProperties3D, Elastic.cpp:206-217);
+            // isotropic: every v_ij v_ji = nu^2, and v21 v32 v13 = nu^3
+            if (E < 0. || G < 0.) { err = "Modulus less than zero"; return false; }
+            if (nu * nu > 1.) { err = "Illegal Poisson's ratio (v12*v21>1)"; return false; }
+            if (nu * nu * nu > (1. - 3. * nu * nu) / 2.) {
+                err = "Illegal Poisson's ratio (v21*v32*v13>(1-v12*v21-v31*v13-v23*v32)/2)"; return false;
+            }
             if (o.rho <= 0.) { err = "Density less than or equal to zero"; return false; }
             o.E = E; o.nu = nu; o.G = G; o.K = E / (3. * (1 - 2 * nu));
             isotropic_props(analysis, E, nu, G, o.rho, o);
