@@ -40,15 +40,56 @@ ALG_BYTES = {
 }
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_j2_block_128.md")
+
+
+def measured_traffic(kernel, cells):
+    """HBM bytes per launch of `kernel` on this workload, from the committed FETCH_SIZE/WRITE_SIZE
+    counter summary (rows '| kernel | cells | fetch KiB | write KiB | traffic bytes |'); None when
+    the kernel or size was not profiled."""
+    try:
+        with open(TRAFFIC_TABLE) as f:
+            for line in f:
+                cols = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cols) >= 5 and cols[0] == kernel and cols[1] == str(cells):
+                    return float(cols[4])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(nbytes, launches=5):
+    """A plain device copy of known size (torch.neg over float64, nbytes read + nbytes written per
+    launch): the reference point that scales the FETCH_SIZE/WRITE_SIZE counters to bytes."""
+    import torch
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(launches):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / launches
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": launches, "ms_per_launch": dt * 1e3,
+                      "GBps": 2 * n * 8 / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy used to scale the HBM byte counters")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU slab "
               "decomposition is a design, not a built path", file=sys.stderr)
@@ -100,7 +141,7 @@ def main():
     alg_bytes = bp * npart + bn * active_nodes
     achieved = alg_bytes / (dom_ms * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(dom, a.cells), "kernel": dom,
                 "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms}
 
     cpu = None
