@@ -215,6 +215,7 @@ int mpmgpu_problem_material_property(MpmProblem *p, int32_t id, const char *prop
     else if (k == "G" || k == "G1") { m.G = value; m.readG = true; }
     else if (k == "K" || k == "Kbulk") m.K = value;
     else if (k == "Lame") m.Lame = value;
+    else if (k == "G2") m.G2 = value;
     else if (k == "yield") m.yield = value;
     else if (k == "yieldMin") m.yieldMin = value;
     else if (k == "Ep") m.Ep = value;
@@ -309,6 +310,7 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     for (int i = 0; i < nmat; i++) {
         int t = mats[i].type;
         if (t != MPMGPU_MAT_ISOTROPIC && t != MPMGPU_MAT_ISOPLASTICITY && t != MPMGPU_MAT_NEOHOOKEAN &&
+            t != MPMGPU_MAT_MOONEY &&
             t != MPMGPU_MAT_COULOMB && t != MPMGPU_MAT_CONTACT)
             return fail_global("material type " + std::to_string(t) + " is not implemented on the GPU path");
     }

@@ -296,10 +296,10 @@ __device__ __forceinline__ void law_isoplastic(const double de[3][3], const MpmM
     s.h0 = alpint;
 }
 
-// Neohookean with HyperElastic::IncrementDeformation (Neohookean.cpp:177-335,
-// HyperElastic.cpp:104-141,171-207), no residual strains or artificial viscosity
+// HyperElastic::IncrementDeformation (HyperElastic.cpp:104-141): F <- dF.F, B <- dF.B.dF^T
+// with dF = Exponential(1) in 3D and Exponential(2) in 2D; returns det(dF). B is left in s.alt.
 template <int DIM>
-__device__ __forceinline__ void law_neohookean(const double du[3][3], const MpmMaterial &m, PState &s) {
+__device__ __forceinline__ double hyper_increment(const double du[3][3], PState &s) {
     double dF[3][3], pF[3][3], F[3][3], pB[3][3], a[3][3];
     double detDf;
     if (DIM == 3) {
@@ -354,12 +354,30 @@ __device__ __forceinline__ void law_neohookean(const double du[3][3], const MpmM
     double Byy = a[1][0] * dF[1][0] + a[1][1] * dF[1][1] + a[1][2] * dF[1][2];
     double Bzz = a[2][0] * dF[2][0] + a[2][1] * dF[2][1] + a[2][2] * dF[2][2];
     s.alt[0] = Bxx; s.alt[5] = Bxy; s.alt[1] = Byy; s.alt[2] = Bzz;
-    double Bxz = 0., Byz = 0.;
     if (DIM == 3) {
-        Bxz = a[0][0] * dF[2][0] + a[0][1] * dF[2][1] + a[0][2] * dF[2][2];
-        Byz = a[1][0] * dF[2][0] + a[1][1] * dF[2][1] + a[1][2] * dF[2][2];
-        s.alt[4] = Bxz; s.alt[3] = Byz;
+        s.alt[4] = a[0][0] * dF[2][0] + a[0][1] * dF[2][1] + a[0][2] * dF[2][2];
+        s.alt[3] = a[1][0] * dF[2][0] + a[1][1] * dF[2][1] + a[1][2] * dF[2][2];
     }
+    return detDf;
+}
+
+// HyperElastic::GetVolumetricTerms (HyperElastic.cpp:171-207); option numbers from
+// HyperElastic.hpp:21
+__device__ __forceinline__ double volumetric_term(int ujOption, double J, double Kred) {
+    double Kterm;
+    if (ujOption == 1) Kterm = Kred * (J - 1.);
+    else if (ujOption == 2) { double lj = log(J); Kterm = Kred * lj; Kterm /= J; }
+    else Kterm = 0.5 * Kred * (J - 1. / J);
+    return Kterm;
+}
+
+// Neohookean::MPMConstitutiveLaw (Neohookean.cpp:177-335), no residual strains or artificial
+// viscosity; the volumetric term takes the specific Lame constant
+template <int DIM>
+__device__ __forceinline__ void law_neohookean(const double du[3][3], const MpmMaterial &m, PState &s) {
+    double detDf = hyper_increment<DIM>(du, s);
+    double Bxx = s.alt[0], Byy = s.alt[1], Bzz = s.alt[2], Bxy = s.alt[5];
+    double Bxz = DIM == 3 ? s.alt[4] : 0., Byz = DIM == 3 ? s.alt[3] : 0.;
     double Jres = s.h1;
     double dJres = exp(0.);
     Jres *= dJres;
@@ -369,10 +387,7 @@ __device__ __forceinline__ void law_neohookean(const double du[3][3], const MpmM
     double J = detDf * s.h0;
     s.h0 = J;
     double Jeff = J / Jres;
-    double Kterm;
-    if (m.ujOption == 1) Kterm = m.Lamesp * (Jeff - 1.);
-    else if (m.ujOption == 2) { double lj = log(Jeff); Kterm = m.Lamesp * lj; Kterm /= Jeff; }
-    else Kterm = 0.5 * m.Lamesp * (Jeff - 1. / Jeff);
+    double Kterm = volumetric_term(m.ujOption, Jeff, m.Lamesp);
     double Pterm = J * Kterm + Jres * m.Gsp * ((Bxx + Byy + Bzz) / (3. * Jres23) - 1.);
     s.P = -Pterm + 0.;
     double GJeff = resStretch * m.Gsp;
@@ -384,16 +399,56 @@ __device__ __forceinline__ void law_neohookean(const double du[3][3], const MpmM
     if (DIM == 3) { s.sp[4] = GJeff * Bxz; s.sp[3] = GJeff * Byz; }
 }
 
+// Mooney::MPMConstitutiveLaw (Mooney.cpp:185-367), plane strain or 3D, no residual strains or
+// artificial viscosity; the volumetric term takes the specific bulk modulus
+template <int DIM>
+__device__ __forceinline__ void law_mooney(const double du[3][3], const MpmMaterial &m, PState &s) {
+    double detDf = hyper_increment<DIM>(du, s);
+    double Bxx = s.alt[0], Byy = s.alt[1], Bzz = s.alt[2], Bxy = s.alt[5];
+    double Jres = s.h1;
+    double dJres = exp(0.);
+    Jres *= dJres;
+    s.h1 = Jres;
+    double J = detDf * s.h0;
+    s.h0 = J;
+    double Jeff = J / Jres;
+    double Kterm = J * volumetric_term(m.ujOption, Jeff, m.Ksp);
+    s.P = -Kterm + 0.;
+    double J23 = pow(J, 2. / 3.);
+    double J43 = J23 * J23;
+    double JforG1 = J23 / Jres;
+    double JforG2 = J43 / Jres;
+    double sxx = (2 * Bxx - Byy - Bzz) * m.G1sp / (3. * JforG1)
+               + (Bxx * (Byy + Bzz) - 2 * Byy * Bzz - Bxy * Bxy) * m.G2sp / (3. * JforG2);
+    double syy = (2 * Byy - Bxx - Bzz) * m.G1sp / (3. * JforG1)
+               + (Byy * (Bxx + Bzz) - 2 * Bxx * Bzz - Bxy * Bxy) * m.G2sp / (3. * JforG2);
+    double szz = (2 * Bzz - Bxx - Byy) * m.G1sp / (3. * JforG1)
+               + (Bzz * (Bxx + Byy) - 2 * Bxx * Byy + 2. * Bxy * Bxy) * m.G2sp / (3. * JforG2);
+    double sxy = Bxy * m.G1sp / JforG1 + (Bzz * Bxy) * m.G2sp / JforG2;
+    if (DIM == 3) {
+        double Bxz = s.alt[4], Byz = s.alt[3];
+        sxx += (2. * Byz * Byz - Bxz * Bxz) * m.G2sp / (3. * JforG2);
+        syy += (2. * Bxz * Bxz - Byz * Byz) * m.G2sp / (3. * JforG2);
+        szz -= (Bxz * Bxz + Byz * Byz) * m.G2sp / (3. * JforG2);
+        sxy -= Bxz * Byz * m.G2sp / JforG2;
+        s.sp[4] = Bxz * m.G1sp / JforG1 + (Byy * Bxz - Bxy * Byz) * m.G2sp / JforG2;
+        s.sp[3] = Byz * m.G1sp / JforG1 + (Bxx * Byz - Bxy * Bxz) * m.G2sp / JforG2;
+    }
+    s.sp[0] = sxx; s.sp[1] = syy; s.sp[2] = szz; s.sp[5] = sxy;
+}
+
 // MAT = 0 dispatches on the particle's material type; a nonzero MAT compiles one law only
 template <int DIM, int MAT = 0>
 __device__ __forceinline__ void constitutive(const double du[3][3], const MpmMaterial &m, PState &s) {
     if (MAT == MPMGPU_MAT_ISOTROPIC) { law_isotropic<DIM>(du, m, s); return; }
     if (MAT == MPMGPU_MAT_ISOPLASTICITY) { law_isoplastic<DIM>(du, m, s); return; }
     if (MAT == MPMGPU_MAT_NEOHOOKEAN) { law_neohookean<DIM>(du, m, s); return; }
+    if (MAT == MPMGPU_MAT_MOONEY) { law_mooney<DIM>(du, m, s); return; }
     switch (m.type) {
         case MPMGPU_MAT_ISOTROPIC: law_isotropic<DIM>(du, m, s); break;
         case MPMGPU_MAT_ISOPLASTICITY: law_isoplastic<DIM>(du, m, s); break;
         case MPMGPU_MAT_NEOHOOKEAN: law_neohookean<DIM>(du, m, s); break;
+        case MPMGPU_MAT_MOONEY: law_mooney<DIM>(du, m, s); break;
         default: break;
     }
 }

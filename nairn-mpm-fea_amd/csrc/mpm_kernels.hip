@@ -477,7 +477,7 @@ __device__ __forceinline__ void load_state(const KPart &P, int p, int type, PSta
         s.alt[5] = P.f[MPF_ALT_XY][p];
         if (DIM == 3) { s.alt[3] = P.f[MPF_ALT_YZ][p]; s.alt[4] = P.f[MPF_ALT_XZ][p]; }
         s.h0 = P.f[MPF_HIST0][p];
-        if (type == MPMGPU_MAT_NEOHOOKEAN) s.h1 = P.f[MPF_HIST1][p];
+        if (type == MPMGPU_MAT_NEOHOOKEAN || type == MPMGPU_MAT_MOONEY) s.h1 = P.f[MPF_HIST1][p];
     }
 }
 
@@ -498,7 +498,7 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
         P.f[MPF_ALT_XY][p] = s.alt[5];
         if (DIM == 3) { P.f[MPF_ALT_YZ][p] = s.alt[3]; P.f[MPF_ALT_XZ][p] = s.alt[4]; }
         P.f[MPF_HIST0][p] = s.h0;
-        if (type == MPMGPU_MAT_NEOHOOKEAN) P.f[MPF_HIST1][p] = s.h1;
+        if (type == MPMGPU_MAT_NEOHOOKEAN || type == MPMGPU_MAT_MOONEY) P.f[MPF_HIST1][p] = s.h1;
     }
 }
 
@@ -971,6 +971,8 @@ static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, con
             hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_NEOHOOKEAN:
             hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+        case MPMGPU_MAT_MOONEY:
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
         default:
             hipLaunchKernelGGL((k_usf<DIM, 0>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
     }
@@ -1012,6 +1014,8 @@ static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &
             hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_NEOHOOKEAN:
             hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+        case MPMGPU_MAT_MOONEY:
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
         default:
             hipLaunchKernelGGL((k_usl_reset<DIM, 0>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
     }

@@ -16,7 +16,7 @@ struct MaterialInput {
     double rho = 1.;                 // MaterialBase default (Common/Materials/MaterialBase.cpp:49)
     double E = -1., nu = -1.e30, G = -1., K = -1., Lame = -1.;
     double yield = 1.e50, yieldMin = 0., Ep = -1., Khard = 0.;
-    double G1 = -1., G2 = -1.;
+    double G2 = 0.;                  // Mooney::Mooney default (neo-Hookean limit)
     double friction = 0.;
     int ujOption = 0;
     bool readE = false, readNu = false, readG = false;

@@ -173,6 +173,36 @@ bool material_verify(int analysis, const MaterialInput &in, MpmMaterial &o, std:
             o.waveSpeed = sqrt((K + 4. * G / 3.) / o.rho);   // Neohookean::WaveSpeed (Neohookean.cpp:359)
             return true;
         }
+        case MPMGPU_MAT_MOONEY: {
+            // Mooney::VerifyAndLoadProperties (Mooney.cpp:99-150) and
+            // HyperElastic::VerifyAndLoadProperties (HyperElastic.cpp:63-80)
+            double G1 = in.readG ? in.G * 1.e6 : -1., G2 = in.G2 * 1.e6;
+            double K = in.K >= 0. ? in.K * 1.e6 : -1.;
+            double E = in.readE ? in.E * 1.e6 : -1., nu = in.readNu ? in.nu : -2.;
+            if (G1 >= 0. && K >= 0.) {
+                if (E >= 0. || nu >= -1.) { err = "Mooney-Rivlin Hyperelastic material needs K and G1 OR E and nu"; return false; }
+                double Glow = G1 + G2;
+                E = 9. * K * Glow / (3. * K + Glow);
+                nu = (3. * K - 2. * Glow) / (6. * K + 2. * Glow);
+            } else if (G1 >= 0. || K >= 0. || E < 0. || nu < -1.) {
+                err = "Mooney-Rivlin Hyperelastic material needs K and G1 OR E and nu"; return false;
+            } else {
+                if (nu > 0.5) { err = "Mooney-Rivlin Hyperelastic Poisson's ratio must be between -1 and 1/2"; return false; }
+                G1 = E / (2. * (1. + nu)) - G2;
+                if (nu < 0.5) K = E / (3. * (1 - 2. * nu));
+                else { err = "Mooney-Rivlin Hyperelastic Poisson's ratio for solid material must be less than 1/2"; return false; }
+            }
+            if (G2 < 0.) { err = "Mooney-Rivlin Hyperelastic material needs non-negative G2"; return false; }
+            if (in.ujOption < 0 || in.ujOption > 2) { err = "Mooney-Rivlin dilational energy (UJOption) must be 0, 1, or 2"; return false; }
+            if (o.rho <= 0.) { err = "Density less than or equal to zero"; return false; }
+            if (K < 0.) { err = "Bulk modulus cannot be negative"; return false; }
+            o.E = E; o.nu = nu; o.G = G1 + G2; o.K = K;
+            o.G1sp = G1 / o.rho;
+            o.G2sp = G2 / o.rho;
+            o.Ksp = K / o.rho;
+            o.waveSpeed = sqrt((K + 4. * (G1 + G2) / 3.) / o.rho);   // Mooney::WaveSpeed (Mooney.cpp:392-394)
+            return true;
+        }
         case MPMGPU_MAT_COULOMB:
         case MPMGPU_MAT_CONTACT:
             o.friction = in.friction;

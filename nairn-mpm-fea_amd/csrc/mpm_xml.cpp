@@ -397,6 +397,7 @@ struct Reader {
             else if (n == "G" || n == "G1") { in.G = v; in.readG = true; }
             else if (n == "K" || n == "Kbulk") in.K = v;
             else if (n == "Lame") in.Lame = v;
+            else if (n == "G2") in.G2 = v;
             else if (n == "yield") in.yield = v;
             else if (n == "yieldMin") in.yieldMin = v;
             else if (n == "Ep") in.Ep = v;

@@ -115,10 +115,29 @@ def j2_block_3d(cells: int = 128, lo: int = 16, hi: int = 112, vx: float = 50000
 """)
 
 
+def mooney_material_xml(G1: float = 8., G2: float = 2., K: float = 50., rho: float = 1.1,
+                        name: str = "mooney") -> str:
+    """<Material Type="8"> (Mooney-Rivlin) with G1, G2 and K in MPa."""
+    return f"""  <Material Type="8" Name="{name}">
+    <rho>{rho!r}</rho>
+    <G1>{G1!r}</G1>
+    <G2>{G2!r}</G2>
+    <K>{K!r}</K>
+  </Material>
+"""
+
+
 def neohookean_block_3d(cells: int = 32, lo: int = 8, hi: int = 24, vx: float = 20000., G: float = 10.,
-                        K: float = 50., rho: float = 1.1, cfl: float = 0.5) -> str:
-    """3D neo-Hookean block (C3/C5 material) with two colliding halves."""
+                        K: float = 50., rho: float = 1.1, cfl: float = 0.5, material_xml: str | None = None) -> str:
+    """3D neo-Hookean block (C3/C5 material) with two colliding halves; material_xml replaces the
+    material (e.g. mooney_material_xml())."""
     mid = (lo + hi) / 2.0
+    mat = material_xml or f"""  <Material Type="28" Name="rubber">
+    <rho>{rho!r}</rho>
+    <G>{G!r}</G>
+    <K>{K!r}</K>
+  </Material>
+"""
     return (_header(12, cfl, 1.0, "    <MatlPtsPerElement>8</MatlPtsPerElement>\n") +
             f"""  <Mesh output="file">
     <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}" zmin="0" zmax="{cells}">
@@ -135,12 +154,7 @@ def neohookean_block_3d(cells: int = 32, lo: int = 8, hi: int = 24, vx: float = 
       <Box xmin="{mid!r}" xmax="{hi}" ymin="{lo}" ymax="{hi}" zmin="{lo}" zmax="{hi}"/>
     </Body>
   </MaterialPoints>
-  <Material Type="28" Name="rubber">
-    <rho>{rho!r}</rho>
-    <G>{G!r}</G>
-    <K>{K!r}</K>
-  </Material>
-</JANFEAInput>
+""" + mat + """</JANFEAInput>
 """)
 
 

@@ -490,6 +490,7 @@ struct Oracle {
             case MPMGPU_MAT_ISOTROPIC: isotropic(p, du, m); break;
             case MPMGPU_MAT_ISOPLASTICITY: isoplastic(p, du, m); break;
             case MPMGPU_MAT_NEOHOOKEAN: neohookean(p, du, m); break;
+            case MPMGPU_MAT_MOONEY: mooney(p, du, m); break;
             default: break;
         }
     }
@@ -650,10 +651,9 @@ struct Oracle {
         F(MPF_HIST0, p) = alpint;                            // UpdatePlasticInternal (:136-139)
     }
 
-    // Neohookean::MPMConstitutiveLaw (Neohookean.cpp:177-335) with
-    // HyperElastic::IncrementDeformation (HyperElastic.cpp:104-141), default U(J),
-    // no residual strains, no artificial viscosity, plane strain or 3D
-    void neohookean(long p, const Mat3 &du, const MpmMaterial &m) {
+    // HyperElastic::IncrementDeformation (HyperElastic.cpp:104-141): F <- dF.F and the left
+    // Cauchy-Green tensor B <- dF.B.dF^T kept in the alt strain; returns det(dF)
+    double hyper_increment(long p, const Mat3 &du) {
         bool is3 = g.dim == 3;
         Mat3 dF = du.exponential(is3 ? 1 : 2);     // incrementalDefGradTerms (StartOutput.cpp:116-119)
         Mat3 pF = get_F(p);
@@ -665,18 +665,33 @@ struct Oracle {
         Mat3 dFoldB = dF.mul(pBold);
         const double (*a)[3] = dFoldB.m;
         const double (*f)[3] = dF.m;
-        double Bxx = a[0][0] * f[0][0] + a[0][1] * f[0][1] + a[0][2] * f[0][2];
-        double Bxy = a[0][0] * f[1][0] + a[0][1] * f[1][1] + a[0][2] * f[1][2];
-        double Byy = a[1][0] * f[1][0] + a[1][1] * f[1][1] + a[1][2] * f[1][2];
-        double Bzz = a[2][0] * f[2][0] + a[2][1] * f[2][1] + a[2][2] * f[2][2];
-        F(MPF_ALT_XX, p) = Bxx; F(MPF_ALT_XY, p) = Bxy; F(MPF_ALT_YY, p) = Byy; F(MPF_ALT_ZZ, p) = Bzz;
-        double Bxz = 0., Byz = 0.;
+        F(MPF_ALT_XX, p) = a[0][0] * f[0][0] + a[0][1] * f[0][1] + a[0][2] * f[0][2];
+        F(MPF_ALT_XY, p) = a[0][0] * f[1][0] + a[0][1] * f[1][1] + a[0][2] * f[1][2];
+        F(MPF_ALT_YY, p) = a[1][0] * f[1][0] + a[1][1] * f[1][1] + a[1][2] * f[1][2];
+        F(MPF_ALT_ZZ, p) = a[2][0] * f[2][0] + a[2][1] * f[2][1] + a[2][2] * f[2][2];
         if (is3) {
-            Bxz = a[0][0] * f[2][0] + a[0][1] * f[2][1] + a[0][2] * f[2][2];
-            Byz = a[1][0] * f[2][0] + a[1][1] * f[2][1] + a[1][2] * f[2][2];
-            F(MPF_ALT_XZ, p) = Bxz; F(MPF_ALT_YZ, p) = Byz;
+            F(MPF_ALT_XZ, p) = a[0][0] * f[2][0] + a[0][1] * f[2][1] + a[0][2] * f[2][2];
+            F(MPF_ALT_YZ, p) = a[1][0] * f[2][0] + a[1][1] * f[2][1] + a[1][2] * f[2][2];
         }
-        double detDf = dF.determinant();
+        return dF.determinant();
+    }
+
+    // HyperElastic::GetVolumetricTerms (HyperElastic.cpp:171-207); option numbers from HyperElastic.hpp:21
+    static double volumetric_term(int ujOption, double J, double Kred) {
+        double Kterm;
+        if (ujOption == 1) Kterm = Kred * (J - 1.);                                  // J_MINUS_1_SQUARED
+        else if (ujOption == 2) { double lj = log(J); Kterm = Kred * lj; Kterm /= J; }   // LN_J_SQUARED
+        else Kterm = 0.5 * Kred * (J - 1. / J);                                      // default (0)
+        return Kterm;
+    }
+
+    // Neohookean::MPMConstitutiveLaw (Neohookean.cpp:177-335), default U(J) options, no
+    // residual strains, no artificial viscosity, plane strain or 3D
+    void neohookean(long p, const Mat3 &du, const MpmMaterial &m) {
+        bool is3 = g.dim == 3;
+        double detDf = hyper_increment(p, du);
+        double Bxx = F(MPF_ALT_XX, p), Byy = F(MPF_ALT_YY, p), Bzz = F(MPF_ALT_ZZ, p), Bxy = F(MPF_ALT_XY, p);
+        double Bxz = is3 ? F(MPF_ALT_XZ, p) : 0., Byz = is3 ? F(MPF_ALT_YZ, p) : 0.;
         double Jres = F(MPF_HIST1, p);
         double dJres = exp(0.);                        // GetIncrementalResJ (HyperElastic.cpp:89-95)
         Jres *= dJres;
@@ -686,11 +701,7 @@ struct Oracle {
         double J = detDf * F(MPF_HIST0, p);
         F(MPF_HIST0, p) = J;
         double Jeff = J / Jres;
-        // GetVolumetricTerms (HyperElastic.cpp:171-207); option numbers from HyperElastic.hpp:21
-        double Kterm;
-        if (m.ujOption == 1) Kterm = m.Lamesp * (Jeff - 1.);                        // J_MINUS_1_SQUARED
-        else if (m.ujOption == 2) { double lj = log(Jeff); Kterm = m.Lamesp * lj; Kterm /= Jeff; }
-        else Kterm = 0.5 * m.Lamesp * (Jeff - 1. / Jeff);                           // default (0)
+        double Kterm = volumetric_term(m.ujOption, Jeff, m.Lamesp);
         double Pterm = J * Kterm + Jres * m.Gsp * ((Bxx + Byy + Bzz) / (3. * Jres23) - 1.);
         F(MPF_PRESSURE, p) = -Pterm + 0.;
         double GJeff = resStretch * m.Gsp;
@@ -700,6 +711,44 @@ struct Oracle {
         F(MPF_SP_ZZ, p) = GJeff * (Bzz - I1third);
         F(MPF_SP_XY, p) = GJeff * Bxy;
         if (is3) { F(MPF_SP_XZ, p) = GJeff * Bxz; F(MPF_SP_YZ, p) = GJeff * Byz; }
+    }
+
+    // Mooney::MPMConstitutiveLaw (Mooney.cpp:185-367), plane strain or 3D, no residual strains,
+    // no artificial viscosity (QAVred = 0)
+    void mooney(long p, const Mat3 &du, const MpmMaterial &m) {
+        bool is3 = g.dim == 3;
+        double detDf = hyper_increment(p, du);
+        double Bxx = F(MPF_ALT_XX, p), Byy = F(MPF_ALT_YY, p), Bzz = F(MPF_ALT_ZZ, p), Bxy = F(MPF_ALT_XY, p);
+        double Jres = F(MPF_HIST1, p);
+        double dJres = exp(0.);
+        Jres *= dJres;
+        F(MPF_HIST1, p) = Jres;
+        double J = detDf * F(MPF_HIST0, p);
+        F(MPF_HIST0, p) = J;
+        double Jeff = J / Jres;
+        double Kterm = J * volumetric_term(m.ujOption, Jeff, m.Ksp);   // Kirchhoff pressure
+        F(MPF_PRESSURE, p) = -Kterm + 0.;
+        double J23 = pow(J, 2. / 3.);
+        double J43 = J23 * J23;
+        double JforG1 = J23 / Jres;
+        double JforG2 = J43 / Jres;
+        double sxx = (2 * Bxx - Byy - Bzz) * m.G1sp / (3. * JforG1)
+                   + (Bxx * (Byy + Bzz) - 2 * Byy * Bzz - Bxy * Bxy) * m.G2sp / (3. * JforG2);
+        double syy = (2 * Byy - Bxx - Bzz) * m.G1sp / (3. * JforG1)
+                   + (Byy * (Bxx + Bzz) - 2 * Bxx * Bzz - Bxy * Bxy) * m.G2sp / (3. * JforG2);
+        double szz = (2 * Bzz - Bxx - Byy) * m.G1sp / (3. * JforG1)
+                   + (Bzz * (Bxx + Byy) - 2 * Bxx * Byy + 2. * Bxy * Bxy) * m.G2sp / (3. * JforG2);
+        double sxy = Bxy * m.G1sp / JforG1 + (Bzz * Bxy) * m.G2sp / JforG2;
+        if (is3) {
+            double Bxz = F(MPF_ALT_XZ, p), Byz = F(MPF_ALT_YZ, p);
+            sxx += (2. * Byz * Byz - Bxz * Bxz) * m.G2sp / (3. * JforG2);
+            syy += (2. * Bxz * Bxz - Byz * Byz) * m.G2sp / (3. * JforG2);
+            szz -= (Bxz * Bxz + Byz * Byz) * m.G2sp / (3. * JforG2);
+            sxy -= Bxz * Byz * m.G2sp / JforG2;
+            F(MPF_SP_XZ, p) = Bxz * m.G1sp / JforG1 + (Byy * Bxz - Bxy * Byz) * m.G2sp / JforG2;
+            F(MPF_SP_YZ, p) = Byz * m.G1sp / JforG1 + (Bxx * Byz - Bxy * Bxz) * m.G2sp / JforG2;
+        }
+        F(MPF_SP_XX, p) = sxx; F(MPF_SP_YY, p) = syy; F(MPF_SP_ZZ, p) = szz; F(MPF_SP_XY, p) = sxy;
     }
 
     // GridForcesTask::Execute (GridForcesTask.cpp:40-153) -> GetFintPlusFext

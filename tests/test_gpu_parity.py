@@ -59,6 +59,24 @@ def test_neohookean_3d_block():
         _check(sim, ora, 3, "NH 3D")
 
 
+def test_mooney_3d_block():
+    xml = configs.neohookean_block_3d(cells=20, lo=4, hi=16, material_xml=configs.mooney_material_xml())
+    prob, sim, ora = _pair(xml)
+    for n in (1, 29):
+        sim.step(n)
+        ora.step(n)
+        _check(sim, ora, 3, "Mooney 3D")
+
+
+def test_mooney_2d_block():
+    mat = configs.mooney_material_xml(G1=300., G2=100., K=2000., rho=1.0)
+    prob, sim, ora = _pair(configs.elastic_block_2d(cells=32, block=16, material_xml=mat))
+    for n in (1, 99):
+        sim.step(n)
+        ora.step(n)
+        _check(sim, ora, 2, "Mooney 2D")
+
+
 def test_j2_2d_block():
     prob, sim, ora = _pair(configs.j2_block_2d())
     for n in (1, 99, 200):

@@ -155,6 +155,67 @@ def test_j2_hydrostatic_never_yields():
     assert max(abs(s[F[c]]) for c in ("sp_xx", "sp_yy", "sp_zz", "sp_xy")) < 1e-3 * abs(s[F["pressure"]])
 
 
+def mooney(G1=8., G2=2., K=50., rho=1.1, analysis=nm.THREED, **kw):
+    return material(analysis, nm.MAT_MOONEY, G1=G1, G2=G2, K=K, rho=rho, **kw)
+
+
+def test_mooney_constants():
+    m = mooney()
+    np.testing.assert_allclose(m.G1sp, 8e6 / 1.1e-3, rtol=1e-14)
+    np.testing.assert_allclose(m.G2sp, 2e6 / 1.1e-3, rtol=1e-14)
+    np.testing.assert_allclose(m.Ksp, 50e6 / 1.1e-3, rtol=1e-14)
+    np.testing.assert_allclose(m.waveSpeed, np.sqrt((50e6 + 4 * 10e6 / 3) / 1.1e-3), rtol=1e-14)
+    # E and nu input: G1 + G2 = E / (2 (1 + nu))
+    m2 = material(nm.THREED, nm.MAT_MOONEY, E=30., nu=0.4, G2=3., rho=1.)
+    np.testing.assert_allclose(m2.G1sp + m2.G2sp, 30e6 / 2.8 / 1e-3, rtol=1e-13)
+
+
+@pytest.mark.parametrize("analysis", [nm.THREED, nm.PLANE_STRAIN])
+def test_mooney_simple_shear(analysis):
+    """Mooney-Rivlin simple shear: sigma_xy = (G1 + G2) gamma at J = 1 (Kirchhoff/rho0), and the
+    normal stress differences N1 = (G1 + G2) gamma^2, N2 = -G2 gamma^2 (Rivlin's universal results)."""
+    G1, G2, r = 8e6, 2e6, 1.1e-3
+    m = mooney(analysis=analysis)
+    gam = 0.05
+    du = np.zeros((3, 3))
+    du[0, 1] = gam
+    s = orc.constitutive(analysis, m, du, state0(nm.MAT_NEOHOOKEAN))
+    np.testing.assert_allclose(s[F["hist0"]], 1.0, rtol=1e-15)
+    np.testing.assert_allclose(s[F["sp_xy"]], (G1 + G2) * gam / r, rtol=1e-13)
+    np.testing.assert_allclose(s[F["sp_xx"]] - s[F["sp_yy"]], (G1 + G2) * gam ** 2 / r, rtol=1e-12)
+    np.testing.assert_allclose(s[F["sp_yy"]] - s[F["sp_zz"]], -G2 * gam ** 2 / r, rtol=1e-12)
+    assert abs(s[F["sp_xx"]] + s[F["sp_yy"]] + s[F["sp_zz"]]) < 1e-9 * abs(s[F["sp_xy"]])
+
+
+@pytest.mark.parametrize("uj", [0, 1, 2])
+def test_mooney_hydrostatic(uj):
+    """Uniform stretch: deviatoric stress vanishes and P = -J U'(J) with the bulk modulus."""
+    K, r = 50e6, 1.1e-3
+    m = mooney(UJOption=uj)
+    e = 0.02
+    s = orc.constitutive(nm.THREED, m, e * np.eye(3), state0(nm.MAT_NEOHOOKEAN))
+    J = (1 + e) ** 3
+    dU = {0: 0.5 * K * (J - 1 / J), 1: K * (J - 1), 2: K * np.log(J) / J}[uj]
+    np.testing.assert_allclose(s[F["pressure"]], -J * dU / r, rtol=1e-12)
+    assert max(abs(s[F[c]]) for c in ("sp_xx", "sp_yy", "sp_zz", "sp_xy")) < 1e-9 * abs(s[F["pressure"]])
+
+
+def test_mooney_with_g2_zero_is_neohookean_deviator():
+    """G2 = 0 reduces the Mooney deviator to G1 dev(B) / J^(2/3) (neo-Hookean, Kirchhoff/rho0)."""
+    m = mooney(G2=0.)
+    rng = np.random.default_rng(3)
+    du = 0.02 * rng.standard_normal((3, 3))
+    s = orc.constitutive(nm.THREED, m, du, state0(nm.MAT_NEOHOOKEAN))
+    Fm = np.eye(3) + du
+    B = Fm @ Fm.T
+    J = np.linalg.det(Fm)
+    dev = (B - np.trace(B) / 3 * np.eye(3)) * 8e6 / 1.1e-3 / J ** (2. / 3.)
+    got = np.array([[s[F["sp_xx"]], s[F["sp_xy"]], s[F["sp_xz"]]],
+                    [s[F["sp_xy"]], s[F["sp_yy"]], s[F["sp_yz"]]],
+                    [s[F["sp_xz"]], s[F["sp_yz"]], s[F["sp_zz"]]]])
+    np.testing.assert_allclose(got, dev, rtol=1e-11, atol=1e-11 * np.abs(dev).max())
+
+
 def nh():
     return material(nm.THREED, nm.MAT_NEOHOOKEAN, G=10., K=50., rho=1.1)
 
