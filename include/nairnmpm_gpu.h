@@ -91,7 +91,8 @@ typedef struct MpmMaterial {
     double Gsp, Lamesp, Ksp;   /* Neohookean specific (Neohookean.cpp:135-138) */
     double G1sp, G2sp;         /* Mooney specific */
     double waveSpeed;          /* mm/s, used for the CFL time step */
-    double friction;           /* CoulombFriction coefficient (contact laws only) */
+    double friction;           /* CoulombFriction frictionCoeff (contact laws only; <0 stick) */
+    double frictionStatic;     /* CoulombFriction frictionCoeffStatic (ignored if <= 0) */
     int32_t ujOption;          /* HyperElastic UofJOption */
     int32_t pad;
 } MpmMaterial;
@@ -105,10 +106,10 @@ typedef struct MpmRunParams {
     int32_t mpmMethod;         /* 2 = USAVG (only method of the hot path) */
     int32_t multiMaterial;     /* 0 single-material mode */
     int32_t contactLaw;        /* material index of the default contact law, -1 none */
-    int32_t normalsMethod;     /* MultiMaterialMode Normals= */
+    int32_t normalsMethod;     /* MultiMaterialMode Normals= (0 MAXG, 1 MAXV, 2 AVGG, 3 OWNG) */
     int32_t deleteLeaving;     /* fmobj->deleteLeavingParticles */
     int32_t maxLeaving;        /* warnings limit before abort (0 = never abort) */
-    int32_t pad;
+    int32_t numFields;         /* material velocity fields (maxMaterialFields); 1 unless multimaterial */
 } MpmRunParams;
 
 /* Counters reported back from the device (ResetElementsTask.cpp:71-151 warnings) */
@@ -147,6 +148,7 @@ enum {
     MPMGPU_KG_NODE_MOM2,      /* USL grid velocity */
     MPMGPU_KG_USL_RESET,      /* USL strain update + ResetElementsTask */
     MPMGPU_KG_BINNING,        /* cell sort of the particles for the next step */
+    MPMGPU_KG_CONTACT,        /* multimaterial contact extrapolations and contact passes */
     MPMGPU_NUM_KERNEL_GROUPS
 };
 
@@ -213,8 +215,14 @@ int mpmgpu_sync(MpmCtx *ctx);
 int mpmgpu_download_particles(MpmCtx *ctx, double *dfields, int32_t *ifields);
 /* Counters and timings (MPMTask::TrackTimes, MPMTask.cpp:51-72; warnings). */
 int mpmgpu_query(MpmCtx *ctx, MpmStats *stats);
-/* Grid node values (mass, momentum, force) after the last task, nodes numbered x fastest. */
+/* Grid node values (mass, momentum, force) after the last task, nodes numbered x fastest; one
+   block per material velocity field (numFields of them, field-major): mass[f*nn + n],
+   pk/ftot[(3*f + c)*nn + n]. */
 int mpmgpu_download_grid(MpmCtx *ctx, double *mass, double *pk, double *ftot);
+/* Multimaterial contact extrapolations of every field (same layout): volume, mass-weighted
+   displacement, volume gradient (MatVelocityField::contactInfo, MatVelocityField.hpp:40-124).
+   Fails in single-material mode. */
+int mpmgpu_download_contact(MpmCtx *ctx, double *volume, double *disp, double *grad);
 /* Per-kernel-group HIP event timing on the context's stream (MPMTask::TrackTimes,
    MPMTask.cpp:51-72). Turning it on or off clears the totals; mpmgpu_step then launches
    eagerly (no graph replay) with an event between kernel groups. */

@@ -104,20 +104,37 @@ static void enqueue_step(MpmCtx *c) {
         c->evMarks.push_back({group, {e, e2}});
         e = e2;
     };
+    const bool mm = c->kg.nfields > 1;
     launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
     done(MPMGPU_KG_P2G_MASS);
     launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
     done(MPMGPU_KG_NODE_MASS);
+    if (mm) {      // PostExtrapolationTask: contact at MASS_MOMENTUM_CALL
+        launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, false, s);
+        launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+        launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
+        done(MPMGPU_KG_CONTACT);
+    }
     launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
     done(MPMGPU_KG_USF);
     launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s);
     done(MPMGPU_KG_P2G_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
     done(MPMGPU_KG_NODE_FORCE);
+    if (mm) {      // UpdateMomentaTask: contact at UPDATE_MOMENTUM_CALL
+        launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
+        done(MPMGPU_KG_CONTACT);
+    }
     launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s);
     done(MPMGPU_KG_G2P_P2G);
     launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
     done(MPMGPU_KG_NODE_MOM2);
+    if (mm) {      // UpdateStrainsLastContactTask: contact at UPDATE_STRAINS_LAST_CALL
+        launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, true, s);
+        launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+        launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
+        done(MPMGPU_KG_CONTACT);
+    }
     launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
     done(MPMGPU_KG_USL_RESET);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, s);
@@ -188,7 +205,13 @@ int mpmgpu_problem_set(MpmProblem *p, const char *key, double value) {
     else if (k == "TimeStep") p->timestepIn = value;
     else if (k == "MaxTime") p->maxTime = value;
     else if (k == "GIMP") p->run.shape = (int)value;
+    else if (k == "MultiMaterialMode") {       // value = Normals method
+        p->mmRequested = true;
+        p->normalsGiven = true;
+        p->run.normalsMethod = (int)value;
+    } else if (k == "ContactLaw") p->contactLawNum = (int)value;   // 1-based material id
     else return fail_global("unknown setting " + k);
+    p->finalized = false;
     return 0;
 }
 
@@ -221,6 +244,7 @@ int mpmgpu_problem_material_property(MpmProblem *p, int32_t id, const char *prop
     else if (k == "Ep") m.Ep = value;
     else if (k == "Khard") m.Khard = value;
     else if (k == "coeff") m.friction = value;
+    else if (k == "coeffStatic") m.frictionStatic = value;
     else if (k == "UJOption") m.ujOption = (int)value;
     else if (k == "alpha" || k == "beta" || k == "Cv" || k == "kCond" || k == "D" || k == "csat") {
         if (value != 0. && (k == "alpha" || k == "beta")) {
@@ -305,7 +329,15 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     if (run->shape != MPMGPU_SHAPE_UGIMP)
         return fail_global("only uGIMP shape functions are implemented on the GPU path");
     if (run->mpmMethod != 2) return fail_global("only the USAVG method is implemented");
-    if (run->multiMaterial) return fail_global("multimaterial mode is not implemented on the GPU path yet");
+    int nf = run->multiMaterial ? run->numFields : 1;
+    if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return fail_global("bad number of material velocity fields");
+    if (nf > 1) {
+        if (run->contactLaw < 0 || run->contactLaw >= nmat ||
+            (mats[run->contactLaw].type != MPMGPU_MAT_COULOMB && mats[run->contactLaw].type != MPMGPU_MAT_CONTACT))
+            return fail_global("multimaterial mode needs a contact law material");
+        if (run->normalsMethod < 0 || run->normalsMethod > 3)
+            return fail_global("contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG)");
+    }
     if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return fail_global("bad material count");
     for (int i = 0; i < nmat; i++) {
         int t = mats[i].type;
@@ -332,36 +364,55 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     g.xmin = grid->xmin; g.ymin = grid->ymin; g.zmin = grid->zmin;
     g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
     g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
+    g.nfields = nf;
+    for (int m = 0; m < MPMGPU_MAX_MATERIALS; m++)
+        g.fieldOf[m] = (nf > 1 && m < nmat && mats[m].field >= 0 && mats[m].field < nf) ? mats[m].field : 0;
+    if (nf > 1) {
+        const MpmMaterial &law = mats[run->contactLaw];
+        g.normals = run->normalsMethod;
+        g.ignoreContact = law.type == MPMGPU_MAT_CONTACT ? 1 : 0;
+        g.friction = law.friction;
+        g.frictionStatic = law.frictionStatic;
+    }
     int foot = c->dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
     long long nn = g.nnodes;
-    c->kb.nkeys = g.ntiles * nc;
+    c->kb.nkeys = nf * g.ntiles * nc;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
         std::string m = std::string("hipStreamCreate: ") + hipGetErrorString(e);
         delete c;
         return fail_global(m);
     }
-    size_t nodeBytes = (size_t)nn * (1 + 3 + 3 + 3) * sizeof(double);
+    // per field: mass, pk, vk, ftot (+ contact volume, displacement, gradient in multimaterial mode)
+    const int perField = 1 + 3 + 3 + 3 + (nf > 1 ? 1 + 3 + 3 : 0);
+    size_t nodeBytes = (size_t)nf * nn * perField * sizeof(double);
     double *nodeMem = nullptr;
     int *cntMem = nullptr;
     double *partMem = nullptr;
-    if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nn * sizeof(int)) != hipSuccess ||
-        hipMalloc(&partMem, (size_t)g.ntiles * NQMAX * foot * sizeof(double)) != hipSuccess ||
+    if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nf * nn * sizeof(int)) != hipSuccess ||
+        hipMalloc(&partMem, (size_t)nf * g.ntiles * NQMAX * foot * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dmats, nmat * sizeof(MpmMaterial)) != hipSuccess ||
         hipMalloc(&c->sbuf, 2 * sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return fail_global("out of device memory allocating the grid");
     }
+    long long fnn = (long long)nf * nn;
     c->kn.mass = nodeMem;
-    c->kn.pk = nodeMem + nn;
-    c->kn.vk = nodeMem + 4 * nn;
-    c->kn.ftot = nodeMem + 7 * nn;
+    c->kn.pk = nodeMem + fnn;
+    c->kn.vk = nodeMem + 4 * fnn;
+    c->kn.ftot = nodeMem + 7 * fnn;
     c->kn.cnt = cntMem;
     c->kn.part = partMem;
+    c->kn.cvol = c->kn.cdisp = c->kn.cgrad = nullptr;
+    if (nf > 1) {
+        c->kn.cvol = nodeMem + 10 * fnn;
+        c->kn.cdisp = nodeMem + 11 * fnn;
+        c->kn.cgrad = nodeMem + 14 * fnn;
+    }
     c->ks.leftGrid = c->sbuf;
     c->ks.nanCount = c->sbuf + 1;
     (void)hipMemsetAsync(nodeMem, 0, nodeBytes, c->stream);
-    (void)hipMemsetAsync(cntMem, 0, (size_t)nn * sizeof(int), c->stream);
+    (void)hipMemsetAsync(cntMem, 0, (size_t)nf * nn * sizeof(int), c->stream);
     (void)hipMemsetAsync(c->sbuf, 0, 2 * sizeof(unsigned long long), c->stream);
     (void)hipMemcpyAsync(c->dmats, mats, nmat * sizeof(MpmMaterial), hipMemcpyHostToDevice, c->stream);
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums
@@ -419,6 +470,8 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         if (m < 0 || m >= nmat) return c->fail("particle with an undefined material");
         int t = c->mats[m].type;
         if (t == MPMGPU_MAT_COULOMB || t == MPMGPU_MAT_CONTACT) return c->fail("particle with a contact-law material");
+        if (c->kg.nfields > 1 && (c->mats[m].field < 0 || c->mats[m].field >= c->kg.nfields))
+            return c->fail("particle material without a velocity field");
         if (e < 0 || e >= nelem) return c->fail("particle element outside the grid");
         if (!(dfields[MPF_MASS * nn + i] > 0.)) return c->fail("particle with a mass that is not positive");
     }
@@ -440,6 +493,9 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
     }
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // the second position buffer starts as a copy: in 2D its z row is never written by a step
+    for (int a = 0; a < 3; a++)
+        HIPCHECK(c, hipMemcpyAsync(c->posB[a], c->posA[a], nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 2 * sizeof(unsigned long long), c->stream));
     set_positions(c);
     launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
@@ -463,14 +519,28 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
             launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
             break;
-        case MPMGPU_TASK_POST_EXTRAPOLATION: break;   // done by the node pass of the scatter
+        case MPMGPU_TASK_POST_EXTRAPOLATION:          // node pass of the scatter did the rest
+            if (c->kg.nfields > 1) {
+                launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, false, s);
+                launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+                launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
+            }
+            break;
         case MPMGPU_TASK_UPDATE_STRAINS_FIRST: launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
         case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
         case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
-        case MPMGPU_TASK_UPDATE_MOMENTA: launch_node_force(c->dim, c->kg, c->kb, c->kn, s); break;
+        case MPMGPU_TASK_UPDATE_MOMENTA:
+            launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
+            if (c->kg.nfields > 1) launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
+            break;
         case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
         case MPMGPU_TASK_UPDATE_STRAINS_LAST:
             launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
+            if (c->kg.nfields > 1) {
+                launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, true, s);
+                launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+                launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
+            }
             launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
             break;
         case MPMGPU_TASK_RESET_ELEMENTS:
@@ -533,9 +603,19 @@ int mpmgpu_download_particles(MpmCtx *c, double *dfields, int32_t *ifields) {
     return 0;
 }
 
+int mpmgpu_download_contact(MpmCtx *c, double *vol, double *disp, double *grad) {
+    if (c->kg.nfields < 2) return c->fail("no contact extrapolations in single-material mode");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    size_t nn = (size_t)c->kg.nnodes * c->kg.nfields;
+    if (vol) HIPCHECK(c, hipMemcpy(vol, c->kn.cvol, nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (disp) HIPCHECK(c, hipMemcpy(disp, c->kn.cdisp, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (grad) HIPCHECK(c, hipMemcpy(grad, c->kn.cgrad, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int mpmgpu_download_grid(MpmCtx *c, double *mass, double *pk, double *ftot) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    size_t nn = (size_t)c->kg.nnodes;
+    size_t nn = (size_t)c->kg.nnodes * c->kg.nfields;
     if (mass) HIPCHECK(c, hipMemcpy(mass, c->kn.mass, nn * sizeof(double), hipMemcpyDeviceToHost));
     if (pk) HIPCHECK(c, hipMemcpy(pk, c->kn.pk, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
     if (ftot) HIPCHECK(c, hipMemcpy(ftot, c->kn.ftot, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
@@ -576,7 +656,7 @@ int mpmgpu_kernel_times(MpmCtx *c, double *totalMs, int64_t *launches) {
 const char *mpmgpu_kernel_group_name(int32_t g) {
     static const char *names[MPMGPU_NUM_KERNEL_GROUPS] = {
         "k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_node_force",
-        "k_g2p_p2g", "k_node_mom2", "k_usl_reset", "binning"};
+        "k_g2p_p2g", "k_node_mom2", "k_usl_reset", "binning", "contact"};
     return g >= 0 && g < MPMGPU_NUM_KERNEL_GROUPS ? names[g] : "";
 }
 

@@ -34,6 +34,13 @@ struct KGrid {
     int uniformLp;             // 1 when every particle has the same lp (then lp[] is used)
     int matType;               // the one material type of all particles, 0 when mixed
     double lp[3];
+    // material velocity fields (multimaterial mode when nfields > 1)
+    int nfields;
+    int fieldOf[MPMGPU_MAX_MATERIALS];   // material -> field
+    // contact (multimaterial mode): normals method, law (CoulombFriction or stick-to-center-of-mass)
+    int normals;
+    int ignoreContact;         // ContactLaw type 60: always move to the center-of-mass velocity
+    double friction, frictionStatic;
 };
 
 struct KPart {
@@ -59,15 +66,24 @@ struct KNodes {
     double *vk;                // 3*nnodes
     double *ftot;              // 3*nnodes
     int *cnt;                  // nnodes: particle-node pairs (numberPoints)
-    double *part;              // per-tile partial sums, ntiles * NQMAX * FOOT
+    double *part;              // per-(field, tile) partial sums, nfields * ntiles * NQMAX * FOOT
+    // multimaterial contact extrapolations: volume, mass-weighted displacement, volume gradient
+    double *cvol;              // nnodes
+    double *cdisp;             // 3*nnodes
+    double *cgrad;             // 3*nnodes
 };
+// Node arrays hold nfields consecutive copies (field f at offset f*nnodes for scalars and
+// f*3*nnodes for vectors); a kernel working on one field takes a view shifted to it.
 
 struct KStats {
     unsigned long long *leftGrid;
     unsigned long long *nanCount;
 };
 
-enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQMAX = 4 };
+enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
+
+// contact call types (UpdateMomentaTask::ContactAndMomentaBCs passType)
+enum { CONTACT_MASS_MOMENTUM = 0, CONTACT_UPDATE_MOMENTUM = 1, CONTACT_UPDATE_STRAINS_LAST = 2 };
 
 // host-side launchers (mpm_kernels.hip); all take the context's stream
 void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
@@ -84,3 +100,9 @@ void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, con
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                       const MpmMaterial *mats, const KStats &st, hipStream_t s);
+// multimaterial mode: contact extrapolations (displacements from the new positions when
+// newPositions is set) and the contact pass on nodes with two or more materials
+void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                        const MpmMaterial *mats, bool newPositions, hipStream_t s);
+void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
+void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s);

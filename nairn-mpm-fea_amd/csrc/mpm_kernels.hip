@@ -36,13 +36,32 @@ __device__ __forceinline__ void tile_origin(const KGrid &g, int t, int &ox, int 
     ox = tx * G::TX; oy = ty * G::TY; oz = tz * G::TZ;
 }
 
+// bin key: field-major, then tile, then cell within the tile, so one (field, tile) pair owns a
+// contiguous run of sorted particles
 template <int DIM>
-__device__ __forceinline__ int cell_key(const KGrid &g, int ci, int cj, int ck) {
+__device__ __forceinline__ int cell_key(const KGrid &g, int fld, int ci, int cj, int ck) {
     using G = TileGeom<DIM>;
     int tx = ci / G::TX, ty = cj / G::TY, tz = ck / G::TZ;
     int t = (tz * g.nty + ty) * g.ntx + tx;
     int l = ((ck - tz * G::TZ) * G::TY + (cj - ty * G::TY)) * G::TX + (ci - tx * G::TX);
-    return t * G::NC + l;
+    return (fld * g.ntiles + t) * G::NC + l;
+}
+
+// node arrays of one velocity field
+__device__ __forceinline__ KNodes field_view(const KGrid &g, const KNodes &N, int f) {
+    KNodes v = N;
+    long long nn = g.nnodes;
+    v.mass = N.mass + f * nn;
+    v.pk = N.pk + 3 * f * nn;
+    v.vk = N.vk + 3 * f * nn;
+    v.ftot = N.ftot + 3 * f * nn;
+    v.cnt = N.cnt + f * nn;
+    if (N.cvol) {
+        v.cvol = N.cvol + f * nn;
+        v.cdisp = N.cdisp + 3 * f * nn;
+        v.cgrad = N.cgrad + 3 * f * nn;
+    }
+    return v;
 }
 
 __device__ __forceinline__ void elem_ijk(const KGrid &g, int e, int &i, int &j, int &k) {
@@ -118,7 +137,7 @@ __global__ void k_bin_keys(KGrid g, KPart P, KBins B) {
     if (p >= P.n) return;
     int i, j, k;
     elem_ijk(g, P.elem[p], i, j, k);
-    int key = cell_key<DIM>(g, i, j, DIM == 3 ? k : 0);
+    int key = cell_key<DIM>(g, g.fieldOf[P.mat[p]], i, j, DIM == 3 ? k : 0);
     B.key[p] = key;
     atomicAdd(&B.count[key], 1);
 }
@@ -298,8 +317,10 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNod
     __shared__ double s_S[NT][DIM][4];
     __shared__ double s_q[NT][4];
     __shared__ double s_acc[NW][NQ_MASS][G::FOOT];
-    const int t = blockIdx.x;
-    load_tile_starts<DIM>(B, t, s_cs);
+    const int tall = blockIdx.x;              // (field, tile) pair
+    const int t = tall % g.ntiles;
+    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     zero_acc<DIM, NQ_MASS>(s_acc);
@@ -341,12 +362,12 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNod
             });
         __syncthreads();
     }
-    write_partials<DIM, NQ_MASS>(N, t, s_acc);
+    write_partials<DIM, NQ_MASS>(N, tall, s_acc);
 }
 
 // sum the partials of the tiles that cover node (i,j,k), in fixed tile order
 template <int DIM, int NQ>
-__device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, const KNodes &N, int i,
+__device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, const KNodes &N, int fld, int i,
                                                 int j, int k, double out[NQ]) {
     using G = TileGeom<DIM>;
 #pragma unroll
@@ -358,9 +379,9 @@ __device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, 
     for (int tz = tzl; tz <= tzh; tz++)
         for (int ty = tyl; ty <= tyh; ty++)
             for (int tx = txl; tx <= txh; tx++) {
-                int t = (tz * g.nty + ty) * g.ntx + tx;
+                int t = (tz * g.nty + ty) * g.ntx + tx + fld * g.ntiles;
                 if (B.cellStart[(t + 1) * G::NC] == B.cellStart[t * G::NC]) continue;
-                int fx = i - (tx * G::TX - 1), fy = j - (ty * G::TY - 1);
+                int fx = i - (tx * G::TX - 1), fy = j - (ty * G::TY - 1);   // slot in the footprint
                 int s = fy * G::FX + fx;
                 if (DIM == 3) s += (k - (tz * G::TZ - 1)) * G::FX * G::FY;
 #pragma unroll
@@ -371,26 +392,29 @@ __device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, 
 // node pass after P2G #1: mass, momentum, count and v = p/m (MatVelocityField.cpp:158-164,239-251)
 template <int DIM>
 __global__ void k_node_mass(KGrid g, KBins B, KNodes N) {
-    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nnodes) return;
+    long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nall >= g.nnodes * g.nfields) return;
+    const int fld = (int)(nall / g.nnodes);
+    const long long n = nall - (long long)fld * g.nnodes;
+    const KNodes Nf = field_view(g, N, fld);
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
     double a[NQ_MASS];
-    gather_partials<DIM, NQ_MASS>(g, B, N, i, j, k, a);
+    gather_partials<DIM, NQ_MASS>(g, B, N, fld, i, j, k, a);
     long long nn = g.nnodes;
-    N.mass[n] = a[0];
-    N.pk[n] = a[1]; N.pk[nn + n] = a[2]; N.pk[2 * nn + n] = a[3];
+    Nf.mass[n] = a[0];
+    Nf.pk[n] = a[1]; Nf.pk[nn + n] = a[2]; Nf.pk[2 * nn + n] = a[3];
     int cnt = a[0] > 0. ? 1 : 0;       // numberPoints > 0 <=> mass > 0 (all mp > 0)
-    N.cnt[n] = cnt;
+    Nf.cnt[n] = cnt;
     double vx = 0., vy = 0., vz = 0.;
     if (cnt > 0 && a[0] != 0.) {
         double s = 1. / a[0];
         vx = s * a[1]; vy = s * a[2]; vz = s * a[3];
     }
-    N.vk[n] = vx; N.vk[nn + n] = vy; N.vk[2 * nn + n] = vz;
-    N.ftot[n] = 0.; N.ftot[nn + n] = 0.; N.ftot[2 * nn + n] = 0.;
+    Nf.vk[n] = vx; Nf.vk[nn + n] = vy; Nf.vk[2 * nn + n] = vz;
+    Nf.ftot[n] = 0.; Nf.ftot[nn + n] = 0.; Nf.ftot[2 * nn + n] = 0.;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -510,14 +534,16 @@ __global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N,
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    const int t = blockIdx.x;
-    load_tile_starts<DIM>(B, t, s_cs);
+    const int tall = blockIdx.x;              // (field, tile) pair
+    const int t = tall % g.ntiles;
+    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     int ox, oy, oz;
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
-        const double *src[3] = {N.vk, N.vk, N.vk};
+        const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
         const long long st[3] = {0, g.nnodes, 2 * g.nnodes};
         load_foot<DIM, 3>(g, ox, oy, oz, src, st, s_v);
     }
@@ -556,8 +582,10 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
     __shared__ double s_st[CH][NS];
     __shared__ double s_acc[NW][NQ_FORCE][G::FOOT];
     __shared__ double s_id[DIM][G::TX];       // 2/dx of the tile's cell columns, rows, layers
-    const int t = blockIdx.x;
-    load_tile_starts<DIM>(B, t, s_cs);
+    const int tall = blockIdx.x;              // (field, tile) pair
+    const int t = tall % g.ntiles;
+    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     int ox, oy, oz;
@@ -631,35 +659,38 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
             });
         __syncthreads();
     }
-    write_partials<DIM, NQ_FORCE>(N, t, s_acc);
+    write_partials<DIM, NQ_FORCE>(N, tall, s_acc);
 }
 
 // node pass after the forces: ftot, restore p (identity without BCs), p += f dt, v = p/m
 // (PostForcesTask.cpp:46-48, MatVelocityField.cpp:301-314, :239-251)
 template <int DIM>
 __global__ void k_node_force(KGrid g, KBins B, KNodes N) {
-    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nnodes) return;
+    long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nall >= g.nnodes * g.nfields) return;
+    const int fld = (int)(nall / g.nnodes);
+    const long long n = nall - (long long)fld * g.nnodes;
+    const KNodes Nf = field_view(g, N, fld);
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
     double f[NQ_FORCE];
-    gather_partials<DIM, NQ_FORCE>(g, B, N, i, j, k, f);
+    gather_partials<DIM, NQ_FORCE>(g, B, N, fld, i, j, k, f);
     long long nn = g.nnodes;
     if (DIM == 2) f[2] = 0.;
-    N.ftot[n] = f[0]; N.ftot[nn + n] = f[1]; N.ftot[2 * nn + n] = f[2];
-    int cnt = N.cnt[n];
+    Nf.ftot[n] = f[0]; Nf.ftot[nn + n] = f[1]; Nf.ftot[2 * nn + n] = f[2];
+    int cnt = Nf.cnt[n];
     if (cnt <= 0) return;
     double dt = g.timestep;
-    double px = N.pk[n] + f[0] * dt;
-    double py = N.pk[nn + n] + f[1] * dt;
-    double pz = N.pk[2 * nn + n] + f[2] * dt;
-    N.pk[n] = px; N.pk[nn + n] = py; N.pk[2 * nn + n] = pz;
-    double m = N.mass[n];
+    double px = Nf.pk[n] + f[0] * dt;
+    double py = Nf.pk[nn + n] + f[1] * dt;
+    double pz = Nf.pk[2 * nn + n] + f[2] * dt;
+    Nf.pk[n] = px; Nf.pk[nn + n] = py; Nf.pk[2 * nn + n] = pz;
+    double m = Nf.mass[n];
     if (m != 0.) {
         double s = 1. / m;
-        N.vk[n] = s * px; N.vk[nn + n] = s * py; N.vk[2 * nn + n] = s * pz;
+        Nf.vk[n] = s * px; Nf.vk[nn + n] = s * py; Nf.vk[2 * nn + n] = s * pz;
     }
 }
 
@@ -677,15 +708,17 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
     __shared__ double s_S[HALF][DIM][4];
     __shared__ double s_q[HALF][4];
     __shared__ double s_acc[NW][NQ_MOM][G::FOOT];
-    const int t = blockIdx.x;
-    load_tile_starts<DIM>(B, t, s_cs);
+    const int tall = blockIdx.x;              // (field, tile) pair
+    const int t = tall % g.ntiles;
+    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     int ox, oy, oz;
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
         long long nn = g.nnodes;
-        const double *src[7] = {N.vk, N.vk, N.vk, N.ftot, N.ftot, N.ftot, N.mass};
+        const double *src[7] = {Nf.vk, Nf.vk, Nf.vk, Nf.ftot, Nf.ftot, Nf.ftot, Nf.mass};
         const long long st[7] = {0, nn, 2 * nn, 0, nn, 2 * nn, 0};
         load_foot<DIM, 7>(g, ox, oy, oz, src, st, s_n);
     }
@@ -773,28 +806,31 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
             __syncthreads();
         }
     }
-    write_partials<DIM, NQ_MOM>(N, t, s_acc);
+    write_partials<DIM, NQ_MOM>(N, tall, s_acc);
 }
 
 // node pass of the USL re-extrapolation: p (rezeroed, RezeroNodeTask6) and v = p/m
 template <int DIM>
 __global__ void k_node_mom2(KGrid g, KBins B, KNodes N) {
-    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nnodes) return;
-    int cnt = N.cnt[n];
+    long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nall >= g.nnodes * g.nfields) return;
+    const int fld = (int)(nall / g.nnodes);
+    const long long n = nall - (long long)fld * g.nnodes;
+    const KNodes Nf = field_view(g, N, fld);
+    int cnt = Nf.cnt[n];
     if (cnt <= 0) return;
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
     double a[NQ_MOM];
-    gather_partials<DIM, NQ_MOM>(g, B, N, i, j, k, a);
+    gather_partials<DIM, NQ_MOM>(g, B, N, fld, i, j, k, a);
     long long nn = g.nnodes;
-    N.pk[n] = a[0]; N.pk[nn + n] = a[1]; N.pk[2 * nn + n] = a[2];
-    double m = N.mass[n];
+    Nf.pk[n] = a[0]; Nf.pk[nn + n] = a[1]; Nf.pk[2 * nn + n] = a[2];
+    double m = Nf.mass[n];
     if (m != 0.) {
         double s = 1. / m;
-        N.vk[n] = s * a[0]; N.vk[nn + n] = s * a[1]; N.vk[2 * nn + n] = s * a[2];
+        Nf.vk[n] = s * a[0]; Nf.vk[nn + n] = s * a[1]; Nf.vk[2 * nn + n] = s * a[2];
     }
 }
 
@@ -821,14 +857,16 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    const int t = blockIdx.x;
-    load_tile_starts<DIM>(B, t, s_cs);
+    const int tall = blockIdx.x;              // (field, tile) pair
+    const int t = tall % g.ntiles;
+    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     int ox, oy, oz;
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
-        const double *src[3] = {N.vk, N.vk, N.vk};
+        const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
         const long long stt[3] = {0, g.nnodes, 2 * g.nnodes};
         load_foot<DIM, 3>(g, ox, oy, oz, src, stt, s_v);
     }
@@ -922,9 +960,312 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
                 if (DIM == 3) P.posn[2][p] = inside[2];
             }
         }
-        int key = cell_key<DIM>(g, ei, ej, DIM == 3 ? ek : 0);
+        int key = cell_key<DIM>(g, tall / g.ntiles, ei, ej, DIM == 3 ? ek : 0);
         B.key[p] = key;
         atomicAdd(&B.count[key], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Multimaterial contact extrapolations (NodalPoint::AddMassMomentum / AddMassMomentumLast extras,
+// NodalPointMPM.cpp:428-437,477-485 -> CrackVelocityField::AddVolumeDisplacement,
+// CrackVelocityField.cpp:395-402, CrackVelocityFieldMulti::AddVolumeGradient, :108-112): per field,
+// volume sum N Vp, displacement sum (x - x0) N mp and volume gradient sum grad N Vp, with
+// Vp = det(F) mp / rho (MatPoint3D::GetVolume, MatPoint2D::GetVolume) and the start-of-step
+// shape functions. The displacement uses the new positions in the USL pass.
+struct RecContact { double wx, wy, wz, dx, dy, dz, vp, mp, ux, uy, uz; };
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+                                                    int newPositions) {
+    using G = TileGeom<DIM>;
+    constexpr int CH = NT / 2;
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_S[CH][DIM][4];
+    __shared__ double s_dS[CH][DIM][4];
+    __shared__ double s_q[CH][5];              // Vp, mp, displacement
+    __shared__ double s_acc[NW][NQ_CONTACT][G::FOOT];
+    __shared__ double s_id[DIM][G::TX];
+    const int tall = blockIdx.x;
+    const int t = tall % g.ntiles;
+    load_tile_starts<DIM>(B, tall, s_cs);
+    const int tb = s_cs[0], te = s_cs[G::NC];
+    if (tb == te) return;
+    int ox, oy, oz;
+    tile_origin<DIM>(g, t, ox, oy, oz);
+    if (threadIdx.x < G::TX) {
+        int i = threadIdx.x;
+        s_id[0][i] = 2.0 / ((g.xmin + (double)(ox + i + 1) * g.dx) - (g.xmin + (double)(ox + i) * g.dx));
+        if (i < G::TY) s_id[1][i] = 2.0 / ((g.ymin + (double)(oy + i + 1) * g.dy) - (g.ymin + (double)(oy + i) * g.dy));
+        if (DIM == 3 && i < G::TZ)
+            s_id[DIM - 1][i] = 2.0 / ((g.zmin + (double)(oz + i + 1) * g.dz) - (g.zmin + (double)(oz + i) * g.dz));
+    }
+    zero_acc<DIM, NQ_CONTACT>(s_acc);
+    for (int c0 = tb; c0 < te; c0 += CH) {
+        int k = c0 + threadIdx.x;
+        if (threadIdx.x < CH && k < te) {
+            int p = B.order[k];
+            int ei, ej, ek;
+            elem_ijk(g, P.elem[p], ei, ej, ek);
+            double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+            double xi[3], invd[3], S[3][4], dS[3][4];
+            particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+            particle_factors<DIM>(g, P, p, xi, S, dS);
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int o = 0; o < 4; o++) {
+                    s_S[threadIdx.x][a][o] = S[a][o];
+                    s_dS[threadIdx.x][a][o] = dS[a][o];
+                }
+            double mp = P.f[MPF_MASS][p];
+            double rho = mats[P.mat[p]].rho;
+            double Vp;
+            if (DIM == 3) {
+                double Fm[3][3];
+                double ep[6] = {P.f[MPF_EP_XX][p], P.f[MPF_EP_YY][p], P.f[MPF_EP_ZZ][p],
+                                P.f[MPF_EP_YZ][p], P.f[MPF_EP_XZ][p], P.f[MPF_EP_XY][p]};
+                double wr[3] = {P.f[MPF_WR_YZ][p], P.f[MPF_WR_XZ][p], P.f[MPF_WR_XY][p]};
+                get_F3(ep, wr, Fm);
+                Vp = (Fm[0][0] * (Fm[1][1] * Fm[2][2] - Fm[2][1] * Fm[1][2])
+                    - Fm[0][1] * (Fm[1][0] * Fm[2][2] - Fm[2][0] * Fm[1][2])
+                    + Fm[0][2] * (Fm[1][0] * Fm[2][1] - Fm[2][0] * Fm[1][1])) * mp / rho;
+            } else {
+                double exy = P.f[MPF_EP_XY][p], wxy = P.f[MPF_WR_XY][p];
+                double f00 = 1. + P.f[MPF_EP_XX][p], f11 = 1. + P.f[MPF_EP_YY][p];
+                double f01 = 0.5 * (exy - wxy), f10 = 0.5 * (exy + wxy);
+                Vp = (f00 * f11 - f10 * f01) * mp / rho;
+            }
+            s_q[threadIdx.x][0] = Vp;
+            s_q[threadIdx.x][1] = mp;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                s_q[threadIdx.x][2 + a] = a < DIM ? (newPositions ? P.posn[a][p] : pos[a]) - P.f[MPF_ORIG_X + a][p] : 0.;
+        }
+        __syncthreads();
+        wave_cells<DIM, NQ_CONTACT>(s_cs, c0, min(c0 + CH, te), s_acc,
+            [&](int lq, int ix, int iy, int iz) {
+                RecContact r;
+                r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
+                r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
+                if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
+                else { r.wz = 1.; r.dz = 0.; }
+                r.vp = s_q[lq][0]; r.mp = s_q[lq][1];
+                r.ux = s_q[lq][2]; r.uy = s_q[lq][3]; r.uz = s_q[lq][4];
+                return r;
+            },
+            [&](const RecContact &r, double *acc, int lx, int ly, int lz) {
+                bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
+                double vp = on ? r.vp : 0., mp = on ? r.mp : 0.;
+                double fn, gx, gy, gz;
+                if (DIM == 3) {
+                    fn = r.wx * r.wy * r.wz;
+                    gx = r.dx * r.wy * r.wz * s_id[0][lx];
+                    gy = r.wx * r.dy * r.wz * s_id[1][ly];
+                    gz = r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
+                } else {
+                    fn = r.wx * r.wy;
+                    gx = r.dx * r.wy * s_id[0][lx];
+                    gy = r.wx * r.dy * s_id[1][ly];
+                    gz = 0.;
+                }
+                acc[0] += fn * vp;
+                double fnmp = fn * mp;
+                acc[1] += r.ux * fnmp;
+                acc[2] += r.uy * fnmp;
+                acc[3] += r.uz * fnmp;
+                acc[4] += gx * vp;
+                acc[5] += gy * vp;
+                acc[6] += gz * vp;
+            });
+        __syncthreads();
+    }
+    write_partials<DIM, NQ_CONTACT>(N, tall, s_acc);
+}
+
+template <int DIM>
+__global__ void k_node_contact(KGrid g, KBins B, KNodes N) {
+    long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nall >= g.nnodes * g.nfields) return;
+    const int fld = (int)(nall / g.nnodes);
+    const long long n = nall - (long long)fld * g.nnodes;
+    const KNodes Nf = field_view(g, N, fld);
+    int i = (int)(n % g.nnx);
+    long long r = n / g.nnx;
+    int j = (int)(r % g.nny);
+    int k = (int)(r / g.nny);
+    double a[NQ_CONTACT];
+    gather_partials<DIM, NQ_CONTACT>(g, B, N, fld, i, j, k, a);
+    long long nn = g.nnodes;
+    Nf.cvol[n] = a[0];
+    Nf.cdisp[n] = a[1]; Nf.cdisp[nn + n] = a[2]; Nf.cdisp[2 * nn + n] = a[3];
+    Nf.cgrad[n] = a[4]; Nf.cgrad[nn + n] = a[5]; Nf.cgrad[2 * nn + n] = a[6];
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// CoulombFriction::GetFrictionalDeltaMomentum and GetSslideAcDt (CoulombFriction.cpp:146-282);
+// false leaves the momenta unchanged
+__device__ __forceinline__ bool coulomb_delta(const KGrid &g, double *delPi, const double *norm, double dotn,
+                                              double deltaDotn, double mred, const double *delFi) {
+    double coeff = g.friction, coeffStatic = g.frictionStatic;
+    int style = coeff < 0. ? 0 : (coeff == 0. ? 1 : 2);      // STICK, FRICTIONLESS, FRICTIONAL
+    if (coeffStatic > 0. && style == 1) style = 2;
+    if (style == 0) return true;
+    double deltime = g.timestep;
+    if (delFi) {
+        double fn = dot3(delFi, norm);
+        deltaDotn += deltime * (dotn - 0.5 * fn * deltime) / mred;
+    }
+    bool inContact = deltaDotn < 0. && dotn < 0.;
+    if (style == 1) {
+        if (inContact) { for (int c = 0; c < 3; c++) delPi[c] = norm[c] * dotn; return true; }
+        return false;
+    }
+    if (!inContact) return false;
+    double dott = 0., tang[3];
+    for (int c = 0; c < 3; c++) tang[c] = delPi[c];
+    for (int c = 0; c < 3; c++) tang[c] += norm[c] * (-dotn);
+    double tangMag = sqrt(dot3(tang, tang));
+    if (tangMag > 0.) {
+        double sc = 1. / tangMag;
+        for (int c = 0; c < 3; c++) tang[c] *= sc;
+        dott = dot3(delPi, tang);
+        if (dott < 0.) { for (int c = 0; c < 3; c++) tang[c] *= -1.; dott = -dott; }
+    }
+    double NAcDt = -dotn, SslideAcDt;
+    if (coeffStatic > 0. && dott <= coeffStatic * NAcDt) SslideAcDt = coeffStatic * NAcDt;
+    else SslideAcDt = coeff * NAcDt;
+    if (SslideAcDt <= 0.) {
+        for (int c = 0; c < 3; c++) delPi[c] = norm[c] * dotn;
+    } else if (dott > SslideAcDt) {
+        for (int c = 0; c < 3; c++) delPi[c] = norm[c] * dotn;
+        for (int c = 0; c < 3; c++) delPi[c] += tang[c] * SslideAcDt;
+    }
+    return true;
+}
+
+// MatVelocityField::ChangeMatMomentum (MatVelocityField.cpp:197-224) on field f of node n, then
+// the field's velocity for the next gather (GridValueCalculation, :239-251)
+__device__ __forceinline__ void change_momentum(const KGrid &g, const KNodes &N, int f, long long n,
+                                                const double *delP, int callType) {
+    long long nn = g.nnodes, b = 3 * f * nn + n;
+    double p[3];
+    for (int c = 0; c < 3; c++) { p[c] = N.pk[b + c * nn] + delP[c]; N.pk[b + c * nn] = p[c]; }
+    if (callType == CONTACT_UPDATE_MOMENTUM) {
+        double sc = 1. / g.timestep;
+        for (int c = 0; c < 3; c++) N.ftot[b + c * nn] += delP[c] * sc;
+    }
+    double m = N.mass[f * nn + n];
+    if (N.cnt[f * nn + n] > 0 && m != 0.) {
+        double s = 1. / m;
+        for (int c = 0; c < 3; c++) N.vk[b + c * nn] = s * p[c];
+    }
+}
+
+// MaterialContactNode::ContactOnKnownNodes -> CrackVelocityFieldMulti::MaterialContactOnCVFLumped
+// (CrackVelocityFieldMulti.cpp:302-671) for non-rigid materials, contact by displacements, normals
+// by GetNormalVector (:960-1069) methods 0-3, no symmetry planes. One thread per node; nodes with
+// fewer than two materials return at once.
+template <int DIM>
+__global__ void k_contact(KGrid g, KNodes N, int callType) {
+    long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nnodes) return;
+    const long long nn = g.nnodes;
+    int act[MPMGPU_MAX_MATERIALS];
+    int numMats = 0;
+    for (int f = 0; f < g.nfields; f++)
+        if (N.cnt[f * nn + n] > 0) act[numMats++] = f;
+    if (numMats <= 1) return;
+    double Pc[3] = {0., 0., 0.}, Mc = 0.;
+    for (int a = 0; a < numMats; a++) {
+        long long b = 3 * act[a] * nn + n;
+        for (int c = 0; c < 3; c++) Pc[c] += N.pk[b + c * nn];
+        Mc += N.mass[act[a] * nn + n];
+    }
+    const int normals = g.normals;
+    const bool doingPairs = numMats == 2 && normals != 3;
+    const int miMax = doingPairs ? numMats - 1 : numMats;
+    const bool postUpdate = callType == CONTACT_UPDATE_MOMENTUM;
+    double dispc[3] = {0., 0., 0.};
+    for (int a = 0; a < numMats; a++)
+        for (int c = 0; c < 3; c++) dispc[c] += N.cdisp[(3 * act[a] + c) * nn + n];
+    double sMc = 1. / Mc;
+    for (int c = 0; c < 3; c++) dispc[c] *= sMc;
+    double volAll = 0.;
+    for (int a = 0; a < numMats; a++) volAll += N.cvol[act[a] * nn + n];
+    for (int mi = 0; mi < miMax; mi++) {
+        const int i = act[mi];
+        double massi = N.mass[i * nn + n];
+        double voli = N.cvol[i * nn + n];
+        double volj = volAll - voli;
+        int j = -1;
+        double gradj[3] = {0., 0., 0.};
+        double maxOther = 0.;
+        for (int kj = 0; kj < numMats; kj++) {
+            if (kj == mi) continue;
+            int jj = act[kj];
+            double mv = N.cvol[jj * nn + n];
+            if (mv > maxOther) { maxOther = mv; j = jj; }
+            for (int c = 0; c < 3; c++) gradj[c] += N.cgrad[(3 * jj + c) * nn + n] * -1.;
+        }
+        if (j < 0) continue;
+        double massRatio = massi / Mc;
+        double mred = 1. - massRatio;
+        double delPi[3];
+        for (int c = 0; c < 3; c++) delPi[c] = N.pk[(3 * i + c) * nn + n] * -1.;
+        for (int c = 0; c < 3; c++) delPi[c] += Pc[c] * massRatio;
+        if (!g.ignoreContact) {
+            double gradi[3], norm[3];
+            for (int c = 0; c < 3; c++) gradi[c] = N.cgrad[(3 * i + c) * nn + n] * 1.;
+            if (normals == 0) {
+                double magi = sqrt(dot3(gradi, gradi)), magj = sqrt(dot3(gradj, gradj));
+                if (magi >= 1. * magj) { double sc = 1. / magi; for (int c = 0; c < 3; c++) norm[c] = gradi[c] * sc; }
+                else { double sc = 1. / magj; for (int c = 0; c < 3; c++) norm[c] = gradj[c] * sc; }
+            } else if (normals == 1) {
+                if (voli >= volj) for (int c = 0; c < 3; c++) norm[c] = gradi[c];
+                else for (int c = 0; c < 3; c++) norm[c] = gradj[c];
+                double sc = 1. / sqrt(dot3(norm, norm));
+                for (int c = 0; c < 3; c++) norm[c] = norm[c] * sc;
+            } else if (normals == 2) {
+                for (int c = 0; c < 3; c++) norm[c] = gradi[c];
+                for (int c = 0; c < 3; c++) norm[c] += gradj[c];
+                double sc = 1. / sqrt(dot3(norm, norm));
+                for (int c = 0; c < 3; c++) norm[c] *= sc;
+            } else {
+                for (int c = 0; c < 3; c++) norm[c] = gradi[c];
+                double sc = 1. / sqrt(dot3(norm, norm));
+                for (int c = 0; c < 3; c++) norm[c] *= sc;
+            }
+            if (norm[0] != norm[0] || norm[1] != norm[1] || norm[2] != norm[2]) continue;
+            double dotn = dot3(delPi, norm);
+            double dispa[3], sa = 1. / massi;
+            for (int c = 0; c < 3; c++) dispa[c] = N.cdisp[(3 * i + c) * nn + n] * sa;
+            double delta[3], sr = 1. / mred;
+            for (int c = 0; c < 3; c++) delta[c] = dispc[c];
+            for (int c = 0; c < 3; c++) delta[c] -= dispa[c];
+            for (int c = 0; c < 3; c++) delta[c] *= sr;
+            double dispb[3];
+            for (int c = 0; c < 3; c++) dispb[c] = delta[c];
+            for (int c = 0; c < 3; c++) dispb[c] += dispa[c];
+            double deln = dot3(dispb, norm) - dot3(dispa, norm);
+            mred *= massi;
+            double delFi[3];
+            if (postUpdate) {
+                double fs[3] = {0., 0., 0.};
+                for (int a = 0; a < numMats; a++)
+                    for (int c = 0; c < 3; c++) fs[c] += N.ftot[(3 * act[a] + c) * nn + n];
+                double sc = massi / Mc;
+                for (int c = 0; c < 3; c++) delFi[c] = fs[c] * sc;
+                for (int c = 0; c < 3; c++) delFi[c] += N.ftot[(3 * i + c) * nn + n] * -1.;
+            }
+            if (!coulomb_delta(g, delPi, norm, dotn, deln, mred, postUpdate ? delFi : nullptr)) continue;
+        }
+        change_momentum(g, N, i, n, delPi, callType);
+        if (doingPairs) {
+            for (int c = 0; c < 3; c++) delPi[c] *= -1.;
+            change_momentum(g, N, j, n, delPi, callType);
+        }
     }
 }
 
@@ -952,13 +1293,13 @@ void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, 
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                      const MpmMaterial *mats, hipStream_t s) {
     (void)mats;
-    if (dim == 3) hipLaunchKernelGGL(k_p2g_mass<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
-    else hipLaunchKernelGGL(k_p2g_mass<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_p2g_mass<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
+    else hipLaunchKernelGGL(k_p2g_mass<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
 }
 
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_mass<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_mass<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 template <int DIM>
@@ -966,15 +1307,15 @@ static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, con
                     const MpmMaterial *mats, hipStream_t s) {
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_MOONEY:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
         default:
-            hipLaunchKernelGGL((k_usf<DIM, 0>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, 0>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
     }
 }
 
@@ -985,23 +1326,23 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
 }
 
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_p2g_force<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
-    else hipLaunchKernelGGL(k_p2g_force<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_p2g_force<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
+    else hipLaunchKernelGGL(k_p2g_force<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
 }
 
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_force<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_force<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_g2p_p2g<3>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
-    else hipLaunchKernelGGL(k_g2p_p2g<2>, dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_g2p_p2g<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
+    else hipLaunchKernelGGL(k_g2p_p2g<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
 }
 
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_mom2<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_mom2<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 template <int DIM>
@@ -1009,15 +1350,15 @@ static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &
                           const MpmMaterial *mats, const KStats &st, hipStream_t s) {
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_MOONEY:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
         default:
-            hipLaunchKernelGGL((k_usl_reset<DIM, 0>), dim3(g.ntiles), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, 0>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
     }
 }
 
@@ -1025,4 +1366,21 @@ void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, c
                       const MpmMaterial *mats, const KStats &st, hipStream_t s) {
     if (dim == 3) usl_reset_mat<3>(g.matType, g, p, b, nd, mats, st, s);
     else usl_reset_mat<2>(g.matType, g, p, b, nd, mats, st, s);
+}
+
+void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                        const MpmMaterial *mats, bool newPositions, hipStream_t s) {
+    int np = newPositions ? 1 : 0;
+    if (dim == 3) hipLaunchKernelGGL(k_p2g_contact<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, np);
+    else hipLaunchKernelGGL(k_p2g_contact<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, np);
+}
+
+void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_node_contact<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_contact<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+}
+
+void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s) {
+    if (dim == 3) hipLaunchKernelGGL(k_contact<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd, callType);
+    else hipLaunchKernelGGL(k_contact<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd, callType);
 }

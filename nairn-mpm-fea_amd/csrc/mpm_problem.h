@@ -17,7 +17,8 @@ struct MaterialInput {
     double E = -1., nu = -1.e30, G = -1., K = -1., Lame = -1.;
     double yield = 1.e50, yieldMin = 0., Ep = -1., Khard = 0.;
     double G2 = 0.;                  // Mooney::Mooney default (neo-Hookean limit)
-    double friction = 0.;
+    double friction = 0.;            // CoulombFriction::CoulombFriction defaults
+    double frictionStatic = -1.;
     int ujOption = 0;
     bool readE = false, readNu = false, readG = false;
 };
@@ -44,6 +45,12 @@ struct MpmProblem {
     int ptsPerSide = 2;              // MatlPtsPerElement (NairnMPM.cpp:98-99)
     bool ptsSet = false;
     std::string archiveRoot;
+    // <MultiMaterialMode> contact settings
+    bool mmRequested = false;
+    bool normalsGiven = false;
+    std::string contactLawName;      // <Friction lawname=>
+    int contactLawNum = 0;           // <Friction law=> (1-based), <0: automatic law -n-1, 0: none
+    std::vector<MaterialInput> autoLaws;
     // particles, in mpm[] order
     std::vector<double> d[MPF_NUM_FIELDS];
     std::vector<int32_t> iv[MPI_NUM_FIELDS];

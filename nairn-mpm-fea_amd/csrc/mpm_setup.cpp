@@ -204,8 +204,17 @@ bool material_verify(int analysis, const MaterialInput &in, MpmMaterial &o, std:
             return true;
         }
         case MPMGPU_MAT_COULOMB:
-        case MPMGPU_MAT_CONTACT:
+            // CoulombFriction::VerifyAndLoadProperties (CoulombFriction.cpp:67-90)
             o.friction = in.friction;
+            o.frictionStatic = in.friction < 0. ? -1. : in.frictionStatic;
+            if (o.frictionStatic > 0. && o.frictionStatic < o.friction) {
+                err = "The static coefficient of friction cannot be lower than the sliding coefficient of friction";
+                return false;
+            }
+            return true;
+        case MPMGPU_MAT_CONTACT:
+            o.friction = 0.;
+            o.frictionStatic = -1.;
             return true;
         default:
             err = "material type " + std::to_string(in.type) + " is not supported on this path";
@@ -379,6 +388,57 @@ bool finalize(MpmProblem &p) {
         MpmMaterial m;
         if (!material_verify(p.analysis, mi, m, p.err)) return false;
         p.mats.push_back(m);
+    }
+    // velocity fields in order of first use (MaterialBase::SetField, MaterialBaseMPM.cpp:615-680,
+    // called from NairnMPM.cpp:617); one field turns multimaterial mode off (NairnMPM.cpp:739-744)
+    {
+        int64_t n = p.nparticles();
+        std::vector<int> field(p.mats.size(), -1);
+        int nf = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int mat = p.iv[MPI_MAT][i];
+            if (mat < 0 || mat >= (int)p.mats.size()) { p.err = "Material point with an undefined material type"; return false; }
+            if (field[mat] < 0) field[mat] = nf++;
+        }
+        p.run.multiMaterial = p.mmRequested && nf >= 2 ? 1 : 0;
+        for (size_t m = 0; m < p.mats.size(); m++)
+            p.mats[m].field = field[m] < 0 ? -1 : (p.run.multiMaterial ? field[m] : 0);
+        p.run.numFields = p.run.multiMaterial ? nf : 1;
+        p.run.contactLaw = -1;
+        if (p.run.multiMaterial) {
+            // MeshInfo::MaterialOutput (MeshInfo.cpp:1126-1200): logistic regression is the default
+            if (!p.normalsGiven) p.run.normalsMethod = 6;
+            if (p.run.normalsMethod > 3) {
+                p.err = "multimaterial contact normals other than 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG) "
+                        "are not on this path; set <MultiMaterialMode Normals=...>";
+                return false;
+            }
+            int law = -1;
+            if (!p.contactLawName.empty()) {
+                for (size_t m = 0; m < p.matIn.size(); m++)
+                    if (p.matIn[m].name == p.contactLawName) law = (int)m;
+            } else if (p.contactLawNum > 0) {
+                law = p.contactLawNum - 1;
+            } else {
+                // automatic law: old-style coefficient, or frictionless when no <Friction> was
+                // given (MPMReadHandler.cpp:1780-1783)
+                MaterialInput a;
+                if (p.contactLawNum < 0) a = p.autoLaws[-p.contactLawNum - 1];
+                else { a.type = MPMGPU_MAT_COULOMB; a.friction = 0.; a.name = "automatic contact law"; }
+                MpmMaterial am;
+                if (!material_verify(p.analysis, a, am, p.err)) return false;
+                am.field = -1;
+                p.mats.push_back(am);
+                law = (int)p.mats.size() - 1;
+            }
+            if (law < 0 || law >= (int)p.mats.size() ||
+                (p.mats[law].type != MPMGPU_MAT_COULOMB && p.mats[law].type != MPMGPU_MAT_CONTACT)) {
+                p.err = "Multimaterial mode must select a default contact law";
+                return false;
+            }
+            if ((int)p.mats.size() > MPMGPU_MAX_MATERIALS) { p.err = "too many materials"; return false; }
+            p.run.contactLaw = law;
+        }
     }
     const MpmGridDesc &g = p.grid;
     double dcell = g.dx;                          // MeshInfo::GetMinCellDimension

@@ -272,10 +272,38 @@ struct Reader {
                 else if (v == "lCPDI" || v == "CPDI" || v == "2") p.run.shape = MPMGPU_SHAPE_LCPDI;
                 else return fail("GIMP type " + v + " is not on this path");
             } else if (k.name == "MultiMaterialMode") {
-                p.run.multiMaterial = 1;
-                if (k.attr("Normals")) p.run.normalsMethod = (int)num(k.attr("Normals"));
-                for (auto &c : k.kids)
-                    if (c->name == "Friction" && c->attr("mat")) p.run.contactLaw = (int)num(c->attr("mat")) - 1;
+                // MPMReadHandler.cpp:603-650 (Normals) and :896-958,1792-1812 (<Friction>)
+                p.mmRequested = true;
+                p.normalsGiven = false;
+                if (k.attr("Normals")) {
+                    double v = num(k.attr("Normals"));
+                    int m = v < 0.5 && v > -0.5 ? 0 : v < 1.5 ? 1 : v < 2.5 ? 2 : v < 3.5 ? 3 : v < 4.5 ? 4 : v < 5.5 ? 5 : 6;
+                    p.run.normalsMethod = m;
+                    p.normalsGiven = true;
+                }
+                if (k.attr("RigidBias") || k.attr("Polar") || k.attr("Azimuth"))
+                    return fail("rigid-material and specified-normal contact settings are not on this path");
+                for (auto &c : k.kids) {
+                    if (c->name == "ContactPosition") return fail("<ContactPosition> (contact by positions) is not on this path");
+                    if (c->name != "Friction") continue;
+                    if (c->attr("lawname")) {
+                        p.contactLawName = c->attr("lawname");
+                    } else if (c->attr("law")) {
+                        p.contactLawNum = (int)num(c->attr("law"));
+                    } else {
+                        // old style: the text is a coefficient turned into an automatic law
+                        // (ContactLaw::ConvertOldStyleToContactLaw, ContactLaw.cpp:168-207)
+                        double f = textnum(*c);
+                        MaterialInput law;
+                        law.name = "automatic contact law";
+                        if (f < -10.) law.type = MPMGPU_MAT_CONTACT;
+                        else if (f < 0.) { law.type = MPMGPU_MAT_COULOMB; law.friction = -1.; }
+                        else if (f < 10.) { law.type = MPMGPU_MAT_COULOMB; law.friction = f; }
+                        else return fail("imperfect-interface contact is not on this path");
+                        p.autoLaws.push_back(law);
+                        p.contactLawNum = -(int)p.autoLaws.size();
+                    }
+                }
             } else if (k.name == "ArchiveRoot") {
                 p.archiveRoot = k.text;
             } else if (k.name == "LeaveLimit") {
@@ -403,7 +431,10 @@ struct Reader {
             else if (n == "Ep") in.Ep = v;
             else if (n == "Khard") in.Khard = v;
             else if (n == "coeff") in.friction = v;
+            else if (n == "coeffStatic") in.frictionStatic = v;
             else if (n == "UJOption") in.ujOption = (int)v;
+            else if (n == "Friction" || n == "SharedMaterialField" || n == "ArtificialVisc" || n == "ArtificialViscosity")
+                return fail("<" + n + "> in <Material> is not on this path");
             else if (n == "Hardening") {
                 if (k.text.find("Linear") == std::string::npos && (int)v != 1 && !k.text.empty() && k.text.find_first_not_of(" \t\r\n") != std::string::npos)
                     return fail("only linear hardening is on this path");

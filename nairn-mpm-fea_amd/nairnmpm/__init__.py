@@ -64,7 +64,7 @@ class Material(ctypes.Structure):
                 ("alphaMax", ctypes.c_double), ("Gsp", ctypes.c_double), ("Lamesp", ctypes.c_double),
                 ("Ksp", ctypes.c_double), ("G1sp", ctypes.c_double), ("G2sp", ctypes.c_double),
                 ("waveSpeed", ctypes.c_double), ("friction", ctypes.c_double),
-                ("ujOption", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("frictionStatic", ctypes.c_double), ("ujOption", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class RunParams(ctypes.Structure):
@@ -72,7 +72,7 @@ class RunParams(ctypes.Structure):
                 ("strainLast", ctypes.c_double), ("shape", ctypes.c_int32), ("mpmMethod", ctypes.c_int32),
                 ("multiMaterial", ctypes.c_int32), ("contactLaw", ctypes.c_int32),
                 ("normalsMethod", ctypes.c_int32), ("deleteLeaving", ctypes.c_int32),
-                ("maxLeaving", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("maxLeaving", ctypes.c_int32), ("numFields", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -110,6 +110,7 @@ _SIGNATURES = {
     "mpmgpu_download_particles": ([_P, _PD, _PI], ctypes.c_int),
     "mpmgpu_query": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
     "mpmgpu_download_grid": ([_P, _PD, _PD, _PD], ctypes.c_int),
+    "mpmgpu_download_contact": ([_P, _PD, _PD, _PD], ctypes.c_int),
     "mpmgpu_destroy": ([_P], None),
     "mpmgpu_set_profiling": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_kernel_times": ([_P, _PD, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
@@ -117,7 +118,7 @@ _SIGNATURES = {
 }
 
 KERNEL_GROUPS = ["k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_node_force", "k_g2p_p2g",
-                 "k_node_mom2", "k_usl_reset", "binning"]
+                 "k_node_mom2", "k_usl_reset", "binning", "contact"]
 
 
 def library() -> ctypes.CDLL:
@@ -232,6 +233,7 @@ class Simulation:
         _check(library().mpmgpu_create(ctypes.byref(grid), arr, len(mats), ctypes.byref(run), ctypes.byref(h)))
         self._h = h
         self.grid, self.run = grid, run
+        self.numFields = run.numFields if run.multiMaterial and run.numFields > 1 else 1
         self.n = 0
 
     @classmethod
@@ -264,14 +266,32 @@ class Simulation:
         _check(library().mpmgpu_download_particles(self._h, _dptr(d), _iptr(i)), self._h)
         return d, i
 
-    def grid_values(self):
+    def _nodes(self):
         g = self.grid
-        nn = (g.nx + 1) * (g.ny + 1) * ((g.nz + 1) if g.analysis == THREED else 1)
-        m = np.zeros(nn)
-        pk = np.zeros(3 * nn)
-        f = np.zeros(3 * nn)
+        return (g.nx + 1) * (g.ny + 1) * ((g.nz + 1) if g.analysis == THREED else 1)
+
+    def grid_fields(self):
+        """mass [nf, nn], momentum and force [nf, 3, nn] of every material velocity field"""
+        nn, nf = self._nodes(), self.numFields
+        m = np.zeros(nf * nn)
+        pk = np.zeros(3 * nf * nn)
+        f = np.zeros(3 * nf * nn)
         _check(library().mpmgpu_download_grid(self._h, _dptr(m), _dptr(pk), _dptr(f)), self._h)
-        return m, pk.reshape(3, nn), f.reshape(3, nn)
+        return m.reshape(nf, nn), pk.reshape(nf, 3, nn), f.reshape(nf, 3, nn)
+
+    def grid_values(self):
+        """field 0: mass [nn], momentum and force [3, nn]"""
+        m, pk, f = self.grid_fields()
+        return m[0], pk[0], f[0]
+
+    def contact_values(self):
+        """volume [nf, nn], displacement and volume gradient [nf, 3, nn] (multimaterial mode)"""
+        nn, nf = self._nodes(), self.numFields
+        v = np.zeros(nf * nn)
+        d = np.zeros(3 * nf * nn)
+        gr = np.zeros(3 * nf * nn)
+        _check(library().mpmgpu_download_contact(self._h, _dptr(v), _dptr(d), _dptr(gr)), self._h)
+        return v.reshape(nf, nn), d.reshape(nf, 3, nn), gr.reshape(nf, 3, nn)
 
     def set_profiling(self, on: bool):
         """HIP-event timing of every kernel group (eager launches while on)."""

@@ -171,3 +171,101 @@ def j2_block_2d(cells: int = 48, block: int = 24, seed: int = 7, E: float = 2000
   </Material>
 """
     return elastic_block_2d(cells=cells, block=block, seed=seed, vmax=vmax, cfl=cfl, material_xml=mat)
+
+
+def contact_cylinders_3d(cells: int = 160, radius: float = 20., length: float = 60., gap: float = 4.,
+                         vx: float = 10000., G: float = 10., K: float = 50., rho: float = 1.1,
+                         friction: float = 0.3, shape: str = "lCPDI", normals: int = 2,
+                         cfl: float = 0.5, law_type: int = 61) -> str:
+    """Two neo-Hookean cylinders (axes along z) closing at 2*vx mm/s, in multimaterial mode with a
+    Coulomb friction law: the contact workload. The cylinders are two materials with the same
+    properties so that each has its own velocity field; the third material is the contact law."""
+    c = cells / 2.0
+    xa, xb = c - radius - gap / 2.0, c + radius + gap / 2.0
+    z0, z1 = c - length / 2.0, c + length / 2.0
+    header = (f"<?xml version='1.0'?>\n<JANFEAInput version='3'>\n  <Header>\n"
+              f"    <Description>contact workload</Description>\n    <Analysis>12</Analysis>\n  </Header>\n"
+              f"  <MPMHeader>\n    <MatlPtsPerElement>8</MatlPtsPerElement>\n"
+              f"    <Timing CFL=\"{cfl!r}\" max=\"1.0\"/>\n    <GIMP type=\"{shape}\"/>\n"
+              f"    <MultiMaterialMode Normals=\"{normals}\">\n      <Friction law=\"3\"/>\n"
+              f"    </MultiMaterialMode>\n  </MPMHeader>\n")
+    mats = ""
+    for k in (1, 2):
+        mats += f"""  <Material Type="28" Name="cylinder {k}">
+    <rho>{rho!r}</rho>
+    <G>{G!r}</G>
+    <K>{K!r}</K>
+  </Material>
+"""
+    mats += f"""  <Material Type="{law_type}" Name="friction">
+    <coeff>{friction!r}</coeff>
+  </Material>
+"""
+    return (header + f"""  <Mesh output="file">
+    <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}" zmin="0" zmax="{cells}">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+      <Depth cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <Body mat="1" vx="{vx!r}" vy="0" vz="0">
+      <Cylinder axis="z" xmin="{xa - radius!r}" xmax="{xa + radius!r}" ymin="{c - radius!r}" ymax="{c + radius!r}" zmin="{z0!r}" zmax="{z1!r}"/>
+    </Body>
+    <Body mat="2" vx="{-vx!r}" vy="0" vz="0">
+      <Cylinder axis="z" xmin="{xb - radius!r}" xmax="{xb + radius!r}" ymin="{c - radius!r}" ymax="{c + radius!r}" zmin="{z0!r}" zmax="{z1!r}"/>
+    </Body>
+  </MaterialPoints>
+""" + mats + "</JANFEAInput>\n")
+
+
+def contact_blocks_2d(cells: int = 32, w: int = 8, h: int = 8, gap: float = 1., vx: float = 5000.,
+                      vy: float = 0., E: float = 500., nu: float = 0.3, rho: float = 1.0, friction: float = 0.3,
+                      normals: int = 2, law_type: int = 61, cfl: float = 0.5, offset_y: float = 0.) -> str:
+    """Two plane-strain elastic blocks (materials 1 and 2) closing along x at 2*vx mm/s, the right
+    one sliding at vy, with the contact law as material 3 (the 2D multimaterial case)."""
+    c = cells / 2.0
+    xa0, xa1 = c - gap / 2.0 - w, c - gap / 2.0
+    xb0, xb1 = c + gap / 2.0, c + gap / 2.0 + w
+    y0, y1 = c - h / 2.0, c + h / 2.0
+    mats = ""
+    for k in (1, 2):
+        mats += f"""  <Material Type="1" Name="block {k}">
+    <rho>{rho!r}</rho>
+    <E>{E!r}</E>
+    <nu>{nu!r}</nu>
+  </Material>
+"""
+    mats += f"""  <Material Type="{law_type}" Name="friction">
+    <coeff>{friction!r}</coeff>
+  </Material>
+"""
+    return (f"""<?xml version='1.0'?>
+<JANFEAInput version='3'>
+  <Header>
+    <Description>2D contact</Description>
+    <Analysis>10</Analysis>
+  </Header>
+  <MPMHeader>
+    <MatlPtsPerElement>4</MatlPtsPerElement>
+    <Timing CFL="{cfl!r}" max="1.0"/>
+    <GIMP type="uGIMP"/>
+    <MultiMaterialMode Normals="{normals}">
+      <Friction law="3"/>
+    </MultiMaterialMode>
+  </MPMHeader>
+  <Mesh output="file">
+    <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <Body mat="1" vx="{vx!r}" vy="0">
+      <Rect xmin="{xa0!r}" xmax="{xa1!r}" ymin="{y0!r}" ymax="{y1!r}"/>
+    </Body>
+    <Body mat="2" vx="{-vx!r}" vy="{vy!r}">
+      <Rect xmin="{xb0!r}" xmax="{xb1!r}" ymin="{y0 + offset_y!r}" ymax="{y1 + offset_y!r}"/>
+    </Body>
+  </MaterialPoints>
+""" + mats + "</JANFEAInput>\n")

@@ -358,15 +358,36 @@ struct Oracle {
     std::vector<double> pd;     // MPF_NUM_FIELDS * np
     std::vector<int32_t> pi;    // MPI_NUM_FIELDS * np
     std::vector<double> xi;     // 3*np: ncpos of the step
-    // node fields (single velocity field, MatVelocityField.hpp:40-124)
+    // node fields: one MatVelocityField per material field (MatVelocityField.hpp:40-124),
+    // node n of field f at index f*nnodes + n (vectors interleaved x,y,z)
+    int nf = 1;
     std::vector<double> mass, pk, ftot, vk, pkc;
     std::vector<int> npts;
+    // multimaterial contact extrapolations (ContactTerms: volume, displacement, volume gradient)
+    std::vector<double> cvol, cdisp, cgrad;
+    std::vector<long> contactNodes;     // MaterialContactNode::materialContactNodes
     int64_t steps = 0, leftGrid = 0, nanCount = 0;
     double mtime = 0.;
     std::string err;
 
     double &F(int f, long p) { return pd[(size_t)f * np + p]; }
     int32_t &I(int f, long p) { return pi[(size_t)f * np + p]; }
+    bool mm() const { return nf > 1; }
+    long fnode(int f, long n) const { return (long)f * g.nnodes + n; }
+    int field(long p) { return mm() ? mats[I(MPI_MAT, p)].field : 0; }
+
+    // MatPoint3D/MatPoint2D::GetVolume(DEFORMED_AREA) (MatPoint3D.cpp:405-409, MatPoint2D.cpp:407-417)
+    double volume(long p) {
+        Mat3 pF = get_F(p);
+        double rho = mats[I(MPI_MAT, p)].rho;
+        double mp = F(MPF_MASS, p);
+        const double (*f)[3] = pF.m;
+        if (g.dim == 3)
+            return (f[0][0] * (f[1][1] * f[2][2] - f[2][1] * f[1][2])
+                  - f[0][1] * (f[1][0] * f[2][2] - f[2][0] * f[1][2])
+                  + f[0][2] * (f[1][0] * f[2][1] - f[2][0] * f[1][1])) * mp / rho;
+        return (f[0][0] * f[1][1] - f[1][0] * f[0][1]) * mp / rho;
+    }
 
     void shape(long p, bool deriv, ShapeOut &o) {
         long e = I(MPI_ELEM, p);
@@ -385,40 +406,250 @@ struct Oracle {
         std::fill(vk.begin(), vk.end(), 0.);
         std::fill(pkc.begin(), pkc.end(), 0.);
         std::fill(npts.begin(), npts.end(), 0);
+        std::fill(cvol.begin(), cvol.end(), 0.);
+        std::fill(cdisp.begin(), cdisp.end(), 0.);
+        std::fill(cgrad.begin(), cgrad.end(), 0.);
+        contactNodes.clear();
         for (long p = 0; p < np; p++) {
             double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
             get_xi(g, I(MPI_ELEM, p), pos, &xi[3 * p]);
         }
     }
 
+    // NodalPoint::AddMassMomentum's multimaterial extras: CrackVelocityField::AddVolumeDisplacement
+    // (CrackVelocityField.cpp:395-402) and CrackVelocityFieldMulti::AddVolumeGradient
+    // (CrackVelocityFieldMulti.cpp:108-112); displacement = pos - origpos, weights shape*Vp and
+    // shape*mp, gradient weight Vp
+    void add_contact_terms(long n, double shape, double Vp, double mp, const double *d,
+                           double dNdx, double dNdy, double dNdz) {
+        double wtVolume = shape * Vp;
+        cvol[n] += wtVolume;
+        double fnmp = shape * mp;
+        for (int c = 0; c < 3; c++) cdisp[3 * n + c] += d[c] * fnmp;
+        cgrad[3 * n] += dNdx * Vp; cgrad[3 * n + 1] += dNdy * Vp; cgrad[3 * n + 2] += dNdz * Vp;
+    }
+
     // MassAndMomentumTask::Execute (MassAndMomentumTask.cpp:48-127) ->
-    // NodalPoint::AddMassMomentum (NodalPointMPM.cpp:419-453)
+    // NodalPoint::AddMassMomentum (NodalPointMPM.cpp:419-453); multimaterial mode extrapolates
+    // with gradients because volumeGradientIndex >= 0 (MassAndMomentumTask.cpp:137-140)
     void task_mass_momentum() {
         ShapeOut o;
         for (long p = 0; p < np; p++) {
-            shape(p, false, o);
+            shape(p, mm(), o);
+            int f = field(p);
             double mp = F(MPF_MASS, p);
             double v[3] = {F(MPF_VEL_X, p), F(MPF_VEL_Y, p), F(MPF_VEL_Z, p)};
+            double Vp = 0., d[3] = {0., 0., 0.};
+            if (mm()) {
+                Vp = volume(p);
+                for (int c = 0; c < 3; c++) d[c] = F(MPF_POS_X + c, p) - F(MPF_ORIG_X + c, p);
+            }
             for (int i = 0; i < o.n; i++) {
-                long n = o.node[i];
+                long n = fnode(f, o.node[i]);
                 double fnmp = o.fn[i] * mp;
                 for (int c = 0; c < 3; c++) pk[3 * n + c] += fnmp * v[c];   // AddMomentumTask1
                 npts[n] += 1;
+                if (mm()) add_contact_terms(n, o.fn[i], Vp, mp, d, o.dx[i], o.dy[i], g.dim == 3 ? o.dz[i] : 0.);
                 mass[n] += fnmp;                                             // AddMass
             }
         }
     }
 
-    // PostExtrapolationTask::Execute (PostExtrapolationTask.cpp:46-165):
-    // CalcTotalMassAndCount copies pk (MatVelocityField.cpp:158-164)
+    // PostExtrapolationTask::Execute (PostExtrapolationTask.cpp:46-165): CalcTotalMassAndCount
+    // copies pk (MatVelocityField.cpp:158-164) and flags nodes with two or more materials
+    // (CrackVelocityFieldMulti.cpp:1620-1637); then contact at MASS_MOMENTUM_CALL (:155)
     void task_post_extrapolation() {
-        for (long n = 0; n < g.nnodes; n++)
-            if (npts[n] >= 1) for (int c = 0; c < 3; c++) pkc[3 * n + c] = pk[3 * n + c];
+        for (long n = 0; n < g.nnodes; n++) {
+            int numberMaterials = 0;
+            for (int f = 0; f < nf; f++) {
+                long k = fnode(f, n);
+                if (npts[k] >= 1) {
+                    numberMaterials++;
+                    for (int c = 0; c < 3; c++) pkc[3 * k + c] = pk[3 * k + c];
+                }
+            }
+            if (numberMaterials > 1) contactNodes.push_back(n);
+        }
+        material_contact(CALL_MASS_MOMENTUM);
+    }
+
+    // ---- multimaterial contact -----------------------------------------------------------
+    enum { CALL_MASS_MOMENTUM = 0, CALL_UPDATE_MOMENTUM = 1, CALL_UPDATE_STRAINS_LAST = 2 };
+    static double dot(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+    // MatVelocityField::ChangeMatMomentum (MatVelocityField.cpp:197-224)
+    void change_momentum(long k, const double *delP, int callType, double deltime) {
+        for (int c = 0; c < 3; c++) pk[3 * k + c] += delP[c];
+        if (callType == CALL_UPDATE_MOMENTUM) {
+            double sc = 1. / deltime;
+            for (int c = 0; c < 3; c++) ftot[3 * k + c] += delP[c] * sc;
+        } else if (callType == CALL_MASS_MOMENTUM) {
+            for (int c = 0; c < 3; c++) pkc[3 * k + c] = pk[3 * k + c];
+        }
+    }
+
+    // CoulombFriction::GetFrictionalDeltaMomentum and GetSslideAcDt (CoulombFriction.cpp:146-282);
+    // returns false when the law leaves the momenta unchanged
+    bool coulomb(const MpmMaterial &law, double *delPi, const double *norm, double dotn, double deltaDotn,
+                 double mred, double deltime, const double *delFi) {
+        double coeff = law.friction, coeffStatic = law.frictionStatic;
+        // frictionStyle (CoulombFriction.cpp:67-85)
+        int style = coeff < 0. ? 0 : (coeff == 0. ? 1 : 2);            // STICK, FRICTIONLESS, FRICTIONAL
+        if (coeffStatic > 0. && style == 1) style = 2;
+        if (style == 0) return true;
+        if (delFi != nullptr) {
+            double fn = dot(delFi, norm);
+            deltaDotn += deltime * (dotn - 0.5 * fn * deltime) / mred;
+        }
+        bool inContact = deltaDotn < 0. && dotn < 0.;
+        if (style == 1) {
+            if (inContact) { for (int c = 0; c < 3; c++) delPi[c] = norm[c] * dotn; return true; }
+            return false;
+        }
+        if (!inContact) return false;                                    // HasFreeSeparation
+        double dott = 0.;
+        double tang[3];
+        for (int c = 0; c < 3; c++) tang[c] = delPi[c];
+        for (int c = 0; c < 3; c++) tang[c] += norm[c] * (-dotn);
+        double tangMag = sqrt(dot(tang, tang));
+        if (tangMag > 0.) {
+            double sc = 1. / tangMag;
+            for (int c = 0; c < 3; c++) tang[c] *= sc;
+            dott = dot(delPi, tang);
+            if (dott < 0.) { for (int c = 0; c < 3; c++) tang[c] *= -1.; dott = -dott; }
+        }
+        double NAcDt = -dotn, SslideAcDt;
+        if (coeffStatic > 0. && dott <= coeffStatic * NAcDt) SslideAcDt = coeffStatic * NAcDt;
+        else SslideAcDt = coeff * NAcDt;
+        if (SslideAcDt <= 0.) {
+            for (int c = 0; c < 3; c++) delPi[c] = norm[c] * dotn;
+        } else if (dott > SslideAcDt) {
+            for (int c = 0; c < 3; c++) delPi[c] = norm[c] * dotn;
+            for (int c = 0; c < 3; c++) delPi[c] += tang[c] * SslideAcDt;
+        }
+        return true;
+    }
+
+    // MaterialContactNode::ContactOnKnownNodes (MaterialContactNode.cpp:196-232) ->
+    // CrackVelocityFieldMulti::MaterialContactOnCVF / MaterialContactOnCVFLumped
+    // (CrackVelocityFieldMulti.cpp:302-671) for non-rigid materials, contact by displacements,
+    // normals by GetNormalVector (:960-1069) methods 0-3, no symmetry planes
+    void material_contact(int callType) {
+        if (!mm()) return;
+        const MpmMaterial &law = mats[run.contactLaw];
+        bool ignoreContact = law.type == MPMGPU_MAT_CONTACT;
+        bool postUpdate = callType == CALL_UPDATE_MOMENTUM;
+        double deltime = run.timestep;
+        int normals = run.normalsMethod;
+        for (long n : contactNodes) {
+            int act[MPMGPU_MAX_MATERIALS];
+            int numMats = 0;
+            double Pc[3] = {0., 0., 0.}, Mc = 0.;
+            for (int f = 0; f < nf; f++) {
+                long k = fnode(f, n);
+                if (npts[k] > 0) {
+                    for (int c = 0; c < 3; c++) Pc[c] += pk[3 * k + c];
+                    Mc += mass[k];
+                    act[numMats++] = f;
+                }
+            }
+            if (numMats <= 1) continue;
+            bool doingPairs = numMats == 2 && normals != 3;
+            int miMax = doingPairs ? numMats - 1 : numMats;
+            // GetCMDisplacement (:1296-1313), scaled by 1/Mc
+            double dispc[3] = {0., 0., 0.};
+            for (int a = 0; a < numMats; a++)
+                for (int c = 0; c < 3; c++) dispc[c] += cdisp[3 * fnode(act[a], n) + c];
+            double sMc = 1. / Mc;
+            for (int c = 0; c < 3; c++) dispc[c] *= sMc;
+            double volAll = 0.;                          // GetContactVolumeNonrigid (:1281-1288)
+            for (int a = 0; a < numMats; a++) volAll += cvol[fnode(act[a], n)];
+            for (int mi = 0; mi < miMax; mi++) {
+                int i = act[mi];
+                long ki = fnode(i, n);
+                double massi = mass[ki];
+                double voli = cvol[ki];
+                double volj = volAll - voli;
+                int j = -1;
+                double gradj[3] = {0., 0., 0.};
+                double maxOtherMaterialVolume = 0.;
+                for (int kj = 0; kj < numMats; kj++) {
+                    if (kj == mi) continue;
+                    int jj = act[kj];
+                    long kk = fnode(jj, n);
+                    double matVolume = cvol[kk];
+                    if (matVolume > maxOtherMaterialVolume) { maxOtherMaterialVolume = matVolume; j = jj; }
+                    for (int c = 0; c < 3; c++) gradj[c] += cgrad[3 * kk + c] * -1.;   // GetVolumeGradient(-1)
+                }
+                if (j < 0) continue;
+                long kj = fnode(j, n);
+                double massRatio = massi / Mc;
+                double mred = 1. - massRatio;
+                double delPi[3];
+                for (int c = 0; c < 3; c++) delPi[c] = pk[3 * ki + c] * -1.;
+                for (int c = 0; c < 3; c++) delPi[c] += Pc[c] * massRatio;
+                double norm[3] = {0., 0., 0.}, dotn = 0., deln = 0.;
+                if (!ignoreContact) {
+                    double gradi[3] = {cgrad[3 * ki] * 1., cgrad[3 * ki + 1] * 1., cgrad[3 * ki + 2] * 1.};
+                    if (normals == 0) {                                   // MAXIMUM_VOLUME_GRADIENT
+                        double magi = sqrt(dot(gradi, gradi));
+                        double magj = sqrt(dot(gradj, gradj));
+                        if (magi >= 1. * magj) { double sc = 1. / magi; for (int c = 0; c < 3; c++) norm[c] = gradi[c] * sc; }
+                        else { double sc = 1. / magj; for (int c = 0; c < 3; c++) norm[c] = gradj[c] * sc; }
+                    } else if (normals == 1) {                            // MAXIMUM_VOLUME
+                        if (voli >= volj) for (int c = 0; c < 3; c++) norm[c] = gradi[c];
+                        else for (int c = 0; c < 3; c++) norm[c] = gradj[c];
+                        double sc = 1. / sqrt(dot(norm, norm));
+                        for (int c = 0; c < 3; c++) norm[c] = norm[c] * sc;
+                    } else if (normals == 2) {                            // AVERAGE_MAT_VOLUME_GRADIENTS
+                        for (int c = 0; c < 3; c++) norm[c] = gradi[c];
+                        for (int c = 0; c < 3; c++) norm[c] += gradj[c];
+                        double magi = dot(norm, norm);
+                        double sc = 1. / sqrt(magi);
+                        for (int c = 0; c < 3; c++) norm[c] *= sc;
+                    } else {                                              // EACH_MATERIALS_MASS_GRADIENT
+                        for (int c = 0; c < 3; c++) norm[c] = gradi[c];
+                        double sc = 1. / sqrt(dot(norm, norm));
+                        for (int c = 0; c < 3; c++) norm[c] *= sc;
+                    }
+                    if (norm[0] != norm[0] || norm[1] != norm[1] || norm[2] != norm[2]) continue;
+                    dotn = dot(delPi, norm);
+                    double dispa[3], sa = 1. / massi;
+                    for (int c = 0; c < 3; c++) dispa[c] = cdisp[3 * ki + c] * sa;
+                    double delta[3], sr = 1. / mred;
+                    for (int c = 0; c < 3; c++) delta[c] = dispc[c];
+                    for (int c = 0; c < 3; c++) delta[c] -= dispa[c];
+                    for (int c = 0; c < 3; c++) delta[c] *= sr;
+                    double dispb[3];
+                    for (int c = 0; c < 3; c++) dispb[c] = delta[c];
+                    for (int c = 0; c < 3; c++) dispb[c] += dispa[c];
+                    // CrackSurfaceContact::MaterialSeparation by displacements (CrackSurfaceContact.cpp:281-287)
+                    deln = dot(dispb, norm) - dot(dispa, norm);
+                    mred *= massi;
+                    double delFi[3], *delFiPtr = nullptr;
+                    if (postUpdate) {
+                        double fsum[3] = {0., 0., 0.};                    // GetCMatFtot (:1316-1326)
+                        for (int a = 0; a < numMats; a++)
+                            for (int c = 0; c < 3; c++) fsum[c] += ftot[3 * fnode(act[a], n) + c];
+                        double sc = massi / Mc;
+                        for (int c = 0; c < 3; c++) delFi[c] = fsum[c] * sc;
+                        for (int c = 0; c < 3; c++) delFi[c] += ftot[3 * ki + c] * -1.;
+                        delFiPtr = delFi;
+                    }
+                    if (!coulomb(law, delPi, norm, dotn, deln, mred, deltime, delFiPtr)) continue;
+                }
+                change_momentum(ki, delPi, callType, deltime);
+                if (doingPairs) {
+                    for (int c = 0; c < 3; c++) delPi[c] *= -1.;
+                    change_momentum(kj, delPi, callType, deltime);
+                }
+            }
+        }
     }
 
     // MatVelocityField::GridValueCalculation (MatVelocityField.cpp:239-251)
     void grid_velocity() {
-        for (long n = 0; n < g.nnodes; n++) {
+        for (long n = 0; n < (long)nf * g.nnodes; n++) {
             if (npts[n] == 0 || mass[n] == 0.) continue;
             double s = 1. / mass[n];
             for (int c = 0; c < 3; c++) vk[3 * n + c] = s * pk[3 * n + c];
@@ -431,10 +662,11 @@ struct Oracle {
         ShapeOut o;
         for (long p = 0; p < np; p++) {
             shape(p, true, o);
+            long off = fnode(field(p), 0);
             Mat3 dv;
             if (g.dim == 3) {       // MatPoint3D::ExtrapVelocityGradient (MatPoint3D.cpp:62-94)
                 for (int i = 0; i < o.n; i++) {
-                    const double *v = &vk[3 * o.node[i]];
+                    const double *v = &vk[3 * (off + o.node[i])];
                     Mat3 t = Mat3::full(v[0] * o.dx[i], v[0] * o.dy[i], v[0] * o.dz[i],
                                         v[1] * o.dx[i], v[1] * o.dy[i], v[1] * o.dz[i],
                                         v[2] * o.dx[i], v[2] * o.dy[i], v[2] * o.dz[i]);
@@ -442,7 +674,7 @@ struct Oracle {
                 }
             } else {                // MatPoint2D::ExtrapVelocityGradient (MatPoint2D.cpp:57-89)
                 for (int i = 0; i < o.n; i++) {
-                    const double *v = &vk[3 * o.node[i]];
+                    const double *v = &vk[3 * (off + o.node[i])];
                     dv.add(Mat3::two(v[0] * o.dx[i], v[0] * o.dy[i], v[1] * o.dx[i], v[1] * o.dy[i], 0.));
                 }
             }
@@ -760,8 +992,9 @@ struct Oracle {
             double mp = F(MPF_MASS, p), P = F(MPF_PRESSURE, p);
             double sxx = F(MPF_SP_XX, p), syy = F(MPF_SP_YY, p), szz = F(MPF_SP_ZZ, p);
             double syz = F(MPF_SP_YZ, p), sxz = F(MPF_SP_XZ, p), sxy = F(MPF_SP_XY, p);
+            long off = fnode(field(p), 0);
             for (int i = 0; i < o.n; i++) {
-                long n = o.node[i];
+                long n = off + o.node[i];
                 double fx, fy, fz;
                 if (g.dim == 3) {
                     fx = -mp * ((sxx - P) * o.dx[i] + sxy * o.dy[i] + sxz * o.dz[i]);
@@ -779,15 +1012,16 @@ struct Oracle {
 
     // PostForcesTask::Execute (PostForcesTask.cpp:43-111) -> RestoreMomenta (MatVelocityField.cpp:177)
     void task_post_forces() {
-        for (long n = 0; n < g.nnodes; n++)
+        for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = pkc[3 * n + c];
     }
 
     // UpdateMomentaTask::Execute (UpdateMomentaTask.cpp:44-64) -> UpdateMomentum (MatVelocityField.cpp:301-314)
     void task_update_momenta() {
         double dt = run.timestep;
-        for (long n = 0; n < g.nnodes; n++)
+        for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] += ftot[3 * n + c] * dt;
+        material_contact(CALL_UPDATE_MOMENTUM);          // ContactAndMomentaBCs (UpdateMomentaTask.cpp:58)
     }
 
     // UpdateParticlesTask::Execute (UpdateParticlesTask.cpp:44-298), FLIP, no damping ->
@@ -798,9 +1032,10 @@ struct Oracle {
         ShapeOut o;
         for (long p = 0; p < np; p++) {
             shape(p, false, o);
+            long off = fnode(field(p), 0);
             double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
             for (int i = 0; i < o.n; i++) {
-                long n = o.node[i];
+                long n = off + o.node[i];
                 for (int c = 0; c < 3; c++) Svk[c] += vk[3 * n + c] * o.fn[i];
                 double mnode = o.fn[i] / mass[n];
                 for (int c = 0; c < 3; c++) Sacc[c] += ftot[3 * n + c] * mnode;
@@ -819,19 +1054,33 @@ struct Oracle {
     // UpdateStrainsLastContactTask::Execute (UpdateStrainsLastContactTask.cpp:60-152):
     // RezeroNodeTask6 + AddMassMomentumLast (NodalPointMPM.cpp:470-486) with start-of-step xi
     void task_update_strains_last() {
-        for (long n = 0; n < g.nnodes; n++)
-            if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = 0.;
+        for (long n = 0; n < (long)nf * g.nnodes; n++)
+            if (npts[n] > 0) {                           // MatVelocityField::RezeroNodeTask6 (:429-444)
+                for (int c = 0; c < 3; c++) pk[3 * n + c] = 0.;
+                if (mm()) {
+                    cvol[n] = 0.;
+                    for (int c = 0; c < 3; c++) { cdisp[3 * n + c] = 0.; cgrad[3 * n + c] = 0.; }
+                }
+            }
         ShapeOut o;
         for (long p = 0; p < np; p++) {
-            shape(p, false, o);
+            shape(p, mm(), o);
+            int f = field(p);
             double mp = F(MPF_MASS, p);
             double v[3] = {F(MPF_VEL_X, p), F(MPF_VEL_Y, p), F(MPF_VEL_Z, p)};
+            double Vp = 0., d[3] = {0., 0., 0.};
+            if (mm()) {
+                Vp = volume(p);
+                for (int c = 0; c < 3; c++) d[c] = F(MPF_POS_X + c, p) - F(MPF_ORIG_X + c, p);
+            }
             for (int i = 0; i < o.n; i++) {
-                long n = o.node[i];
+                long n = fnode(f, o.node[i]);
                 double fnmp = o.fn[i] * mp;
                 for (int c = 0; c < 3; c++) pk[3 * n + c] += v[c] * fnmp;   // AddMomentumTask6
+                if (mm()) add_contact_terms(n, o.fn[i], Vp, mp, d, o.dx[i], o.dy[i], g.dim == 3 ? o.dz[i] : 0.);
             }
         }
+        material_contact(CALL_UPDATE_STRAINS_LAST);      // UpdateStrainsLastContactTask.cpp:145
         full_strain_update(run.strainLast);
     }
 
@@ -930,8 +1179,11 @@ void *oracle_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nm
     o->pi.assign(ifields, ifields + (size_t)MPI_NUM_FIELDS * n);
     o->xi.assign(3 * (size_t)n, 0.);
     size_t nn = (size_t)o->g.nnodes;
-    o->mass.assign(nn, 0.); o->npts.assign(nn, 0);
-    o->pk.assign(3 * nn, 0.); o->ftot.assign(3 * nn, 0.); o->vk.assign(3 * nn, 0.); o->pkc.assign(3 * nn, 0.);
+    o->nf = run->multiMaterial && run->numFields > 1 ? run->numFields : 1;
+    size_t fnn = (size_t)o->nf * nn;
+    o->mass.assign(fnn, 0.); o->npts.assign(fnn, 0);
+    o->pk.assign(3 * fnn, 0.); o->ftot.assign(3 * fnn, 0.); o->vk.assign(3 * fnn, 0.); o->pkc.assign(3 * fnn, 0.);
+    if (o->nf > 1) { o->cvol.assign(fnn, 0.); o->cdisp.assign(3 * fnn, 0.); o->cgrad.assign(3 * fnn, 0.); }
     return o;
 }
 
@@ -971,14 +1223,27 @@ int oracle_get_particles(void *h, double *dfields, int32_t *ifields) {
 }
 
 int64_t oracle_num_nodes(void *h) { return ((Oracle *)h)->g.nnodes; }
+int32_t oracle_num_fields(void *h) { return ((Oracle *)h)->nf; }
 
+/* node values of every field: field f node n at f*nnodes + n (vectors interleaved) */
 int oracle_get_grid(void *h, double *mass, double *pk, double *ftot, int32_t *npts) {
     Oracle *o = (Oracle *)h;
-    size_t nn = (size_t)o->g.nnodes;
+    size_t nn = (size_t)o->nf * o->g.nnodes;
     if (mass) memcpy(mass, o->mass.data(), nn * sizeof(double));
     if (pk) memcpy(pk, o->pk.data(), 3 * nn * sizeof(double));
     if (ftot) memcpy(ftot, o->ftot.data(), 3 * nn * sizeof(double));
     if (npts) memcpy(npts, o->npts.data(), nn * sizeof(int32_t));
+    return 0;
+}
+
+/* contact extrapolations of every field (interleaved vectors); 1 in single-material mode */
+int oracle_get_contact(void *h, double *vol, double *disp, double *grad) {
+    Oracle *o = (Oracle *)h;
+    if (o->nf < 2) return 1;
+    size_t nn = (size_t)o->nf * o->g.nnodes;
+    if (vol) memcpy(vol, o->cvol.data(), nn * sizeof(double));
+    if (disp) memcpy(disp, o->cdisp.data(), 3 * nn * sizeof(double));
+    if (grad) memcpy(grad, o->cgrad.data(), 3 * nn * sizeof(double));
     return 0;
 }
 

@@ -41,7 +41,10 @@ def lib():
         _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
         _lib.oracle_num_nodes.argtypes = [_P]
         _lib.oracle_num_nodes.restype = ctypes.c_int64
+        _lib.oracle_num_fields.argtypes = [_P]
+        _lib.oracle_num_fields.restype = ctypes.c_int32
         _lib.oracle_get_grid.argtypes = [_P, _PD, _PD, _PD, _PI]
+        _lib.oracle_get_contact.argtypes = [_P, _PD, _PD, _PD]
         _lib.oracle_counters.argtypes = [_P, _PL]
         _lib.oracle_shape.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.c_int32, ctypes.c_int64, _PD, _PD,
                                       ctypes.c_int32, _PL, _PD, _PD, _PD, _PD]
@@ -85,14 +88,33 @@ class Oracle:
         lib().oracle_get_particles(self._h, _d(d), i.ctypes.data_as(_PI))
         return d, i
 
-    def grid_values(self):
+    def grid_fields(self):
+        """mass [nf, nn], momentum and force [nf, 3, nn], point counts [nf, nn] of every field"""
         nn = lib().oracle_num_nodes(self._h)
-        m = np.zeros(nn)
-        pk = np.zeros(3 * nn)
-        f = np.zeros(3 * nn)
-        cnt = np.zeros(nn, dtype=np.int32)
+        nf = lib().oracle_num_fields(self._h)
+        m = np.zeros(nf * nn)
+        pk = np.zeros(3 * nf * nn)
+        f = np.zeros(3 * nf * nn)
+        cnt = np.zeros(nf * nn, dtype=np.int32)
         lib().oracle_get_grid(self._h, _d(m), _d(pk), _d(f), cnt.ctypes.data_as(_PI))
-        return m, pk.reshape(nn, 3).T, f.reshape(nn, 3).T, cnt
+        return (m.reshape(nf, nn), pk.reshape(nf, nn, 3).transpose(0, 2, 1),
+                f.reshape(nf, nn, 3).transpose(0, 2, 1), cnt.reshape(nf, nn))
+
+    def contact_values(self):
+        """volume [nf, nn], displacement and volume gradient [nf, 3, nn] (multimaterial mode)"""
+        nn = lib().oracle_num_nodes(self._h)
+        nf = lib().oracle_num_fields(self._h)
+        v = np.zeros(nf * nn)
+        d = np.zeros(3 * nf * nn)
+        g = np.zeros(3 * nf * nn)
+        if lib().oracle_get_contact(self._h, _d(v), _d(d), _d(g)) != 0:
+            raise RuntimeError("no contact extrapolations in single-material mode")
+        return v.reshape(nf, nn), d.reshape(nf, nn, 3).transpose(0, 2, 1), g.reshape(nf, nn, 3).transpose(0, 2, 1)
+
+    def grid_values(self):
+        """field 0: mass [nn], momentum and force [3, nn], point count [nn]"""
+        m, pk, f, cnt = self.grid_fields()
+        return m[0], pk[0], f[0], cnt[0]
 
     def counters(self):
         c = np.zeros(3, dtype=np.int64)
