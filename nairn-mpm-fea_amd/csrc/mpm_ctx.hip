@@ -33,6 +33,7 @@ struct MpmCtx {
     MpmMaterial *dmats = nullptr;
     double *pbuf = nullptr;           // particle double fields + second position buffer
     int *ibuf = nullptr;              // particle int fields
+    double *cbuf = nullptr;           // lCPDI semi-side vectors of the step
     int *bbuf = nullptr;              // binning arrays
     unsigned long long *sbuf = nullptr;
     long long ncap = 0;
@@ -105,7 +106,7 @@ static void enqueue_step(MpmCtx *c) {
         e = e2;
     };
     const bool mm = c->kg.nfields > 1;
-    launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+    launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
     done(MPMGPU_KG_P2G_MASS);
     launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
     done(MPMGPU_KG_NODE_MASS);
@@ -326,8 +327,8 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     *out = nullptr;
     if (grid->analysis != MPMGPU_PLANE_STRAIN && grid->analysis != MPMGPU_THREED)
         return fail_global("analysis type must be 10 (plane strain) or 12 (3D)");
-    if (run->shape != MPMGPU_SHAPE_UGIMP)
-        return fail_global("only uGIMP shape functions are implemented on the GPU path");
+    if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI)
+        return fail_global("shape functions must be uGIMP (1) or lCPDI (10) on the GPU path");
     if (run->mpmMethod != 2) return fail_global("only the USAVG method is implemented");
     int nf = run->multiMaterial ? run->numFields : 1;
     if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return fail_global("bad number of material velocity fields");
@@ -365,6 +366,7 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
     g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
     g.nfields = nf;
+    g.shape = run->shape;
     for (int m = 0; m < MPMGPU_MAX_MATERIALS; m++)
         g.fieldOf[m] = (nf > 1 && m < nmat && mats[m].field >= 0 && mats[m].field < nf) ? mats[m].field : 0;
     if (nf > 1) {
@@ -392,7 +394,7 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nf * nn * sizeof(int)) != hipSuccess ||
         hipMalloc(&partMem, (size_t)nf * g.ntiles * NQMAX * foot * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dmats, nmat * sizeof(MpmMaterial)) != hipSuccess ||
-        hipMalloc(&c->sbuf, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&c->sbuf, 3 * sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return fail_global("out of device memory allocating the grid");
     }
@@ -411,9 +413,10 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     }
     c->ks.leftGrid = c->sbuf;
     c->ks.nanCount = c->sbuf + 1;
+    c->ks.cpdiErr = c->sbuf + 2;
     (void)hipMemsetAsync(nodeMem, 0, nodeBytes, c->stream);
     (void)hipMemsetAsync(cntMem, 0, (size_t)nf * nn * sizeof(int), c->stream);
-    (void)hipMemsetAsync(c->sbuf, 0, 2 * sizeof(unsigned long long), c->stream);
+    (void)hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream);
     (void)hipMemcpyAsync(c->dmats, mats, nmat * sizeof(MpmMaterial), hipMemcpyHostToDevice, c->stream);
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums
     long long nk = c->kb.nkeys;
@@ -445,8 +448,11 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         if (c->ibuf) { (void)hipFree(c->ibuf); c->ibuf = nullptr; }
         HIPCHECK(c, hipMalloc(&c->pbuf, (size_t)(MPF_NUM_FIELDS + 3) * n * sizeof(double)));
         HIPCHECK(c, hipMalloc(&c->ibuf, (size_t)(MPI_NUM_FIELDS + 2) * n * sizeof(int)));
+        if (c->cbuf) { (void)hipFree(c->cbuf); c->cbuf = nullptr; }
+        if (c->run.shape == MPMGPU_SHAPE_LCPDI) HIPCHECK(c, hipMalloc(&c->cbuf, (size_t)9 * n * sizeof(double)));
         c->ncap = n;
     }
+    for (int a = 0; a < 9; a++) c->kp.cpr[a] = c->cbuf ? c->cbuf + (size_t)a * n : nullptr;
     size_t nn = (size_t)n;
     for (int f = 0; f < MPF_NUM_FIELDS; f++) c->kp.f[f] = c->pbuf + f * nn;
     for (int a = 0; a < 3; a++) {
@@ -496,7 +502,7 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
     // the second position buffer starts as a copy: in 2D its z row is never written by a step
     for (int a = 0; a < 3; a++)
         HIPCHECK(c, hipMemcpyAsync(c->posB[a], c->posA[a], nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream));
     set_positions(c);
     launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
@@ -516,7 +522,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
     switch (task) {
         case MPMGPU_TASK_INITIALIZATION: break;   // natural coordinates are found in each pass
         case MPMGPU_TASK_MASS_MOMENTUM:
-            launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+            launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
             launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
             break;
         case MPMGPU_TASK_POST_EXTRAPOLATION:          // node pass of the scatter did the rest
@@ -584,8 +590,19 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
     return 0;
 }
 
+// lCPDI corner faults are fatal, as the reference's exception (MatPoint3D.cpp:610-615)
+static int check_cpdi(MpmCtx *c) {
+    if (c->run.shape != MPMGPU_SHAPE_LCPDI) return 0;
+    unsigned long long e = 0;
+    HIPCHECK(c, hipMemcpy(&e, c->ks.cpdiErr, sizeof(e), hipMemcpyDeviceToHost));
+    if (e) return c->fail("A CPDI particle domain node has left the grid, or a particle domain reaches "
+                          "beyond the neighbouring elements (" + std::to_string(e) + " particle passes)");
+    return 0;
+}
+
 int mpmgpu_sync(MpmCtx *c) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (check_cpdi(c)) return 1;
     collect_marks(c);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->lastStepMs = ms;
@@ -594,6 +611,7 @@ int mpmgpu_sync(MpmCtx *c) {
 
 int mpmgpu_download_particles(MpmCtx *c, double *dfields, int32_t *ifields) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (check_cpdi(c)) return 1;
     size_t nn = (size_t)c->kp.n;
     if (nn == 0) return 0;
     set_positions(c);
@@ -624,6 +642,7 @@ int mpmgpu_download_grid(MpmCtx *c, double *mass, double *pk, double *ftot) {
 
 int mpmgpu_query(MpmCtx *c, MpmStats *st) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (check_cpdi(c)) return 1;
     unsigned long long h[2] = {0, 0};
     HIPCHECK(c, hipMemcpy(h, c->sbuf, sizeof(h), hipMemcpyDeviceToHost));
     st->steps = c->steps;
@@ -667,6 +686,7 @@ void mpmgpu_destroy(MpmCtx *c) {
     destroy_graphs(c);
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);
+    if (c->cbuf) (void)hipFree(c->cbuf);
     if (c->bbuf) (void)hipFree(c->bbuf);
     if (c->kn.mass) (void)hipFree(c->kn.mass);
     if (c->kn.cnt) (void)hipFree(c->kn.cnt);

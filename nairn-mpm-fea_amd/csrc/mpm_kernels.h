@@ -41,11 +41,13 @@ struct KGrid {
     int normals;
     int ignoreContact;         // ContactLaw type 60: always move to the center-of-mass velocity
     double friction, frictionStatic;
+    int shape;                 // MPMGPU_SHAPE_UGIMP or MPMGPU_SHAPE_LCPDI
 };
 
 struct KPart {
     double *f[MPF_NUM_FIELDS]; // f[MPF_POS_*] is the start-of-step position
     double *posn[3];           // end-of-step position (written by the particle update)
+    double *cpr[9];            // lCPDI: semi-side vectors r1, r2, r3 of the step (null otherwise)
     int *elem, *mat, *id, *left;
     long long n;
 };
@@ -78,6 +80,7 @@ struct KNodes {
 struct KStats {
     unsigned long long *leftGrid;
     unsigned long long *nanCount;
+    unsigned long long *cpdiErr;   // lCPDI corners outside the grid or beyond the neighbour cells
 };
 
 enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
@@ -89,7 +92,7 @@ enum { CONTACT_MASS_MOMENTUM = 0, CONTACT_UPDATE_MOMENTUM = 1, CONTACT_UPDATE_ST
 void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
 void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                     const MpmMaterial *mats, hipStream_t s);
+                     const MpmMaterial *mats, const KStats &st, hipStream_t s);
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                 const MpmMaterial *mats, hipStream_t s);

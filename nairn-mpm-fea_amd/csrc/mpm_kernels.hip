@@ -109,6 +109,232 @@ __device__ __forceinline__ void particle_factors(const KGrid &g, const KPart &P,
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// lCPDI (MatPoint3D::GetCPDINodesAndWeights, MatPoint3D.cpp:538-627; MatPoint2D.cpp:519-571;
+// ElementBase::GetCPDIFunctions, MoreMPMElementBase.cpp:581-656). The semi-side vectors
+// r1 = F(psz.x,0,0), r2, r3 are taken once per step, from the start-of-step F, by the first
+// scatter and kept in P.cpr; every pass rebuilds the corners from them and the start position.
+// A corner's element must be the particle's element or a neighbour of it (the 64-node candidate
+// block); otherwise, or when a corner leaves the grid, the error counter is raised and the corner
+// is clamped so that every access stays in bounds.
+struct CpdiCorners {
+    double xi[8][3];           // natural coordinates of each corner in its element
+    double wg[8][3];           // gradient weights
+    int off[8][3];             // element of the corner relative to the particle's element (-1..1)
+};
+
+// MeshInfo::FindElementFromPoint on one axis of a regular grid (MeshInfo.cpp:597-625)
+__device__ __forceinline__ int find_cell(double x, double xmin, double d, int n, bool &ok) {
+    int c = (int)((x - xmin) / d);
+    if (c < 0 || c >= n) {
+        if (x == xmin + n * d) c = n - 1;
+        else { ok = false; c = c < 0 ? 0 : n - 1; }
+    }
+    return c;
+}
+
+// semi-side vectors from the particle's F and size (GetSemiSideVectors, GetParticleSize)
+template <int DIM>
+__device__ __forceinline__ void cpdi_semi_sides(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek,
+                                                double r[9]) {
+    double lpx, lpy, lpz;
+    if (g.uniformLp) { lpx = g.lp[0]; lpy = g.lp[1]; lpz = g.lp[2]; }
+    else { lpx = P.f[MPF_LP_X][p]; lpy = P.f[MPF_LP_Y][p]; lpz = P.f[MPF_LP_Z][p]; }
+    double psx = ((g.xmin + (double)(ei + 1) * g.dx) - (g.xmin + (double)ei * g.dx)) * (0.5 * lpx);
+    double psy = ((g.ymin + (double)(ej + 1) * g.dy) - (g.ymin + (double)ej * g.dy)) * (0.5 * lpy);
+    double ep[6] = {P.f[MPF_EP_XX][p], P.f[MPF_EP_YY][p], P.f[MPF_EP_ZZ][p], 0., 0., P.f[MPF_EP_XY][p]};
+    double wr[3] = {0., 0., P.f[MPF_WR_XY][p]};
+    if (DIM == 3) {
+        ep[3] = P.f[MPF_EP_YZ][p]; ep[4] = P.f[MPF_EP_XZ][p];
+        wr[0] = P.f[MPF_WR_YZ][p]; wr[1] = P.f[MPF_WR_XZ][p];
+    }
+    double F[3][3];
+    get_F3(ep, wr, F);
+    if (DIM == 3) {
+        double psz = ((g.zmin + (double)(ek + 1) * g.dz) - (g.zmin + (double)ek * g.dz)) * (0.5 * lpz);
+        r[0] = F[0][0] * psx; r[1] = F[1][0] * psx; r[2] = F[2][0] * psx;
+        r[3] = F[0][1] * psy; r[4] = F[1][1] * psy; r[5] = F[2][1] * psy;
+        r[6] = F[0][2] * psz; r[7] = F[1][2] * psz; r[8] = F[2][2] * psz;
+    } else {
+        r[0] = F[0][0] * psx; r[1] = F[1][0] * psx; r[2] = 0.;
+        r[3] = F[0][1] * psy; r[4] = F[1][1] * psy; r[5] = 0.;
+        r[6] = r[7] = r[8] = 0.;
+    }
+}
+
+// corners, their natural coordinates and the gradient weights; false if a corner is unusable
+template <int DIM>
+__device__ __forceinline__ bool cpdi_corners(const KGrid &g, const double pos[3], const double r[9], int ei, int ej,
+                                             int ek, CpdiCorners &cc) {
+    bool ok = true;
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+#pragma unroll
+    for (int c = 0; c < NCOR; c++) {
+        // corner signs: 3D r1s/r2s/r3s (MatPoint3D.cpp:23-25), 2D ccw from (-,-) (MatPoint2D.cpp:538-556)
+        const int q = c & 3;
+        const double s1 = (q == 1 || q == 2) ? 1. : -1.;
+        const double s2 = q >= 2 ? 1. : -1.;
+        const double s3 = c >= 4 ? 1. : -1.;
+        double cx[3];
+        if (DIM == 3) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) cx[a] = pos[a] + s1 * r[a] + s2 * r[3 + a] + s3 * r[6 + a];
+        } else {
+            cx[0] = pos[0] + s1 * r[0] + s2 * r[3];
+            cx[1] = pos[1] + s1 * r[1] + s2 * r[4];
+            cx[2] = 0.;
+        }
+        int ci = find_cell(cx[0], g.xmin, g.dx, g.nx, ok);
+        int cj = find_cell(cx[1], g.ymin, g.dy, g.ny, ok);
+        int ck = DIM == 3 ? find_cell(cx[2], g.zmin, g.dz, g.nz, ok) : 0;
+        int o[3] = {ci - ei, cj - ej, DIM == 3 ? ck - ek : 0};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (o[a] < -1 || o[a] > 1) { ok = false; o[a] = o[a] < -1 ? -1 : 1; }
+            cc.off[c][a] = o[a];
+        }
+        double invd[3];
+        particle_xi<DIM>(g, ei + o[0], ej + o[1], ek + o[2], cx[0], cx[1], cx[2], cc.xi[c], invd);
+    }
+    if (DIM == 3) {
+        const double x1 = r[0], y1 = r[1], z1 = r[2], x2 = r[3], y2 = r[4], z2 = r[5];
+        const double x3 = r[6], y3 = r[7], z3 = r[8];
+        double Vp = 8. * (x1 * (y2 * z3 - z2 * y3) + y1 * (z2 * x3 - x2 * z3) + z1 * (x2 * y3 - y2 * x3));
+        Vp = 1. / Vp;
+        double (*w)[3] = cc.wg;
+        w[0][0] = (z1*y2 - y1*z2 - z1*y3 + z2*y3 + y1*z3 - y2*z3)*Vp;
+        w[0][1] = (-(z1*x2) + x1*z2 + z1*x3 - z2*x3 - x1*z3 + x2*z3)*Vp;
+        w[0][2] = (y1*x2 - x1*y2 - y1*x3 + y2*x3 + x1*y3 - x2*y3)*Vp;
+        w[1][0] = (z1*y2 - y1*z2 - z1*y3 - z2*y3 + y1*z3 + y2*z3)*Vp;
+        w[1][1] = (-(z1*x2) + x1*z2 + z1*x3 + z2*x3 - x1*z3 - x2*z3)*Vp;
+        w[1][2] = (y1*x2 - x1*y2 - y1*x3 - y2*x3 + x1*y3 + x2*y3)*Vp;
+        w[2][0] = (z1*y2 - y1*z2 + z1*y3 - z2*y3 - y1*z3 + y2*z3)*Vp;
+        w[2][1] = (-(z1*x2) + x1*z2 - z1*x3 + z2*x3 + x1*z3 - x2*z3)*Vp;
+        w[2][2] = (y1*x2 - x1*y2 + y1*x3 - y2*x3 - x1*y3 + x2*y3)*Vp;
+        w[3][0] = (z1*y2 - y1*z2 + z1*y3 + z2*y3 - y1*z3 - y2*z3)*Vp;
+        w[3][1] = (-(z1*x2) + x1*z2 - z1*x3 - z2*x3 + x1*z3 + x2*z3)*Vp;
+        w[3][2] = (y1*x2 - x1*y2 + y1*x3 + y2*x3 - x1*y3 - x2*y3)*Vp;
+        w[4][0] = (-(z1*y2) + y1*z2 - z1*y3 + z2*y3 + y1*z3 - y2*z3)*Vp;
+        w[4][1] = (z1*x2 - x1*z2 + z1*x3 - z2*x3 - x1*z3 + x2*z3)*Vp;
+        w[4][2] = (-(y1*x2) + x1*y2 - y1*x3 + y2*x3 + x1*y3 - x2*y3)*Vp;
+        w[5][0] = (-(z1*y2) + y1*z2 - z1*y3 - z2*y3 + y1*z3 + y2*z3)*Vp;
+        w[5][1] = (z1*x2 - x1*z2 + z1*x3 + z2*x3 - x1*z3 - x2*z3)*Vp;
+        w[5][2] = (-(y1*x2) + x1*y2 - y1*x3 - y2*x3 + x1*y3 + x2*y3)*Vp;
+        w[6][0] = (-(z1*y2) + y1*z2 + z1*y3 - z2*y3 - y1*z3 + y2*z3)*Vp;
+        w[6][1] = (z1*x2 - x1*z2 - z1*x3 + z2*x3 + x1*z3 - x2*z3)*Vp;
+        w[6][2] = (-(y1*x2) + x1*y2 + y1*x3 - y2*x3 - x1*y3 + x2*y3)*Vp;
+        w[7][0] = (-(z1*y2) + y1*z2 + z1*y3 + z2*y3 - y1*z3 - y2*z3)*Vp;
+        w[7][1] = (z1*x2 - x1*z2 - z1*x3 - z2*x3 + x1*z3 + x2*z3)*Vp;
+        w[7][2] = (-(y1*x2) + x1*y2 + y1*x3 + y2*x3 - x1*y3 - x2*y3)*Vp;
+    } else {
+        double Ap = 4. * (r[0] * r[4] - r[1] * r[3]);
+        Ap = 1. / Ap;
+        cc.wg[0][0] = (r[1] - r[4]) * Ap;
+        cc.wg[1][0] = (r[1] + r[4]) * Ap;
+        cc.wg[2][0] = (-r[1] + r[4]) * Ap;
+        cc.wg[3][0] = (-r[1] - r[4]) * Ap;
+        cc.wg[0][1] = (-r[0] + r[3]) * Ap;
+        cc.wg[1][1] = (-r[0] - r[3]) * Ap;
+        cc.wg[2][1] = (r[0] - r[3]) * Ap;
+        cc.wg[3][1] = (r[0] + r[3]) * Ap;
+#pragma unroll
+        for (int c = 0; c < 4; c++) cc.wg[c][2] = 0.;
+    }
+    return ok;
+}
+
+// linear function of element node (jx,jy,jz) at natural coordinates x (EightNodeIsoparamBrick.cpp:87-108,
+// FourNodeIsoparam.cpp:189-214): 0.125 (1 +- xi)(1 +- eta)(1 +- zeta)
+template <int DIM>
+__device__ __forceinline__ double cpdi_cfn(const double x[3], int jx, int jy, int jz) {
+    double t1 = 1. + (jx ? 1. : -1.) * x[0];
+    double t2 = 1. + (jy ? 1. : -1.) * x[1];
+    if (DIM == 3) {
+        double t3 = 1. + (jz ? 1. : -1.) * x[2];
+        return 0.125 * t1 * t2 * t3;
+    }
+    return 0.25 * t1 * t2;
+}
+
+// merged weight and gradient of node n (offset from the particle's element) over corners
+// c0..last, in corner order; corners whose element lacks the node or whose function is below
+// 1e-15 are skipped (GetCPDIFunctions)
+template <int DIM>
+__device__ __forceinline__ void cpdi_node(const double (*xi)[3], const double (*wg)[3], const int (*off)[3], int c0,
+                                          int nx, int ny, int nz, double &w, double gr[3]) {
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
+    w = 0.; gr[0] = gr[1] = gr[2] = 0.;
+#pragma unroll
+    for (int c = 0; c < NCOR; c++) {
+        if (c < c0) continue;
+        int jx = nx - off[c][0], jy = ny - off[c][1], jz = DIM == 3 ? nz - off[c][2] : 0;
+        if ((unsigned)jx > 1u || (unsigned)jy > 1u || (unsigned)jz > 1u) continue;
+        double cfn = cpdi_cfn<DIM>(xi[c], jx, jy, jz);
+        if (cfn < 1e-15) continue;
+        w += ws * cfn;
+        gr[0] += wg[c][0] * cfn;
+        gr[1] += wg[c][1] * cfn;
+        gr[2] += wg[c][2] * cfn;
+    }
+}
+
+// merged weight only (no gradient) of node n over all corners
+template <int DIM>
+__device__ __forceinline__ double cpdi_w(const double (*xi)[3], const int (*off)[3], int nx, int ny, int nz) {
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
+    double w = 0.;
+#pragma unroll
+    for (int c = 0; c < NCOR; c++) {
+        int jx = nx - off[c][0], jy = ny - off[c][1], jz = DIM == 3 ? nz - off[c][2] : 0;
+        if ((unsigned)jx > 1u || (unsigned)jy > 1u || (unsigned)jz > 1u) continue;
+        double cfn = cpdi_cfn<DIM>(xi[c], jx, jy, jz);
+        if (cfn < 1e-15) continue;
+        w += ws * cfn;
+    }
+    return w;
+}
+
+// corners of particle p for the step: from its saved semi-side vectors and start position
+template <int DIM>
+__device__ __forceinline__ bool cpdi_particle(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek,
+                                              CpdiCorners &cc) {
+    double r[9];
+#pragma unroll
+    for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+    double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+    return cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc);
+}
+
+// visit the particle's nodes in the reference's list order (corner by corner, element node
+// order, first appearance) with their merged weights: f(nx, ny, nz, w, grad)
+template <int DIM, typename Visit>
+__device__ __forceinline__ void cpdi_for_nodes(const CpdiCorners &cc, Visit visit) {
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    for (int c = 0; c < NCOR; c++) {
+#pragma unroll
+        for (int jn = 0; jn < NCOR; jn++) {
+            // element node order (Generators.cpp:1919-1926): x fastest around the face, then z
+            const int q = jn & 3;
+            const int jx = (q == 1 || q == 2) ? 1 : 0, jy = q >= 2 ? 1 : 0, jz = jn >= 4 ? 1 : 0;
+            double cfn = cpdi_cfn<DIM>(cc.xi[c], jx, jy, jz);
+            if (cfn < 1e-15) continue;
+            int nx = cc.off[c][0] + jx, ny = cc.off[c][1] + jy, nz = DIM == 3 ? cc.off[c][2] + jz : 0;
+            bool seen = false;
+            for (int e = 0; e < c; e++) {
+                int ex = nx - cc.off[e][0], ey = ny - cc.off[e][1], ez = DIM == 3 ? nz - cc.off[e][2] : 0;
+                if ((unsigned)ex > 1u || (unsigned)ey > 1u || (unsigned)ez > 1u) continue;
+                if (cpdi_cfn<DIM>(cc.xi[e], ex, ey, ez) >= 1e-15) { seen = true; break; }
+            }
+            if (seen) continue;
+            double w, gr[3];
+            cpdi_node<DIM>(cc.xi, cc.wg, cc.off, c, nx, ny, nz, w, gr);
+            visit(nx, ny, nz, w, gr);
+        }
+    }
+}
+
 template <int DIM>
 __device__ __forceinline__ int foot_slot(int lx, int ly, int lz) {
     using G = TileGeom<DIM>;
@@ -310,44 +536,68 @@ __device__ __forceinline__ void load_tile_starts(const KBins &B, int t, int *s_c
 // P2G #1: mass and momentum (MassAndMomentumTask -> NodalPoint::AddMassMomentum,
 // NodalPointMPM.cpp:419-453). The point count of a node is not summed: every particle has
 // mp > 0 and every in-support weight is > 0, so numberPoints > 0 exactly when the mass is > 0.
-template <int DIM>
-__global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNodes N) {
+template <int DIM, int SH>
+__global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNodes N, KStats st) {
     using G = TileGeom<DIM>;
+    constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
+    constexpr int CH = CP ? 64 : NT;          // particles staged per chunk
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_S[NT][DIM][4];
-    __shared__ double s_q[NT][4];
+    __shared__ double s_S[CP ? 1 : CH][DIM][4];
+    __shared__ double s_cxi[CP ? CH : 1][8][3];
+    __shared__ int s_coff[CP ? CH : 1][8][3];
+    __shared__ double s_q[CH][4];
     __shared__ double s_acc[NW][NQ_MASS][G::FOOT];
     const int tall = blockIdx.x;              // (field, tile) pair
-    const int t = tall % g.ntiles;
-    const KNodes Nf = field_view(g, N, tall / g.ntiles);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     zero_acc<DIM, NQ_MASS>(s_acc);
-    for (int c0 = tb; c0 < te; c0 += NT) {
+    for (int c0 = tb; c0 < te; c0 += CH) {
         int k = c0 + threadIdx.x;
-        if (k < te) {
+        if (threadIdx.x < CH && k < te) {
             int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
-            double xi[3], invd[3], S[3][4], dS[3][4];
-            particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
-                             DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
-            particle_factors<DIM>(g, P, p, xi, S, dS);
+            double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+            if (CP) {
+                // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step
+                double r[9];
+                cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);
 #pragma unroll
-            for (int a = 0; a < DIM; a++)
+                for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
+                CpdiCorners cc;
+                if (!cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc)) atomicAdd(st.cpdiErr, 1ull);
 #pragma unroll
-                for (int o = 0; o < 4; o++) s_S[threadIdx.x][a][o] = S[a][o];
+                for (int c = 0; c < 8; c++)
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
+                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
+                    }
+            } else {
+                double xi[3], invd[3], S[3][4], dS[3][4];
+                particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+                particle_factors<DIM>(g, P, p, xi, S, dS);
+#pragma unroll
+                for (int a = 0; a < DIM; a++)
+#pragma unroll
+                    for (int o = 0; o < 4; o++) s_S[threadIdx.x][a][o] = S[a][o];
+            }
             s_q[threadIdx.x][0] = P.f[MPF_MASS][p];
             s_q[threadIdx.x][1] = P.f[MPF_VEL_X][p];
             s_q[threadIdx.x][2] = P.f[MPF_VEL_Y][p];
             s_q[threadIdx.x][3] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
         }
         __syncthreads();
-        wave_cells<DIM, NQ_MASS>(s_cs, c0, min(c0 + NT, te), s_acc,
+        wave_cells<DIM, NQ_MASS>(s_cs, c0, min(c0 + CH, te), s_acc,
             [&](int lq, int ix, int iy, int iz) {
                 RecMass r;
-                r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                if (CP) {
+                    r.wx = cpdi_w<DIM>(s_cxi[lq], s_coff[lq], ix - 1, iy - 1, iz - 1);
+                    r.wy = 1.; r.wz = 1.;
+                } else {
+                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                }
                 r.mp = s_q[lq][0]; r.vx = s_q[lq][1]; r.vy = s_q[lq][2]; r.vz = s_q[lq][3];
                 return r;
             },
@@ -479,6 +729,41 @@ __device__ __forceinline__ void scale_gradient(double L[3][3], double f) {
     if (DIM == 3) { L[0][2] *= f; L[1][2] *= f; L[2][0] *= f; L[2][1] *= f; }
 }
 
+// velocity gradient of particle p from footprint velocities (lx,ly,lz = its cell in the tile):
+// uGIMP candidates in table order, or lCPDI nodes in list order
+template <int DIM, int SH>
+__device__ __forceinline__ void particle_gradient(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek,
+                                                  int lx, int ly, int lz, const double (*s_v)[TileGeom<DIM>::FOOT],
+                                                  double L[3][3]) {
+    if (SH == MPMGPU_SHAPE_LCPDI) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) L[a][b] = 0.;
+        CpdiCorners cc;
+        cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
+        cpdi_for_nodes<DIM>(cc, [&](int nx, int ny, int nz, double, const double *gr) {
+            int s = foot_slot<DIM>(lx + nx, ly + ny, lz + nz);
+            double vx = s_v[0][s], vy = s_v[1][s], vz = s_v[2][s];
+            if (DIM == 3) {
+                L[0][0] += vx * gr[0]; L[0][1] += vx * gr[1]; L[0][2] += vx * gr[2];
+                L[1][0] += vy * gr[0]; L[1][1] += vy * gr[1]; L[1][2] += vy * gr[2];
+                L[2][0] += vz * gr[0]; L[2][1] += vz * gr[1]; L[2][2] += vz * gr[2];
+            } else {
+                L[0][0] += vx * gr[0]; L[0][1] += vx * gr[1];
+                L[1][0] += vy * gr[0]; L[1][1] += vy * gr[1];
+                L[2][2] += 0.;
+            }
+        });
+    } else {
+        double xi[3], invd[3], S[3][4], dS[3][4];
+        particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                         DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+        particle_factors<DIM>(g, P, p, xi, S, dS);
+        velocity_gradient<DIM>(S, dS, invd, lx, ly, lz, s_v, L);
+    }
+}
+
 template <int DIM>
 __device__ __forceinline__ void load_state(const KPart &P, int p, int type, PState &s) {
     s.ep[0] = P.f[MPF_EP_XX][p]; s.ep[1] = P.f[MPF_EP_YY][p]; s.ep[2] = P.f[MPF_EP_ZZ][p];
@@ -529,7 +814,7 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
 // ---------------------------------------------------------------------------------------
 // USF strain update, first half step (UpdateStrainsFirstTask::FullStrainUpdate,
 // UpdateStrainsFirstTask.cpp:101-168 -> MatPoint3D::UpdateStrain, MatPoint3D.cpp:45-60)
-template <int DIM, int MAT>
+template <int DIM, int MAT, int SH>
 __global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
@@ -552,12 +837,8 @@ __global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N,
         int p = B.order[k];
         int ei, ej, ek;
         elem_ijk(g, P.elem[p], ei, ej, ek);
-        double xi[3], invd[3], S[3][4], dS[3][4];
-        particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
-                         DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
-        particle_factors<DIM>(g, P, p, xi, S, dS);
         double L[3][3];
-        velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+        particle_gradient<DIM, SH>(g, P, p, ei, ej, ek, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
         scale_gradient<DIM>(L, g.strainFirst);
         const MpmMaterial &m = mats[P.mat[p]];
         const int mtype = MAT ? MAT : m.type;
@@ -571,14 +852,18 @@ __global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N,
 // internal-force scatter (GridForcesTask::Execute, GridForcesTask.cpp:40-153 ->
 // MatPoint3D::GetFintPlusFext, MatPoint3D.cpp:248-270; MatPoint2D.cpp:242-263) with the stress
 // just updated by k_usf; pFext is zero (no particle loads)
-template <int DIM>
+template <int DIM, int SH>
 __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
+    constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
-    constexpr int CH = NT / 2;                // particles staged per chunk (keeps LDS < 80 KB)
+    constexpr int CH = CP ? 64 : NT / 2;      // particles staged per chunk (keeps LDS < 80 KB)
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_S[CH][DIM][4];
-    __shared__ double s_dS[CH][DIM][4];
+    __shared__ double s_S[CP ? 1 : CH][DIM][4];
+    __shared__ double s_dS[CP ? 1 : CH][DIM][4];
+    __shared__ double s_cxi[CP ? CH : 1][8][3];
+    __shared__ double s_cwg[CP ? CH : 1][8][3];
+    __shared__ int s_coff[CP ? CH : 1][8][3];
     __shared__ double s_st[CH][NS];
     __shared__ double s_acc[NW][NQ_FORCE][G::FOOT];
     __shared__ double s_id[DIM][G::TX];       // 2/dx of the tile's cell columns, rows, layers
@@ -604,17 +889,30 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
             int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
-            double xi[3], invd[3], S[3][4], dS[3][4];
-            particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
-                             DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
-            particle_factors<DIM>(g, P, p, xi, S, dS);
+            if (CP) {
+                CpdiCorners cc;
+                cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
 #pragma unroll
-            for (int a = 0; a < DIM; a++)
+                for (int c = 0; c < 8; c++)
 #pragma unroll
-                for (int o = 0; o < 4; o++) {
-                    s_S[threadIdx.x][a][o] = S[a][o];
-                    s_dS[threadIdx.x][a][o] = dS[a][o];
-                }
+                    for (int a = 0; a < 3; a++) {
+                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
+                        s_cwg[threadIdx.x][c][a] = cc.wg[c][a];
+                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
+                    }
+            } else {
+                double xi[3], invd[3], S[3][4], dS[3][4];
+                particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                                 DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+                particle_factors<DIM>(g, P, p, xi, S, dS);
+#pragma unroll
+                for (int a = 0; a < DIM; a++)
+#pragma unroll
+                    for (int o = 0; o < 4; o++) {
+                        s_S[threadIdx.x][a][o] = S[a][o];
+                        s_dS[threadIdx.x][a][o] = dS[a][o];
+                    }
+            }
             double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
             s_st[threadIdx.x][0] = -P.f[MPF_MASS][p];
             s_st[threadIdx.x][1] = P.f[MPF_SP_XX][p] - Pr;
@@ -630,10 +928,17 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
         wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
             [&](int lq, int ix, int iy, int iz) {
                 RecForce r;
-                r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
-                r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
-                if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
-                else { r.wz = 1.; r.dz = 0.; }
+                if (CP) {
+                    double gr[3];
+                    cpdi_node<DIM>(s_cxi[lq], s_cwg[lq], s_coff[lq], 0, ix - 1, iy - 1, iz - 1, r.wx, gr);
+                    r.wy = r.wz = 1.;
+                    r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
+                } else {
+                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
+                    r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
+                    if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
+                    else { r.wz = 1.; r.dz = 0.; }
+                }
                 r.negmp = s_st[lq][0]; r.s0 = s_st[lq][1]; r.s1 = s_st[lq][2]; r.s2 = s_st[lq][3];
                 if (DIM == 3) { r.s3 = s_st[lq][NS - 3]; r.s4 = s_st[lq][NS - 2]; r.s5 = s_st[lq][NS - 1]; }
                 else { r.s3 = r.s4 = r.s5 = 0.; }
@@ -643,16 +948,16 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
                 bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
                 double negmp = on ? r.negmp : 0.;
                 if (DIM == 3) {
-                    double gx = r.dx * r.wy * r.wz * s_id[0][lx];
-                    double gy = r.wx * r.dy * r.wz * s_id[1][ly];
-                    double gz = r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
+                    double gx = CP ? r.dx : r.dx * r.wy * r.wz * s_id[0][lx];
+                    double gy = CP ? r.dy : r.wx * r.dy * r.wz * s_id[1][ly];
+                    double gz = CP ? r.dz : r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
                     // s0 = sxx-P, s1 = syy-P, s2 = sxy, s3 = szz-P, s4 = sxz, s5 = syz
                     acc[0] += negmp * (r.s0 * gx + r.s2 * gy + r.s4 * gz);
                     acc[1] += negmp * (r.s2 * gx + r.s1 * gy + r.s5 * gz);
                     acc[2] += negmp * (r.s4 * gx + r.s5 * gy + r.s3 * gz);
                 } else {
-                    double gx = r.dx * r.wy * s_id[0][lx];
-                    double gy = r.wx * r.dy * s_id[1][ly];
+                    double gx = CP ? r.dx : r.dx * r.wy * s_id[0][lx];
+                    double gy = CP ? r.dy : r.wx * r.dy * s_id[1][ly];
                     acc[0] += negmp * (r.s0 * gx + r.s2 * gy);
                     acc[1] += negmp * (r.s2 * gx + r.s1 * gy);
                 }
@@ -699,13 +1004,16 @@ __global__ void k_node_force(KGrid g, KBins B, KNodes N) {
 // with start-of-step shape functions (MatVelocityField::IncrementNodalVelAcc,
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
-template <int DIM>
+template <int DIM, int SH>
 __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
+    constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int HALF = NT / 2;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_n[7][G::FOOT];         // v(3), f(3), mass
-    __shared__ double s_S[HALF][DIM][4];
+    __shared__ double s_S[CP ? 1 : HALF][DIM][4];
+    __shared__ double s_cxi[CP ? HALF : 1][8][3];
+    __shared__ int s_coff[CP ? HALF : 1][8][3];
     __shared__ double s_q[HALF][4];
     __shared__ double s_acc[NW][NQ_MOM][G::FOOT];
     const int tall = blockIdx.x;              // (field, tile) pair
@@ -729,16 +1037,30 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
         const int c1 = min(c0 + NT, te);
         const int k = c0 + threadIdx.x;
         double S[3][4], q4[4];
+        CpdiCorners cc;
         if (k < c1) {
             int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
             double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+            int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
+            double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
+            if (CP) {
+                cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
+                cpdi_for_nodes<DIM>(cc, [&](int nx, int ny, int nz, double fi, const double *) {
+                    int s = foot_slot<DIM>(lx + nx, ly + ny, lz + nz);
+                    Svk[0] += s_n[0][s] * fi;
+                    Svk[1] += s_n[1][s] * fi;
+                    Svk[2] += s_n[2][s] * fi;
+                    double mnode = fi / s_n[6][s];
+                    Sacc[0] += s_n[3][s] * mnode;
+                    Sacc[1] += s_n[4][s] * mnode;
+                    Sacc[2] += s_n[5][s] * mnode;
+                });
+            } else {
             double xi[3], invd[3], dS[3][4];
             particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
             particle_factors<DIM>(g, P, p, xi, S, dS);
-            int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
-            double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
             constexpr int ncand = DIM == 3 ? 64 : 16;
 #pragma unroll
             for (int id = 0; id < ncand; id++) {
@@ -762,6 +1084,7 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
                 Sacc[1] += s_n[4][s] * mnode;
                 Sacc[2] += s_n[5][s] * mnode;
             }
+            }
             double vnew[3] = {0., 0., 0.};
 #pragma unroll
             for (int c = 0; c < DIM; c++) {
@@ -780,10 +1103,17 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
             if (r0 >= r1) break;
             if ((int)(threadIdx.x / HALF) == h && k < c1) {
                 const int i = threadIdx.x - h * HALF;
+                if (CP) {
+#pragma unroll
+                    for (int c = 0; c < 8; c++)
+#pragma unroll
+                        for (int a = 0; a < 3; a++) { s_cxi[i][c][a] = cc.xi[c][a]; s_coff[i][c][a] = cc.off[c][a]; }
+                } else {
 #pragma unroll
                 for (int a = 0; a < DIM; a++)
 #pragma unroll
                     for (int o = 0; o < 4; o++) s_S[i][a][o] = S[a][o];
+                }
 #pragma unroll
                 for (int q = 0; q < 4; q++) s_q[i][q] = q4[q];
             }
@@ -791,7 +1121,12 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
             wave_cells<DIM, NQ_MOM>(s_cs, r0, r1, s_acc,
                 [&](int lq, int ix, int iy, int iz) {
                     RecMass r;
-                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                    if (CP) {
+                        r.wx = cpdi_w<DIM>(s_cxi[lq], s_coff[lq], ix - 1, iy - 1, iz - 1);
+                        r.wy = 1.; r.wz = 1.;
+                    } else {
+                        r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                    }
                     r.mp = s_q[lq][0]; r.vx = s_q[lq][1]; r.vy = s_q[lq][2]; r.vz = s_q[lq][3];
                     return r;
                 },
@@ -851,7 +1186,7 @@ __device__ __forceinline__ bool pt_in_elem(const KGrid &g, int ei, int ej, int e
 
 // USL strain update (second half step) and ResetElementsTask; also the cell key and histogram
 // of the next binning
-template <int DIM, int MAT>
+template <int DIM, int MAT, int SH>
 __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
                                                   KStats st) {
     using G = TileGeom<DIM>;
@@ -876,12 +1211,8 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
         int e = P.elem[p];
         int ei, ej, ek;
         elem_ijk(g, e, ei, ej, ek);
-        double xi[3], invd[3], S[3][4], dS[3][4];
-        particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
-                         DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
-        particle_factors<DIM>(g, P, p, xi, S, dS);
         double L[3][3];
-        velocity_gradient<DIM>(S, dS, invd, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+        particle_gradient<DIM, SH>(g, P, p, ei, ej, ek, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
         scale_gradient<DIM>(L, g.strainLast);
         const MpmMaterial &m = mats[P.mat[p]];
         const int mtype = MAT ? MAT : m.type;
@@ -975,14 +1306,18 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
 // shape functions. The displacement uses the new positions in the USL pass.
 struct RecContact { double wx, wy, wz, dx, dy, dz, vp, mp, ux, uy, uz; };
 
-template <int DIM>
+template <int DIM, int SH>
 __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
                                                     int newPositions) {
     using G = TileGeom<DIM>;
-    constexpr int CH = NT / 2;
+    constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
+    constexpr int CH = CP ? 32 : NT / 2;
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_S[CH][DIM][4];
-    __shared__ double s_dS[CH][DIM][4];
+    __shared__ double s_S[CP ? 1 : CH][DIM][4];
+    __shared__ double s_dS[CP ? 1 : CH][DIM][4];
+    __shared__ double s_cxi[CP ? CH : 1][8][3];
+    __shared__ double s_cwg[CP ? CH : 1][8][3];
+    __shared__ int s_coff[CP ? CH : 1][8][3];
     __shared__ double s_q[CH][5];              // Vp, mp, displacement
     __shared__ double s_acc[NW][NQ_CONTACT][G::FOOT];
     __shared__ double s_id[DIM][G::TX];
@@ -1008,16 +1343,29 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, K
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
             double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-            double xi[3], invd[3], S[3][4], dS[3][4];
-            particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
-            particle_factors<DIM>(g, P, p, xi, S, dS);
+            if (CP) {
+                CpdiCorners cc;
+                cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
 #pragma unroll
-            for (int a = 0; a < DIM; a++)
+                for (int c = 0; c < 8; c++)
 #pragma unroll
-                for (int o = 0; o < 4; o++) {
-                    s_S[threadIdx.x][a][o] = S[a][o];
-                    s_dS[threadIdx.x][a][o] = dS[a][o];
-                }
+                    for (int a = 0; a < 3; a++) {
+                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
+                        s_cwg[threadIdx.x][c][a] = cc.wg[c][a];
+                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
+                    }
+            } else {
+                double xi[3], invd[3], S[3][4], dS[3][4];
+                particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+                particle_factors<DIM>(g, P, p, xi, S, dS);
+#pragma unroll
+                for (int a = 0; a < DIM; a++)
+#pragma unroll
+                    for (int o = 0; o < 4; o++) {
+                        s_S[threadIdx.x][a][o] = S[a][o];
+                        s_dS[threadIdx.x][a][o] = dS[a][o];
+                    }
+            }
             double mp = P.f[MPF_MASS][p];
             double rho = mats[P.mat[p]].rho;
             double Vp;
@@ -1046,10 +1394,17 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, K
         wave_cells<DIM, NQ_CONTACT>(s_cs, c0, min(c0 + CH, te), s_acc,
             [&](int lq, int ix, int iy, int iz) {
                 RecContact r;
-                r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
-                r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
-                if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
-                else { r.wz = 1.; r.dz = 0.; }
+                if (CP) {
+                    double gr[3];
+                    cpdi_node<DIM>(s_cxi[lq], s_cwg[lq], s_coff[lq], 0, ix - 1, iy - 1, iz - 1, r.wx, gr);
+                    r.wy = r.wz = 1.;
+                    r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
+                } else {
+                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
+                    r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
+                    if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
+                    else { r.wz = 1.; r.dz = 0.; }
+                }
                 r.vp = s_q[lq][0]; r.mp = s_q[lq][1];
                 r.ux = s_q[lq][2]; r.uy = s_q[lq][3]; r.uz = s_q[lq][4];
                 return r;
@@ -1058,7 +1413,9 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, K
                 bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
                 double vp = on ? r.vp : 0., mp = on ? r.mp : 0.;
                 double fn, gx, gy, gz;
-                if (DIM == 3) {
+                if (CP) {
+                    fn = r.wx; gx = r.dx; gy = r.dy; gz = DIM == 3 ? r.dz : 0.;
+                } else if (DIM == 3) {
                     fn = r.wx * r.wy * r.wz;
                     gx = r.dx * r.wy * r.wz * s_id[0][lx];
                     gy = r.wx * r.dy * r.wz * s_id[1][ly];
@@ -1291,10 +1648,17 @@ void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, 
 }
 
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                     const MpmMaterial *mats, hipStream_t s) {
+                     const MpmMaterial *mats, const KStats &st, hipStream_t s) {
     (void)mats;
-    if (dim == 3) hipLaunchKernelGGL(k_p2g_mass<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
-    else hipLaunchKernelGGL(k_p2g_mass<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
+    dim3 grid(g.ntiles * g.nfields), block(NT);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) hipLaunchKernelGGL((k_p2g_mass<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, st);
+        else hipLaunchKernelGGL((k_p2g_mass<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, st);
+    } else {
+        if (cp) hipLaunchKernelGGL((k_p2g_mass<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, st);
+        else hipLaunchKernelGGL((k_p2g_mass<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, st);
+    }
 }
 
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
@@ -1302,32 +1666,46 @@ void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
     else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
-template <int DIM>
+template <int DIM, int SH>
 static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                     const MpmMaterial *mats, hipStream_t s) {
+    dim3 grid(g.ntiles * g.nfields), block(NT);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
         case MPMGPU_MAT_MOONEY:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_MOONEY, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
         default:
-            hipLaunchKernelGGL((k_usf<DIM, 0>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats); break;
+            hipLaunchKernelGGL((k_usf<DIM, 0, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
     }
 }
 
 void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                 const MpmMaterial *mats, hipStream_t s) {
-    if (dim == 3) usf_mat<3>(g.matType, g, p, b, nd, mats, s);
-    else usf_mat<2>(g.matType, g, p, b, nd, mats, s);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) usf_mat<3, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, s);
+        else usf_mat<3, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, s);
+    } else {
+        if (cp) usf_mat<2, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, s);
+        else usf_mat<2, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, s);
+    }
 }
 
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_p2g_force<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
-    else hipLaunchKernelGGL(k_p2g_force<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
+    dim3 grid(g.ntiles * g.nfields), block(NT);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) hipLaunchKernelGGL((k_p2g_force<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
+        else hipLaunchKernelGGL((k_p2g_force<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+    } else {
+        if (cp) hipLaunchKernelGGL((k_p2g_force<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
+        else hipLaunchKernelGGL((k_p2g_force<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+    }
 }
 
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
@@ -1336,8 +1714,15 @@ void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd
 }
 
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_g2p_p2g<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
-    else hipLaunchKernelGGL(k_g2p_p2g<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd);
+    dim3 grid(g.ntiles * g.nfields), block(NT);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) hipLaunchKernelGGL((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
+        else hipLaunchKernelGGL((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+    } else {
+        if (cp) hipLaunchKernelGGL((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
+        else hipLaunchKernelGGL((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+    }
 }
 
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
@@ -1345,34 +1730,48 @@ void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
     else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
-template <int DIM>
+template <int DIM, int SH>
 static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                           const MpmMaterial *mats, const KStats &st, hipStream_t s) {
+    dim3 grid(g.ntiles * g.nfields), block(NT);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
         case MPMGPU_MAT_MOONEY:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_MOONEY>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_MOONEY, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
         default:
-            hipLaunchKernelGGL((k_usl_reset<DIM, 0>), dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, st); break;
+            hipLaunchKernelGGL((k_usl_reset<DIM, 0, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
     }
 }
 
 void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                       const MpmMaterial *mats, const KStats &st, hipStream_t s) {
-    if (dim == 3) usl_reset_mat<3>(g.matType, g, p, b, nd, mats, st, s);
-    else usl_reset_mat<2>(g.matType, g, p, b, nd, mats, st, s);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) usl_reset_mat<3, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, st, s);
+        else usl_reset_mat<3, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, st, s);
+    } else {
+        if (cp) usl_reset_mat<2, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, st, s);
+        else usl_reset_mat<2, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, st, s);
+    }
 }
 
 void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                         const MpmMaterial *mats, bool newPositions, hipStream_t s) {
     int np = newPositions ? 1 : 0;
-    if (dim == 3) hipLaunchKernelGGL(k_p2g_contact<3>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, np);
-    else hipLaunchKernelGGL(k_p2g_contact<2>, dim3(g.ntiles * g.nfields), dim3(NT), 0, s, g, p, b, nd, mats, np);
+    dim3 grid(g.ntiles * g.nfields), block(NT);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) hipLaunchKernelGGL((k_p2g_contact<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, mats, np);
+        else hipLaunchKernelGGL((k_p2g_contact<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, mats, np);
+    } else {
+        if (cp) hipLaunchKernelGGL((k_p2g_contact<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, mats, np);
+        else hipLaunchKernelGGL((k_p2g_contact<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, mats, np);
+    }
 }
 
 void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {

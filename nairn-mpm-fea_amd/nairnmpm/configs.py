@@ -28,7 +28,7 @@ class SplitMix64:
         return (self.next_u64() >> 11) * (1.0 / (1 << 53))
 
 
-def _header(analysis: int, cfl: float, max_time: float, extra: str = "") -> str:
+def _header(analysis: int, cfl: float, max_time: float, extra: str = "", shape: str = "uGIMP") -> str:
     return f"""<?xml version='1.0'?>
 <JANFEAInput version='3'>
   <Header>
@@ -37,14 +37,15 @@ def _header(analysis: int, cfl: float, max_time: float, extra: str = "") -> str:
   </Header>
   <MPMHeader>
     <Timing CFL="{cfl!r}" max="{max_time!r}"/>
-    <GIMP type="uGIMP"/>
+    <GIMP type="{shape}"/>
 {extra}  </MPMHeader>
 """
 
 
 def elastic_block_2d(cells: int = 64, block: int = 32, seed: int = 20260101, E: float = 1000.,
                      nu: float = 0.33, rho: float = 1.0, vmax: float = 10000., cfl: float = 0.2,
-                     jitter: float = 0.05, steps_hint: int = 1000, material_xml: str | None = None) -> str:
+                     jitter: float = 0.05, steps_hint: int = 1000, material_xml: str | None = None,
+                     shape: str = "uGIMP") -> str:
     """Plain 2D plane-strain elastic block: cells x cells grid of 1 mm cells, 2x2 particles per
     cell in the central block x block cells, each coordinate jittered by U(-jitter, jitter) mm
     with a SplitMix64 stream, v_x = vmax*sin(pi*(y-y0)/L), v_y = 0 (mm/s)."""
@@ -66,7 +67,7 @@ def elastic_block_2d(cells: int = 64, block: int = 32, seed: int = 20260101, E: 
     <nu>{nu!r}</nu>
   </Material>
 """
-    return (_header(10, cfl, 1.0) +
+    return (_header(10, cfl, 1.0, shape=shape) +
             f"""  <Mesh output="file">
     <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}">
       <Horiz cellsize="1"/>
@@ -83,12 +84,12 @@ def elastic_block_2d(cells: int = 64, block: int = 32, seed: int = 20260101, E: 
 
 def j2_block_3d(cells: int = 128, lo: int = 16, hi: int = 112, vx: float = 50000., E: float = 70000.,
                 nu: float = 0.33, yield_: float = 300., Ep: float = 1000., rho: float = 2.7,
-                cfl: float = 0.5, ppe: int = 8) -> str:
+                cfl: float = 0.5, ppe: int = 8, shape: str = "uGIMP") -> str:
     """3D elastic-plastic block: cells^3 grid of 1 mm cells; a box [lo, hi]^3 filled with ppe
     particles per cell of IsoPlasticity (J2, linear hardening); the half with x < mid moves at
     +vx, the other half at -vx (mm/s): two halves colliding."""
     mid = (lo + hi) / 2.0
-    return (_header(12, cfl, 1.0, f"    <MatlPtsPerElement>{ppe}</MatlPtsPerElement>\n") +
+    return (_header(12, cfl, 1.0, f"    <MatlPtsPerElement>{ppe}</MatlPtsPerElement>\n", shape=shape) +
             f"""  <Mesh output="file">
     <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}" zmin="0" zmax="{cells}">
       <Horiz cellsize="1"/>
@@ -128,7 +129,8 @@ def mooney_material_xml(G1: float = 8., G2: float = 2., K: float = 50., rho: flo
 
 
 def neohookean_block_3d(cells: int = 32, lo: int = 8, hi: int = 24, vx: float = 20000., G: float = 10.,
-                        K: float = 50., rho: float = 1.1, cfl: float = 0.5, material_xml: str | None = None) -> str:
+                        K: float = 50., rho: float = 1.1, cfl: float = 0.5, material_xml: str | None = None,
+                        shape: str = "uGIMP") -> str:
     """3D neo-Hookean block (C3/C5 material) with two colliding halves; material_xml replaces the
     material (e.g. mooney_material_xml())."""
     mid = (lo + hi) / 2.0
@@ -138,7 +140,7 @@ def neohookean_block_3d(cells: int = 32, lo: int = 8, hi: int = 24, vx: float = 
     <K>{K!r}</K>
   </Material>
 """
-    return (_header(12, cfl, 1.0, "    <MatlPtsPerElement>8</MatlPtsPerElement>\n") +
+    return (_header(12, cfl, 1.0, "    <MatlPtsPerElement>8</MatlPtsPerElement>\n", shape=shape) +
             f"""  <Mesh output="file">
     <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}" zmin="0" zmax="{cells}">
       <Horiz cellsize="1"/>

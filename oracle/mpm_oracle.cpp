@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <stdexcept>
 #include <vector>
 
 #include "../include/nairnmpm_gpu.h"
@@ -370,6 +371,10 @@ struct Oracle {
     double mtime = 0.;
     std::string err;
 
+    // lCPDI corners of the step (MPMBase CPDIDomain: inElem, ncpos, wg; ws = 1/8 or 1/4)
+    struct Cpdi { long elem[8]; double xi[8][3]; double wg[8][3]; };
+    std::vector<Cpdi> cpdi;
+
     double &F(int f, long p) { return pd[(size_t)f * np + p]; }
     int32_t &I(int f, long p) { return pi[(size_t)f * np + p]; }
     bool mm() const { return nf > 1; }
@@ -389,7 +394,143 @@ struct Oracle {
         return (f[0][0] * f[1][1] - f[1][0] * f[0][1]) * mp / rho;
     }
 
+    // MeshInfo::FindElementFromPoint for a regular grid (MeshInfo.cpp:593-633); -1 = left the grid
+    long find_element(const double *c) {
+        const MpmGridDesc &d = g.d;
+        long col = (long)(int)((c[0] - d.xmin) / d.dx);
+        if (col < 0 || col >= d.nx) { if (c[0] == d.xmin + d.nx * d.dx) col = d.nx - 1; else return -1; }
+        long row = (long)(int)((c[1] - d.ymin) / d.dy);
+        if (row < 0 || row >= d.ny) { if (c[1] == d.ymin + d.ny * d.dy) row = d.ny - 1; else return -1; }
+        if (g.dim == 3) {
+            long zr = (long)(int)((c[2] - d.zmin) / d.dz);
+            if (zr < 0 || zr >= d.nz) { if (c[2] == d.zmin + d.nz * d.dz) zr = d.nz - 1; else return -1; }
+            return (long)d.nx * (zr * d.ny + row) + col;
+        }
+        return row * d.nx + col;
+    }
+
+    // MatPoint3D::GetCPDINodesAndWeights (MatPoint3D.cpp:538-627), MatPoint2D (MatPoint2D.cpp:519-571)
+    // with GetSemiSideVectors (MatPoint3D.cpp:413-434, MatPoint2D.cpp:423-440); no rcrit rescaling
+    bool cpdi_init(long p) {
+        Mat3 pFm = get_F(p);
+        const double (*pF)[3] = pFm.m;
+        long i, j, k; g.elem_ijk(I(MPI_ELEM, p), i, j, k);
+        double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
+        Cpdi &cd = cpdi[p];
+        double psx = (g.xpts[i + 1] - g.xpts[i]) * (0.5 * F(MPF_LP_X, p));   // GetParticleSize
+        double psy = (g.ypts[j + 1] - g.ypts[j]) * (0.5 * F(MPF_LP_Y, p));
+        if (g.dim == 3) {
+            static const double r1s[8] = {-1., 1., 1., -1., -1., 1., 1., -1.};
+            static const double r2s[8] = {-1., -1., 1., 1., -1., -1., 1., 1.};
+            static const double r3s[8] = {-1., -1., -1., -1., 1., 1., 1., 1.};
+            double psz = (g.zpts[k + 1] - g.zpts[k]) * (0.5 * F(MPF_LP_Z, p));
+            double r1[3] = {pF[0][0] * psx, pF[1][0] * psx, pF[2][0] * psx};
+            double r2[3] = {pF[0][1] * psy, pF[1][1] * psy, pF[2][1] * psy};
+            double r3[3] = {pF[0][2] * psz, pF[1][2] * psz, pF[2][2] * psz};
+            double Vp = 8. * (r1[0] * (r2[1] * r3[2] - r2[2] * r3[1])
+                            + r1[1] * (r2[2] * r3[0] - r2[0] * r3[2])
+                            + r1[2] * (r2[0] * r3[1] - r2[1] * r3[0]));
+            for (int c = 0; c < 8; c++) {
+                double cx[3];
+                for (int a = 0; a < 3; a++) cx[a] = pos[a] + r1s[c] * r1[a] + r2s[c] * r2[a] + r3s[c] * r3[a];
+                cd.elem[c] = find_element(cx);
+                if (cd.elem[c] < 0) return false;
+                get_xi(g, cd.elem[c], cx, cd.xi[c]);
+            }
+            Vp = 1. / Vp;
+            const double x1 = r1[0], y1 = r1[1], z1 = r1[2], x2 = r2[0], y2 = r2[1], z2 = r2[2];
+            const double x3 = r3[0], y3 = r3[1], z3 = r3[2];
+            double (*w)[3] = cd.wg;
+            w[0][0] = (z1*y2 - y1*z2 - z1*y3 + z2*y3 + y1*z3 - y2*z3)*Vp;
+            w[0][1] = (-(z1*x2) + x1*z2 + z1*x3 - z2*x3 - x1*z3 + x2*z3)*Vp;
+            w[0][2] = (y1*x2 - x1*y2 - y1*x3 + y2*x3 + x1*y3 - x2*y3)*Vp;
+            w[1][0] = (z1*y2 - y1*z2 - z1*y3 - z2*y3 + y1*z3 + y2*z3)*Vp;
+            w[1][1] = (-(z1*x2) + x1*z2 + z1*x3 + z2*x3 - x1*z3 - x2*z3)*Vp;
+            w[1][2] = (y1*x2 - x1*y2 - y1*x3 - y2*x3 + x1*y3 + x2*y3)*Vp;
+            w[2][0] = (z1*y2 - y1*z2 + z1*y3 - z2*y3 - y1*z3 + y2*z3)*Vp;
+            w[2][1] = (-(z1*x2) + x1*z2 - z1*x3 + z2*x3 + x1*z3 - x2*z3)*Vp;
+            w[2][2] = (y1*x2 - x1*y2 + y1*x3 - y2*x3 - x1*y3 + x2*y3)*Vp;
+            w[3][0] = (z1*y2 - y1*z2 + z1*y3 + z2*y3 - y1*z3 - y2*z3)*Vp;
+            w[3][1] = (-(z1*x2) + x1*z2 - z1*x3 - z2*x3 + x1*z3 + x2*z3)*Vp;
+            w[3][2] = (y1*x2 - x1*y2 + y1*x3 + y2*x3 - x1*y3 - x2*y3)*Vp;
+            w[4][0] = (-(z1*y2) + y1*z2 - z1*y3 + z2*y3 + y1*z3 - y2*z3)*Vp;
+            w[4][1] = (z1*x2 - x1*z2 + z1*x3 - z2*x3 - x1*z3 + x2*z3)*Vp;
+            w[4][2] = (-(y1*x2) + x1*y2 - y1*x3 + y2*x3 + x1*y3 - x2*y3)*Vp;
+            w[5][0] = (-(z1*y2) + y1*z2 - z1*y3 - z2*y3 + y1*z3 + y2*z3)*Vp;
+            w[5][1] = (z1*x2 - x1*z2 + z1*x3 + z2*x3 - x1*z3 - x2*z3)*Vp;
+            w[5][2] = (-(y1*x2) + x1*y2 - y1*x3 - y2*x3 + x1*y3 + x2*y3)*Vp;
+            w[6][0] = (-(z1*y2) + y1*z2 + z1*y3 - z2*y3 - y1*z3 + y2*z3)*Vp;
+            w[6][1] = (z1*x2 - x1*z2 - z1*x3 + z2*x3 + x1*z3 - x2*z3)*Vp;
+            w[6][2] = (-(y1*x2) + x1*y2 + y1*x3 - y2*x3 - x1*y3 + x2*y3)*Vp;
+            w[7][0] = (-(z1*y2) + y1*z2 + z1*y3 + z2*y3 - y1*z3 - y2*z3)*Vp;
+            w[7][1] = (z1*x2 - x1*z2 - z1*x3 - z2*x3 + x1*z3 + x2*z3)*Vp;
+            w[7][2] = (-(y1*x2) + x1*y2 + y1*x3 + y2*x3 - x1*y3 - x2*y3)*Vp;
+        } else {
+            double r1[2] = {pF[0][0] * psx, pF[1][0] * psx};
+            double r2[2] = {pF[0][1] * psy, pF[1][1] * psy};
+            double Ap = 4. * (r1[0] * r2[1] - r1[1] * r2[0]);
+            double cs[4][3] = {{pos[0] - r1[0] - r2[0], pos[1] - r1[1] - r2[1], 0.},
+                               {pos[0] + r1[0] - r2[0], pos[1] + r1[1] - r2[1], 0.},
+                               {pos[0] + r1[0] + r2[0], pos[1] + r1[1] + r2[1], 0.},
+                               {pos[0] - r1[0] + r2[0], pos[1] - r1[1] + r2[1], 0.}};
+            for (int c = 0; c < 4; c++) {
+                cd.elem[c] = find_element(cs[c]);
+                if (cd.elem[c] < 0) return false;
+                get_xi(g, cd.elem[c], cs[c], cd.xi[c]);
+            }
+            Ap = 1. / Ap;
+            cd.wg[0][0] = (r1[1] - r2[1]) * Ap;
+            cd.wg[1][0] = (r1[1] + r2[1]) * Ap;
+            cd.wg[2][0] = (-r1[1] + r2[1]) * Ap;
+            cd.wg[3][0] = (-r1[1] - r2[1]) * Ap;
+            cd.wg[0][1] = (-r1[0] + r2[0]) * Ap;
+            cd.wg[1][1] = (-r1[0] - r2[0]) * Ap;
+            cd.wg[2][1] = (r1[0] - r2[0]) * Ap;
+            cd.wg[3][1] = (r1[0] + r2[0]) * Ap;
+            for (int c = 0; c < 4; c++) cd.wg[c][2] = 0.;
+        }
+        return true;
+    }
+
+    // ElementBase::GetCPDIFunctions (MoreMPMElementBase.cpp:581-656): linear element functions at
+    // each corner (GetShapeFunctionsForTractions, :91-108) weighted by ws and wg, merged per node
+    // in order of first appearance; functions below 1e-15 are dropped
+    void cpdi_shape(long p, bool deriv, ShapeOut &o) {
+        static const double xii[8] = {-1., 1., 1., -1., -1., 1., 1., -1.};
+        static const double eti[8] = {-1., -1., 1., 1., -1., -1., 1., 1.};
+        static const double zti[8] = {-1., -1., -1., -1., 1., 1., 1., 1.};
+        const Cpdi &cd = cpdi[p];
+        int ncorner = g.dim == 3 ? 8 : 4;
+        double ws = g.dim == 3 ? 0.125 : 0.25;
+        int ind = 0;
+        for (int c = 0; c < ncorner; c++) {
+            long ei, ej, ek; g.elem_ijk(cd.elem[c], ei, ej, ek);
+            const double *x = cd.xi[c];
+            for (int jn = 0; jn < ncorner; jn++) {
+                double cfn;
+                if (g.dim == 3) cfn = 0.125 * (1. + xii[jn] * x[0]) * (1. + eti[jn] * x[1]) * (1. + zti[jn] * x[2]);
+                else cfn = 0.25 * (1. + xii[jn] * x[0]) * (1. + eti[jn] * x[1]);
+                if (cfn < 1e-15) continue;
+                long node = g.node_index(ei + (xii[jn] > 0. ? 1 : 0), ej + (eti[jn] > 0. ? 1 : 0),
+                                         ek + (g.dim == 3 && zti[jn] > 0. ? 1 : 0));
+                int look = ind - 1;
+                for (; look >= 0; look--) if (o.node[look] == node) break;
+                if (look >= 0) {
+                    o.fn[look] += ws * cfn;
+                    if (deriv) { o.dx[look] += cd.wg[c][0] * cfn; o.dy[look] += cd.wg[c][1] * cfn; o.dz[look] += cd.wg[c][2] * cfn; }
+                } else {
+                    o.node[ind] = node;
+                    o.fn[ind] = ws * cfn;
+                    if (deriv) { o.dx[ind] = cd.wg[c][0] * cfn; o.dy[ind] = cd.wg[c][1] * cfn; o.dz[ind] = cd.wg[c][2] * cfn; }
+                    ind++;
+                }
+            }
+        }
+        o.n = ind;
+    }
+
     void shape(long p, bool deriv, ShapeOut &o) {
+        if (run.shape == MPMGPU_SHAPE_LCPDI) { cpdi_shape(p, deriv, o); return; }
         long e = I(MPI_ELEM, p);
         const double *x = &xi[3 * p];
         double lp[3] = {F(MPF_LP_X, p), F(MPF_LP_Y, p), F(MPF_LP_Z, p)};
@@ -410,7 +551,13 @@ struct Oracle {
         std::fill(cdisp.begin(), cdisp.end(), 0.);
         std::fill(cgrad.begin(), cgrad.end(), 0.);
         contactNodes.clear();
+        if (run.shape == MPMGPU_SHAPE_LCPDI && (long)cpdi.size() != np) cpdi.assign(np, Cpdi());
         for (long p = 0; p < np; p++) {
+            // ElementBase::GetShapeFunctionData (MoreMPMElementBase.cpp:50-58)
+            if (run.shape == MPMGPU_SHAPE_LCPDI && !cpdi_init(p)) {
+                err = "A CPDI particle domain node has left the grid";
+                throw std::runtime_error(err);
+            }
             double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
             get_xi(g, I(MPI_ELEM, p), pos, &xi[3 * p]);
         }
@@ -1191,13 +1338,18 @@ void oracle_destroy(void *h) { delete (Oracle *)h; }
 
 int oracle_step(void *h, int32_t nsteps) {
     Oracle *o = (Oracle *)h;
-    for (int s = 0; s < nsteps; s++) o->step();
+    try {
+        for (int s = 0; s < nsteps; s++) o->step();
+    } catch (const std::exception &) {
+        return 1;
+    }
     return 0;
 }
 
 /* run one task of the step (same ids as MPMGPU_TASK_*) */
 int oracle_run_task(void *h, int32_t task) {
     Oracle *o = (Oracle *)h;
+    try {
     switch (task) {
         case MPMGPU_TASK_INITIALIZATION: o->task_initialize(); break;
         case MPMGPU_TASK_MASS_MOMENTUM: o->task_mass_momentum(); break;
@@ -1211,6 +1363,9 @@ int oracle_run_task(void *h, int32_t task) {
         case MPMGPU_TASK_RESET_ELEMENTS:
             o->task_reset_elements(); o->steps++; o->mtime += o->run.timestep; break;
         default: return 1;
+    }
+    } catch (const std::exception &) {
+        return 1;
     }
     return 0;
 }
@@ -1234,6 +1389,28 @@ int oracle_get_grid(void *h, double *mass, double *pk, double *ftot, int32_t *np
     if (ftot) memcpy(ftot, o->ftot.data(), 3 * nn * sizeof(double));
     if (npts) memcpy(npts, o->npts.data(), nn * sizeof(int32_t));
     return 0;
+}
+
+/* lCPDI shape functions of one particle (unit tests): state holds pos, lp and F (ep/wr fields);
+   returns the node count or -1 when a corner leaves the grid */
+int oracle_cpdi_shape(const MpmGridDesc *grid, int64_t elem, const double *state, int32_t getDeriv,
+                      int64_t *nodes, double *fn, double *dx, double *dy, double *dz) {
+    Oracle o;
+    o.g.build(*grid);
+    o.run.shape = MPMGPU_SHAPE_LCPDI;
+    o.np = 1;
+    o.pd.assign(state, state + MPF_NUM_FIELDS);
+    o.pi.assign(MPI_NUM_FIELDS, 0);
+    o.pi[MPI_ELEM] = (int32_t)elem;
+    o.cpdi.assign(1, Oracle::Cpdi());
+    if (!o.cpdi_init(0)) return -1;
+    ShapeOut so;
+    o.cpdi_shape(0, getDeriv != 0, so);
+    for (int i = 0; i < so.n; i++) {
+        nodes[i] = so.node[i]; fn[i] = so.fn[i];
+        if (getDeriv) { dx[i] = so.dx[i]; dy[i] = so.dy[i]; dz[i] = so.dz[i]; }
+    }
+    return so.n;
 }
 
 /* contact extrapolations of every field (interleaved vectors); 1 in single-material mode */

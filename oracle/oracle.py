@@ -49,6 +49,8 @@ def lib():
         _lib.oracle_shape.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.c_int32, ctypes.c_int64, _PD, _PD,
                                       ctypes.c_int32, _PL, _PD, _PD, _PD, _PD]
         _lib.oracle_constitutive.argtypes = [ctypes.c_int32, ctypes.POINTER(nm.Material), _PD, _PD]
+        _lib.oracle_cpdi_shape.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.c_int64, _PD, ctypes.c_int32,
+                                           _PL, _PD, _PD, _PD, _PD]
     return _lib
 
 
@@ -76,10 +78,12 @@ class Oracle:
         return cls(g, mats, run, d, i)
 
     def step(self, n=1):
-        lib().oracle_step(self._h, n)
+        if lib().oracle_step(self._h, n) != 0:
+            raise RuntimeError("oracle step failed (a CPDI particle domain node has left the grid)")
 
     def run_task(self, t):
-        lib().oracle_run_task(self._h, t)
+        if lib().oracle_run_task(self._h, t) != 0:
+            raise RuntimeError("oracle task failed")
 
     def particles(self):
         import nairnmpm as nm
@@ -146,3 +150,15 @@ def constitutive(analysis, mat, du, state):
     d = np.ascontiguousarray(du, dtype=np.float64).reshape(9)
     lib().oracle_constitutive(analysis, ctypes.byref(mat), _d(d), _d(s))
     return s
+
+
+def cpdi_shape(grid, elem, state, deriv=True):
+    """lCPDI nodes and functions of one particle whose pos, lp and F are in the state vector."""
+    nodes = np.zeros(64, dtype=np.int64)
+    fn, dx, dy, dz = (np.zeros(64) for _ in range(4))
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    k = lib().oracle_cpdi_shape(ctypes.byref(grid), elem, _d(st), 1 if deriv else 0,
+                                nodes.ctypes.data_as(_PL), _d(fn), _d(dx), _d(dy), _d(dz))
+    if k < 0:
+        raise RuntimeError("a CPDI corner left the grid")
+    return nodes[:k], fn[:k], dx[:k], dy[:k], dz[:k]

@@ -207,3 +207,59 @@ def test_contact_task_by_task_equals_fused():
     da, _ = a.particles()
     db, _ = b.particles()
     assert np.array_equal(da, db)
+
+
+# ---- lCPDI -------------------------------------------------------------------------------
+
+def test_lcpdi_neohookean_3d():
+    prob, sim, ora = _pair(configs.neohookean_block_3d(cells=20, lo=4, hi=16, shape="lCPDI"))
+    done = 0
+    for target in (1, 30):
+        sim.step(target - done)
+        ora.step(target - done)
+        done = target
+        _check(sim, ora, 3, f"lCPDI NH 3D step {target}")
+
+
+def test_lcpdi_elastic_2d():
+    prob, sim, ora = _pair(configs.elastic_block_2d(cells=32, block=16, shape="lCPDI"))
+    done = 0
+    for target in (1, 100, 300):
+        sim.step(target - done)
+        ora.step(target - done)
+        done = target
+        _check(sim, ora, 2, f"lCPDI elastic 2D step {target}")
+
+
+def test_lcpdi_j2_3d():
+    prob, sim, ora = _pair(configs.j2_block_3d(cells=16, lo=4, hi=12, shape="lCPDI"))
+    sim.step(40)
+    ora.step(40)
+    _check(sim, ora, 3, "lCPDI J2 3D")
+
+
+@pytest.mark.parametrize("law_type, friction", [(61, 0.0), (60, 0.0)])
+def test_lcpdi_contact_cylinders(law_type, friction):
+    xml = configs.contact_cylinders_3d(cells=24, radius=5., length=8., gap=1., shape="lCPDI",
+                                       law_type=law_type, friction=friction)
+    prob, sim, ora = _pair(xml)
+    done = 0
+    for target in (1, 60):
+        sim.step(target - done)
+        ora.step(target - done)
+        done = target
+        _check(sim, ora, 3, f"lCPDI contact law {law_type} step {target}")
+
+
+def test_lcpdi_sliding_contact_friction():
+    xml = configs.contact_cylinders_3d(cells=24, radius=5., length=8., gap=1., shape="lCPDI")
+    prob = nm.Problem.from_xml(xml)
+    d, iv = prob.particles()
+    d[nm.F["vel_y"]][iv[1] == 1] = 6000.
+    g, mats, run, _, _ = prob.describe()
+    sim = nm.Simulation(g, mats, run)
+    sim.upload(d, iv)
+    ora = orc.Oracle(g, mats, run, d, iv)
+    sim.step(60)
+    ora.step(60)
+    _check(sim, ora, 3, "lCPDI sliding contact")

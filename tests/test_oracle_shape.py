@@ -143,3 +143,70 @@ def test_3d_z_factor_uses_lp_y():
     assert abs(fn.sum() - 1.) < 1e-14
     n, fn, *_ = orc.shape(g, nm.SHAPE_UGIMP, elem, pos, np.array([0.5, 0.3, 0.4]))
     assert abs(fn.sum() - 1.) > 1e-4
+
+
+def _cpdi_state(pos, lp, Fm):
+    s = np.zeros(nm.NUM_FIELDS)
+    F = nm.F
+    s[F["pos_x"]], s[F["pos_y"]], s[F["pos_z"]] = pos
+    s[F["lp_x"]], s[F["lp_y"]], s[F["lp_z"]] = lp
+    s[F["ep_xx"]], s[F["ep_yy"]], s[F["ep_zz"]] = Fm[0, 0] - 1., Fm[1, 1] - 1., Fm[2, 2] - 1.
+    s[F["ep_xy"]], s[F["wr_xy"]] = Fm[1, 0] + Fm[0, 1], Fm[1, 0] - Fm[0, 1]
+    s[F["ep_xz"]], s[F["wr_xz"]] = Fm[2, 0] + Fm[0, 2], Fm[2, 0] - Fm[0, 2]
+    s[F["ep_yz"]], s[F["wr_yz"]] = Fm[2, 1] + Fm[1, 2], Fm[2, 1] - Fm[1, 2]
+    return s
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lcpdi_reproduces_linear_fields(dim):
+    """lCPDI (Sadeghirad et al. 2011) for an arbitrary deformed parallelepiped: sum N = 1,
+    sum grad N = 0, sum N x_i = x_p and sum grad N_i (x) x_i = I. The last identity fixes every
+    gradient weight wg of MatPoint3D.cpp:571-594 (a wrong sign in one term breaks it)."""
+    analysis = nm.THREED if dim == 3 else nm.PLANE_STRAIN
+    g = grid(analysis)
+    rng = np.random.default_rng(21)
+    for trial in range(25):
+        e_ijk = (3, 4, 3)
+        elem = e_ijk[0] + g.nx * (e_ijk[1] + (g.ny * e_ijk[2] if dim == 3 else 0))
+        pos = np.array([g.xmin + (e_ijk[0] + rng.uniform(0.2, 0.8)) * g.dx,
+                        g.ymin + (e_ijk[1] + rng.uniform(0.2, 0.8)) * g.dy,
+                        g.zmin + (e_ijk[2] + rng.uniform(0.2, 0.8)) * g.dz if dim == 3 else 0.])
+        Fm = np.eye(3) + 0.3 * rng.uniform(-1, 1, (3, 3))
+        if dim == 2:
+            Fm[2, :2] = Fm[:2, 2] = 0.
+        lp = np.array([0.5, 0.5, 0.5 if dim == 3 else 1.])
+        nodes, fn, dx, dy, dz = orc.cpdi_shape(g, elem, _cpdi_state(pos, lp, Fm))
+        assert len(set(nodes.tolist())) == len(nodes)                  # merged, no duplicates
+        xyz = np.array([node_xyz(g, n) for n in nodes])
+        np.testing.assert_allclose(fn.sum(), 1., atol=1e-14)
+        np.testing.assert_allclose([dx.sum(), dy.sum(), dz.sum()], 0., atol=1e-13)
+        np.testing.assert_allclose(fn @ xyz[:, :dim], pos[:dim], atol=1e-13)
+        grads = np.stack([dx, dy, dz])[:dim]
+        np.testing.assert_allclose(grads @ xyz[:, :dim], np.eye(dim), atol=1e-12)
+
+
+def test_lcpdi_undeformed_is_trilinear_average():
+    """With F = I the particle domain is the box pos +- lp/2 cells and lCPDI's function of a node
+    is the average of the node's trilinear function over the 8 corners (ws = 1/8)."""
+    g = grid(nm.THREED)
+    elem = 3 + g.nx * (3 + g.ny * 3)
+    pos = np.array([g.xmin + 3.6, g.ymin + 3.3, g.zmin + 3.45])
+    nodes, fn, *_ = orc.cpdi_shape(g, elem, _cpdi_state(pos, [0.5, 0.5, 0.5], np.eye(3)))
+    r = 0.25
+    expect = {}
+    for sx in (-1, 1):
+        for sy in (-1, 1):
+            for sz in (-1, 1):
+                c = pos + r * np.array([sx, sy, sz])
+                ijk = np.floor((c - [g.xmin, g.ymin, g.zmin]) / g.dx).astype(int)
+                loc = (c - [g.xmin, g.ymin, g.zmin]) / g.dx - ijk
+                for a in (0, 1):
+                    for b in (0, 1):
+                        for d in (0, 1):
+                            wgt = (loc[0] if a else 1 - loc[0]) * (loc[1] if b else 1 - loc[1]) * (loc[2] if d else 1 - loc[2])
+                            n = (ijk[0] + a) + (g.nx + 1) * ((ijk[1] + b) + (g.ny + 1) * (ijk[2] + d))
+                            expect[n] = expect.get(n, 0.) + wgt / 8.
+    got = dict(zip(nodes.tolist(), fn))
+    for n, w in expect.items():
+        if w > 1e-14:
+            np.testing.assert_allclose(got[n], w, atol=1e-15)
