@@ -37,7 +37,11 @@ ALG_BYTES = {
     "k_node_mom2": (0, 24 + 24),
     "k_usl_reset": (24 + 24 + 4 + 2 * 184 + 8, 24),   # pos, new pos, elem, state r/w, elem+key out; v in
     "binning": (16, 0),
+    # multimaterial: two contact extrapolations (pos, orig, F, mass, elem in; volume, displacement,
+    # gradient out per node) and three node-local contact passes
+    "contact": (2 * (24 + 24 + 72 + 8 + 4), 2 * 56 + 3 * 160),
 }
+CPDI_EXTRA = 72      # lCPDI: semi-side vectors read by every tile kernel (written by the first)
 
 
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_j2_block_128.md")
@@ -84,6 +88,8 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["j2", "contact"], default="j2",
+                    help="j2: the 7.08M-particle J2 GIMP block (default); contact: the lCPDI contact pair")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy used to scale the HBM byte counters")
     a = ap.parse_args()
@@ -97,7 +103,19 @@ def main():
 
     lo, hi = a.cells // 8, a.cells - a.cells // 8
     t0 = time.time()
-    prob = nm.Problem.from_xml(configs.j2_block_3d(cells=a.cells, lo=lo, hi=hi))
+    if a.workload == "contact":
+        # two neo-Hookean cylinders (radius 20, length 60 cells) closing at 20 m/s, lCPDI,
+        # multimaterial contact with Coulomb friction 0.3, AVGG normals
+        cells = a.cells if a.cells != 128 else 160
+        xml = configs.contact_cylinders_3d(cells=cells, radius=cells / 8., length=cells * 3. / 8., gap=4. * cells / 160.)
+        workload = (f"3D two-body contact: neo-Hookean cylinders r={cells / 8:g} L={cells * 3 / 8:g} cells on a "
+                    f"{cells}^3 grid, lCPDI, Coulomb mu=0.3, closing at 20 m/s")
+        a.cells = cells
+    else:
+        xml = configs.j2_block_3d(cells=a.cells, lo=lo, hi=hi)
+        workload = (f"3D J2 GIMP block, {a.cells}^3 cells of 1 mm, 2x2x2 particles per cell "
+                    f"in [{lo},{hi}]^3 mm, halves colliding at +-50 m/s")
+    prob = nm.Problem.from_xml(xml)
     grid, mats, run, npart, _ = prob.describe()
     dfields, ifields = prob.particles()
     setup_s = time.time() - t0
@@ -108,8 +126,8 @@ def main():
     sim.upload(dfields, ifields)
     sim.step(a.warmup)
     sim.sync()
-    mass, _, _ = sim.grid_values()
-    active_nodes = int(np.count_nonzero(mass))
+    mass, _, _ = sim.grid_fields()
+    active_nodes = int(np.count_nonzero(mass))        # (field, node) pairs with mass
 
     # timed region: K steps, HIP events between kernel groups on the simulation's stream
     sim.set_profiling(True)
@@ -138,6 +156,8 @@ def main():
     dom = max(times, key=lambda g: times[g][0])
     dom_ms = times[dom][0] / max(times[dom][1], 1)
     bp, bn = ALG_BYTES[dom]
+    if a.workload == "contact" and dom not in ("k_node_mass", "k_node_force", "k_node_mom2", "binning"):
+        bp += CPDI_EXTRA
     alg_bytes = bp * npart + bn * active_nodes
     achieved = alg_bytes / (dom_ms * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -170,10 +190,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3D J2 GIMP block, {a.cells}^3 cells of 1 mm, 2x2x2 particles per cell "
-                               f"in [{lo},{hi}]^3 mm, halves colliding at +-50 m/s",
+        "config": {"workload": workload,
                    "particles": npart, "grid_cells": [grid.nx, grid.ny, grid.nz], "active_nodes": active_nodes,
-                   "material": "IsoPlasticity J2, linear hardening", "shape": "uGIMP", "method": "USAVG+ FLIP",
+                   "material": "IsoPlasticity J2, linear hardening" if a.workload == "j2" else "Neohookean x2 + CoulombFriction",
+                   "shape": "uGIMP" if a.workload == "j2" else "lCPDI", "method": "USAVG+ FLIP",
                    "graph_replay_value": graph_value},
         "roofline": roofline,
         "cpu_baseline": cpu,
