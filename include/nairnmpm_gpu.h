@@ -202,6 +202,24 @@ void mpmgpu_problem_destroy(MpmProblem *prob);
 /* Create a device context for one simulation on the current HIP device. */
 int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat,
                   const MpmRunParams *run, MpmCtx **out);
+/* ---- batches of independent simulations ----------------------------------------------- */
+/* No counterpart in the reference (one analysis per process): many independent instances, each
+   with its own grid, time step, materials and particles, advance together in one context. Every
+   step launches each kernel once for all instances. Instance results equal those of the same
+   instance run alone, bit for bit (each keeps its own coordinates and sums in the same order). */
+#define MPMGPU_MAX_BATCH_MATERIALS 65536
+typedef struct MpmInstanceDesc {
+    MpmGridDesc grid;          /* the instance's own grid (all instances: same analysis type) */
+    MpmRunParams run;          /* its time step; same shape functions for all; no multimaterial */
+    int32_t firstMaterial;     /* its materials: mats[firstMaterial .. +numMaterials) */
+    int32_t numMaterials;
+} MpmInstanceDesc;
+int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *inst, const MpmMaterial *mats, int32_t nmat,
+                        MpmCtx **ctx);
+/* A batch takes its particles through mpmgpu_upload_particles: MPI_MAT holds batch material
+   indices (which name the instance), MPI_ELEM the element in the instance's own grid, and
+   positions are in the instance's coordinates. */
+
 /* Upload particles (layout as mpmgpu_problem_particles); replaces the mpm[] globals
    (MPM_Classes/MPMBase.hpp:269-270). */
 int mpmgpu_upload_particles(MpmCtx *ctx, int64_t n, const double *dfields, const int32_t *ifields);

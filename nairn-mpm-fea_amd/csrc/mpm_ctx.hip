@@ -34,6 +34,11 @@ struct MpmCtx {
     double *pbuf = nullptr;           // particle double fields + second position buffer
     int *ibuf = nullptr;              // particle int fields
     double *cbuf = nullptr;           // lCPDI semi-side vectors of the step
+    // batch of independent instances (empty for one simulation)
+    std::vector<KGrid> inst;          // host copies of the instance grids
+    std::vector<int> matInstH;        // material -> instance
+    KGrid *dinst = nullptr;
+    int *dtables = nullptr;           // tileInst, nodeBlockInst, matInst
     int *bbuf = nullptr;              // binning arrays
     unsigned long long *sbuf = nullptr;
     long long ncap = 0;
@@ -322,69 +327,58 @@ void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 
 // ---- device context --------------------------------------------------------------------
 
-int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat, const MpmRunParams *run,
-                  MpmCtx **out) {
-    *out = nullptr;
+static const char *check_run(const MpmGridDesc *grid, const MpmRunParams *run) {
     if (grid->analysis != MPMGPU_PLANE_STRAIN && grid->analysis != MPMGPU_THREED)
-        return fail_global("analysis type must be 10 (plane strain) or 12 (3D)");
+        return "analysis type must be 10 (plane strain) or 12 (3D)";
     if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI)
-        return fail_global("shape functions must be uGIMP (1) or lCPDI (10) on the GPU path");
-    if (run->mpmMethod != 2) return fail_global("only the USAVG method is implemented");
-    int nf = run->multiMaterial ? run->numFields : 1;
-    if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return fail_global("bad number of material velocity fields");
-    if (nf > 1) {
-        if (run->contactLaw < 0 || run->contactLaw >= nmat ||
-            (mats[run->contactLaw].type != MPMGPU_MAT_COULOMB && mats[run->contactLaw].type != MPMGPU_MAT_CONTACT))
-            return fail_global("multimaterial mode needs a contact law material");
-        if (run->normalsMethod < 0 || run->normalsMethod > 3)
-            return fail_global("contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG)");
-    }
-    if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return fail_global("bad material count");
+        return "shape functions must be uGIMP (1) or lCPDI (10) on the GPU path";
+    if (run->mpmMethod != 2) return "only the USAVG method is implemented";
+    return nullptr;
+}
+
+static const char *check_materials(const MpmMaterial *mats, int nmat) {
     for (int i = 0; i < nmat; i++) {
         int t = mats[i].type;
         if (t != MPMGPU_MAT_ISOTROPIC && t != MPMGPU_MAT_ISOPLASTICITY && t != MPMGPU_MAT_NEOHOOKEAN &&
-            t != MPMGPU_MAT_MOONEY &&
-            t != MPMGPU_MAT_COULOMB && t != MPMGPU_MAT_CONTACT)
-            return fail_global("material type " + std::to_string(t) + " is not implemented on the GPU path");
+            t != MPMGPU_MAT_MOONEY && t != MPMGPU_MAT_COULOMB && t != MPMGPU_MAT_CONTACT)
+            return "a material type is not implemented on the GPU path";
     }
-    MpmCtx *c = new MpmCtx();
-    c->dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
-    c->gd = *grid;
-    c->run = *run;
-    c->mats.assign(mats, mats + nmat);
-    KGrid &g = c->kg;
-    g.dim = c->dim;
-    g.nx = grid->nx; g.ny = grid->ny; g.nz = c->dim == 3 ? grid->nz : 1;
-    g.nnx = g.nx + 1; g.nny = g.ny + 1; g.nnz = c->dim == 3 ? g.nz + 1 : 1;
-    int tx, ty, tz, nc;
-    if (c->dim == 3) { tx = TileGeom<3>::TX; ty = TileGeom<3>::TY; tz = TileGeom<3>::TZ; nc = TileGeom<3>::NC; }
-    else { tx = TileGeom<2>::TX; ty = TileGeom<2>::TY; tz = 1; nc = TileGeom<2>::NC; }
+    return nullptr;
+}
+
+// geometry, tiles and time step of one grid
+static void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run) {
+    memset(&g, 0, sizeof(g));
+    g.dim = dim;
+    g.nx = grid->nx; g.ny = grid->ny; g.nz = dim == 3 ? grid->nz : 1;
+    g.nnx = g.nx + 1; g.nny = g.ny + 1; g.nnz = dim == 3 ? g.nz + 1 : 1;
+    int tx, ty, tz;
+    if (dim == 3) { tx = TileGeom<3>::TX; ty = TileGeom<3>::TY; tz = TileGeom<3>::TZ; }
+    else { tx = TileGeom<2>::TX; ty = TileGeom<2>::TY; tz = 1; }
     g.ntx = (g.nx + tx - 1) / tx; g.nty = (g.ny + ty - 1) / ty; g.ntz = (g.nz + tz - 1) / tz;
     g.ntiles = g.ntx * g.nty * g.ntz;
     g.nnodes = (long long)g.nnx * g.nny * g.nnz;
+    g.nstride = g.nnodes;
+    g.tilesTotal = g.ntiles;
     g.xmin = grid->xmin; g.ymin = grid->ymin; g.zmin = grid->zmin;
     g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
     g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
-    g.nfields = nf;
+    g.nfields = 1;
     g.shape = run->shape;
-    for (int m = 0; m < MPMGPU_MAX_MATERIALS; m++)
-        g.fieldOf[m] = (nf > 1 && m < nmat && mats[m].field >= 0 && mats[m].field < nf) ? mats[m].field : 0;
-    if (nf > 1) {
-        const MpmMaterial &law = mats[run->contactLaw];
-        g.normals = run->normalsMethod;
-        g.ignoreContact = law.type == MPMGPU_MAT_CONTACT ? 1 : 0;
-        g.friction = law.friction;
-        g.frictionStatic = law.frictionStatic;
-    }
+    g.uniformLp = 0;
+}
+
+// device buffers of the grid, bins and counters for c->kg (nstride nodes, tilesTotal tiles per field)
+static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
+    KGrid &g = c->kg;
+    const int nf = g.nfields;
     int foot = c->dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
-    long long nn = g.nnodes;
-    c->kb.nkeys = nf * g.ntiles * nc;
+    int nc = c->dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
+    long long nn = g.nstride;
+    c->kb.nkeys = nf * g.tilesTotal * nc;
     hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
-        std::string m = std::string("hipStreamCreate: ") + hipGetErrorString(e);
-        delete c;
-        return fail_global(m);
-    }
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail_global(std::string("hipStreamCreate: ") + hipGetErrorString(e));
     // per field: mass, pk, vk, ftot (+ contact volume, displacement, gradient in multimaterial mode)
     const int perField = 1 + 3 + 3 + 3 + (nf > 1 ? 1 + 3 + 3 : 0);
     size_t nodeBytes = (size_t)nf * nn * perField * sizeof(double);
@@ -392,12 +386,10 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     int *cntMem = nullptr;
     double *partMem = nullptr;
     if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nf * nn * sizeof(int)) != hipSuccess ||
-        hipMalloc(&partMem, (size_t)nf * g.ntiles * NQMAX * foot * sizeof(double)) != hipSuccess ||
+        hipMalloc(&partMem, (size_t)nf * g.tilesTotal * NQMAX * foot * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dmats, nmat * sizeof(MpmMaterial)) != hipSuccess ||
-        hipMalloc(&c->sbuf, 3 * sizeof(unsigned long long)) != hipSuccess) {
-        delete c;
+        hipMalloc(&c->sbuf, 3 * sizeof(unsigned long long)) != hipSuccess)
         return fail_global("out of device memory allocating the grid");
-    }
     long long fnn = (long long)nf * nn;
     c->kn.mass = nodeMem;
     c->kn.pk = nodeMem + fnn;
@@ -421,10 +413,8 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums
     long long nk = c->kb.nkeys;
     long long nblk = (nk + 1 + 1023) / 1024 + 1;
-    if (hipMalloc(&c->bbuf, (size_t)(3 * nk + 2 + nblk) * sizeof(int)) != hipSuccess) {
-        delete c;
+    if (hipMalloc(&c->bbuf, (size_t)(3 * nk + 2 + nblk) * sizeof(int)) != hipSuccess)
         return fail_global("out of device memory allocating the bins");
-    }
     c->kb.cellStart = c->bbuf;
     c->kb.count = c->bbuf + nk + 1;
     c->kb.cursor = c->bbuf + 2 * nk + 2;
@@ -432,10 +422,111 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     (void)hipMemsetAsync(c->bbuf, 0, (size_t)(3 * nk + 2 + nblk) * sizeof(int), c->stream);
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) {
-        delete c;
-        return fail_global("device initialisation failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail_global("device initialisation failed");
+    return 0;
+}
+
+int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat, const MpmRunParams *run,
+                  MpmCtx **out) {
+    *out = nullptr;
+    if (const char *m = check_run(grid, run)) return fail_global(m);
+    int nf = run->multiMaterial ? run->numFields : 1;
+    if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return fail_global("bad number of material velocity fields");
+    if (nf > 1) {
+        if (run->contactLaw < 0 || run->contactLaw >= nmat ||
+            (mats[run->contactLaw].type != MPMGPU_MAT_COULOMB && mats[run->contactLaw].type != MPMGPU_MAT_CONTACT))
+            return fail_global("multimaterial mode needs a contact law material");
+        if (run->normalsMethod < 0 || run->normalsMethod > 3)
+            return fail_global("contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG)");
     }
+    if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return fail_global("bad material count");
+    if (const char *m = check_materials(mats, nmat)) return fail_global(m);
+    MpmCtx *c = new MpmCtx();
+    c->dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
+    c->gd = *grid;
+    c->run = *run;
+    c->mats.assign(mats, mats + nmat);
+    KGrid &g = c->kg;
+    fill_grid(g, c->dim, grid, run);
+    g.nfields = nf;
+    for (int m = 0; m < MPMGPU_MAX_MATERIALS; m++)
+        g.fieldOf[m] = (nf > 1 && m < nmat && mats[m].field >= 0 && mats[m].field < nf) ? mats[m].field : 0;
+    if (nf > 1) {
+        const MpmMaterial &law = mats[run->contactLaw];
+        g.normals = run->normalsMethod;
+        g.ignoreContact = law.type == MPMGPU_MAT_CONTACT ? 1 : 0;
+        g.friction = law.friction;
+        g.frictionStatic = law.frictionStatic;
+    }
+    if (alloc_context(c, mats, nmat)) { mpmgpu_destroy(c); return 1; }
+    *out = c;
+    return 0;
+}
+
+// Independent instances (each its own grid, time step, materials and particles) stepped together:
+// their tiles and nodes are laid out one after another in one context, every kernel looks up the
+// instance of its tile or node, and one launch per kernel serves all of them.
+int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMaterial *mats, int32_t nmat,
+                        MpmCtx **out) {
+    *out = nullptr;
+    if (ninst < 1) return fail_global("a batch needs at least one instance");
+    if (nmat < 1 || nmat > MPMGPU_MAX_BATCH_MATERIALS) return fail_global("bad material count");
+    if (const char *m = check_materials(mats, nmat)) return fail_global(m);
+    int analysis = desc[0].grid.analysis, shape = desc[0].run.shape;
+    std::vector<int> matInst(nmat, -1);
+    for (int i = 0; i < ninst; i++) {
+        const MpmInstanceDesc &d = desc[i];
+        if (const char *m = check_run(&d.grid, &d.run)) return fail_global(m);
+        if (d.grid.analysis != analysis || d.run.shape != shape)
+            return fail_global("all instances of a batch need the same analysis type and shape functions");
+        if (d.run.multiMaterial) return fail_global("multimaterial instances cannot be batched");
+        if (d.firstMaterial < 0 || d.numMaterials < 1 || d.firstMaterial + d.numMaterials > nmat)
+            return fail_global("instance material range outside the material table");
+        for (int m = d.firstMaterial; m < d.firstMaterial + d.numMaterials; m++) {
+            if (matInst[m] >= 0) return fail_global("a material belongs to two instances");
+            matInst[m] = i;
+        }
+    }
+    MpmCtx *c = new MpmCtx();
+    c->dim = analysis == MPMGPU_THREED ? 3 : 2;
+    c->gd = desc[0].grid;
+    c->run = desc[0].run;
+    c->mats.assign(mats, mats + nmat);
+    c->matInstH = matInst;
+    c->inst.resize(ninst);
+    long long nodeAcc = 0;
+    int tileAcc = 0;
+    for (int i = 0; i < ninst; i++) {
+        KGrid &gi = c->inst[i];
+        fill_grid(gi, c->dim, &desc[i].grid, &desc[i].run);
+        gi.tileBase = tileAcc;
+        gi.nodeBase = nodeAcc;
+        tileAcc += gi.ntiles;
+        nodeAcc += (gi.nnodes + 255) / 256 * 256;
+    }
+    for (auto &gi : c->inst) { gi.nstride = nodeAcc; gi.tilesTotal = tileAcc; }
+    KGrid &g = c->kg;
+    g = c->inst[0];
+    g.nstride = nodeAcc;
+    g.tilesTotal = tileAcc;
+    if (alloc_context(c, mats, nmat)) { mpmgpu_destroy(c); return 1; }
+    // instance tables on the device
+    std::vector<int> tables;
+    tables.reserve(tileAcc + nodeAcc / 256 + nmat);
+    for (int i = 0; i < ninst; i++) tables.insert(tables.end(), c->inst[i].ntiles, i);
+    for (int i = 0; i < ninst; i++) tables.insert(tables.end(), (c->inst[i].nnodes + 255) / 256, i);
+    tables.insert(tables.end(), matInst.begin(), matInst.end());
+    if (hipMalloc(&c->dinst, ninst * sizeof(KGrid)) != hipSuccess ||
+        hipMalloc(&c->dtables, tables.size() * sizeof(int)) != hipSuccess) {
+        mpmgpu_destroy(c);
+        return fail_global("out of device memory allocating the batch tables");
+    }
+    (void)hipMemcpy(c->dtables, tables.data(), tables.size() * sizeof(int), hipMemcpyHostToDevice);
+    g.inst = c->dinst;
+    g.tileInst = c->dtables;
+    g.nodeBlockInst = c->dtables + tileAcc;
+    g.matInst = c->dtables + tileAcc + nodeAcc / 256;
+    (void)hipMemcpy(c->dinst, c->inst.data(), ninst * sizeof(KGrid), hipMemcpyHostToDevice);
     *out = c;
     return 0;
 }
@@ -469,11 +560,16 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
     c->kp.n = n;
     // validate materials and elements on the host before anything reaches the device
     int nmat = (int)c->mats.size();
+    const bool batch = !c->inst.empty();
     long long nelem = (long long)c->gd.nx * c->gd.ny * (c->dim == 3 ? c->gd.nz : 1);
     for (int64_t i = 0; i < n; i++) {
         int m = ifields[MPI_MAT * nn + i];
         long long e = ifields[MPI_ELEM * nn + i];
         if (m < 0 || m >= nmat) return c->fail("particle with an undefined material");
+        if (batch) {
+            const KGrid &gi = c->inst[c->matInstH[m]];
+            nelem = (long long)gi.nx * gi.ny * gi.nz;
+        }
         int t = c->mats[m].type;
         if (t == MPMGPU_MAT_COULOMB || t == MPMGPU_MAT_CONTACT) return c->fail("particle with a contact-law material");
         if (c->kg.nfields > 1 && (c->mats[m].field < 0 || c->mats[m].field >= c->kg.nfields))
@@ -496,6 +592,25 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         for (int64_t i = 0; i < n && c->kg.uniformLp; i++)
             for (int a = 0; a < 3; a++)
                 if (dfields[(MPF_LP_X + a) * nn + i] != c->kg.lp[a]) { c->kg.uniformLp = 0; break; }
+    }
+    if (batch) {
+        // the kernels read the instance grids: same size test per instance
+        std::vector<int> seen(c->inst.size(), 0);
+        for (auto &gi : c->inst) gi.uniformLp = 1;
+        for (int64_t i = 0; i < n; i++) {
+            KGrid &gi = c->inst[c->matInstH[ifields[MPI_MAT * nn + i]]];
+            int k = (int)(&gi - c->inst.data());
+            if (!seen[k]) {
+                seen[k] = 1;
+                for (int a = 0; a < 3; a++) gi.lp[a] = dfields[(MPF_LP_X + a) * nn + i];
+            } else if (gi.uniformLp) {
+                for (int a = 0; a < 3; a++)
+                    if (dfields[(MPF_LP_X + a) * nn + i] != gi.lp[a]) { gi.uniformLp = 0; break; }
+            }
+        }
+        for (auto &gi : c->inst) gi.matType = c->kg.matType;
+        HIPCHECK(c, hipMemcpyAsync(c->dinst, c->inst.data(), c->inst.size() * sizeof(KGrid), hipMemcpyHostToDevice,
+                                   c->stream));
     }
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -687,6 +802,8 @@ void mpmgpu_destroy(MpmCtx *c) {
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);
     if (c->cbuf) (void)hipFree(c->cbuf);
+    if (c->dinst) (void)hipFree(c->dinst);
+    if (c->dtables) (void)hipFree(c->dtables);
     if (c->bbuf) (void)hipFree(c->bbuf);
     if (c->kn.mass) (void)hipFree(c->kn.mass);
     if (c->kn.cnt) (void)hipFree(c->kn.cnt);

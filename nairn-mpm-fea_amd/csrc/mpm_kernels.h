@@ -42,6 +42,17 @@ struct KGrid {
     int ignoreContact;         // ContactLaw type 60: always move to the center-of-mass velocity
     double friction, frictionStatic;
     int shape;                 // MPMGPU_SHAPE_UGIMP or MPMGPU_SHAPE_LCPDI
+    // Several independent grids (a batch of instances) share the context's arrays: instance i
+    // owns nodes [nodeBase, nodeBase+nnodes) and tiles [tileBase, tileBase+ntiles) of every field.
+    // For one grid: nodeBase = tileBase = 0, nstride = nnodes, tilesTotal = ntiles, inst = null.
+    long long nstride;         // nodes per field in the context (vector component stride)
+    long long nodeBase;
+    int tileBase;
+    int tilesTotal;            // tiles per field in the context
+    const struct KGrid *inst;  // device table of the instances' grids (null for one grid)
+    const int *tileInst;       // context tile -> instance
+    const int *nodeBlockInst;  // block of NT context nodes -> instance (ranges padded to NT)
+    const int *matInst;        // material -> instance
 };
 
 struct KPart {

@@ -44,24 +44,36 @@ __device__ __forceinline__ int cell_key(const KGrid &g, int fld, int ci, int cj,
     int tx = ci / G::TX, ty = cj / G::TY, tz = ck / G::TZ;
     int t = (tz * g.nty + ty) * g.ntx + tx;
     int l = ((ck - tz * G::TZ) * G::TY + (cj - ty * G::TY)) * G::TX + (ci - tx * G::TX);
-    return (fld * g.ntiles + t) * G::NC + l;
+    return (fld * g.tilesTotal + g.tileBase + t) * G::NC + l;
 }
 
 // node arrays of one velocity field
 __device__ __forceinline__ KNodes field_view(const KGrid &g, const KNodes &N, int f) {
     KNodes v = N;
-    long long nn = g.nnodes;
-    v.mass = N.mass + f * nn;
-    v.pk = N.pk + 3 * f * nn;
-    v.vk = N.vk + 3 * f * nn;
-    v.ftot = N.ftot + 3 * f * nn;
-    v.cnt = N.cnt + f * nn;
+    long long nn = g.nstride, b = g.nodeBase;
+    v.mass = N.mass + f * nn + b;
+    v.pk = N.pk + 3 * f * nn + b;
+    v.vk = N.vk + 3 * f * nn + b;
+    v.ftot = N.ftot + 3 * f * nn + b;
+    v.cnt = N.cnt + f * nn + b;
     if (N.cvol) {
-        v.cvol = N.cvol + f * nn;
-        v.cdisp = N.cdisp + 3 * f * nn;
-        v.cgrad = N.cgrad + 3 * f * nn;
+        v.cvol = N.cvol + f * nn + b;
+        v.cdisp = N.cdisp + 3 * f * nn + b;
+        v.cgrad = N.cgrad + 3 * f * nn + b;
     }
     return v;
+}
+
+// grid of the instance that owns context tile tg (the kernel's grid for one instance)
+__device__ __forceinline__ KGrid tile_grid(const KGrid &g0, int tg) {
+    if (!g0.inst) return g0;
+    return g0.inst[g0.tileInst[tg]];
+}
+
+// grid of the instance that owns context node ng (node ranges are padded to NT per instance)
+__device__ __forceinline__ KGrid node_grid(const KGrid &g0, long long ng) {
+    if (!g0.inst) return g0;
+    return g0.inst[g0.nodeBlockInst[ng / 256]];
 }
 
 __device__ __forceinline__ void elem_ijk(const KGrid &g, int e, int &i, int &j, int &k) {
@@ -358,9 +370,10 @@ __device__ __forceinline__ long long foot_node(const KGrid &g, int s, int ox, in
 // binning: key + histogram, exclusive scan, scatter, per-cell sort (stable by particle index)
 
 template <int DIM>
-__global__ void k_bin_keys(KGrid g, KPart P, KBins B) {
+__global__ void k_bin_keys(KGrid g0, KPart P, KBins B) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
+    const KGrid g = g0.inst ? g0.inst[g0.matInst[P.mat[p]]] : g0;
     int i, j, k;
     elem_ijk(g, P.elem[p], i, j, k);
     int key = cell_key<DIM>(g, g.fieldOf[P.mat[p]], i, j, DIM == 3 ? k : 0);
@@ -537,7 +550,7 @@ __device__ __forceinline__ void load_tile_starts(const KBins &B, int t, int *s_c
 // NodalPointMPM.cpp:419-453). The point count of a node is not summed: every particle has
 // mp > 0 and every in-support weight is > 0, so numberPoints > 0 exactly when the mass is > 0.
 template <int DIM, int SH>
-__global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNodes N, KStats st) {
+__global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNodes N, KStats st) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int CH = CP ? 64 : NT;          // particles staged per chunk
@@ -547,7 +560,8 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g, KPart P, KBins B, KNod
     __shared__ int s_coff[CP ? CH : 1][8][3];
     __shared__ double s_q[CH][4];
     __shared__ double s_acc[NW][NQ_MASS][G::FOOT];
-    const int tall = blockIdx.x;              // (field, tile) pair
+    const int tall = blockIdx.x;              // (field, context tile) pair
+    const KGrid g = tile_grid(g0, tall % g0.tilesTotal);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
@@ -629,7 +643,7 @@ __device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, 
     for (int tz = tzl; tz <= tzh; tz++)
         for (int ty = tyl; ty <= tyh; ty++)
             for (int tx = txl; tx <= txh; tx++) {
-                int t = (tz * g.nty + ty) * g.ntx + tx + fld * g.ntiles;
+                int t = (tz * g.nty + ty) * g.ntx + tx + g.tileBase + fld * g.tilesTotal;
                 if (B.cellStart[(t + 1) * G::NC] == B.cellStart[t * G::NC]) continue;
                 int fx = i - (tx * G::TX - 1), fy = j - (ty * G::TY - 1);   // slot in the footprint
                 int s = fy * G::FX + fx;
@@ -641,11 +655,14 @@ __device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, 
 
 // node pass after P2G #1: mass, momentum, count and v = p/m (MatVelocityField.cpp:158-164,239-251)
 template <int DIM>
-__global__ void k_node_mass(KGrid g, KBins B, KNodes N) {
+__global__ void k_node_mass(KGrid g0, KBins B, KNodes N) {
     long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (nall >= g.nnodes * g.nfields) return;
-    const int fld = (int)(nall / g.nnodes);
-    const long long n = nall - (long long)fld * g.nnodes;
+    if (nall >= g0.nstride * g0.nfields) return;
+    const int fld = (int)(nall / g0.nstride);
+    const long long ng = nall - (long long)fld * g0.nstride;
+    const KGrid g = node_grid(g0, ng);
+    const long long n = ng - g.nodeBase;
+    if (n >= g.nnodes) return;                // padding between instances
     const KNodes Nf = field_view(g, N, fld);
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
@@ -653,7 +670,7 @@ __global__ void k_node_mass(KGrid g, KBins B, KNodes N) {
     int k = (int)(r / g.nny);
     double a[NQ_MASS];
     gather_partials<DIM, NQ_MASS>(g, B, N, fld, i, j, k, a);
-    long long nn = g.nnodes;
+    long long nn = g.nstride;
     Nf.mass[n] = a[0];
     Nf.pk[n] = a[1]; Nf.pk[nn + n] = a[2]; Nf.pk[2 * nn + n] = a[3];
     int cnt = a[0] > 0. ? 1 : 0;       // numberPoints > 0 <=> mass > 0 (all mp > 0)
@@ -815,13 +832,15 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
 // USF strain update, first half step (UpdateStrainsFirstTask::FullStrainUpdate,
 // UpdateStrainsFirstTask.cpp:101-168 -> MatPoint3D::UpdateStrain, MatPoint3D.cpp:45-60)
 template <int DIM, int MAT, int SH>
-__global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
+__global__ __launch_bounds__(NT) void k_usf(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    const int tall = blockIdx.x;              // (field, tile) pair
-    const int t = tall % g.ntiles;
-    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    const int tall = blockIdx.x;              // (field, context tile) pair
+    const int fld = tall / g0.tilesTotal;
+    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+    const int t = tall - fld * g0.tilesTotal - g.tileBase;
+    const KNodes Nf = field_view(g, N, fld);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
@@ -829,7 +848,7 @@ __global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N,
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
         const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
-        const long long st[3] = {0, g.nnodes, 2 * g.nnodes};
+        const long long st[3] = {0, g.nstride, 2 * g.nstride};
         load_foot<DIM, 3>(g, ox, oy, oz, src, st, s_v);
     }
     __syncthreads();
@@ -853,7 +872,7 @@ __global__ __launch_bounds__(NT) void k_usf(KGrid g, KPart P, KBins B, KNodes N,
 // MatPoint3D::GetFintPlusFext, MatPoint3D.cpp:248-270; MatPoint2D.cpp:242-263) with the stress
 // just updated by k_usf; pFext is zero (no particle loads)
 template <int DIM, int SH>
-__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNodes N) {
+__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
@@ -867,9 +886,11 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
     __shared__ double s_st[CH][NS];
     __shared__ double s_acc[NW][NQ_FORCE][G::FOOT];
     __shared__ double s_id[DIM][G::TX];       // 2/dx of the tile's cell columns, rows, layers
-    const int tall = blockIdx.x;              // (field, tile) pair
-    const int t = tall % g.ntiles;
-    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    const int tall = blockIdx.x;              // (field, context tile) pair
+    const int fld = tall / g0.tilesTotal;
+    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+    const int t = tall - fld * g0.tilesTotal - g.tileBase;
+    const KNodes Nf = field_view(g, N, fld);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
@@ -970,11 +991,14 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g, KPart P, KBins B, KNo
 // node pass after the forces: ftot, restore p (identity without BCs), p += f dt, v = p/m
 // (PostForcesTask.cpp:46-48, MatVelocityField.cpp:301-314, :239-251)
 template <int DIM>
-__global__ void k_node_force(KGrid g, KBins B, KNodes N) {
+__global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
     long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (nall >= g.nnodes * g.nfields) return;
-    const int fld = (int)(nall / g.nnodes);
-    const long long n = nall - (long long)fld * g.nnodes;
+    if (nall >= g0.nstride * g0.nfields) return;
+    const int fld = (int)(nall / g0.nstride);
+    const long long ng = nall - (long long)fld * g0.nstride;
+    const KGrid g = node_grid(g0, ng);
+    const long long n = ng - g.nodeBase;
+    if (n >= g.nnodes) return;                // padding between instances
     const KNodes Nf = field_view(g, N, fld);
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
@@ -982,7 +1006,7 @@ __global__ void k_node_force(KGrid g, KBins B, KNodes N) {
     int k = (int)(r / g.nny);
     double f[NQ_FORCE];
     gather_partials<DIM, NQ_FORCE>(g, B, N, fld, i, j, k, f);
-    long long nn = g.nnodes;
+    long long nn = g.nstride;
     if (DIM == 2) f[2] = 0.;
     Nf.ftot[n] = f[0]; Nf.ftot[nn + n] = f[1]; Nf.ftot[2 * nn + n] = f[2];
     int cnt = Nf.cnt[n];
@@ -1005,7 +1029,7 @@ __global__ void k_node_force(KGrid g, KBins B, KNodes N) {
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
 template <int DIM, int SH>
-__global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNodes N) {
+__global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int HALF = NT / 2;
@@ -1016,16 +1040,18 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
     __shared__ int s_coff[CP ? HALF : 1][8][3];
     __shared__ double s_q[HALF][4];
     __shared__ double s_acc[NW][NQ_MOM][G::FOOT];
-    const int tall = blockIdx.x;              // (field, tile) pair
-    const int t = tall % g.ntiles;
-    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    const int tall = blockIdx.x;              // (field, context tile) pair
+    const int fld = tall / g0.tilesTotal;
+    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+    const int t = tall - fld * g0.tilesTotal - g.tileBase;
+    const KNodes Nf = field_view(g, N, fld);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     int ox, oy, oz;
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
-        long long nn = g.nnodes;
+        long long nn = g.nstride;
         const double *src[7] = {Nf.vk, Nf.vk, Nf.vk, Nf.ftot, Nf.ftot, Nf.ftot, Nf.mass};
         const long long st[7] = {0, nn, 2 * nn, 0, nn, 2 * nn, 0};
         load_foot<DIM, 7>(g, ox, oy, oz, src, st, s_n);
@@ -1146,11 +1172,14 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g, KPart P, KBins B, KNode
 
 // node pass of the USL re-extrapolation: p (rezeroed, RezeroNodeTask6) and v = p/m
 template <int DIM>
-__global__ void k_node_mom2(KGrid g, KBins B, KNodes N) {
+__global__ void k_node_mom2(KGrid g0, KBins B, KNodes N) {
     long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (nall >= g.nnodes * g.nfields) return;
-    const int fld = (int)(nall / g.nnodes);
-    const long long n = nall - (long long)fld * g.nnodes;
+    if (nall >= g0.nstride * g0.nfields) return;
+    const int fld = (int)(nall / g0.nstride);
+    const long long ng = nall - (long long)fld * g0.nstride;
+    const KGrid g = node_grid(g0, ng);
+    const long long n = ng - g.nodeBase;
+    if (n >= g.nnodes) return;                // padding between instances
     const KNodes Nf = field_view(g, N, fld);
     int cnt = Nf.cnt[n];
     if (cnt <= 0) return;
@@ -1160,7 +1189,7 @@ __global__ void k_node_mom2(KGrid g, KBins B, KNodes N) {
     int k = (int)(r / g.nny);
     double a[NQ_MOM];
     gather_partials<DIM, NQ_MOM>(g, B, N, fld, i, j, k, a);
-    long long nn = g.nnodes;
+    long long nn = g.nstride;
     Nf.pk[n] = a[0]; Nf.pk[nn + n] = a[1]; Nf.pk[2 * nn + n] = a[2];
     double m = Nf.mass[n];
     if (m != 0.) {
@@ -1187,14 +1216,16 @@ __device__ __forceinline__ bool pt_in_elem(const KGrid &g, int ei, int ej, int e
 // USL strain update (second half step) and ResetElementsTask; also the cell key and histogram
 // of the next binning
 template <int DIM, int MAT, int SH>
-__global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+__global__ __launch_bounds__(NT) void k_usl_reset(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
                                                   KStats st) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    const int tall = blockIdx.x;              // (field, tile) pair
-    const int t = tall % g.ntiles;
-    const KNodes Nf = field_view(g, N, tall / g.ntiles);
+    const int tall = blockIdx.x;              // (field, context tile) pair
+    const int fld = tall / g0.tilesTotal;
+    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+    const int t = tall - fld * g0.tilesTotal - g.tileBase;
+    const KNodes Nf = field_view(g, N, fld);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
@@ -1202,7 +1233,7 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
         const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
-        const long long stt[3] = {0, g.nnodes, 2 * g.nnodes};
+        const long long stt[3] = {0, g.nstride, 2 * g.nstride};
         load_foot<DIM, 3>(g, ox, oy, oz, src, stt, s_v);
     }
     __syncthreads();
@@ -1291,7 +1322,7 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
                 if (DIM == 3) P.posn[2][p] = inside[2];
             }
         }
-        int key = cell_key<DIM>(g, tall / g.ntiles, ei, ej, DIM == 3 ? ek : 0);
+        int key = cell_key<DIM>(g, fld, ei, ej, DIM == 3 ? ek : 0);
         B.key[p] = key;
         atomicAdd(&B.count[key], 1);
     }
@@ -1307,7 +1338,7 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g, KPart P, KBins B, KNo
 struct RecContact { double wx, wy, wz, dx, dy, dz, vp, mp, ux, uy, uz; };
 
 template <int DIM, int SH>
-__global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+__global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
                                                     int newPositions) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
@@ -1322,7 +1353,9 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, K
     __shared__ double s_acc[NW][NQ_CONTACT][G::FOOT];
     __shared__ double s_id[DIM][G::TX];
     const int tall = blockIdx.x;
-    const int t = tall % g.ntiles;
+    const int fld = tall / g0.tilesTotal;
+    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+    const int t = tall - fld * g0.tilesTotal - g.tileBase;
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
@@ -1441,11 +1474,14 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g, KPart P, KBins B, K
 }
 
 template <int DIM>
-__global__ void k_node_contact(KGrid g, KBins B, KNodes N) {
+__global__ void k_node_contact(KGrid g0, KBins B, KNodes N) {
     long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (nall >= g.nnodes * g.nfields) return;
-    const int fld = (int)(nall / g.nnodes);
-    const long long n = nall - (long long)fld * g.nnodes;
+    if (nall >= g0.nstride * g0.nfields) return;
+    const int fld = (int)(nall / g0.nstride);
+    const long long ng = nall - (long long)fld * g0.nstride;
+    const KGrid g = node_grid(g0, ng);
+    const long long n = ng - g.nodeBase;
+    if (n >= g.nnodes) return;                // padding between instances
     const KNodes Nf = field_view(g, N, fld);
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
@@ -1453,7 +1489,7 @@ __global__ void k_node_contact(KGrid g, KBins B, KNodes N) {
     int k = (int)(r / g.nny);
     double a[NQ_CONTACT];
     gather_partials<DIM, NQ_CONTACT>(g, B, N, fld, i, j, k, a);
-    long long nn = g.nnodes;
+    long long nn = g.nstride;
     Nf.cvol[n] = a[0];
     Nf.cdisp[n] = a[1]; Nf.cdisp[nn + n] = a[2]; Nf.cdisp[2 * nn + n] = a[3];
     Nf.cgrad[n] = a[4]; Nf.cgrad[nn + n] = a[5]; Nf.cgrad[2 * nn + n] = a[6];
@@ -1506,7 +1542,7 @@ __device__ __forceinline__ bool coulomb_delta(const KGrid &g, double *delPi, con
 // the field's velocity for the next gather (GridValueCalculation, :239-251)
 __device__ __forceinline__ void change_momentum(const KGrid &g, const KNodes &N, int f, long long n,
                                                 const double *delP, int callType) {
-    long long nn = g.nnodes, b = 3 * f * nn + n;
+    long long nn = g.nstride, b = 3 * f * nn + n;
     double p[3];
     for (int c = 0; c < 3; c++) { p[c] = N.pk[b + c * nn] + delP[c]; N.pk[b + c * nn] = p[c]; }
     if (callType == CONTACT_UPDATE_MOMENTUM) {
@@ -1528,7 +1564,7 @@ template <int DIM>
 __global__ void k_contact(KGrid g, KNodes N, int callType) {
     long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nnodes) return;
-    const long long nn = g.nnodes;
+    const long long nn = g.nstride;
     int act[MPMGPU_MAX_MATERIALS];
     int numMats = 0;
     for (int f = 0; f < g.nfields; f++)
@@ -1650,7 +1686,7 @@ void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, 
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                      const MpmMaterial *mats, const KStats &st, hipStream_t s) {
     (void)mats;
-    dim3 grid(g.ntiles * g.nfields), block(NT);
+    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
         if (cp) hipLaunchKernelGGL((k_p2g_mass<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, st);
@@ -1662,14 +1698,14 @@ void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, co
 }
 
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_mass<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_mass<3>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_mass<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 template <int DIM, int SH>
 static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                     const MpmMaterial *mats, hipStream_t s) {
-    dim3 grid(g.ntiles * g.nfields), block(NT);
+    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
             hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
@@ -1697,7 +1733,7 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
 }
 
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    dim3 grid(g.ntiles * g.nfields), block(NT);
+    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
         if (cp) hipLaunchKernelGGL((k_p2g_force<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
@@ -1709,12 +1745,12 @@ void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, c
 }
 
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_force<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_force<3>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    dim3 grid(g.ntiles * g.nfields), block(NT);
+    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
         if (cp) hipLaunchKernelGGL((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
@@ -1726,14 +1762,14 @@ void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, con
 }
 
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_mom2<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_mom2<3>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_mom2<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 template <int DIM, int SH>
 static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                           const MpmMaterial *mats, const KStats &st, hipStream_t s) {
-    dim3 grid(g.ntiles * g.nfields), block(NT);
+    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
             hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
@@ -1763,7 +1799,7 @@ void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, c
 void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                         const MpmMaterial *mats, bool newPositions, hipStream_t s) {
     int np = newPositions ? 1 : 0;
-    dim3 grid(g.ntiles * g.nfields), block(NT);
+    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
         if (cp) hipLaunchKernelGGL((k_p2g_contact<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, mats, np);
@@ -1775,8 +1811,8 @@ void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b,
 }
 
 void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
-    if (dim == 3) hipLaunchKernelGGL(k_node_contact<3>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
-    else hipLaunchKernelGGL(k_node_contact<2>, dim3(blocks_for(g.nnodes * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    if (dim == 3) hipLaunchKernelGGL(k_node_contact<3>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
+    else hipLaunchKernelGGL(k_node_contact<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s) {

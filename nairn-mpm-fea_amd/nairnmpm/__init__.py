@@ -75,6 +75,11 @@ class RunParams(ctypes.Structure):
                 ("maxLeaving", ctypes.c_int32), ("numFields", ctypes.c_int32)]
 
 
+class InstanceDesc(ctypes.Structure):
+    _fields_ = [("grid", GridDesc), ("run", RunParams), ("firstMaterial", ctypes.c_int32),
+                ("numMaterials", ctypes.c_int32)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("nparticles", ctypes.c_int64), ("leftGrid", ctypes.c_int64),
                 ("nanParticles", ctypes.c_int64), ("time", ctypes.c_double), ("lastStepMs", ctypes.c_double)]
@@ -104,6 +109,8 @@ _SIGNATURES = {
     "mpmgpu_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Material), ctypes.c_int32,
                        ctypes.POINTER(RunParams), ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_upload_particles": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
+    "mpmgpu_batch_create": ([ctypes.c_int32, ctypes.POINTER(InstanceDesc), ctypes.POINTER(Material), ctypes.c_int32,
+                             ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_step": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_sync": ([_P], ctypes.c_int),
@@ -320,3 +327,41 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+class Batch(Simulation):
+    """Independent problems advanced together on one device context (mpmgpu_batch_create): each
+    keeps its own grid, time step, materials and particles; step() moves every instance one of its
+    own time steps with one launch per kernel."""
+
+    def __init__(self, problems):
+        descs, mats, ds, iss = [], [], [], []
+        for prob in problems:
+            g, m, run, n, _ = prob.describe()
+            d, i = prob.particles()
+            i = i.copy()
+            i[IFIELDS.index("mat")] += len(mats)
+            descs.append(InstanceDesc(g, run, len(mats), len(m)))
+            mats.extend(m)
+            ds.append(d)
+            iss.append(i)
+        arr = (InstanceDesc * len(descs))(*descs)
+        marr = (Material * len(mats))(*mats)
+        h = _P()
+        _check(library().mpmgpu_batch_create(len(descs), arr, marr, len(mats), ctypes.byref(h)))
+        self._h = h
+        self.grid, self.run = descs[0].grid, descs[0].run
+        self.numFields = 1
+        self.descs = descs
+        self.counts = [d.shape[1] for d in ds]
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        self.n = 0
+        self.upload(np.concatenate(ds, axis=1), np.concatenate(iss, axis=1))
+
+    def instance_particles(self, k: int):
+        """particles of instance k (its materials renumbered from 0, as in its own problem)"""
+        d, i = self.particles()
+        a, b = self.offsets[k], self.offsets[k + 1]
+        i = i[:, a:b].copy()
+        i[IFIELDS.index("mat")] -= self.descs[k].firstMaterial
+        return d[:, a:b], i

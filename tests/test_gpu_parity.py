@@ -263,3 +263,53 @@ def test_lcpdi_sliding_contact_friction():
     sim.step(60)
     ora.step(60)
     _check(sim, ora, 3, "lCPDI sliding contact")
+
+
+# ---- batches of independent instances -----------------------------------------------------
+
+def _alone(prob, steps):
+    sim = nm.Simulation.from_problem(prob)
+    sim.step(steps)
+    return sim.particles()
+
+
+def test_batch_2d_instances_equal_alone():
+    """Blocks of different sizes, materials (elastic and J2) and time steps stepped as one batch
+    give every instance the same particles, bit for bit, as running it alone."""
+    probs = [nm.Problem.from_xml(configs.elastic_block_2d(cells=16, block=6, seed=3, E=800.)),
+             nm.Problem.from_xml(configs.j2_block_2d(cells=40, block=24, seed=4)),
+             nm.Problem.from_xml(configs.elastic_block_2d(cells=24, block=12, seed=5, E=3000., nu=0.25)),
+             nm.Problem.from_xml(configs.j2_block_2d(cells=20, block=8, seed=6, yield_=30.))]
+    batch = nm.Batch(probs)
+    batch.step(50)
+    for k, prob in enumerate(probs):
+        d, i = batch.instance_particles(k)
+        da, ia = _alone(prob, 50)
+        assert np.array_equal(d, da), f"instance {k} differs from its solo run"
+        assert np.array_equal(i, ia)
+
+
+def test_batch_3d_instances_equal_alone():
+    probs = [nm.Problem.from_xml(configs.j2_block_3d(cells=12, lo=3, hi=9)),
+             nm.Problem.from_xml(configs.neohookean_block_3d(cells=16, lo=4, hi=12)),
+             nm.Problem.from_xml(configs.j2_block_3d(cells=8, lo=2, hi=6, E=20000.))]
+    batch = nm.Batch(probs)
+    batch.step(20)
+    for k, prob in enumerate(probs):
+        d, _ = batch.instance_particles(k)
+        da, _ = _alone(prob, 20)
+        assert np.array_equal(d, da), f"instance {k} differs from its solo run"
+
+
+def test_batch_matches_oracle():
+    probs = [nm.Problem.from_xml(configs.elastic_block_2d(cells=16, block=6, seed=3)),
+             nm.Problem.from_xml(configs.j2_block_2d(cells=20, block=8, seed=6))]
+    batch = nm.Batch(probs)
+    batch.step(40)
+    for k, prob in enumerate(probs):
+        ora = orc.Oracle.from_problem(prob)
+        ora.step(40)
+        d, _ = batch.instance_particles(k)
+        r, _ = ora.particles()
+        errs = compare(d, r, 2)
+        assert max(errs.values()) <= TOL, errs
