@@ -43,6 +43,20 @@ ALG_BYTES = {
 }
 CPDI_EXTRA = 72      # lCPDI: semi-side vectors read by every tile kernel (written by the first)
 
+# The same groups for 2D plane strain: positions, velocities and node vectors have two components,
+# the constitutive state is ep(4) + wrot(1) + sp(4) + pressure + alt(4) + history = 15 words.
+ALG_BYTES_2D = {
+    "k_p2g_mass": (16 + 16 + 8 + 4, 24),
+    "k_node_mass": (0, 24 + 24 + 16),
+    "k_usf": (16 + 4 + 4 + 2 * 120, 16),
+    "k_p2g_force": (16 + 4 + 8 + 40, 16),
+    "k_node_force": (0, 16 + 32 + 16),
+    "k_g2p_p2g": (16 + 16 + 8 + 4 + 32, 40 + 16),
+    "k_node_mom2": (0, 16 + 16),
+    "k_usl_reset": (16 + 16 + 4 + 2 * 120 + 8, 16),
+    "binning": (16, 0),
+}
+
 
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_j2_block_128.md")
 
@@ -80,6 +94,117 @@ def copy_calibration(nbytes, launches=5):
                       "GBps": 2 * n * 8 / dt / 1e9}), flush=True)
 
 
+def cpu_sample(pairs, seconds, steps=3):
+    """Oracle particle-steps/s on one host core over [(problem, (d, i))] run in order until about
+    `seconds` of CPU work is done."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    done = ps = 0
+    t = 0.0
+    for prob, (d, i) in pairs:
+        g, mats, run, n, _ = prob.describe()
+        o = orc.Oracle(g, mats, run, d, i)
+        tc = time.perf_counter()
+        o.step(steps)
+        t += time.perf_counter() - tc
+        del o
+        done += 1
+        ps += n * steps
+        if t >= seconds:
+            break
+    return ps / t, done, ps, t
+
+
+def bench_batch(a):
+    """The batched 2D sweep: a.instances independent plane-strain blocks (elastic or J2, 2k-200k
+    particles log-uniform, their own materials and time steps) stepped together in one context."""
+    t0 = time.time()
+    pairs = []
+    for k in range(a.instances):
+        xml, fix, _ = configs.sweep_instance_2d(k)
+        prob = nm.Problem.from_xml(xml)
+        d, i = prob.particles()
+        fix(d)
+        pairs.append((prob, (d, i)))
+    batch = nm.Batch([p for p, _ in pairs], [q for _, q in pairs])
+    npart = int(sum(batch.counts))
+    print(f"# setup {time.time() - t0:.1f} s: {a.instances} instances, {npart} particles "
+          f"({min(batch.counts)}..{max(batch.counts)} per instance)", file=sys.stderr, flush=True)
+    batch.step(a.warmup)
+    batch.sync()
+    gsteps, tune_ms = batch.autotune(trial_steps=32)
+    print(f"# autotuner: graphs of {gsteps} steps ({tune_ms:.3f} ms/step in trials)", file=sys.stderr, flush=True)
+
+    # timed region: K batch steps replayed from the tuned graph
+    batch.sync()
+    t1 = time.perf_counter()
+    batch.step(a.steps)
+    batch.sync()
+    elapsed = time.perf_counter() - t1
+    value = npart * a.steps / elapsed
+
+    # kernel-group times from HIP events on the context's stream (eager launches)
+    pst = min(a.steps, 20)
+    batch.set_profiling(True)
+    batch.step(pst)
+    batch.sync()
+    times = batch.kernel_times()
+    batch.set_profiling(False)
+    mass, _, _ = batch.grid_fields()
+    active_nodes = int(np.count_nonzero(mass))
+    for g, (ms, n) in times.items():
+        print(f"#   {g:14s} {ms / max(n, 1):9.4f} ms/launch ({n} launches)", file=sys.stderr)
+    dom = max(times, key=lambda g: times[g][0])
+    dom_ms = times[dom][0] / max(times[dom][1], 1)
+    bp, bn = ALG_BYTES_2D[dom]
+    alg_bytes = bp * npart + bn * active_nodes
+    achieved = alg_bytes / (dom_ms * 1e-3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms}
+    batch.close()
+
+    # the same instances solved one at a time (own context, default two-step graphs)
+    ser_ps = 0
+    ser_t = 0.0
+    for prob, (d, i) in pairs:
+        g, mats, run, n, _ = prob.describe()
+        sim = nm.Simulation(g, mats, run)
+        sim.upload(d, i)
+        sim.step(2)
+        sim.sync()
+        tc = time.perf_counter()
+        sim.step(a.serial_steps)
+        sim.sync()
+        ser_t += time.perf_counter() - tc
+        ser_ps += n * a.serial_steps
+        sim.close()
+    serial = ser_ps / ser_t
+    print(f"# batch {value:.4e} particle-steps/s, one at a time {serial:.4e} ({value / serial:.2f}x)",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        v, done, ps, t = cpu_sample(pairs, a.cpu_seconds)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"3 steps of each of the first {done} sweep instances ({ps} particle-steps) on one "
+                         f"host core (oracle/mpm_oracle.cpp, serial), {t:.1f} s"}
+    return {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched 2D sweep: {a.instances} plane-strain elastic/J2 blocks, 2k-200k particles "
+                               "log-uniform, E 0.5-5 GPa, nu 0.25-0.4, yield 5-50 MPa, uGIMP, one step of "
+                               "each instance per batch step",
+                   "particles": npart, "instances": a.instances, "active_nodes": active_nodes,
+                   "graph_steps": gsteps, "one_at_a_time_value": serial,
+                   "one_at_a_time_sample": f"{a.serial_steps} graph-replayed steps of every instance alone",
+                   "speedup_vs_one_at_a_time": value / serial},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,8 +213,12 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["j2", "contact"], default="j2",
-                    help="j2: the 7.08M-particle J2 GIMP block (default); contact: the lCPDI contact pair")
+    ap.add_argument("--workload", choices=["j2", "contact", "batch"], default="j2",
+                    help="j2: the 7.08M-particle J2 GIMP block (default); contact: the lCPDI contact pair; "
+                         "batch: the 256-instance 2D sweep")
+    ap.add_argument("--instances", type=int, default=256, help="batch: number of sweep instances")
+    ap.add_argument("--serial-steps", type=int, default=20, help="batch: steps of each instance run alone")
+    ap.add_argument("--cpu-seconds", type=float, default=10., help="batch: CPU baseline sample length")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy used to scale the HBM byte counters")
     a = ap.parse_args()
@@ -100,6 +229,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU slab "
               "decomposition is a design, not a built path", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "batch":
+        print(json.dumps(bench_batch(a)), flush=True)
+        return
 
     lo, hi = a.cells // 8, a.cells - a.cells // 8
     t0 = time.time()

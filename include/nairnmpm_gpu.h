@@ -227,6 +227,11 @@ int mpmgpu_upload_particles(MpmCtx *ctx, int64_t n, const double *dfields, const
 int mpmgpu_run_task(MpmCtx *ctx, int32_t task);
 /* Run nsteps whole steps (NairnMPM::MPMStep, NairnMPM.cpp:284-335), fused and graph-replayed. */
 int mpmgpu_step(MpmCtx *ctx, int32_t nsteps);
+/* Steps captured per graph launch (even, 2..64; default 2). No counterpart in the reference. */
+int mpmgpu_set_graph_steps(MpmCtx *ctx, int32_t steps);
+/* Launch-shape tuner (no counterpart in the reference): times graph lengths 2..32 over
+   trialSteps steps each, keeps the fastest, and restores the simulation state it started from. */
+int mpmgpu_autotune(MpmCtx *ctx, int32_t trialSteps, int32_t *graphSteps, double *msPerStep);
 /* Wait for all queued device work of this context. */
 int mpmgpu_sync(MpmCtx *ctx);
 /* Download particles in creation (mpm[]) order. */
@@ -235,7 +240,8 @@ int mpmgpu_download_particles(MpmCtx *ctx, double *dfields, int32_t *ifields);
 int mpmgpu_query(MpmCtx *ctx, MpmStats *stats);
 /* Grid node values (mass, momentum, force) after the last task, nodes numbered x fastest; one
    block per material velocity field (numFields of them, field-major): mass[f*nn + n],
-   pk/ftot[(3*f + c)*nn + n]. */
+   pk/ftot[(3*f + c)*nn + n]. In a batch context nn covers every instance: instance i's nodes
+   start at the sum over earlier instances of their node counts rounded up to 256. */
 int mpmgpu_download_grid(MpmCtx *ctx, double *mass, double *pk, double *ftot);
 /* Multimaterial contact extrapolations of every field (same layout): volume, mass-weighted
    displacement, volume gradient (MatVelocityField::contactInfo, MatVelocityField.hpp:40-124).

@@ -43,7 +43,8 @@ struct MpmCtx {
     unsigned long long *sbuf = nullptr;
     long long ncap = 0;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};   // two-step graphs, by position parity
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // graphs of graphSteps steps, by position parity
+    int graphSteps = 2;              // even: a graph ends on the parity it started from
     int parity = 0;                  // which position buffer holds the current positions
     double *posA[3] = {nullptr, nullptr, nullptr}, *posB[3] = {nullptr, nullptr, nullptr};
     int nextTask = 0;                // task sequence check for mpmgpu_run_task
@@ -682,26 +683,85 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
     if (c->kp.n == 0) { c->steps += nsteps; c->mtime += nsteps * c->run.timestep; return 0; }
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
     int done = 0;
-    if (nsteps >= 2 && !c->profiling) {
+    const int G = c->graphSteps;
+    if (nsteps >= G && !c->profiling) {
         int par = c->parity;
         if (!c->graph[par]) {
             hipGraph_t gr;
             HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(c);
-            enqueue_step(c);
+            for (int k = 0; k < G; k++) enqueue_step(c);
             HIPCHECK(c, hipStreamEndCapture(c->stream, &gr));
             hipError_t e = hipGraphInstantiate(&c->graph[par], gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
             if (e != hipSuccess) return c->fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
             c->parity = par;      // capture does not run the work
         }
-        for (; done + 2 <= nsteps; done += 2) HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
+        for (; done + G <= nsteps; done += G) HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
     }
     for (; done < nsteps; done++) enqueue_step(c);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
     c->steps += nsteps;
     c->mtime += nsteps * c->run.timestep;
+    return 0;
+}
+
+int mpmgpu_set_graph_steps(MpmCtx *c, int32_t steps) {
+    if (steps < 2 || steps > 64 || (steps & 1)) return c->fail("graph steps must be even, 2 to 64");
+    destroy_graphs(c);
+    c->graphSteps = steps;
+    return 0;
+}
+
+// Launch-shape tuner: times whole steps captured in graphs of 2 to 32 steps and keeps the fastest.
+// The particle state, bins and counters are saved before the trials and restored after them, so
+// tuning leaves the simulation where it was.
+int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *msPerStep) {
+    if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
+    if (c->kp.n == 0 || c->profiling) return c->fail("nothing to tune");
+    size_t n = (size_t)c->ncap;
+    size_t pb = (MPF_NUM_FIELDS + 3) * n * sizeof(double), ib = (MPI_NUM_FIELDS + 2) * n * sizeof(int);
+    long long nk = c->kb.nkeys;
+    size_t bb = (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1) * sizeof(int);
+    char *save = nullptr;
+    HIPCHECK(c, hipMalloc(&save, pb + ib + bb + 3 * sizeof(unsigned long long)));
+    HIPCHECK(c, hipMemcpyAsync(save, c->pbuf, pb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(save + pb, c->ibuf, ib, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(save + pb + ib, c->bbuf, bb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                               c->stream));
+    const int parity = c->parity, oldG = c->graphSteps;
+    const int64_t steps = c->steps;
+    const double mtime = c->mtime;
+    int best = oldG;
+    double bestMs = 1e300;
+    static const int cand[] = {2, 4, 8, 16, 32};
+    for (int G : cand) {
+        int k = trialSteps < G ? G : (trialSteps / G) * G;
+        mpmgpu_set_graph_steps(c, G);
+        if (mpmgpu_step(c, G)) break;                      // capture and warm up
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (mpmgpu_step(c, k)) break;
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        double per = ms / k;
+        if (per < bestMs) { bestMs = per; best = G; }
+    }
+    // restore the state from before the trials
+    HIPCHECK(c, hipMemcpyAsync(c->pbuf, save, pb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->ibuf, save + pb, ib, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->bbuf, save + pb + ib, bb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->sbuf, save + pb + ib + bb, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(save);
+    c->parity = parity;
+    c->steps = steps;
+    c->mtime = mtime;
+    mpmgpu_set_graph_steps(c, best);
+    if (graphSteps) *graphSteps = best;
+    if (msPerStep) *msPerStep = bestMs;
     return 0;
 }
 
@@ -739,7 +799,7 @@ int mpmgpu_download_particles(MpmCtx *c, double *dfields, int32_t *ifields) {
 int mpmgpu_download_contact(MpmCtx *c, double *vol, double *disp, double *grad) {
     if (c->kg.nfields < 2) return c->fail("no contact extrapolations in single-material mode");
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    size_t nn = (size_t)c->kg.nnodes * c->kg.nfields;
+    size_t nn = (size_t)c->kg.nstride * c->kg.nfields;     // every instance of a batch
     if (vol) HIPCHECK(c, hipMemcpy(vol, c->kn.cvol, nn * sizeof(double), hipMemcpyDeviceToHost));
     if (disp) HIPCHECK(c, hipMemcpy(disp, c->kn.cdisp, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
     if (grad) HIPCHECK(c, hipMemcpy(grad, c->kn.cgrad, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
@@ -748,7 +808,7 @@ int mpmgpu_download_contact(MpmCtx *c, double *vol, double *disp, double *grad) 
 
 int mpmgpu_download_grid(MpmCtx *c, double *mass, double *pk, double *ftot) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    size_t nn = (size_t)c->kg.nnodes * c->kg.nfields;
+    size_t nn = (size_t)c->kg.nstride * c->kg.nfields;     // every instance of a batch
     if (mass) HIPCHECK(c, hipMemcpy(mass, c->kn.mass, nn * sizeof(double), hipMemcpyDeviceToHost));
     if (pk) HIPCHECK(c, hipMemcpy(pk, c->kn.pk, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));
     if (ftot) HIPCHECK(c, hipMemcpy(ftot, c->kn.ftot, 3 * nn * sizeof(double), hipMemcpyDeviceToHost));

@@ -376,7 +376,10 @@ __global__ void k_bin_keys(KGrid g0, KPart P, KBins B) {
     const KGrid g = g0.inst ? g0.inst[g0.matInst[P.mat[p]]] : g0;
     int i, j, k;
     elem_ijk(g, P.elem[p], i, j, k);
-    int key = cell_key<DIM>(g, g.fieldOf[P.mat[p]], i, j, DIM == 3 ? k : 0);
+    // fieldOf covers the materials of one multimaterial context; batch instances are single-field
+    // and their material numbers run past its end
+    const int fld = g.nfields > 1 ? g.fieldOf[P.mat[p]] : 0;
+    int key = cell_key<DIM>(g, fld, i, j, DIM == 3 ? k : 0);
     B.key[p] = key;
     atomicAdd(&B.count[key], 1);
 }
@@ -890,7 +893,6 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     const int fld = tall / g0.tilesTotal;
     const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
     const int t = tall - fld * g0.tilesTotal - g.tileBase;
-    const KNodes Nf = field_view(g, N, fld);
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;

@@ -113,6 +113,9 @@ _SIGNATURES = {
                              ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_step": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_set_graph_steps": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_autotune": ([_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
+                        ctypes.c_int),
     "mpmgpu_sync": ([_P], ctypes.c_int),
     "mpmgpu_download_particles": ([_P, _PD, _PI], ctypes.c_int),
     "mpmgpu_query": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -261,6 +264,15 @@ class Simulation:
     def step(self, nsteps: int = 1):
         _check(library().mpmgpu_step(self._h, int(nsteps)), self._h)
 
+    def set_graph_steps(self, steps: int):
+        _check(library().mpmgpu_set_graph_steps(self._h, int(steps)), self._h)
+
+    def autotune(self, trial_steps: int = 32):
+        """pick the graph length with the fastest steps; the state is left unchanged"""
+        g, ms = ctypes.c_int32(), ctypes.c_double()
+        _check(library().mpmgpu_autotune(self._h, int(trial_steps), ctypes.byref(g), ctypes.byref(ms)), self._h)
+        return g.value, ms.value
+
     def run_task(self, task: int):
         _check(library().mpmgpu_run_task(self._h, int(task)), self._h)
 
@@ -334,11 +346,13 @@ class Batch(Simulation):
     keeps its own grid, time step, materials and particles; step() moves every instance one of its
     own time steps with one launch per kernel."""
 
-    def __init__(self, problems):
+    def __init__(self, problems, particles=None):
+        """problems: Problem objects; particles: optional list of (dfields, ifields) per problem that
+        replaces the problem's own initial particles (same count and order)"""
         descs, mats, ds, iss = [], [], [], []
-        for prob in problems:
+        for k, prob in enumerate(problems):
             g, m, run, n, _ = prob.describe()
-            d, i = prob.particles()
+            d, i = prob.particles() if particles is None or particles[k] is None else particles[k]
             i = i.copy()
             i[IFIELDS.index("mat")] += len(mats)
             descs.append(InstanceDesc(g, run, len(mats), len(m)))
@@ -355,8 +369,20 @@ class Batch(Simulation):
         self.descs = descs
         self.counts = [d.shape[1] for d in ds]
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        self.node_counts = [(g.nx + 1) * (g.ny + 1) * ((g.nz + 1) if g.analysis == THREED else 1)
+                            for g in (d.grid for d in descs)]
+        self.node_offsets = np.concatenate([[0], np.cumsum([(c + 255) // 256 * 256 for c in self.node_counts])])
         self.n = 0
         self.upload(np.concatenate(ds, axis=1), np.concatenate(iss, axis=1))
+
+    def _nodes(self):
+        return int(self.node_offsets[-1])
+
+    def instance_grid(self, k: int):
+        """mass [nn], momentum and force [3, nn] of instance k's own grid"""
+        m, pk, f = self.grid_fields()
+        a, b = self.node_offsets[k], self.node_offsets[k] + self.node_counts[k]
+        return m[0, a:b], pk[0, :, a:b], f[0, :, a:b]
 
     def instance_particles(self, k: int):
         """particles of instance k (its materials renumbered from 0, as in its own problem)"""

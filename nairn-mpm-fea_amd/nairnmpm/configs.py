@@ -27,6 +27,17 @@ class SplitMix64:
     def uniform(self) -> float:
         return (self.next_u64() >> 11) * (1.0 / (1 << 53))
 
+    def uniform_array(self, n: int):
+        """the next n uniform() values as a numpy array (same stream, vectorised)"""
+        import numpy as np
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + np.uint64(0x9E3779B97F4A7C15) * np.arange(1, n + 1, dtype=np.uint64)
+            self.state = (self.state + n * 0x9E3779B97F4A7C15) & _MASK
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
 
 def _header(analysis: int, cfl: float, max_time: float, extra: str = "", shape: str = "uGIMP") -> str:
     return f"""<?xml version='1.0'?>
@@ -271,3 +282,65 @@ def contact_blocks_2d(cells: int = 32, w: int = 8, h: int = 8, gap: float = 1., 
     </Body>
   </MaterialPoints>
 """ + mats + "</JANFEAInput>\n")
+
+
+def sweep_instance_2d(k: int, pad: int = 8, cfl: float = 0.2):
+    """Member k of the batched 2D sweep: seed 1000+k; Np = round(exp(U(ln 2000, ln 200000)))
+    particles as a square block at 4 per cell with `pad` cells of grid around it; elastic or J2
+    with probability 1/2; E ~ U(500, 5000) MPa, nu ~ U(0.25, 0.40), yield ~ U(5, 50) MPa,
+    Ep = 0.01 E, rho = 1 g/cm^3. Returns (xml, particle_fix, is_j2): particle_fix(d) applies the
+    plain 2D block's velocity profile and +-0.05 mm jitter to the particle fields d in place."""
+    rng = SplitMix64(1000 + k)
+    npart = round(math.exp(math.log(2000.) + rng.uniform() * (math.log(200000.) - math.log(2000.))))
+    side = max(2, round(math.sqrt(npart / 4.)))
+    cells = side + 2 * pad
+    E = 500. + 4500. * rng.uniform()
+    nu = 0.25 + 0.15 * rng.uniform()
+    j2 = rng.uniform() < 0.5
+    yld = 5. + 45. * rng.uniform()
+    if j2:
+        mat = f"""  <Material Type="9" Name="J2 {k}">
+    <rho>1.0</rho>
+    <E>{E!r}</E>
+    <nu>{nu!r}</nu>
+    <yield>{yld!r}</yield>
+    <Ep>{0.01 * E!r}</Ep>
+  </Material>
+"""
+    else:
+        mat = f"""  <Material Type="1" Name="elastic {k}">
+    <rho>1.0</rho>
+    <E>{E!r}</E>
+    <nu>{nu!r}</nu>
+  </Material>
+"""
+    xml = (_header(10, cfl, 1.0, "    <MatlPtsPerElement>4</MatlPtsPerElement>\n") +
+           f"""  <Mesh output="file">
+    <Grid xmin="0" xmax="{cells}" ymin="0" ymax="{cells}">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <Body mat="1" vx="0" vy="0">
+      <Rect xmin="{pad}" xmax="{pad + side}" ymin="{pad}" ymax="{pad + side}"/>
+    </Body>
+  </MaterialPoints>
+""" + mat + "</JANFEAInput>\n")
+
+    def particle_fix(d):
+        """C1 velocity profile v_x = 10 m/s sin(pi (y - y0)/L) and +-0.05 mm position jitter"""
+        import numpy as np
+        n = d.shape[1]
+        u = rng.uniform_array(2 * n).reshape(n, 2)
+        d[F_POS_X] += 0.05 * (2.0 * u[:, 0] - 1.0)
+        d[F_POS_Y] += 0.05 * (2.0 * u[:, 1] - 1.0)
+        d[F_VEL_X] = 10000. * np.sin(np.pi * (d[F_POS_Y] - pad) / side)
+        d[F_ORIG_X] = d[F_POS_X]
+        d[F_ORIG_Y] = d[F_POS_Y]
+
+    return xml, particle_fix, j2
+
+
+# rows of the particle field array (nairnmpm.FIELDS)
+F_POS_X, F_POS_Y, F_VEL_X, F_ORIG_X, F_ORIG_Y = 0, 1, 3, 34, 35

@@ -123,6 +123,34 @@ def test_graph_replay_matches_eager():
     assert np.array_equal(a.particles()[0], b.particles()[0])
 
 
+@pytest.mark.parametrize("graph_steps", [4, 16])
+def test_graph_length_matches_eager(graph_steps):
+    prob = nm.Problem.from_xml(configs.j2_block_2d(cells=24, block=12))
+    a = nm.Simulation.from_problem(prob)
+    b = nm.Simulation.from_problem(prob)
+    a.set_graph_steps(graph_steps)
+    a.step(37)                       # replays of the longer graph, then eager steps
+    for _ in range(37):
+        b.step(1)
+    assert np.array_equal(a.particles()[0], b.particles()[0])
+    assert a.stats().steps == b.stats().steps == 37
+
+
+def test_autotune_leaves_state_unchanged():
+    prob = nm.Problem.from_xml(configs.j2_block_2d(cells=24, block=12))
+    a = nm.Simulation.from_problem(prob)
+    b = nm.Simulation.from_problem(prob)
+    a.step(5)
+    b.step(5)
+    g, ms = a.autotune(trial_steps=16)
+    assert g in (2, 4, 8, 16, 32) and ms > 0
+    assert a.stats().steps == 5
+    assert np.array_equal(a.particles()[0], b.particles()[0])
+    a.step(30)
+    b.step(30)
+    assert np.array_equal(a.particles()[0], b.particles()[0])
+
+
 # ---- multimaterial contact (two velocity fields) -----------------------------------------
 
 # Head-on frictional contact is not a parity case: where the tangential momentum change is zero,
@@ -287,6 +315,49 @@ def test_batch_2d_instances_equal_alone():
         da, ia = _alone(prob, 50)
         assert np.array_equal(d, da), f"instance {k} differs from its solo run"
         assert np.array_equal(i, ia)
+
+
+def _sweep_member(k):
+    xml, fix, _ = configs.sweep_instance_2d(k)
+    prob = nm.Problem.from_xml(xml)
+    d, i = prob.particles()
+    fix(d)
+    return prob, (d, i)
+
+
+def test_batch_sweep_members_equal_alone_and_grid():
+    """Members of the batched 2D sweep (uneven sizes, elastic and J2, their own time steps):
+    particles and grid values of each equal its solo run, also after the autotuner's trials."""
+    ks = [0, 12, 30, 13, 24, 34, 35, 18]
+    members = [_sweep_member(k) for k in ks]
+    batch = nm.Batch([m[0] for m in members], [m[1] for m in members])
+    g, ms = batch.autotune(trial_steps=8)
+    assert g in (2, 4, 8, 16, 32) and ms > 0
+    batch.step(30)
+    for k, (prob, (d0, i0)) in enumerate(members):
+        g_, mats, run, _, _ = prob.describe()
+        sim = nm.Simulation(g_, mats, run)
+        sim.upload(d0, i0)
+        sim.step(30)
+        d, _ = batch.instance_particles(k)
+        assert np.array_equal(d, sim.particles()[0]), f"member {ks[k]} differs from its solo run"
+        bm, bp, bf = batch.instance_grid(k)
+        sm, sp, sf = sim.grid_values()
+        assert np.array_equal(bm, sm) and np.array_equal(bp[:2], sp[:2]) and np.array_equal(bf[:2], sf[:2])
+
+
+def test_batch_more_materials_than_one_context_holds():
+    """A batch of 20 single-material instances numbers its materials 0-19, past the 16 materials
+    of one context; each instance still equals its solo run."""
+    probs = [nm.Problem.from_xml(configs.elastic_block_2d(cells=12 + (k % 5), block=4 + (k % 3), seed=40 + k,
+                                                          E=600. + 100. * k))
+             for k in range(20)]
+    batch = nm.Batch(probs)
+    batch.step(12)
+    for k, prob in enumerate(probs):
+        d, _ = batch.instance_particles(k)
+        da, _ = _alone(prob, 12)
+        assert np.array_equal(d, da), f"instance {k} differs from its solo run"
 
 
 def test_batch_3d_instances_equal_alone():
