@@ -205,6 +205,108 @@ def bench_batch(a):
     }
 
 
+def bench_serve(a):
+    """The served 3D stream: a.requests J2 / neo-Hookean blocks (50k-2M particles log-uniform,
+    200-2000 steps) arriving as a Poisson process of rate a.rate per second, served by the solve
+    server's request loop; then the same solves one at a time as plain solves, in arrival order."""
+    t0 = time.time()
+    reqs = []
+    for k in range(a.requests):
+        xml, steps, j2 = configs.serve_request_3d(k)
+        prob = nm.Problem.from_xml(xml)
+        g, mats, run, n, _ = prob.describe()
+        d, i = prob.particles()
+        reqs.append({"k": k, "prob": prob, "grid": g, "mats": mats, "run": run, "n": n, "d": d, "i": i,
+                     "steps": steps, "j2": j2})
+    rng = configs.SplitMix64(777)
+    gaps = -np.log(1.0 - rng.uniform_array(a.requests)) / a.rate
+    arrivals = np.cumsum(gaps) - gaps[0]               # first request at t = 0
+    total_ps = sum(r["n"] * r["steps"] for r in reqs)
+    print(f"# setup {time.time() - t0:.1f} s: {a.requests} requests, {sum(r['n'] for r in reqs)} particles, "
+          f"{total_ps:.3e} particle-steps, arrivals over {arrivals[-1]:.2f} s", file=sys.stderr, flush=True)
+
+    # warm-up: one small solve through a server (module loads, first graph capture)
+    srv = nm.Server(analysis=nm.THREED, graph_steps=a.graph_steps)
+    w = reqs[min(range(len(reqs)), key=lambda j: reqs[j]["n"])]
+    wid = srv.submit(w["grid"], w["mats"], w["run"], w["d"], w["i"], 4)
+    while srv.info(wid)["state"] < nm.REQ_DONE:
+        srv.advance(4)
+    srv.close()
+
+    # served: the request loop runs on the server's thread; this thread submits at arrival times
+    srv = nm.Server(analysis=nm.THREED, graph_steps=a.graph_steps)
+    srv.start()
+    base = srv.clock()
+    wall = time.perf_counter()
+    ids = []
+    for r, t_arr in zip(reqs, arrivals):
+        delay = t_arr - (time.perf_counter() - wall)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(r["grid"], r["mats"], r["run"], r["d"], r["i"], r["steps"]))
+    for rid in ids:
+        srv.wait(rid)
+    srv.stop()
+    infos = [srv.info(rid) for rid in ids]
+    lat = np.array([inf["finished"] - base - t for inf, t in zip(infos, arrivals)])
+    span = max(inf["finished"] for inf in infos) - base
+    served = total_ps / span
+    layouts = srv.counts()["layouts"]
+    served_out = [srv.collect(rid) for rid in ids]
+    srv.close()
+    print(f"# served: {served:.4e} particle-steps/s over {span:.2f} s, {layouts} layouts, latency p50 "
+          f"{np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s", file=sys.stderr, flush=True)
+
+    # one at a time: a plain solve per request (create, upload, step, download), first come first served
+    service = []
+    equal = 0
+    for r, (d_srv, i_srv) in zip(reqs, served_out):
+        tc = time.perf_counter()
+        sim = nm.Simulation(r["grid"], r["mats"], r["run"])
+        sim.upload(r["d"], r["i"])
+        sim.step(r["steps"])
+        d_one, i_one = sim.particles()
+        sim.close()
+        service.append(time.perf_counter() - tc)
+        equal += int(np.array_equal(d_one, d_srv) and np.array_equal(i_one, i_srv))
+    fin = 0.0
+    slat = []
+    for t_arr, sv in zip(arrivals, service):
+        fin = max(fin, t_arr) + sv
+        slat.append(fin - t_arr)
+    slat = np.array(slat)
+    serial_span = fin
+    serial = total_ps / serial_span
+    print(f"# one at a time: {serial:.4e} particle-steps/s over {serial_span:.2f} s (busy {sum(service):.2f} s), "
+          f"latency p50 {np.percentile(slat, 50):.2f} s p99 {np.percentile(slat, 99):.2f} s; "
+          f"{equal}/{len(reqs)} served results equal their solo runs", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        pairs = [(r["prob"], (r["d"], r["i"])) for r in sorted(reqs, key=lambda r: r["n"])]
+        v, done, ps, t = cpu_sample(pairs, a.cpu_seconds, steps=2)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"2 steps of each of the {done} smallest requests ({ps} particle-steps) on one host "
+                         f"core (oracle/mpm_oracle.cpp, serial), {t:.1f} s"}
+    return {
+        "metric": METRIC, "value": served, "unit": "particle-steps/s", "n_gpus": 1, "steps": int(total_ps),
+        "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"served 3D stream: {a.requests} J2 or neo-Hookean blocks, 50k-2M particles "
+                               f"log-uniform, 200-2000 steps, uGIMP, Poisson arrivals at {a.rate:g}/s",
+                   "particle_steps": int(total_ps), "span_s": span, "layouts": layouts,
+                   "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+                   "one_at_a_time_value": serial, "one_at_a_time_span_s": serial_span,
+                   "one_at_a_time_busy_s": float(sum(service)),
+                   "one_at_a_time_latency_p50_s": float(np.percentile(slat, 50)),
+                   "one_at_a_time_latency_p99_s": float(np.percentile(slat, 99)),
+                   "speedup_vs_one_at_a_time": served / serial,
+                   "results_equal_alone": f"{equal}/{len(reqs)}"},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,9 +315,12 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["j2", "contact", "batch"], default="j2",
+    ap.add_argument("--workload", choices=["j2", "contact", "batch", "serve"], default="j2",
                     help="j2: the 7.08M-particle J2 GIMP block (default); contact: the lCPDI contact pair; "
-                         "batch: the 256-instance 2D sweep")
+                         "batch: the 256-instance 2D sweep; serve: the served 3D stream")
+    ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--graph-steps", type=int, default=8, help="serve: batch steps per graph / admission check")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of sweep instances")
     ap.add_argument("--serial-steps", type=int, default=20, help="batch: steps of each instance run alone")
     ap.add_argument("--cpu-seconds", type=float, default=10., help="batch: CPU baseline sample length")
@@ -232,6 +337,9 @@ def main():
 
     if a.workload == "batch":
         print(json.dumps(bench_batch(a)), flush=True)
+        return
+    if a.workload == "serve":
+        print(json.dumps(bench_serve(a)), flush=True)
         return
 
     lo, hi = a.cells // 8, a.cells - a.cells // 8

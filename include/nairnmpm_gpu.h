@@ -258,6 +258,68 @@ const char *mpmgpu_last_error(const MpmCtx *ctx);
 const char *mpmgpu_global_error(void);
 void mpmgpu_destroy(MpmCtx *ctx);
 
+/* ---- solve server: continuous batching of independent solves -------------------------- */
+/* No counterpart in the reference (NairnMPM runs one analysis per process). A request is a whole
+   solve: grid, materials, run parameters, initial particles and a number of steps. Requests wait
+   in arrival order until their device footprint fits the server's memory budget; admitted
+   requests stay resident on the device and advance together as one batch (one step of every
+   resident solve per batch step, one launch per kernel). When solves finish or new ones are
+   admitted, the batch is laid out again and the resident particle state moves to the new layout
+   on the device. Every result equals the same solve run alone (mpmgpu_create + mpmgpu_step) bit
+   for bit. All requests of a server share its analysis type and shape functions and are
+   single-material-mode problems. */
+typedef struct MpmServer MpmServer;
+
+typedef struct MpmServerParams {
+    int64_t memoryBudget;      /* device bytes for resident solves, moves between layouts included
+                                  (0: 90% of the free device memory at creation) */
+    int32_t maxResident;       /* most solves resident at once (0: no limit) */
+    int32_t graphSteps;        /* batch steps per captured graph and per admission check (even,
+                                  2..64; 0: 8) */
+    int32_t analysis;          /* MPMGPU_PLANE_STRAIN or MPMGPU_THREED */
+    int32_t shape;             /* MPMGPU_SHAPE_UGIMP or MPMGPU_SHAPE_LCPDI */
+} MpmServerParams;
+
+enum { MPMGPU_REQ_QUEUED = 0, MPMGPU_REQ_RESIDENT = 1, MPMGPU_REQ_DONE = 2, MPMGPU_REQ_COLLECTED = 3 };
+
+typedef struct MpmRequestInfo {
+    int32_t state;             /* MPMGPU_REQ_* */
+    int32_t stepsTotal;
+    int32_t stepsDone;
+    int32_t pad;
+    int64_t nparticles;
+    int64_t footprint;         /* device bytes while resident (mpmgpu_footprint) */
+    double submitted;          /* seconds on the server clock (mpmgpu_server_clock) */
+    double admitted;           /* -1 until admitted */
+    double finished;           /* -1 until done */
+} MpmRequestInfo;
+
+/* Device bytes a single-material solve holds: particle rows, grid, tile partials and bins. */
+int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64_t nparticles);
+int mpmgpu_server_create(const MpmServerParams *params, MpmServer **srv);
+/* Queue a solve (the particle rows are copied). Fails for a request that can never fit. */
+int mpmgpu_server_submit(MpmServer *srv, const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat,
+                         const MpmRunParams *run, int64_t n, const double *dfields, const int32_t *ifields,
+                         int32_t steps, int64_t *id);
+/* One round of the request loop on the caller's thread: admit what fits, lay out the batch again
+   if its membership changed, run up to maxSteps batch steps (fewer when a solve finishes sooner)
+   and retire finished solves; *completed = number retired. */
+int mpmgpu_server_advance(MpmServer *srv, int32_t maxSteps, int32_t *completed);
+/* Run the request loop on a thread of the server until mpmgpu_server_stop (advance is then not
+   called by the caller). */
+int mpmgpu_server_start(MpmServer *srv);
+int mpmgpu_server_stop(MpmServer *srv);
+/* Wait until request id is done (or timeoutSec passes: returns 0 with *done = 0). */
+int mpmgpu_server_wait(MpmServer *srv, int64_t id, double timeoutSec, int32_t *done);
+int mpmgpu_server_info(MpmServer *srv, int64_t id, MpmRequestInfo *info);
+/* Final particles of a done request, in its own numbering (materials from 0, mpm[] order). */
+int mpmgpu_server_collect(MpmServer *srv, int64_t id, double *dfields, int32_t *ifields);
+int mpmgpu_server_counts(MpmServer *srv, int32_t *queued, int32_t *resident, int64_t *residentBytes,
+                         int64_t *layouts);
+double mpmgpu_server_clock(const MpmServer *srv);
+const char *mpmgpu_server_error(const MpmServer *srv);
+void mpmgpu_server_destroy(MpmServer *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,87 @@
+// mpm_ctx.h -- the device context shared by the C ABI (mpm_ctx.hip) and the solve server
+// (mpm_server.hip). Internal: not part of the public header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mpm_kernels.h"
+
+struct MpmCtx {
+    int dim = 3;
+    MpmGridDesc gd{};
+    MpmRunParams run{};
+    std::vector<MpmMaterial> mats;
+    KGrid kg{};
+    KPart kp{};
+    KBins kb{};
+    KNodes kn{};
+    KStats ks{};
+    MpmMaterial *dmats = nullptr;
+    double *pbuf = nullptr;           // particle double fields + second position buffer
+    int *ibuf = nullptr;              // particle int fields
+    double *cbuf = nullptr;           // lCPDI semi-side vectors of the step
+    // batch of independent instances (empty for one simulation)
+    std::vector<KGrid> inst;          // host copies of the instance grids
+    std::vector<int> matInstH;        // material -> instance
+    KGrid *dinst = nullptr;
+    int *dtables = nullptr;           // tileInst, nodeBlockInst, matInst
+    int *bbuf = nullptr;              // binning arrays
+    unsigned long long *sbuf = nullptr;
+    long long ncap = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // graphs of graphSteps steps, by position parity
+    int graphSteps = 2;              // even: a graph ends on the parity it started from
+    int parity = 0;                  // which position buffer holds the current positions
+    double *posA[3] = {nullptr, nullptr, nullptr}, *posB[3] = {nullptr, nullptr, nullptr};
+    int nextTask = 0;                // task sequence check for mpmgpu_run_task
+    int64_t steps = 0;
+    double mtime = 0.;
+    float lastStepMs = 0.f;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // kernel timing: events around each kernel group of a step, on this context's stream
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::pair<int, std::pair<int, int>>> evMarks;   // (group, (start ev, end ev))
+    size_t evUsed = 0;
+    double groupMs[MPMGPU_NUM_KERNEL_GROUPS] = {0};
+    int64_t groupLaunches[MPMGPU_NUM_KERNEL_GROUPS] = {0};
+    std::string err;
+
+    int mark() {
+        if (evUsed == evPool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return -1;
+            evPool.push_back(e);
+        }
+        int i = (int)evUsed++;
+        (void)hipEventRecord(evPool[i], stream);
+        return i;
+    }
+
+    int fail(const std::string &m) {
+        err = m;
+        return 1;
+    }
+};
+
+#define HIPCHECK(ctx, call)                                                               \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return (ctx)->fail(std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+namespace mpmi {
+// message for calls without a context (mpmgpu_global_error); returns 1
+int fail_global(const std::string &m);
+// particle buffers for n particles (kept when big enough) and the field pointers into them
+int place_particles(MpmCtx *c, int64_t n);
+// after the particle rows are filled: second position buffer, counters, bins; waits for the stream
+int prepare_particles(MpmCtx *c);
+// current and next position rows by parity
+void set_positions(MpmCtx *c);
+}  // namespace mpmi

@@ -8,84 +8,18 @@
 #include <string>
 #include <vector>
 
-#include "mpm_kernels.h"
+#include "mpm_ctx.h"
 #include "mpm_problem.h"
 
-namespace {
-thread_local std::string g_err;
+static thread_local std::string g_err;
 
-int fail_global(const std::string &m) {
+int mpmi::fail_global(const std::string &m) {
     g_err = m;
     return 1;
 }
-}  // namespace
+using mpmi::fail_global;
 
-struct MpmCtx {
-    int dim = 3;
-    MpmGridDesc gd{};
-    MpmRunParams run{};
-    std::vector<MpmMaterial> mats;
-    KGrid kg{};
-    KPart kp{};
-    KBins kb{};
-    KNodes kn{};
-    KStats ks{};
-    MpmMaterial *dmats = nullptr;
-    double *pbuf = nullptr;           // particle double fields + second position buffer
-    int *ibuf = nullptr;              // particle int fields
-    double *cbuf = nullptr;           // lCPDI semi-side vectors of the step
-    // batch of independent instances (empty for one simulation)
-    std::vector<KGrid> inst;          // host copies of the instance grids
-    std::vector<int> matInstH;        // material -> instance
-    KGrid *dinst = nullptr;
-    int *dtables = nullptr;           // tileInst, nodeBlockInst, matInst
-    int *bbuf = nullptr;              // binning arrays
-    unsigned long long *sbuf = nullptr;
-    long long ncap = 0;
-    hipStream_t stream = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};   // graphs of graphSteps steps, by position parity
-    int graphSteps = 2;              // even: a graph ends on the parity it started from
-    int parity = 0;                  // which position buffer holds the current positions
-    double *posA[3] = {nullptr, nullptr, nullptr}, *posB[3] = {nullptr, nullptr, nullptr};
-    int nextTask = 0;                // task sequence check for mpmgpu_run_task
-    int64_t steps = 0;
-    double mtime = 0.;
-    float lastStepMs = 0.f;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // kernel timing: events around each kernel group of a step, on this context's stream
-    bool profiling = false;
-    std::vector<hipEvent_t> evPool;
-    std::vector<std::pair<int, std::pair<int, int>>> evMarks;   // (group, (start ev, end ev))
-    size_t evUsed = 0;
-    double groupMs[MPMGPU_NUM_KERNEL_GROUPS] = {0};
-    int64_t groupLaunches[MPMGPU_NUM_KERNEL_GROUPS] = {0};
-    std::string err;
-
-    int mark() {
-        if (evUsed == evPool.size()) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return -1;
-            evPool.push_back(e);
-        }
-        int i = (int)evUsed++;
-        (void)hipEventRecord(evPool[i], stream);
-        return i;
-    }
-
-    int fail(const std::string &m) {
-        err = m;
-        return 1;
-    }
-};
-
-#define HIPCHECK(ctx, call)                                                               \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return (ctx)->fail(std::string(#call) + ": " + hipGetErrorString(e_));        \
-    } while (0)
-
-static void set_positions(MpmCtx *c) {
+void mpmi::set_positions(MpmCtx *c) {
     double **cur = c->parity == 0 ? c->posA : c->posB;
     double **nxt = c->parity == 0 ? c->posB : c->posA;
     for (int a = 0; a < 3; a++) {
@@ -93,6 +27,7 @@ static void set_positions(MpmCtx *c) {
         c->kp.posn[a] = nxt[a];
     }
 }
+using mpmi::set_positions;
 
 static void destroy_graphs(MpmCtx *c) {
     for (int i = 0; i < 2; i++)
@@ -331,8 +266,8 @@ void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 static const char *check_run(const MpmGridDesc *grid, const MpmRunParams *run) {
     if (grid->analysis != MPMGPU_PLANE_STRAIN && grid->analysis != MPMGPU_THREED)
         return "analysis type must be 10 (plane strain) or 12 (3D)";
-    if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI)
-        return "shape functions must be uGIMP (1) or lCPDI (10) on the GPU path";
+    if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI && run->shape != MPMGPU_SHAPE_LINEAR)
+        return "shape functions must be classic (0), uGIMP (1) or lCPDI (10) on the GPU path";
     if (run->mpmMethod != 2) return "only the USAVG method is implemented";
     return nullptr;
 }
@@ -464,6 +399,23 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
     return 0;
 }
 
+int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64_t n) {
+    if (check_run(grid, run)) return -1;
+    const int dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
+    KGrid g;
+    fill_grid(g, dim, grid, run);
+    const long long foot = dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
+    const long long nc = dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
+    const long long nodes = (g.nnodes + 255) / 256 * 256;        // as laid out in a batch
+    const long long nk = (long long)g.ntiles * nc;
+    int64_t bytes = nodes * (10 * sizeof(double) + sizeof(int));   // mass, pk, vk, ftot, cnt
+    bytes += (int64_t)g.ntiles * NQMAX * foot * sizeof(double);    // tile partials
+    bytes += (3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1) * sizeof(int);   // bins
+    bytes += n * (int64_t)((MPF_NUM_FIELDS + 3) * sizeof(double) + (MPI_NUM_FIELDS + 2) * sizeof(int));
+    if (run->shape == MPMGPU_SHAPE_LCPDI) bytes += n * 9 * (int64_t)sizeof(double);
+    return bytes;
+}
+
 // Independent instances (each its own grid, time step, materials and particles) stepped together:
 // their tiles and nodes are laid out one after another in one context, every kernel looks up the
 // instance of its tile or node, and one launch per kernel serves all of them.
@@ -532,7 +484,11 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
     return 0;
 }
 
-int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const int32_t *ifields) {
+}  // extern "C"
+
+// ---- particle placement (shared with the solve server) ----------------------------------
+
+int mpmi::place_particles(MpmCtx *c, int64_t n) {
     destroy_graphs(c);
     if (n > 0x7fffffffLL) return c->fail("too many particles for one context");
     if (n > c->ncap) {
@@ -559,6 +515,31 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
     c->kb.key = c->ibuf + 4 * nn;
     c->kb.order = c->ibuf + 5 * nn;
     c->kp.n = n;
+    return 0;
+}
+
+int mpmi::prepare_particles(MpmCtx *c) {
+    const size_t nn = (size_t)c->kp.n;
+    // the second position buffer starts as a copy: in 2D its z row is never written by a step
+    for (int a = 0; a < 3; a++)
+        HIPCHECK(c, hipMemcpyAsync(c->posB[a], c->posA[a], nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream));
+    set_positions(c);
+    launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
+    launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->nextTask = 0;
+    c->steps = 0;
+    c->mtime = 0.;
+    return 0;
+}
+
+extern "C" {
+
+int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const int32_t *ifields) {
+    if (n < 0 || n > 0x7fffffffLL) return c->fail("too many particles for one context");
+    size_t nn = (size_t)n;
     // validate materials and elements on the host before anything reaches the device
     int nmat = (int)c->mats.size();
     const bool batch = !c->inst.empty();
@@ -613,21 +594,10 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         HIPCHECK(c, hipMemcpyAsync(c->dinst, c->inst.data(), c->inst.size() * sizeof(KGrid), hipMemcpyHostToDevice,
                                    c->stream));
     }
+    if (mpmi::place_particles(c, n)) return 1;
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    // the second position buffer starts as a copy: in 2D its z row is never written by a step
-    for (int a = 0; a < 3; a++)
-        HIPCHECK(c, hipMemcpyAsync(c->posB[a], c->posA[a], nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream));
-    set_positions(c);
-    launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
-    launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
-    HIPCHECK(c, hipGetLastError());
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    c->nextTask = 0;
-    c->steps = 0;
-    c->mtime = 0.;
-    return 0;
+    return mpmi::prepare_particles(c);
 }
 
 int mpmgpu_run_task(MpmCtx *c, int32_t task) {

@@ -57,6 +57,17 @@ __device__ __forceinline__ void gimp_axis2(double xi, double lp, double S[4], do
     }
 }
 
+// classic MPM (ElementBase::useGimp = POINT_GIMP): the element's linear functions on one axis,
+// (1 - xi)/2 and (1 + xi)/2 at its two nodes, so that their products are the reference's
+// 0.25*(1+xii*xi)*(1+eti*eta) and 0.125*(...)(...)(...) bit for bit (FourNodeIsoparam.cpp:189-214,
+// EightNodeIsoparamBrick.cpp:87-108); candidates -1 and 2 are not nodes of the particle.
+__device__ __forceinline__ void linear_axis(double xi, double S[4], double dS[4]) {
+    S[0] = -1.; dS[0] = 0.;
+    S[1] = 0.5 * (1. - xi); dS[1] = -0.5;
+    S[2] = 0.5 * (1. + xi); dS[2] = 0.5;
+    S[3] = -1.; dS[3] = 0.;
+}
+
 // candidate tables: offset (-1..2) of candidate id on each axis, in the reference's order
 __constant__ static const signed char c_off3x[64] = {0,1,1,0,0,1,1,0,
     -1,0,1,2,2,2,2,1,0,-1,-1,-1, -1,0,1,2,2,2,2,1,0,-1,-1,-1,

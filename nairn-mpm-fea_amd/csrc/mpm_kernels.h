@@ -120,3 +120,20 @@ void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b,
                         const MpmMaterial *mats, bool newPositions, hipStream_t s);
 void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s);
+
+// A run of particles copied between the particle buffers of two contexts (the solve server moves
+// a resident solve into the next batch layout). Double rows are read from sd (row stride sn)
+// except the positions, which come from the source's current position rows spos; int rows MPI_*
+// likewise from si; material numbers shift by matShift.
+struct KMove {
+    const double *sd;
+    const double *spos[3];
+    const int *si;
+    long long sn, soff;
+    double *dd;
+    int *di;
+    long long dn, doff;
+    long long cnt;
+    int matShift;
+};
+void launch_move(const KMove &m, hipStream_t s);

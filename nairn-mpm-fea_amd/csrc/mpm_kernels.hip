@@ -104,10 +104,16 @@ __device__ __forceinline__ void particle_xi(const KGrid &g, int ei, int ej, int 
     }
 }
 
-// per-axis GIMP factors of one particle
+// per-axis factors of one particle: uGIMP, or classic linear (the same kernels serve both; the
+// choice is uniform over a launch)
 template <int DIM>
 __device__ __forceinline__ void particle_factors(const KGrid &g, const KPart &P, int p, const double xi[3],
                                                  double S[3][4], double dS[3][4]) {
+    if (g.shape == MPMGPU_SHAPE_LINEAR) {
+#pragma unroll
+        for (int a = 0; a < DIM; a++) linear_axis(xi[a], S[a], dS[a]);
+        return;
+    }
     double lpx, lpy, lpz;
     if (g.uniformLp) { lpx = g.lp[0]; lpy = g.lp[1]; lpz = g.lp[2]; }
     else { lpx = P.f[MPF_LP_X][p]; lpy = P.f[MPF_LP_Y][p]; lpz = P.f[MPF_LP_Z][p]; }
@@ -1820,4 +1826,28 @@ void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s) {
     if (dim == 3) hipLaunchKernelGGL(k_contact<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd, callType);
     else hipLaunchKernelGGL(k_contact<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd, callType);
+}
+
+// ---------------------------------------------------------------------------------------
+// particle runs between contexts: one thread per (particle, row); rows 0..MPF_NUM_FIELDS-1 are
+// the double fields, the next MPI_NUM_FIELDS the int fields
+
+__global__ void k_move(KMove m) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (p >= m.cnt) return;
+    if (r < MPF_NUM_FIELDS) {
+        const double *src = (r >= MPF_POS_X && r <= MPF_POS_Z) ? m.spos[r - MPF_POS_X] : m.sd + r * m.sn;
+        m.dd[r * m.dn + m.doff + p] = src[m.soff + p];
+    } else {
+        const int q = r - MPF_NUM_FIELDS;
+        int v = m.si[q * m.sn + m.soff + p];
+        if (q == MPI_MAT) v += m.matShift;
+        m.di[q * m.dn + m.doff + p] = v;
+    }
+}
+
+void launch_move(const KMove &m, hipStream_t s) {
+    if (m.cnt <= 0) return;
+    hipLaunchKernelGGL(k_move, dim3(blocks_for(m.cnt, NT), MPF_NUM_FIELDS + MPI_NUM_FIELDS), dim3(NT), 0, s, m);
 }

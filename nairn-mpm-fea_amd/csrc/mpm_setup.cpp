@@ -44,6 +44,17 @@ bool build_grid(MpmProblem &p) {
     g.thickness = p.thickness;
     long nelems = (long)nx * ny * (p.dim == 3 ? nz : 1);
     p.filled.assign(nelems, 0u);
+    // MPMReadHandler::SetGIMPBorderAsHoles (Generators.cpp:1607-1619, called when </Grid> ends):
+    // with GIMP or CPDI shape functions the elements of the outer layer take no particles
+    // (MeshInfo::EdgeElement2D/3D, MeshInfo.cpp:171-199)
+    if (p.run.shape != MPMGPU_SHAPE_LINEAR) {
+        const long nzc = p.dim == 3 ? nz : 1;
+        for (long e = 0; e < nelems; e++) {
+            long i = e % nx, j = (e / nx) % ny, k = e / ((long)nx * ny);
+            bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || (p.dim == 3 && (k == 0 || k == nzc - 1));
+            if (edge) p.filled[e] = 0xffffffffu;
+        }
+    }
     p.gridBuilt = true;
     return true;
 }

@@ -80,6 +80,20 @@ class InstanceDesc(ctypes.Structure):
                 ("numMaterials", ctypes.c_int32)]
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [("memoryBudget", ctypes.c_int64), ("maxResident", ctypes.c_int32), ("graphSteps", ctypes.c_int32),
+                ("analysis", ctypes.c_int32), ("shape", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("stepsTotal", ctypes.c_int32), ("stepsDone", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("nparticles", ctypes.c_int64), ("footprint", ctypes.c_int64),
+                ("submitted", ctypes.c_double), ("admitted", ctypes.c_double), ("finished", ctypes.c_double)]
+
+
+REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_COLLECTED = 0, 1, 2, 3
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("nparticles", ctypes.c_int64), ("leftGrid", ctypes.c_int64),
                 ("nanParticles", ctypes.c_int64), ("time", ctypes.c_double), ("lastStepMs", ctypes.c_double)]
@@ -125,6 +139,22 @@ _SIGNATURES = {
     "mpmgpu_set_profiling": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_kernel_times": ([_P, _PD, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "mpmgpu_kernel_group_name": ([ctypes.c_int32], ctypes.c_char_p),
+    "mpmgpu_footprint": ([ctypes.POINTER(GridDesc), ctypes.POINTER(RunParams), ctypes.c_int64], ctypes.c_int64),
+    "mpmgpu_server_create": ([ctypes.POINTER(ServerParams), ctypes.POINTER(_P)], ctypes.c_int),
+    "mpmgpu_server_submit": ([_P, ctypes.POINTER(GridDesc), ctypes.POINTER(Material), ctypes.c_int32,
+                              ctypes.POINTER(RunParams), ctypes.c_int64, _PD, _PI, ctypes.c_int32,
+                              ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "mpmgpu_server_advance": ([_P, ctypes.c_int32, _PI], ctypes.c_int),
+    "mpmgpu_server_start": ([_P], ctypes.c_int),
+    "mpmgpu_server_stop": ([_P], ctypes.c_int),
+    "mpmgpu_server_wait": ([_P, ctypes.c_int64, ctypes.c_double, _PI], ctypes.c_int),
+    "mpmgpu_server_info": ([_P, ctypes.c_int64, ctypes.POINTER(RequestInfo)], ctypes.c_int),
+    "mpmgpu_server_collect": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
+    "mpmgpu_server_counts": ([_P, _PI, _PI, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+                             ctypes.c_int),
+    "mpmgpu_server_clock": ([_P], ctypes.c_double),
+    "mpmgpu_server_error": ([_P], ctypes.c_char_p),
+    "mpmgpu_server_destroy": ([_P], None),
 }
 
 KERNEL_GROUPS = ["k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_node_force", "k_g2p_p2g",
@@ -391,3 +421,92 @@ class Batch(Simulation):
         i = i[:, a:b].copy()
         i[IFIELDS.index("mat")] -= self.descs[k].firstMaterial
         return d[:, a:b], i
+
+
+def footprint(grid: GridDesc, run: RunParams, nparticles: int) -> int:
+    """device bytes a single-material solve holds (mpmgpu_footprint)"""
+    return int(library().mpmgpu_footprint(ctypes.byref(grid), ctypes.byref(run), int(nparticles)))
+
+
+class Server:
+    """Solve server (mpmgpu_server_*): queued solves are admitted by device footprint, stay
+    resident and advance together as one batch; each result equals the solve run alone."""
+
+    def __init__(self, analysis: int = THREED, shape: int = SHAPE_UGIMP, memory_budget: int = 0,
+                 max_resident: int = 0, graph_steps: int = 8):
+        p = ServerParams(int(memory_budget), int(max_resident), int(graph_steps), int(analysis), int(shape))
+        h = _P()
+        if library().mpmgpu_server_create(ctypes.byref(p), ctypes.byref(h)) != 0:
+            raise MPMError(library().mpmgpu_server_error(None).decode())
+        self._h = h
+        self._n = {}
+
+    def _ok(self, status: int):
+        if status != 0:
+            msg = library().mpmgpu_server_error(self._h)
+            raise MPMError(msg.decode() if msg else "server error")
+
+    def submit(self, grid, mats, run, dfields, ifields, steps: int) -> int:
+        d = np.ascontiguousarray(dfields, dtype=np.float64)
+        i = np.ascontiguousarray(ifields, dtype=np.int32)
+        arr = (Material * len(mats))(*mats)
+        rid = ctypes.c_int64()
+        self._ok(library().mpmgpu_server_submit(self._h, ctypes.byref(grid), arr, len(mats), ctypes.byref(run),
+                                                d.shape[1], _dptr(d), _iptr(i), int(steps), ctypes.byref(rid)))
+        self._n[rid.value] = d.shape[1]
+        return rid.value
+
+    def submit_problem(self, prob: Problem, steps: int, particles=None) -> int:
+        g, mats, run, _, _ = prob.describe()
+        d, i = prob.particles() if particles is None else particles
+        return self.submit(g, mats, run, d, i, steps)
+
+    def advance(self, max_steps: int = 8) -> int:
+        """one round of the request loop on this thread; returns the number of solves retired"""
+        c = ctypes.c_int32()
+        self._ok(library().mpmgpu_server_advance(self._h, int(max_steps), ctypes.byref(c)))
+        return c.value
+
+    def start(self):
+        self._ok(library().mpmgpu_server_start(self._h))
+
+    def stop(self):
+        self._ok(library().mpmgpu_server_stop(self._h))
+
+    def wait(self, rid: int, timeout: float = -1.0) -> bool:
+        done = ctypes.c_int32()
+        self._ok(library().mpmgpu_server_wait(self._h, int(rid), float(timeout), ctypes.byref(done)))
+        return bool(done.value)
+
+    def info(self, rid: int) -> dict:
+        r = RequestInfo()
+        self._ok(library().mpmgpu_server_info(self._h, int(rid), ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in RequestInfo._fields_ if k != "pad"}
+
+    def collect(self, rid: int):
+        n = self._n[rid]
+        d = np.zeros((NUM_FIELDS, n), dtype=np.float64)
+        i = np.zeros((NUM_IFIELDS, n), dtype=np.int32)
+        self._ok(library().mpmgpu_server_collect(self._h, int(rid), _dptr(d), _iptr(i)))
+        return d, i
+
+    def counts(self) -> dict:
+        q, r = ctypes.c_int32(), ctypes.c_int32()
+        b, lay = ctypes.c_int64(), ctypes.c_int64()
+        self._ok(library().mpmgpu_server_counts(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b),
+                                                ctypes.byref(lay)))
+        return {"queued": q.value, "resident": r.value, "resident_bytes": b.value, "layouts": lay.value}
+
+    def clock(self) -> float:
+        return float(library().mpmgpu_server_clock(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mpmgpu_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -342,5 +342,25 @@ def sweep_instance_2d(k: int, pad: int = 8, cfl: float = 0.2):
     return xml, particle_fix, j2
 
 
+def serve_request_3d(k: int, pad: int = 8, nmin: int = 50000, nmax: int = 2000000,
+                     smin: int = 200, smax: int = 2000):
+    """Request k of the served 3D stream: seed 5000+k; Np = round(exp(U(ln nmin, ln nmax)))
+    particles as a cube block at 2x2x2 per 1 mm cell with `pad` cells of grid around it; J2 (the
+    7M-particle block's material, halves colliding at +-50 m/s) or neo-Hookean (G 10 MPa, K 50 MPa,
+    rho 1.1, halves at +-5 m/s) with probability 1/2; steps uniform in [smin, smax].
+    Returns (xml, steps, is_j2)."""
+    rng = SplitMix64(5000 + k)
+    npart = math.exp(math.log(nmin) + rng.uniform() * (math.log(nmax) - math.log(nmin)))
+    side = max(2, round((npart / 8.) ** (1. / 3.)))
+    j2 = rng.uniform() < 0.5
+    steps = smin + int(rng.uniform() * (smax - smin + 1))
+    cells = side + 2 * pad
+    if j2:
+        xml = j2_block_3d(cells=cells, lo=pad, hi=pad + side)
+    else:
+        xml = neohookean_block_3d(cells=cells, lo=pad, hi=pad + side, vx=5000.)
+    return xml, steps, j2
+
+
 # rows of the particle field array (nairnmpm.FIELDS)
 F_POS_X, F_POS_Y, F_VEL_X, F_ORIG_X, F_ORIG_Y = 0, 1, 3, 34, 35

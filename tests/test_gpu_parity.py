@@ -1,5 +1,5 @@
-"""HIP path against the CPU oracle on the same seeded inputs (tolerance 1e-9 relative L2, the
-north-star bound; the gap is summation order of the grid sums only)."""
+"""HIP path against the CPU oracle on the same seeded inputs (tolerance 1e-9 relative L2 after up
+to 1000 steps; the gap is summation order of the grid sums only)."""
 import numpy as np
 import pytest
 
@@ -257,6 +257,34 @@ def test_lcpdi_elastic_2d():
         ora.step(target - done)
         done = target
         _check(sim, ora, 2, f"lCPDI elastic 2D step {target}")
+
+
+# ---- classic MPM (linear element shape functions) ----------------------------------------
+
+def test_classic_elastic_2d():
+    prob, sim, ora = _pair(configs.elastic_block_2d(cells=32, block=16, shape="Classic"))
+    done = 0
+    for target in (1, 100, 300):
+        sim.step(target - done)
+        ora.step(target - done)
+        done = target
+        _check(sim, ora, 2, f"classic elastic 2D step {target}")
+
+
+def test_classic_j2_3d():
+    prob, sim, ora = _pair(configs.j2_block_3d(cells=12, lo=3, hi=9, shape="Classic"))
+    sim.step(40)
+    ora.step(40)
+    _check(sim, ora, 3, "classic J2 3D step 40")
+
+
+def test_classic_contact_cylinders():
+    xml = configs.contact_cylinders_3d(cells=24, radius=5., length=8., gap=1., shape="Classic",
+                                       law_type=61, friction=0.0)
+    prob, sim, ora = _pair(xml)
+    sim.step(60)
+    ora.step(60)
+    _check(sim, ora, 3, "classic contact 3D step 60")
 
 
 def test_lcpdi_j2_3d():

@@ -151,3 +151,34 @@ def test_programmatic_problem_matches_xml():
     ob = np.lexsort(db[:3][::-1])
     np.testing.assert_array_equal(da[:, oa], db[:, ob])
     np.testing.assert_array_equal(ia[:2, oa], ib[:2, ob])
+
+
+@pytest.mark.parametrize("gimp, expect", [("uGIMP", 10 * 10 * 4), ("Classic", 12 * 12 * 4), ("lCPDI", 10 * 10 * 4)])
+def test_gimp_border_layer_takes_no_particles(gimp, expect):
+    """A body larger than the grid fills the one-cell padding layer only with classic shape
+    functions: GIMP and CPDI mark the border elements as holes (Generators.cpp:1607-1619)."""
+    xml = configs._header(10, 0.5, 1.0, "    <MatlPtsPerElement>4</MatlPtsPerElement>\n", shape=gimp) + """  <Mesh output="file">
+    <Grid xmin="0" xmax="10" ymin="0" ymax="10">
+      <Horiz cellsize="1"/>
+      <Vert cellsize="1"/>
+    </Grid>
+  </Mesh>
+  <MaterialPoints>
+    <Body mat="1" vx="0" vy="0">
+      <Rect xmin="-5" xmax="15" ymin="-5" ymax="15"/>
+    </Body>
+  </MaterialPoints>
+  <Material Type="1" Name="e">
+    <rho>1.0</rho>
+    <E>1000</E>
+    <nu>0.3</nu>
+  </Material>
+</JANFEAInput>
+"""
+    prob = nm.Problem.from_xml(xml)
+    g, _, run, n, _ = prob.describe()
+    assert (g.nx, g.ny) == (12, 12)
+    assert n == expect
+    d, _ = prob.particles()
+    if gimp != "Classic":
+        assert d[nm.F["pos_x"]].min() > 0. and d[nm.F["pos_x"]].max() < 10.
