@@ -558,6 +558,10 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
             return c->fail("particle material without a velocity field");
         if (e < 0 || e >= nelem) return c->fail("particle element outside the grid");
         if (!(dfields[MPF_MASS * nn + i] > 0.)) return c->fail("particle with a mass that is not positive");
+        for (int a = 0; a < c->dim; a++) {
+            const double l = dfields[(MPF_LP_X + a) * nn + i];
+            if (!(l > 0. && l <= 1.)) return c->fail("particle half size must be in (0, 1] cell half widths");
+        }
     }
     // one material type for all particles lets the strain kernels compile a single law
     c->kg.matType = 0;

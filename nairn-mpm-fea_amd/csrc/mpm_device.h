@@ -68,18 +68,6 @@ __device__ __forceinline__ void linear_axis(double xi, double S[4], double dS[4]
     S[3] = -1.; dS[3] = 0.;
 }
 
-// candidate tables: offset (-1..2) of candidate id on each axis, in the reference's order
-__constant__ static const signed char c_off3x[64] = {0,1,1,0,0,1,1,0,
-    -1,0,1,2,2,2,2,1,0,-1,-1,-1, -1,0,1,2,2,2,2,1,0,-1,-1,-1,
-    0,1,1,0,-1,0,1,2,2,2,2,1,0,-1,-1,-1, 0,1,1,0,-1,0,1,2,2,2,2,1,0,-1,-1,-1};
-__constant__ static const signed char c_off3y[64] = {0,0,1,1,0,0,1,1,
-    -1,-1,-1,-1,0,1,2,2,2,2,1,0, -1,-1,-1,-1,0,1,2,2,2,2,1,0,
-    0,0,1,1,-1,-1,-1,-1,0,1,2,2,2,2,1,0, 0,0,1,1,-1,-1,-1,-1,0,1,2,2,2,2,1,0};
-__constant__ static const signed char c_off3z[64] = {0,0,0,0,1,1,1,1,
-    0,0,0,0,0,0,0,0,0,0,0,0, 1,1,1,1,1,1,1,1,1,1,1,1,
-    -1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1,-1, 2,2,2,2,2,2,2,2,2,2,2,2,2,2,2,2};
-__constant__ static const signed char c_off2x[16] = {0, 1, 1, 0, -1, 0, 1, 2, 2, 2, 2, 1, 0, -1, -1, -1};
-__constant__ static const signed char c_off2y[16] = {0, 0, 1, 1, -1, -1, -1, -1, 0, 1, 2, 2, 2, 2, 1, 0};
 
 // ---------------------------------------------------------------------------------------
 // 3x3 matrix with the reference's 2D flag semantics (Common/System/Matrix3.cpp)

@@ -482,9 +482,18 @@ __global__ void k_bin_sort_cells(KBins B) {
 // therefore the same fixed-order sum on every run.
 #define NW (NT / 64)
 
+// Footprint columns one wave's cells touch: wave w takes cells c = w, w+NW, ..., whose x index
+// runs from w to w + TX - NW (TX is a multiple of NW), so with node offsets -1..2 its accumulator
+// needs footprint columns w .. w + WX - 1 only.
+template <int DIM> struct WaveFoot {
+    static_assert(TileGeom<DIM>::TX % NW == 0, "tile width must be a multiple of the waves per block");
+    static constexpr int WX = TileGeom<DIM>::TX - NW + 4;
+    static constexpr int SIZE = WX * TileGeom<DIM>::FY * TileGeom<DIM>::FZ;
+};
+
 template <int DIM, int NQ, typename Load, typename Add>
 __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
-                                           double (*s_acc)[NQ][TileGeom<DIM>::FOOT], Load load, Add add) {
+                                           double (*s_acc)[NQ][WaveFoot<DIM>::SIZE], Load load, Add add) {
     using G = TileGeom<DIM>;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int o1x = l & 3, o1y = (l >> 2) & 3, o1z = DIM == 3 ? (l >> 4) : 1;   // offset + 1
@@ -513,7 +522,8 @@ __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
                 auto a0 = load(k - r0, o1x, o1y, o1z);
                 add(a0, acc, lx, ly, lz);
             }
-            int slot = foot_slot<DIM>(lx + o1x - 1, ly + o1y - 1, lz + o1z - 1);
+            const int row = DIM == 3 ? (lz + o1z) * G::FY + (ly + o1y) : ly + o1y;   // footprint row
+            const int slot = row * WaveFoot<DIM>::WX + (lx + o1x - w);
 #pragma unroll
             for (int q = 0; q < NQ; q++) s_acc[w][q][slot] += acc[q];
         }
@@ -524,24 +534,33 @@ __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
 // particle records of the scatters: per-axis factors of the lane's candidate and the payload
 struct RecMass { double wx, wy, wz, mp, vx, vy, vz; };
 struct RecForce { double wx, wy, wz, dx, dy, dz, negmp, s0, s1, s2, s3, s4, s5; };
+struct RecForce2 { double2 x, y, z; double a[6]; };
 
 template <int DIM, int NQ>
-__device__ __forceinline__ void zero_acc(double (*s_acc)[NQ][TileGeom<DIM>::FOOT]) {
-    using G = TileGeom<DIM>;
+__device__ __forceinline__ void zero_acc(double (*s_acc)[NQ][WaveFoot<DIM>::SIZE]) {
     double *a = &s_acc[0][0][0];
-    for (int i = threadIdx.x; i < NW * NQ * G::FOOT; i += NT) a[i] = 0.;
+    for (int i = threadIdx.x; i < NW * NQ * WaveFoot<DIM>::SIZE; i += NT) a[i] = 0.;
 }
 
-// add the wave copies in wave order and store the tile's footprint sums
+// add the wave copies that cover each footprint node in wave order and store the tile's sums
 template <int DIM, int NQ>
-__device__ __forceinline__ void write_partials(const KNodes &N, int t, double (*s_acc)[NQ][TileGeom<DIM>::FOOT]) {
+__device__ __forceinline__ void write_partials(const KNodes &N, int t, double (*s_acc)[NQ][WaveFoot<DIM>::SIZE]) {
     using G = TileGeom<DIM>;
+    constexpr int WX = WaveFoot<DIM>::WX;
     for (int s = threadIdx.x; s < G::FOOT; s += NT) {
+        const int fx = s % G::FX, row = s / G::FX;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
-            double v = s_acc[0][q][s];
+            double v = 0.;
+            bool first = true;
 #pragma unroll
-            for (int w = 1; w < NW; w++) v += s_acc[w][q][s];
+            for (int w = 0; w < NW; w++) {
+                const int c = fx - w;
+                if (c < 0 || c >= WX) continue;
+                const double a = s_acc[w][q][row * WX + c];
+                v = first ? a : v + a;
+                first = false;
+            }
             N.part[((size_t)t * NQMAX + q) * G::FOOT + s] = v;
         }
     }
@@ -562,13 +581,13 @@ template <int DIM, int SH>
 __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNodes N, KStats st) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
-    constexpr int CH = CP ? 64 : NT;          // particles staged per chunk
+    constexpr int CH = CP ? 64 : NT / 2;      // particles staged per chunk (three blocks per CU)
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_S[CP ? 1 : CH][DIM][4];
     __shared__ double s_cxi[CP ? CH : 1][8][3];
     __shared__ int s_coff[CP ? CH : 1][8][3];
     __shared__ double s_q[CH][4];
-    __shared__ double s_acc[NW][NQ_MASS][G::FOOT];
+    __shared__ double s_acc[NW][NQ_MASS][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
     const KGrid g = tile_grid(g0, tall % g0.tilesTotal);
     load_tile_starts<DIM>(B, tall, s_cs);
@@ -598,13 +617,18 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
                         s_coff[threadIdx.x][c][a] = cc.off[c][a];
                     }
             } else {
+                // factors with zeros outside the support; the x factor carries mp
                 double xi[3], invd[3], S[3][4], dS[3][4];
                 particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
                 particle_factors<DIM>(g, P, p, xi, S, dS);
+                const double mp = P.f[MPF_MASS][p];
 #pragma unroll
                 for (int a = 0; a < DIM; a++)
 #pragma unroll
-                    for (int o = 0; o < 4; o++) s_S[threadIdx.x][a][o] = S[a][o];
+                    for (int o = 0; o < 4; o++) {
+                        const double v = S[a][o] >= 0. ? S[a][o] : 0.;
+                        s_S[threadIdx.x][a][o] = a == 0 ? v * mp : v;
+                    }
             }
             s_q[threadIdx.x][0] = P.f[MPF_MASS][p];
             s_q[threadIdx.x][1] = P.f[MPF_VEL_X][p];
@@ -625,13 +649,13 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
                 return r;
             },
             [&](const RecMass &r, double *acc, int, int, int) {
-                bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
-                double w = DIM == 3 ? r.wx * r.wy * r.wz : r.wx * r.wy;
-                double fnmp = on ? w * r.mp : 0.;
-                acc[1] += fnmp * r.vx;
-                acc[2] += fnmp * r.vy;
-                acc[3] += fnmp * r.vz;
+                // lCPDI: merged weight (zero for non-member nodes) times mp; otherwise the
+                // product of the staged factors, mp included
+                const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
                 acc[0] += fnmp;
+                acc[1] = fma(fnmp, r.vx, acc[1]);
+                acc[2] = fma(fnmp, r.vy, acc[2]);
+                acc[3] = fma(fnmp, r.vz, acc[3]);
             });
         __syncthreads();
     }
@@ -706,46 +730,61 @@ __device__ __forceinline__ void load_foot(const KGrid &g, int ox, int oy, int oz
     }
 }
 
-// velocity gradient of one particle from footprint velocities, candidates in reference order
+// Support window of one particle: with a half size lp <= 1 (natural units) a particle with xi < 0
+// touches no node at offset 2 and one with xi >= 0 none at offset -1, so three offsets per axis
+// starting at w = 0 (offsets -1..1) or w = 1 (0..2) hold every node of its support. The window's
+// factors are stored with zeros for nodes outside the support, and derivatives times 2/dx.
+template <int DIM>
+__device__ __forceinline__ void support_window(const double xi[3], const double S[3][4], const double dS[3][4],
+                                               const double invd[3], int w[3], double Sw[3][3], double Dw[3][3]) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        w[a] = a < DIM && xi[a] >= 0. ? 1 : 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (a >= DIM) { Sw[a][j] = j == 1 ? 1. : 0.; Dw[a][j] = 0.; continue; }
+            const double sv = w[a] ? S[a][j + 1] : S[a][j];
+            const double dv = w[a] ? dS[a][j + 1] : dS[a][j];
+            const bool in = sv >= 0.;
+            Sw[a][j] = in ? sv : 0.;
+            Dw[a][j] = in ? dv * invd[a] : 0.;
+        }
+    }
+}
+
+// velocity gradient of one particle from footprint velocities over its support window
 // (MatPoint3D::ExtrapVelocityGradient, MatPoint3D.cpp:62-94; MatPoint2D.cpp:57-89)
 template <int DIM>
-__device__ __forceinline__ void velocity_gradient(const double S[3][4], const double dS[3][4],
+__device__ __forceinline__ void velocity_gradient(const double xi[3], const double S[3][4], const double dS[3][4],
                                                   const double invd[3], int lx, int ly, int lz,
                                                   const double (*s_v)[TileGeom<DIM>::FOOT], double L[3][3]) {
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) L[a][b] = 0.;
-    if (DIM == 3) {
+    int w[3];
+    double Sw[3][3], Dw[3][3];
+    support_window<DIM>(xi, S, dS, invd, w, Sw, Dw);
+    constexpr int NZ = DIM == 3 ? 3 : 1;
+#pragma unroll 1
+    for (int jz = 0; jz < NZ; jz++)
 #pragma unroll
-        for (int id = 0; id < 64; id++) {
-            int ox = c_off3x[id] + 1, oy = c_off3y[id] + 1, oz = c_off3z[id] + 1;
-            double Sx = S[0][ox], Sy = S[1][oy], Sz = S[2][oz];
-            if (Sx < 0. || Sy < 0. || Sz < 0.) continue;
-            double gx = dS[0][ox] * Sy * Sz * invd[0];
-            double gy = Sx * dS[1][oy] * Sz * invd[1];
-            double gz = Sx * Sy * dS[2][oz] * invd[2];
-            int s = foot_slot<DIM>(lx + ox - 1, ly + oy - 1, lz + oz - 1);
-            double vx = s_v[0][s], vy = s_v[1][s], vz = s_v[2][s];
-            L[0][0] += vx * gx; L[0][1] += vx * gy; L[0][2] += vx * gz;
-            L[1][0] += vy * gx; L[1][1] += vy * gy; L[1][2] += vy * gz;
-            L[2][0] += vz * gx; L[2][1] += vz * gy; L[2][2] += vz * gz;
-        }
-    } else {
+        for (int jy = 0; jy < 3; jy++)
 #pragma unroll
-        for (int id = 0; id < 16; id++) {
-            int ox = c_off2x[id] + 1, oy = c_off2y[id] + 1;
-            double Sx = S[0][ox], Sy = S[1][oy];
-            if (Sx < 0. || Sy < 0.) continue;
-            double gx = dS[0][ox] * Sy * invd[0];
-            double gy = Sx * dS[1][oy] * invd[1];
-            int s = foot_slot<DIM>(lx + ox - 1, ly + oy - 1, 0);
-            double vx = s_v[0][s], vy = s_v[1][s];
-            L[0][0] += vx * gx; L[0][1] += vx * gy;
-            L[1][0] += vy * gx; L[1][1] += vy * gy;
-            L[2][2] += 0.;
-        }
-    }
+            for (int jx = 0; jx < 3; jx++) {
+                const int zz = DIM == 3 ? jz : 1;
+                const double syz = Sw[1][jy] * Sw[2][zz], sxz = Sw[0][jx] * Sw[2][zz], sxy = Sw[0][jx] * Sw[1][jy];
+                const double gx = Dw[0][jx] * syz, gy = Dw[1][jy] * sxz, gz = Dw[2][zz] * sxy;
+                const int s = foot_slot<DIM>(lx + w[0] + jx - 1, ly + w[1] + jy - 1, DIM == 3 ? lz + w[2] + jz - 1 : 0);
+                const double vx = s_v[0][s], vy = s_v[1][s];
+                L[0][0] = fma(vx, gx, L[0][0]); L[0][1] = fma(vx, gy, L[0][1]);
+                L[1][0] = fma(vy, gx, L[1][0]); L[1][1] = fma(vy, gy, L[1][1]);
+                if (DIM == 3) {
+                    const double vz = s_v[2][s];
+                    L[0][2] = fma(vx, gz, L[0][2]); L[1][2] = fma(vy, gz, L[1][2]);
+                    L[2][0] = fma(vz, gx, L[2][0]); L[2][1] = fma(vz, gy, L[2][1]); L[2][2] = fma(vz, gz, L[2][2]);
+                }
+            }
 }
 
 // Matrix3::Scale (Matrix3.cpp:410-424)
@@ -756,7 +795,7 @@ __device__ __forceinline__ void scale_gradient(double L[3][3], double f) {
 }
 
 // velocity gradient of particle p from footprint velocities (lx,ly,lz = its cell in the tile):
-// uGIMP candidates in table order, or lCPDI nodes in list order
+// the support window of its shape functions, or lCPDI nodes in list order
 template <int DIM, int SH>
 __device__ __forceinline__ void particle_gradient(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek,
                                                   int lx, int ly, int lz, const double (*s_v)[TileGeom<DIM>::FOOT],
@@ -786,7 +825,7 @@ __device__ __forceinline__ void particle_gradient(const KGrid &g, const KPart &P
         particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
                          DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
         particle_factors<DIM>(g, P, p, xi, S, dS);
-        velocity_gradient<DIM>(S, dS, invd, lx, ly, lz, s_v, L);
+        velocity_gradient<DIM>(xi, S, dS, invd, lx, ly, lz, s_v, L);
     }
 }
 
@@ -886,31 +925,23 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
     constexpr int CH = CP ? 64 : NT / 2;      // particles staged per chunk (keeps LDS < 80 KB)
+    constexpr int NA = DIM == 3 ? 6 : 3;      // -mp times the stress terms (shape-function path)
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_S[CP ? 1 : CH][DIM][4];
-    __shared__ double s_dS[CP ? 1 : CH][DIM][4];
+    // shape-function path: per axis and candidate offset, the factor and its derivative times
+    // 2/dx (zero outside the support), read as one pair
+    __shared__ double2 s_F[CP ? 1 : CH][DIM][4];
+    __shared__ double s_a[CP ? 1 : CH][NA];
     __shared__ double s_cxi[CP ? CH : 1][8][3];
     __shared__ double s_cwg[CP ? CH : 1][8][3];
     __shared__ int s_coff[CP ? CH : 1][8][3];
-    __shared__ double s_st[CH][NS];
-    __shared__ double s_acc[NW][NQ_FORCE][G::FOOT];
-    __shared__ double s_id[DIM][G::TX];       // 2/dx of the tile's cell columns, rows, layers
+    __shared__ double s_st[CP ? CH : 1][NS];
+    __shared__ double s_acc[NW][NQ_FORCE][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
     const int fld = tall / g0.tilesTotal;
     const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
-    const int t = tall - fld * g0.tilesTotal - g.tileBase;
     load_tile_starts<DIM>(B, tall, s_cs);
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
-    int ox, oy, oz;
-    tile_origin<DIM>(g, t, ox, oy, oz);
-    if (threadIdx.x < G::TX) {
-        int i = threadIdx.x;
-        s_id[0][i] = 2.0 / ((g.xmin + (double)(ox + i + 1) * g.dx) - (g.xmin + (double)(ox + i) * g.dx));
-        if (i < G::TY) s_id[1][i] = 2.0 / ((g.ymin + (double)(oy + i + 1) * g.dy) - (g.ymin + (double)(oy + i) * g.dy));
-        if (DIM == 3 && i < G::TZ)
-            s_id[DIM - 1][i] = 2.0 / ((g.zmin + (double)(oz + i + 1) * g.dz) - (g.zmin + (double)(oz + i) * g.dz));
-    }
     zero_acc<DIM, NQ_FORCE>(s_acc);
     for (int c0 = tb; c0 < te; c0 += CH) {
         int k = c0 + threadIdx.x;
@@ -929,6 +960,20 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                         s_cwg[threadIdx.x][c][a] = cc.wg[c][a];
                         s_coff[threadIdx.x][c][a] = cc.off[c][a];
                     }
+            }
+            double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
+            const double negmp = -P.f[MPF_MASS][p];
+            const double sxx = P.f[MPF_SP_XX][p] - Pr, syy = P.f[MPF_SP_YY][p] - Pr, sxy = P.f[MPF_SP_XY][p];
+            if (CP) {
+                s_st[threadIdx.x][0] = negmp;
+                s_st[threadIdx.x][1] = sxx;
+                s_st[threadIdx.x][2] = syy;
+                s_st[threadIdx.x][3] = sxy;
+                if (DIM == 3) {
+                    s_st[threadIdx.x][NS - 3] = P.f[MPF_SP_ZZ][p] - Pr;
+                    s_st[threadIdx.x][NS - 2] = P.f[MPF_SP_XZ][p];
+                    s_st[threadIdx.x][NS - 1] = P.f[MPF_SP_YZ][p];
+                }
             } else {
                 double xi[3], invd[3], S[3][4], dS[3][4];
                 particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
@@ -938,55 +983,75 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                 for (int a = 0; a < DIM; a++)
 #pragma unroll
                     for (int o = 0; o < 4; o++) {
-                        s_S[threadIdx.x][a][o] = S[a][o];
-                        s_dS[threadIdx.x][a][o] = dS[a][o];
+                        const bool in = S[a][o] >= 0.;
+                        s_F[threadIdx.x][a][o] = make_double2(in ? S[a][o] : 0., in ? dS[a][o] * invd[a] : 0.);
                     }
-            }
-            double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
-            s_st[threadIdx.x][0] = -P.f[MPF_MASS][p];
-            s_st[threadIdx.x][1] = P.f[MPF_SP_XX][p] - Pr;
-            s_st[threadIdx.x][2] = P.f[MPF_SP_YY][p] - Pr;
-            s_st[threadIdx.x][3] = P.f[MPF_SP_XY][p];
-            if (DIM == 3) {
-                s_st[threadIdx.x][NS - 3] = P.f[MPF_SP_ZZ][p] - Pr;
-                s_st[threadIdx.x][NS - 2] = P.f[MPF_SP_XZ][p];
-                s_st[threadIdx.x][NS - 1] = P.f[MPF_SP_YZ][p];
+                s_a[threadIdx.x][0] = negmp * sxx;
+                s_a[threadIdx.x][1] = negmp * syy;
+                s_a[threadIdx.x][2] = negmp * sxy;
+                if (DIM == 3) {
+                    s_a[threadIdx.x][3] = negmp * (P.f[MPF_SP_ZZ][p] - Pr);
+                    s_a[threadIdx.x][4] = negmp * P.f[MPF_SP_XZ][p];
+                    s_a[threadIdx.x][5] = negmp * P.f[MPF_SP_YZ][p];
+                }
             }
         }
         __syncthreads();
+        if (!CP) {
+            // f = -mp (sigma - P) . grad N; grad N from the staged factor pairs, accumulated with FMA
+            wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
+                [&](int lq, int ix, int iy, int iz) {
+                    RecForce2 r;
+                    r.x = s_F[lq][0][ix];
+                    r.y = s_F[lq][1][iy];
+                    r.z = DIM == 3 ? s_F[lq][DIM - 1][iz] : make_double2(1., 0.);
+#pragma unroll
+                    for (int q = 0; q < NA; q++) r.a[q] = s_a[lq][q];
+                    return r;
+                },
+                [&](const RecForce2 &r, double *acc, int, int, int) {
+                    if (DIM == 3) {
+                        const double gx = r.x.y * (r.y.x * r.z.x);
+                        const double gy = r.y.y * (r.x.x * r.z.x);
+                        const double gz = r.z.y * (r.x.x * r.y.x);
+                        // a: xx, yy, xy, zz, xz, yz
+                        acc[0] = fma(r.a[0], gx, fma(r.a[2], gy, fma(r.a[4], gz, acc[0])));
+                        acc[1] = fma(r.a[2], gx, fma(r.a[1], gy, fma(r.a[5], gz, acc[1])));
+                        acc[2] = fma(r.a[4], gx, fma(r.a[5], gy, fma(r.a[3], gz, acc[2])));
+                    } else {
+                        const double gx = r.x.y * r.y.x;
+                        const double gy = r.y.y * r.x.x;
+                        acc[0] = fma(r.a[0], gx, fma(r.a[2], gy, acc[0]));
+                        acc[1] = fma(r.a[2], gx, fma(r.a[1], gy, acc[1]));
+                    }
+                });
+            __syncthreads();
+            continue;
+        }
         wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
             [&](int lq, int ix, int iy, int iz) {
                 RecForce r;
-                if (CP) {
+                {
                     double gr[3];
                     cpdi_node<DIM>(s_cxi[lq], s_cwg[lq], s_coff[lq], 0, ix - 1, iy - 1, iz - 1, r.wx, gr);
                     r.wy = r.wz = 1.;
                     r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
-                } else {
-                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
-                    r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
-                    if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
-                    else { r.wz = 1.; r.dz = 0.; }
                 }
                 r.negmp = s_st[lq][0]; r.s0 = s_st[lq][1]; r.s1 = s_st[lq][2]; r.s2 = s_st[lq][3];
                 if (DIM == 3) { r.s3 = s_st[lq][NS - 3]; r.s4 = s_st[lq][NS - 2]; r.s5 = s_st[lq][NS - 1]; }
                 else { r.s3 = r.s4 = r.s5 = 0.; }
                 return r;
             },
-            [&](const RecForce &r, double *acc, int lx, int ly, int lz) {
-                bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
-                double negmp = on ? r.negmp : 0.;
+            [&](const RecForce &r, double *acc, int, int, int) {
+                const double negmp = r.negmp;        // lCPDI: non-member nodes have zero gradient
                 if (DIM == 3) {
-                    double gx = CP ? r.dx : r.dx * r.wy * r.wz * s_id[0][lx];
-                    double gy = CP ? r.dy : r.wx * r.dy * r.wz * s_id[1][ly];
-                    double gz = CP ? r.dz : r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
+                    double gx = r.dx, gy = r.dy, gz = r.dz;
                     // s0 = sxx-P, s1 = syy-P, s2 = sxy, s3 = szz-P, s4 = sxz, s5 = syz
                     acc[0] += negmp * (r.s0 * gx + r.s2 * gy + r.s4 * gz);
                     acc[1] += negmp * (r.s2 * gx + r.s1 * gy + r.s5 * gz);
                     acc[2] += negmp * (r.s4 * gx + r.s5 * gy + r.s3 * gz);
                 } else {
-                    double gx = CP ? r.dx : r.dx * r.wy * s_id[0][lx];
-                    double gy = CP ? r.dy : r.wx * r.dy * s_id[1][ly];
+                    double gx = r.dx, gy = r.dy;
                     acc[0] += negmp * (r.s0 * gx + r.s2 * gy);
                     acc[1] += negmp * (r.s2 * gx + r.s1 * gy);
                 }
@@ -1042,12 +1107,12 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int HALF = NT / 2;
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_n[7][G::FOOT];         // v(3), f(3), mass
+    __shared__ double s_n[6][G::FOOT];         // v(3), f/m(3)
     __shared__ double s_S[CP ? 1 : HALF][DIM][4];
     __shared__ double s_cxi[CP ? HALF : 1][8][3];
     __shared__ int s_coff[CP ? HALF : 1][8][3];
     __shared__ double s_q[HALF][4];
-    __shared__ double s_acc[NW][NQ_MOM][G::FOOT];
+    __shared__ double s_acc[NW][NQ_MOM][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
     const int fld = tall / g0.tilesTotal;
     const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
@@ -1059,10 +1124,19 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
     int ox, oy, oz;
     tile_origin<DIM>(g, t, ox, oy, oz);
     {
-        long long nn = g.nstride;
-        const double *src[7] = {Nf.vk, Nf.vk, Nf.vk, Nf.ftot, Nf.ftot, Nf.ftot, Nf.mass};
-        const long long st[7] = {0, nn, 2 * nn, 0, nn, 2 * nn, 0};
-        load_foot<DIM, 7>(g, ox, oy, oz, src, st, s_n);
+        // footprint velocities and accelerations f/m (zero outside the grid and on nodes without
+        // mass, which no particle of the tile touches)
+        const long long nn = g.nstride;
+        for (int s = threadIdx.x; s < G::FOOT; s += NT) {
+            const long long n = foot_node<DIM>(g, s, ox, oy, oz);
+            const double m = n >= 0 ? Nf.mass[n] : 0.;
+            const double r = m > 0. ? 1. / m : 0.;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                s_n[c][s] = n >= 0 ? Nf.vk[c * nn + n] : 0.;
+                s_n[3 + c][s] = n >= 0 ? Nf.ftot[c * nn + n] * r : 0.;
+            }
+        }
     }
     zero_acc<DIM, NQ_MOM>(s_acc);
     __syncthreads();
@@ -1083,41 +1157,37 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
                 cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
                 cpdi_for_nodes<DIM>(cc, [&](int nx, int ny, int nz, double fi, const double *) {
                     int s = foot_slot<DIM>(lx + nx, ly + ny, lz + nz);
-                    Svk[0] += s_n[0][s] * fi;
-                    Svk[1] += s_n[1][s] * fi;
-                    Svk[2] += s_n[2][s] * fi;
-                    double mnode = fi / s_n[6][s];
-                    Sacc[0] += s_n[3][s] * mnode;
-                    Sacc[1] += s_n[4][s] * mnode;
-                    Sacc[2] += s_n[5][s] * mnode;
+                    Svk[0] = fma(s_n[0][s], fi, Svk[0]);
+                    Svk[1] = fma(s_n[1][s], fi, Svk[1]);
+                    Svk[2] = fma(s_n[2][s], fi, Svk[2]);
+                    Sacc[0] = fma(s_n[3][s], fi, Sacc[0]);
+                    Sacc[1] = fma(s_n[4][s], fi, Sacc[1]);
+                    Sacc[2] = fma(s_n[5][s], fi, Sacc[2]);
                 });
             } else {
             double xi[3], invd[3], dS[3][4];
             particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
             particle_factors<DIM>(g, P, p, xi, S, dS);
-            constexpr int ncand = DIM == 3 ? 64 : 16;
+            int w[3];
+            double Sw[3][3], Dw[3][3];
+            support_window<DIM>(xi, S, dS, invd, w, Sw, Dw);
+            constexpr int NZ = DIM == 3 ? 3 : 1;
 #pragma unroll
-            for (int id = 0; id < ncand; id++) {
-                int ax, ay, az;
-                if (DIM == 3) { ax = c_off3x[id] + 1; ay = c_off3y[id] + 1; az = c_off3z[id] + 1; }
-                else { ax = c_off2x[id] + 1; ay = c_off2y[id] + 1; az = 1; }
-                double Sx = S[0][ax], Sy = S[1][ay];
-                if (Sx < 0. || Sy < 0.) continue;
-                double fi;
-                if (DIM == 3) {
-                    double Sz = S[2][az];
-                    if (Sz < 0.) continue;
-                    fi = Sx * Sy * Sz;
-                } else fi = Sx * Sy;
-                int s = foot_slot<DIM>(lx + ax - 1, ly + ay - 1, lz + az - 1);
-                Svk[0] += s_n[0][s] * fi;
-                Svk[1] += s_n[1][s] * fi;
-                Svk[2] += s_n[2][s] * fi;
-                double mnode = fi / s_n[6][s];
-                Sacc[0] += s_n[3][s] * mnode;
-                Sacc[1] += s_n[4][s] * mnode;
-                Sacc[2] += s_n[5][s] * mnode;
-            }
+            for (int jz = 0; jz < NZ; jz++)
+#pragma unroll
+                for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+                    for (int jx = 0; jx < 3; jx++) {
+                        const double fi = DIM == 3 ? Sw[0][jx] * (Sw[1][jy] * Sw[2][jz]) : Sw[0][jx] * Sw[1][jy];
+                        const int s = foot_slot<DIM>(lx + w[0] + jx - 1, ly + w[1] + jy - 1,
+                                                     DIM == 3 ? lz + w[2] + jz - 1 : 0);
+                        Svk[0] = fma(s_n[0][s], fi, Svk[0]);
+                        Svk[1] = fma(s_n[1][s], fi, Svk[1]);
+                        Svk[2] = fma(s_n[2][s], fi, Svk[2]);
+                        Sacc[0] = fma(s_n[3][s], fi, Sacc[0]);
+                        Sacc[1] = fma(s_n[4][s], fi, Sacc[1]);
+                        Sacc[2] = fma(s_n[5][s], fi, Sacc[2]);
+                    }
             }
             double vnew[3] = {0., 0., 0.};
 #pragma unroll
@@ -1143,10 +1213,14 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
 #pragma unroll
                         for (int a = 0; a < 3; a++) { s_cxi[i][c][a] = cc.xi[c][a]; s_coff[i][c][a] = cc.off[c][a]; }
                 } else {
+                    // factors with zeros outside the support; the x factor carries mp
 #pragma unroll
-                for (int a = 0; a < DIM; a++)
+                    for (int a = 0; a < DIM; a++)
 #pragma unroll
-                    for (int o = 0; o < 4; o++) s_S[i][a][o] = S[a][o];
+                        for (int o = 0; o < 4; o++) {
+                            const double v = S[a][o] >= 0. ? S[a][o] : 0.;
+                            s_S[i][a][o] = a == 0 ? v * q4[0] : v;
+                        }
                 }
 #pragma unroll
                 for (int q = 0; q < 4; q++) s_q[i][q] = q4[q];
@@ -1165,12 +1239,10 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
                     return r;
                 },
                 [&](const RecMass &r, double *acc, int, int, int) {
-                    bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
-                    double w = DIM == 3 ? r.wx * r.wy * r.wz : r.wx * r.wy;
-                    double fnmp = on ? w * r.mp : 0.;
-                    acc[0] += r.vx * fnmp;
-                    acc[1] += r.vy * fnmp;
-                    acc[2] += r.vz * fnmp;
+                    const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
+                    acc[0] = fma(r.vx, fnmp, acc[0]);
+                    acc[1] = fma(r.vy, fnmp, acc[1]);
+                    acc[2] = fma(r.vz, fnmp, acc[2]);
                 });
             __syncthreads();
         }
@@ -1358,7 +1430,7 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
     __shared__ double s_cwg[CP ? CH : 1][8][3];
     __shared__ int s_coff[CP ? CH : 1][8][3];
     __shared__ double s_q[CH][5];              // Vp, mp, displacement
-    __shared__ double s_acc[NW][NQ_CONTACT][G::FOOT];
+    __shared__ double s_acc[NW][NQ_CONTACT][WaveFoot<DIM>::SIZE];
     __shared__ double s_id[DIM][G::TX];
     const int tall = blockIdx.x;
     const int fld = tall / g0.tilesTotal;

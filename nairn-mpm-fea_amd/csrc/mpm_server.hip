@@ -342,6 +342,10 @@ int mpmgpu_server_submit(MpmServer *s, const MpmGridDesc *grid, const MpmMateria
         if (m < 0 || m >= nmat) return submit_fail(s, "particle with an undefined material");
         if (e < 0 || e >= nelem) return submit_fail(s, "particle element outside the grid");
         if (!(dfields[MPF_MASS * nn + i] > 0.)) return submit_fail(s, "particle with a mass that is not positive");
+        for (int a = 0; a < (grid->analysis == MPMGPU_THREED ? 3 : 2); a++) {
+            const double l = dfields[(MPF_LP_X + a) * nn + i];
+            if (!(l > 0. && l <= 1.)) return submit_fail(s, "particle half size must be in (0, 1] cell half widths");
+        }
     }
     r->matType = mats[ifields[MPI_MAT * nn]].type;
     for (size_t i = 1; i < nn && r->matType; i++)
