@@ -229,6 +229,10 @@ int mpmgpu_run_task(MpmCtx *ctx, int32_t task);
 int mpmgpu_step(MpmCtx *ctx, int32_t nsteps);
 /* Steps captured per graph launch (even, 2..64; default 2). No counterpart in the reference. */
 int mpmgpu_set_graph_steps(MpmCtx *ctx, int32_t steps);
+/* Steps between sorts of the particle rows into tile order in device memory (default 50; 0 =
+   only at upload). A memory-layout choice with no counterpart in the reference: results are the
+   same for any interval, since particles inside a cell are summed in upload order. */
+int mpmgpu_set_sort_interval(MpmCtx *ctx, int32_t steps);
 /* Launch-shape tuner (no counterpart in the reference): times graph lengths 2..32 over
    trialSteps steps each, keeps the fastest, and restores the simulation state it started from. */
 int mpmgpu_autotune(MpmCtx *ctx, int32_t trialSteps, int32_t *graphSteps, double *msPerStep);

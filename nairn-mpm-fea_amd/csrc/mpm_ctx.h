@@ -10,6 +10,16 @@
 
 #include "mpm_kernels.h"
 
+// Particle rows: PROWS double rows (the MPF_* fields, then the second position buffer) and IROWS
+// int rows (the MPI_* fields, the cell key, the sorted slot -> particle order, the index at
+// upload and the sorted slot -> upload index). Rows are n entries apart; the memory order of
+// particles is tile-major after a sort.
+constexpr int PROWS = MPF_NUM_FIELDS + 3;
+constexpr int IROWS = MPI_NUM_FIELDS + 4;
+constexpr int IROW_KEY = MPI_NUM_FIELDS, IROW_ORDER = MPI_NUM_FIELDS + 1, IROW_HOME = MPI_NUM_FIELDS + 2,
+              IROW_SHOME = MPI_NUM_FIELDS + 3;
+constexpr int64_t PARTICLE_BYTES = PROWS * sizeof(double) + IROWS * sizeof(int) + sizeof(double);
+
 struct MpmCtx {
     int dim = 3;
     MpmGridDesc gd{};
@@ -24,6 +34,9 @@ struct MpmCtx {
     double *pbuf = nullptr;           // particle double fields + second position buffer
     int *ibuf = nullptr;              // particle int fields
     double *cbuf = nullptr;           // lCPDI semi-side vectors of the step
+    double *scr = nullptr;            // one particle row of scratch (row permutations)
+    int sortInterval = 50;           // steps between sorts of the rows into tile order (0: never)
+    int sinceSort = 0;
     // batch of independent instances (empty for one simulation)
     std::vector<KGrid> inst;          // host copies of the instance grids
     std::vector<int> matInstH;        // material -> instance
@@ -80,8 +93,11 @@ namespace mpmi {
 int fail_global(const std::string &m);
 // particle buffers for n particles (kept when big enough) and the field pointers into them
 int place_particles(MpmCtx *c, int64_t n);
-// after the particle rows are filled: second position buffer, counters, bins; waits for the stream
+// after the particle rows are filled (the upload-index row included): second position buffer,
+// counters, bins, rows sorted into tile order; waits for the stream
 int prepare_particles(MpmCtx *c);
+// rows into the order of the current bins (tile-major), enqueued on the context's stream
+int sort_rows(MpmCtx *c);
 // current and next position rows by parity
 void set_positions(MpmCtx *c);
 }  // namespace mpmi

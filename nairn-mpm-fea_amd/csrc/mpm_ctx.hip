@@ -411,7 +411,7 @@ int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64
     int64_t bytes = nodes * (10 * sizeof(double) + sizeof(int));   // mass, pk, vk, ftot, cnt
     bytes += (int64_t)g.ntiles * NQMAX * foot * sizeof(double);    // tile partials
     bytes += (3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1) * sizeof(int);   // bins
-    bytes += n * (int64_t)((MPF_NUM_FIELDS + 3) * sizeof(double) + (MPI_NUM_FIELDS + 2) * sizeof(int));
+    bytes += n * PARTICLE_BYTES;
     if (run->shape == MPMGPU_SHAPE_LCPDI) bytes += n * 9 * (int64_t)sizeof(double);
     return bytes;
 }
@@ -494,8 +494,10 @@ int mpmi::place_particles(MpmCtx *c, int64_t n) {
     if (n > c->ncap) {
         if (c->pbuf) { (void)hipFree(c->pbuf); c->pbuf = nullptr; }
         if (c->ibuf) { (void)hipFree(c->ibuf); c->ibuf = nullptr; }
-        HIPCHECK(c, hipMalloc(&c->pbuf, (size_t)(MPF_NUM_FIELDS + 3) * n * sizeof(double)));
-        HIPCHECK(c, hipMalloc(&c->ibuf, (size_t)(MPI_NUM_FIELDS + 2) * n * sizeof(int)));
+        if (c->scr) { (void)hipFree(c->scr); c->scr = nullptr; }
+        HIPCHECK(c, hipMalloc(&c->pbuf, (size_t)PROWS * n * sizeof(double)));
+        HIPCHECK(c, hipMalloc(&c->ibuf, (size_t)IROWS * n * sizeof(int)));
+        HIPCHECK(c, hipMalloc(&c->scr, (size_t)n * sizeof(double)));
         if (c->cbuf) { (void)hipFree(c->cbuf); c->cbuf = nullptr; }
         if (c->run.shape == MPMGPU_SHAPE_LCPDI) HIPCHECK(c, hipMalloc(&c->cbuf, (size_t)9 * n * sizeof(double)));
         c->ncap = n;
@@ -512,9 +514,36 @@ int mpmi::place_particles(MpmCtx *c, int64_t n) {
     c->kp.mat = c->ibuf + nn;
     c->kp.id = c->ibuf + 2 * nn;
     c->kp.left = c->ibuf + 3 * nn;
-    c->kb.key = c->ibuf + 4 * nn;
-    c->kb.order = c->ibuf + 5 * nn;
+    c->kb.key = c->ibuf + IROW_KEY * nn;
+    c->kb.order = c->ibuf + IROW_ORDER * nn;
+    c->kb.home = c->ibuf + IROW_HOME * nn;
+    c->kb.shome = c->ibuf + IROW_SHOME * nn;
+    c->sinceSort = 0;
     c->kp.n = n;
+    return 0;
+}
+
+// Sort the rows into the order of the current bins, so that a tile's particles sit together in
+// memory. Results do not depend on it: inside a cell the particles are summed in upload order.
+int mpmi::sort_rows(MpmCtx *c) {
+    const long long n = c->kp.n;
+    c->sinceSort = 0;
+    if (n == 0) return 0;
+    hipStream_t s = c->stream;
+    for (int r = 0; r < PROWS; r++) {
+        double *row = c->pbuf + (size_t)r * n;
+        launch_gather_row(row, c->scr, c->kb.order, n, s);
+        HIPCHECK(c, hipMemcpyAsync(row, c->scr, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    int *iscr = reinterpret_cast<int *>(c->scr);
+    for (int r = 0; r < IROWS; r++) {
+        if (r == IROW_ORDER || r == IROW_SHOME) continue;      // per slot, not per particle
+        int *row = c->ibuf + (size_t)r * n;
+        launch_gather_irow(row, iscr, c->kb.order, n, s);
+        HIPCHECK(c, hipMemcpyAsync(row, iscr, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
+    launch_iota(c->kb.order, n, 0, s);
+    HIPCHECK(c, hipGetLastError());
     return 0;
 }
 
@@ -527,6 +556,7 @@ int mpmi::prepare_particles(MpmCtx *c) {
     set_positions(c);
     launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
+    if (mpmi::sort_rows(c)) return 1;
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->nextTask = 0;
@@ -562,6 +592,10 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
             const double l = dfields[(MPF_LP_X + a) * nn + i];
             if (!(l > 0. && l <= 1.)) return c->fail("particle half size must be in (0, 1] cell half widths");
         }
+        if (t == MPMGPU_MAT_ISOPLASTICITY && dfields[MPF_HIST0 * nn + i] == 0.)
+            for (int f = MPF_ALT_XX; f <= MPF_ALT_XY; f++)
+                if (dfields[f * nn + i] != 0.)
+                    return c->fail("IsoPlasticity particle with plastic strain but zero cumulative plastic strain");
     }
     // one material type for all particles lets the strain kernels compile a single law
     c->kg.matType = 0;
@@ -601,6 +635,7 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
     if (mpmi::place_particles(c, n)) return 1;
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    launch_iota(c->ibuf + IROW_HOME * nn, n, 0, c->stream);
     return mpmi::prepare_particles(c);
 }
 
@@ -670,13 +705,27 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
             if (e != hipSuccess) return c->fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
             c->parity = par;      // capture does not run the work
         }
-        for (; done + G <= nsteps; done += G) HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
+        for (; done + G <= nsteps; done += G) {
+            HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
+            c->sinceSort += G;
+            if (c->sortInterval > 0 && c->sinceSort >= c->sortInterval && mpmi::sort_rows(c)) return 1;
+        }
     }
-    for (; done < nsteps; done++) enqueue_step(c);
+    for (; done < nsteps; done++) {
+        enqueue_step(c);
+        c->sinceSort++;
+        if (c->sortInterval > 0 && c->sinceSort >= c->sortInterval && mpmi::sort_rows(c)) return 1;
+    }
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
     c->steps += nsteps;
     c->mtime += nsteps * c->run.timestep;
+    return 0;
+}
+
+int mpmgpu_set_sort_interval(MpmCtx *c, int32_t steps) {
+    if (steps < 0) return c->fail("sort interval must be >= 0");
+    c->sortInterval = steps;
     return 0;
 }
 
@@ -694,7 +743,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
     if (c->kp.n == 0 || c->profiling) return c->fail("nothing to tune");
     size_t n = (size_t)c->ncap;
-    size_t pb = (MPF_NUM_FIELDS + 3) * n * sizeof(double), ib = (MPI_NUM_FIELDS + 2) * n * sizeof(int);
+    size_t pb = PROWS * n * sizeof(double), ib = IROWS * n * sizeof(int);
     long long nk = c->kb.nkeys;
     size_t bb = (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1) * sizeof(int);
     char *save = nullptr;
@@ -704,7 +753,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     HIPCHECK(c, hipMemcpyAsync(save + pb + ib, c->bbuf, bb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                                c->stream));
-    const int parity = c->parity, oldG = c->graphSteps;
+    const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort;
     const int64_t steps = c->steps;
     const double mtime = c->mtime;
     int best = oldG;
@@ -731,6 +780,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(save);
     c->parity = parity;
+    c->sinceSort = sinceSort;
     c->steps = steps;
     c->mtime = mtime;
     mpmgpu_set_graph_steps(c, best);
@@ -764,9 +814,21 @@ int mpmgpu_download_particles(MpmCtx *c, double *dfields, int32_t *ifields) {
     size_t nn = (size_t)c->kp.n;
     if (nn == 0) return 0;
     set_positions(c);
-    for (int f = 0; f < MPF_NUM_FIELDS; f++)
-        HIPCHECK(c, hipMemcpy(dfields + f * nn, c->kp.f[f], nn * sizeof(double), hipMemcpyDeviceToHost));
-    if (ifields) HIPCHECK(c, hipMemcpy(ifields, c->ibuf, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyDeviceToHost));
+    // rows back to upload order through the scratch row
+    hipStream_t s = c->stream;
+    for (int f = 0; f < MPF_NUM_FIELDS; f++) {
+        launch_scatter_row(c->kp.f[f], c->scr, c->kb.home, (long long)nn, s);
+        HIPCHECK(c, hipMemcpyAsync(dfields + f * nn, c->scr, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (ifields) {
+        int *iscr = reinterpret_cast<int *>(c->scr);
+        for (int q = 0; q < MPI_NUM_FIELDS; q++) {
+            launch_scatter_irow(c->ibuf + q * nn, iscr, c->kb.home, (long long)nn, s);
+            HIPCHECK(c, hipMemcpyAsync(ifields + q * nn, iscr, nn * sizeof(int), hipMemcpyDeviceToHost, s));
+        }
+    }
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(s));
     return 0;
 }
 
@@ -835,6 +897,7 @@ void mpmgpu_destroy(MpmCtx *c) {
     destroy_graphs(c);
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);
+    if (c->scr) (void)hipFree(c->scr);
     if (c->cbuf) (void)hipFree(c->cbuf);
     if (c->dinst) (void)hipFree(c->dinst);
     if (c->dtables) (void)hipFree(c->dtables);

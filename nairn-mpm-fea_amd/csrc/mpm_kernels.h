@@ -70,6 +70,8 @@ struct KBins {
     int *count;                // nkeys histogram
     int *cursor;               // nkeys scatter cursor
     int *blockSums;            // scan scratch
+    const int *home;           // particle -> its index at upload: the order inside a cell
+    int *shome;                // sorted slot -> upload index of its particle
     int nkeys;
 };
 
@@ -135,5 +137,14 @@ struct KMove {
     long long dn, doff;
     long long cnt;
     int matShift;
+    int homeShift;             // added to the upload-index row (int row MPI_NUM_FIELDS + 2)
 };
 void launch_move(const KMove &m, hipStream_t s);
+
+// Particle rows in memory order: dst[k] = src[order[k]] (gather into sorted order) and
+// dst[home[k]] = src[k] (back to upload order); n entries of 8 (double) or 4 (int) bytes.
+void launch_gather_row(const double *src, double *dst, const int *order, long long n, hipStream_t s);
+void launch_gather_irow(const int *src, int *dst, const int *order, long long n, hipStream_t s);
+void launch_scatter_row(const double *src, double *dst, const int *home, long long n, hipStream_t s);
+void launch_scatter_irow(const int *src, int *dst, const int *home, long long n, hipStream_t s);
+void launch_iota(int *a, long long n, int base, hipStream_t s);

@@ -454,20 +454,26 @@ __global__ void k_bin_scatter(KPart P, KBins B) {
     int key = B.key[p];
     int slot = B.cellStart[key] + atomicAdd(&B.cursor[key], 1);
     B.order[slot] = (int)p;
+    B.shome[slot] = B.home[p];
 }
 
+// the particles of a cell in upload order (their index at upload, which the memory order of the
+// rows does not change), so every sum over a cell's particles has the same order in any layout
 __global__ void k_bin_sort_cells(KBins B) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= B.nkeys) return;
     int b = B.cellStart[c], e = B.cellStart[c + 1];
     for (int i = b + 1; i < e; i++) {
-        int v = B.order[i];
+        const int v = B.order[i];
+        const int hv = B.shome[i];
         int j = i - 1;
-        while (j >= b && B.order[j] > v) {
+        while (j >= b && B.shome[j] > hv) {
             B.order[j + 1] = B.order[j];
+            B.shome[j + 1] = B.shome[j];
             j--;
         }
         B.order[j + 1] = v;
+        B.shome[j + 1] = hv;
     }
     B.cursor[c] = 0;
     B.count[c] = 0;
@@ -829,6 +835,14 @@ __device__ __forceinline__ void particle_gradient(const KGrid &g, const KPart &P
     }
 }
 
+// IsoPlasticity: the plastic strain (alt rows) and the cumulative plastic strain alpha (hist0) grow
+// together, and rotating a zero plastic strain keeps it zero, so a particle with alpha = 0 has
+// zero plastic strain: its alt rows (and alpha) are neither read nor written until it yields.
+// Upload checks that the state keeps to this.
+__device__ __forceinline__ bool alt_rows(int type, double h0) {
+    return type != MPMGPU_MAT_ISOPLASTICITY || h0 != 0.;
+}
+
 template <int DIM>
 __device__ __forceinline__ void load_state(const KPart &P, int p, int type, PState &s) {
     s.ep[0] = P.f[MPF_EP_XX][p]; s.ep[1] = P.f[MPF_EP_YY][p]; s.ep[2] = P.f[MPF_EP_ZZ][p];
@@ -847,10 +861,12 @@ __device__ __forceinline__ void load_state(const KPart &P, int p, int type, PSta
     for (int c = 0; c < 6; c++) s.alt[c] = 0.;
     if (type != MPMGPU_MAT_ISOTROPIC) {
         s.P = P.f[MPF_PRESSURE][p];
-        s.alt[0] = P.f[MPF_ALT_XX][p]; s.alt[1] = P.f[MPF_ALT_YY][p]; s.alt[2] = P.f[MPF_ALT_ZZ][p];
-        s.alt[5] = P.f[MPF_ALT_XY][p];
-        if (DIM == 3) { s.alt[3] = P.f[MPF_ALT_YZ][p]; s.alt[4] = P.f[MPF_ALT_XZ][p]; }
         s.h0 = P.f[MPF_HIST0][p];
+        if (alt_rows(type, s.h0)) {
+            s.alt[0] = P.f[MPF_ALT_XX][p]; s.alt[1] = P.f[MPF_ALT_YY][p]; s.alt[2] = P.f[MPF_ALT_ZZ][p];
+            s.alt[5] = P.f[MPF_ALT_XY][p];
+            if (DIM == 3) { s.alt[3] = P.f[MPF_ALT_YZ][p]; s.alt[4] = P.f[MPF_ALT_XZ][p]; }
+        }
         if (type == MPMGPU_MAT_NEOHOOKEAN || type == MPMGPU_MAT_MOONEY) s.h1 = P.f[MPF_HIST1][p];
     }
 }
@@ -868,10 +884,12 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
     }
     if (type != MPMGPU_MAT_ISOTROPIC) {
         P.f[MPF_PRESSURE][p] = s.P;
-        P.f[MPF_ALT_XX][p] = s.alt[0]; P.f[MPF_ALT_YY][p] = s.alt[1]; P.f[MPF_ALT_ZZ][p] = s.alt[2];
-        P.f[MPF_ALT_XY][p] = s.alt[5];
-        if (DIM == 3) { P.f[MPF_ALT_YZ][p] = s.alt[3]; P.f[MPF_ALT_XZ][p] = s.alt[4]; }
-        P.f[MPF_HIST0][p] = s.h0;
+        if (alt_rows(type, s.h0)) {
+            P.f[MPF_ALT_XX][p] = s.alt[0]; P.f[MPF_ALT_YY][p] = s.alt[1]; P.f[MPF_ALT_ZZ][p] = s.alt[2];
+            P.f[MPF_ALT_XY][p] = s.alt[5];
+            if (DIM == 3) { P.f[MPF_ALT_YZ][p] = s.alt[3]; P.f[MPF_ALT_XZ][p] = s.alt[4]; }
+            P.f[MPF_HIST0][p] = s.h0;
+        }
         if (type == MPMGPU_MAT_NEOHOOKEAN || type == MPMGPU_MAT_MOONEY) P.f[MPF_HIST1][p] = s.h1;
     }
 }
@@ -1912,14 +1930,49 @@ __global__ void k_move(KMove m) {
         const double *src = (r >= MPF_POS_X && r <= MPF_POS_Z) ? m.spos[r - MPF_POS_X] : m.sd + r * m.sn;
         m.dd[r * m.dn + m.doff + p] = src[m.soff + p];
     } else {
-        const int q = r - MPF_NUM_FIELDS;
+        // int rows MPI_*, then the upload-index row (MPI_NUM_FIELDS + 2)
+        const int q = r - MPF_NUM_FIELDS < MPI_NUM_FIELDS ? r - MPF_NUM_FIELDS : MPI_NUM_FIELDS + 2;
         int v = m.si[q * m.sn + m.soff + p];
         if (q == MPI_MAT) v += m.matShift;
+        if (q == MPI_NUM_FIELDS + 2) v += m.homeShift;
         m.di[q * m.dn + m.doff + p] = v;
     }
 }
 
 void launch_move(const KMove &m, hipStream_t s) {
     if (m.cnt <= 0) return;
-    hipLaunchKernelGGL(k_move, dim3(blocks_for(m.cnt, NT), MPF_NUM_FIELDS + MPI_NUM_FIELDS), dim3(NT), 0, s, m);
+    hipLaunchKernelGGL(k_move, dim3(blocks_for(m.cnt, NT), MPF_NUM_FIELDS + MPI_NUM_FIELDS + 1), dim3(NT), 0, s, m);
+}
+
+template <typename T>
+__global__ void k_gather_row(const T *src, T *dst, const int *order, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[order[k]];
+}
+
+template <typename T>
+__global__ void k_scatter_row(const T *src, T *dst, const int *home, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[home[k]] = src[k];
+}
+
+__global__ void k_iota(int *a, long long n, int base) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = base + (int)k;
+}
+
+void launch_gather_row(const double *src, double *dst, const int *order, long long n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_row<double>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, src, dst, order, n);
+}
+void launch_gather_irow(const int *src, int *dst, const int *order, long long n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_row<int>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, src, dst, order, n);
+}
+void launch_scatter_row(const double *src, double *dst, const int *home, long long n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_scatter_row<double>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, src, dst, home, n);
+}
+void launch_scatter_irow(const int *src, int *dst, const int *home, long long n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_scatter_row<int>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, src, dst, home, n);
+}
+void launch_iota(int *a, long long n, int base, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, NT)), dim3(NT), 0, s, a, n, base);
 }

@@ -160,6 +160,7 @@ int relayout(MpmServer *s, const std::vector<Request *> &members) {
             mv.doff = offs[k];
             mv.cnt = r->n;
             mv.matShift = firsts[k] - r->firstMat;
+            mv.homeShift = (int)(offs[k] - r->off);
             launch_move(mv, nc->stream);
         } else {
             const size_t n = (size_t)r->n;
@@ -174,6 +175,7 @@ int relayout(MpmServer *s, const std::vector<Request *> &members) {
                 if (hipMemcpyAsync(nc->ibuf + q * N + offs[k], iv.data() + q * n, n * sizeof(int32_t),
                                    hipMemcpyHostToDevice, nc->stream) != hipSuccess)
                     return bail("particle upload");
+            launch_iota(nc->ibuf + IROW_HOME * N + offs[k], (long long)n, (int)offs[k], nc->stream);
         }
     }
     if (hipGetLastError() != hipSuccess) return bail("particle move");
@@ -196,21 +198,32 @@ int relayout(MpmServer *s, const std::vector<Request *> &members) {
     return 0;
 }
 
-// final particle rows of a resident solve, in its own material numbering
+// final particle rows of a resident solve, in its own order (its upload-index row tells where each
+// row entry belongs) and material numbering
 int fetch(MpmServer *s, Request *r) {
     MpmCtx *c = s->ctx;
     mpmi::set_positions(c);
     const size_t n = (size_t)r->n, N = (size_t)c->kp.n;
-    r->d.assign((size_t)MPF_NUM_FIELDS * n, 0.);
-    r->iv.assign((size_t)MPI_NUM_FIELDS * n, 0);
+    std::vector<double> d((size_t)MPF_NUM_FIELDS * n);
+    std::vector<int32_t> iv((size_t)MPI_NUM_FIELDS * n), home(n);
     for (int f = 0; f < MPF_NUM_FIELDS; f++)
-        if (hipMemcpy(r->d.data() + f * n, c->kp.f[f] + r->off, n * sizeof(double), hipMemcpyDeviceToHost) !=
-            hipSuccess)
+        if (hipMemcpy(d.data() + f * n, c->kp.f[f] + r->off, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
             return srv_fail(s, "result download failed");
     for (int q = 0; q < MPI_NUM_FIELDS; q++)
-        if (hipMemcpy(r->iv.data() + q * n, c->ibuf + q * N + r->off, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
+        if (hipMemcpy(iv.data() + q * n, c->ibuf + q * N + r->off, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
             hipSuccess)
             return srv_fail(s, "result download failed");
+    if (hipMemcpy(home.data(), c->ibuf + IROW_HOME * N + r->off, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return srv_fail(s, "result download failed");
+    r->d.assign((size_t)MPF_NUM_FIELDS * n, 0.);
+    r->iv.assign((size_t)MPI_NUM_FIELDS * n, 0);
+    for (size_t i = 0; i < n; i++) {
+        const long long h = (long long)home[i] - r->off;
+        if (h < 0 || h >= (long long)n) return srv_fail(s, "particle rows of a solve left its range");
+        for (int f = 0; f < MPF_NUM_FIELDS; f++) r->d[f * n + h] = d[f * n + i];
+        for (int q = 0; q < MPI_NUM_FIELDS; q++) r->iv[q * n + h] = iv[q * n + i];
+    }
     for (size_t i = 0; i < n; i++) r->iv[MPI_MAT * n + i] -= r->firstMat;
     return 0;
 }
@@ -346,6 +359,10 @@ int mpmgpu_server_submit(MpmServer *s, const MpmGridDesc *grid, const MpmMateria
             const double l = dfields[(MPF_LP_X + a) * nn + i];
             if (!(l > 0. && l <= 1.)) return submit_fail(s, "particle half size must be in (0, 1] cell half widths");
         }
+        if (mats[m].type == MPMGPU_MAT_ISOPLASTICITY && dfields[MPF_HIST0 * nn + i] == 0.)
+            for (int f = MPF_ALT_XX; f <= MPF_ALT_XY; f++)
+                if (dfields[f * nn + i] != 0.)
+                    return submit_fail(s, "IsoPlasticity particle with plastic strain but zero cumulative plastic strain");
     }
     r->matType = mats[ifields[MPI_MAT * nn]].type;
     for (size_t i = 1; i < nn && r->matType; i++)
@@ -362,7 +379,7 @@ int mpmgpu_server_submit(MpmServer *s, const MpmGridDesc *grid, const MpmMateria
     r->iv.assign(ifields, ifields + (size_t)MPI_NUM_FIELDS * nn);
     r->steps = steps;
     r->footprint = fp;
-    r->particleBytes = n * (int64_t)((MPF_NUM_FIELDS + 3) * sizeof(double) + (MPI_NUM_FIELDS + 2) * sizeof(int));
+    r->particleBytes = n * PARTICLE_BYTES;
     std::lock_guard<std::mutex> lk(s->m);
     if (s->failed) return 1;
     if (fp > s->p.memoryBudget) {

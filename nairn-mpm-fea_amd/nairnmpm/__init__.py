@@ -128,6 +128,7 @@ _SIGNATURES = {
     "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_step": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_set_graph_steps": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_set_sort_interval": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_autotune": ([_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
     "mpmgpu_sync": ([_P], ctypes.c_int),
@@ -296,6 +297,10 @@ class Simulation:
 
     def set_graph_steps(self, steps: int):
         _check(library().mpmgpu_set_graph_steps(self._h, int(steps)), self._h)
+
+    def set_sort_interval(self, steps: int):
+        """steps between sorts of the particle rows into tile order (0: only at upload)"""
+        _check(library().mpmgpu_set_sort_interval(self._h, int(steps)), self._h)
 
     def autotune(self, trial_steps: int = 32):
         """pick the graph length with the fastest steps; the state is left unchanged"""

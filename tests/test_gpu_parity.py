@@ -136,6 +136,37 @@ def test_graph_length_matches_eager(graph_steps):
     assert a.stats().steps == b.stats().steps == 37
 
 
+@pytest.mark.parametrize("xml", [
+    configs.j2_block_3d(cells=12, lo=3, hi=9),
+    configs.contact_blocks_2d(cells=24, w=6, h=6, gap=1., vy=2000.),
+    configs.elastic_block_2d(cells=24, block=12, shape="lCPDI"),
+], ids=["j2_3d", "contact_2d", "lcpdi_2d"])
+def test_row_sorting_does_not_change_results(xml):
+    """Particles inside a cell are summed in upload order, so sorting the rows into tile order
+    after every step, every few steps or never gives the same bits."""
+    prob = nm.Problem.from_xml(xml)
+    runs = []
+    for interval in (1, 3, 0):
+        sim = nm.Simulation.from_problem(prob)
+        sim.set_sort_interval(interval)
+        sim.step(25)
+        runs.append(sim.particles())
+    for d, i in runs[1:]:
+        assert np.array_equal(d, runs[0][0]) and np.array_equal(i, runs[0][1])
+
+
+def test_upload_rejects_plastic_strain_without_alpha():
+    prob = nm.Problem.from_xml(configs.j2_block_2d(cells=16, block=6))
+    g, mats, run, _, _ = prob.describe()
+    d, i = prob.particles()
+    d[nm.F["alt_xx"], 3] = 1e-3                     # plastic strain, alpha still zero
+    sim = nm.Simulation(g, mats, run)
+    with pytest.raises(nm.MPMError, match="plastic strain"):
+        sim.upload(d, i)
+    d[nm.F["hist0"], 3] = 1e-3
+    sim.upload(d, i)                                 # consistent state is accepted
+
+
 def test_autotune_leaves_state_unchanged():
     prob = nm.Problem.from_xml(configs.j2_block_2d(cells=24, block=12))
     a = nm.Simulation.from_problem(prob)
@@ -372,6 +403,22 @@ def test_batch_sweep_members_equal_alone_and_grid():
         bm, bp, bf = batch.instance_grid(k)
         sm, sp, sf = sim.grid_values()
         assert np.array_equal(bm, sm) and np.array_equal(bp[:2], sp[:2]) and np.array_equal(bf[:2], sf[:2])
+
+
+def test_batch_sorted_rows_equal_alone():
+    """Sorting a batch's rows every step keeps each instance bitwise equal to its solo run."""
+    probs = [nm.Problem.from_xml(configs.j2_block_2d(cells=20, block=8, seed=6, yield_=30.)),
+             nm.Problem.from_xml(configs.elastic_block_2d(cells=16, block=6, seed=3, E=800.))]
+    batch = nm.Batch(probs)
+    batch.set_sort_interval(1)
+    batch.step(30)
+    for k, prob in enumerate(probs):
+        d, i = batch.instance_particles(k)
+        sim = nm.Simulation.from_problem(prob)
+        sim.set_sort_interval(0)
+        sim.step(30)
+        da, ia = sim.particles()
+        assert np.array_equal(d, da) and np.array_equal(i, ia), f"instance {k}"
 
 
 def test_batch_more_materials_than_one_context_holds():
