@@ -375,6 +375,19 @@ __device__ __forceinline__ long long foot_node(const KGrid &g, int s, int ox, in
 // ---------------------------------------------------------------------------------------
 // binning: key + histogram, exclusive scan, scatter, per-cell sort (stable by particle index)
 
+// count[key] += 1 for every active lane, with one atomic per distinct key of the wave
+__device__ __forceinline__ void wave_count(int *count, int key) {
+    const int lane = (int)__lane_id();
+    unsigned long long active = __ballot(1);
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long same = __ballot(key == lkey) & active;
+        if (lane == leader) atomicAdd(&count[lkey], __popcll(same));
+        active &= ~same;
+    }
+}
+
 template <int DIM>
 __global__ void k_bin_keys(KGrid g0, KPart P, KBins B) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -387,7 +400,7 @@ __global__ void k_bin_keys(KGrid g0, KPart P, KBins B) {
     const int fld = g.nfields > 1 ? g.fieldOf[P.mat[p]] : 0;
     int key = cell_key<DIM>(g, fld, i, j, DIM == 3 ? k : 0);
     B.key[p] = key;
-    atomicAdd(&B.count[key], 1);
+    wave_count(B.count, key);
 }
 
 #define SCAN_PER_THREAD 4
@@ -448,13 +461,28 @@ __global__ void k_scan_add(int *out, const int *blockSums, int n) {
     if (i < n) out[i] += blockSums[i / SCAN_BLOCK];
 }
 
+// Slots by key: lanes of a wave with the same key (neighbours, since the rows are kept in tile
+// order) take consecutive slots with one atomic per key instead of one per particle.
 __global__ void k_bin_scatter(KPart P, KBins B) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.n) return;
-    int key = B.key[p];
-    int slot = B.cellStart[key] + atomicAdd(&B.cursor[key], 1);
-    B.order[slot] = (int)p;
-    B.shome[slot] = B.home[p];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < P.n;
+    const int key = valid ? B.key[p] : -1;
+    const int lane = (int)__lane_id();
+    unsigned long long active = __ballot(valid);
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int lkey = __shfl(key, leader);
+        const unsigned long long same = __ballot(key == lkey) & active;
+        int base = 0;
+        if (lane == leader) base = B.cellStart[lkey] + atomicAdd(&B.cursor[lkey], __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) {
+            const int slot = base + __popcll(same & ((1ull << lane) - 1ull));
+            B.order[slot] = (int)p;
+            B.shome[slot] = B.home[p];
+        }
+        active &= ~same;
+    }
 }
 
 // the particles of a cell in upload order (their index at upload, which the memory order of the
@@ -600,49 +628,8 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
     const int tb = s_cs[0], te = s_cs[G::NC];
     if (tb == te) return;
     zero_acc<DIM, NQ_MASS>(s_acc);
-    for (int c0 = tb; c0 < te; c0 += CH) {
-        int k = c0 + threadIdx.x;
-        if (threadIdx.x < CH && k < te) {
-            int p = B.order[k];
-            int ei, ej, ek;
-            elem_ijk(g, P.elem[p], ei, ej, ek);
-            double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-            if (CP) {
-                // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step
-                double r[9];
-                cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);
-#pragma unroll
-                for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
-                CpdiCorners cc;
-                if (!cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc)) atomicAdd(st.cpdiErr, 1ull);
-#pragma unroll
-                for (int c = 0; c < 8; c++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
-                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
-                    }
-            } else {
-                // factors with zeros outside the support; the x factor carries mp
-                double xi[3], invd[3], S[3][4], dS[3][4];
-                particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
-                particle_factors<DIM>(g, P, p, xi, S, dS);
-                const double mp = P.f[MPF_MASS][p];
-#pragma unroll
-                for (int a = 0; a < DIM; a++)
-#pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        const double v = S[a][o] >= 0. ? S[a][o] : 0.;
-                        s_S[threadIdx.x][a][o] = a == 0 ? v * mp : v;
-                    }
-            }
-            s_q[threadIdx.x][0] = P.f[MPF_MASS][p];
-            s_q[threadIdx.x][1] = P.f[MPF_VEL_X][p];
-            s_q[threadIdx.x][2] = P.f[MPF_VEL_Y][p];
-            s_q[threadIdx.x][3] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
-        }
-        __syncthreads();
-        wave_cells<DIM, NQ_MASS>(s_cs, c0, min(c0 + CH, te), s_acc,
+    auto scatter = [&](int c0, int c1) {
+        wave_cells<DIM, NQ_MASS>(s_cs, c0, c1, s_acc,
             [&](int lq, int ix, int iy, int iz) {
                 RecMass r;
                 if (CP) {
@@ -663,7 +650,90 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
                 acc[2] = fma(fnmp, r.vy, acc[2]);
                 acc[3] = fma(fnmp, r.vz, acc[3]);
             });
-        __syncthreads();
+    };
+    if (!CP) {
+        // Rounds of 2*NT slots: every thread first loads the particle words of its two slots of
+        // the round (tb + t and tb + t + NT), so the round's global loads overlap; then the four
+        // chunks of CH slots are staged from registers by the half of the block that holds them.
+        for (int r0 = tb; r0 < te; r0 += 2 * NT) {
+            int rp[2], re[2];
+            double rx[2][3], rm[2], rv[2][3];
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int k = r0 + threadIdx.x + j * NT;
+                rp[j] = k < te ? B.order[k] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int p = rp[j];
+                if (p < 0) continue;
+                re[j] = P.elem[p];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    rx[j][a] = a < DIM ? P.f[MPF_POS_X + a][p] : 0.;
+                    rv[j][a] = a < DIM ? P.f[MPF_VEL_X + a][p] : 0.;
+                }
+                rm[j] = P.f[MPF_MASS][p];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int c0 = r0 + c * CH;
+                if (c0 >= te) break;
+                const int j = c >> 1;
+                if ((int)(threadIdx.x / CH) == (c & 1) && rp[j] >= 0) {
+                    const int i = threadIdx.x - (c & 1) * CH;
+                    int ei, ej, ek;
+                    elem_ijk(g, re[j], ei, ej, ek);
+                    // factors with zeros outside the support; the x factor carries mp
+                    double xi[3], invd[3], S[3][4], dS[3][4];
+                    particle_xi<DIM>(g, ei, ej, ek, rx[j][0], rx[j][1], rx[j][2], xi, invd);
+                    particle_factors<DIM>(g, P, rp[j], xi, S, dS);
+#pragma unroll
+                    for (int a = 0; a < DIM; a++)
+#pragma unroll
+                        for (int o = 0; o < 4; o++) {
+                            const double v = S[a][o] >= 0. ? S[a][o] : 0.;
+                            s_S[i][a][o] = a == 0 ? v * rm[j] : v;
+                        }
+                    s_q[i][0] = rm[j];
+                    s_q[i][1] = rv[j][0]; s_q[i][2] = rv[j][1]; s_q[i][3] = rv[j][2];
+                }
+                __syncthreads();
+                scatter(c0, min(c0 + CH, te));
+                __syncthreads();
+            }
+        }
+    } else {
+        for (int c0 = tb; c0 < te; c0 += CH) {
+            int k = c0 + threadIdx.x;
+            if (threadIdx.x < CH && k < te) {
+                int p = B.order[k];
+                int ei, ej, ek;
+                elem_ijk(g, P.elem[p], ei, ej, ek);
+                double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+                // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step
+                double r[9];
+                cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);
+#pragma unroll
+                for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
+                CpdiCorners cc;
+                if (!cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc)) atomicAdd(st.cpdiErr, 1ull);
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
+                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
+                    }
+                s_q[threadIdx.x][0] = P.f[MPF_MASS][p];
+                s_q[threadIdx.x][1] = P.f[MPF_VEL_X][p];
+                s_q[threadIdx.x][2] = P.f[MPF_VEL_Y][p];
+                s_q[threadIdx.x][3] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+            }
+            __syncthreads();
+            scatter(c0, min(c0 + CH, te));
+            __syncthreads();
+        }
     }
     write_partials<DIM, NQ_MASS>(N, tall, s_acc);
 }
@@ -1422,7 +1492,7 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g0, KPart P, KBins B, KN
         }
         int key = cell_key<DIM>(g, fld, ei, ej, DIM == 3 ? ek : 0);
         B.key[p] = key;
-        atomicAdd(&B.count[key], 1);
+        wave_count(B.count, key);
     }
 }
 
