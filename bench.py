@@ -251,10 +251,12 @@ def bench_serve(a):
     lat = np.array([inf["finished"] - base - t for inf, t in zip(infos, arrivals)])
     span = max(inf["finished"] for inf in infos) - base
     served = total_ps / span
-    layouts = srv.counts()["layouts"]
+    cnt = srv.counts()
+    layouts = cnt["layouts"]
     served_out = [srv.collect(rid) for rid in ids]
     srv.close()
-    print(f"# served: {served:.4e} particle-steps/s over {span:.2f} s, {layouts} layouts, latency p50 "
+    print(f"# served: {served:.4e} particle-steps/s over {span:.2f} s, {layouts} layouts "
+          f"({cnt['layout_seconds']:.2f} s building them, {cnt['step_seconds']:.2f} s in batch steps), latency p50 "
           f"{np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s", file=sys.stderr, flush=True)
 
     # one at a time: a plain solve per request (create, upload, step, download), first come first served
@@ -295,6 +297,7 @@ def bench_serve(a):
         "config": {"workload": f"served 3D stream: {a.requests} J2 or neo-Hookean blocks, 50k-2M particles "
                                f"log-uniform, 200-2000 steps, uGIMP, Poisson arrivals at {a.rate:g}/s",
                    "particle_steps": int(total_ps), "span_s": span, "layouts": layouts,
+                   "layout_s": cnt["layout_seconds"], "batch_step_s": cnt["step_seconds"],
                    "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
                    "one_at_a_time_value": serial, "one_at_a_time_span_s": serial_span,
                    "one_at_a_time_busy_s": float(sum(service)),

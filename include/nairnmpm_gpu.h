@@ -275,8 +275,11 @@ void mpmgpu_destroy(MpmCtx *ctx);
 typedef struct MpmServer MpmServer;
 
 typedef struct MpmServerParams {
-    int64_t memoryBudget;      /* device bytes for resident solves, moves between layouts included
-                                  (0: 90% of the free device memory at creation) */
+    int64_t memoryBudget;      /* device bytes for resident solves, moves between layouts,
+                                  uploads staged on the device and results not yet collected
+                                  (0: 90% of the free device memory at creation); a solve is
+                                  admitted alone when nothing is resident even if staged rows of
+                                  others would exceed it */
     int32_t maxResident;       /* most solves resident at once (0: no limit) */
     int32_t graphSteps;        /* batch steps per captured graph and per admission check (even,
                                   2..64; 0: 8) */
@@ -301,7 +304,9 @@ typedef struct MpmRequestInfo {
 /* Device bytes a single-material solve holds: particle rows, grid, tile partials and bins. */
 int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64_t nparticles);
 int mpmgpu_server_create(const MpmServerParams *params, MpmServer **srv);
-/* Queue a solve (the particle rows are copied). Fails for a request that can never fit. */
+/* Queue a solve. Its rows are copied to a device staging buffer on the calling thread when the
+   budget has room (else to host memory, uploaded at admission). Fails for a request that can
+   never fit. */
 int mpmgpu_server_submit(MpmServer *srv, const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat,
                          const MpmRunParams *run, int64_t n, const double *dfields, const int32_t *ifields,
                          int32_t steps, int64_t *id);
@@ -316,10 +321,13 @@ int mpmgpu_server_stop(MpmServer *srv);
 /* Wait until request id is done (or timeoutSec passes: returns 0 with *done = 0). */
 int mpmgpu_server_wait(MpmServer *srv, int64_t id, double timeoutSec, int32_t *done);
 int mpmgpu_server_info(MpmServer *srv, int64_t id, MpmRequestInfo *info);
-/* Final particles of a done request, in its own numbering (materials from 0, mpm[] order). */
+/* Final particles of a done request, in its own numbering (materials from 0, mpm[] order); the
+   result rows wait on the device until this call. */
 int mpmgpu_server_collect(MpmServer *srv, int64_t id, double *dfields, int32_t *ifields);
+/* Queue and residency now; layouts built so far; seconds the request loop spent building layouts
+   and running batch steps. */
 int mpmgpu_server_counts(MpmServer *srv, int32_t *queued, int32_t *resident, int64_t *residentBytes,
-                         int64_t *layouts);
+                         int64_t *layouts, double *layoutSeconds, double *stepSeconds);
 double mpmgpu_server_clock(const MpmServer *srv);
 const char *mpmgpu_server_error(const MpmServer *srv);
 void mpmgpu_server_destroy(MpmServer *srv);

@@ -4,8 +4,11 @@
 // solves are resident: their particle rows live in one batch context (mpmgpu_batch_create layout)
 // and every batch step advances each of them by one of its own steps. When the membership
 // changes (a solve finishes, or queued ones fit), a new batch context is built for the new set;
-// the survivors' particle rows move into it on the device (k_move), newcomers are copied from the
-// host, and the old context is released. Between layouts the steps replay from captured graphs.
+// the survivors' particle rows move into it on the device (k_move), and the old context is
+// released. Between layouts the steps replay from captured graphs. Uploads and downloads stay off
+// the request loop where memory allows: a submitted solve's rows are copied to a device staging
+// buffer by the submitting thread, and a finished solve's rows are copied to a device result
+// buffer that mpmgpu_server_collect brings to the host.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,7 +47,23 @@ struct Request {
     bool placed = false;
     int64_t off = 0;
     int firstMat = 0;
+    // device rows: the staged upload (before admission) or the result (after it finishes); int
+    // rows laid out as a context's (MPI_* rows, then the upload-index row at IROW_HOME)
+    double *dd = nullptr;
+    int *di = nullptr;
+    int64_t devBytes = 0;
+    int64_t homeBase = 0;          // result rows: upload index of the solve's first particle
+    int resultFirstMat = 0;        // result rows: batch number of the solve's first material
 };
+
+int64_t row_bytes(int64_t n) { return n * (int64_t)(MPF_NUM_FIELDS * sizeof(double) + IROWS * sizeof(int)); }
+
+void free_rows(Request *r) {
+    if (r->dd) (void)hipFree(r->dd);
+    if (r->di) (void)hipFree(r->di);
+    r->dd = nullptr;
+    r->di = nullptr;
+}
 
 }  // namespace
 
@@ -61,6 +80,9 @@ struct MpmServer {
     int64_t nextId = 1;
     int64_t layouts = 0;
     int64_t residentBytes = 0;
+    double layoutSeconds = 0., stepSeconds = 0.;   // request-loop time in layouts and in batch steps
+    int64_t deviceRowBytes = 0;                  // staged uploads and results held on the device
+    hipStream_t upStream = nullptr;              // staging copies of submitting threads
     std::string err;
     bool failed = false;
     std::thread loop;
@@ -162,6 +184,22 @@ int relayout(MpmServer *s, const std::vector<Request *> &members) {
             mv.matShift = firsts[k] - r->firstMat;
             mv.homeShift = (int)(offs[k] - r->off);
             launch_move(mv, nc->stream);
+        } else if (r->dd) {
+            // staged on the device at submission: rows in the solve's own order and numbering
+            KMove mv{};
+            mv.sd = r->dd;
+            for (int a = 0; a < 3; a++) mv.spos[a] = r->dd + (size_t)(MPF_POS_X + a) * r->n;
+            mv.si = r->di;
+            mv.sn = r->n;
+            mv.soff = 0;
+            mv.dd = nc->pbuf;
+            mv.di = nc->ibuf;
+            mv.dn = N;
+            mv.doff = offs[k];
+            mv.cnt = r->n;
+            mv.matShift = firsts[k];
+            mv.homeShift = (int)offs[k];
+            launch_move(mv, nc->stream);
         } else {
             const size_t n = (size_t)r->n;
             for (int f = 0; f < MPF_NUM_FIELDS; f++)
@@ -183,48 +221,52 @@ int relayout(MpmServer *s, const std::vector<Request *> &members) {
     if (mpmgpu_set_graph_steps(nc, s->p.graphSteps)) return bail("graph length");
     if (old) mpmgpu_destroy(old);
     s->ctx = nc;
+    int64_t freed = 0;
     for (size_t k = 0; k < ni; k++) {
         Request *r = members[k];
         if (!r->placed) {
-            std::vector<double>().swap(r->d);       // the device holds the state now
+            std::vector<double>().swap(r->d);       // the layout holds the state now
             std::vector<int32_t>().swap(r->iv);
+            if (r->dd) {
+                free_rows(r);
+                freed += r->devBytes;
+                r->devBytes = 0;
+            }
         }
         r->placed = true;
         r->off = offs[k];
         r->firstMat = firsts[k];
     }
     std::lock_guard<std::mutex> lk(s->m);
+    s->deviceRowBytes -= freed;
     s->layouts++;
     return 0;
 }
 
-// final particle rows of a resident solve, in its own order (its upload-index row tells where each
-// row entry belongs) and material numbering
-int fetch(MpmServer *s, Request *r) {
+// Final rows of a finished solve, copied on the device to its result buffer (in layout order;
+// mpmgpu_server_collect puts them in the solve's own order on the host).
+int keep_result(MpmServer *s, Request *r) {
     MpmCtx *c = s->ctx;
     mpmi::set_positions(c);
     const size_t n = (size_t)r->n, N = (size_t)c->kp.n;
-    std::vector<double> d((size_t)MPF_NUM_FIELDS * n);
-    std::vector<int32_t> iv((size_t)MPI_NUM_FIELDS * n), home(n);
-    for (int f = 0; f < MPF_NUM_FIELDS; f++)
-        if (hipMemcpy(d.data() + f * n, c->kp.f[f] + r->off, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-            return srv_fail(s, "result download failed");
-    for (int q = 0; q < MPI_NUM_FIELDS; q++)
-        if (hipMemcpy(iv.data() + q * n, c->ibuf + q * N + r->off, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
-            hipSuccess)
-            return srv_fail(s, "result download failed");
-    if (hipMemcpy(home.data(), c->ibuf + IROW_HOME * N + r->off, n * sizeof(int32_t), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-        return srv_fail(s, "result download failed");
-    r->d.assign((size_t)MPF_NUM_FIELDS * n, 0.);
-    r->iv.assign((size_t)MPI_NUM_FIELDS * n, 0);
-    for (size_t i = 0; i < n; i++) {
-        const long long h = (long long)home[i] - r->off;
-        if (h < 0 || h >= (long long)n) return srv_fail(s, "particle rows of a solve left its range");
-        for (int f = 0; f < MPF_NUM_FIELDS; f++) r->d[f * n + h] = d[f * n + i];
-        for (int q = 0; q < MPI_NUM_FIELDS; q++) r->iv[q * n + h] = iv[q * n + i];
+    if (hipMalloc(&r->dd, (size_t)MPF_NUM_FIELDS * n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&r->di, (size_t)IROWS * n * sizeof(int)) != hipSuccess) {
+        free_rows(r);
+        return srv_fail(s, "out of device memory keeping a result");
     }
-    for (size_t i = 0; i < n; i++) r->iv[MPI_MAT * n + i] -= r->firstMat;
+    for (int f = 0; f < MPF_NUM_FIELDS; f++)
+        if (hipMemcpyAsync(r->dd + f * n, c->kp.f[f] + r->off, n * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream) != hipSuccess)
+            return srv_fail(s, "result copy failed");
+    for (int q = 0; q < IROWS; q++) {
+        if (q != IROW_HOME && q >= MPI_NUM_FIELDS) continue;
+        if (hipMemcpyAsync(r->di + q * n, c->ibuf + q * N + r->off, n * sizeof(int), hipMemcpyDeviceToDevice,
+                           c->stream) != hipSuccess)
+            return srv_fail(s, "result copy failed");
+    }
+    r->devBytes = row_bytes(r->n);
+    r->homeBase = r->off;
+    r->resultFirstMat = r->firstMat;
     return 0;
 }
 
@@ -248,7 +290,10 @@ int advance(MpmServer *s, int32_t maxSteps, int32_t *completed) {
         while (!s->queue.empty()) {
             Request *r = s->queue.front();
             if (s->p.maxResident > 0 && (int)s->resident.size() >= s->p.maxResident) break;
-            if (bytes + r->footprint + moving > s->p.memoryBudget) break;
+            // device bytes held for other solves' staged uploads and kept results count too; with
+            // nothing resident the head is admitted anyway, so staged rows never block progress
+            const int64_t held = s->deviceRowBytes - r->devBytes;
+            if (!s->resident.empty() && bytes + r->footprint + moving + held > s->p.memoryBudget) break;
             s->queue.pop_front();
             r->state = MPMGPU_REQ_RESIDENT;
             r->tAdmit = s->now();
@@ -261,20 +306,34 @@ int advance(MpmServer *s, int32_t maxSteps, int32_t *completed) {
         relay = s->dirty;
         s->dirty = false;
     }
-    if (relay && relayout(s, members)) return 1;
+    if (relay) {
+        const double t0 = s->now();
+        if (relayout(s, members)) return 1;
+        const double t1 = s->now();
+        std::lock_guard<std::mutex> lk(s->m);
+        s->layoutSeconds += t1 - t0;
+    }
     if (members.empty()) return 0;
     int k = maxSteps;
     for (const Request *r : members) k = std::min(k, r->steps - r->done);
+    const double t2 = s->now();
     if (k > 0 && (mpmgpu_step(s->ctx, k) || mpmgpu_sync(s->ctx)))
         return srv_fail(s, std::string("batch step: ") + s->ctx->err);
+    const double t3 = s->now();
     std::vector<Request *> fin;
     for (Request *r : members)
         if (r->done + k >= r->steps) fin.push_back(r);
-    for (Request *r : fin)
-        if (fetch(s, r)) return 1;
+    int64_t kept = 0;
+    for (Request *r : fin) {
+        if (keep_result(s, r)) return 1;
+        kept += r->devBytes;
+    }
+    if (!fin.empty() && mpmgpu_sync(s->ctx)) return srv_fail(s, std::string("result copy: ") + s->ctx->err);
     {
         std::lock_guard<std::mutex> lk(s->m);
         const double t = s->now();
+        s->stepSeconds += t3 - t2;
+        s->deviceRowBytes += kept;
         for (Request *r : members) r->done += k;
         for (Request *r : fin) {
             r->state = MPMGPU_REQ_DONE;
@@ -322,6 +381,10 @@ int mpmgpu_server_create(const MpmServerParams *params, MpmServer **out) {
         p.memoryBudget = (int64_t)(0.9 * (double)freeB);
     }
     MpmServer *s = new MpmServer();
+    if (hipStreamCreateWithFlags(&s->upStream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return create_fail("hipStreamCreate failed");
+    }
     s->p = p;
     s->device = dev;
     s->t0 = std::chrono::steady_clock::now();
@@ -375,17 +438,45 @@ int mpmgpu_server_submit(MpmServer *s, const MpmGridDesc *grid, const MpmMateria
     r->run = *run;
     r->mats.assign(mats, mats + nmat);
     r->n = n;
-    r->d.assign(dfields, dfields + (size_t)MPF_NUM_FIELDS * nn);
-    r->iv.assign(ifields, ifields + (size_t)MPI_NUM_FIELDS * nn);
     r->steps = steps;
     r->footprint = fp;
     r->particleBytes = n * PARTICLE_BYTES;
+    if (fp > s->p.memoryBudget) return submit_fail(s, "request footprint exceeds the server's memory budget");
+    // stage the rows on the device now if the budget has room beside what is resident
+    const int64_t sb = row_bytes(n);
+    bool stage = false;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        if (s->failed) return 1;
+        if (s->residentBytes + s->deviceRowBytes + sb + fp <= s->p.memoryBudget) {
+            s->deviceRowBytes += sb;
+            stage = true;
+        }
+    }
+    if (stage) {
+        bool ok = hipMalloc(&r->dd, (size_t)MPF_NUM_FIELDS * nn * sizeof(double)) == hipSuccess &&
+                  hipMalloc(&r->di, (size_t)IROWS * nn * sizeof(int)) == hipSuccess;
+        ok = ok && hipMemcpyAsync(r->dd, dfields, (size_t)MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice,
+                                  s->upStream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(r->di, ifields, (size_t)MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice,
+                                  s->upStream) == hipSuccess;
+        if (ok) launch_iota(r->di + IROW_HOME * nn, n, 0, s->upStream);
+        ok = ok && hipStreamSynchronize(s->upStream) == hipSuccess;
+        if (ok) {
+            r->devBytes = sb;
+        } else {                               // fall back to a host copy that the loop uploads
+            free_rows(r.get());
+            std::lock_guard<std::mutex> lk(s->m);
+            s->deviceRowBytes -= sb;
+            stage = false;
+        }
+    }
+    if (!stage) {
+        r->d.assign(dfields, dfields + (size_t)MPF_NUM_FIELDS * nn);
+        r->iv.assign(ifields, ifields + (size_t)MPI_NUM_FIELDS * nn);
+    }
     std::lock_guard<std::mutex> lk(s->m);
     if (s->failed) return 1;
-    if (fp > s->p.memoryBudget) {
-        s->err = "request footprint exceeds the server's memory budget";
-        return 1;
-    }
     r->id = s->nextId++;
     r->tSubmit = s->now();
     *id = r->id;
@@ -465,27 +556,55 @@ int mpmgpu_server_info(MpmServer *s, int64_t id, MpmRequestInfo *info) {
 }
 
 int mpmgpu_server_collect(MpmServer *s, int64_t id, double *dfields, int32_t *ifields) {
-    std::lock_guard<std::mutex> lk(s->m);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) {
-        s->err = "unknown request id";
-        return 1;
+    Request *r;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) {
+            s->err = "unknown request id";
+            return 1;
+        }
+        r = it->second.get();
+        if (r->state != MPMGPU_REQ_DONE) {
+            s->err = r->state == MPMGPU_REQ_COLLECTED ? "request already collected" : "request not done";
+            return 1;
+        }
+        r->state = MPMGPU_REQ_COLLECTED;       // the result rows are this thread's now
     }
-    Request *r = it->second.get();
-    if (r->state != MPMGPU_REQ_DONE) {
-        s->err = r->state == MPMGPU_REQ_COLLECTED ? "request already collected" : "request not done";
-        return 1;
+    const size_t n = (size_t)r->n;
+    std::vector<double> d((size_t)MPF_NUM_FIELDS * n);
+    std::vector<int32_t> iv((size_t)IROWS * n);
+    bool ok = hipMemcpy(d.data(), r->dd, d.size() * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(iv.data(), r->di, iv.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    free_rows(r);
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->deviceRowBytes -= r->devBytes;
+        r->devBytes = 0;
+        if (!ok) {
+            s->err = "result download failed";
+            return 1;
+        }
     }
-    std::copy(r->d.begin(), r->d.end(), dfields);
-    if (ifields) std::copy(r->iv.begin(), r->iv.end(), ifields);
-    std::vector<double>().swap(r->d);
-    std::vector<int32_t>().swap(r->iv);
-    r->state = MPMGPU_REQ_COLLECTED;
+    // back to the solve's own order (upload-index row) and material numbering
+    const int32_t *home = iv.data() + (size_t)IROW_HOME * n;
+    for (size_t i = 0; i < n; i++) {
+        const long long h = (long long)home[i] - r->homeBase;
+        if (h < 0 || h >= (long long)n) {
+            std::lock_guard<std::mutex> lk(s->m);
+            s->err = "particle rows of a solve left its range";
+            return 1;
+        }
+        for (int f = 0; f < MPF_NUM_FIELDS; f++) dfields[f * n + h] = d[f * n + i];
+        if (ifields)
+            for (int q = 0; q < MPI_NUM_FIELDS; q++)
+                ifields[q * n + h] = iv[q * n + i] - (q == MPI_MAT ? r->resultFirstMat : 0);
+    }
     return 0;
 }
 
 int mpmgpu_server_counts(MpmServer *s, int32_t *queued, int32_t *resident, int64_t *residentBytes,
-                         int64_t *layouts) {
+                         int64_t *layouts, double *layoutSeconds, double *stepSeconds) {
     std::lock_guard<std::mutex> lk(s->m);
     if (queued) *queued = (int32_t)s->queue.size();
     if (resident) {
@@ -495,6 +614,8 @@ int mpmgpu_server_counts(MpmServer *s, int32_t *queued, int32_t *resident, int64
     }
     if (residentBytes) *residentBytes = s->residentBytes;
     if (layouts) *layouts = s->layouts;
+    if (layoutSeconds) *layoutSeconds = s->layoutSeconds;
+    if (stepSeconds) *stepSeconds = s->stepSeconds;
     return 0;
 }
 
@@ -506,6 +627,8 @@ void mpmgpu_server_destroy(MpmServer *s) {
     if (!s) return;
     mpmgpu_server_stop(s);
     if (s->ctx) mpmgpu_destroy(s->ctx);
+    for (auto &kv : s->reqs) free_rows(kv.second.get());
+    if (s->upStream) (void)hipStreamDestroy(s->upStream);
     delete s;
 }
 

@@ -151,8 +151,8 @@ _SIGNATURES = {
     "mpmgpu_server_wait": ([_P, ctypes.c_int64, ctypes.c_double, _PI], ctypes.c_int),
     "mpmgpu_server_info": ([_P, ctypes.c_int64, ctypes.POINTER(RequestInfo)], ctypes.c_int),
     "mpmgpu_server_collect": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
-    "mpmgpu_server_counts": ([_P, _PI, _PI, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
-                             ctypes.c_int),
+    "mpmgpu_server_counts": ([_P, _PI, _PI, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                              _PD, _PD], ctypes.c_int),
     "mpmgpu_server_clock": ([_P], ctypes.c_double),
     "mpmgpu_server_error": ([_P], ctypes.c_char_p),
     "mpmgpu_server_destroy": ([_P], None),
@@ -498,9 +498,11 @@ class Server:
     def counts(self) -> dict:
         q, r = ctypes.c_int32(), ctypes.c_int32()
         b, lay = ctypes.c_int64(), ctypes.c_int64()
+        ls, ss = ctypes.c_double(), ctypes.c_double()
         self._ok(library().mpmgpu_server_counts(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b),
-                                                ctypes.byref(lay)))
-        return {"queued": q.value, "resident": r.value, "resident_bytes": b.value, "layouts": lay.value}
+                                                ctypes.byref(lay), ctypes.byref(ls), ctypes.byref(ss)))
+        return {"queued": q.value, "resident": r.value, "resident_bytes": b.value, "layouts": lay.value,
+                "layout_seconds": ls.value, "step_seconds": ss.value}
 
     def clock(self) -> float:
         return float(library().mpmgpu_server_clock(self._h))
