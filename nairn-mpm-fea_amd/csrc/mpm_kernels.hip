@@ -1190,7 +1190,7 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
 template <int DIM, int SH>
-__global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
+__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int HALF = NT / 2;
@@ -1236,9 +1236,14 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
         CpdiCorners cc;
         if (k < c1) {
             int p = B.order[k];
-            int ei, ej, ek;
-            elem_ijk(g, P.elem[p], ei, ej, ek);
+            // every word of the particle is loaded before anything is stored, so the loads are
+            // issued together (a later load could not move above a store to another row)
+            const int e0 = P.elem[p];
             double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+            double vold[3] = {P.f[MPF_VEL_X][p], P.f[MPF_VEL_Y][p], DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.};
+            const double mp = P.f[MPF_MASS][p];
+            int ei, ej, ek;
+            elem_ijk(g, e0, ei, ej, ek);
             int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
             double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
             if (CP) {
@@ -1282,12 +1287,12 @@ __global__ __launch_bounds__(NT) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNod
             for (int c = 0; c < DIM; c++) {
                 double vm = Svk[c] - Sacc[c] * dt;
                 double delV = Sacc[c] * dt;
-                vnew[c] = P.f[MPF_VEL_X + c][p] + delV;
+                vnew[c] = vold[c] + delV;
                 P.f[MPF_VEL_X + c][p] = vnew[c];
                 double delXRate = vm + 0.5 * delV;
                 P.posn[c][p] = pos[c] + delXRate * dt;
             }
-            q4[0] = P.f[MPF_MASS][p];
+            q4[0] = mp;
             q4[1] = vnew[0]; q4[2] = vnew[1]; q4[3] = vnew[2];
         }
         for (int h = 0; h < 2; h++) {
@@ -1408,6 +1413,9 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g0, KPart P, KBins B, KN
     for (int k = tb + threadIdx.x; k < te; k += NT) {
         int p = B.order[k];
         int e = P.elem[p];
+        // the end-of-step position is loaded here, before the state stores below, so its load is
+        // issued with the others instead of after them
+        double pos[3] = {P.posn[0][p], P.posn[1][p], DIM == 3 ? P.posn[2][p] : 0.};
         int ei, ej, ek;
         elem_ijk(g, e, ei, ej, ek);
         double L[3][3];
@@ -1421,7 +1429,6 @@ __global__ __launch_bounds__(NT) void k_usl_reset(KGrid g0, KPart P, KBins B, KN
         store_state<DIM>(P, p, mtype, s);
 
         // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
-        double pos[3] = {P.posn[0][p], P.posn[1][p], DIM == 3 ? P.posn[2][p] : 0.};
         if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
             atomicAdd(st.nanCount, 1ull);
         } else if (!pt_in_elem<DIM>(g, ei, ej, ek, pos)) {
