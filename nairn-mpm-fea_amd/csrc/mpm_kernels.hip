@@ -617,10 +617,10 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int CH = CP ? 64 : NT / 2;      // particles staged per chunk (three blocks per CU)
     __shared__ int s_cs[G::NC + 1];
-    __shared__ double s_S[CP ? 1 : CH][DIM][4];
+    __shared__ double s_S[CP ? 1 : DIM * 4][CH + 1];   // [axis*4 + offset][particle]
     __shared__ double s_cxi[CP ? CH : 1][8][3];
     __shared__ int s_coff[CP ? CH : 1][8][3];
-    __shared__ double s_q[CH][4];
+    __shared__ double s_q[4][CH + 1];
     __shared__ double s_acc[NW][NQ_MASS][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
     const KGrid g = tile_grid(g0, tall % g0.tilesTotal);
@@ -636,9 +636,9 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
                     r.wx = cpdi_w<DIM>(s_cxi[lq], s_coff[lq], ix - 1, iy - 1, iz - 1);
                     r.wy = 1.; r.wz = 1.;
                 } else {
-                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                    r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
                 }
-                r.mp = s_q[lq][0]; r.vx = s_q[lq][1]; r.vy = s_q[lq][2]; r.vz = s_q[lq][3];
+                r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
                 return r;
             },
             [&](const RecMass &r, double *acc, int, int, int) {
@@ -693,10 +693,10 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
 #pragma unroll
                         for (int o = 0; o < 4; o++) {
                             const double v = S[a][o] >= 0. ? S[a][o] : 0.;
-                            s_S[i][a][o] = a == 0 ? v * rm[j] : v;
+                            s_S[a * 4 + o][i] = a == 0 ? v * rm[j] : v;
                         }
-                    s_q[i][0] = rm[j];
-                    s_q[i][1] = rv[j][0]; s_q[i][2] = rv[j][1]; s_q[i][3] = rv[j][2];
+                    s_q[0][i] = rm[j];
+                    s_q[1][i] = rv[j][0]; s_q[2][i] = rv[j][1]; s_q[3][i] = rv[j][2];
                 }
                 __syncthreads();
                 scatter(c0, min(c0 + CH, te));
@@ -725,10 +725,10 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
                         s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
                         s_coff[threadIdx.x][c][a] = cc.off[c][a];
                     }
-                s_q[threadIdx.x][0] = P.f[MPF_MASS][p];
-                s_q[threadIdx.x][1] = P.f[MPF_VEL_X][p];
-                s_q[threadIdx.x][2] = P.f[MPF_VEL_Y][p];
-                s_q[threadIdx.x][3] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+                s_q[0][threadIdx.x] = P.f[MPF_MASS][p];
+                s_q[1][threadIdx.x] = P.f[MPF_VEL_X][p];
+                s_q[2][threadIdx.x] = P.f[MPF_VEL_Y][p];
+                s_q[3][threadIdx.x] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
             }
             __syncthreads();
             scatter(c0, min(c0 + CH, te));
@@ -1196,10 +1196,10 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
     constexpr int HALF = NT / 2;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_n[6][G::FOOT];         // v(3), f/m(3)
-    __shared__ double s_S[CP ? 1 : HALF][DIM][4];
+    __shared__ double s_S[CP ? 1 : DIM * 4][HALF + 1];   // [axis*4 + offset][particle]
     __shared__ double s_cxi[CP ? HALF : 1][8][3];
     __shared__ int s_coff[CP ? HALF : 1][8][3];
-    __shared__ double s_q[HALF][4];
+    __shared__ double s_q[4][HALF + 1];
     __shared__ double s_acc[NW][NQ_MOM][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
     const int fld = tall / g0.tilesTotal;
@@ -1312,11 +1312,11 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
 #pragma unroll
                         for (int o = 0; o < 4; o++) {
                             const double v = S[a][o] >= 0. ? S[a][o] : 0.;
-                            s_S[i][a][o] = a == 0 ? v * q4[0] : v;
+                            s_S[a * 4 + o][i] = a == 0 ? v * q4[0] : v;
                         }
                 }
 #pragma unroll
-                for (int q = 0; q < 4; q++) s_q[i][q] = q4[q];
+                for (int q = 0; q < 4; q++) s_q[q][i] = q4[q];
             }
             __syncthreads();
             wave_cells<DIM, NQ_MOM>(s_cs, r0, r1, s_acc,
@@ -1326,9 +1326,9 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
                         r.wx = cpdi_w<DIM>(s_cxi[lq], s_coff[lq], ix - 1, iy - 1, iz - 1);
                         r.wy = 1.; r.wz = 1.;
                     } else {
-                        r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy]; r.wz = DIM == 3 ? s_S[lq][DIM - 1][iz] : 1.;
+                        r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
                     }
-                    r.mp = s_q[lq][0]; r.vx = s_q[lq][1]; r.vy = s_q[lq][2]; r.vz = s_q[lq][3];
+                    r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
                     return r;
                 },
                 [&](const RecMass &r, double *acc, int, int, int) {
