@@ -325,34 +325,6 @@ __device__ __forceinline__ bool cpdi_particle(const KGrid &g, const KPart &P, in
     return cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc);
 }
 
-// visit the particle's nodes in the reference's list order (corner by corner, element node
-// order, first appearance) with their merged weights: f(nx, ny, nz, w, grad)
-template <int DIM, typename Visit>
-__device__ __forceinline__ void cpdi_for_nodes(const CpdiCorners &cc, Visit visit) {
-    constexpr int NCOR = DIM == 3 ? 8 : 4;
-    for (int c = 0; c < NCOR; c++) {
-#pragma unroll
-        for (int jn = 0; jn < NCOR; jn++) {
-            // element node order (Generators.cpp:1919-1926): x fastest around the face, then z
-            const int q = jn & 3;
-            const int jx = (q == 1 || q == 2) ? 1 : 0, jy = q >= 2 ? 1 : 0, jz = jn >= 4 ? 1 : 0;
-            double cfn = cpdi_cfn<DIM>(cc.xi[c], jx, jy, jz);
-            if (cfn < 1e-15) continue;
-            int nx = cc.off[c][0] + jx, ny = cc.off[c][1] + jy, nz = DIM == 3 ? cc.off[c][2] + jz : 0;
-            bool seen = false;
-            for (int e = 0; e < c; e++) {
-                int ex = nx - cc.off[e][0], ey = ny - cc.off[e][1], ez = DIM == 3 ? nz - cc.off[e][2] : 0;
-                if ((unsigned)ex > 1u || (unsigned)ey > 1u || (unsigned)ez > 1u) continue;
-                if (cpdi_cfn<DIM>(cc.xi[e], ex, ey, ez) >= 1e-15) { seen = true; break; }
-            }
-            if (seen) continue;
-            double w, gr[3];
-            cpdi_node<DIM>(cc.xi, cc.wg, cc.off, c, nx, ny, nz, w, gr);
-            visit(nx, ny, nz, w, gr);
-        }
-    }
-}
-
 template <int DIM>
 __device__ __forceinline__ int foot_slot(int lx, int ly, int lz) {
     using G = TileGeom<DIM>;
@@ -806,6 +778,93 @@ __device__ __forceinline__ void load_foot(const KGrid &g, int ox, int oy, int oz
     }
 }
 
+// lCPDI gathers corner by corner. A node's merged weight and gradient are sums over the corners
+// whose element holds it (cpdi_node), so a gather over the merged node list is the same sum taken
+// corner by corner: sum_n w(n) v(n) = ws sum_c sum_j cfn_c(j) v(n_cj) and
+// sum_n v(n) grad(n) = sum_c (sum_j cfn_c(j) v(n_cj)) wg_c, where j runs over the nodes of corner c's
+// element (cfn below 1e-15 left out, as in the reference's list). Each corner is built on the fly.
+
+// corner c of a particle (the body of cpdi_corners for one corner; errors are counted by the
+// first scatter of the step)
+template <int DIM>
+__device__ __forceinline__ void cpdi_corner(const KGrid &g, const double pos[3], const double r[9], int ei, int ej,
+                                            int ek, int c, double xi[3], int off[3]) {
+    bool ok = true;
+    const int q = c & 3;
+    const double s1 = (q == 1 || q == 2) ? 1. : -1.;
+    const double s2 = q >= 2 ? 1. : -1.;
+    const double s3 = c >= 4 ? 1. : -1.;
+    double cx[3];
+    if (DIM == 3) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) cx[a] = pos[a] + s1 * r[a] + s2 * r[3 + a] + s3 * r[6 + a];
+    } else {
+        cx[0] = pos[0] + s1 * r[0] + s2 * r[3];
+        cx[1] = pos[1] + s1 * r[1] + s2 * r[4];
+        cx[2] = 0.;
+    }
+    const int ci = find_cell(cx[0], g.xmin, g.dx, g.nx, ok);
+    const int cj = find_cell(cx[1], g.ymin, g.dy, g.ny, ok);
+    const int ck = DIM == 3 ? find_cell(cx[2], g.zmin, g.dz, g.nz, ok) : 0;
+    off[0] = ci - ei; off[1] = cj - ej; off[2] = DIM == 3 ? ck - ek : 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) off[a] = off[a] < -1 ? -1 : (off[a] > 1 ? 1 : off[a]);
+    double invd[3];
+    particle_xi<DIM>(g, ei + off[0], ej + off[1], ek + off[2], cx[0], cx[1], cx[2], xi, invd);
+}
+
+// gradient weight of corner c: the reference's per-corner expressions (MatPoint3D.cpp:571-594,
+// MatPoint2D.cpp:551-566) collected as (s3 r1 x r2 - s2 r1 x r3 + s1 r2 x r3) / (8 V) in 3D and
+// (s1 r2y - s2 r1y, s2 r1x - s1 r2x) / (4 A) in 2D; inv = 1/(8 V) or 1/(4 A)
+template <int DIM>
+__device__ __forceinline__ void cpdi_wg(const double r[9], double inv, int c, double w[3]) {
+    const int q = c & 3;
+    const double s1 = (q == 1 || q == 2) ? 1. : -1.;
+    const double s2 = q >= 2 ? 1. : -1.;
+    if (DIM == 3) {
+        const double s3 = c >= 4 ? 1. : -1.;
+        const double *r1 = r, *r2 = r + 3, *r3 = r + 6;
+        const double c12[3] = {r1[1] * r2[2] - r1[2] * r2[1], r1[2] * r2[0] - r1[0] * r2[2], r1[0] * r2[1] - r1[1] * r2[0]};
+        const double c13[3] = {r1[1] * r3[2] - r1[2] * r3[1], r1[2] * r3[0] - r1[0] * r3[2], r1[0] * r3[1] - r1[1] * r3[0]};
+        const double c23[3] = {r2[1] * r3[2] - r2[2] * r3[1], r2[2] * r3[0] - r2[0] * r3[2], r2[0] * r3[1] - r2[1] * r3[0]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) w[a] = (s3 * c12[a] - s2 * c13[a] + s1 * c23[a]) * inv;
+    } else {
+        w[0] = (s1 * r[4] - s2 * r[1]) * inv;
+        w[1] = (s2 * r[0] - s1 * r[3]) * inv;
+        w[2] = 0.;
+    }
+}
+
+// 1/(8 V) (3D) or 1/(4 A) (2D) of the particle domain with semi-side vectors r
+template <int DIM>
+__device__ __forceinline__ double cpdi_inv_measure(const double r[9]) {
+    if (DIM == 3)
+        return 1. / (8. * (r[0] * (r[4] * r[8] - r[5] * r[7]) + r[1] * (r[5] * r[6] - r[3] * r[8]) +
+                           r[2] * (r[3] * r[7] - r[4] * r[6])));
+    return 1. / (4. * (r[0] * r[4] - r[1] * r[3]));
+}
+
+// sum over the nodes of corner c's element of cfn * rows[v][node], footprint slots relative to the
+// particle's cell (lx, ly, lz) in its tile
+template <int DIM, int NV>
+__device__ __forceinline__ void cpdi_corner_sum(const double xi[3], const int off[3], int lx, int ly, int lz,
+                                                const double (*rows)[TileGeom<DIM>::FOOT], double out[NV]) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = 0.;
+    constexpr int NN = DIM == 3 ? 8 : 4;
+#pragma unroll
+    for (int jn = 0; jn < NN; jn++) {
+        const int q = jn & 3;
+        const int jx = (q == 1 || q == 2) ? 1 : 0, jy = q >= 2 ? 1 : 0, jz = jn >= 4 ? 1 : 0;
+        double cfn = cpdi_cfn<DIM>(xi, jx, jy, jz);
+        cfn = cfn < 1e-15 ? 0. : cfn;
+        const int sl = foot_slot<DIM>(lx + off[0] + jx, ly + off[1] + jy, DIM == 3 ? lz + off[2] + jz : 0);
+#pragma unroll
+        for (int v = 0; v < NV; v++) out[v] = fma(cfn, rows[v][sl], out[v]);
+    }
+}
+
 // Support window of one particle: with a half size lp <= 1 (natural units) a particle with xi < 0
 // touches no node at offset 2 and one with xi >= 0 none at offset -1, so three offsets per axis
 // starting at w = 0 (offsets -1..1) or w = 1 (0..2) hold every node of its support. The window's
@@ -881,21 +940,24 @@ __device__ __forceinline__ void particle_gradient(const KGrid &g, const KPart &P
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int b = 0; b < 3; b++) L[a][b] = 0.;
-        CpdiCorners cc;
-        cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
-        cpdi_for_nodes<DIM>(cc, [&](int nx, int ny, int nz, double, const double *gr) {
-            int s = foot_slot<DIM>(lx + nx, ly + ny, lz + nz);
-            double vx = s_v[0][s], vy = s_v[1][s], vz = s_v[2][s];
-            if (DIM == 3) {
-                L[0][0] += vx * gr[0]; L[0][1] += vx * gr[1]; L[0][2] += vx * gr[2];
-                L[1][0] += vy * gr[0]; L[1][1] += vy * gr[1]; L[1][2] += vy * gr[2];
-                L[2][0] += vz * gr[0]; L[2][1] += vz * gr[1]; L[2][2] += vz * gr[2];
-            } else {
-                L[0][0] += vx * gr[0]; L[0][1] += vx * gr[1];
-                L[1][0] += vy * gr[0]; L[1][1] += vy * gr[1];
-                L[2][2] += 0.;
-            }
-        });
+        double r[9];
+#pragma unroll
+        for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+        const double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+        const double inv = cpdi_inv_measure<DIM>(r);
+        constexpr int NCOR = DIM == 3 ? 8 : 4;
+#pragma unroll 1
+        for (int c = 0; c < NCOR; c++) {
+            double xi[3], vc[3], w[3];
+            int off[3];
+            cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
+            cpdi_corner_sum<DIM, 3>(xi, off, lx, ly, lz, s_v, vc);
+            cpdi_wg<DIM>(r, inv, c, w);
+#pragma unroll
+            for (int a = 0; a < DIM; a++)
+#pragma unroll
+                for (int b = 0; b < DIM; b++) L[a][b] = fma(vc[a], w[b], L[a][b]);
+        }
     } else {
         double xi[3], invd[3], S[3][4], dS[3][4];
         particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
@@ -1248,15 +1310,18 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
             double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
             if (CP) {
                 cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
-                cpdi_for_nodes<DIM>(cc, [&](int nx, int ny, int nz, double fi, const double *) {
-                    int s = foot_slot<DIM>(lx + nx, ly + ny, lz + nz);
-                    Svk[0] = fma(s_n[0][s], fi, Svk[0]);
-                    Svk[1] = fma(s_n[1][s], fi, Svk[1]);
-                    Svk[2] = fma(s_n[2][s], fi, Svk[2]);
-                    Sacc[0] = fma(s_n[3][s], fi, Sacc[0]);
-                    Sacc[1] = fma(s_n[4][s], fi, Sacc[1]);
-                    Sacc[2] = fma(s_n[5][s], fi, Sacc[2]);
-                });
+                constexpr int NCOR = DIM == 3 ? 8 : 4;
+                constexpr double ws = DIM == 3 ? 0.125 : 0.25;
+                double sum[6] = {0., 0., 0., 0., 0., 0.};
+#pragma unroll
+                for (int c = 0; c < NCOR; c++) {
+                    double vc[6];
+                    cpdi_corner_sum<DIM, 6>(cc.xi[c], cc.off[c], lx, ly, lz, s_n, vc);
+#pragma unroll
+                    for (int v = 0; v < 6; v++) sum[v] += vc[v];
+                }
+#pragma unroll
+                for (int a = 0; a < 3; a++) { Svk[a] = ws * sum[a]; Sacc[a] = ws * sum[3 + a]; }
             } else {
             double xi[3], invd[3], dS[3][4];
             particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
