@@ -1030,7 +1030,7 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
 // USF strain update, first half step (UpdateStrainsFirstTask::FullStrainUpdate,
 // UpdateStrainsFirstTask.cpp:101-168 -> MatPoint3D::UpdateStrain, MatPoint3D.cpp:45-60)
 template <int DIM, int MAT, int SH>
-__global__ __launch_bounds__(NT) void k_usf(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
+__global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
@@ -1454,7 +1454,7 @@ __device__ __forceinline__ bool pt_in_elem(const KGrid &g, int ei, int ej, int e
 // USL strain update (second half step) and ResetElementsTask; also the cell key and histogram
 // of the next binning
 template <int DIM, int MAT, int SH>
-__global__ __launch_bounds__(NT) void k_usl_reset(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+__global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
                                                   KStats st) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
