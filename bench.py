@@ -234,7 +234,7 @@ def bench_serve(a):
     srv.close()
 
     # served: the request loop runs on the server's thread; this thread submits at arrival times
-    srv = nm.Server(analysis=nm.THREED, graph_steps=a.graph_steps)
+    srv = nm.Server(analysis=nm.THREED, graph_steps=a.graph_steps, max_resident=a.max_resident)
     srv.start()
     base = srv.clock()
     wall = time.perf_counter()
@@ -296,6 +296,7 @@ def bench_serve(a):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"served 3D stream: {a.requests} J2 or neo-Hookean blocks, 50k-2M particles "
                                f"log-uniform, 200-2000 steps, uGIMP, Poisson arrivals at {a.rate:g}/s",
+                   "max_resident": a.max_resident,
                    "particle_steps": int(total_ps), "span_s": span, "layouts": layouts,
                    "layout_s": cnt["layout_seconds"], "batch_step_s": cnt["step_seconds"],
                    "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
@@ -324,6 +325,7 @@ def main():
     ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--graph-steps", type=int, default=8, help="serve: batch steps per graph / admission check")
+    ap.add_argument("--max-resident", type=int, default=0, help="serve: most solves resident at once (0: no limit)")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of sweep instances")
     ap.add_argument("--serial-steps", type=int, default=20, help="batch: steps of each instance run alone")
     ap.add_argument("--cpu-seconds", type=float, default=10., help="batch: CPU baseline sample length")
