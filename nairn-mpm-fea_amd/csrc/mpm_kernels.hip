@@ -137,7 +137,6 @@ __device__ __forceinline__ void particle_factors(const KGrid &g, const KPart &P,
 // is clamped so that every access stays in bounds.
 struct CpdiCorners {
     double xi[8][3];           // natural coordinates of each corner in its element
-    double wg[8][3];           // gradient weights
     int off[8][3];             // element of the corner relative to the particle's element (-1..1)
 };
 
@@ -180,87 +179,6 @@ __device__ __forceinline__ void cpdi_semi_sides(const KGrid &g, const KPart &P, 
     }
 }
 
-// corners, their natural coordinates and the gradient weights; false if a corner is unusable
-template <int DIM>
-__device__ __forceinline__ bool cpdi_corners(const KGrid &g, const double pos[3], const double r[9], int ei, int ej,
-                                             int ek, CpdiCorners &cc) {
-    bool ok = true;
-    constexpr int NCOR = DIM == 3 ? 8 : 4;
-#pragma unroll
-    for (int c = 0; c < NCOR; c++) {
-        // corner signs: 3D r1s/r2s/r3s (MatPoint3D.cpp:23-25), 2D ccw from (-,-) (MatPoint2D.cpp:538-556)
-        const int q = c & 3;
-        const double s1 = (q == 1 || q == 2) ? 1. : -1.;
-        const double s2 = q >= 2 ? 1. : -1.;
-        const double s3 = c >= 4 ? 1. : -1.;
-        double cx[3];
-        if (DIM == 3) {
-#pragma unroll
-            for (int a = 0; a < 3; a++) cx[a] = pos[a] + s1 * r[a] + s2 * r[3 + a] + s3 * r[6 + a];
-        } else {
-            cx[0] = pos[0] + s1 * r[0] + s2 * r[3];
-            cx[1] = pos[1] + s1 * r[1] + s2 * r[4];
-            cx[2] = 0.;
-        }
-        int ci = find_cell(cx[0], g.xmin, g.dx, g.nx, ok);
-        int cj = find_cell(cx[1], g.ymin, g.dy, g.ny, ok);
-        int ck = DIM == 3 ? find_cell(cx[2], g.zmin, g.dz, g.nz, ok) : 0;
-        int o[3] = {ci - ei, cj - ej, DIM == 3 ? ck - ek : 0};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            if (o[a] < -1 || o[a] > 1) { ok = false; o[a] = o[a] < -1 ? -1 : 1; }
-            cc.off[c][a] = o[a];
-        }
-        double invd[3];
-        particle_xi<DIM>(g, ei + o[0], ej + o[1], ek + o[2], cx[0], cx[1], cx[2], cc.xi[c], invd);
-    }
-    if (DIM == 3) {
-        const double x1 = r[0], y1 = r[1], z1 = r[2], x2 = r[3], y2 = r[4], z2 = r[5];
-        const double x3 = r[6], y3 = r[7], z3 = r[8];
-        double Vp = 8. * (x1 * (y2 * z3 - z2 * y3) + y1 * (z2 * x3 - x2 * z3) + z1 * (x2 * y3 - y2 * x3));
-        Vp = 1. / Vp;
-        double (*w)[3] = cc.wg;
-        w[0][0] = (z1*y2 - y1*z2 - z1*y3 + z2*y3 + y1*z3 - y2*z3)*Vp;
-        w[0][1] = (-(z1*x2) + x1*z2 + z1*x3 - z2*x3 - x1*z3 + x2*z3)*Vp;
-        w[0][2] = (y1*x2 - x1*y2 - y1*x3 + y2*x3 + x1*y3 - x2*y3)*Vp;
-        w[1][0] = (z1*y2 - y1*z2 - z1*y3 - z2*y3 + y1*z3 + y2*z3)*Vp;
-        w[1][1] = (-(z1*x2) + x1*z2 + z1*x3 + z2*x3 - x1*z3 - x2*z3)*Vp;
-        w[1][2] = (y1*x2 - x1*y2 - y1*x3 - y2*x3 + x1*y3 + x2*y3)*Vp;
-        w[2][0] = (z1*y2 - y1*z2 + z1*y3 - z2*y3 - y1*z3 + y2*z3)*Vp;
-        w[2][1] = (-(z1*x2) + x1*z2 - z1*x3 + z2*x3 + x1*z3 - x2*z3)*Vp;
-        w[2][2] = (y1*x2 - x1*y2 + y1*x3 - y2*x3 - x1*y3 + x2*y3)*Vp;
-        w[3][0] = (z1*y2 - y1*z2 + z1*y3 + z2*y3 - y1*z3 - y2*z3)*Vp;
-        w[3][1] = (-(z1*x2) + x1*z2 - z1*x3 - z2*x3 + x1*z3 + x2*z3)*Vp;
-        w[3][2] = (y1*x2 - x1*y2 + y1*x3 + y2*x3 - x1*y3 - x2*y3)*Vp;
-        w[4][0] = (-(z1*y2) + y1*z2 - z1*y3 + z2*y3 + y1*z3 - y2*z3)*Vp;
-        w[4][1] = (z1*x2 - x1*z2 + z1*x3 - z2*x3 - x1*z3 + x2*z3)*Vp;
-        w[4][2] = (-(y1*x2) + x1*y2 - y1*x3 + y2*x3 + x1*y3 - x2*y3)*Vp;
-        w[5][0] = (-(z1*y2) + y1*z2 - z1*y3 - z2*y3 + y1*z3 + y2*z3)*Vp;
-        w[5][1] = (z1*x2 - x1*z2 + z1*x3 + z2*x3 - x1*z3 - x2*z3)*Vp;
-        w[5][2] = (-(y1*x2) + x1*y2 - y1*x3 - y2*x3 + x1*y3 + x2*y3)*Vp;
-        w[6][0] = (-(z1*y2) + y1*z2 + z1*y3 - z2*y3 - y1*z3 + y2*z3)*Vp;
-        w[6][1] = (z1*x2 - x1*z2 - z1*x3 + z2*x3 + x1*z3 - x2*z3)*Vp;
-        w[6][2] = (-(y1*x2) + x1*y2 + y1*x3 - y2*x3 - x1*y3 + x2*y3)*Vp;
-        w[7][0] = (-(z1*y2) + y1*z2 + z1*y3 + z2*y3 - y1*z3 - y2*z3)*Vp;
-        w[7][1] = (z1*x2 - x1*z2 - z1*x3 - z2*x3 + x1*z3 + x2*z3)*Vp;
-        w[7][2] = (-(y1*x2) + x1*y2 + y1*x3 + y2*x3 - x1*y3 - x2*y3)*Vp;
-    } else {
-        double Ap = 4. * (r[0] * r[4] - r[1] * r[3]);
-        Ap = 1. / Ap;
-        cc.wg[0][0] = (r[1] - r[4]) * Ap;
-        cc.wg[1][0] = (r[1] + r[4]) * Ap;
-        cc.wg[2][0] = (-r[1] + r[4]) * Ap;
-        cc.wg[3][0] = (-r[1] - r[4]) * Ap;
-        cc.wg[0][1] = (-r[0] + r[3]) * Ap;
-        cc.wg[1][1] = (-r[0] - r[3]) * Ap;
-        cc.wg[2][1] = (r[0] - r[3]) * Ap;
-        cc.wg[3][1] = (r[0] + r[3]) * Ap;
-#pragma unroll
-        for (int c = 0; c < 4; c++) cc.wg[c][2] = 0.;
-    }
-    return ok;
-}
-
 // linear function of element node (jx,jy,jz) at natural coordinates x (EightNodeIsoparamBrick.cpp:87-108,
 // FourNodeIsoparam.cpp:189-214): 0.125 (1 +- xi)(1 +- eta)(1 +- zeta)
 template <int DIM>
@@ -274,56 +192,6 @@ __device__ __forceinline__ double cpdi_cfn(const double x[3], int jx, int jy, in
     return 0.25 * t1 * t2;
 }
 
-// merged weight and gradient of node n (offset from the particle's element) over corners
-// c0..last, in corner order; corners whose element lacks the node or whose function is below
-// 1e-15 are skipped (GetCPDIFunctions)
-template <int DIM>
-__device__ __forceinline__ void cpdi_node(const double (*xi)[3], const double (*wg)[3], const int (*off)[3], int c0,
-                                          int nx, int ny, int nz, double &w, double gr[3]) {
-    constexpr int NCOR = DIM == 3 ? 8 : 4;
-    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
-    w = 0.; gr[0] = gr[1] = gr[2] = 0.;
-#pragma unroll
-    for (int c = 0; c < NCOR; c++) {
-        if (c < c0) continue;
-        int jx = nx - off[c][0], jy = ny - off[c][1], jz = DIM == 3 ? nz - off[c][2] : 0;
-        if ((unsigned)jx > 1u || (unsigned)jy > 1u || (unsigned)jz > 1u) continue;
-        double cfn = cpdi_cfn<DIM>(xi[c], jx, jy, jz);
-        if (cfn < 1e-15) continue;
-        w += ws * cfn;
-        gr[0] += wg[c][0] * cfn;
-        gr[1] += wg[c][1] * cfn;
-        gr[2] += wg[c][2] * cfn;
-    }
-}
-
-// merged weight only (no gradient) of node n over all corners
-template <int DIM>
-__device__ __forceinline__ double cpdi_w(const double (*xi)[3], const int (*off)[3], int nx, int ny, int nz) {
-    constexpr int NCOR = DIM == 3 ? 8 : 4;
-    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
-    double w = 0.;
-#pragma unroll
-    for (int c = 0; c < NCOR; c++) {
-        int jx = nx - off[c][0], jy = ny - off[c][1], jz = DIM == 3 ? nz - off[c][2] : 0;
-        if ((unsigned)jx > 1u || (unsigned)jy > 1u || (unsigned)jz > 1u) continue;
-        double cfn = cpdi_cfn<DIM>(xi[c], jx, jy, jz);
-        if (cfn < 1e-15) continue;
-        w += ws * cfn;
-    }
-    return w;
-}
-
-// corners of particle p for the step: from its saved semi-side vectors and start position
-template <int DIM>
-__device__ __forceinline__ bool cpdi_particle(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek,
-                                              CpdiCorners &cc) {
-    double r[9];
-#pragma unroll
-    for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
-    double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-    return cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc);
-}
 
 template <int DIM>
 __device__ __forceinline__ int foot_slot(int lx, int ly, int lz) {
@@ -342,6 +210,122 @@ __device__ __forceinline__ long long foot_node(const KGrid &g, int s, int ox, in
     int i = ox + fx - 1, j = oy + fy - 1, k = DIM == 3 ? oz + fz - 1 : 0;
     if (i < 0 || i >= g.nnx || j < 0 || j >= g.nny || k < 0 || k >= g.nnz) return -1;
     return ((long long)k * g.nny + j) * g.nnx + i;
+}
+
+// corner c of a particle (the body of cpdi_corners for one corner); false if the corner is unusable
+template <int DIM>
+__device__ __forceinline__ bool cpdi_corner(const KGrid &g, const double pos[3], const double r[9], int ei, int ej,
+                                            int ek, int c, double xi[3], int off[3]) {
+    bool ok = true;
+    const int q = c & 3;
+    const double s1 = (q == 1 || q == 2) ? 1. : -1.;
+    const double s2 = q >= 2 ? 1. : -1.;
+    const double s3 = c >= 4 ? 1. : -1.;
+    double cx[3];
+    if (DIM == 3) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) cx[a] = pos[a] + s1 * r[a] + s2 * r[3 + a] + s3 * r[6 + a];
+    } else {
+        cx[0] = pos[0] + s1 * r[0] + s2 * r[3];
+        cx[1] = pos[1] + s1 * r[1] + s2 * r[4];
+        cx[2] = 0.;
+    }
+    const int ci = find_cell(cx[0], g.xmin, g.dx, g.nx, ok);
+    const int cj = find_cell(cx[1], g.ymin, g.dy, g.ny, ok);
+    const int ck = DIM == 3 ? find_cell(cx[2], g.zmin, g.dz, g.nz, ok) : 0;
+    off[0] = ci - ei; off[1] = cj - ej; off[2] = DIM == 3 ? ck - ek : 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (off[a] < -1 || off[a] > 1) { ok = false; off[a] = off[a] < -1 ? -1 : 1; }
+    }
+    double invd[3];
+    particle_xi<DIM>(g, ei + off[0], ej + off[1], ek + off[2], cx[0], cx[1], cx[2], xi, invd);
+    return ok;
+}
+
+// corners of particle p for the step: from its saved semi-side vectors and start position
+template <int DIM>
+__device__ __forceinline__ bool cpdi_particle(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek,
+                                              CpdiCorners &cc) {
+    double r[9];
+#pragma unroll
+    for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+    double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+    bool ok = true;
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+#pragma unroll
+    for (int c = 0; c < NCOR; c++) ok &= cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, cc.xi[c], cc.off[c]);
+    return ok;
+}
+
+// gradient weight of corner c as the reference writes it out per corner (MatPoint3D.cpp:571-594,
+// MatPoint2D.cpp:551-566): the same six products summed in the same order, each term's sign taken
+// from the corner's signs s1, s2, s3 (the scatters use these; the gathers use cpdi_wg)
+template <int DIM>
+__device__ __forceinline__ void cpdi_wg_listed(const double r[9], int c, double w[3]) {
+    const int q = c & 3;
+    const double s1 = (q == 1 || q == 2) ? 1. : -1.;
+    const double s2 = q >= 2 ? 1. : -1.;
+    if (DIM == 3) {
+        const double s3 = c >= 4 ? 1. : -1.;
+        const double x1 = r[0], y1 = r[1], z1 = r[2], x2 = r[3], y2 = r[4], z2 = r[5];
+        const double x3 = r[6], y3 = r[7], z3 = r[8];
+        double Vp = 8. * (x1 * (y2 * z3 - z2 * y3) + y1 * (z2 * x3 - x2 * z3) + z1 * (x2 * y3 - y2 * x3));
+        Vp = 1. / Vp;
+        w[0] = ((((-s3 * (z1 * y2) + s3 * (y1 * z2)) + s2 * (z1 * y3)) - s1 * (z2 * y3)) - s2 * (y1 * z3) + s1 * (y2 * z3)) * Vp;
+        w[1] = ((((s3 * (z1 * x2) - s3 * (x1 * z2)) - s2 * (z1 * x3)) + s1 * (z2 * x3)) + s2 * (x1 * z3) - s1 * (x2 * z3)) * Vp;
+        w[2] = ((((-s3 * (y1 * x2) + s3 * (x1 * y2)) + s2 * (y1 * x3)) - s1 * (y2 * x3)) - s2 * (x1 * y3) + s1 * (x2 * y3)) * Vp;
+    } else {
+        double Ap = 4. * (r[0] * r[4] - r[1] * r[3]);
+        Ap = 1. / Ap;
+        w[0] = (-s2 * r[1] + s1 * r[4]) * Ap;
+        w[1] = (s2 * r[0] - s1 * r[3]) * Ap;
+        w[2] = 0.;
+    }
+}
+
+// Corner tables of the lCPDI scatters. A scatter lane owns candidate node offset o1 - 1 per axis
+// (o1 = 0..3); corner c's element holds the nodes at offsets off and off + 1, where the corner's
+// linear factors are 1 - xi and 1 + xi. The table of an axis holds those two factors at o1 = off + 1
+// and off + 2 and zeros elsewhere, so a lane reads the corner's function for its node with one load
+// per axis, zero for nodes outside the element. Built by one thread per (particle, corner).
+template <int DIM>
+__device__ __forceinline__ void cpdi_table(const double xi[3], const int off[3], double (*T)[4]) {
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+#pragma unroll
+        for (int o = 0; o < 4; o++)
+            T[a][o] = o == off[a] + 1 ? 1. - xi[a] : (o == off[a] + 2 ? 1. + xi[a] : 0.);
+}
+
+// corner function of the lane's node from a corner table: cpdi_cfn of the node in the corner's
+// element (same products in the same order), zero outside it or below 1e-15 (GetCPDIFunctions)
+template <int DIM>
+__device__ __forceinline__ double cpdi_table_cfn(const double (*T)[4], int o1x, int o1y, int o1z) {
+    const double cfn = DIM == 3 ? 0.125 * T[0][o1x] * T[1][o1y] * T[2][o1z] : 0.25 * T[0][o1x] * T[1][o1y];
+    return cfn < 1e-15 ? 0. : cfn;
+}
+
+// merged weight (and gradient) of the lane's node over the corners of one staged particle, summed
+// in corner order; corners whose element lacks the node or whose function is below 1e-15 add
+// nothing (GetCPDIFunctions, MoreMPMElementBase.cpp:581-656)
+template <int DIM, bool GRAD>
+__device__ __forceinline__ void cpdi_table_node(const double (*T)[DIM][4], const double (*wg)[3], int o1x, int o1y,
+                                                int o1z, double &w, double gr[3]) {
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
+    w = 0.;
+    if (GRAD) gr[0] = gr[1] = gr[2] = 0.;
+#pragma unroll
+    for (int c = 0; c < NCOR; c++) {
+        const double cfn = cpdi_table_cfn<DIM>(T[c], o1x, o1y, o1z);
+        w += ws * cfn;
+        if (GRAD) {
+            gr[0] += wg[c][0] * cfn;
+            gr[1] += wg[c][1] * cfn;
+            if (DIM == 3) gr[2] += wg[c][2] * cfn;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -587,11 +571,12 @@ template <int DIM, int SH>
 __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNodes N, KStats st) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
-    constexpr int CH = CP ? 64 : NT / 2;      // particles staged per chunk (three blocks per CU)
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    // particles staged per chunk (three blocks per CU); lCPDI: one thread per particle corner
+    constexpr int CH = CP ? NT / NCOR : NT / 2;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_S[CP ? 1 : DIM * 4][CH + 1];   // [axis*4 + offset][particle]
-    __shared__ double s_cxi[CP ? CH : 1][8][3];
-    __shared__ int s_coff[CP ? CH : 1][8][3];
+    __shared__ double s_ct[CP ? CH : 1][NCOR][DIM][4];
     __shared__ double s_q[4][CH + 1];
     __shared__ double s_acc[NW][NQ_MASS][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
@@ -605,7 +590,8 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
             [&](int lq, int ix, int iy, int iz) {
                 RecMass r;
                 if (CP) {
-                    r.wx = cpdi_w<DIM>(s_cxi[lq], s_coff[lq], ix - 1, iy - 1, iz - 1);
+                    double gr[3];
+                    cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
                     r.wy = 1.; r.wz = 1.;
                 } else {
                     r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
@@ -676,32 +662,35 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
             }
         }
     } else {
+        const int i = threadIdx.x / NCOR, c = threadIdx.x % NCOR;
         for (int c0 = tb; c0 < te; c0 += CH) {
-            int k = c0 + threadIdx.x;
-            if (threadIdx.x < CH && k < te) {
-                int p = B.order[k];
+            const int k = c0 + i;
+            bool ok = true;
+            if (k < te) {
+                const int p = B.order[k];
                 int ei, ej, ek;
                 elem_ijk(g, P.elem[p], ei, ej, ek);
                 double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-                // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step
+                // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step (each
+                // corner's thread builds them; the first keeps them for the step's other passes)
                 double r[9];
                 cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);
+                double xi[3];
+                int off[3];
+                ok = cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
+                cpdi_table<DIM>(xi, off, s_ct[i][c]);
+                if (c == 0) {
 #pragma unroll
-                for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
-                CpdiCorners cc;
-                if (!cpdi_corners<DIM>(g, pos, r, ei, ej, ek, cc)) atomicAdd(st.cpdiErr, 1ull);
-#pragma unroll
-                for (int c = 0; c < 8; c++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
-                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
-                    }
-                s_q[0][threadIdx.x] = P.f[MPF_MASS][p];
-                s_q[1][threadIdx.x] = P.f[MPF_VEL_X][p];
-                s_q[2][threadIdx.x] = P.f[MPF_VEL_Y][p];
-                s_q[3][threadIdx.x] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+                    for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
+                    s_q[0][i] = P.f[MPF_MASS][p];
+                    s_q[1][i] = P.f[MPF_VEL_X][p];
+                    s_q[2][i] = P.f[MPF_VEL_Y][p];
+                    s_q[3][i] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+                }
             }
+            // one count per particle with an unusable corner (its corner threads are adjacent lanes)
+            const unsigned long long bad = __ballot(!ok);
+            if (c == 0 && ((bad >> (threadIdx.x & 63)) & ((1ull << NCOR) - 1ull))) atomicAdd(st.cpdiErr, 1ull);
             __syncthreads();
             scatter(c0, min(c0 + CH, te));
             __syncthreads();
@@ -779,39 +768,10 @@ __device__ __forceinline__ void load_foot(const KGrid &g, int ox, int oy, int oz
 }
 
 // lCPDI gathers corner by corner. A node's merged weight and gradient are sums over the corners
-// whose element holds it (cpdi_node), so a gather over the merged node list is the same sum taken
+// whose element holds it (cpdi_table_node), so a gather over the merged node list is the same sum taken
 // corner by corner: sum_n w(n) v(n) = ws sum_c sum_j cfn_c(j) v(n_cj) and
 // sum_n v(n) grad(n) = sum_c (sum_j cfn_c(j) v(n_cj)) wg_c, where j runs over the nodes of corner c's
 // element (cfn below 1e-15 left out, as in the reference's list). Each corner is built on the fly.
-
-// corner c of a particle (the body of cpdi_corners for one corner; errors are counted by the
-// first scatter of the step)
-template <int DIM>
-__device__ __forceinline__ void cpdi_corner(const KGrid &g, const double pos[3], const double r[9], int ei, int ej,
-                                            int ek, int c, double xi[3], int off[3]) {
-    bool ok = true;
-    const int q = c & 3;
-    const double s1 = (q == 1 || q == 2) ? 1. : -1.;
-    const double s2 = q >= 2 ? 1. : -1.;
-    const double s3 = c >= 4 ? 1. : -1.;
-    double cx[3];
-    if (DIM == 3) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) cx[a] = pos[a] + s1 * r[a] + s2 * r[3 + a] + s3 * r[6 + a];
-    } else {
-        cx[0] = pos[0] + s1 * r[0] + s2 * r[3];
-        cx[1] = pos[1] + s1 * r[1] + s2 * r[4];
-        cx[2] = 0.;
-    }
-    const int ci = find_cell(cx[0], g.xmin, g.dx, g.nx, ok);
-    const int cj = find_cell(cx[1], g.ymin, g.dy, g.ny, ok);
-    const int ck = DIM == 3 ? find_cell(cx[2], g.zmin, g.dz, g.nz, ok) : 0;
-    off[0] = ci - ei; off[1] = cj - ej; off[2] = DIM == 3 ? ck - ek : 0;
-#pragma unroll
-    for (int a = 0; a < 3; a++) off[a] = off[a] < -1 ? -1 : (off[a] > 1 ? 1 : off[a]);
-    double invd[3];
-    particle_xi<DIM>(g, ei + off[0], ej + off[1], ek + off[2], cx[0], cx[1], cx[2], xi, invd);
-}
 
 // gradient weight of corner c: the reference's per-corner expressions (MatPoint3D.cpp:571-594,
 // MatPoint2D.cpp:551-566) collected as (s3 r1 x r2 - s2 r1 x r3 + s1 r2 x r3) / (8 V) in 3D and
@@ -1074,16 +1034,16 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
-    constexpr int CH = CP ? 64 : NT / 2;      // particles staged per chunk (keeps LDS < 80 KB)
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    constexpr int CH = CP ? NT / NCOR : NT / 2;   // particles staged per chunk (lCPDI: a thread per corner)
     constexpr int NA = DIM == 3 ? 6 : 3;      // -mp times the stress terms (shape-function path)
     __shared__ int s_cs[G::NC + 1];
     // shape-function path: per axis and candidate offset, the factor and its derivative times
     // 2/dx (zero outside the support), read as one pair
     __shared__ double2 s_F[CP ? 1 : CH][DIM][4];
     __shared__ double s_a[CP ? 1 : CH][NA];
-    __shared__ double s_cxi[CP ? CH : 1][8][3];
-    __shared__ double s_cwg[CP ? CH : 1][8][3];
-    __shared__ int s_coff[CP ? CH : 1][8][3];
+    __shared__ double s_ct[CP ? CH : 1][NCOR][DIM][4];
+    __shared__ double s_cwg[CP ? CH : 1][NCOR][3];
     __shared__ double s_st[CP ? CH : 1][NS];
     __shared__ double s_acc[NW][NQ_FORCE][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
@@ -1094,37 +1054,47 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     if (tb == te) return;
     zero_acc<DIM, NQ_FORCE>(s_acc);
     for (int c0 = tb; c0 < te; c0 += CH) {
+        if (CP) {
+            // thread (i, c): corner c of staged particle i; the first corner's thread stages the stress
+            const int i = threadIdx.x / NCOR, c = threadIdx.x % NCOR;
+            const int k = c0 + i;
+            if (k < te) {
+                const int p = B.order[k];
+                int ei, ej, ek;
+                elem_ijk(g, P.elem[p], ei, ej, ek);
+                double r[9];
+#pragma unroll
+                for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+                double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+                double xi[3], w[3];
+                int off[3];
+                cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
+                cpdi_table<DIM>(xi, off, s_ct[i][c]);
+                cpdi_wg_listed<DIM>(r, c, w);
+                s_cwg[i][c][0] = w[0]; s_cwg[i][c][1] = w[1]; s_cwg[i][c][2] = w[2];
+                if (c == 0) {
+                    const double Pr = P.f[MPF_PRESSURE][p];
+                    s_st[i][0] = -P.f[MPF_MASS][p];
+                    s_st[i][1] = P.f[MPF_SP_XX][p] - Pr;
+                    s_st[i][2] = P.f[MPF_SP_YY][p] - Pr;
+                    s_st[i][3] = P.f[MPF_SP_XY][p];
+                    if (DIM == 3) {
+                        s_st[i][NS - 3] = P.f[MPF_SP_ZZ][p] - Pr;
+                        s_st[i][NS - 2] = P.f[MPF_SP_XZ][p];
+                        s_st[i][NS - 1] = P.f[MPF_SP_YZ][p];
+                    }
+                }
+            }
+        }
         int k = c0 + threadIdx.x;
-        if (threadIdx.x < CH && k < te) {
+        if (!CP && threadIdx.x < CH && k < te) {
             int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
-            if (CP) {
-                CpdiCorners cc;
-                cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
-#pragma unroll
-                for (int c = 0; c < 8; c++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
-                        s_cwg[threadIdx.x][c][a] = cc.wg[c][a];
-                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
-                    }
-            }
             double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
             const double negmp = -P.f[MPF_MASS][p];
             const double sxx = P.f[MPF_SP_XX][p] - Pr, syy = P.f[MPF_SP_YY][p] - Pr, sxy = P.f[MPF_SP_XY][p];
-            if (CP) {
-                s_st[threadIdx.x][0] = negmp;
-                s_st[threadIdx.x][1] = sxx;
-                s_st[threadIdx.x][2] = syy;
-                s_st[threadIdx.x][3] = sxy;
-                if (DIM == 3) {
-                    s_st[threadIdx.x][NS - 3] = P.f[MPF_SP_ZZ][p] - Pr;
-                    s_st[threadIdx.x][NS - 2] = P.f[MPF_SP_XZ][p];
-                    s_st[threadIdx.x][NS - 1] = P.f[MPF_SP_YZ][p];
-                }
-            } else {
+            {
                 double xi[3], invd[3], S[3][4], dS[3][4];
                 particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
                                  DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
@@ -1183,7 +1153,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                 RecForce r;
                 {
                     double gr[3];
-                    cpdi_node<DIM>(s_cxi[lq], s_cwg[lq], s_coff[lq], 0, ix - 1, iy - 1, iz - 1, r.wx, gr);
+                    cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
                     r.wy = r.wz = 1.;
                     r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
                 }
@@ -1255,13 +1225,14 @@ template <int DIM, int SH>
 __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
-    constexpr int HALF = NT / 2;
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    // particles per scatter round: half the block, or (lCPDI) as many as the corner tables hold
+    constexpr int SUB = CP ? NT / NCOR : NT / 2;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_n[6][G::FOOT];         // v(3), f/m(3)
-    __shared__ double s_S[CP ? 1 : DIM * 4][HALF + 1];   // [axis*4 + offset][particle]
-    __shared__ double s_cxi[CP ? HALF : 1][8][3];
-    __shared__ int s_coff[CP ? HALF : 1][8][3];
-    __shared__ double s_q[4][HALF + 1];
+    __shared__ double s_S[CP ? 1 : DIM * 4][SUB + 1];   // [axis*4 + offset][particle]
+    __shared__ double s_ct[CP ? SUB : 1][NCOR][DIM][4];
+    __shared__ double s_q[4][SUB + 1];
     __shared__ double s_acc[NW][NQ_MOM][WaveFoot<DIM>::SIZE];
     const int tall = blockIdx.x;              // (field, context tile) pair
     const int fld = tall / g0.tilesTotal;
@@ -1360,16 +1331,25 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
             q4[0] = mp;
             q4[1] = vnew[0]; q4[2] = vnew[1]; q4[3] = vnew[2];
         }
-        for (int h = 0; h < 2; h++) {
-            const int r0 = c0 + h * HALF, r1 = min(r0 + HALF, c1);
+        for (int h = 0; h < NT / SUB; h++) {
+            const int r0 = c0 + h * SUB, r1 = min(r0 + SUB, c1);
             if (r0 >= r1) break;
-            if ((int)(threadIdx.x / HALF) == h && k < c1) {
-                const int i = threadIdx.x - h * HALF;
+            if ((int)(threadIdx.x / SUB) == h && k < c1) {
+                const int i = threadIdx.x - h * SUB;
                 if (CP) {
+                    // the tables are built here, at the thread's turn (an empty asm makes the corner
+                    // coordinates opaque, so the compiler cannot build them once before the rounds
+                    // and hold all of them in registers)
 #pragma unroll
-                    for (int c = 0; c < 8; c++)
+                    for (int c = 0; c < NCOR; c++) {
+                        double xo[3];
 #pragma unroll
-                        for (int a = 0; a < 3; a++) { s_cxi[i][c][a] = cc.xi[c][a]; s_coff[i][c][a] = cc.off[c][a]; }
+                        for (int a = 0; a < 3; a++) {
+                            xo[a] = cc.xi[c][a];
+                            asm volatile("" : "+v"(xo[a]));
+                        }
+                        cpdi_table<DIM>(xo, cc.off[c], s_ct[i][c]);
+                    }
                 } else {
                     // factors with zeros outside the support; the x factor carries mp
 #pragma unroll
@@ -1388,7 +1368,8 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
                 [&](int lq, int ix, int iy, int iz) {
                     RecMass r;
                     if (CP) {
-                        r.wx = cpdi_w<DIM>(s_cxi[lq], s_coff[lq], ix - 1, iy - 1, iz - 1);
+                        double gr[3];
+                        cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
                         r.wy = 1.; r.wz = 1.;
                     } else {
                         r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
@@ -1582,13 +1563,13 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
                                                     int newPositions) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
-    constexpr int CH = CP ? 32 : NT / 2;
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    constexpr int CH = CP ? NT / NCOR : NT / 2;   // lCPDI: a thread per particle corner
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_S[CP ? 1 : CH][DIM][4];
     __shared__ double s_dS[CP ? 1 : CH][DIM][4];
-    __shared__ double s_cxi[CP ? CH : 1][8][3];
-    __shared__ double s_cwg[CP ? CH : 1][8][3];
-    __shared__ int s_coff[CP ? CH : 1][8][3];
+    __shared__ double s_ct[CP ? CH : 1][NCOR][DIM][4];
+    __shared__ double s_cwg[CP ? CH : 1][NCOR][3];
     __shared__ double s_q[CH][5];              // Vp, mp, displacement
     __shared__ double s_acc[NW][NQ_CONTACT][WaveFoot<DIM>::SIZE];
     __shared__ double s_id[DIM][G::TX];
@@ -1609,24 +1590,26 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
             s_id[DIM - 1][i] = 2.0 / ((g.zmin + (double)(oz + i + 1) * g.dz) - (g.zmin + (double)(oz + i) * g.dz));
     }
     zero_acc<DIM, NQ_CONTACT>(s_acc);
+    // staging thread: particle si of the chunk, and (lCPDI) its corner sc; the particle's own
+    // quantities are staged by its first thread
+    const int si = CP ? threadIdx.x / NCOR : threadIdx.x, sc = CP ? threadIdx.x % NCOR : 0;
     for (int c0 = tb; c0 < te; c0 += CH) {
-        int k = c0 + threadIdx.x;
-        if (threadIdx.x < CH && k < te) {
-            int p = B.order[k];
+        const int k = c0 + si;
+        if (si < CH && k < te) {
+            const int p = B.order[k];
             int ei, ej, ek;
             elem_ijk(g, P.elem[p], ei, ej, ek);
             double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
             if (CP) {
-                CpdiCorners cc;
-                cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
+                double r[9];
 #pragma unroll
-                for (int c = 0; c < 8; c++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        s_cxi[threadIdx.x][c][a] = cc.xi[c][a];
-                        s_cwg[threadIdx.x][c][a] = cc.wg[c][a];
-                        s_coff[threadIdx.x][c][a] = cc.off[c][a];
-                    }
+                for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+                double xi[3], w[3];
+                int off[3];
+                cpdi_corner<DIM>(g, pos, r, ei, ej, ek, sc, xi, off);
+                cpdi_table<DIM>(xi, off, s_ct[si][sc]);
+                cpdi_wg_listed<DIM>(r, sc, w);
+                s_cwg[si][sc][0] = w[0]; s_cwg[si][sc][1] = w[1]; s_cwg[si][sc][2] = w[2];
             } else {
                 double xi[3], invd[3], S[3][4], dS[3][4];
                 particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
@@ -1639,29 +1622,31 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
                         s_dS[threadIdx.x][a][o] = dS[a][o];
                     }
             }
-            double mp = P.f[MPF_MASS][p];
-            double rho = mats[P.mat[p]].rho;
-            double Vp;
-            if (DIM == 3) {
-                double Fm[3][3];
-                double ep[6] = {P.f[MPF_EP_XX][p], P.f[MPF_EP_YY][p], P.f[MPF_EP_ZZ][p],
-                                P.f[MPF_EP_YZ][p], P.f[MPF_EP_XZ][p], P.f[MPF_EP_XY][p]};
-                double wr[3] = {P.f[MPF_WR_YZ][p], P.f[MPF_WR_XZ][p], P.f[MPF_WR_XY][p]};
-                get_F3(ep, wr, Fm);
-                Vp = (Fm[0][0] * (Fm[1][1] * Fm[2][2] - Fm[2][1] * Fm[1][2])
-                    - Fm[0][1] * (Fm[1][0] * Fm[2][2] - Fm[2][0] * Fm[1][2])
-                    + Fm[0][2] * (Fm[1][0] * Fm[2][1] - Fm[2][0] * Fm[1][1])) * mp / rho;
-            } else {
-                double exy = P.f[MPF_EP_XY][p], wxy = P.f[MPF_WR_XY][p];
-                double f00 = 1. + P.f[MPF_EP_XX][p], f11 = 1. + P.f[MPF_EP_YY][p];
-                double f01 = 0.5 * (exy - wxy), f10 = 0.5 * (exy + wxy);
-                Vp = (f00 * f11 - f10 * f01) * mp / rho;
-            }
-            s_q[threadIdx.x][0] = Vp;
-            s_q[threadIdx.x][1] = mp;
+            if (sc == 0) {
+                double mp = P.f[MPF_MASS][p];
+                double rho = mats[P.mat[p]].rho;
+                double Vp;
+                if (DIM == 3) {
+                    double Fm[3][3];
+                    double ep[6] = {P.f[MPF_EP_XX][p], P.f[MPF_EP_YY][p], P.f[MPF_EP_ZZ][p],
+                                    P.f[MPF_EP_YZ][p], P.f[MPF_EP_XZ][p], P.f[MPF_EP_XY][p]};
+                    double wr[3] = {P.f[MPF_WR_YZ][p], P.f[MPF_WR_XZ][p], P.f[MPF_WR_XY][p]};
+                    get_F3(ep, wr, Fm);
+                    Vp = (Fm[0][0] * (Fm[1][1] * Fm[2][2] - Fm[2][1] * Fm[1][2])
+                        - Fm[0][1] * (Fm[1][0] * Fm[2][2] - Fm[2][0] * Fm[1][2])
+                        + Fm[0][2] * (Fm[1][0] * Fm[2][1] - Fm[2][0] * Fm[1][1])) * mp / rho;
+                } else {
+                    double exy = P.f[MPF_EP_XY][p], wxy = P.f[MPF_WR_XY][p];
+                    double f00 = 1. + P.f[MPF_EP_XX][p], f11 = 1. + P.f[MPF_EP_YY][p];
+                    double f01 = 0.5 * (exy - wxy), f10 = 0.5 * (exy + wxy);
+                    Vp = (f00 * f11 - f10 * f01) * mp / rho;
+                }
+                s_q[si][0] = Vp;
+                s_q[si][1] = mp;
 #pragma unroll
-            for (int a = 0; a < 3; a++)
-                s_q[threadIdx.x][2 + a] = a < DIM ? (newPositions ? P.posn[a][p] : pos[a]) - P.f[MPF_ORIG_X + a][p] : 0.;
+                for (int a = 0; a < 3; a++)
+                    s_q[si][2 + a] = a < DIM ? (newPositions ? P.posn[a][p] : pos[a]) - P.f[MPF_ORIG_X + a][p] : 0.;
+            }
         }
         __syncthreads();
         wave_cells<DIM, NQ_CONTACT>(s_cs, c0, min(c0 + CH, te), s_acc,
@@ -1669,7 +1654,7 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
                 RecContact r;
                 if (CP) {
                     double gr[3];
-                    cpdi_node<DIM>(s_cxi[lq], s_cwg[lq], s_coff[lq], 0, ix - 1, iy - 1, iz - 1, r.wx, gr);
+                    cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
                     r.wy = r.wz = 1.;
                     r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
                 } else {
