@@ -304,6 +304,11 @@ static void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunPa
     g.uniformLp = 0;
 }
 
+// ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
+static size_t bin_words(long long nk, long long ntall) {
+    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * TILE_SLOT_STRIDE);
+}
+
 // device buffers of the grid, bins and counters for c->kg (nstride nodes, tilesTotal tiles per field)
 static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
     KGrid &g = c->kg;
@@ -346,16 +351,30 @@ static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
     (void)hipMemsetAsync(cntMem, 0, (size_t)nf * nn * sizeof(int), c->stream);
     (void)hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream);
     (void)hipMemcpyAsync(c->dmats, mats, nmat * sizeof(MpmMaterial), hipMemcpyHostToDevice, c->stream);
-    // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums
+    // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
+    // (nfields * tilesTotal) and its length and work counters
     long long nk = c->kb.nkeys;
     long long nblk = (nk + 1 + 1023) / 1024 + 1;
-    if (hipMalloc(&c->bbuf, (size_t)(3 * nk + 2 + nblk) * sizeof(int)) != hipSuccess)
+    c->kb.ntall = nf * g.tilesTotal;
+    const size_t bwords = bin_words(nk, c->kb.ntall);
+    if (hipMalloc(&c->bbuf, bwords * sizeof(int)) != hipSuccess)
         return fail_global("out of device memory allocating the bins");
     c->kb.cellStart = c->bbuf;
     c->kb.count = c->bbuf + nk + 1;
     c->kb.cursor = c->bbuf + 2 * nk + 2;
     c->kb.blockSums = c->bbuf + 3 * nk + 2;
-    (void)hipMemsetAsync(c->bbuf, 0, (size_t)(3 * nk + 2 + nblk) * sizeof(int), c->stream);
+    c->kb.tiles = c->kb.blockSums + nblk;
+    c->kb.ntiles = c->kb.tiles + c->kb.ntall;
+    c->kb.work = c->kb.ntiles + 1;
+    c->kb.slot = 0;
+    (void)hipMemsetAsync(c->bbuf, 0, bwords * sizeof(int), c->stream);
+    {
+        // the tile kernels' grids fill every CU at the kernels' occupancy
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            set_tile_cus(cus);
+    }
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail_global("device initialisation failed");
@@ -410,7 +429,7 @@ int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64
     const long long nk = (long long)g.ntiles * nc;
     int64_t bytes = nodes * (10 * sizeof(double) + sizeof(int));   // mass, pk, vk, ftot, cnt
     bytes += (int64_t)g.ntiles * NQMAX * foot * sizeof(double);    // tile partials
-    bytes += (3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1) * sizeof(int);   // bins
+    bytes += (int64_t)bin_words(nk, g.ntiles) * sizeof(int);            // bins
     bytes += n * PARTICLE_BYTES;
     if (run->shape == MPMGPU_SHAPE_LCPDI) bytes += n * 9 * (int64_t)sizeof(double);
     return bytes;
@@ -744,8 +763,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     if (c->kp.n == 0 || c->profiling) return c->fail("nothing to tune");
     size_t n = (size_t)c->ncap;
     size_t pb = PROWS * n * sizeof(double), ib = IROWS * n * sizeof(int);
-    long long nk = c->kb.nkeys;
-    size_t bb = (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1) * sizeof(int);
+    size_t bb = bin_words(c->kb.nkeys, c->kb.ntall) * sizeof(int);
     char *save = nullptr;
     HIPCHECK(c, hipMalloc(&save, pb + ib + bb + 3 * sizeof(unsigned long long)));
     HIPCHECK(c, hipMemcpyAsync(save, c->pbuf, pb, hipMemcpyDeviceToDevice, c->stream));

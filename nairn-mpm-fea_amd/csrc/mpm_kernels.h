@@ -72,8 +72,18 @@ struct KBins {
     int *blockSums;            // scan scratch
     const int *home;           // particle -> its index at upload: the order inside a cell
     int *shome;                // sorted slot -> upload index of its particle
+    int *tiles;                // the non-empty (field, tile) pairs of the binning, in no fixed order
+    int *ntiles;               // length of tiles
+    int *work;                 // lCPDI tile kernels' list cursors, one per launch of a step, 32 ints apart
     int nkeys;
+    int ntall;                 // (field, tile) pairs: nfields * tilesTotal
+    int slot;                  // the launch's cursor (set by the launch functions)
 };
+
+// cursors of the lCPDI tile kernels of one step (cleared when the tile list is built)
+enum { TILE_SLOT_MASS, TILE_SLOT_CONTACT_FIRST, TILE_SLOT_USF, TILE_SLOT_FORCE, TILE_SLOT_G2P,
+       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOTS };
+#define TILE_SLOT_STRIDE 32
 
 struct KNodes {
     double *mass;              // nnodes
@@ -102,6 +112,7 @@ enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
 enum { CONTACT_MASS_MOMENTUM = 0, CONTACT_UPDATE_MOMENTUM = 1, CONTACT_UPDATE_STRAINS_LAST = 2 };
 
 // host-side launchers (mpm_kernels.hip); all take the context's stream
+void set_tile_cus(int cus);        // compute units the tile kernels' grids are sized for
 void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
 void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,

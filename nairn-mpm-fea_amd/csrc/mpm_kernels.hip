@@ -298,34 +298,40 @@ __device__ __forceinline__ void cpdi_table(const double xi[3], const int off[3],
             T[a][o] = o == off[a] + 1 ? 1. - xi[a] : (o == off[a] + 2 ? 1. + xi[a] : 0.);
 }
 
-// corner function of the lane's node from a corner table: cpdi_cfn of the node in the corner's
-// element (same products in the same order), zero outside it or below 1e-15 (GetCPDIFunctions)
-template <int DIM>
-__device__ __forceinline__ double cpdi_table_cfn(const double (*T)[4], int o1x, int o1y, int o1z) {
-    const double cfn = DIM == 3 ? 0.125 * T[0][o1x] * T[1][o1y] * T[2][o1z] : 0.25 * T[0][o1x] * T[1][o1y];
-    return cfn < 1e-15 ? 0. : cfn;
-}
-
-// merged weight (and gradient) of the lane's node over the corners of one staged particle, summed
+// Merged weight (and gradient) of the lane's node over the corners of one staged particle, summed
 // in corner order; corners whose element lacks the node or whose function is below 1e-15 add
-// nothing (GetCPDIFunctions, MoreMPMElementBase.cpp:581-656)
+// nothing (GetCPDIFunctions, MoreMPMElementBase.cpp:581-656). The corner function 1/8 t1 t2 t3
+// (1/4 t1 t2 in 2D) is formed without its power-of-two factor, which is exact to move: the
+// threshold is scaled by 8 (4), the weight sum by 1/8 (1/4) once at the end, and the staged
+// gradient weights carry it (cpdi_wg_staged). The weight is then the same to the last bit.
 template <int DIM, bool GRAD>
 __device__ __forceinline__ void cpdi_table_node(const double (*T)[DIM][4], const double (*wg)[3], int o1x, int o1y,
                                                 int o1z, double &w, double gr[3]) {
     constexpr int NCOR = DIM == 3 ? 8 : 4;
-    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
-    w = 0.;
+    constexpr double thr = (DIM == 3 ? 8. : 4.) * 1e-15;
+    constexpr double wscale = DIM == 3 ? 0.125 * 0.125 : 0.25 * 0.25;   // ws times the corner factor
+    double sum = 0.;
     if (GRAD) gr[0] = gr[1] = gr[2] = 0.;
 #pragma unroll
     for (int c = 0; c < NCOR; c++) {
-        const double cfn = cpdi_table_cfn<DIM>(T[c], o1x, o1y, o1z);
-        w += ws * cfn;
+        double f = DIM == 3 ? T[c][0][o1x] * T[c][1][o1y] * T[c][2][o1z] : T[c][0][o1x] * T[c][1][o1y];
+        f = f < thr ? 0. : f;
+        sum = c == 0 ? f : sum + f;
         if (GRAD) {
-            gr[0] += wg[c][0] * cfn;
-            gr[1] += wg[c][1] * cfn;
-            if (DIM == 3) gr[2] += wg[c][2] * cfn;
+            gr[0] = fma(wg[c][0], f, gr[0]);
+            gr[1] = fma(wg[c][1], f, gr[1]);
+            if (DIM == 3) gr[2] = fma(wg[c][2], f, gr[2]);
         }
     }
+    w = wscale * sum;
+}
+
+// gradient weight of corner c as the scatters stage it: times the corner function's factor
+template <int DIM>
+__device__ __forceinline__ void cpdi_wg_staged(const double r[9], int c, double w[3]) {
+    cpdi_wg_listed<DIM>(r, c, w);
+#pragma unroll
+    for (int a = 0; a < 3; a++) w[a] *= DIM == 3 ? 0.125 : 0.25;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -399,8 +405,9 @@ __global__ __launch_bounds__(NT) void k_scan_blocks(const int *in, int *out, int
     if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(NT) void k_scan_sums(int *blockSums, int nb) {
+__global__ __launch_bounds__(NT) void k_scan_sums(int *blockSums, int nb, int *ntiles) {
     __shared__ int s_tmp[NT];
+    if (threadIdx.x == 0) ntiles[0] = 0;      // the tile list is rebuilt after the scan
     int carry = 0;
     for (int b0 = 0; b0 < nb; b0 += NT) {
         int i = b0 + threadIdx.x;
@@ -463,6 +470,50 @@ __global__ void k_bin_sort_cells(KBins B) {
     B.count[c] = 0;
 }
 
+// list of the non-empty (field, tile) pairs; clears the lCPDI kernels' cursors
+__global__ void k_tile_list(KBins B, int nc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < TILE_SLOTS) B.work[threadIdx.x * TILE_SLOT_STRIDE] = 0;
+    const bool on = t < B.ntall && B.cellStart[(size_t)(t + 1) * nc] > B.cellStart[(size_t)t * nc];
+    const unsigned long long m = __ballot(on);
+    if (m == 0ull) return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(B.ntiles, __popcll(m));
+    base = __shfl(base, leader);
+    if (on) B.tiles[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+}
+
+// A tile kernel is launched with as many workgroups as the device holds at once, and its blocks
+// walk the list of non-empty tiles, so no workgroup is spent on an empty tile. Block b takes entry
+// b first. Then, with uniform per-particle work (shape-function kernels), it takes b + gridDim.x,
+// b + 2 gridDim.x, ... (no counter shared between blocks); with lCPDI, whose tiles are few and
+// heavy, it takes the next unclaimed entry from the launch's cursor, claimed while its current
+// tile runs. Results do not depend on which block takes a tile.
+template <bool DYN, typename F>
+__device__ __forceinline__ void for_each_tile(const KBins &B, F &tile) {
+    const int n = B.ntiles[0];
+    if (!DYN) {
+        for (int li = blockIdx.x; li < n; li += gridDim.x) {
+            tile(B.tiles[li]);
+            __syncthreads();
+        }
+        return;
+    }
+    __shared__ int s_li;
+    int li = blockIdx.x;
+    while (li < n) {
+        int next = 0;
+        if (threadIdx.x == 0) next = (int)gridDim.x + atomicAdd(&B.work[B.slot * TILE_SLOT_STRIDE], 1);
+        tile(B.tiles[li]);
+        if (threadIdx.x == 0) s_li = next;
+        __syncthreads();
+        li = s_li;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Deterministic per-wave scatter. Wave w of the block takes the tile's cells w, w+NW, ... in
 // order; for one cell, lane l owns the candidate node offset (l%4-1, (l/4)%4-1, l/16-1) (2D: the
@@ -523,7 +574,7 @@ __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
 
 // particle records of the scatters: per-axis factors of the lane's candidate and the payload
 struct RecMass { double wx, wy, wz, mp, vx, vy, vz; };
-struct RecForce { double wx, wy, wz, dx, dy, dz, negmp, s0, s1, s2, s3, s4, s5; };
+struct RecForce { double wx, wy, wz, dx, dy, dz, s0, s1, s2, s3, s4, s5; };
 struct RecForce2 { double2 x, y, z; double a[6]; };
 
 template <int DIM, int NQ>
@@ -579,124 +630,126 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
     __shared__ double s_ct[CP ? CH : 1][NCOR][DIM][4];
     __shared__ double s_q[4][CH + 1];
     __shared__ double s_acc[NW][NQ_MASS][WaveFoot<DIM>::SIZE];
-    const int tall = blockIdx.x;              // (field, context tile) pair
-    const KGrid g = tile_grid(g0, tall % g0.tilesTotal);
-    load_tile_starts<DIM>(B, tall, s_cs);
-    const int tb = s_cs[0], te = s_cs[G::NC];
-    if (tb == te) return;
-    zero_acc<DIM, NQ_MASS>(s_acc);
-    auto scatter = [&](int c0, int c1) {
-        wave_cells<DIM, NQ_MASS>(s_cs, c0, c1, s_acc,
-            [&](int lq, int ix, int iy, int iz) {
-                RecMass r;
-                if (CP) {
-                    double gr[3];
-                    cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
-                    r.wy = 1.; r.wz = 1.;
-                } else {
-                    r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
+    auto tile = [&](const int tall) {      // (field, context tile) pair
+        const KGrid g = tile_grid(g0, tall % g0.tilesTotal);
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        zero_acc<DIM, NQ_MASS>(s_acc);
+        auto scatter = [&](int c0, int c1) {
+            wave_cells<DIM, NQ_MASS>(s_cs, c0, c1, s_acc,
+                [&](int lq, int ix, int iy, int iz) {
+                    RecMass r;
+                    if (CP) {
+                        double gr[3];
+                        cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
+                        r.wy = 1.; r.wz = 1.;
+                    } else {
+                        r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
+                    }
+                    r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
+                    return r;
+                },
+                [&](const RecMass &r, double *acc, int, int, int) {
+                    // lCPDI: merged weight (zero for non-member nodes) times mp; otherwise the
+                    // product of the staged factors, mp included
+                    const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
+                    acc[0] += fnmp;
+                    acc[1] = fma(fnmp, r.vx, acc[1]);
+                    acc[2] = fma(fnmp, r.vy, acc[2]);
+                    acc[3] = fma(fnmp, r.vz, acc[3]);
+                });
+        };
+        if (!CP) {
+            // Rounds of 2*NT slots: every thread first loads the particle words of its two slots of
+            // the round (tb + t and tb + t + NT), so the round's global loads overlap; then the four
+            // chunks of CH slots are staged from registers by the half of the block that holds them.
+            for (int r0 = tb; r0 < te; r0 += 2 * NT) {
+                int rp[2], re[2];
+                double rx[2][3], rm[2], rv[2][3];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const int k = r0 + threadIdx.x + j * NT;
+                    rp[j] = k < te ? B.order[k] : -1;
                 }
-                r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
-                return r;
-            },
-            [&](const RecMass &r, double *acc, int, int, int) {
-                // lCPDI: merged weight (zero for non-member nodes) times mp; otherwise the
-                // product of the staged factors, mp included
-                const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
-                acc[0] += fnmp;
-                acc[1] = fma(fnmp, r.vx, acc[1]);
-                acc[2] = fma(fnmp, r.vy, acc[2]);
-                acc[3] = fma(fnmp, r.vz, acc[3]);
-            });
-    };
-    if (!CP) {
-        // Rounds of 2*NT slots: every thread first loads the particle words of its two slots of
-        // the round (tb + t and tb + t + NT), so the round's global loads overlap; then the four
-        // chunks of CH slots are staged from registers by the half of the block that holds them.
-        for (int r0 = tb; r0 < te; r0 += 2 * NT) {
-            int rp[2], re[2];
-            double rx[2][3], rm[2], rv[2][3];
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int k = r0 + threadIdx.x + j * NT;
-                rp[j] = k < te ? B.order[k] : -1;
-            }
+                for (int j = 0; j < 2; j++) {
+                    const int p = rp[j];
+                    if (p < 0) continue;
+                    re[j] = P.elem[p];
 #pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int p = rp[j];
-                if (p < 0) continue;
-                re[j] = P.elem[p];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    rx[j][a] = a < DIM ? P.f[MPF_POS_X + a][p] : 0.;
-                    rv[j][a] = a < DIM ? P.f[MPF_VEL_X + a][p] : 0.;
+                    for (int a = 0; a < 3; a++) {
+                        rx[j][a] = a < DIM ? P.f[MPF_POS_X + a][p] : 0.;
+                        rv[j][a] = a < DIM ? P.f[MPF_VEL_X + a][p] : 0.;
+                    }
+                    rm[j] = P.f[MPF_MASS][p];
                 }
-                rm[j] = P.f[MPF_MASS][p];
-            }
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int c0 = r0 + c * CH;
-                if (c0 >= te) break;
-                const int j = c >> 1;
-                if ((int)(threadIdx.x / CH) == (c & 1) && rp[j] >= 0) {
-                    const int i = threadIdx.x - (c & 1) * CH;
+                for (int c = 0; c < 4; c++) {
+                    const int c0 = r0 + c * CH;
+                    if (c0 >= te) break;
+                    const int j = c >> 1;
+                    if ((int)(threadIdx.x / CH) == (c & 1) && rp[j] >= 0) {
+                        const int i = threadIdx.x - (c & 1) * CH;
+                        int ei, ej, ek;
+                        elem_ijk(g, re[j], ei, ej, ek);
+                        // factors with zeros outside the support; the x factor carries mp
+                        double xi[3], invd[3], S[3][4], dS[3][4];
+                        particle_xi<DIM>(g, ei, ej, ek, rx[j][0], rx[j][1], rx[j][2], xi, invd);
+                        particle_factors<DIM>(g, P, rp[j], xi, S, dS);
+#pragma unroll
+                        for (int a = 0; a < DIM; a++)
+#pragma unroll
+                            for (int o = 0; o < 4; o++) {
+                                const double v = S[a][o] >= 0. ? S[a][o] : 0.;
+                                s_S[a * 4 + o][i] = a == 0 ? v * rm[j] : v;
+                            }
+                        s_q[0][i] = rm[j];
+                        s_q[1][i] = rv[j][0]; s_q[2][i] = rv[j][1]; s_q[3][i] = rv[j][2];
+                    }
+                    __syncthreads();
+                    scatter(c0, min(c0 + CH, te));
+                    __syncthreads();
+                }
+            }
+        } else {
+            const int i = threadIdx.x / NCOR, c = threadIdx.x % NCOR;
+            for (int c0 = tb; c0 < te; c0 += CH) {
+                const int k = c0 + i;
+                bool ok = true;
+                if (k < te) {
+                    const int p = B.order[k];
                     int ei, ej, ek;
-                    elem_ijk(g, re[j], ei, ej, ek);
-                    // factors with zeros outside the support; the x factor carries mp
-                    double xi[3], invd[3], S[3][4], dS[3][4];
-                    particle_xi<DIM>(g, ei, ej, ek, rx[j][0], rx[j][1], rx[j][2], xi, invd);
-                    particle_factors<DIM>(g, P, rp[j], xi, S, dS);
+                    elem_ijk(g, P.elem[p], ei, ej, ek);
+                    double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+                    // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step (each
+                    // corner's thread builds them; the first keeps them for the step's other passes)
+                    double r[9];
+                    cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);
+                    double xi[3];
+                    int off[3];
+                    ok = cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
+                    cpdi_table<DIM>(xi, off, s_ct[i][c]);
+                    if (c == 0) {
 #pragma unroll
-                    for (int a = 0; a < DIM; a++)
-#pragma unroll
-                        for (int o = 0; o < 4; o++) {
-                            const double v = S[a][o] >= 0. ? S[a][o] : 0.;
-                            s_S[a * 4 + o][i] = a == 0 ? v * rm[j] : v;
-                        }
-                    s_q[0][i] = rm[j];
-                    s_q[1][i] = rv[j][0]; s_q[2][i] = rv[j][1]; s_q[3][i] = rv[j][2];
+                        for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
+                        s_q[0][i] = P.f[MPF_MASS][p];
+                        s_q[1][i] = P.f[MPF_VEL_X][p];
+                        s_q[2][i] = P.f[MPF_VEL_Y][p];
+                        s_q[3][i] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
+                    }
                 }
+                // one count per particle with an unusable corner (its corner threads are adjacent lanes)
+                const unsigned long long bad = __ballot(!ok);
+                if (c == 0 && ((bad >> (threadIdx.x & 63)) & ((1ull << NCOR) - 1ull))) atomicAdd(st.cpdiErr, 1ull);
                 __syncthreads();
                 scatter(c0, min(c0 + CH, te));
                 __syncthreads();
             }
         }
-    } else {
-        const int i = threadIdx.x / NCOR, c = threadIdx.x % NCOR;
-        for (int c0 = tb; c0 < te; c0 += CH) {
-            const int k = c0 + i;
-            bool ok = true;
-            if (k < te) {
-                const int p = B.order[k];
-                int ei, ej, ek;
-                elem_ijk(g, P.elem[p], ei, ej, ek);
-                double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-                // InitializationTask -> GetCPDINodesAndWeights: semi-side vectors of the step (each
-                // corner's thread builds them; the first keeps them for the step's other passes)
-                double r[9];
-                cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);
-                double xi[3];
-                int off[3];
-                ok = cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
-                cpdi_table<DIM>(xi, off, s_ct[i][c]);
-                if (c == 0) {
-#pragma unroll
-                    for (int a = 0; a < 9; a++) P.cpr[a][p] = r[a];
-                    s_q[0][i] = P.f[MPF_MASS][p];
-                    s_q[1][i] = P.f[MPF_VEL_X][p];
-                    s_q[2][i] = P.f[MPF_VEL_Y][p];
-                    s_q[3][i] = DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.;
-                }
-            }
-            // one count per particle with an unusable corner (its corner threads are adjacent lanes)
-            const unsigned long long bad = __ballot(!ok);
-            if (c == 0 && ((bad >> (threadIdx.x & 63)) & ((1ull << NCOR) - 1ull))) atomicAdd(st.cpdiErr, 1ull);
-            __syncthreads();
-            scatter(c0, min(c0 + CH, te));
-            __syncthreads();
-        }
-    }
-    write_partials<DIM, NQ_MASS>(N, tall, s_acc);
+        write_partials<DIM, NQ_MASS>(N, tall, s_acc);
+    };
+    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
 }
 
 // sum the partials of the tiles that cover node (i,j,k), in fixed tile order
@@ -994,36 +1047,38 @@ __global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNode
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    const int tall = blockIdx.x;              // (field, context tile) pair
-    const int fld = tall / g0.tilesTotal;
-    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
-    const int t = tall - fld * g0.tilesTotal - g.tileBase;
-    const KNodes Nf = field_view(g, N, fld);
-    load_tile_starts<DIM>(B, tall, s_cs);
-    const int tb = s_cs[0], te = s_cs[G::NC];
-    if (tb == te) return;
-    int ox, oy, oz;
-    tile_origin<DIM>(g, t, ox, oy, oz);
-    {
-        const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
-        const long long st[3] = {0, g.nstride, 2 * g.nstride};
-        load_foot<DIM, 3>(g, ox, oy, oz, src, st, s_v);
-    }
-    __syncthreads();
-    for (int k = tb + threadIdx.x; k < te; k += NT) {
-        int p = B.order[k];
-        int ei, ej, ek;
-        elem_ijk(g, P.elem[p], ei, ej, ek);
-        double L[3][3];
-        particle_gradient<DIM, SH>(g, P, p, ei, ej, ek, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
-        scale_gradient<DIM>(L, g.strainFirst);
-        const MpmMaterial &m = mats[P.mat[p]];
-        const int mtype = MAT ? MAT : m.type;
-        PState s;
-        load_state<DIM>(P, p, mtype, s);
-        constitutive<DIM, MAT>(L, m, s);
-        store_state<DIM>(P, p, mtype, s);
-    }
+    auto tile = [&](const int tall) {      // (field, context tile) pair
+        const int fld = tall / g0.tilesTotal;
+        const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+        const int t = tall - fld * g0.tilesTotal - g.tileBase;
+        const KNodes Nf = field_view(g, N, fld);
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        int ox, oy, oz;
+        tile_origin<DIM>(g, t, ox, oy, oz);
+        {
+            const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
+            const long long st[3] = {0, g.nstride, 2 * g.nstride};
+            load_foot<DIM, 3>(g, ox, oy, oz, src, st, s_v);
+        }
+        __syncthreads();
+        for (int k = tb + threadIdx.x; k < te; k += NT) {
+            int p = B.order[k];
+            int ei, ej, ek;
+            elem_ijk(g, P.elem[p], ei, ej, ek);
+            double L[3][3];
+            particle_gradient<DIM, SH>(g, P, p, ei, ej, ek, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+            scale_gradient<DIM>(L, g.strainFirst);
+            const MpmMaterial &m = mats[P.mat[p]];
+            const int mtype = MAT ? MAT : m.type;
+            PState s;
+            load_state<DIM>(P, p, mtype, s);
+            constitutive<DIM, MAT>(L, m, s);
+            store_state<DIM>(P, p, mtype, s);
+        }
+    };
+    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
 }
 
 // internal-force scatter (GridForcesTask::Execute, GridForcesTask.cpp:40-153 ->
@@ -1033,7 +1088,6 @@ template <int DIM, int SH>
 __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
-    constexpr int NS = DIM == 3 ? 7 : 4;      // staged stress terms
     constexpr int NCOR = DIM == 3 ? 8 : 4;
     constexpr int CH = CP ? NT / NCOR : NT / 2;   // particles staged per chunk (lCPDI: a thread per corner)
     constexpr int NA = DIM == 3 ? 6 : 3;      // -mp times the stress terms (shape-function path)
@@ -1044,141 +1098,140 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     __shared__ double s_a[CP ? 1 : CH][NA];
     __shared__ double s_ct[CP ? CH : 1][NCOR][DIM][4];
     __shared__ double s_cwg[CP ? CH : 1][NCOR][3];
-    __shared__ double s_st[CP ? CH : 1][NS];
+    __shared__ double s_st[CP ? CH : 1][NA];
     __shared__ double s_acc[NW][NQ_FORCE][WaveFoot<DIM>::SIZE];
-    const int tall = blockIdx.x;              // (field, context tile) pair
-    const int fld = tall / g0.tilesTotal;
-    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
-    load_tile_starts<DIM>(B, tall, s_cs);
-    const int tb = s_cs[0], te = s_cs[G::NC];
-    if (tb == te) return;
-    zero_acc<DIM, NQ_FORCE>(s_acc);
-    for (int c0 = tb; c0 < te; c0 += CH) {
-        if (CP) {
-            // thread (i, c): corner c of staged particle i; the first corner's thread stages the stress
-            const int i = threadIdx.x / NCOR, c = threadIdx.x % NCOR;
-            const int k = c0 + i;
-            if (k < te) {
-                const int p = B.order[k];
+    auto tile = [&](const int tall) {      // (field, context tile) pair
+        const int fld = tall / g0.tilesTotal;
+        const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        zero_acc<DIM, NQ_FORCE>(s_acc);
+        for (int c0 = tb; c0 < te; c0 += CH) {
+            if (CP) {
+                // thread (i, c): corner c of staged particle i; the first corner's thread stages the stress
+                const int i = threadIdx.x / NCOR, c = threadIdx.x % NCOR;
+                const int k = c0 + i;
+                if (k < te) {
+                    const int p = B.order[k];
+                    int ei, ej, ek;
+                    elem_ijk(g, P.elem[p], ei, ej, ek);
+                    double r[9];
+#pragma unroll
+                    for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+                    double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+                    double xi[3], w[3];
+                    int off[3];
+                    cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
+                    cpdi_table<DIM>(xi, off, s_ct[i][c]);
+                    cpdi_wg_staged<DIM>(r, c, w);
+                    s_cwg[i][c][0] = w[0]; s_cwg[i][c][1] = w[1]; s_cwg[i][c][2] = w[2];
+                    if (c == 0) {
+                        const double Pr = P.f[MPF_PRESSURE][p];
+                        const double negmp = -P.f[MPF_MASS][p];
+                        s_st[i][0] = negmp * (P.f[MPF_SP_XX][p] - Pr);
+                        s_st[i][1] = negmp * (P.f[MPF_SP_YY][p] - Pr);
+                        s_st[i][2] = negmp * P.f[MPF_SP_XY][p];
+                        if (DIM == 3) {
+                            s_st[i][3] = negmp * (P.f[MPF_SP_ZZ][p] - Pr);
+                            s_st[i][4] = negmp * P.f[MPF_SP_XZ][p];
+                            s_st[i][5] = negmp * P.f[MPF_SP_YZ][p];
+                        }
+                    }
+                }
+            }
+            int k = c0 + threadIdx.x;
+            if (!CP && threadIdx.x < CH && k < te) {
+                int p = B.order[k];
                 int ei, ej, ek;
                 elem_ijk(g, P.elem[p], ei, ej, ek);
-                double r[9];
+                double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
+                const double negmp = -P.f[MPF_MASS][p];
+                const double sxx = P.f[MPF_SP_XX][p] - Pr, syy = P.f[MPF_SP_YY][p] - Pr, sxy = P.f[MPF_SP_XY][p];
+                {
+                    double xi[3], invd[3], S[3][4], dS[3][4];
+                    particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
+                                     DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+                    particle_factors<DIM>(g, P, p, xi, S, dS);
 #pragma unroll
-                for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
-                double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-                double xi[3], w[3];
-                int off[3];
-                cpdi_corner<DIM>(g, pos, r, ei, ej, ek, c, xi, off);
-                cpdi_table<DIM>(xi, off, s_ct[i][c]);
-                cpdi_wg_listed<DIM>(r, c, w);
-                s_cwg[i][c][0] = w[0]; s_cwg[i][c][1] = w[1]; s_cwg[i][c][2] = w[2];
-                if (c == 0) {
-                    const double Pr = P.f[MPF_PRESSURE][p];
-                    s_st[i][0] = -P.f[MPF_MASS][p];
-                    s_st[i][1] = P.f[MPF_SP_XX][p] - Pr;
-                    s_st[i][2] = P.f[MPF_SP_YY][p] - Pr;
-                    s_st[i][3] = P.f[MPF_SP_XY][p];
+                    for (int a = 0; a < DIM; a++)
+#pragma unroll
+                        for (int o = 0; o < 4; o++) {
+                            const bool in = S[a][o] >= 0.;
+                            s_F[threadIdx.x][a][o] = make_double2(in ? S[a][o] : 0., in ? dS[a][o] * invd[a] : 0.);
+                        }
+                    s_a[threadIdx.x][0] = negmp * sxx;
+                    s_a[threadIdx.x][1] = negmp * syy;
+                    s_a[threadIdx.x][2] = negmp * sxy;
                     if (DIM == 3) {
-                        s_st[i][NS - 3] = P.f[MPF_SP_ZZ][p] - Pr;
-                        s_st[i][NS - 2] = P.f[MPF_SP_XZ][p];
-                        s_st[i][NS - 1] = P.f[MPF_SP_YZ][p];
+                        s_a[threadIdx.x][3] = negmp * (P.f[MPF_SP_ZZ][p] - Pr);
+                        s_a[threadIdx.x][4] = negmp * P.f[MPF_SP_XZ][p];
+                        s_a[threadIdx.x][5] = negmp * P.f[MPF_SP_YZ][p];
                     }
                 }
             }
-        }
-        int k = c0 + threadIdx.x;
-        if (!CP && threadIdx.x < CH && k < te) {
-            int p = B.order[k];
-            int ei, ej, ek;
-            elem_ijk(g, P.elem[p], ei, ej, ek);
-            double Pr = P.f[MPF_PRESSURE][p];      // stays 0 for small-strain elastic particles
-            const double negmp = -P.f[MPF_MASS][p];
-            const double sxx = P.f[MPF_SP_XX][p] - Pr, syy = P.f[MPF_SP_YY][p] - Pr, sxy = P.f[MPF_SP_XY][p];
-            {
-                double xi[3], invd[3], S[3][4], dS[3][4];
-                particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p],
-                                 DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
-                particle_factors<DIM>(g, P, p, xi, S, dS);
+            __syncthreads();
+            if (!CP) {
+                // f = -mp (sigma - P) . grad N; grad N from the staged factor pairs, accumulated with FMA
+                wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
+                    [&](int lq, int ix, int iy, int iz) {
+                        RecForce2 r;
+                        r.x = s_F[lq][0][ix];
+                        r.y = s_F[lq][1][iy];
+                        r.z = DIM == 3 ? s_F[lq][DIM - 1][iz] : make_double2(1., 0.);
 #pragma unroll
-                for (int a = 0; a < DIM; a++)
-#pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        const bool in = S[a][o] >= 0.;
-                        s_F[threadIdx.x][a][o] = make_double2(in ? S[a][o] : 0., in ? dS[a][o] * invd[a] : 0.);
-                    }
-                s_a[threadIdx.x][0] = negmp * sxx;
-                s_a[threadIdx.x][1] = negmp * syy;
-                s_a[threadIdx.x][2] = negmp * sxy;
-                if (DIM == 3) {
-                    s_a[threadIdx.x][3] = negmp * (P.f[MPF_SP_ZZ][p] - Pr);
-                    s_a[threadIdx.x][4] = negmp * P.f[MPF_SP_XZ][p];
-                    s_a[threadIdx.x][5] = negmp * P.f[MPF_SP_YZ][p];
-                }
+                        for (int q = 0; q < NA; q++) r.a[q] = s_a[lq][q];
+                        return r;
+                    },
+                    [&](const RecForce2 &r, double *acc, int, int, int) {
+                        if (DIM == 3) {
+                            const double gx = r.x.y * (r.y.x * r.z.x);
+                            const double gy = r.y.y * (r.x.x * r.z.x);
+                            const double gz = r.z.y * (r.x.x * r.y.x);
+                            // a: xx, yy, xy, zz, xz, yz
+                            acc[0] = fma(r.a[0], gx, fma(r.a[2], gy, fma(r.a[4], gz, acc[0])));
+                            acc[1] = fma(r.a[2], gx, fma(r.a[1], gy, fma(r.a[5], gz, acc[1])));
+                            acc[2] = fma(r.a[4], gx, fma(r.a[5], gy, fma(r.a[3], gz, acc[2])));
+                        } else {
+                            const double gx = r.x.y * r.y.x;
+                            const double gy = r.y.y * r.x.x;
+                            acc[0] = fma(r.a[0], gx, fma(r.a[2], gy, acc[0]));
+                            acc[1] = fma(r.a[2], gx, fma(r.a[1], gy, acc[1]));
+                        }
+                    });
+                __syncthreads();
+                continue;
             }
-        }
-        __syncthreads();
-        if (!CP) {
-            // f = -mp (sigma - P) . grad N; grad N from the staged factor pairs, accumulated with FMA
             wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
                 [&](int lq, int ix, int iy, int iz) {
-                    RecForce2 r;
-                    r.x = s_F[lq][0][ix];
-                    r.y = s_F[lq][1][iy];
-                    r.z = DIM == 3 ? s_F[lq][DIM - 1][iz] : make_double2(1., 0.);
-#pragma unroll
-                    for (int q = 0; q < NA; q++) r.a[q] = s_a[lq][q];
+                    RecForce r;
+                    {
+                        double gr[3];
+                        cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
+                        r.wy = r.wz = 1.;
+                        r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
+                    }
+                    r.s0 = s_st[lq][0]; r.s1 = s_st[lq][1]; r.s2 = s_st[lq][2];
+                    if (DIM == 3) { r.s3 = s_st[lq][3]; r.s4 = s_st[lq][4]; r.s5 = s_st[lq][5]; }
+                    else { r.s3 = r.s4 = r.s5 = 0.; }
                     return r;
                 },
-                [&](const RecForce2 &r, double *acc, int, int, int) {
+                [&](const RecForce &r, double *acc, int, int, int) {
+                    // s: -mp times xx, yy, xy, zz, xz, yz of sigma - P; non-member nodes have zero gradient
                     if (DIM == 3) {
-                        const double gx = r.x.y * (r.y.x * r.z.x);
-                        const double gy = r.y.y * (r.x.x * r.z.x);
-                        const double gz = r.z.y * (r.x.x * r.y.x);
-                        // a: xx, yy, xy, zz, xz, yz
-                        acc[0] = fma(r.a[0], gx, fma(r.a[2], gy, fma(r.a[4], gz, acc[0])));
-                        acc[1] = fma(r.a[2], gx, fma(r.a[1], gy, fma(r.a[5], gz, acc[1])));
-                        acc[2] = fma(r.a[4], gx, fma(r.a[5], gy, fma(r.a[3], gz, acc[2])));
+                        acc[0] = fma(r.s0, r.dx, fma(r.s2, r.dy, fma(r.s4, r.dz, acc[0])));
+                        acc[1] = fma(r.s2, r.dx, fma(r.s1, r.dy, fma(r.s5, r.dz, acc[1])));
+                        acc[2] = fma(r.s4, r.dx, fma(r.s5, r.dy, fma(r.s3, r.dz, acc[2])));
                     } else {
-                        const double gx = r.x.y * r.y.x;
-                        const double gy = r.y.y * r.x.x;
-                        acc[0] = fma(r.a[0], gx, fma(r.a[2], gy, acc[0]));
-                        acc[1] = fma(r.a[2], gx, fma(r.a[1], gy, acc[1]));
+                        acc[0] = fma(r.s0, r.dx, fma(r.s2, r.dy, acc[0]));
+                        acc[1] = fma(r.s2, r.dx, fma(r.s1, r.dy, acc[1]));
                     }
                 });
             __syncthreads();
-            continue;
         }
-        wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
-            [&](int lq, int ix, int iy, int iz) {
-                RecForce r;
-                {
-                    double gr[3];
-                    cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
-                    r.wy = r.wz = 1.;
-                    r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
-                }
-                r.negmp = s_st[lq][0]; r.s0 = s_st[lq][1]; r.s1 = s_st[lq][2]; r.s2 = s_st[lq][3];
-                if (DIM == 3) { r.s3 = s_st[lq][NS - 3]; r.s4 = s_st[lq][NS - 2]; r.s5 = s_st[lq][NS - 1]; }
-                else { r.s3 = r.s4 = r.s5 = 0.; }
-                return r;
-            },
-            [&](const RecForce &r, double *acc, int, int, int) {
-                const double negmp = r.negmp;        // lCPDI: non-member nodes have zero gradient
-                if (DIM == 3) {
-                    double gx = r.dx, gy = r.dy, gz = r.dz;
-                    // s0 = sxx-P, s1 = syy-P, s2 = sxy, s3 = szz-P, s4 = sxz, s5 = syz
-                    acc[0] += negmp * (r.s0 * gx + r.s2 * gy + r.s4 * gz);
-                    acc[1] += negmp * (r.s2 * gx + r.s1 * gy + r.s5 * gz);
-                    acc[2] += negmp * (r.s4 * gx + r.s5 * gy + r.s3 * gz);
-                } else {
-                    double gx = r.dx, gy = r.dy;
-                    acc[0] += negmp * (r.s0 * gx + r.s2 * gy);
-                    acc[1] += negmp * (r.s2 * gx + r.s1 * gy);
-                }
-            });
-        __syncthreads();
-    }
-    write_partials<DIM, NQ_FORCE>(N, tall, s_acc);
+        write_partials<DIM, NQ_FORCE>(N, tall, s_acc);
+    };
+    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
 }
 
 // node pass after the forces: ftot, restore p (identity without BCs), p += f dt, v = p/m
@@ -1234,159 +1287,161 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
     __shared__ double s_ct[CP ? SUB : 1][NCOR][DIM][4];
     __shared__ double s_q[4][SUB + 1];
     __shared__ double s_acc[NW][NQ_MOM][WaveFoot<DIM>::SIZE];
-    const int tall = blockIdx.x;              // (field, context tile) pair
-    const int fld = tall / g0.tilesTotal;
-    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
-    const int t = tall - fld * g0.tilesTotal - g.tileBase;
-    const KNodes Nf = field_view(g, N, fld);
-    load_tile_starts<DIM>(B, tall, s_cs);
-    const int tb = s_cs[0], te = s_cs[G::NC];
-    if (tb == te) return;
-    int ox, oy, oz;
-    tile_origin<DIM>(g, t, ox, oy, oz);
-    {
-        // footprint velocities and accelerations f/m (zero outside the grid and on nodes without
-        // mass, which no particle of the tile touches)
-        const long long nn = g.nstride;
-        for (int s = threadIdx.x; s < G::FOOT; s += NT) {
-            const long long n = foot_node<DIM>(g, s, ox, oy, oz);
-            const double m = n >= 0 ? Nf.mass[n] : 0.;
-            const double r = m > 0. ? 1. / m : 0.;
+    auto tile = [&](const int tall) {      // (field, context tile) pair
+        const int fld = tall / g0.tilesTotal;
+        const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+        const int t = tall - fld * g0.tilesTotal - g.tileBase;
+        const KNodes Nf = field_view(g, N, fld);
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        int ox, oy, oz;
+        tile_origin<DIM>(g, t, ox, oy, oz);
+        {
+            // footprint velocities and accelerations f/m (zero outside the grid and on nodes without
+            // mass, which no particle of the tile touches)
+            const long long nn = g.nstride;
+            for (int s = threadIdx.x; s < G::FOOT; s += NT) {
+                const long long n = foot_node<DIM>(g, s, ox, oy, oz);
+                const double m = n >= 0 ? Nf.mass[n] : 0.;
+                const double r = m > 0. ? 1. / m : 0.;
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                s_n[c][s] = n >= 0 ? Nf.vk[c * nn + n] : 0.;
-                s_n[3 + c][s] = n >= 0 ? Nf.ftot[c * nn + n] * r : 0.;
-            }
-        }
-    }
-    zero_acc<DIM, NQ_MOM>(s_acc);
-    __syncthreads();
-    const double dt = g.timestep;
-    for (int c0 = tb; c0 < te; c0 += NT) {
-        const int c1 = min(c0 + NT, te);
-        const int k = c0 + threadIdx.x;
-        double S[3][4], q4[4];
-        CpdiCorners cc;
-        if (k < c1) {
-            int p = B.order[k];
-            // every word of the particle is loaded before anything is stored, so the loads are
-            // issued together (a later load could not move above a store to another row)
-            const int e0 = P.elem[p];
-            double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-            double vold[3] = {P.f[MPF_VEL_X][p], P.f[MPF_VEL_Y][p], DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.};
-            const double mp = P.f[MPF_MASS][p];
-            int ei, ej, ek;
-            elem_ijk(g, e0, ei, ej, ek);
-            int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
-            double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
-            if (CP) {
-                cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
-                constexpr int NCOR = DIM == 3 ? 8 : 4;
-                constexpr double ws = DIM == 3 ? 0.125 : 0.25;
-                double sum[6] = {0., 0., 0., 0., 0., 0.};
-#pragma unroll
-                for (int c = 0; c < NCOR; c++) {
-                    double vc[6];
-                    cpdi_corner_sum<DIM, 6>(cc.xi[c], cc.off[c], lx, ly, lz, s_n, vc);
-#pragma unroll
-                    for (int v = 0; v < 6; v++) sum[v] += vc[v];
+                for (int c = 0; c < 3; c++) {
+                    s_n[c][s] = n >= 0 ? Nf.vk[c * nn + n] : 0.;
+                    s_n[3 + c][s] = n >= 0 ? Nf.ftot[c * nn + n] * r : 0.;
                 }
-#pragma unroll
-                for (int a = 0; a < 3; a++) { Svk[a] = ws * sum[a]; Sacc[a] = ws * sum[3 + a]; }
-            } else {
-            double xi[3], invd[3], dS[3][4];
-            particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
-            particle_factors<DIM>(g, P, p, xi, S, dS);
-            int w[3];
-            double Sw[3][3], Dw[3][3];
-            support_window<DIM>(xi, S, dS, invd, w, Sw, Dw);
-            constexpr int NZ = DIM == 3 ? 3 : 1;
-#pragma unroll
-            for (int jz = 0; jz < NZ; jz++)
-#pragma unroll
-                for (int jy = 0; jy < 3; jy++)
-#pragma unroll
-                    for (int jx = 0; jx < 3; jx++) {
-                        const double fi = DIM == 3 ? Sw[0][jx] * (Sw[1][jy] * Sw[2][jz]) : Sw[0][jx] * Sw[1][jy];
-                        const int s = foot_slot<DIM>(lx + w[0] + jx - 1, ly + w[1] + jy - 1,
-                                                     DIM == 3 ? lz + w[2] + jz - 1 : 0);
-                        Svk[0] = fma(s_n[0][s], fi, Svk[0]);
-                        Svk[1] = fma(s_n[1][s], fi, Svk[1]);
-                        Svk[2] = fma(s_n[2][s], fi, Svk[2]);
-                        Sacc[0] = fma(s_n[3][s], fi, Sacc[0]);
-                        Sacc[1] = fma(s_n[4][s], fi, Sacc[1]);
-                        Sacc[2] = fma(s_n[5][s], fi, Sacc[2]);
-                    }
             }
-            double vnew[3] = {0., 0., 0.};
-#pragma unroll
-            for (int c = 0; c < DIM; c++) {
-                double vm = Svk[c] - Sacc[c] * dt;
-                double delV = Sacc[c] * dt;
-                vnew[c] = vold[c] + delV;
-                P.f[MPF_VEL_X + c][p] = vnew[c];
-                double delXRate = vm + 0.5 * delV;
-                P.posn[c][p] = pos[c] + delXRate * dt;
-            }
-            q4[0] = mp;
-            q4[1] = vnew[0]; q4[2] = vnew[1]; q4[3] = vnew[2];
         }
-        for (int h = 0; h < NT / SUB; h++) {
-            const int r0 = c0 + h * SUB, r1 = min(r0 + SUB, c1);
-            if (r0 >= r1) break;
-            if ((int)(threadIdx.x / SUB) == h && k < c1) {
-                const int i = threadIdx.x - h * SUB;
+        zero_acc<DIM, NQ_MOM>(s_acc);
+        __syncthreads();
+        const double dt = g.timestep;
+        for (int c0 = tb; c0 < te; c0 += NT) {
+            const int c1 = min(c0 + NT, te);
+            const int k = c0 + threadIdx.x;
+            double S[3][4], q4[4];
+            CpdiCorners cc;
+            if (k < c1) {
+                int p = B.order[k];
+                // every word of the particle is loaded before anything is stored, so the loads are
+                // issued together (a later load could not move above a store to another row)
+                const int e0 = P.elem[p];
+                double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+                double vold[3] = {P.f[MPF_VEL_X][p], P.f[MPF_VEL_Y][p], DIM == 3 ? P.f[MPF_VEL_Z][p] : 0.};
+                const double mp = P.f[MPF_MASS][p];
+                int ei, ej, ek;
+                elem_ijk(g, e0, ei, ej, ek);
+                int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
+                double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
                 if (CP) {
-                    // the tables are built here, at the thread's turn (an empty asm makes the corner
-                    // coordinates opaque, so the compiler cannot build them once before the rounds
-                    // and hold all of them in registers)
+                    cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
+                    constexpr int NCOR = DIM == 3 ? 8 : 4;
+                    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
+                    double sum[6] = {0., 0., 0., 0., 0., 0.};
 #pragma unroll
                     for (int c = 0; c < NCOR; c++) {
-                        double xo[3];
+                        double vc[6];
+                        cpdi_corner_sum<DIM, 6>(cc.xi[c], cc.off[c], lx, ly, lz, s_n, vc);
 #pragma unroll
-                        for (int a = 0; a < 3; a++) {
-                            xo[a] = cc.xi[c][a];
-                            asm volatile("" : "+v"(xo[a]));
-                        }
-                        cpdi_table<DIM>(xo, cc.off[c], s_ct[i][c]);
+                        for (int v = 0; v < 6; v++) sum[v] += vc[v];
                     }
+#pragma unroll
+                    for (int a = 0; a < 3; a++) { Svk[a] = ws * sum[a]; Sacc[a] = ws * sum[3 + a]; }
                 } else {
-                    // factors with zeros outside the support; the x factor carries mp
+                double xi[3], invd[3], dS[3][4];
+                particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+                particle_factors<DIM>(g, P, p, xi, S, dS);
+                int w[3];
+                double Sw[3][3], Dw[3][3];
+                support_window<DIM>(xi, S, dS, invd, w, Sw, Dw);
+                constexpr int NZ = DIM == 3 ? 3 : 1;
 #pragma unroll
-                    for (int a = 0; a < DIM; a++)
+                for (int jz = 0; jz < NZ; jz++)
 #pragma unroll
-                        for (int o = 0; o < 4; o++) {
-                            const double v = S[a][o] >= 0. ? S[a][o] : 0.;
-                            s_S[a * 4 + o][i] = a == 0 ? v * q4[0] : v;
+                    for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+                        for (int jx = 0; jx < 3; jx++) {
+                            const double fi = DIM == 3 ? Sw[0][jx] * (Sw[1][jy] * Sw[2][jz]) : Sw[0][jx] * Sw[1][jy];
+                            const int s = foot_slot<DIM>(lx + w[0] + jx - 1, ly + w[1] + jy - 1,
+                                                         DIM == 3 ? lz + w[2] + jz - 1 : 0);
+                            Svk[0] = fma(s_n[0][s], fi, Svk[0]);
+                            Svk[1] = fma(s_n[1][s], fi, Svk[1]);
+                            Svk[2] = fma(s_n[2][s], fi, Svk[2]);
+                            Sacc[0] = fma(s_n[3][s], fi, Sacc[0]);
+                            Sacc[1] = fma(s_n[4][s], fi, Sacc[1]);
+                            Sacc[2] = fma(s_n[5][s], fi, Sacc[2]);
                         }
                 }
+                double vnew[3] = {0., 0., 0.};
 #pragma unroll
-                for (int q = 0; q < 4; q++) s_q[q][i] = q4[q];
+                for (int c = 0; c < DIM; c++) {
+                    double vm = Svk[c] - Sacc[c] * dt;
+                    double delV = Sacc[c] * dt;
+                    vnew[c] = vold[c] + delV;
+                    P.f[MPF_VEL_X + c][p] = vnew[c];
+                    double delXRate = vm + 0.5 * delV;
+                    P.posn[c][p] = pos[c] + delXRate * dt;
+                }
+                q4[0] = mp;
+                q4[1] = vnew[0]; q4[2] = vnew[1]; q4[3] = vnew[2];
             }
-            __syncthreads();
-            wave_cells<DIM, NQ_MOM>(s_cs, r0, r1, s_acc,
-                [&](int lq, int ix, int iy, int iz) {
-                    RecMass r;
+            for (int h = 0; h < NT / SUB; h++) {
+                const int r0 = c0 + h * SUB, r1 = min(r0 + SUB, c1);
+                if (r0 >= r1) break;
+                if ((int)(threadIdx.x / SUB) == h && k < c1) {
+                    const int i = threadIdx.x - h * SUB;
                     if (CP) {
-                        double gr[3];
-                        cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
-                        r.wy = 1.; r.wz = 1.;
+                        // the tables are built here, at the thread's turn (an empty asm makes the corner
+                        // coordinates opaque, so the compiler cannot build them once before the rounds
+                        // and hold all of them in registers)
+#pragma unroll
+                        for (int c = 0; c < NCOR; c++) {
+                            double xo[3];
+#pragma unroll
+                            for (int a = 0; a < 3; a++) {
+                                xo[a] = cc.xi[c][a];
+                                asm volatile("" : "+v"(xo[a]));
+                            }
+                            cpdi_table<DIM>(xo, cc.off[c], s_ct[i][c]);
+                        }
                     } else {
-                        r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
+                        // factors with zeros outside the support; the x factor carries mp
+#pragma unroll
+                        for (int a = 0; a < DIM; a++)
+#pragma unroll
+                            for (int o = 0; o < 4; o++) {
+                                const double v = S[a][o] >= 0. ? S[a][o] : 0.;
+                                s_S[a * 4 + o][i] = a == 0 ? v * q4[0] : v;
+                            }
                     }
-                    r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
-                    return r;
-                },
-                [&](const RecMass &r, double *acc, int, int, int) {
-                    const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
-                    acc[0] = fma(r.vx, fnmp, acc[0]);
-                    acc[1] = fma(r.vy, fnmp, acc[1]);
-                    acc[2] = fma(r.vz, fnmp, acc[2]);
-                });
-            __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < 4; q++) s_q[q][i] = q4[q];
+                }
+                __syncthreads();
+                wave_cells<DIM, NQ_MOM>(s_cs, r0, r1, s_acc,
+                    [&](int lq, int ix, int iy, int iz) {
+                        RecMass r;
+                        if (CP) {
+                            double gr[3];
+                            cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
+                            r.wy = 1.; r.wz = 1.;
+                        } else {
+                            r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
+                        }
+                        r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
+                        return r;
+                    },
+                    [&](const RecMass &r, double *acc, int, int, int) {
+                        const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
+                        acc[0] = fma(r.vx, fnmp, acc[0]);
+                        acc[1] = fma(r.vy, fnmp, acc[1]);
+                        acc[2] = fma(r.vz, fnmp, acc[2]);
+                    });
+                __syncthreads();
+            }
         }
-    }
-    write_partials<DIM, NQ_MOM>(N, tall, s_acc);
+        write_partials<DIM, NQ_MOM>(N, tall, s_acc);
+    };
+    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
 }
 
 // node pass of the USL re-extrapolation: p (rezeroed, RezeroNodeTask6) and v = p/m
@@ -1440,113 +1495,115 @@ __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B,
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
-    const int tall = blockIdx.x;              // (field, context tile) pair
-    const int fld = tall / g0.tilesTotal;
-    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
-    const int t = tall - fld * g0.tilesTotal - g.tileBase;
-    const KNodes Nf = field_view(g, N, fld);
-    load_tile_starts<DIM>(B, tall, s_cs);
-    const int tb = s_cs[0], te = s_cs[G::NC];
-    if (tb == te) return;
-    int ox, oy, oz;
-    tile_origin<DIM>(g, t, ox, oy, oz);
-    {
-        const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
-        const long long stt[3] = {0, g.nstride, 2 * g.nstride};
-        load_foot<DIM, 3>(g, ox, oy, oz, src, stt, s_v);
-    }
-    __syncthreads();
-    for (int k = tb + threadIdx.x; k < te; k += NT) {
-        int p = B.order[k];
-        int e = P.elem[p];
-        // the end-of-step position is loaded here, before the state stores below, so its load is
-        // issued with the others instead of after them
-        double pos[3] = {P.posn[0][p], P.posn[1][p], DIM == 3 ? P.posn[2][p] : 0.};
-        int ei, ej, ek;
-        elem_ijk(g, e, ei, ej, ek);
-        double L[3][3];
-        particle_gradient<DIM, SH>(g, P, p, ei, ej, ek, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
-        scale_gradient<DIM>(L, g.strainLast);
-        const MpmMaterial &m = mats[P.mat[p]];
-        const int mtype = MAT ? MAT : m.type;
-        PState s;
-        load_state<DIM>(P, p, mtype, s);
-        constitutive<DIM, MAT>(L, m, s);
-        store_state<DIM>(P, p, mtype, s);
-
-        // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
-        if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
-            atomicAdd(st.nanCount, 1ull);
-        } else if (!pt_in_elem<DIM>(g, ei, ej, ek, pos)) {
-            // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640) and EdgeElement2D/3D (:171-199)
-            bool left = false;
-            long long horiz = g.nx, vert = g.ny, depth = DIM == 3 ? g.nz : 1;
-            long long total = horiz * vert * depth;
-            long long col = (long long)(int)((pos[0] - g.xmin) / g.dx);
-            if (col < 0 || col >= horiz) {
-                if (pos[0] == g.xmin + horiz * g.dx) col = horiz - 1; else left = true;
-            }
-            long long row = 0, zrow = 0;
-            if (!left) {
-                row = (long long)(int)((pos[1] - g.ymin) / g.dy);
-                if (row < 0 || row >= vert) {
-                    if (pos[1] == g.ymin + vert * g.dy) row = vert - 1; else left = true;
-                }
-            }
-            if (!left && DIM == 3) {
-                zrow = (long long)(int)((pos[2] - g.zmin) / g.dz);
-                if (zrow < 0 || zrow >= depth) {
-                    if (pos[2] == g.zmin + depth * g.dz) zrow = depth - 1; else left = true;
-                }
-            }
-            long long num = 0;
-            if (!left) {
-                num = (DIM == 3 ? horiz * (zrow * vert + row) + col : row * horiz + col) + 1;
-                if (DIM == 3) {
-                    if (num <= horiz * vert || num > total - horiz * vert) left = true;
-                    else if (num % horiz <= 1) left = true;
-                    else {
-                        long long xz = num % (horiz * vert);
-                        if (xz <= horiz || xz > horiz * (vert - 1)) left = true;
-                    }
-                } else {
-                    if (num <= horiz || num > total - horiz) left = true;
-                    else if (num % horiz <= 1) left = true;
-                }
-            }
-            if (!left) {
-                e = (int)(num - 1);
-                P.elem[p] = e;
-                elem_ijk(g, e, ei, ej, ek);
-            } else {
-                if (!P.left[p]) { P.left[p] = 1; atomicAdd(st.leftGrid, 1ull); }
-                // ReturnToElement (ResetElementsTask.cpp:232-265)
-                double dtt = g.timestep;
-                double outside[3] = {pos[0], pos[1], pos[2]};
-                double inside[3];
-                inside[0] = outside[0] - dtt * P.f[MPF_VEL_X][p];
-                inside[1] = outside[1] - dtt * P.f[MPF_VEL_Y][p];
-                inside[2] = DIM == 3 ? outside[2] - dtt * P.f[MPF_VEL_Z][p] : 0.;
-                if (!pt_in_elem<DIM>(g, ei, ej, ek, inside)) {
-                    inside[0] = ((g.xmin + (double)ei * g.dx) + (g.xmin + (double)(ei + 1) * g.dx)) / 2.;
-                    inside[1] = ((g.ymin + (double)ej * g.dy) + (g.ymin + (double)(ej + 1) * g.dy)) / 2.;
-                    inside[2] = DIM == 3 ? ((g.zmin + (double)ek * g.dz) + (g.zmin + (double)(ek + 1) * g.dz)) / 2. : 0.;
-                }
-                for (int pass = 1; pass <= 10; pass++) {
-                    double mid[3] = {(outside[0] + inside[0]) / 2., (outside[1] + inside[1]) / 2.,
-                                     (outside[2] + inside[2]) / 2.};
-                    if (pt_in_elem<DIM>(g, ei, ej, ek, mid)) { inside[0] = mid[0]; inside[1] = mid[1]; inside[2] = mid[2]; }
-                    else { outside[0] = mid[0]; outside[1] = mid[1]; outside[2] = mid[2]; }
-                }
-                P.posn[0][p] = inside[0];
-                P.posn[1][p] = inside[1];
-                if (DIM == 3) P.posn[2][p] = inside[2];
-            }
+    auto tile = [&](const int tall) {      // (field, context tile) pair
+        const int fld = tall / g0.tilesTotal;
+        const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+        const int t = tall - fld * g0.tilesTotal - g.tileBase;
+        const KNodes Nf = field_view(g, N, fld);
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        int ox, oy, oz;
+        tile_origin<DIM>(g, t, ox, oy, oz);
+        {
+            const double *src[3] = {Nf.vk, Nf.vk, Nf.vk};
+            const long long stt[3] = {0, g.nstride, 2 * g.nstride};
+            load_foot<DIM, 3>(g, ox, oy, oz, src, stt, s_v);
         }
-        int key = cell_key<DIM>(g, fld, ei, ej, DIM == 3 ? ek : 0);
-        B.key[p] = key;
-        wave_count(B.count, key);
-    }
+        __syncthreads();
+        for (int k = tb + threadIdx.x; k < te; k += NT) {
+            int p = B.order[k];
+            int e = P.elem[p];
+            // the end-of-step position is loaded here, before the state stores below, so its load is
+            // issued with the others instead of after them
+            double pos[3] = {P.posn[0][p], P.posn[1][p], DIM == 3 ? P.posn[2][p] : 0.};
+            int ei, ej, ek;
+            elem_ijk(g, e, ei, ej, ek);
+            double L[3][3];
+            particle_gradient<DIM, SH>(g, P, p, ei, ej, ek, ei - ox, ej - oy, DIM == 3 ? ek - oz : 0, s_v, L);
+            scale_gradient<DIM>(L, g.strainLast);
+            const MpmMaterial &m = mats[P.mat[p]];
+            const int mtype = MAT ? MAT : m.type;
+            PState s;
+            load_state<DIM>(P, p, mtype, s);
+            constitutive<DIM, MAT>(L, m, s);
+            store_state<DIM>(P, p, mtype, s);
+
+            // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
+            if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
+                atomicAdd(st.nanCount, 1ull);
+            } else if (!pt_in_elem<DIM>(g, ei, ej, ek, pos)) {
+                // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640) and EdgeElement2D/3D (:171-199)
+                bool left = false;
+                long long horiz = g.nx, vert = g.ny, depth = DIM == 3 ? g.nz : 1;
+                long long total = horiz * vert * depth;
+                long long col = (long long)(int)((pos[0] - g.xmin) / g.dx);
+                if (col < 0 || col >= horiz) {
+                    if (pos[0] == g.xmin + horiz * g.dx) col = horiz - 1; else left = true;
+                }
+                long long row = 0, zrow = 0;
+                if (!left) {
+                    row = (long long)(int)((pos[1] - g.ymin) / g.dy);
+                    if (row < 0 || row >= vert) {
+                        if (pos[1] == g.ymin + vert * g.dy) row = vert - 1; else left = true;
+                    }
+                }
+                if (!left && DIM == 3) {
+                    zrow = (long long)(int)((pos[2] - g.zmin) / g.dz);
+                    if (zrow < 0 || zrow >= depth) {
+                        if (pos[2] == g.zmin + depth * g.dz) zrow = depth - 1; else left = true;
+                    }
+                }
+                long long num = 0;
+                if (!left) {
+                    num = (DIM == 3 ? horiz * (zrow * vert + row) + col : row * horiz + col) + 1;
+                    if (DIM == 3) {
+                        if (num <= horiz * vert || num > total - horiz * vert) left = true;
+                        else if (num % horiz <= 1) left = true;
+                        else {
+                            long long xz = num % (horiz * vert);
+                            if (xz <= horiz || xz > horiz * (vert - 1)) left = true;
+                        }
+                    } else {
+                        if (num <= horiz || num > total - horiz) left = true;
+                        else if (num % horiz <= 1) left = true;
+                    }
+                }
+                if (!left) {
+                    e = (int)(num - 1);
+                    P.elem[p] = e;
+                    elem_ijk(g, e, ei, ej, ek);
+                } else {
+                    if (!P.left[p]) { P.left[p] = 1; atomicAdd(st.leftGrid, 1ull); }
+                    // ReturnToElement (ResetElementsTask.cpp:232-265)
+                    double dtt = g.timestep;
+                    double outside[3] = {pos[0], pos[1], pos[2]};
+                    double inside[3];
+                    inside[0] = outside[0] - dtt * P.f[MPF_VEL_X][p];
+                    inside[1] = outside[1] - dtt * P.f[MPF_VEL_Y][p];
+                    inside[2] = DIM == 3 ? outside[2] - dtt * P.f[MPF_VEL_Z][p] : 0.;
+                    if (!pt_in_elem<DIM>(g, ei, ej, ek, inside)) {
+                        inside[0] = ((g.xmin + (double)ei * g.dx) + (g.xmin + (double)(ei + 1) * g.dx)) / 2.;
+                        inside[1] = ((g.ymin + (double)ej * g.dy) + (g.ymin + (double)(ej + 1) * g.dy)) / 2.;
+                        inside[2] = DIM == 3 ? ((g.zmin + (double)ek * g.dz) + (g.zmin + (double)(ek + 1) * g.dz)) / 2. : 0.;
+                    }
+                    for (int pass = 1; pass <= 10; pass++) {
+                        double mid[3] = {(outside[0] + inside[0]) / 2., (outside[1] + inside[1]) / 2.,
+                                         (outside[2] + inside[2]) / 2.};
+                        if (pt_in_elem<DIM>(g, ei, ej, ek, mid)) { inside[0] = mid[0]; inside[1] = mid[1]; inside[2] = mid[2]; }
+                        else { outside[0] = mid[0]; outside[1] = mid[1]; outside[2] = mid[2]; }
+                    }
+                    P.posn[0][p] = inside[0];
+                    P.posn[1][p] = inside[1];
+                    if (DIM == 3) P.posn[2][p] = inside[2];
+                }
+            }
+            int key = cell_key<DIM>(g, fld, ei, ej, DIM == 3 ? ek : 0);
+            B.key[p] = key;
+            wave_count(B.count, key);
+        }
+    };
+    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1573,129 +1630,131 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
     __shared__ double s_q[CH][5];              // Vp, mp, displacement
     __shared__ double s_acc[NW][NQ_CONTACT][WaveFoot<DIM>::SIZE];
     __shared__ double s_id[DIM][G::TX];
-    const int tall = blockIdx.x;
-    const int fld = tall / g0.tilesTotal;
-    const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
-    const int t = tall - fld * g0.tilesTotal - g.tileBase;
-    load_tile_starts<DIM>(B, tall, s_cs);
-    const int tb = s_cs[0], te = s_cs[G::NC];
-    if (tb == te) return;
-    int ox, oy, oz;
-    tile_origin<DIM>(g, t, ox, oy, oz);
-    if (threadIdx.x < G::TX) {
-        int i = threadIdx.x;
-        s_id[0][i] = 2.0 / ((g.xmin + (double)(ox + i + 1) * g.dx) - (g.xmin + (double)(ox + i) * g.dx));
-        if (i < G::TY) s_id[1][i] = 2.0 / ((g.ymin + (double)(oy + i + 1) * g.dy) - (g.ymin + (double)(oy + i) * g.dy));
-        if (DIM == 3 && i < G::TZ)
-            s_id[DIM - 1][i] = 2.0 / ((g.zmin + (double)(oz + i + 1) * g.dz) - (g.zmin + (double)(oz + i) * g.dz));
-    }
-    zero_acc<DIM, NQ_CONTACT>(s_acc);
-    // staging thread: particle si of the chunk, and (lCPDI) its corner sc; the particle's own
-    // quantities are staged by its first thread
-    const int si = CP ? threadIdx.x / NCOR : threadIdx.x, sc = CP ? threadIdx.x % NCOR : 0;
-    for (int c0 = tb; c0 < te; c0 += CH) {
-        const int k = c0 + si;
-        if (si < CH && k < te) {
-            const int p = B.order[k];
-            int ei, ej, ek;
-            elem_ijk(g, P.elem[p], ei, ej, ek);
-            double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
-            if (CP) {
-                double r[9];
-#pragma unroll
-                for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
-                double xi[3], w[3];
-                int off[3];
-                cpdi_corner<DIM>(g, pos, r, ei, ej, ek, sc, xi, off);
-                cpdi_table<DIM>(xi, off, s_ct[si][sc]);
-                cpdi_wg_listed<DIM>(r, sc, w);
-                s_cwg[si][sc][0] = w[0]; s_cwg[si][sc][1] = w[1]; s_cwg[si][sc][2] = w[2];
-            } else {
-                double xi[3], invd[3], S[3][4], dS[3][4];
-                particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
-                particle_factors<DIM>(g, P, p, xi, S, dS);
-#pragma unroll
-                for (int a = 0; a < DIM; a++)
-#pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        s_S[threadIdx.x][a][o] = S[a][o];
-                        s_dS[threadIdx.x][a][o] = dS[a][o];
-                    }
-            }
-            if (sc == 0) {
-                double mp = P.f[MPF_MASS][p];
-                double rho = mats[P.mat[p]].rho;
-                double Vp;
-                if (DIM == 3) {
-                    double Fm[3][3];
-                    double ep[6] = {P.f[MPF_EP_XX][p], P.f[MPF_EP_YY][p], P.f[MPF_EP_ZZ][p],
-                                    P.f[MPF_EP_YZ][p], P.f[MPF_EP_XZ][p], P.f[MPF_EP_XY][p]};
-                    double wr[3] = {P.f[MPF_WR_YZ][p], P.f[MPF_WR_XZ][p], P.f[MPF_WR_XY][p]};
-                    get_F3(ep, wr, Fm);
-                    Vp = (Fm[0][0] * (Fm[1][1] * Fm[2][2] - Fm[2][1] * Fm[1][2])
-                        - Fm[0][1] * (Fm[1][0] * Fm[2][2] - Fm[2][0] * Fm[1][2])
-                        + Fm[0][2] * (Fm[1][0] * Fm[2][1] - Fm[2][0] * Fm[1][1])) * mp / rho;
-                } else {
-                    double exy = P.f[MPF_EP_XY][p], wxy = P.f[MPF_WR_XY][p];
-                    double f00 = 1. + P.f[MPF_EP_XX][p], f11 = 1. + P.f[MPF_EP_YY][p];
-                    double f01 = 0.5 * (exy - wxy), f10 = 0.5 * (exy + wxy);
-                    Vp = (f00 * f11 - f10 * f01) * mp / rho;
-                }
-                s_q[si][0] = Vp;
-                s_q[si][1] = mp;
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-                    s_q[si][2 + a] = a < DIM ? (newPositions ? P.posn[a][p] : pos[a]) - P.f[MPF_ORIG_X + a][p] : 0.;
-            }
+    auto tile = [&](const int tall) {      // (field, context tile) pair
+        const int fld = tall / g0.tilesTotal;
+        const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+        const int t = tall - fld * g0.tilesTotal - g.tileBase;
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        int ox, oy, oz;
+        tile_origin<DIM>(g, t, ox, oy, oz);
+        if (threadIdx.x < G::TX) {
+            int i = threadIdx.x;
+            s_id[0][i] = 2.0 / ((g.xmin + (double)(ox + i + 1) * g.dx) - (g.xmin + (double)(ox + i) * g.dx));
+            if (i < G::TY) s_id[1][i] = 2.0 / ((g.ymin + (double)(oy + i + 1) * g.dy) - (g.ymin + (double)(oy + i) * g.dy));
+            if (DIM == 3 && i < G::TZ)
+                s_id[DIM - 1][i] = 2.0 / ((g.zmin + (double)(oz + i + 1) * g.dz) - (g.zmin + (double)(oz + i) * g.dz));
         }
-        __syncthreads();
-        wave_cells<DIM, NQ_CONTACT>(s_cs, c0, min(c0 + CH, te), s_acc,
-            [&](int lq, int ix, int iy, int iz) {
-                RecContact r;
+        zero_acc<DIM, NQ_CONTACT>(s_acc);
+        // staging thread: particle si of the chunk, and (lCPDI) its corner sc; the particle's own
+        // quantities are staged by its first thread
+        const int si = CP ? threadIdx.x / NCOR : threadIdx.x, sc = CP ? threadIdx.x % NCOR : 0;
+        for (int c0 = tb; c0 < te; c0 += CH) {
+            const int k = c0 + si;
+            if (si < CH && k < te) {
+                const int p = B.order[k];
+                int ei, ej, ek;
+                elem_ijk(g, P.elem[p], ei, ej, ek);
+                double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
                 if (CP) {
-                    double gr[3];
-                    cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
-                    r.wy = r.wz = 1.;
-                    r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
+                    double r[9];
+#pragma unroll
+                    for (int a = 0; a < 9; a++) r[a] = P.cpr[a][p];
+                    double xi[3], w[3];
+                    int off[3];
+                    cpdi_corner<DIM>(g, pos, r, ei, ej, ek, sc, xi, off);
+                    cpdi_table<DIM>(xi, off, s_ct[si][sc]);
+                    cpdi_wg_staged<DIM>(r, sc, w);
+                    s_cwg[si][sc][0] = w[0]; s_cwg[si][sc][1] = w[1]; s_cwg[si][sc][2] = w[2];
                 } else {
-                    r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
-                    r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
-                    if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
-                    else { r.wz = 1.; r.dz = 0.; }
+                    double xi[3], invd[3], S[3][4], dS[3][4];
+                    particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+                    particle_factors<DIM>(g, P, p, xi, S, dS);
+#pragma unroll
+                    for (int a = 0; a < DIM; a++)
+#pragma unroll
+                        for (int o = 0; o < 4; o++) {
+                            s_S[threadIdx.x][a][o] = S[a][o];
+                            s_dS[threadIdx.x][a][o] = dS[a][o];
+                        }
                 }
-                r.vp = s_q[lq][0]; r.mp = s_q[lq][1];
-                r.ux = s_q[lq][2]; r.uy = s_q[lq][3]; r.uz = s_q[lq][4];
-                return r;
-            },
-            [&](const RecContact &r, double *acc, int lx, int ly, int lz) {
-                bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
-                double vp = on ? r.vp : 0., mp = on ? r.mp : 0.;
-                double fn, gx, gy, gz;
-                if (CP) {
-                    fn = r.wx; gx = r.dx; gy = r.dy; gz = DIM == 3 ? r.dz : 0.;
-                } else if (DIM == 3) {
-                    fn = r.wx * r.wy * r.wz;
-                    gx = r.dx * r.wy * r.wz * s_id[0][lx];
-                    gy = r.wx * r.dy * r.wz * s_id[1][ly];
-                    gz = r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
-                } else {
-                    fn = r.wx * r.wy;
-                    gx = r.dx * r.wy * s_id[0][lx];
-                    gy = r.wx * r.dy * s_id[1][ly];
-                    gz = 0.;
+                if (sc == 0) {
+                    double mp = P.f[MPF_MASS][p];
+                    double rho = mats[P.mat[p]].rho;
+                    double Vp;
+                    if (DIM == 3) {
+                        double Fm[3][3];
+                        double ep[6] = {P.f[MPF_EP_XX][p], P.f[MPF_EP_YY][p], P.f[MPF_EP_ZZ][p],
+                                        P.f[MPF_EP_YZ][p], P.f[MPF_EP_XZ][p], P.f[MPF_EP_XY][p]};
+                        double wr[3] = {P.f[MPF_WR_YZ][p], P.f[MPF_WR_XZ][p], P.f[MPF_WR_XY][p]};
+                        get_F3(ep, wr, Fm);
+                        Vp = (Fm[0][0] * (Fm[1][1] * Fm[2][2] - Fm[2][1] * Fm[1][2])
+                            - Fm[0][1] * (Fm[1][0] * Fm[2][2] - Fm[2][0] * Fm[1][2])
+                            + Fm[0][2] * (Fm[1][0] * Fm[2][1] - Fm[2][0] * Fm[1][1])) * mp / rho;
+                    } else {
+                        double exy = P.f[MPF_EP_XY][p], wxy = P.f[MPF_WR_XY][p];
+                        double f00 = 1. + P.f[MPF_EP_XX][p], f11 = 1. + P.f[MPF_EP_YY][p];
+                        double f01 = 0.5 * (exy - wxy), f10 = 0.5 * (exy + wxy);
+                        Vp = (f00 * f11 - f10 * f01) * mp / rho;
+                    }
+                    s_q[si][0] = Vp;
+                    s_q[si][1] = mp;
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+                        s_q[si][2 + a] = a < DIM ? (newPositions ? P.posn[a][p] : pos[a]) - P.f[MPF_ORIG_X + a][p] : 0.;
                 }
-                acc[0] += fn * vp;
-                double fnmp = fn * mp;
-                acc[1] += r.ux * fnmp;
-                acc[2] += r.uy * fnmp;
-                acc[3] += r.uz * fnmp;
-                acc[4] += gx * vp;
-                acc[5] += gy * vp;
-                acc[6] += gz * vp;
-            });
-        __syncthreads();
-    }
-    write_partials<DIM, NQ_CONTACT>(N, tall, s_acc);
+            }
+            __syncthreads();
+            wave_cells<DIM, NQ_CONTACT>(s_cs, c0, min(c0 + CH, te), s_acc,
+                [&](int lq, int ix, int iy, int iz) {
+                    RecContact r;
+                    if (CP) {
+                        double gr[3];
+                        cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
+                        r.wy = r.wz = 1.;
+                        r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
+                    } else {
+                        r.wx = s_S[lq][0][ix]; r.wy = s_S[lq][1][iy];
+                        r.dx = s_dS[lq][0][ix]; r.dy = s_dS[lq][1][iy];
+                        if (DIM == 3) { r.wz = s_S[lq][DIM - 1][iz]; r.dz = s_dS[lq][DIM - 1][iz]; }
+                        else { r.wz = 1.; r.dz = 0.; }
+                    }
+                    r.vp = s_q[lq][0]; r.mp = s_q[lq][1];
+                    r.ux = s_q[lq][2]; r.uy = s_q[lq][3]; r.uz = s_q[lq][4];
+                    return r;
+                },
+                [&](const RecContact &r, double *acc, int lx, int ly, int lz) {
+                    bool on = r.wx >= 0. && r.wy >= 0. && r.wz >= 0.;
+                    double vp = on ? r.vp : 0., mp = on ? r.mp : 0.;
+                    double fn, gx, gy, gz;
+                    if (CP) {
+                        fn = r.wx; gx = r.dx; gy = r.dy; gz = DIM == 3 ? r.dz : 0.;
+                    } else if (DIM == 3) {
+                        fn = r.wx * r.wy * r.wz;
+                        gx = r.dx * r.wy * r.wz * s_id[0][lx];
+                        gy = r.wx * r.dy * r.wz * s_id[1][ly];
+                        gz = r.wx * r.wy * r.dz * s_id[DIM - 1][lz];
+                    } else {
+                        fn = r.wx * r.wy;
+                        gx = r.dx * r.wy * s_id[0][lx];
+                        gy = r.wx * r.dy * s_id[1][ly];
+                        gz = 0.;
+                    }
+                    acc[0] = fma(fn, vp, acc[0]);
+                    const double fnmp = fn * mp;
+                    acc[1] = fma(r.ux, fnmp, acc[1]);
+                    acc[2] = fma(r.uy, fnmp, acc[2]);
+                    acc[3] = fma(r.uz, fnmp, acc[3]);
+                    acc[4] = fma(gx, vp, acc[4]);
+                    acc[5] = fma(gy, vp, acc[5]);
+                    acc[6] = fma(gz, vp, acc[6]);
+                });
+            __syncthreads();
+        }
+        write_partials<DIM, NQ_CONTACT>(N, tall, s_acc);
+    };
+    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
 }
 
 template <int DIM>
@@ -1898,27 +1957,55 @@ void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hi
 }
 
 void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s) {
-    (void)dim; (void)g;
+    (void)g;
     int n = b.nkeys + 1;          // count[nkeys] stays zero, so cellStart[nkeys] = total
     int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(NT), 0, s, b.count, b.cellStart, b.blockSums, n);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(NT), 0, s, b.blockSums, nb);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(NT), 0, s, b.blockSums, nb, b.ntiles);
     hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n, NT)), dim3(NT), 0, s, b.cellStart, b.blockSums, n);
     if (p.n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(p.n, NT)), dim3(NT), 0, s, p, b);
     hipLaunchKernelGGL(k_bin_sort_cells, dim3(blocks_for(b.nkeys, NT)), dim3(NT), 0, s, b);
+    hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(b.ntall, NT)), dim3(NT), 0, s, b,
+                       dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC);
 }
+
+static int g_cus = 256;
+void set_tile_cus(int cus) { g_cus = cus > 0 ? cus : 256; }
+
+// workgroups of one tile-kernel launch: those the device holds at once at the kernel's occupancy
+// (found once per kernel), at most one per (field, tile) pair
+template <typename Kern>
+static int tile_blocks_per_cu(Kern kernel) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+static dim3 tile_grid_dim(const KBins &b, int perCu) {
+    const long long want = (long long)perCu * g_cus;
+    return dim3((unsigned)(b.ntall < want ? b.ntall : want));
+}
+static KBins tile_bins(const KBins &b, int slot) {
+    KBins r = b;
+    r.slot = slot;
+    return r;
+}
+#define TILE_LAUNCH(KERNEL, B, S, ...)                                                          \
+    do {                                                                                         \
+        static const int perCu_ = tile_blocks_per_cu(KERNEL);                                    \
+        hipLaunchKernelGGL(KERNEL, tile_grid_dim(B, perCu_), dim3(NT), 0, S, __VA_ARGS__);      \
+    } while (0)
 
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                      const MpmMaterial *mats, const KStats &st, hipStream_t s) {
+    const KBins tb = tile_bins(b, TILE_SLOT_MASS);
     (void)mats;
-    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
-        if (cp) hipLaunchKernelGGL((k_p2g_mass<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, st);
-        else hipLaunchKernelGGL((k_p2g_mass<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, st);
+        if (cp) TILE_LAUNCH((k_p2g_mass<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd, st);
+        else TILE_LAUNCH((k_p2g_mass<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd, st);
     } else {
-        if (cp) hipLaunchKernelGGL((k_p2g_mass<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, st);
-        else hipLaunchKernelGGL((k_p2g_mass<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, st);
+        if (cp) TILE_LAUNCH((k_p2g_mass<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd, st);
+        else TILE_LAUNCH((k_p2g_mass<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd, st);
     }
 }
 
@@ -1930,18 +2017,18 @@ void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
 template <int DIM, int SH>
 static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                     const MpmMaterial *mats, hipStream_t s) {
-    dim3 grid(g.tilesTotal * g.nfields), block(NT);
+    const KBins tb = tile_bins(b, TILE_SLOT_USF);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOTROPIC, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_ISOTROPIC, SH>), tb, s, g, p, tb, nd, mats); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), tb, s, g, p, tb, nd, mats); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), tb, s, g, p, tb, nd, mats); break;
         case MPMGPU_MAT_MOONEY:
-            hipLaunchKernelGGL((k_usf<DIM, MPMGPU_MAT_MOONEY, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_MOONEY, SH>), tb, s, g, p, tb, nd, mats); break;
         default:
-            hipLaunchKernelGGL((k_usf<DIM, 0, SH>), grid, block, 0, s, g, p, b, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, 0, SH>), tb, s, g, p, tb, nd, mats); break;
     }
 }
 
@@ -1958,14 +2045,14 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
 }
 
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    dim3 grid(g.tilesTotal * g.nfields), block(NT);
+    const KBins tb = tile_bins(b, TILE_SLOT_FORCE);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
-        if (cp) hipLaunchKernelGGL((k_p2g_force<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
-        else hipLaunchKernelGGL((k_p2g_force<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+        if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
+        else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
     } else {
-        if (cp) hipLaunchKernelGGL((k_p2g_force<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
-        else hipLaunchKernelGGL((k_p2g_force<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+        if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
+        else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
     }
 }
 
@@ -1975,14 +2062,14 @@ void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd
 }
 
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
-    dim3 grid(g.tilesTotal * g.nfields), block(NT);
+    const KBins tb = tile_bins(b, TILE_SLOT_G2P);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
-        if (cp) hipLaunchKernelGGL((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
-        else hipLaunchKernelGGL((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
+        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
     } else {
-        if (cp) hipLaunchKernelGGL((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd);
-        else hipLaunchKernelGGL((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
+        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
     }
 }
 
@@ -1994,18 +2081,18 @@ void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
 template <int DIM, int SH>
 static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                           const MpmMaterial *mats, const KStats &st, hipStream_t s) {
-    dim3 grid(g.tilesTotal * g.nfields), block(NT);
+    const KBins tb = tile_bins(b, TILE_SLOT_USL);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC, SH>), tb, s, g, p, tb, nd, mats, st); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), tb, s, g, p, tb, nd, mats, st); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), tb, s, g, p, tb, nd, mats, st); break;
         case MPMGPU_MAT_MOONEY:
-            hipLaunchKernelGGL((k_usl_reset<DIM, MPMGPU_MAT_MOONEY, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_MOONEY, SH>), tb, s, g, p, tb, nd, mats, st); break;
         default:
-            hipLaunchKernelGGL((k_usl_reset<DIM, 0, SH>), grid, block, 0, s, g, p, b, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, 0, SH>), tb, s, g, p, tb, nd, mats, st); break;
     }
 }
 
@@ -2023,15 +2110,15 @@ void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, c
 
 void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                         const MpmMaterial *mats, bool newPositions, hipStream_t s) {
+    const KBins tb = tile_bins(b, newPositions ? TILE_SLOT_CONTACT_LAST : TILE_SLOT_CONTACT_FIRST);
     int np = newPositions ? 1 : 0;
-    dim3 grid(g.tilesTotal * g.nfields), block(NT);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
-        if (cp) hipLaunchKernelGGL((k_p2g_contact<3, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, mats, np);
-        else hipLaunchKernelGGL((k_p2g_contact<3, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, mats, np);
+        if (cp) TILE_LAUNCH((k_p2g_contact<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd, mats, np);
+        else TILE_LAUNCH((k_p2g_contact<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd, mats, np);
     } else {
-        if (cp) hipLaunchKernelGGL((k_p2g_contact<2, MPMGPU_SHAPE_LCPDI>), grid, block, 0, s, g, p, b, nd, mats, np);
-        else hipLaunchKernelGGL((k_p2g_contact<2, MPMGPU_SHAPE_UGIMP>), grid, block, 0, s, g, p, b, nd, mats, np);
+        if (cp) TILE_LAUNCH((k_p2g_contact<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd, mats, np);
+        else TILE_LAUNCH((k_p2g_contact<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd, mats, np);
     }
 }
 
