@@ -306,7 +306,7 @@ static void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunPa
 
 // ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
 static size_t bin_words(long long nk, long long ntall) {
-    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * TILE_SLOT_STRIDE);
+    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * 8 * TILE_SLOT_STRIDE);
 }
 
 // device buffers of the grid, bins and counters for c->kg (nstride nodes, tilesTotal tiles per field)

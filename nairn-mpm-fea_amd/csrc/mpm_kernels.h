@@ -74,13 +74,13 @@ struct KBins {
     int *shome;                // sorted slot -> upload index of its particle
     int *tiles;                // the non-empty (field, tile) pairs of the binning, in no fixed order
     int *ntiles;               // length of tiles
-    int *work;                 // lCPDI tile kernels' list cursors, one per launch of a step, 32 ints apart
+    int *work;                 // tile kernels' list cursors: 8 per launch of a step, 32 ints apart
     int nkeys;
     int ntall;                 // (field, tile) pairs: nfields * tilesTotal
     int slot;                  // the launch's cursor (set by the launch functions)
 };
 
-// cursors of the lCPDI tile kernels of one step (cleared when the tile list is built)
+// launches of the tile kernels in one step, each with its own cursors (cleared when the list is built)
 enum { TILE_SLOT_MASS, TILE_SLOT_CONTACT_FIRST, TILE_SLOT_USF, TILE_SLOT_FORCE, TILE_SLOT_G2P,
        TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOTS };
 #define TILE_SLOT_STRIDE 32

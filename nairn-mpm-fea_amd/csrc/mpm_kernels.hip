@@ -473,7 +473,7 @@ __global__ void k_bin_sort_cells(KBins B) {
 // list of the non-empty (field, tile) pairs; clears the lCPDI kernels' cursors
 __global__ void k_tile_list(KBins B, int nc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x < TILE_SLOTS) B.work[threadIdx.x * TILE_SLOT_STRIDE] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < TILE_SLOTS * 8) B.work[threadIdx.x * TILE_SLOT_STRIDE] = 0;
     const bool on = t < B.ntall && B.cellStart[(size_t)(t + 1) * nc] > B.cellStart[(size_t)t * nc];
     const unsigned long long m = __ballot(on);
     if (m == 0ull) return;
@@ -485,27 +485,21 @@ __global__ void k_tile_list(KBins B, int nc) {
     if (on) B.tiles[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
 }
 
-// A tile kernel is launched with as many workgroups as the device holds at once, and its blocks
-// walk the list of non-empty tiles, so no workgroup is spent on an empty tile. Block b takes entry
-// b first. Then, with uniform per-particle work (shape-function kernels), it takes b + gridDim.x,
-// b + 2 gridDim.x, ... (no counter shared between blocks); with lCPDI, whose tiles are few and
-// heavy, it takes the next unclaimed entry from the launch's cursor, claimed while its current
-// tile runs. Results do not depend on which block takes a tile.
-template <bool DYN, typename F>
+// A tile kernel's blocks walk the list of non-empty tiles, so no workgroup is spent on an empty
+// tile. Block b takes entry b first, then further entries claimed from one of eight cursors (the
+// block's cursor is b mod 8, so the blocks sharing a cursor sit on one XCD and a claim never waits
+// behind the whole device), entry gridDim.x + c + 8 k for the k-th claim on cursor c. The claim for
+// the next tile is made while the current one runs. Results do not depend on which block takes a tile.
+template <typename F>
 __device__ __forceinline__ void for_each_tile(const KBins &B, F &tile) {
     const int n = B.ntiles[0];
-    if (!DYN) {
-        for (int li = blockIdx.x; li < n; li += gridDim.x) {
-            tile(B.tiles[li]);
-            __syncthreads();
-        }
-        return;
-    }
     __shared__ int s_li;
+    const int cur = blockIdx.x & 7;
+    int *cursor = &B.work[(B.slot * 8 + cur) * TILE_SLOT_STRIDE];
     int li = blockIdx.x;
     while (li < n) {
         int next = 0;
-        if (threadIdx.x == 0) next = (int)gridDim.x + atomicAdd(&B.work[B.slot * TILE_SLOT_STRIDE], 1);
+        if (threadIdx.x == 0) next = (int)gridDim.x + cur + 8 * atomicAdd(cursor, 1);
         tile(B.tiles[li]);
         if (threadIdx.x == 0) s_li = next;
         __syncthreads();
@@ -749,7 +743,7 @@ __global__ __launch_bounds__(NT) void k_p2g_mass(KGrid g0, KPart P, KBins B, KNo
         }
         write_partials<DIM, NQ_MASS>(N, tall, s_acc);
     };
-    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
+    for_each_tile(B, tile);
 }
 
 // sum the partials of the tiles that cover node (i,j,k), in fixed tile order
@@ -1078,7 +1072,7 @@ __global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNode
             store_state<DIM>(P, p, mtype, s);
         }
     };
-    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
+    for_each_tile(B, tile);
 }
 
 // internal-force scatter (GridForcesTask::Execute, GridForcesTask.cpp:40-153 ->
@@ -1231,7 +1225,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
         }
         write_partials<DIM, NQ_FORCE>(N, tall, s_acc);
     };
-    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
+    for_each_tile(B, tile);
 }
 
 // node pass after the forces: ftot, restore p (identity without BCs), p += f dt, v = p/m
@@ -1441,7 +1435,7 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
         }
         write_partials<DIM, NQ_MOM>(N, tall, s_acc);
     };
-    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
+    for_each_tile(B, tile);
 }
 
 // node pass of the USL re-extrapolation: p (rezeroed, RezeroNodeTask6) and v = p/m
@@ -1603,7 +1597,7 @@ __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B,
             wave_count(B.count, key);
         }
     };
-    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
+    for_each_tile(B, tile);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1754,7 +1748,7 @@ __global__ __launch_bounds__(NT) void k_p2g_contact(KGrid g0, KPart P, KBins B, 
         }
         write_partials<DIM, NQ_CONTACT>(N, tall, s_acc);
     };
-    for_each_tile<SH == MPMGPU_SHAPE_LCPDI>(B, tile);
+    for_each_tile(B, tile);
 }
 
 template <int DIM>
@@ -1972,16 +1966,10 @@ void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, 
 static int g_cus = 256;
 void set_tile_cus(int cus) { g_cus = cus > 0 ? cus : 256; }
 
-// workgroups of one tile-kernel launch: those the device holds at once at the kernel's occupancy
-// (found once per kernel), at most one per (field, tile) pair
-template <typename Kern>
-static int tile_blocks_per_cu(Kern kernel) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, NT, 0) != hipSuccess || nb < 1) nb = 1;
-    return nb;
-}
-static dim3 tile_grid_dim(const KBins &b, int perCu) {
-    const long long want = (long long)perCu * g_cus;
+// workgroups of one tile-kernel launch: eight per CU (at least as many as the device holds at once
+// for any of the tile kernels), at most one per (field, tile) pair
+static dim3 tile_grid_dim(const KBins &b) {
+    const long long want = 8LL * g_cus;
     return dim3((unsigned)(b.ntall < want ? b.ntall : want));
 }
 static KBins tile_bins(const KBins &b, int slot) {
@@ -1989,11 +1977,7 @@ static KBins tile_bins(const KBins &b, int slot) {
     r.slot = slot;
     return r;
 }
-#define TILE_LAUNCH(KERNEL, B, S, ...)                                                          \
-    do {                                                                                         \
-        static const int perCu_ = tile_blocks_per_cu(KERNEL);                                    \
-        hipLaunchKernelGGL(KERNEL, tile_grid_dim(B, perCu_), dim3(NT), 0, S, __VA_ARGS__);      \
-    } while (0)
+#define TILE_LAUNCH(KERNEL, B, S, ...) hipLaunchKernelGGL(KERNEL, tile_grid_dim(B), dim3(NT), 0, S, __VA_ARGS__)
 
 void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                      const MpmMaterial *mats, const KStats &st, hipStream_t s) {
