@@ -84,6 +84,9 @@ static void enqueue_step(MpmCtx *c) {
     c->parity ^= 1;
 }
 
+// events recorded between two collections of the kernel-group timings (profiling mode)
+#define MAX_LIVE_EVENTS 1024
+
 // fold the recorded kernel-group events into per-group totals (after a stream sync)
 static void collect_marks(MpmCtx *c) {
     for (auto &m : c->evMarks) {
@@ -734,6 +737,12 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
         enqueue_step(c);
         c->sinceSort++;
         if (c->sortInterval > 0 && c->sinceSort >= c->sortInterval && mpmi::sort_rows(c)) return 1;
+        if (c->evUsed >= MAX_LIVE_EVENTS) {
+            // the group timings recorded so far are folded in, so the events can be recorded again:
+            // a long profiled run holds at most this many events
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+            collect_marks(c);
+        }
     }
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
