@@ -94,9 +94,30 @@ def copy_calibration(nbytes, launches=5):
                       "GBps": 2 * n * 8 / dt / 1e9}), flush=True)
 
 
-def cpu_sample(pairs, seconds, steps=3):
-    """Oracle particle-steps/s on one host core over [(problem, (d, i))] run in order until about
-    `seconds` of CPU work is done."""
+def cpu_threads(asked=0):
+    """Host threads of the CPU baseline: the asked number, else OMP_NUM_THREADS, else the host's
+    cores, at most 16 (one GPU's share of the host)."""
+    if asked > 0:
+        return asked
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_sample(pairs, seconds, steps=3, threads=1):
+    """Oracle particle-steps/s with `threads` host threads over [(problem, (d, i))] run in order
+    until about `seconds` of wall time is spent stepping."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     done = ps = 0
@@ -104,6 +125,7 @@ def cpu_sample(pairs, seconds, steps=3):
     for prob, (d, i) in pairs:
         g, mats, run, n, _ = prob.describe()
         o = orc.Oracle(g, mats, run, d, i)
+        o.set_threads(threads)
         tc = time.perf_counter()
         o.step(steps)
         t += time.perf_counter() - tc
@@ -185,10 +207,11 @@ def bench_batch(a):
 
     cpu = None
     if not a.no_cpu:
-        v, done, ps, t = cpu_sample(pairs, a.cpu_seconds)
-        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"3 steps of each of the first {done} sweep instances ({ps} particle-steps) on one "
-                         f"host core (oracle/mpm_oracle.cpp, serial), {t:.1f} s"}
+        th = cpu_threads(a.cpu_threads)
+        v, done, ps, t = cpu_sample(pairs, a.cpu_seconds, threads=th)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": th, "kind": "port",
+               "sample": f"3 steps of each of the first {done} sweep instances ({ps} particle-steps), "
+                         f"oracle/mpm_oracle.cpp in its threaded mode on {th} threads of {cpu_model()}, {t:.1f} s"}
     return {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True,
@@ -286,10 +309,11 @@ def bench_serve(a):
     cpu = None
     if not a.no_cpu:
         pairs = [(r["prob"], (r["d"], r["i"])) for r in sorted(reqs, key=lambda r: r["n"])]
-        v, done, ps, t = cpu_sample(pairs, a.cpu_seconds, steps=2)
-        cpu = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"2 steps of each of the {done} smallest requests ({ps} particle-steps) on one host "
-                         f"core (oracle/mpm_oracle.cpp, serial), {t:.1f} s"}
+        th = cpu_threads(a.cpu_threads)
+        v, done, ps, t = cpu_sample(pairs, a.cpu_seconds, steps=2, threads=th)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": th, "kind": "port",
+               "sample": f"2 steps of each of the {done} smallest requests ({ps} particle-steps), "
+                         f"oracle/mpm_oracle.cpp in its threaded mode on {th} threads of {cpu_model()}, {t:.1f} s"}
     return {
         "metric": METRIC, "value": served, "unit": "particle-steps/s", "n_gpus": 1, "steps": int(total_ps),
         "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -318,7 +342,9 @@ def main():
                     help="timed steps (default: 500, the J2 block's run length; contact 200; batch 100)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=3, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="least steps of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (default: OMP_NUM_THREADS, else the cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["j2", "contact", "batch", "serve"], default="j2",
                     help="j2: the 7.08M-particle J2 GIMP block (default); contact: the lCPDI contact pair; "
@@ -329,7 +355,7 @@ def main():
     ap.add_argument("--max-resident", type=int, default=0, help="serve: most solves resident at once (0: no limit)")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of sweep instances")
     ap.add_argument("--serial-steps", type=int, default=20, help="batch: steps of each instance run alone")
-    ap.add_argument("--cpu-seconds", type=float, default=10., help="batch: CPU baseline sample length")
+    ap.add_argument("--cpu-seconds", type=float, default=10., help="CPU baseline sample length (wall seconds)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy used to scale the HBM byte counters")
     a = ap.parse_args()
@@ -418,13 +444,21 @@ def main():
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
+        th = cpu_threads(a.cpu_threads)
         o = orc.Oracle(grid, mats, run, dfields, ifields)
-        tc = time.perf_counter()
-        o.step(a.cpu_steps)
-        cpu_s = time.perf_counter() - tc
-        cpu = {"value": npart * a.cpu_steps / cpu_s, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{a.cpu_steps} step(s) of the same {npart}-particle block on one host core "
-                         f"(oracle/mpm_oracle.cpp, serial), {cpu_s:.1f} s"}
+        o.set_threads(th)
+        # steps of the same block until the sample has run about --cpu-seconds (at least one)
+        cpu_steps = cpu_s = 0
+        while cpu_steps < max(1, a.cpu_steps) or cpu_s < a.cpu_seconds:
+            tc = time.perf_counter()
+            o.step(1)
+            cpu_s += time.perf_counter() - tc
+            cpu_steps += 1
+            if cpu_steps >= 200:
+                break
+        cpu = {"value": npart * cpu_steps / cpu_s, "unit": "particle-steps/s", "cores": th, "kind": "port",
+               "sample": f"{cpu_steps} step(s) of the same {npart}-particle block, oracle/mpm_oracle.cpp in its "
+                         f"threaded mode on {th} threads of {cpu_model()}, {cpu_s:.1f} s"}
         del o
 
     out = {

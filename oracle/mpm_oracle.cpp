@@ -14,6 +14,7 @@
  * Pinned by tests/test_oracle_*.py against closed forms, invariants and brute force (there are
  * no reference golden vectors: the reference needs Xerces-C, absent from this image).
  */
+#include <omp.h>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -371,6 +372,17 @@ struct Oracle {
     double mtime = 0.;
     std::string err;
 
+    // Threaded timing mode (nthreads > 1, the CPU baseline only; parity checks run serial):
+    // particle loops split into contiguous per-thread ranges; the scatters add into per-thread
+    // copies of the node arrays, summed into the shared arrays in thread order afterwards, as the
+    // reference's patches sum their ghost nodes after each scatter (GridPatch.cpp:143-195). The
+    // grid sums then differ from the serial ones in rounding only.
+    int nthreads = 1;
+    struct Acc { double *mass, *pk, *ftot, *cvol, *cdisp, *cgrad; int *npts; };
+    enum { ACC_MASS = 1, ACC_PK = 2, ACC_FTOT = 4, ACC_NPTS = 8, ACC_CONTACT = 16 };
+    std::vector<std::vector<double>> tb;   // per thread: mass, pk(3), ftot(3), cvol, cdisp(3), cgrad(3)
+    std::vector<std::vector<int>> tnp;
+
     // lCPDI corners of the step (MPMBase CPDIDomain: inElem, ncpos, wg; ws = 1/8 or 1/4)
     struct Cpdi { long elem[8]; double xi[8][3]; double wg[8][3]; };
     std::vector<Cpdi> cpdi;
@@ -529,6 +541,70 @@ struct Oracle {
         o.n = ind;
     }
 
+    Acc shared_acc() { return {mass.data(), pk.data(), ftot.data(), cvol.data(), cdisp.data(), cgrad.data(), npts.data()}; }
+
+    // body(p, acc) for every particle, its grid sums going into acc (the `which` arrays)
+    template <class Body>
+    void scatter(int which, Body body) {
+        if (nthreads <= 1) {
+            Acc a = shared_acc();
+            for (long p = 0; p < np; p++) body(p, a);
+            return;
+        }
+        const size_t nn = (size_t)nf * g.nnodes;
+        if ((int)tb.size() < nthreads) {
+            tb.resize(nthreads);
+            tnp.resize(nthreads);
+        }
+        int used = 1;
+#pragma omp parallel num_threads(nthreads)
+        {
+            const int t = omp_get_thread_num(), T = omp_get_num_threads();
+#pragma omp single
+            used = T;
+            std::vector<double> &b = tb[t];
+            if (b.size() != 14 * nn) b.assign(14 * nn, 0.);
+            if (tnp[t].size() != nn) tnp[t].assign(nn, 0);
+            Acc a = {b.data(), b.data() + nn, b.data() + 4 * nn, b.data() + 7 * nn, b.data() + 8 * nn,
+                     b.data() + 11 * nn, tnp[t].data()};
+            if (which & ACC_MASS) std::fill(a.mass, a.mass + nn, 0.);
+            if (which & ACC_PK) std::fill(a.pk, a.pk + 3 * nn, 0.);
+            if (which & ACC_FTOT) std::fill(a.ftot, a.ftot + 3 * nn, 0.);
+            if (which & ACC_NPTS) std::fill(a.npts, a.npts + nn, 0);
+            if (which & ACC_CONTACT) std::fill(a.cvol, a.cvol + 7 * nn, 0.);
+            const long p0 = np * t / T, p1 = np * (t + 1) / T;
+            for (long p = p0; p < p1; p++) body(p, a);
+        }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long i = 0; i < (long)nn; i++) {
+            for (int t = 0; t < used; t++) {
+                const double *b = tb[t].data();
+                if (which & ACC_MASS) mass[i] += b[i];
+                if (which & ACC_PK) for (int c = 0; c < 3; c++) pk[3 * i + c] += b[nn + 3 * i + c];
+                if (which & ACC_FTOT) for (int c = 0; c < 3; c++) ftot[3 * i + c] += b[4 * nn + 3 * i + c];
+                if (which & ACC_NPTS) npts[i] += tnp[t][i];
+                if (which & ACC_CONTACT) {
+                    cvol[i] += b[7 * nn + i];
+                    for (int c = 0; c < 3; c++) {
+                        cdisp[3 * i + c] += b[8 * nn + 3 * i + c];
+                        cgrad[3 * i + c] += b[11 * nn + 3 * i + c];
+                    }
+                }
+            }
+        }
+    }
+
+    // body(p) for every particle; bodies that write only particle p's words
+    template <class Body>
+    void particles(Body body) {
+        if (nthreads <= 1) {
+            for (long p = 0; p < np; p++) body(p);
+            return;
+        }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (long p = 0; p < np; p++) body(p);
+    }
+
     void shape(long p, bool deriv, ShapeOut &o) {
         if (run.shape == MPMGPU_SHAPE_LCPDI) { cpdi_shape(p, deriv, o); return; }
         long e = I(MPI_ELEM, p);
@@ -541,25 +617,32 @@ struct Oracle {
 
     // InitializationTask::Execute (InitializationTask.cpp:43-85): zero the grid, find xi
     void task_initialize() {
-        std::fill(mass.begin(), mass.end(), 0.);
-        std::fill(pk.begin(), pk.end(), 0.);
-        std::fill(ftot.begin(), ftot.end(), 0.);
-        std::fill(vk.begin(), vk.end(), 0.);
-        std::fill(pkc.begin(), pkc.end(), 0.);
-        std::fill(npts.begin(), npts.end(), 0);
+        const long nn = (long)mass.size();
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
+        for (long n = 0; n < nn; n++) {
+            mass[n] = 0.;
+            npts[n] = 0;
+            for (int c = 0; c < 3; c++) pk[3 * n + c] = ftot[3 * n + c] = vk[3 * n + c] = pkc[3 * n + c] = 0.;
+        }
         std::fill(cvol.begin(), cvol.end(), 0.);
         std::fill(cdisp.begin(), cdisp.end(), 0.);
         std::fill(cgrad.begin(), cgrad.end(), 0.);
         contactNodes.clear();
         if (run.shape == MPMGPU_SHAPE_LCPDI && (long)cpdi.size() != np) cpdi.assign(np, Cpdi());
-        for (long p = 0; p < np; p++) {
+        long bad = 0;
+        particles([&](long p) {
             // ElementBase::GetShapeFunctionData (MoreMPMElementBase.cpp:50-58)
             if (run.shape == MPMGPU_SHAPE_LCPDI && !cpdi_init(p)) {
-                err = "A CPDI particle domain node has left the grid";
-                throw std::runtime_error(err);
+#pragma omp atomic
+                bad++;
+                return;
             }
             double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
             get_xi(g, I(MPI_ELEM, p), pos, &xi[3 * p]);
+        });
+        if (bad) {
+            err = "A CPDI particle domain node has left the grid";
+            throw std::runtime_error(err);
         }
     }
 
@@ -567,21 +650,21 @@ struct Oracle {
     // (CrackVelocityField.cpp:395-402) and CrackVelocityFieldMulti::AddVolumeGradient
     // (CrackVelocityFieldMulti.cpp:108-112); displacement = pos - origpos, weights shape*Vp and
     // shape*mp, gradient weight Vp
-    void add_contact_terms(long n, double shape, double Vp, double mp, const double *d,
-                           double dNdx, double dNdy, double dNdz) {
+    static void add_contact_terms(Acc &a, long n, double shape, double Vp, double mp, const double *d,
+                                  double dNdx, double dNdy, double dNdz) {
         double wtVolume = shape * Vp;
-        cvol[n] += wtVolume;
+        a.cvol[n] += wtVolume;
         double fnmp = shape * mp;
-        for (int c = 0; c < 3; c++) cdisp[3 * n + c] += d[c] * fnmp;
-        cgrad[3 * n] += dNdx * Vp; cgrad[3 * n + 1] += dNdy * Vp; cgrad[3 * n + 2] += dNdz * Vp;
+        for (int c = 0; c < 3; c++) a.cdisp[3 * n + c] += d[c] * fnmp;
+        a.cgrad[3 * n] += dNdx * Vp; a.cgrad[3 * n + 1] += dNdy * Vp; a.cgrad[3 * n + 2] += dNdz * Vp;
     }
 
     // MassAndMomentumTask::Execute (MassAndMomentumTask.cpp:48-127) ->
     // NodalPoint::AddMassMomentum (NodalPointMPM.cpp:419-453); multimaterial mode extrapolates
     // with gradients because volumeGradientIndex >= 0 (MassAndMomentumTask.cpp:137-140)
     void task_mass_momentum() {
-        ShapeOut o;
-        for (long p = 0; p < np; p++) {
+        scatter(ACC_MASS | ACC_PK | ACC_NPTS | (mm() ? ACC_CONTACT : 0), [&](long p, Acc &a) {
+            ShapeOut o;
             shape(p, mm(), o);
             int f = field(p);
             double mp = F(MPF_MASS, p);
@@ -594,28 +677,27 @@ struct Oracle {
             for (int i = 0; i < o.n; i++) {
                 long n = fnode(f, o.node[i]);
                 double fnmp = o.fn[i] * mp;
-                for (int c = 0; c < 3; c++) pk[3 * n + c] += fnmp * v[c];   // AddMomentumTask1
-                npts[n] += 1;
-                if (mm()) add_contact_terms(n, o.fn[i], Vp, mp, d, o.dx[i], o.dy[i], g.dim == 3 ? o.dz[i] : 0.);
-                mass[n] += fnmp;                                             // AddMass
+                for (int c = 0; c < 3; c++) a.pk[3 * n + c] += fnmp * v[c];   // AddMomentumTask1
+                a.npts[n] += 1;
+                if (mm()) add_contact_terms(a, n, o.fn[i], Vp, mp, d, o.dx[i], o.dy[i], g.dim == 3 ? o.dz[i] : 0.);
+                a.mass[n] += fnmp;                                             // AddMass
             }
-        }
+        });
     }
 
     // PostExtrapolationTask::Execute (PostExtrapolationTask.cpp:46-165): CalcTotalMassAndCount
     // copies pk (MatVelocityField.cpp:158-164) and flags nodes with two or more materials
     // (CrackVelocityFieldMulti.cpp:1620-1637); then contact at MASS_MOMENTUM_CALL (:155)
     void task_post_extrapolation() {
-        for (long n = 0; n < g.nnodes; n++) {
-            int numberMaterials = 0;
-            for (int f = 0; f < nf; f++) {
-                long k = fnode(f, n);
-                if (npts[k] >= 1) {
-                    numberMaterials++;
-                    for (int c = 0; c < 3; c++) pkc[3 * k + c] = pk[3 * k + c];
-                }
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
+        for (long k = 0; k < (long)nf * g.nnodes; k++)
+            if (npts[k] >= 1) for (int c = 0; c < 3; c++) pkc[3 * k + c] = pk[3 * k + c];
+        if (mm()) {
+            for (long n = 0; n < g.nnodes; n++) {
+                int numberMaterials = 0;
+                for (int f = 0; f < nf; f++) numberMaterials += npts[fnode(f, n)] >= 1 ? 1 : 0;
+                if (numberMaterials > 1) contactNodes.push_back(n);
             }
-            if (numberMaterials > 1) contactNodes.push_back(n);
         }
         material_contact(CALL_MASS_MOMENTUM);
     }
@@ -796,6 +878,7 @@ struct Oracle {
 
     // MatVelocityField::GridValueCalculation (MatVelocityField.cpp:239-251)
     void grid_velocity() {
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++) {
             if (npts[n] == 0 || mass[n] == 0.) continue;
             double s = 1. / mass[n];
@@ -806,8 +889,8 @@ struct Oracle {
     // UpdateStrainsFirstTask::FullStrainUpdate (UpdateStrainsFirstTask.cpp:101-168)
     void full_strain_update(double strainTime) {
         grid_velocity();
-        ShapeOut o;
-        for (long p = 0; p < np; p++) {
+        particles([&](long p) {
+            ShapeOut o;
             shape(p, true, o);
             long off = fnode(field(p), 0);
             Mat3 dv;
@@ -827,7 +910,7 @@ struct Oracle {
             }
             dv.scale(strainTime);
             constitutive(p, dv);
-        }
+        });
     }
 
     // ---- constitutive laws ------------------------------------------------------------
@@ -1133,8 +1216,8 @@ struct Oracle {
     // GridForcesTask::Execute (GridForcesTask.cpp:40-153) -> GetFintPlusFext
     // (MatPoint3D.cpp:248-270, MatPoint2D.cpp:242-263); pFext is zero (no particle loads)
     void task_grid_forces() {
-        ShapeOut o;
-        for (long p = 0; p < np; p++) {
+        scatter(ACC_FTOT, [&](long p, Acc &a) {
+            ShapeOut o;
             shape(p, true, o);
             double mp = F(MPF_MASS, p), P = F(MPF_PRESSURE, p);
             double sxx = F(MPF_SP_XX, p), syy = F(MPF_SP_YY, p), szz = F(MPF_SP_ZZ, p);
@@ -1152,13 +1235,14 @@ struct Oracle {
                     fy = -mp * (sxy * o.dx[i] + (syy - P) * o.dy[i]);
                     fz = 0.0;
                 }
-                ftot[3 * n] += fx; ftot[3 * n + 1] += fy; ftot[3 * n + 2] += fz;   // AddFtotTask3
+                a.ftot[3 * n] += fx; a.ftot[3 * n + 1] += fy; a.ftot[3 * n + 2] += fz;   // AddFtotTask3
             }
-        }
+        });
     }
 
     // PostForcesTask::Execute (PostForcesTask.cpp:43-111) -> RestoreMomenta (MatVelocityField.cpp:177)
     void task_post_forces() {
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = pkc[3 * n + c];
     }
@@ -1166,6 +1250,7 @@ struct Oracle {
     // UpdateMomentaTask::Execute (UpdateMomentaTask.cpp:44-64) -> UpdateMomentum (MatVelocityField.cpp:301-314)
     void task_update_momenta() {
         double dt = run.timestep;
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] += ftot[3 * n + c] * dt;
         material_contact(CALL_UPDATE_MOMENTUM);          // ContactAndMomentaBCs (UpdateMomentaTask.cpp:58)
@@ -1176,8 +1261,8 @@ struct Oracle {
     void task_update_particles() {
         grid_velocity();
         double dt = run.timestep;
-        ShapeOut o;
-        for (long p = 0; p < np; p++) {
+        particles([&](long p) {
+            ShapeOut o;
             shape(p, false, o);
             long off = fnode(field(p), 0);
             double Svk[3] = {0., 0., 0.}, Sacc[3] = {0., 0., 0.};
@@ -1195,12 +1280,13 @@ struct Oracle {
                 double delXRate = vm + 0.5 * delV;
                 F(MPF_POS_X + c, p) += delXRate * dt;
             }
-        }
+        });
     }
 
     // UpdateStrainsLastContactTask::Execute (UpdateStrainsLastContactTask.cpp:60-152):
     // RezeroNodeTask6 + AddMassMomentumLast (NodalPointMPM.cpp:470-486) with start-of-step xi
     void task_update_strains_last() {
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) {                           // MatVelocityField::RezeroNodeTask6 (:429-444)
                 for (int c = 0; c < 3; c++) pk[3 * n + c] = 0.;
@@ -1209,8 +1295,8 @@ struct Oracle {
                     for (int c = 0; c < 3; c++) { cdisp[3 * n + c] = 0.; cgrad[3 * n + c] = 0.; }
                 }
             }
-        ShapeOut o;
-        for (long p = 0; p < np; p++) {
+        scatter(ACC_PK | (mm() ? ACC_CONTACT : 0), [&](long p, Acc &a) {
+            ShapeOut o;
             shape(p, mm(), o);
             int f = field(p);
             double mp = F(MPF_MASS, p);
@@ -1223,10 +1309,10 @@ struct Oracle {
             for (int i = 0; i < o.n; i++) {
                 long n = fnode(f, o.node[i]);
                 double fnmp = o.fn[i] * mp;
-                for (int c = 0; c < 3; c++) pk[3 * n + c] += v[c] * fnmp;   // AddMomentumTask6
-                if (mm()) add_contact_terms(n, o.fn[i], Vp, mp, d, o.dx[i], o.dy[i], g.dim == 3 ? o.dz[i] : 0.);
+                for (int c = 0; c < 3; c++) a.pk[3 * n + c] += v[c] * fnmp;   // AddMomentumTask6
+                if (mm()) add_contact_terms(a, n, o.fn[i], Vp, mp, d, o.dx[i], o.dy[i], g.dim == 3 ? o.dz[i] : 0.);
             }
-        }
+        });
         material_contact(CALL_UPDATE_STRAINS_LAST);      // UpdateStrainsLastContactTask.cpp:145
         full_strain_update(run.strainLast);
     }
@@ -1237,11 +1323,15 @@ struct Oracle {
         const MpmGridDesc &d = g.d;
         long horiz = d.nx, vert = d.ny, depth = g.dim == 3 ? d.nz : 1;
         long totalElems = horiz * vert * depth;
-        for (long p = 0; p < np; p++) {
+        particles([&](long p) {
             double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
-            if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) { nanCount++; continue; }
+            if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
+#pragma omp atomic
+                nanCount++;
+                return;
+            }
             long e = I(MPI_ELEM, p);
-            if (pt_in_element(g, e, pos)) continue;
+            if (pt_in_element(g, e, pos)) return;
             bool left = false;
             long col = (long)(int)((pos[0] - d.xmin) / d.dx);
             if (col < 0 || col >= horiz) {
@@ -1270,8 +1360,12 @@ struct Oracle {
                     else if (num % horiz <= 1) left = true;
                 }
             }
-            if (!left) { I(MPI_ELEM, p) = (int32_t)(num - 1); continue; }
-            if (!I(MPI_LEFT, p)) { leftGrid++; I(MPI_LEFT, p) = 1; }
+            if (!left) { I(MPI_ELEM, p) = (int32_t)(num - 1); return; }
+            if (!I(MPI_LEFT, p)) {
+#pragma omp atomic
+                leftGrid++;
+                I(MPI_LEFT, p) = 1;
+            }
             // ReturnToElement
             double dt = run.timestep;
             double outside[3] = {pos[0], pos[1], pos[2]};
@@ -1291,7 +1385,7 @@ struct Oracle {
             }
             F(MPF_POS_X, p) = inside[0]; F(MPF_POS_Y, p) = inside[1];
             if (g.dim == 3) F(MPF_POS_Z, p) = inside[2];
-        }
+        });
     }
 
     // NairnMPM::MPMStep (NairnMPM.cpp:284-335) with the task list of CreateTasks (:870-1110)
@@ -1374,6 +1468,12 @@ int oracle_get_particles(void *h, double *dfields, int32_t *ifields) {
     Oracle *o = (Oracle *)h;
     memcpy(dfields, o->pd.data(), o->pd.size() * sizeof(double));
     if (ifields) memcpy(ifields, o->pi.data(), o->pi.size() * sizeof(int32_t));
+    return 0;
+}
+
+// threads of the timing mode (1: the serial oracle the parity checks use)
+int oracle_set_threads(void *h, int32_t n) {
+    ((Oracle *)h)->nthreads = n < 1 ? 1 : n;
     return 0;
 }
 

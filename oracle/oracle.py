@@ -38,6 +38,7 @@ def lib():
         _lib.oracle_destroy.argtypes = [_P]
         _lib.oracle_step.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_run_task.argtypes = [_P, ctypes.c_int32]
+        _lib.oracle_set_threads.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
         _lib.oracle_num_nodes.argtypes = [_P]
         _lib.oracle_num_nodes.restype = ctypes.c_int64
@@ -76,6 +77,12 @@ class Oracle:
         g, mats, run, _, _ = prob.describe()
         d, i = prob.particles()
         return cls(g, mats, run, d, i)
+
+    def set_threads(self, n):
+        """Threads of the timing mode (the CPU baseline): particle loops split over n threads and
+        the scatters summed through per-thread grid copies. 1 (the default) is the serial
+        restatement that the parity checks use; n > 1 differs from it in grid-sum rounding."""
+        lib().oracle_set_threads(self._h, int(n))
 
     def step(self, n=1):
         if lib().oracle_step(self._h, n) != 0:

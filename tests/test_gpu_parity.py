@@ -112,6 +112,25 @@ def test_deterministic_rerun():
     assert np.array_equal(runs[0], runs[1])
 
 
+def test_long_profiled_run_counts_every_launch_and_matches():
+    """Profiling mode records events between kernel groups; a run longer than the live-event
+    limit folds them in on the way (several times here) without losing a launch, and the
+    profiled (eager) run gives the same bits as a graph-replayed one."""
+    xml = configs.elastic_block_2d(cells=24, block=12)
+    prob = nm.Problem.from_xml(xml)
+    a = nm.Simulation.from_problem(prob)
+    b = nm.Simulation.from_problem(prob)
+    a.set_profiling(True)
+    a.step(300)
+    times = a.kernel_times()
+    a.set_profiling(False)
+    b.step(300)
+    for g in ("k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_g2p_p2g", "k_usl_reset", "binning"):
+        assert times[g][1] == 300, (g, times[g])
+        assert times[g][0] > 0.
+    assert np.array_equal(a.particles()[0], b.particles()[0])
+
+
 def test_graph_replay_matches_eager():
     xml = configs.elastic_block_2d(cells=24, block=12)
     prob = nm.Problem.from_xml(xml)
