@@ -195,7 +195,27 @@ int mpmgpu_problem_describe(const MpmProblem *prob, MpmGridDesc *grid, MpmMateri
 /* Initial particle state in mpm[] order: dfields is MPF_NUM_FIELDS*n doubles, ifields is
    MPI_NUM_FIELDS*n int32 */
 int mpmgpu_problem_particles(const MpmProblem *prob, double *dfields, int32_t *ifields);
+/* Archive settings of a problem: the MPM archive order of <MPMArchiveOrder> and <MPMArchive>
+   (up to 25 characters and a NUL; "iY" when not given), <ArchiveTime> in s (< 0 when not given)
+   and each particle's thickness (2D). Replaces ArchiveData's XML settings
+   (Read_MPM/MPMReadHandler.cpp:227-240,654-800). */
+int mpmgpu_problem_archive(const MpmProblem *prob, char *order, double *archiveTime, double *thickness);
 void mpmgpu_problem_destroy(MpmProblem *prob);
+
+/* ---- archive files (ArchiveData, System/ArchiveData.cpp), host only ----------------- */
+/* Normalises an MPM archive order as ArchiveData::CalcArchiveSize does (ArchiveData.cpp:266-283)
+   into out[26]; fails for items this implementation does not carry (energies, temperature,
+   concentration, element crossings, damage normal, spin, histories past the fourth, size). */
+int mpmgpu_archive_order(const char *mpmOrder, char *out);
+/* Record bytes of an order (CalcArchiveSize, :266-424); -1 if the order is not carried */
+int32_t mpmgpu_archive_record_size(const char *mpmOrder, int32_t dim);
+/* Writes a version-6 archive (header SetArchiveHeader :443-470, records ArchiveResults :721-1000)
+   of n particles given in the MPF_ and MPI_ row layout in mpm[] order; little endian. */
+int mpmgpu_archive_write(const char *path, int32_t dim, const char *mpmOrder, double timeMs, int64_t n,
+                         const double *dfields, const int32_t *ifields, const MpmMaterial *mats,
+                         int32_t nmat, const double *thickness, double defaultThickness);
+/* Message of the last failed archive call */
+const char *mpmgpu_archive_error(void);
 
 /* ---- device context (NairnMPM::CMPreparations/CMAnalysis state, NairnMPM.cpp:135-280) -- */
 
@@ -240,6 +260,11 @@ int mpmgpu_autotune(MpmCtx *ctx, int32_t trialSteps, int32_t *graphSteps, double
 int mpmgpu_sync(MpmCtx *ctx);
 /* Download particles in creation (mpm[]) order. */
 int mpmgpu_download_particles(MpmCtx *ctx, double *dfields, int32_t *ifields);
+/* Archive of the context's particles at timeMs (ms), NairnMPM's archive file format version 6
+   (ArchiveData::ArchiveResults, System/ArchiveData.cpp:721-1000); thickness per particle (2D) or
+   null for defaultThickness. Not for batches. */
+int mpmgpu_write_archive(MpmCtx *ctx, const char *path, const char *mpmOrder, double timeMs,
+                         const double *thickness, double defaultThickness);
 /* Counters and timings (MPMTask::TrackTimes, MPMTask.cpp:51-72; warnings). */
 int mpmgpu_query(MpmCtx *ctx, MpmStats *stats);
 /* Grid node values (mass, momentum, force) after the last task, nodes numbered x fastest; one

@@ -262,6 +262,21 @@ int mpmgpu_problem_particles(const MpmProblem *pc, double *dfields, int32_t *ifi
     return 0;
 }
 
+int mpmgpu_problem_archive(const MpmProblem *pc, char *order, double *archiveTime, double *thickness) {
+    MpmProblem *p = const_cast<MpmProblem *>(pc);
+    if (!mpmsetup::finalize(*p)) return fail_global(p->err);
+    if (order) {
+        std::string o = p->archiveOrder;
+        if (o.size() > 25) o.resize(25);
+        memcpy(order, o.c_str(), o.size() + 1);
+    }
+    if (archiveTime) *archiveTime = p->archiveTime;
+    if (thickness)
+        for (size_t i = 0; i < (size_t)p->nparticles(); i++)
+            thickness[i] = i < p->pthick.size() ? p->pthick[i] : p->thickness;
+    return 0;
+}
+
 void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 
 // ---- device context --------------------------------------------------------------------
@@ -832,6 +847,19 @@ int mpmgpu_sync(MpmCtx *c) {
     collect_marks(c);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->lastStepMs = ms;
+    return 0;
+}
+
+int mpmgpu_write_archive(MpmCtx *c, const char *path, const char *mpmOrder, double timeMs, const double *thickness,
+                         double defaultThickness) {
+    if (!c->inst.empty()) return c->fail("archives are written per simulation, not for a batch");
+    const size_t n = (size_t)c->kp.n;
+    std::vector<double> d((size_t)MPF_NUM_FIELDS * n);
+    std::vector<int32_t> iv((size_t)MPI_NUM_FIELDS * n);
+    if (n && mpmgpu_download_particles(c, d.data(), iv.data())) return 1;
+    if (mpmgpu_archive_write(path, c->dim, mpmOrder, timeMs, (int64_t)n, d.data(), iv.data(), c->mats.data(),
+                             (int32_t)c->mats.size(), thickness, defaultThickness))
+        return c->fail(mpmgpu_archive_error());
     return 0;
 }
 

@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                         s_st[i][0] = negmp * (P.f[MPF_SP_XX][p] - Pr);
                         s_st[i][1] = negmp * (P.f[MPF_SP_YY][p] - Pr);
                         s_st[i][2] = negmp * P.f[MPF_SP_XY][p];
-                        if (DIM == 3) {
+                        if constexpr (DIM == 3) {
                             s_st[i][3] = negmp * (P.f[MPF_SP_ZZ][p] - Pr);
                             s_st[i][4] = negmp * P.f[MPF_SP_XZ][p];
                             s_st[i][5] = negmp * P.f[MPF_SP_YZ][p];
@@ -1206,7 +1206,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                         r.dx = gr[0]; r.dy = gr[1]; r.dz = gr[2];
                     }
                     r.s0 = s_st[lq][0]; r.s1 = s_st[lq][1]; r.s2 = s_st[lq][2];
-                    if (DIM == 3) { r.s3 = s_st[lq][3]; r.s4 = s_st[lq][4]; r.s5 = s_st[lq][5]; }
+                    if constexpr (DIM == 3) { r.s3 = s_st[lq][3]; r.s4 = s_st[lq][4]; r.s5 = s_st[lq][5]; }
                     else { r.s3 = r.s4 = r.s5 = 0.; }
                     return r;
                 },

@@ -45,6 +45,8 @@ struct MpmProblem {
     int ptsPerSide = 2;              // MatlPtsPerElement (NairnMPM.cpp:98-99)
     bool ptsSet = false;
     std::string archiveRoot;
+    std::string archiveOrder = "iY";   // <MPMArchiveOrder>, <MPMArchive> items (ArchiveData defaults)
+    double archiveTime = -1.;          // <ArchiveTime> (s), < 0 if not given
     // <MultiMaterialMode> contact settings
     bool mmRequested = false;
     bool normalsGiven = false;

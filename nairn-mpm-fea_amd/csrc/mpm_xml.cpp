@@ -306,6 +306,43 @@ struct Reader {
                 }
             } else if (k.name == "ArchiveRoot") {
                 p.archiveRoot = k.text;
+            } else if (k.name == "MPMArchiveOrder") {
+                // ArchiveData::SetMPMOrder (ArchiveData.cpp:2131); normalised when an archive is written
+                std::string t;
+                for (char ch : k.text) if (!std::isspace((unsigned char)ch)) t.push_back(ch);
+                p.archiveOrder = t;
+            } else if (k.name == "MPMArchive") {
+                // <MPMArchive result="..." setting="Y|N"/> (MPMReadHandler.cpp:674-800): one item by
+                // name; history1-4 set or clear one bit of the history byte ('0' | bits, 'N' when none)
+                static const struct { const char *name; int item; } items[] = {
+                    {"velocity", 2}, {"stress", 3}, {"strain", 4}, {"rotstrain", 17}, {"plasticstrain", 5},
+                    {"strainenergy", 12}, {"workenergy", 7}, {"plasticenergy", 9}, {"heatenergy", 15},
+                    {"lp", 19}, {"wp", 20}, {"temperature", 8}, {"concentration", 14}, {"porepressure", 14},
+                    {"elementcrossings", 16}, {"damagenormal", 18}, {"size", 24}};
+                const char *r = k.attr("result");
+                const char *set = k.attr("setting");
+                const char setting = set && set[0] ? set[0] : 'Y';
+                if (!r) return fail("<MPMArchive> needs a result attribute");
+                std::string name;
+                for (const char *c = r; *c; c++) name.push_back((char)std::tolower((unsigned char)*c));
+                while ((int)p.archiveOrder.size() < 25) p.archiveOrder.push_back('N');
+                if (name.size() == 8 && name.compare(0, 7, "history") == 0 && name[7] >= '1' && name[7] <= '4') {
+                    char h = p.archiveOrder[13];
+                    if (h == 'Y') h = 0x31;
+                    else if (h == 'N') h = 0x30;
+                    const char bit = (char)(1 << (name[7] - '1'));
+                    if (setting == 'Y') h |= bit;
+                    else if (h & bit) h -= bit;
+                    p.archiveOrder[13] = h == 0x30 ? 'N' : h;
+                } else {
+                    int item = -1;
+                    for (const auto &it : items) if (name == it.name) item = it.item;
+                    if (name.compare(0, 7, "history") == 0) item = 21;   // histories past 4: not carried
+                    if (item < 0) return fail(std::string("<MPMArchive> result '") + r + "' is not an archive item here");
+                    p.archiveOrder[item] = setting;
+                }
+            } else if (k.name == "ArchiveTime") {
+                p.archiveTime = textnum(k) * time_units(k);
             } else if (k.name == "LeaveLimit") {
                 p.run.maxLeaving = (int)textnum(k);
             }

@@ -119,7 +119,16 @@ _SIGNATURES = {
     "mpmgpu_problem_describe": ([_P, ctypes.POINTER(GridDesc), ctypes.POINTER(Material), _PI,
                                  ctypes.POINTER(RunParams), ctypes.POINTER(ctypes.c_int64), _PD], ctypes.c_int),
     "mpmgpu_problem_particles": ([_P, _PD, _PI], ctypes.c_int),
+    "mpmgpu_problem_archive": ([_P, ctypes.c_char_p, _PD, _PD], ctypes.c_int),
     "mpmgpu_problem_destroy": ([_P], None),
+    "mpmgpu_archive_order": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
+    "mpmgpu_archive_record_size": ([ctypes.c_char_p, ctypes.c_int32], ctypes.c_int32),
+    "mpmgpu_archive_write": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_double, ctypes.c_int64,
+                              _PD, _PI, ctypes.POINTER(Material), ctypes.c_int32, _PD, ctypes.c_double],
+                             ctypes.c_int),
+    "mpmgpu_archive_error": ([], ctypes.c_char_p),
+    "mpmgpu_write_archive": ([_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, _PD, ctypes.c_double],
+                             ctypes.c_int),
     "mpmgpu_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Material), ctypes.c_int32,
                        ctypes.POINTER(RunParams), ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_upload_particles": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
@@ -253,6 +262,16 @@ class Problem:
         _check(library().mpmgpu_problem_particles(self._h, _dptr(d), _iptr(i)))
         return d, i
 
+    def archive_settings(self):
+        """(MPM archive order as given, archive interval in s or None, per-particle thickness)
+        from <MPMArchiveOrder>, <MPMArchive> and <ArchiveTime>"""
+        _, _, _, n, _ = self.describe()
+        order = ctypes.create_string_buffer(32)
+        t = ctypes.c_double()
+        thick = np.zeros(n, dtype=np.float64)
+        _check(library().mpmgpu_problem_archive(self._h, order, ctypes.byref(t), _dptr(thick)))
+        return order.value.decode(), (t.value if t.value >= 0. else None), thick
+
     def close(self):
         if self._h:
             library().mpmgpu_problem_destroy(self._h)
@@ -319,6 +338,14 @@ class Simulation:
         i = np.zeros((NUM_IFIELDS, self.n), dtype=np.int32)
         _check(library().mpmgpu_download_particles(self._h, _dptr(d), _iptr(i)), self._h)
         return d, i
+
+    def write_archive(self, path, order: str = "iY", time_ms: float = 0., thickness=None,
+                      default_thickness: float = 1.):
+        """NairnMPM archive file (format version 6) of the current particles; read it back with
+        nairnmpm.archive.read"""
+        t = None if thickness is None else np.ascontiguousarray(thickness, dtype=np.float64)
+        _check(library().mpmgpu_write_archive(self._h, os.fsencode(path), order.encode(), float(time_ms),
+                                              None if t is None else _dptr(t), float(default_thickness)), self._h)
 
     def _nodes(self):
         g = self.grid
