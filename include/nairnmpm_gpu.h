@@ -200,6 +200,26 @@ int mpmgpu_problem_particles(const MpmProblem *prob, double *dfields, int32_t *i
    and each particle's thickness (2D). Replaces ArchiveData's XML settings
    (Read_MPM/MPMReadHandler.cpp:227-240,654-800). */
 int mpmgpu_problem_archive(const MpmProblem *prob, char *order, double *archiveTime, double *thickness);
+
+/* One grid velocity condition on one node (NodalVelBC, Boundary_Conditions/NodalVelBC.hpp:17-72):
+   node index (0-based, x fastest), axis dir (1 x, 2 y, 3 z), style (BoundaryCondition styles:
+   1 constant, 2 linear, 3 sine, 4 cosine), value in mm/s (mm/s^2 for linear) and ftime in s
+   (styles 1-2: start time) or 1/s (styles 3-4: angular frequency). The value at step start time
+   t is value (t >= ftime), value*(t-ftime) (t >= ftime), value*sin(t*ftime) or
+   value*cos(t*ftime); styles 1-2 leave the node free before ftime
+   (BoundaryCondition::BCValue and GetNodeNum, BoundaryCondition.cpp:72-140). */
+typedef struct MpmVelocityBC {
+    int64_t node;
+    int32_t dir;
+    int32_t style;
+    double value;
+    double ftime;
+} MpmVelocityBC;
+/* The velocity conditions of <GridBCs> (<BCLine> in 2D, <BCBox> in 3D, each holding <DisBC>),
+   in the reference's list order: BC by BC in document order, nodes in node order
+   (MPMReadHandler BCLine/BCBox/DisBC, Read_MPM/Generators.cpp:831-1010). Returns the count in
+   *n; copies min(*n on entry, count) entries when bcs is not null. */
+int mpmgpu_problem_velocity_bcs(const MpmProblem *prob, MpmVelocityBC *bcs, int64_t *n);
 void mpmgpu_problem_destroy(MpmProblem *prob);
 
 /* ---- archive files (ArchiveData, System/ArchiveData.cpp), host only ----------------- */
@@ -243,6 +263,14 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *inst, const MpmMat
 /* Upload particles (layout as mpmgpu_problem_particles); replaces the mpm[] globals
    (MPM_Classes/MPMBase.hpp:269-270). */
 int mpmgpu_upload_particles(MpmCtx *ctx, int64_t n, const double *dfields, const int32_t *ifields);
+/* Grid velocity conditions of a single simulation (not a batch); n = 0 removes them. Each step
+   imposes them as NodalVelBC::GridVelocityConditions does (NodalVelBC.cpp:321-400): after the
+   mass-and-momentum contact (the strain update's velocities; the force pass starts again from the
+   momenta before it, MatVelocityField::RestoreMomenta), on the forces before the momentum update,
+   after the momentum update and after the second extrapolation, each pass zeroing the controlled
+   component of every field with points, then adding mass * value. Values are taken at the step's
+   start time, counted from the time of this call's step count (mpmgpu_query's time). */
+int mpmgpu_set_velocity_bcs(MpmCtx *ctx, int64_t n, const MpmVelocityBC *bcs);
 /* Run one hot-path MPMTask's Execute() on the device (MPMGPU_TASK_*), in reference order. */
 int mpmgpu_run_task(MpmCtx *ctx, int32_t task);
 /* Run nsteps whole steps (NairnMPM::MPMStep, NairnMPM.cpp:284-335), fused and graph-replayed. */

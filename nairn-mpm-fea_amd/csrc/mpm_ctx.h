@@ -53,6 +53,11 @@ struct MpmCtx {
     int nextTask = 0;                // task sequence check for mpmgpu_run_task
     int64_t steps = 0;
     double mtime = 0.;
+    // grid velocity conditions (kbc.nb = 0: none) in one device buffer; bcTime is the step start
+    // time as the device counts it (one addition of the time step per step)
+    KBcs kbc{};
+    char *bcbuf = nullptr;
+    double bcTime = 0.;
     float lastStepMs = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // kernel timing: events around each kernel group of a step, on this context's stream

@@ -3,6 +3,7 @@
 // stream (default priority), and whole steps are replayed from a captured HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -34,6 +35,14 @@ static void destroy_graphs(MpmCtx *c) {
         if (c->graph[i]) { (void)hipGraphExecDestroy(c->graph[i]); c->graph[i] = nullptr; }
 }
 
+// the device's copy of the step start time of the velocity conditions (none: nothing to do)
+static int put_bc_time(MpmCtx *c) {
+    if (!c->kbc.nb) return 0;
+    HIPCHECK(c, hipMemcpyAsync(c->kbc.time, &c->bcTime, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108); with
 // profiling on, an event is recorded between kernel groups
 static void enqueue_step(MpmCtx *c) {
@@ -49,12 +58,19 @@ static void enqueue_step(MpmCtx *c) {
     const bool mm = c->kg.nfields > 1;
     launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
     done(MPMGPU_KG_P2G_MASS);
+    // grid velocity conditions follow each contact pass (UpdateMomentaTask::ContactAndMomentaBCs)
+    const bool bcs = c->kbc.nb > 0;
+    auto vel_bcs = [&](int pass) {
+        if (bcs) launch_vel_bcs(c->kg, c->kn, c->kbc, pass, s);
+    };
     launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
+    if (!mm) vel_bcs(BC_PASS_MASS_MOMENTUM);
     done(MPMGPU_KG_NODE_MASS);
     if (mm) {      // PostExtrapolationTask: contact at MASS_MOMENTUM_CALL
         launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, false, s);
         launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
         launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
+        vel_bcs(BC_PASS_MASS_MOMENTUM);
         done(MPMGPU_KG_CONTACT);
     }
     launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
@@ -62,19 +78,23 @@ static void enqueue_step(MpmCtx *c) {
     launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s);
     done(MPMGPU_KG_P2G_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
+    vel_bcs(mm ? BC_PASS_GRID_FORCES : BC_PASS_FORCES_MOMENTA);
     done(MPMGPU_KG_NODE_FORCE);
     if (mm) {      // UpdateMomentaTask: contact at UPDATE_MOMENTUM_CALL
         launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
+        vel_bcs(BC_PASS_UPDATE_MOMENTUM);
         done(MPMGPU_KG_CONTACT);
     }
     launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s);
     done(MPMGPU_KG_G2P_P2G);
     launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
+    if (!mm) vel_bcs(BC_PASS_UPDATE_STRAINS_LAST);
     done(MPMGPU_KG_NODE_MOM2);
     if (mm) {      // UpdateStrainsLastContactTask: contact at UPDATE_STRAINS_LAST_CALL
         launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, true, s);
         launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
         launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
+        vel_bcs(BC_PASS_UPDATE_STRAINS_LAST);
         done(MPMGPU_KG_CONTACT);
     }
     launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
@@ -274,6 +294,19 @@ int mpmgpu_problem_archive(const MpmProblem *pc, char *order, double *archiveTim
     if (thickness)
         for (size_t i = 0; i < (size_t)p->nparticles(); i++)
             thickness[i] = i < p->pthick.size() ? p->pthick[i] : p->thickness;
+    return 0;
+}
+
+int mpmgpu_problem_velocity_bcs(const MpmProblem *pc, MpmVelocityBC *bcs, int64_t *n) {
+    MpmProblem *p = const_cast<MpmProblem *>(pc);
+    if (!n) return fail_global("null count");
+    if (!mpmsetup::finalize(*p)) return fail_global(p->err);
+    const int64_t have = (int64_t)p->velBCs.size();
+    if (bcs) {
+        const int64_t k = std::min(*n, have);
+        for (int64_t i = 0; i < k; i++) bcs[i] = p->velBCs[i];
+    }
+    *n = have;
     return 0;
 }
 
@@ -599,7 +632,8 @@ int mpmi::prepare_particles(MpmCtx *c) {
     c->nextTask = 0;
     c->steps = 0;
     c->mtime = 0.;
-    return 0;
+    c->bcTime = 0.;
+    return put_bc_time(c);
 }
 
 extern "C" {
@@ -676,6 +710,75 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
     return mpmi::prepare_particles(c);
 }
 
+int mpmgpu_set_velocity_bcs(MpmCtx *c, int64_t n, const MpmVelocityBC *bcs) {
+    if (n < 0 || n > 0x7fffffffLL) return c->fail("bad velocity condition count");
+    if (n > 0 && !c->inst.empty()) return c->fail("velocity conditions are for single simulations, not batches");
+    if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
+    const long long nnodes = c->kg.nnodes;
+    for (int64_t i = 0; i < n; i++) {
+        if (bcs[i].node < 0 || bcs[i].node >= nnodes) return c->fail("velocity condition on a node outside the grid");
+        if (bcs[i].dir < 1 || bcs[i].dir > c->dim) return c->fail("velocity condition direction must be 1 to the dimension");
+        if (bcs[i].style < 1 || bcs[i].style > 4)
+            return c->fail("velocity condition styles other than 1 to 4 are not on this path");
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    destroy_graphs(c);                    // captured steps hold the old launches
+    if (c->bcbuf) { HIPCHECK(c, hipFree(c->bcbuf)); c->bcbuf = nullptr; }
+    c->kbc = KBcs{};
+    if (n == 0) return 0;
+    // group by node, list order kept within a node
+    std::vector<int64_t> ord((size_t)n);
+    for (int64_t i = 0; i < n; i++) ord[(size_t)i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return bcs[a].node < bcs[b].node; });
+    std::vector<long long> node;
+    std::vector<int> start, dir, style;
+    std::vector<double> value, ftime;
+    for (size_t k = 0; k < ord.size(); k++) {
+        const MpmVelocityBC &b = bcs[ord[k]];
+        if (node.empty() || node.back() != b.node) {
+            node.push_back(b.node);
+            start.push_back((int)k);
+        }
+        dir.push_back(b.dir - 1);
+        style.push_back(b.style);
+        value.push_back(b.value);
+        ftime.push_back(b.ftime);
+    }
+    start.push_back((int)n);
+    const size_t nb = node.size(), ne = (size_t)n, nf = (size_t)c->kg.nfields;
+    // one buffer: doubles first (value, ftime, cur, save, time), then node indices, then ints
+    const size_t nd = 3 * ne + 3 * nb * nf + 1;
+    const size_t bytes = nd * sizeof(double) + nb * sizeof(long long) + (nb + 1 + 3 * ne) * sizeof(int);
+    HIPCHECK(c, hipMalloc(&c->bcbuf, bytes));
+    std::vector<char> h(bytes, 0);
+    double *hd = (double *)h.data();
+    memcpy(hd, value.data(), ne * sizeof(double));
+    memcpy(hd + ne, ftime.data(), ne * sizeof(double));
+    long long *hn = (long long *)(hd + nd);
+    memcpy(hn, node.data(), nb * sizeof(long long));
+    int *hi = (int *)(hn + nb);
+    memcpy(hi, start.data(), (nb + 1) * sizeof(int));
+    memcpy(hi + nb + 1, dir.data(), ne * sizeof(int));
+    memcpy(hi + nb + 1 + ne, style.data(), ne * sizeof(int));
+    HIPCHECK(c, hipMemcpy(c->bcbuf, h.data(), bytes, hipMemcpyHostToDevice));
+    double *dd = (double *)c->bcbuf;
+    long long *dn = (long long *)(dd + nd);
+    int *di = (int *)(dn + nb);
+    KBcs &k = c->kbc;
+    k.nb = (int)nb;
+    k.value = dd;
+    k.ftime = dd + ne;
+    k.cur = dd + 2 * ne;
+    k.save = dd + 3 * ne;
+    k.time = dd + 3 * ne + 3 * nb * nf;
+    k.node = dn;
+    k.start = di;
+    k.dir = di + nb + 1;
+    k.style = di + nb + 1 + ne;
+    k.act = di + nb + 1 + 2 * ne;
+    return put_bc_time(c);
+}
+
 int mpmgpu_run_task(MpmCtx *c, int32_t task) {
     if (task != c->nextTask)
         return c->fail("tasks must run in the step order (expected task " + std::to_string(c->nextTask) + ")");
@@ -693,13 +796,16 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
                 launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
             }
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_MASS_MOMENTUM, s);
             break;
         case MPMGPU_TASK_UPDATE_STRAINS_FIRST: launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
         case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
         case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
         case MPMGPU_TASK_UPDATE_MOMENTA:
             launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_GRID_FORCES, s);
             if (c->kg.nfields > 1) launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_MOMENTUM, s);
             break;
         case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
         case MPMGPU_TASK_UPDATE_STRAINS_LAST:
@@ -709,6 +815,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
                 launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
             }
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_STRAINS_LAST, s);
             launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
             break;
         case MPMGPU_TASK_RESET_ELEMENTS:
@@ -716,6 +823,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             c->parity ^= 1;
             c->steps++;
             c->mtime += c->run.timestep;
+            c->bcTime += c->run.timestep;
             break;
         default: return c->fail("unknown task id");
     }
@@ -726,7 +834,12 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
 
 int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
     if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
-    if (c->kp.n == 0) { c->steps += nsteps; c->mtime += nsteps * c->run.timestep; return 0; }
+    for (int32_t k = 0; k < nsteps; k++) c->bcTime += c->run.timestep;
+    if (c->kp.n == 0) {
+        c->steps += nsteps;
+        c->mtime += nsteps * c->run.timestep;
+        return put_bc_time(c);
+    }
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
     int done = 0;
     const int G = c->graphSteps;
@@ -797,7 +910,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
                                c->stream));
     const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort;
     const int64_t steps = c->steps;
-    const double mtime = c->mtime;
+    const double mtime = c->mtime, bcTime = c->bcTime;
     int best = oldG;
     double bestMs = 1e300;
     static const int cand[] = {2, 4, 8, 16, 32};
@@ -825,6 +938,8 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     c->sinceSort = sinceSort;
     c->steps = steps;
     c->mtime = mtime;
+    c->bcTime = bcTime;
+    if (put_bc_time(c)) return 1;
     mpmgpu_set_graph_steps(c, best);
     if (graphSteps) *graphSteps = best;
     if (msPerStep) *msPerStep = bestMs;
@@ -846,7 +961,9 @@ int mpmgpu_sync(MpmCtx *c) {
     if (check_cpdi(c)) return 1;
     collect_marks(c);
     float ms = 0.f;
+    // before any timed step (task by task) the events hold nothing; that error is not kept
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->lastStepMs = ms;
+    else (void)hipGetLastError();
     return 0;
 }
 
@@ -962,6 +1079,7 @@ void mpmgpu_destroy(MpmCtx *c) {
     if (c->kn.part) (void)hipFree(c->kn.part);
     if (c->dmats) (void)hipFree(c->dmats);
     if (c->sbuf) (void)hipFree(c->sbuf);
+    if (c->bcbuf) (void)hipFree(c->bcbuf);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -134,6 +134,29 @@ void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b,
 void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s);
 
+// Grid velocity conditions of one simulation (single grid), grouped by node: node b's conditions
+// are entries start[b]..start[b+1]-1, in the order of the condition list. One thread per
+// (node, field) applies them, so no two threads touch the same node values.
+struct KBcs {
+    int nb;                    // nodes with conditions
+    const long long *node;     // nb node indices
+    const int *start;          // nb + 1
+    const int *dir;            // per entry: axis 0..2
+    const int *style;          // 1 constant, 2 linear, 3 sine, 4 cosine
+    const double *value;       // per entry (internal units)
+    const double *ftime;
+    double *cur;               // per entry: value of the step (written by the mass-momentum pass)
+    int *act;                  // per entry: acts in this step
+    double *save;              // nb * nfields * 3: momenta before the mass-momentum pass
+    double *time;              // the step's start time; the strains-last pass advances it
+};
+// passes of NodalVelBC::GridVelocityConditions; GRID_FORCES runs the force pass and the momentum
+// update of the nodes with conditions, FORCES_MOMENTA adds the UPDATE_MOMENTUM pass in the same
+// launch (single-material mode, where no contact pass comes between)
+enum { BC_PASS_MASS_MOMENTUM = 0, BC_PASS_GRID_FORCES, BC_PASS_UPDATE_MOMENTUM, BC_PASS_FORCES_MOMENTA,
+       BC_PASS_UPDATE_STRAINS_LAST };
+void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, hipStream_t s);
+
 // A run of particles copied between the particle buffers of two contexts (the solve server moves
 // a resident solve into the next batch layout). Double rows are read from sd (row stride sn)
 // except the positions, which come from the source's current position rows spos; int rows MPI_*

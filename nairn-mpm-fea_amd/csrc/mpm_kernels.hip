@@ -2111,6 +2111,124 @@ void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &
     else hipLaunchKernelGGL(k_node_contact<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
+// ---------------------------------------------------------------------------------------
+// Grid velocity conditions (NodalVelBC::GridVelocityConditions -> VelocityBCLoop,
+// NodalVelBC.cpp:321-400; MatVelocityField::ZeroVelocityBC and AddVelocityBC,
+// MatVelocityField.cpp:490-575, norm a unit axis). The reference zeroes for every condition of
+// the list, then adds for every one; conditions on different nodes touch different values, so
+// taking each node's conditions in list order, zeroes first, gives the same numbers.
+
+// BoundaryCondition::GetNodeNum and BCValue (BoundaryCondition.cpp:72-140): 1 if the condition
+// acts at time t, its value in v
+__device__ __forceinline__ int bc_value(const KBcs &bc, int e, double t, double &v) {
+    const double val = bc.value[e], ft = bc.ftime[e];
+    v = 0.;
+    switch (bc.style[e]) {
+        case 1: if (t >= ft) { v = val; return 1; } return 0;
+        case 2: if (t >= ft) { v = 0. + val * (t - ft); return 1; } return 0;
+        case 3: v = val * sin(t * ft); return 1;
+        case 4: v = val * cos(t * ft); return 1;
+        default: return 0;
+    }
+}
+
+// one pass on momenta p and forces fo of one field of one node (ZeroVelocityBC, then AddVelocityBC)
+template <int PASS>
+__device__ __forceinline__ void bc_pass(const KBcs &bc, int e0, int e1, double t, double m, double dt,
+                                        double p[3], double fo[3]) {
+    for (int e = e0; e < e1; e++) {
+        double v;
+        const int on = PASS == BC_PASS_MASS_MOMENTUM ? bc_value(bc, e, t, v) : bc.act[e];
+        if (!on) continue;
+        const int a = bc.dir[e];
+        if (PASS == BC_PASS_GRID_FORCES) {
+            const double dotf = fo[a], dotp = p[a];
+            fo[a] += -dotf - dotp / dt;
+        } else {
+            const double dotn = p[a];
+            p[a] += -dotn;
+            if (PASS == BC_PASS_UPDATE_MOMENTUM) fo[a] += -dotn / dt;
+        }
+    }
+    for (int e = e0; e < e1; e++) {
+        double v;
+        int on;
+        if (PASS == BC_PASS_MASS_MOMENTUM) on = bc_value(bc, e, t, v);
+        else { on = bc.act[e]; v = bc.cur[e]; }
+        if (!on) continue;
+        const int a = bc.dir[e];
+        const double mv = m * v;
+        if (PASS == BC_PASS_GRID_FORCES) {
+            fo[a] += mv / dt;
+        } else {
+            p[a] += mv;
+            if (PASS == BC_PASS_UPDATE_MOMENTUM) fo[a] += mv / dt;
+        }
+    }
+}
+
+// one thread per (node with conditions, field). MASS_MOMENTUM (after the contact pass, before the
+// first strain update) keeps the momenta it starts from; GRID_FORCES starts from those again
+// (MatVelocityField::RestoreMomenta), sets the forces and redoes the momentum update of the node
+// (pk + ftot*dt, as k_node_force); the strains-last pass advances the start time of the next step.
+template <int PASS>
+__global__ void k_vel_bcs(KGrid g, KNodes N, KBcs bc) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // no thread of this pass reads the time
+    if (PASS == BC_PASS_UPDATE_STRAINS_LAST && idx == 0) *bc.time = *bc.time + g.timestep;
+    if (idx >= (long long)bc.nb * g.nfields) return;
+    const int f = (int)(idx / bc.nb);
+    const int b = (int)(idx - (long long)f * bc.nb);
+    const int e0 = bc.start[b], e1 = bc.start[b + 1];
+    double t = 0.;
+    if (PASS == BC_PASS_MASS_MOMENTUM) {
+        t = *bc.time;
+        if (f == 0)
+            for (int e = e0; e < e1; e++) {
+                double v;
+                bc.act[e] = bc_value(bc, e, t, v);
+                bc.cur[e] = v;
+            }
+    }
+    const KNodes Nf = field_view(g, N, f);
+    const long long n = bc.node[b], nn = g.nstride;
+    if (Nf.cnt[n] <= 0) return;
+    const double m = Nf.mass[n], dt = g.timestep;
+    double p[3] = {Nf.pk[n], Nf.pk[nn + n], Nf.pk[2 * nn + n]};
+    double fo[3] = {Nf.ftot[n], Nf.ftot[nn + n], Nf.ftot[2 * nn + n]};
+    double *sv = bc.save + ((long long)f * bc.nb + b) * 3;
+    if (PASS == BC_PASS_MASS_MOMENTUM) {
+        sv[0] = p[0]; sv[1] = p[1]; sv[2] = p[2];
+        bc_pass<BC_PASS_MASS_MOMENTUM>(bc, e0, e1, t, m, dt, p, fo);
+    } else if (PASS == BC_PASS_GRID_FORCES || PASS == BC_PASS_FORCES_MOMENTA) {
+        p[0] = sv[0]; p[1] = sv[1]; p[2] = sv[2];
+        bc_pass<BC_PASS_GRID_FORCES>(bc, e0, e1, t, m, dt, p, fo);
+        for (int c = 0; c < 3; c++) p[c] = p[c] + fo[c] * dt;
+        if (PASS == BC_PASS_FORCES_MOMENTA) bc_pass<BC_PASS_UPDATE_MOMENTUM>(bc, e0, e1, t, m, dt, p, fo);
+    } else {
+        bc_pass<PASS>(bc, e0, e1, t, m, dt, p, fo);
+    }
+    Nf.pk[n] = p[0]; Nf.pk[nn + n] = p[1]; Nf.pk[2 * nn + n] = p[2];
+    if (PASS != BC_PASS_MASS_MOMENTUM && PASS != BC_PASS_UPDATE_STRAINS_LAST) {
+        Nf.ftot[n] = fo[0]; Nf.ftot[nn + n] = fo[1]; Nf.ftot[2 * nn + n] = fo[2];
+    }
+    if (m != 0.) {
+        const double s = 1. / m;
+        Nf.vk[n] = s * p[0]; Nf.vk[nn + n] = s * p[1]; Nf.vk[2 * nn + n] = s * p[2];
+    }
+}
+
+void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, hipStream_t s) {
+    const dim3 grid(blocks_for((long long)bc.nb * g.nfields, NT)), blk(NT);
+    switch (pass) {
+        case BC_PASS_MASS_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_MASS_MOMENTUM>, grid, blk, 0, s, g, nd, bc); break;
+        case BC_PASS_GRID_FORCES: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_GRID_FORCES>, grid, blk, 0, s, g, nd, bc); break;
+        case BC_PASS_UPDATE_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_MOMENTUM>, grid, blk, 0, s, g, nd, bc); break;
+        case BC_PASS_FORCES_MOMENTA: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_FORCES_MOMENTA>, grid, blk, 0, s, g, nd, bc); break;
+        default: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_STRAINS_LAST>, grid, blk, 0, s, g, nd, bc); break;
+    }
+}
+
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s) {
     if (dim == 3) hipLaunchKernelGGL(k_contact<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd, callType);
     else hipLaunchKernelGGL(k_contact<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd, callType);

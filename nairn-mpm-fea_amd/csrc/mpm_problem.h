@@ -33,6 +33,7 @@ struct MpmProblem {
     int nhoriz = 0, nvert = 0, ndepth = 0;
     double thickness = 1.;           // MeshInfo default thickness for 2D
     MpmGridDesc grid{};
+    double meshLim[6] = {0, 0, 0, 0, 0, 0};   // grid limits before the extra GIMP/CPDI elements
     bool gridBuilt = false;
     // materials
     std::vector<MaterialInput> matIn;
@@ -53,6 +54,8 @@ struct MpmProblem {
     std::string contactLawName;      // <Friction lawname=>
     int contactLawNum = 0;           // <Friction law=> (1-based), <0: automatic law -n-1, 0: none
     std::vector<MaterialInput> autoLaws;
+    // <GridBCs> velocity conditions in the reference's list order
+    std::vector<MpmVelocityBC> velBCs;
     // particles, in mpm[] order
     std::vector<double> d[MPF_NUM_FIELDS];
     std::vector<int32_t> iv[MPI_NUM_FIELDS];

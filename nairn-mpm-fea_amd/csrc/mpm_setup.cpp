@@ -9,13 +9,17 @@ namespace mpmsetup {
 
 // MPMReadHandler::grid() for a regular grid (Read_MPM/Generators.cpp:1675-1833): cell counts
 // from cell sizes, then one extra element per side for GIMP/CPDI.
-static bool axis_cells(double &vmin, double &vmax, double cell, int &n) {
+// lim receives the limits before the extra elements (the mesh limits mxmin..mzmax that
+// ReadGridPoint's min and max keywords refer to, Generators.cpp:1760-1766).
+static bool axis_cells(double &vmin, double &vmax, double cell, int &n, double *lim) {
     if (cell > 0.) {
         n = (int)((vmax - vmin) / cell);
         double newMax = vmin + n * cell;
         if (newMax < vmax) { vmax = newMax + cell; n++; }
     }
     if (n < 1) return false;
+    lim[0] = vmin;
+    lim[1] = vmax;
     double c = (vmax - vmin) / (double)n;
     n += 2;
     vmin -= c;
@@ -27,8 +31,8 @@ bool build_grid(MpmProblem &p) {
     if (!p.haveGrid) { p.err = "no <Grid> in the input"; return false; }
     double xmn = p.gxmin, xmx = p.gxmax, ymn = p.gymin, ymx = p.gymax, zmn = p.gzmin, zmx = p.gzmax;
     int nx = p.nhoriz, ny = p.nvert, nz = p.ndepth;
-    if (!axis_cells(xmn, xmx, p.cellx, nx) || !axis_cells(ymn, ymx, p.celly, ny) ||
-        (p.dim == 3 && !axis_cells(zmn, zmx, p.cellz, nz))) {
+    if (!axis_cells(xmn, xmx, p.cellx, nx, &p.meshLim[0]) || !axis_cells(ymn, ymx, p.celly, ny, &p.meshLim[2]) ||
+        (p.dim == 3 && !axis_cells(zmn, zmx, p.cellz, nz, &p.meshLim[4]))) {
         p.err = "Number of grid elements in all direction must be >= 1.";
         return false;
     }

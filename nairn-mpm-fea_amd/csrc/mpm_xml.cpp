@@ -4,6 +4,7 @@
 // Read_MPM/MPMReadHandler.cpp:83-1330, Read_MPM/Generators.cpp:89-1545) with a small in-memory
 // parser: elements, attributes, text, comments, processing instructions and a DOCTYPE internal
 // subset with <!ENTITY name "value"> declarations (as used by NairnMPM/input/XML_Input).
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
@@ -484,6 +485,154 @@ struct Reader {
         return true;
     }
 
+    // CommonReadHandler::ReadGridPoint (Common/Read_XML/CommonReadHandler.cpp:611-658): a number
+    // times the length scale, or the mesh's min or max on the axis plus an optional number of
+    // minimum cell sizes ("max", "max+", "max-", "max3", "min+0.5", ...)
+    static double grid_point(const char *v, double ls, double amin, double amax, double minCell) {
+        const size_t len = strlen(v);
+        if (len >= 3 && v[0] == 'm' && ((v[1] == 'a' && v[2] == 'x') || (v[1] == 'i' && v[2] == 'n'))) {
+            double d = 0.;
+            if (len == 4) {
+                if (v[3] == '-') d = -1.;
+                else if (v[3] == '+') d = 1.;
+                else if (v[3] >= '0' && v[3] <= '9') d = (float)(v[3] - '0');
+            } else if (len > 4) {
+                d = strtod(v + 3, nullptr);
+            }
+            return (v[1] == 'a' ? amax : amin) + d * minCell;
+        }
+        return strtod(v, nullptr) * ls;
+    }
+
+    // DbleEqual (Common/System/CommonUtilities.cpp:46-61)
+    static bool dble_equal(double a, double b) {
+        double diff = std::fabs(a - b);
+        if (diff <= 1.e-16) return true;
+        double l = std::max(std::fabs(a), std::fabs(b));
+        return diff <= l * 1.e-7;
+    }
+
+    // <GridBCs>: <BCLine> (2D) or <BCBox> (3D) node sets holding <DisBC> velocity conditions
+    // (MPMReadHandler, Read_MPM/Generators.cpp:831-1010). Each DisBC adds one condition per node
+    // the shape contains, nodes in node order (ShapeController::nextNode,
+    // Common/Read_XML/ShapeController.cpp:231-244), so the list runs DisBC by DisBC.
+    bool gridbcs(const XNode &gb) {
+        if (!p.gridBuilt) return fail("<GridBCs> needs the <Mesh> first");
+        const MpmGridDesc &g = p.grid;
+        const int dim = p.dim;
+        const long nnx = g.nx + 1, nny = g.ny + 1, nnz = dim == 3 ? g.nz + 1 : 1;
+        auto xn = [&](long i) { return g.xmin + (double)i * g.dx; };
+        auto yn = [&](long j) { return g.ymin + (double)j * g.dy; };
+        auto zn = [&](long k) { return g.zmin + (double)k * g.dz; };
+        // ElementBase::gridTolerance, 0.1 of the shortest element side (ElementBase.cpp:60-84,
+        // ElementBase3D.cpp:30-44); GetMinimumCellSize divides it by 0.1 again (:150)
+        double gtol = 1.e20;
+        for (long i = 0; i + 1 < nnx; i++) gtol = std::min(gtol, 0.1 * (xn(i + 1) - xn(i)));
+        for (long j = 0; j + 1 < nny; j++) gtol = std::min(gtol, 0.1 * (yn(j + 1) - yn(j)));
+        if (dim == 3) for (long k = 0; k + 1 < nnz; k++) gtol = std::min(gtol, 0.1 * (zn(k + 1) - zn(k)));
+        const double minCell = gtol / 0.1;
+        for (auto &sp : gb.kids) {
+            const XNode &s = *sp;
+            const bool line = s.name == "BCLine", box = s.name == "BCBox";
+            if (!line && !box) return fail("<" + s.name + "> in <GridBCs> is not on this path");
+            if (line && dim != 2) return fail("<BCLine> is for 2D analyses");
+            if (box && dim != 3) return fail("<BCBox> is for 3D analyses");
+            const double ls = length_units(s);
+            // ShapeController::SetProperty (ShapeController.cpp:105-125): x1/xmin ... z2/zmax,
+            // the later attribute of a pair wins
+            double lo[3] = {0., 0., 0.}, hi[3] = {0., 0., 0.};
+            double tol = gtol;
+            for (auto &a : s.attrs) {
+                const std::string &k = a.first;
+                const char *v = a.second.c_str();
+                for (int c = 0; c < 3; c++) {
+                    const char ax = "xyz"[c];
+                    const double amin = p.meshLim[2 * c], amax = p.meshLim[2 * c + 1];
+                    if (k == std::string(1, ax) + "1" || k == std::string(1, ax) + "min")
+                        lo[c] = grid_point(v, ls, amin, amax, minCell);
+                    else if (k == std::string(1, ax) + "2" || k == std::string(1, ax) + "max")
+                        hi[c] = grid_point(v, ls, amin, amax, minCell);
+                }
+                if (k == "tolerance" && line)      // LineController::SetProperty (LineController.cpp:91-105)
+                    tol = v[0] == '*' ? strtod(v + 1, nullptr) * minCell : strtod(v, nullptr) * ls;
+                if (k == "axis" && box && std::string(v) != "0") {     // BoxController::SetProperty (:22-36)
+                    const std::string av(v);
+                    if (av == "x" || av == "X" || av == "1" || av == "y" || av == "Y" || av == "2" || av == "z" ||
+                        av == "Z" || av == "3")
+                        return fail("<BCBox axis=> (cylinder node sets) is not on this path");
+                    return fail("Box axis must be x, y, z, 1, 2, or 3");
+                }
+            }
+            double dtol = 0., distSq = 0.;
+            if (line) {
+                // LineController::FinishSetup (LineController.cpp:33-42)
+                distSq = (hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]);
+                dtol = std::sqrt(distSq) * tol;
+            } else {
+                // ShapeController::FinishSetup (ShapeController.cpp:164-197): ordered, nonzero extents
+                for (int c = 0; c < 3; c++) {
+                    if (lo[c] > hi[c]) std::swap(lo[c], hi[c]);
+                    if (dble_equal(lo[c], hi[c]))
+                        return fail(std::string("Box: ") + "xyz"[c] + "max cannot equal " + "xyz"[c] + "min in input parameters.");
+                }
+            }
+            auto contains = [&](double x, double y, double z) {
+                if (box)                           // BoxController::ContainsPoint, axis 0 (BoxController.cpp:70-74)
+                    return x <= hi[0] && x >= lo[0] && y <= hi[1] && y >= lo[1] && z <= hi[2] && z >= lo[2];
+                double dx = hi[0] - lo[0], dy = hi[1] - lo[1];     // LineController::ContainsPoint (:49-66)
+                double px = x - lo[0], py = y - lo[1];
+                double ypr = -dy * px + dx * py;
+                if (ypr > dtol || ypr < -dtol) return false;
+                double xpr = dx * px + dy * py;
+                return !(xpr < 0. || xpr > distSq);
+            };
+            for (auto &bp : s.kids) {
+                const XNode &b = *bp;
+                if (b.name != "DisBC")
+                    return fail("<" + b.name + "> in <" + s.name + "> is not on this path");
+                int dof = 0, style = 1, id = 0;
+                double vel = 0., ftime = 0., angle = 0.;
+                for (auto &a : b.attrs) {
+                    const std::string &k = a.first;
+                    const char *v = a.second.c_str();
+                    if (k == "dir") dof = (int)strtol(v, nullptr, 10);
+                    else if (k == "disp" || k == "vel") vel = strtod(v, nullptr);
+                    else if (k == "time" || k == "ftime") ftime = strtod(v, nullptr);
+                    else if (k == "style") style = (int)strtol(v, nullptr, 10);
+                    else if (k == "angle") angle = strtod(v, nullptr);
+                    else if (k == "id") id = (int)strtol(v, nullptr, 10);
+                    else if (k == "function" || k == "gradfxn" || k == "dispfxn" || k == "depth" || k == "angle2")
+                        return fail("<DisBC " + k + "=> is not on this path");
+                }
+                if (dim == 3 && dof != 1 && dof != 2 && dof != 3 && dof != 12 && dof != 13 && dof != 23 && dof != 123)
+                    return fail("'dir' in DisBC element must be 1, 2, 3, 12, 13, 23, or 123 for 3D analyses.");
+                if (dim == 2 && dof != 1 && dof != 2 && dof != 12)
+                    return fail("'dir' in DisBC element must be 1, 2, or 12 for 2D analyses.");
+                if (id > 0) return fail("'id' for velocity boundary conditions must be <= 0.");
+                if (dof > 10) {
+                    if (!dble_equal(angle, 0.)) return fail("skewed velocity conditions are not on this path");
+                    dof = (dof == 12 || dof == 13) ? 1 : (dof == 23 ? 2 : 3);
+                }
+                if (style < 1 || style > 4) return fail("velocity condition styles other than 1 to 4 are not on this path");
+                // BoundaryCondition::SetBCValue and SetBCFirstTime (BoundaryCondition.cpp:182-206):
+                // linear values per ms to per s, times ms to s, frequencies 1/ms to 1/s
+                MpmVelocityBC bc{};
+                bc.dir = dof;
+                bc.style = style;
+                bc.value = style == 2 ? vel * 1.e3 : vel;
+                bc.ftime = (style == 3 || style == 4) ? ftime * 1.e3 : ftime * 1.e-3;
+                for (long k = 0; k < nnz; k++)
+                    for (long j = 0; j < nny; j++)
+                        for (long i = 0; i < nnx; i++)
+                            if (contains(xn(i), yn(j), dim == 3 ? zn(k) : 0.)) {
+                                bc.node = (k * nny + j) * nnx + i;
+                                p.velBCs.push_back(bc);
+                            }
+            }
+        }
+        return true;
+    }
+
     bool run(const XNode &root) {
         if (root.name != "JANFEAInput") return fail("root element must be <JANFEAInput>");
         p.run.shape = MPMGPU_SHAPE_LINEAR;       // ElementBase::useGimp default until <GIMP>
@@ -500,7 +649,11 @@ struct Reader {
         }
         for (auto &k : root.kids) {
             const std::string &n = k->name;
-            if (n == "GridBCs" || n == "ParticleBCs" || n == "CrackList" || n == "Thermal" || n == "CustomTasks" || n == "Gravity")
+            if (n == "GridBCs") {
+                if (!gridbcs(*k)) return false;
+                continue;
+            }
+            if (n == "ParticleBCs" || n == "CrackList" || n == "Thermal" || n == "CustomTasks" || n == "Gravity")
                 if (!k->kids.empty() || !k->text.empty())
                     return fail("<" + n + "> is not on this path");
         }

@@ -99,6 +99,13 @@ class Stats(ctypes.Structure):
                 ("nanParticles", ctypes.c_int64), ("time", ctypes.c_double), ("lastStepMs", ctypes.c_double)]
 
 
+class VelocityBC(ctypes.Structure):
+    """one grid velocity condition (MpmVelocityBC): node (0-based), dir 1..3, style 1..4,
+    value (mm/s, linear: mm/s^2) and ftime (s, or 1/s for the sine and cosine styles)"""
+    _fields_ = [("node", ctypes.c_int64), ("dir", ctypes.c_int32), ("style", ctypes.c_int32),
+                ("value", ctypes.c_double), ("ftime", ctypes.c_double)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -120,6 +127,7 @@ _SIGNATURES = {
                                  ctypes.POINTER(RunParams), ctypes.POINTER(ctypes.c_int64), _PD], ctypes.c_int),
     "mpmgpu_problem_particles": ([_P, _PD, _PI], ctypes.c_int),
     "mpmgpu_problem_archive": ([_P, ctypes.c_char_p, _PD, _PD], ctypes.c_int),
+    "mpmgpu_problem_velocity_bcs": ([_P, ctypes.POINTER(VelocityBC), ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "mpmgpu_problem_destroy": ([_P], None),
     "mpmgpu_archive_order": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
     "mpmgpu_archive_record_size": ([ctypes.c_char_p, ctypes.c_int32], ctypes.c_int32),
@@ -132,6 +140,7 @@ _SIGNATURES = {
     "mpmgpu_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Material), ctypes.c_int32,
                        ctypes.POINTER(RunParams), ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_upload_particles": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
+    "mpmgpu_set_velocity_bcs": ([_P, ctypes.c_int64, ctypes.POINTER(VelocityBC)], ctypes.c_int),
     "mpmgpu_batch_create": ([ctypes.c_int32, ctypes.POINTER(InstanceDesc), ctypes.POINTER(Material), ctypes.c_int32,
                              ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
@@ -272,6 +281,14 @@ class Problem:
         _check(library().mpmgpu_problem_archive(self._h, order, ctypes.byref(t), _dptr(thick)))
         return order.value.decode(), (t.value if t.value >= 0. else None), thick
 
+    def velocity_bcs(self):
+        """the <GridBCs> velocity conditions in the reference's list order (VelocityBC list)"""
+        n = ctypes.c_int64(0)
+        _check(library().mpmgpu_problem_velocity_bcs(self._h, None, ctypes.byref(n)))
+        arr = (VelocityBC * max(n.value, 1))()
+        _check(library().mpmgpu_problem_velocity_bcs(self._h, arr, ctypes.byref(n)))
+        return [arr[k] for k in range(n.value)]
+
     def close(self):
         if self._h:
             library().mpmgpu_problem_destroy(self._h)
@@ -302,7 +319,15 @@ class Simulation:
         sim = cls(g, mats, run)
         d, i = prob.particles()
         sim.upload(d, i)
+        bcs = prob.velocity_bcs()
+        if bcs:
+            sim.set_velocity_bcs(bcs)
         return sim
+
+    def set_velocity_bcs(self, bcs):
+        """grid velocity conditions (VelocityBC list, reference list order); [] removes them"""
+        arr = (VelocityBC * max(len(bcs), 1))(*bcs)
+        _check(library().mpmgpu_set_velocity_bcs(self._h, len(bcs), arr), self._h)
 
     def upload(self, dfields: np.ndarray, ifields: np.ndarray):
         d = np.ascontiguousarray(dfields, dtype=np.float64)
@@ -413,6 +438,8 @@ class Batch(Simulation):
         replaces the problem's own initial particles (same count and order)"""
         descs, mats, ds, iss = [], [], [], []
         for k, prob in enumerate(problems):
+            if prob.velocity_bcs():
+                raise MPMError("grid velocity conditions are for single simulations, not batches")
             g, m, run, n, _ = prob.describe()
             d, i = prob.particles() if particles is None or particles[k] is None else particles[k]
             i = i.copy()
@@ -489,6 +516,8 @@ class Server:
         return rid.value
 
     def submit_problem(self, prob: Problem, steps: int, particles=None) -> int:
+        if prob.velocity_bcs():
+            raise MPMError("grid velocity conditions are for single simulations, not served solves")
         g, mats, run, _, _ = prob.describe()
         d, i = prob.particles() if particles is None else particles
         return self.submit(g, mats, run, d, i, steps)

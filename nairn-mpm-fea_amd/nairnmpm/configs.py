@@ -127,6 +127,22 @@ def j2_block_3d(cells: int = 128, lo: int = 16, hi: int = 112, vx: float = 50000
 """)
 
 
+def grid_bcs(xml: str, shapes) -> str:
+    """xml with a <GridBCs> block added: shapes is a list of (tag, attributes, [DisBC attributes]),
+    tag "BCLine" (2D) or "BCBox" (3D), e.g.
+    ("BCLine", {"x1": 4, "y1": 4, "x2": 12, "y2": 4}, [{"dir": 2, "vel": 0}])"""
+    def attrs(d):
+        return " ".join(f'{k}="{v!r}"' if isinstance(v, float) else f'{k}="{v}"' for k, v in d.items())
+    body = "  <GridBCs>\n"
+    for tag, a, bcs in shapes:
+        body += f"    <{tag} {attrs(a)}>\n"
+        for b in bcs:
+            body += f"      <DisBC {attrs(b)}/>\n"
+        body += f"    </{tag}>\n"
+    body += "  </GridBCs>\n"
+    return xml.replace("</JANFEAInput>", body + "</JANFEAInput>", 1)
+
+
 def mooney_material_xml(G1: float = 8., G2: float = 2., K: float = 50., rho: float = 1.1,
                         name: str = "mooney") -> str:
     """<Material Type="8"> (Mooney-Rivlin) with G1, G2 and K in MPa."""

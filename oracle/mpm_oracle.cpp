@@ -383,6 +383,13 @@ struct Oracle {
     std::vector<std::vector<double>> tb;   // per thread: mass, pk(3), ftot(3), cvol, cdisp(3), cgrad(3)
     std::vector<std::vector<int>> tnp;
 
+    // grid velocity conditions in NodalVelBC list order (firstVelocityBC), their values at the
+    // step's start time and whether each acts (GetNodeNum > 0)
+    std::vector<MpmVelocityBC> vbcs;
+    std::vector<double> bcValue;
+    std::vector<char> bcOn;
+    enum { BC_MASS_MOMENTUM = 0, BC_GRID_FORCES = 1, BC_UPDATE_MOMENTUM = 2, BC_UPDATE_STRAINS_LAST = 3 };
+
     // lCPDI corners of the step (MPMBase CPDIDomain: inElem, ncpos, wg; ws = 1/8 or 1/4)
     struct Cpdi { long elem[8]; double xi[8][3]; double wg[8][3]; };
     std::vector<Cpdi> cpdi;
@@ -700,6 +707,69 @@ struct Oracle {
             }
         }
         material_contact(CALL_MASS_MOMENTUM);
+        // ContactAndMomentaBCs(MASS_MOMENTUM_CALL) (PostExtrapolationTask.cpp:153-159): the
+        // conditions change pk but not its copy, which RestoreMomenta brings back
+        bc_values();
+        velocity_bcs(BC_MASS_MOMENTUM);
+    }
+
+    // ---- grid velocity conditions -------------------------------------------------------
+    // NodalVelBC::GridVelocityBCValues (NodalVelBC.cpp:293-303, called from
+    // PostExtrapolationTask.cpp:154) -> BoundaryCondition::GetNodeNum and BCValue
+    // (BoundaryCondition.cpp:72-140) at mtime, the step's start time
+    void bc_values() {
+        bcValue.assign(vbcs.size(), 0.);
+        bcOn.assign(vbcs.size(), 1);
+        for (size_t e = 0; e < vbcs.size(); e++) {
+            const MpmVelocityBC &b = vbcs[e];
+            switch (b.style) {
+                case 1: bcOn[e] = mtime >= b.ftime; if (bcOn[e]) bcValue[e] = b.value; break;
+                case 2: bcOn[e] = mtime >= b.ftime; if (bcOn[e]) bcValue[e] = 0. + b.value * (mtime - b.ftime); break;
+                case 3: bcValue[e] = b.value * sin(mtime * b.ftime); break;
+                case 4: bcValue[e] = b.value * cos(mtime * b.ftime); break;
+                default: bcOn[e] = 0; break;
+            }
+        }
+    }
+
+    // NodalVelBC::GridVelocityConditions -> VelocityBCLoop (NodalVelBC.cpp:321-400): every acting
+    // condition first zeroes its component on each field of its node that has points
+    // (CrackVelocityFieldSingle/Multi::ZeroVelocityBC), then every one adds mass * value
+    // (AddVelocityBC); MatVelocityField::ZeroVelocityBC and AddVelocityBC (MatVelocityField.cpp:490-575)
+    // with norm the condition's unit axis, so dot products pick the axis component
+    void velocity_bcs(int pass) {
+        if (vbcs.empty()) return;
+        const double dt = run.timestep;
+        for (int add = 0; add < 2; add++) {
+            for (size_t e = 0; e < vbcs.size(); e++) {
+                if (!bcOn[e]) continue;
+                const int a = vbcs[e].dir - 1;
+                for (int f = 0; f < nf; f++) {
+                    const long k = fnode(f, (long)vbcs[e].node);
+                    if (npts[k] <= 0) continue;
+                    double &p = pk[3 * k + a], &fo = ftot[3 * k + a];
+                    if (!add) {
+                        if (pass == BC_GRID_FORCES) {
+                            // force so that the updated momentum is zero: -dotf - dotp/deltime
+                            double dotf = fo, dotp = p;
+                            fo += -dotf - dotp / dt;
+                        } else {
+                            double dotn = p;
+                            p += -dotn;
+                            if (pass == BC_UPDATE_MOMENTUM) fo += -dotn / dt;
+                        }
+                    } else {
+                        const double mv = mass[k] * bcValue[e];
+                        if (pass == BC_GRID_FORCES) {
+                            fo += mv / dt;
+                        } else {
+                            p += mv;
+                            if (pass == BC_UPDATE_MOMENTUM) fo += mv / dt;
+                        }
+                    }
+                }
+            }
+        }
     }
 
     // ---- multimaterial contact -----------------------------------------------------------
@@ -1245,6 +1315,7 @@ struct Oracle {
 #pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = pkc[3 * n + c];
+        velocity_bcs(BC_GRID_FORCES);                    // PostForcesTask.cpp:94
     }
 
     // UpdateMomentaTask::Execute (UpdateMomentaTask.cpp:44-64) -> UpdateMomentum (MatVelocityField.cpp:301-314)
@@ -1254,6 +1325,7 @@ struct Oracle {
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] += ftot[3 * n + c] * dt;
         material_contact(CALL_UPDATE_MOMENTUM);          // ContactAndMomentaBCs (UpdateMomentaTask.cpp:58)
+        velocity_bcs(BC_UPDATE_MOMENTUM);                // (UpdateMomentaTask.cpp:93)
     }
 
     // UpdateParticlesTask::Execute (UpdateParticlesTask.cpp:44-298), FLIP, no damping ->
@@ -1314,6 +1386,7 @@ struct Oracle {
             }
         });
         material_contact(CALL_UPDATE_STRAINS_LAST);      // UpdateStrainsLastContactTask.cpp:145
+        velocity_bcs(BC_UPDATE_STRAINS_LAST);
         full_strain_update(run.strainLast);
     }
 
@@ -1472,6 +1545,19 @@ int oracle_get_particles(void *h, double *dfields, int32_t *ifields) {
 }
 
 // threads of the timing mode (1: the serial oracle the parity checks use)
+/* grid velocity conditions in list order (n = 0 removes them); nodes must be in the grid */
+int oracle_set_velocity_bcs(void *h, int64_t n, const MpmVelocityBC *bcs) {
+    Oracle *o = (Oracle *)h;
+    for (int64_t i = 0; i < n; i++) {
+        if (bcs[i].node < 0 || bcs[i].node >= o->g.nnodes) return 1;
+        if (bcs[i].dir < 1 || bcs[i].dir > o->g.dim) return 1;
+        if (bcs[i].style < 1 || bcs[i].style > 4) return 1;
+    }
+    o->vbcs.assign(bcs, bcs + n);
+    o->bc_values();                      // values at the current time until the next step's
+    return 0;
+}
+
 int oracle_set_threads(void *h, int32_t n) {
     ((Oracle *)h)->nthreads = n < 1 ? 1 : n;
     return 0;

@@ -39,6 +39,7 @@ def lib():
         _lib.oracle_step.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_run_task.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_set_threads.argtypes = [_P, ctypes.c_int32]
+        _lib.oracle_set_velocity_bcs.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.VelocityBC)]
         _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
         _lib.oracle_num_nodes.argtypes = [_P]
         _lib.oracle_num_nodes.restype = ctypes.c_int64
@@ -76,7 +77,18 @@ class Oracle:
     def from_problem(cls, prob):
         g, mats, run, _, _ = prob.describe()
         d, i = prob.particles()
-        return cls(g, mats, run, d, i)
+        o = cls(g, mats, run, d, i)
+        bcs = prob.velocity_bcs()
+        if bcs:
+            o.set_velocity_bcs(bcs)
+        return o
+
+    def set_velocity_bcs(self, bcs):
+        """grid velocity conditions in list order (nairnmpm.VelocityBC list)"""
+        import nairnmpm as nm
+        arr = (nm.VelocityBC * max(len(bcs), 1))(*bcs)
+        if lib().oracle_set_velocity_bcs(self._h, len(bcs), arr) != 0:
+            raise RuntimeError("bad velocity condition")
 
     def set_threads(self, n):
         """Threads of the timing mode (the CPU baseline): particle loops split over n threads and
