@@ -220,6 +220,25 @@ typedef struct MpmVelocityBC {
    (MPMReadHandler BCLine/BCBox/DisBC, Read_MPM/Generators.cpp:831-1010). Returns the count in
    *n; copies min(*n on entry, count) entries when bcs is not null. */
 int mpmgpu_problem_velocity_bcs(const MpmProblem *prob, MpmVelocityBC *bcs, int64_t *n);
+/* One external load on one particle (MatPtLoadBC, Boundary_Conditions/MatPtLoadBC.hpp): particle
+   index in mpm[] order (0-based), axis dir (1 x, 2 y, 3 z), style 1-4 as MpmVelocityBC, value in
+   internal force units (g mm/s^2 = uN; uN/s for linear; a <net/> load already divided by the
+   number of particles) and ftime as MpmVelocityBC. Each step the particle's pFext is zeroed, then
+   every load adds its value at the step's start time (MatPtLoadBC::SetParticleFext,
+   MatPtLoadBC.cpp:210-221), and the force pass adds N * pFext to the nodes
+   (MatPoint3D::GetFintPlusFext, MatPoint3D.cpp:248-252). */
+typedef struct MpmParticleLoad {
+    int64_t particle;
+    int32_t dir;
+    int32_t style;
+    double value;
+    double ftime;
+} MpmParticleLoad;
+/* The <LoadBC> loads of <ParticleBCs> (<BCLine> in 2D, <BCBox> in 3D; <net/> and <perParticle/>),
+   in list order: load by load in document order, particles in mpm[] order
+   (Read_MPM/Generators.cpp:1012-1023,1105-1188; MatPtLoadBC::SetBCValue, MatPtLoadBC.cpp:202-206).
+   Count and copy as mpmgpu_problem_velocity_bcs. */
+int mpmgpu_problem_particle_loads(const MpmProblem *prob, MpmParticleLoad *loads, int64_t *n);
 void mpmgpu_problem_destroy(MpmProblem *prob);
 
 /* ---- archive files (ArchiveData, System/ArchiveData.cpp), host only ----------------- */
@@ -271,6 +290,10 @@ int mpmgpu_upload_particles(MpmCtx *ctx, int64_t n, const double *dfields, const
    component of every field with points, then adding mass * value. Values are taken at the step's
    start time, counted from the time of this call's step count (mpmgpu_query's time). */
 int mpmgpu_set_velocity_bcs(MpmCtx *ctx, int64_t n, const MpmVelocityBC *bcs);
+/* External particle loads of a single simulation (not a batch); n = 0 removes them. Particle
+   indices are upload indices (the order of mpmgpu_upload_particles); upload the particles first.
+   Values are taken at the step's start time, as for the velocity conditions. */
+int mpmgpu_set_particle_loads(MpmCtx *ctx, int64_t n, const MpmParticleLoad *loads);
 /* Run one hot-path MPMTask's Execute() on the device (MPMGPU_TASK_*), in reference order. */
 int mpmgpu_run_task(MpmCtx *ctx, int32_t task);
 /* Run nsteps whole steps (NairnMPM::MPMStep, NairnMPM.cpp:284-335), fused and graph-replayed. */

@@ -58,6 +58,10 @@ struct MpmCtx {
     KBcs kbc{};
     char *bcbuf = nullptr;
     double bcTime = 0.;
+    // external particle loads (kld.nl = 0: none) in one device buffer
+    KLoads kld{};
+    char *ldbuf = nullptr;
+    double *tbuf = nullptr;          // two step start time slots (by position parity)
     float lastStepMs = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // kernel timing: events around each kernel group of a step, on this context's stream

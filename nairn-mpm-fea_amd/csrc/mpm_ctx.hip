@@ -35,11 +35,18 @@ static void destroy_graphs(MpmCtx *c) {
         if (c->graph[i]) { (void)hipGraphExecDestroy(c->graph[i]); c->graph[i] = nullptr; }
 }
 
-// the device's copy of the step start time of the velocity conditions (none: nothing to do)
+// the device's copy of the step start time of the velocity conditions and loads, in the slot of
+// the current position parity (none allocated: nothing to do)
 static int put_bc_time(MpmCtx *c) {
-    if (!c->kbc.nb) return 0;
-    HIPCHECK(c, hipMemcpyAsync(c->kbc.time, &c->bcTime, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!c->tbuf) return 0;
+    HIPCHECK(c, hipMemcpyAsync(c->tbuf + c->parity, &c->bcTime, sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int ensure_time_slots(MpmCtx *c) {
+    if (c->tbuf) return 0;
+    HIPCHECK(c, hipMalloc(&c->tbuf, 2 * sizeof(double)));
     return 0;
 }
 
@@ -60,8 +67,9 @@ static void enqueue_step(MpmCtx *c) {
     done(MPMGPU_KG_P2G_MASS);
     // grid velocity conditions follow each contact pass (UpdateMomentaTask::ContactAndMomentaBCs)
     const bool bcs = c->kbc.nb > 0;
+    const int par = c->parity;
     auto vel_bcs = [&](int pass) {
-        if (bcs) launch_vel_bcs(c->kg, c->kn, c->kbc, pass, s);
+        if (bcs) launch_vel_bcs(c->kg, c->kn, c->kbc, pass, par, s);
     };
     launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
     if (!mm) vel_bcs(BC_PASS_MASS_MOMENTUM);
@@ -75,7 +83,8 @@ static void enqueue_step(MpmCtx *c) {
     }
     launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
     done(MPMGPU_KG_USF);
-    launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s);
+    if (c->kld.nl) launch_particle_loads(c->kg, c->kld, par, s);    // InitializationTask's pFext
+    launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, s);
     done(MPMGPU_KG_P2G_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
     vel_bcs(mm ? BC_PASS_GRID_FORCES : BC_PASS_FORCES_MOMENTA);
@@ -305,6 +314,19 @@ int mpmgpu_problem_velocity_bcs(const MpmProblem *pc, MpmVelocityBC *bcs, int64_
     if (bcs) {
         const int64_t k = std::min(*n, have);
         for (int64_t i = 0; i < k; i++) bcs[i] = p->velBCs[i];
+    }
+    *n = have;
+    return 0;
+}
+
+int mpmgpu_problem_particle_loads(const MpmProblem *pc, MpmParticleLoad *loads, int64_t *n) {
+    MpmProblem *p = const_cast<MpmProblem *>(pc);
+    if (!n) return fail_global("null count");
+    if (!mpmsetup::finalize(*p)) return fail_global(p->err);
+    const int64_t have = (int64_t)p->loads.size();
+    if (loads) {
+        const int64_t k = std::min(*n, have);
+        for (int64_t i = 0; i < k; i++) loads[i] = p->loads[i];
     }
     *n = have;
     return 0;
@@ -703,6 +725,8 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         HIPCHECK(c, hipMemcpyAsync(c->dinst, c->inst.data(), c->inst.size() * sizeof(KGrid), hipMemcpyHostToDevice,
                                    c->stream));
     }
+    // loads name particles of the previous upload
+    if (c->kld.nl && mpmgpu_set_particle_loads(c, 0, nullptr)) return 1;
     if (mpmi::place_particles(c, n)) return 1;
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -747,7 +771,8 @@ int mpmgpu_set_velocity_bcs(MpmCtx *c, int64_t n, const MpmVelocityBC *bcs) {
     start.push_back((int)n);
     const size_t nb = node.size(), ne = (size_t)n, nf = (size_t)c->kg.nfields;
     // one buffer: doubles first (value, ftime, cur, save, time), then node indices, then ints
-    const size_t nd = 3 * ne + 3 * nb * nf + 1;
+    if (ensure_time_slots(c)) return 1;
+    const size_t nd = 3 * ne + 3 * nb * nf;
     const size_t bytes = nd * sizeof(double) + nb * sizeof(long long) + (nb + 1 + 3 * ne) * sizeof(int);
     HIPCHECK(c, hipMalloc(&c->bcbuf, bytes));
     std::vector<char> h(bytes, 0);
@@ -770,12 +795,76 @@ int mpmgpu_set_velocity_bcs(MpmCtx *c, int64_t n, const MpmVelocityBC *bcs) {
     k.ftime = dd + ne;
     k.cur = dd + 2 * ne;
     k.save = dd + 3 * ne;
-    k.time = dd + 3 * ne + 3 * nb * nf;
+    k.time = c->tbuf;
     k.node = dn;
     k.start = di;
     k.dir = di + nb + 1;
     k.style = di + nb + 1 + ne;
     k.act = di + nb + 1 + 2 * ne;
+    return put_bc_time(c);
+}
+
+int mpmgpu_set_particle_loads(MpmCtx *c, int64_t n, const MpmParticleLoad *loads) {
+    if (n < 0 || n > 0x7fffffffLL) return c->fail("bad particle load count");
+    if (n > 0 && !c->inst.empty()) return c->fail("particle loads are for single simulations, not batches");
+    if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
+    const long long np = c->kp.n;
+    for (int64_t i = 0; i < n; i++) {
+        if (loads[i].particle < 0 || loads[i].particle >= np) return c->fail("particle load on a particle that was not uploaded");
+        if (loads[i].dir < 1 || loads[i].dir > c->dim) return c->fail("particle load direction must be 1 to the dimension");
+        if (loads[i].style < 1 || loads[i].style > 4) return c->fail("load styles other than 1 to 4 are not on this path");
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    destroy_graphs(c);
+    if (c->ldbuf) { HIPCHECK(c, hipFree(c->ldbuf)); c->ldbuf = nullptr; }
+    c->kld = KLoads{};
+    if (n == 0) return 0;
+    // group by particle, list order kept within a particle
+    std::vector<int64_t> ord((size_t)n);
+    for (int64_t i = 0; i < n; i++) ord[(size_t)i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return loads[a].particle < loads[b].particle; });
+    std::vector<int> map((size_t)np, -1), start, dir, style;
+    std::vector<double> value, ftime;
+    for (size_t k = 0; k < ord.size(); k++) {
+        const MpmParticleLoad &l = loads[ord[k]];
+        if (map[(size_t)l.particle] < 0) {
+            map[(size_t)l.particle] = (int)start.size();
+            start.push_back((int)k);
+        }
+        dir.push_back(l.dir - 1);
+        style.push_back(l.style);
+        value.push_back(l.value);
+        ftime.push_back(l.ftime);
+    }
+    start.push_back((int)n);
+    const size_t nl = start.size() - 1, ne = (size_t)n;
+    if (ensure_time_slots(c)) return 1;
+    // doubles (value, ftime, fext), then ints (map, start, dir, style)
+    const size_t nd = 2 * ne + 3 * nl, ni = (size_t)np + nl + 1 + 2 * ne;
+    const size_t bytes = nd * sizeof(double) + ni * sizeof(int);
+    HIPCHECK(c, hipMalloc(&c->ldbuf, bytes));
+    std::vector<char> h(bytes, 0);
+    double *hd = (double *)h.data();
+    memcpy(hd, value.data(), ne * sizeof(double));
+    memcpy(hd + ne, ftime.data(), ne * sizeof(double));
+    int *hi = (int *)(hd + nd);
+    memcpy(hi, map.data(), (size_t)np * sizeof(int));
+    memcpy(hi + np, start.data(), (nl + 1) * sizeof(int));
+    memcpy(hi + np + nl + 1, dir.data(), ne * sizeof(int));
+    memcpy(hi + np + nl + 1 + ne, style.data(), ne * sizeof(int));
+    HIPCHECK(c, hipMemcpy(c->ldbuf, h.data(), bytes, hipMemcpyHostToDevice));
+    double *dd = (double *)c->ldbuf;
+    int *di = (int *)(dd + nd);
+    KLoads &L = c->kld;
+    L.nl = (int)nl;
+    L.value = dd;
+    L.ftime = dd + ne;
+    L.fext = dd + 2 * ne;
+    L.map = di;
+    L.start = di + np;
+    L.dir = di + np + nl + 1;
+    L.style = di + np + nl + 1 + ne;
+    L.time = c->tbuf;
     return put_bc_time(c);
 }
 
@@ -785,7 +874,9 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
     hipStream_t s = c->stream;
     set_positions(c);
     switch (task) {
-        case MPMGPU_TASK_INITIALIZATION: break;   // natural coordinates are found in each pass
+        case MPMGPU_TASK_INITIALIZATION:          // natural coordinates are found in each pass
+            if (c->kld.nl) launch_particle_loads(c->kg, c->kld, c->parity, s);
+            break;
         case MPMGPU_TASK_MASS_MOMENTUM:
             launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
             launch_node_mass(c->dim, c->kg, c->kb, c->kn, s);
@@ -796,16 +887,16 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
                 launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
             }
-            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_MASS_MOMENTUM, s);
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_MASS_MOMENTUM, c->parity, s);
             break;
         case MPMGPU_TASK_UPDATE_STRAINS_FIRST: launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
-        case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
+        case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, s); break;
         case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
         case MPMGPU_TASK_UPDATE_MOMENTA:
             launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
-            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_GRID_FORCES, s);
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_GRID_FORCES, c->parity, s);
             if (c->kg.nfields > 1) launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
-            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_MOMENTUM, s);
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_MOMENTUM, c->parity, s);
             break;
         case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
         case MPMGPU_TASK_UPDATE_STRAINS_LAST:
@@ -815,7 +906,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
                 launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
             }
-            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_STRAINS_LAST, s);
+            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_STRAINS_LAST, c->parity, s);
             launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
             break;
         case MPMGPU_TASK_RESET_ELEMENTS:
@@ -1080,6 +1171,8 @@ void mpmgpu_destroy(MpmCtx *c) {
     if (c->dmats) (void)hipFree(c->dmats);
     if (c->sbuf) (void)hipFree(c->sbuf);
     if (c->bcbuf) (void)hipFree(c->bcbuf);
+    if (c->ldbuf) (void)hipFree(c->ldbuf);
+    if (c->tbuf) (void)hipFree(c->tbuf);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

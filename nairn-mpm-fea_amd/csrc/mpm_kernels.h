@@ -111,6 +111,8 @@ enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
 // contact call types (UpdateMomentaTask::ContactAndMomentaBCs passType)
 enum { CONTACT_MASS_MOMENTUM = 0, CONTACT_UPDATE_MOMENTUM = 1, CONTACT_UPDATE_STRAINS_LAST = 2 };
 
+struct KLoads;
+
 // host-side launchers (mpm_kernels.hip); all take the context's stream
 void set_tile_cus(int cus);        // compute units the tile kernels' grids are sized for
 void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
@@ -120,7 +122,8 @@ void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, co
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                 const MpmMaterial *mats, hipStream_t s);
-void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s);
+void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
+                      hipStream_t s);
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                     hipStream_t s);
@@ -148,14 +151,28 @@ struct KBcs {
     double *cur;               // per entry: value of the step (written by the mass-momentum pass)
     int *act;                  // per entry: acts in this step
     double *save;              // nb * nfields * 3: momenta before the mass-momentum pass
-    double *time;              // the step's start time; the strains-last pass advances it
+    double *time;              // two slots: slot par is the step's start time, slot par ^ 1 the next's
 };
 // passes of NodalVelBC::GridVelocityConditions; GRID_FORCES runs the force pass and the momentum
 // update of the nodes with conditions, FORCES_MOMENTA adds the UPDATE_MOMENTUM pass in the same
 // launch (single-material mode, where no contact pass comes between)
 enum { BC_PASS_MASS_MOMENTUM = 0, BC_PASS_GRID_FORCES, BC_PASS_UPDATE_MOMENTUM, BC_PASS_FORCES_MOMENTA,
        BC_PASS_UPDATE_STRAINS_LAST };
-void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, hipStream_t s);
+void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, int par, hipStream_t s);
+
+// External particle loads of one simulation: map[upload index] is -1 or the loaded particle's
+// slot q, whose loads are entries start[q]..start[q+1]-1 in list order; fext holds the step's
+// pFext of each slot (3 doubles), read by the force scatter through the particle's upload index.
+struct KLoads {
+    int nl;                    // loaded particles (0: no loads)
+    const int *map;
+    const int *start;
+    const int *dir, *style;
+    const double *value, *ftime;
+    double *fext;
+    double *time;              // the two step start time slots shared with KBcs
+};
+void launch_particle_loads(const KGrid &g, const KLoads &L, int par, hipStream_t s);
 
 // A run of particles copied between the particle buffers of two contexts (the solve server moves
 // a resident solve into the next batch layout). Double rows are read from sd (row stride sn)

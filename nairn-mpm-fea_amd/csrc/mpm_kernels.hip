@@ -21,6 +21,8 @@
 //   k_usl_reset   UpdateStrainsFirstTask::FullStrainUpdate(last) + ResetElementsTask.cpp:44-265
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mpm_device.h"
 #include "mpm_kernels.h"
 
@@ -570,6 +572,9 @@ __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
 struct RecMass { double wx, wy, wz, mp, vx, vy, vz; };
 struct RecForce { double wx, wy, wz, dx, dy, dz, s0, s1, s2, s3, s4, s5; };
 struct RecForce2 { double2 x, y, z; double a[6]; };
+// with external particle loads: pFext too
+struct RecForceX : RecForce { double e[3]; };
+struct RecForce2X : RecForce2 { double e[3]; };
 
 template <int DIM, int NQ>
 __device__ __forceinline__ void zero_acc(double (*s_acc)[NQ][WaveFoot<DIM>::SIZE]) {
@@ -1077,9 +1082,9 @@ __global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNode
 
 // internal-force scatter (GridForcesTask::Execute, GridForcesTask.cpp:40-153 ->
 // MatPoint3D::GetFintPlusFext, MatPoint3D.cpp:248-270; MatPoint2D.cpp:242-263) with the stress
-// just updated by k_usf; pFext is zero (no particle loads)
-template <int DIM, int SH>
-__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KNodes N) {
+// just updated by k_usf; FX adds N * pFext of the loaded particles (pFext is zero otherwise)
+template <int DIM, int SH, bool FX>
+__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KNodes N, KLoads L) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NCOR = DIM == 3 ? 8 : 4;
@@ -1093,7 +1098,16 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     __shared__ double s_ct[CP ? CH : 1][NCOR][DIM][4];
     __shared__ double s_cwg[CP ? CH : 1][NCOR][3];
     __shared__ double s_st[CP ? CH : 1][NA];
+    __shared__ double s_e[FX ? CH : 1][3];    // pFext of the staged particles
     __shared__ double s_acc[NW][NQ_FORCE][WaveFoot<DIM>::SIZE];
+    using R1 = std::conditional_t<FX, RecForceX, RecForce>;
+    using R2 = std::conditional_t<FX, RecForce2X, RecForce2>;
+    // MatPtLoadBC's pFext of particle p (0 when it carries no load)
+    auto stage_fext = [&](int slot, int p) {
+        const int q = L.map[B.home[p]];
+#pragma unroll
+        for (int c = 0; c < 3; c++) s_e[slot][c] = q >= 0 ? L.fext[3 * q + c] : 0.;
+    };
     auto tile = [&](const int tall) {      // (field, context tile) pair
         const int fld = tall / g0.tilesTotal;
         const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
@@ -1121,6 +1135,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                     cpdi_wg_staged<DIM>(r, c, w);
                     s_cwg[i][c][0] = w[0]; s_cwg[i][c][1] = w[1]; s_cwg[i][c][2] = w[2];
                     if (c == 0) {
+                        if constexpr (FX) stage_fext(i, p);
                         const double Pr = P.f[MPF_PRESSURE][p];
                         const double negmp = -P.f[MPF_MASS][p];
                         s_st[i][0] = negmp * (P.f[MPF_SP_XX][p] - Pr);
@@ -1154,6 +1169,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                             const bool in = S[a][o] >= 0.;
                             s_F[threadIdx.x][a][o] = make_double2(in ? S[a][o] : 0., in ? dS[a][o] * invd[a] : 0.);
                         }
+                    if constexpr (FX) stage_fext(threadIdx.x, p);
                     s_a[threadIdx.x][0] = negmp * sxx;
                     s_a[threadIdx.x][1] = negmp * syy;
                     s_a[threadIdx.x][2] = negmp * sxy;
@@ -1169,15 +1185,23 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                 // f = -mp (sigma - P) . grad N; grad N from the staged factor pairs, accumulated with FMA
                 wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
                     [&](int lq, int ix, int iy, int iz) {
-                        RecForce2 r;
+                        R2 r;
                         r.x = s_F[lq][0][ix];
                         r.y = s_F[lq][1][iy];
                         r.z = DIM == 3 ? s_F[lq][DIM - 1][iz] : make_double2(1., 0.);
 #pragma unroll
                         for (int q = 0; q < NA; q++) r.a[q] = s_a[lq][q];
+                        if constexpr (FX) { r.e[0] = s_e[lq][0]; r.e[1] = s_e[lq][1]; r.e[2] = s_e[lq][2]; }
                         return r;
                     },
-                    [&](const RecForce2 &r, double *acc, int, int, int) {
+                    [&](const R2 &r, double *acc, int, int, int) {
+                        if constexpr (FX) {
+                            // + N pFext, N the product of the axis factors
+                            const double w = DIM == 3 ? r.x.x * (r.y.x * r.z.x) : r.x.x * r.y.x;
+                            acc[0] = fma(r.e[0], w, acc[0]);
+                            acc[1] = fma(r.e[1], w, acc[1]);
+                            if (DIM == 3) acc[2] = fma(r.e[2], w, acc[2]);
+                        }
                         if (DIM == 3) {
                             const double gx = r.x.y * (r.y.x * r.z.x);
                             const double gy = r.y.y * (r.x.x * r.z.x);
@@ -1198,7 +1222,7 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
             }
             wave_cells<DIM, NQ_FORCE>(s_cs, c0, min(c0 + CH, te), s_acc,
                 [&](int lq, int ix, int iy, int iz) {
-                    RecForce r;
+                    R1 r;
                     {
                         double gr[3];
                         cpdi_table_node<DIM, true>(s_ct[lq], s_cwg[lq], ix, iy, iz, r.wx, gr);
@@ -1208,9 +1232,15 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                     r.s0 = s_st[lq][0]; r.s1 = s_st[lq][1]; r.s2 = s_st[lq][2];
                     if constexpr (DIM == 3) { r.s3 = s_st[lq][3]; r.s4 = s_st[lq][4]; r.s5 = s_st[lq][5]; }
                     else { r.s3 = r.s4 = r.s5 = 0.; }
+                    if constexpr (FX) { r.e[0] = s_e[lq][0]; r.e[1] = s_e[lq][1]; r.e[2] = s_e[lq][2]; }
                     return r;
                 },
-                [&](const RecForce &r, double *acc, int, int, int) {
+                [&](const R1 &r, double *acc, int, int, int) {
+                    if constexpr (FX) {              // + N pFext, N the merged corner weight
+                        acc[0] = fma(r.e[0], r.wx, acc[0]);
+                        acc[1] = fma(r.e[1], r.wx, acc[1]);
+                        if (DIM == 3) acc[2] = fma(r.e[2], r.wx, acc[2]);
+                    }
                     // s: -mp times xx, yy, xy, zz, xz, yz of sigma - P; non-member nodes have zero gradient
                     if (DIM == 3) {
                         acc[0] = fma(r.s0, r.dx, fma(r.s2, r.dy, fma(r.s4, r.dz, acc[0])));
@@ -2028,15 +2058,26 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
     }
 }
 
-void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
+void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
+                      hipStream_t s) {
     const KBins tb = tile_bins(b, TILE_SLOT_FORCE);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (L.nl > 0) {
+        if (dim == 3) {
+            if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, L);
+            else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, L);
+        } else {
+            if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, L);
+            else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, L);
+        }
+        return;
+    }
     if (dim == 3) {
-        if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
-        else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
+        if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, L);
+        else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, L);
     } else {
-        if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
-        else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
+        if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, L);
+        else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, L);
     }
 }
 
@@ -2120,16 +2161,18 @@ void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &
 
 // BoundaryCondition::GetNodeNum and BCValue (BoundaryCondition.cpp:72-140): 1 if the condition
 // acts at time t, its value in v
-__device__ __forceinline__ int bc_value(const KBcs &bc, int e, double t, double &v) {
-    const double val = bc.value[e], ft = bc.ftime[e];
+__device__ __forceinline__ int style_value(int style, double val, double ft, double t, double &v) {
     v = 0.;
-    switch (bc.style[e]) {
+    switch (style) {
         case 1: if (t >= ft) { v = val; return 1; } return 0;
         case 2: if (t >= ft) { v = 0. + val * (t - ft); return 1; } return 0;
         case 3: v = val * sin(t * ft); return 1;
         case 4: v = val * cos(t * ft); return 1;
         default: return 0;
     }
+}
+__device__ __forceinline__ int bc_value(const KBcs &bc, int e, double t, double &v) {
+    return style_value(bc.style[e], bc.value[e], bc.ftime[e], t, v);
 }
 
 // one pass on momenta p and forces fo of one field of one node (ZeroVelocityBC, then AddVelocityBC)
@@ -2170,19 +2213,19 @@ __device__ __forceinline__ void bc_pass(const KBcs &bc, int e0, int e1, double t
 // one thread per (node with conditions, field). MASS_MOMENTUM (after the contact pass, before the
 // first strain update) keeps the momenta it starts from; GRID_FORCES starts from those again
 // (MatVelocityField::RestoreMomenta), sets the forces and redoes the momentum update of the node
-// (pk + ftot*dt, as k_node_force); the strains-last pass advances the start time of the next step.
+// (pk + ftot*dt, as k_node_force). The mass-momentum pass reads the step's start time from slot
+// par and writes the next step's into slot par ^ 1, which no thread of this step reads.
 template <int PASS>
-__global__ void k_vel_bcs(KGrid g, KNodes N, KBcs bc) {
+__global__ void k_vel_bcs(KGrid g, KNodes N, KBcs bc, int par) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    // no thread of this pass reads the time
-    if (PASS == BC_PASS_UPDATE_STRAINS_LAST && idx == 0) *bc.time = *bc.time + g.timestep;
+    if (PASS == BC_PASS_MASS_MOMENTUM && idx == 0) bc.time[par ^ 1] = bc.time[par] + g.timestep;
     if (idx >= (long long)bc.nb * g.nfields) return;
     const int f = (int)(idx / bc.nb);
     const int b = (int)(idx - (long long)f * bc.nb);
     const int e0 = bc.start[b], e1 = bc.start[b + 1];
     double t = 0.;
     if (PASS == BC_PASS_MASS_MOMENTUM) {
-        t = *bc.time;
+        t = bc.time[par];
         if (f == 0)
             for (int e = e0; e < e1; e++) {
                 double v;
@@ -2218,15 +2261,37 @@ __global__ void k_vel_bcs(KGrid g, KNodes N, KBcs bc) {
     }
 }
 
-void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, hipStream_t s) {
+void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, int par, hipStream_t s) {
     const dim3 grid(blocks_for((long long)bc.nb * g.nfields, NT)), blk(NT);
     switch (pass) {
-        case BC_PASS_MASS_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_MASS_MOMENTUM>, grid, blk, 0, s, g, nd, bc); break;
-        case BC_PASS_GRID_FORCES: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_GRID_FORCES>, grid, blk, 0, s, g, nd, bc); break;
-        case BC_PASS_UPDATE_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_MOMENTUM>, grid, blk, 0, s, g, nd, bc); break;
-        case BC_PASS_FORCES_MOMENTA: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_FORCES_MOMENTA>, grid, blk, 0, s, g, nd, bc); break;
-        default: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_STRAINS_LAST>, grid, blk, 0, s, g, nd, bc); break;
+        case BC_PASS_MASS_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_MASS_MOMENTUM>, grid, blk, 0, s, g, nd, bc, par); break;
+        case BC_PASS_GRID_FORCES: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_GRID_FORCES>, grid, blk, 0, s, g, nd, bc, par); break;
+        case BC_PASS_UPDATE_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_MOMENTUM>, grid, blk, 0, s, g, nd, bc, par); break;
+        case BC_PASS_FORCES_MOMENTA: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_FORCES_MOMENTA>, grid, blk, 0, s, g, nd, bc, par); break;
+        default: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_STRAINS_LAST>, grid, blk, 0, s, g, nd, bc, par); break;
     }
+}
+
+// MatPtLoadBC::SetParticleFext (MatPtLoadBC.cpp:210-221, called from InitializationTask.cpp:91):
+// one thread per loaded particle zeroes its pFext and adds each of its loads' value at the
+// step's start time (BCValue, zero before a constant or linear load starts), in list order.
+// Like the mass-momentum velocity pass it writes the next step's start time into slot par ^ 1.
+__global__ void k_particle_loads(KGrid g, KLoads L, int par) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const double t = L.time[par];
+    if (q == 0) L.time[par ^ 1] = t + g.timestep;
+    if (q >= L.nl) return;
+    double fe[3] = {0., 0., 0.};
+    for (int e = L.start[q]; e < L.start[q + 1]; e++) {
+        double v;
+        style_value(L.style[e], L.value[e], L.ftime[e], t, v);
+        fe[L.dir[e]] += v;
+    }
+    for (int c = 0; c < 3; c++) L.fext[3 * q + c] = fe[c];
+}
+
+void launch_particle_loads(const KGrid &g, const KLoads &L, int par, hipStream_t s) {
+    hipLaunchKernelGGL(k_particle_loads, dim3(blocks_for(L.nl, NT)), dim3(NT), 0, s, g, L, par);
 }
 
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s) {

@@ -56,6 +56,8 @@ struct MpmProblem {
     std::vector<MaterialInput> autoLaws;
     // <GridBCs> velocity conditions in the reference's list order
     std::vector<MpmVelocityBC> velBCs;
+    // <ParticleBCs> loads in the reference's list order
+    std::vector<MpmParticleLoad> loads;
     // particles, in mpm[] order
     std::vector<double> d[MPF_NUM_FIELDS];
     std::vector<int32_t> iv[MPI_NUM_FIELDS];

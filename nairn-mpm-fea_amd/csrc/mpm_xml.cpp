@@ -512,6 +512,93 @@ struct Reader {
         return diff <= l * 1.e-7;
     }
 
+    // a <BCLine> (2D) or <BCBox> (3D) of <GridBCs> or <ParticleBCs> (Read_MPM/Generators.cpp:831-891)
+    struct BcShape {
+        bool line = false;
+        double lo[3] = {0., 0., 0.}, hi[3] = {0., 0., 0.};
+        double dtol = 0., distSq = 0.;
+        // LineController::ContainsPoint (LineController.cpp:49-66); BoxController::ContainsPoint,
+        // axis 0 (BoxController.cpp:70-74)
+        bool contains(double x, double y, double z) const {
+            if (!line)
+                return x <= hi[0] && x >= lo[0] && y <= hi[1] && y >= lo[1] && z <= hi[2] && z >= lo[2];
+            double dx = hi[0] - lo[0], dy = hi[1] - lo[1];
+            double px = x - lo[0], py = y - lo[1];
+            double ypr = -dy * px + dx * py;
+            if (ypr > dtol || ypr < -dtol) return false;
+            double xpr = dx * px + dy * py;
+            return !(xpr < 0. || xpr > distSq);
+        }
+    };
+
+    // ElementBase::gridTolerance, 0.1 of the shortest element side (ElementBase.cpp:60-84,
+    // ElementBase3D.cpp:30-44); GetMinimumCellSize divides it by 0.1 again (:150)
+    double grid_tolerance() const {
+        const MpmGridDesc &g = p.grid;
+        double gtol = 1.e20;
+        const long nn[3] = {g.nx + 1L, g.ny + 1L, p.dim == 3 ? g.nz + 1L : 1L};
+        const double o[3] = {g.xmin, g.ymin, g.zmin}, d[3] = {g.dx, g.dy, g.dz};
+        for (int c = 0; c < p.dim; c++)
+            for (long i = 0; i + 1 < nn[c]; i++)
+                gtol = std::min(gtol, 0.1 * ((o[c] + (double)(i + 1) * d[c]) - (o[c] + (double)i * d[c])));
+        return gtol;
+    }
+
+    bool bc_shape(const XNode &s, BcShape &sh) {
+        const int dim = p.dim;
+        const bool line = s.name == "BCLine", box = s.name == "BCBox";
+        if (!line && !box) return fail("<" + s.name + "> in <GridBCs> or <ParticleBCs> is not on this path");
+        if (line && dim != 2) return fail("<BCLine> is for 2D analyses");
+        if (box && dim != 3) return fail("<BCBox> is for 3D analyses");
+        const double gtol = grid_tolerance(), minCell = gtol / 0.1;
+        const double ls = length_units(s);
+        sh.line = line;
+        double tol = gtol;
+        // ShapeController::SetProperty (ShapeController.cpp:105-125): x1/xmin ... z2/zmax,
+        // the later attribute of a pair wins
+        for (auto &a : s.attrs) {
+            const std::string &k = a.first;
+            const char *v = a.second.c_str();
+            for (int c = 0; c < 3; c++) {
+                const char ax = "xyz"[c];
+                const double amin = p.meshLim[2 * c], amax = p.meshLim[2 * c + 1];
+                if (k == std::string(1, ax) + "1" || k == std::string(1, ax) + "min")
+                    sh.lo[c] = grid_point(v, ls, amin, amax, minCell);
+                else if (k == std::string(1, ax) + "2" || k == std::string(1, ax) + "max")
+                    sh.hi[c] = grid_point(v, ls, amin, amax, minCell);
+            }
+            if (k == "tolerance" && line)      // LineController::SetProperty (LineController.cpp:91-105)
+                tol = v[0] == '*' ? strtod(v + 1, nullptr) * minCell : strtod(v, nullptr) * ls;
+            if (k == "axis" && box && std::string(v) != "0") {     // BoxController::SetProperty (:22-36)
+                const std::string av(v);
+                if (av == "x" || av == "X" || av == "1" || av == "y" || av == "Y" || av == "2" || av == "z" ||
+                    av == "Z" || av == "3")
+                    return fail("<BCBox axis=> (cylinder node sets) is not on this path");
+                return fail("Box axis must be x, y, z, 1, 2, or 3");
+            }
+        }
+        if (line) {
+            // LineController::FinishSetup (LineController.cpp:33-42)
+            sh.distSq = (sh.hi[0] - sh.lo[0]) * (sh.hi[0] - sh.lo[0]) + (sh.hi[1] - sh.lo[1]) * (sh.hi[1] - sh.lo[1]);
+            sh.dtol = std::sqrt(sh.distSq) * tol;
+        } else {
+            // ShapeController::FinishSetup (ShapeController.cpp:164-197): ordered, nonzero extents
+            for (int c = 0; c < 3; c++) {
+                if (sh.lo[c] > sh.hi[c]) std::swap(sh.lo[c], sh.hi[c]);
+                if (dble_equal(sh.lo[c], sh.hi[c]))
+                    return fail(std::string("Box: ") + "xyz"[c] + "max cannot equal " + "xyz"[c] + "min in input parameters.");
+            }
+        }
+        return true;
+    }
+
+    // common attributes of <DisBC> and <LoadBC>: the style, and ftime converted as
+    // BoundaryCondition::SetBCFirstTime does (BoundaryCondition.cpp:200-206): ms -> s, or
+    // 1/ms -> 1/s for the sine and cosine styles
+    static double first_time(int style, double ftime) {
+        return (style == 3 || style == 4) ? ftime * 1.e3 : ftime * 1.e-3;
+    }
+
     // <GridBCs>: <BCLine> (2D) or <BCBox> (3D) node sets holding <DisBC> velocity conditions
     // (MPMReadHandler, Read_MPM/Generators.cpp:831-1010). Each DisBC adds one condition per node
     // the shape contains, nodes in node order (ShapeController::nextNode,
@@ -521,71 +608,10 @@ struct Reader {
         const MpmGridDesc &g = p.grid;
         const int dim = p.dim;
         const long nnx = g.nx + 1, nny = g.ny + 1, nnz = dim == 3 ? g.nz + 1 : 1;
-        auto xn = [&](long i) { return g.xmin + (double)i * g.dx; };
-        auto yn = [&](long j) { return g.ymin + (double)j * g.dy; };
-        auto zn = [&](long k) { return g.zmin + (double)k * g.dz; };
-        // ElementBase::gridTolerance, 0.1 of the shortest element side (ElementBase.cpp:60-84,
-        // ElementBase3D.cpp:30-44); GetMinimumCellSize divides it by 0.1 again (:150)
-        double gtol = 1.e20;
-        for (long i = 0; i + 1 < nnx; i++) gtol = std::min(gtol, 0.1 * (xn(i + 1) - xn(i)));
-        for (long j = 0; j + 1 < nny; j++) gtol = std::min(gtol, 0.1 * (yn(j + 1) - yn(j)));
-        if (dim == 3) for (long k = 0; k + 1 < nnz; k++) gtol = std::min(gtol, 0.1 * (zn(k + 1) - zn(k)));
-        const double minCell = gtol / 0.1;
         for (auto &sp : gb.kids) {
             const XNode &s = *sp;
-            const bool line = s.name == "BCLine", box = s.name == "BCBox";
-            if (!line && !box) return fail("<" + s.name + "> in <GridBCs> is not on this path");
-            if (line && dim != 2) return fail("<BCLine> is for 2D analyses");
-            if (box && dim != 3) return fail("<BCBox> is for 3D analyses");
-            const double ls = length_units(s);
-            // ShapeController::SetProperty (ShapeController.cpp:105-125): x1/xmin ... z2/zmax,
-            // the later attribute of a pair wins
-            double lo[3] = {0., 0., 0.}, hi[3] = {0., 0., 0.};
-            double tol = gtol;
-            for (auto &a : s.attrs) {
-                const std::string &k = a.first;
-                const char *v = a.second.c_str();
-                for (int c = 0; c < 3; c++) {
-                    const char ax = "xyz"[c];
-                    const double amin = p.meshLim[2 * c], amax = p.meshLim[2 * c + 1];
-                    if (k == std::string(1, ax) + "1" || k == std::string(1, ax) + "min")
-                        lo[c] = grid_point(v, ls, amin, amax, minCell);
-                    else if (k == std::string(1, ax) + "2" || k == std::string(1, ax) + "max")
-                        hi[c] = grid_point(v, ls, amin, amax, minCell);
-                }
-                if (k == "tolerance" && line)      // LineController::SetProperty (LineController.cpp:91-105)
-                    tol = v[0] == '*' ? strtod(v + 1, nullptr) * minCell : strtod(v, nullptr) * ls;
-                if (k == "axis" && box && std::string(v) != "0") {     // BoxController::SetProperty (:22-36)
-                    const std::string av(v);
-                    if (av == "x" || av == "X" || av == "1" || av == "y" || av == "Y" || av == "2" || av == "z" ||
-                        av == "Z" || av == "3")
-                        return fail("<BCBox axis=> (cylinder node sets) is not on this path");
-                    return fail("Box axis must be x, y, z, 1, 2, or 3");
-                }
-            }
-            double dtol = 0., distSq = 0.;
-            if (line) {
-                // LineController::FinishSetup (LineController.cpp:33-42)
-                distSq = (hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]);
-                dtol = std::sqrt(distSq) * tol;
-            } else {
-                // ShapeController::FinishSetup (ShapeController.cpp:164-197): ordered, nonzero extents
-                for (int c = 0; c < 3; c++) {
-                    if (lo[c] > hi[c]) std::swap(lo[c], hi[c]);
-                    if (dble_equal(lo[c], hi[c]))
-                        return fail(std::string("Box: ") + "xyz"[c] + "max cannot equal " + "xyz"[c] + "min in input parameters.");
-                }
-            }
-            auto contains = [&](double x, double y, double z) {
-                if (box)                           // BoxController::ContainsPoint, axis 0 (BoxController.cpp:70-74)
-                    return x <= hi[0] && x >= lo[0] && y <= hi[1] && y >= lo[1] && z <= hi[2] && z >= lo[2];
-                double dx = hi[0] - lo[0], dy = hi[1] - lo[1];     // LineController::ContainsPoint (:49-66)
-                double px = x - lo[0], py = y - lo[1];
-                double ypr = -dy * px + dx * py;
-                if (ypr > dtol || ypr < -dtol) return false;
-                double xpr = dx * px + dy * py;
-                return !(xpr < 0. || xpr > distSq);
-            };
+            BcShape sh;
+            if (!bc_shape(s, sh)) return false;
             for (auto &bp : s.kids) {
                 const XNode &b = *bp;
                 if (b.name != "DisBC")
@@ -614,20 +640,76 @@ struct Reader {
                     dof = (dof == 12 || dof == 13) ? 1 : (dof == 23 ? 2 : 3);
                 }
                 if (style < 1 || style > 4) return fail("velocity condition styles other than 1 to 4 are not on this path");
-                // BoundaryCondition::SetBCValue and SetBCFirstTime (BoundaryCondition.cpp:182-206):
-                // linear values per ms to per s, times ms to s, frequencies 1/ms to 1/s
+                // BoundaryCondition::SetBCValue (BoundaryCondition.cpp:182-188): linear values per ms to per s
                 MpmVelocityBC bc{};
                 bc.dir = dof;
                 bc.style = style;
                 bc.value = style == 2 ? vel * 1.e3 : vel;
-                bc.ftime = (style == 3 || style == 4) ? ftime * 1.e3 : ftime * 1.e-3;
+                bc.ftime = first_time(style, ftime);
                 for (long k = 0; k < nnz; k++)
                     for (long j = 0; j < nny; j++)
                         for (long i = 0; i < nnx; i++)
-                            if (contains(xn(i), yn(j), dim == 3 ? zn(k) : 0.)) {
+                            if (sh.contains(g.xmin + (double)i * g.dx, g.ymin + (double)j * g.dy,
+                                            dim == 3 ? g.zmin + (double)k * g.dz : 0.)) {
                                 bc.node = (k * nny + j) * nnx + i;
                                 p.velBCs.push_back(bc);
                             }
+            }
+        }
+        return true;
+    }
+
+    // <ParticleBCs>: <LoadBC> external loads on the particles a <BCLine> (2D) or <BCBox> (3D)
+    // contains, particles in mpm[] order (ShapeController::nextParticle, ShapeController.cpp:277-290);
+    // after <net/> the load is divided by the shape's particle count, <perParticle/> returns to
+    // one load per particle (Generators.cpp:1012-1023, ShapeController::setNetBC, :292-309)
+    bool particlebcs(const XNode &pb) {
+        const int dim = p.dim;
+        const int64_t n = p.nparticles();
+        for (auto &sp : pb.kids) {
+            const XNode &s = *sp;
+            BcShape sh;
+            if (!bc_shape(s, sh)) return false;
+            std::vector<int64_t> inside;
+            for (int64_t i = 0; i < n; i++)
+                if (sh.contains(p.d[MPF_POS_X][i], p.d[MPF_POS_Y][i], dim == 3 ? p.d[MPF_POS_Z][i] : 0.))
+                    inside.push_back(i);
+            bool net = false;
+            for (auto &bp : s.kids) {
+                const XNode &b = *bp;
+                if (b.name == "net") { net = true; continue; }
+                if (b.name == "perParticle") { net = false; continue; }
+                if (b.name != "LoadBC")
+                    return fail("<" + b.name + "> in <" + s.name + "> is not on this path");
+                int dof = 0, style = 1;
+                double load = 0., ftime = 0.;
+                for (auto &a : b.attrs) {
+                    const std::string &k = a.first;
+                    const char *v = a.second.c_str();
+                    if (k == "dir") dof = (int)strtol(v, nullptr, 10);
+                    else if (k == "style") style = (int)strtol(v, nullptr, 10);
+                    else if (k == "load" || k == "value" || k == "stress") load = strtod(v, nullptr);
+                    else if (k == "ftime" || k == "time" || k == "bath") ftime = strtod(v, nullptr);
+                    else if (k == "function") return fail("<LoadBC function=> is not on this path");
+                }
+                if (dim == 3 && (dof < 1 || dof > 3)) return fail("'dir' in <LoadBC> must be 1-3 for 3D analyses.");
+                if (dim == 2 && (dof < 1 || dof > 2))
+                    return fail("'dir' in <LoadBC> or <ConcFluxBC> must be 1 or 2 and in <TractionBC> must be 1, 2, 11, or 12 for 2D analyses.");
+                if (style < 1 || style > 4) return fail("load styles other than 1 to 4 are not on this path");
+                // MatPtLoadBC::SetBCValue (MatPtLoadBC.cpp:202-206): N -> uN, then the linear
+                // style's per ms -> per s; a net load is shared by the shape's particles
+                const double normalize = net && !inside.empty() ? (double)inside.size() : 1.;
+                double v = 1.e6 * (load / normalize);
+                if (style == 2) v = v * 1.e3;
+                MpmParticleLoad ld{};
+                ld.dir = dof;
+                ld.style = style;
+                ld.value = v;
+                ld.ftime = first_time(style, ftime);
+                for (int64_t i : inside) {
+                    ld.particle = i;
+                    p.loads.push_back(ld);
+                }
             }
         }
         return true;
@@ -653,7 +735,11 @@ struct Reader {
                 if (!gridbcs(*k)) return false;
                 continue;
             }
-            if (n == "ParticleBCs" || n == "CrackList" || n == "Thermal" || n == "CustomTasks" || n == "Gravity")
+            if (n == "ParticleBCs") {
+                if (!particlebcs(*k)) return false;
+                continue;
+            }
+            if (n == "CrackList" || n == "Thermal" || n == "CustomTasks" || n == "Gravity")
                 if (!k->kids.empty() || !k->text.empty())
                     return fail("<" + n + "> is not on this path");
         }

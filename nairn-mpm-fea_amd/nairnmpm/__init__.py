@@ -106,6 +106,13 @@ class VelocityBC(ctypes.Structure):
                 ("value", ctypes.c_double), ("ftime", ctypes.c_double)]
 
 
+class ParticleLoad(ctypes.Structure):
+    """one external particle load (MpmParticleLoad): particle (upload index), dir 1..3, style 1..4,
+    value (uN; linear: uN/s) and ftime (s, or 1/s for the sine and cosine styles)"""
+    _fields_ = [("particle", ctypes.c_int64), ("dir", ctypes.c_int32), ("style", ctypes.c_int32),
+                ("value", ctypes.c_double), ("ftime", ctypes.c_double)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -128,6 +135,8 @@ _SIGNATURES = {
     "mpmgpu_problem_particles": ([_P, _PD, _PI], ctypes.c_int),
     "mpmgpu_problem_archive": ([_P, ctypes.c_char_p, _PD, _PD], ctypes.c_int),
     "mpmgpu_problem_velocity_bcs": ([_P, ctypes.POINTER(VelocityBC), ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "mpmgpu_problem_particle_loads": ([_P, ctypes.POINTER(ParticleLoad), ctypes.POINTER(ctypes.c_int64)],
+                                      ctypes.c_int),
     "mpmgpu_problem_destroy": ([_P], None),
     "mpmgpu_archive_order": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
     "mpmgpu_archive_record_size": ([ctypes.c_char_p, ctypes.c_int32], ctypes.c_int32),
@@ -141,6 +150,7 @@ _SIGNATURES = {
                        ctypes.POINTER(RunParams), ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_upload_particles": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
     "mpmgpu_set_velocity_bcs": ([_P, ctypes.c_int64, ctypes.POINTER(VelocityBC)], ctypes.c_int),
+    "mpmgpu_set_particle_loads": ([_P, ctypes.c_int64, ctypes.POINTER(ParticleLoad)], ctypes.c_int),
     "mpmgpu_batch_create": ([ctypes.c_int32, ctypes.POINTER(InstanceDesc), ctypes.POINTER(Material), ctypes.c_int32,
                              ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
@@ -289,6 +299,14 @@ class Problem:
         _check(library().mpmgpu_problem_velocity_bcs(self._h, arr, ctypes.byref(n)))
         return [arr[k] for k in range(n.value)]
 
+    def particle_loads(self):
+        """the <ParticleBCs> loads in the reference's list order (ParticleLoad list)"""
+        n = ctypes.c_int64(0)
+        _check(library().mpmgpu_problem_particle_loads(self._h, None, ctypes.byref(n)))
+        arr = (ParticleLoad * max(n.value, 1))()
+        _check(library().mpmgpu_problem_particle_loads(self._h, arr, ctypes.byref(n)))
+        return [arr[k] for k in range(n.value)]
+
     def close(self):
         if self._h:
             library().mpmgpu_problem_destroy(self._h)
@@ -322,7 +340,16 @@ class Simulation:
         bcs = prob.velocity_bcs()
         if bcs:
             sim.set_velocity_bcs(bcs)
+        loads = prob.particle_loads()
+        if loads:
+            sim.set_particle_loads(loads)
         return sim
+
+    def set_particle_loads(self, loads):
+        """external particle loads (ParticleLoad list, reference list order, upload indices);
+        [] removes them; a later upload removes them too"""
+        arr = (ParticleLoad * max(len(loads), 1))(*loads)
+        _check(library().mpmgpu_set_particle_loads(self._h, len(loads), arr), self._h)
 
     def set_velocity_bcs(self, bcs):
         """grid velocity conditions (VelocityBC list, reference list order); [] removes them"""
@@ -438,8 +465,8 @@ class Batch(Simulation):
         replaces the problem's own initial particles (same count and order)"""
         descs, mats, ds, iss = [], [], [], []
         for k, prob in enumerate(problems):
-            if prob.velocity_bcs():
-                raise MPMError("grid velocity conditions are for single simulations, not batches")
+            if prob.velocity_bcs() or prob.particle_loads():
+                raise MPMError("grid velocity conditions and particle loads are for single simulations, not batches")
             g, m, run, n, _ = prob.describe()
             d, i = prob.particles() if particles is None or particles[k] is None else particles[k]
             i = i.copy()
@@ -516,8 +543,8 @@ class Server:
         return rid.value
 
     def submit_problem(self, prob: Problem, steps: int, particles=None) -> int:
-        if prob.velocity_bcs():
-            raise MPMError("grid velocity conditions are for single simulations, not served solves")
+        if prob.velocity_bcs() or prob.particle_loads():
+            raise MPMError("grid velocity conditions and particle loads are for single simulations, not served solves")
         g, mats, run, _, _ = prob.describe()
         d, i = prob.particles() if particles is None else particles
         return self.submit(g, mats, run, d, i, steps)

@@ -127,20 +127,27 @@ def j2_block_3d(cells: int = 128, lo: int = 16, hi: int = 112, vx: float = 50000
 """)
 
 
-def grid_bcs(xml: str, shapes) -> str:
+def grid_bcs(xml: str, shapes, block: str = "GridBCs", child: str = "DisBC") -> str:
     """xml with a <GridBCs> block added: shapes is a list of (tag, attributes, [DisBC attributes]),
     tag "BCLine" (2D) or "BCBox" (3D), e.g.
-    ("BCLine", {"x1": 4, "y1": 4, "x2": 12, "y2": 4}, [{"dir": 2, "vel": 0}])"""
+    ("BCLine", {"x1": 4, "y1": 4, "x2": 12, "y2": 4}, [{"dir": 2, "vel": 0}]); a string in the
+    list is an empty element of that name (<net/>, <perParticle/>)"""
     def attrs(d):
         return " ".join(f'{k}="{v!r}"' if isinstance(v, float) else f'{k}="{v}"' for k, v in d.items())
-    body = "  <GridBCs>\n"
+    body = f"  <{block}>\n"
     for tag, a, bcs in shapes:
         body += f"    <{tag} {attrs(a)}>\n"
         for b in bcs:
-            body += f"      <DisBC {attrs(b)}/>\n"
+            body += f"      <{b}/>\n" if isinstance(b, str) else f"      <{child} {attrs(b)}/>\n"
         body += f"    </{tag}>\n"
-    body += "  </GridBCs>\n"
+    body += f"  </{block}>\n"
     return xml.replace("</JANFEAInput>", body + "</JANFEAInput>", 1)
+
+
+def particle_bcs(xml: str, shapes) -> str:
+    """xml with a <ParticleBCs> block of <LoadBC> loads, shapes as for grid_bcs, e.g.
+    ("BCLine", {...}, ["net", {"dir": 1, "load": 2.0, "style": 1}])"""
+    return grid_bcs(xml, shapes, block="ParticleBCs", child="LoadBC")
 
 
 def mooney_material_xml(G1: float = 8., G2: float = 2., K: float = 50., rho: float = 1.1,

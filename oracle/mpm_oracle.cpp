@@ -389,6 +389,33 @@ struct Oracle {
     std::vector<double> bcValue;
     std::vector<char> bcOn;
     enum { BC_MASS_MOMENTUM = 0, BC_GRID_FORCES = 1, BC_UPDATE_MOMENTUM = 2, BC_UPDATE_STRAINS_LAST = 3 };
+    // external particle loads in MatPtLoadBC list order (firstLoadedPt) and the particles' pFext
+    std::vector<MpmParticleLoad> loads;
+    std::vector<double> pfext;          // 3 per particle
+
+    // BoundaryCondition::BCValue (BoundaryCondition.cpp:72-104); acts = GetNodeNum != 0 (:124-137)
+    static double style_value(int style, double value, double ftime, double t, bool &acts) {
+        acts = true;
+        switch (style) {
+            case 1: acts = t >= ftime; return acts ? value : 0.;
+            case 2: acts = t >= ftime; return acts ? 0. + value * (t - ftime) : 0.;
+            case 3: return value * sin(t * ftime);
+            case 4: return value * cos(t * ftime);
+            default: acts = false; return 0.;
+        }
+    }
+
+    // MatPtLoadBC::SetParticleFext (MatPtLoadBC.cpp:210-221): ZeroMPLoad for every load, then
+    // AddMPLoad (:91-102) adds BCValue at the step's start time to its component
+    void set_particle_fext() {
+        if (loads.empty()) return;
+        for (const MpmParticleLoad &l : loads)
+            for (int c = 0; c < 3; c++) pfext[3 * l.particle + c] = 0.;
+        for (const MpmParticleLoad &l : loads) {
+            bool acts;
+            pfext[3 * l.particle + (l.dir - 1)] += style_value(l.style, l.value, l.ftime, mtime, acts);
+        }
+    }
 
     // lCPDI corners of the step (MPMBase CPDIDomain: inElem, ncpos, wg; ws = 1/8 or 1/4)
     struct Cpdi { long elem[8]; double xi[8][3]; double wg[8][3]; };
@@ -651,6 +678,7 @@ struct Oracle {
             err = "A CPDI particle domain node has left the grid";
             throw std::runtime_error(err);
         }
+        set_particle_fext();                 // InitializationTask.cpp:91
     }
 
     // NodalPoint::AddMassMomentum's multimaterial extras: CrackVelocityField::AddVolumeDisplacement
@@ -722,13 +750,9 @@ struct Oracle {
         bcOn.assign(vbcs.size(), 1);
         for (size_t e = 0; e < vbcs.size(); e++) {
             const MpmVelocityBC &b = vbcs[e];
-            switch (b.style) {
-                case 1: bcOn[e] = mtime >= b.ftime; if (bcOn[e]) bcValue[e] = b.value; break;
-                case 2: bcOn[e] = mtime >= b.ftime; if (bcOn[e]) bcValue[e] = 0. + b.value * (mtime - b.ftime); break;
-                case 3: bcValue[e] = b.value * sin(mtime * b.ftime); break;
-                case 4: bcValue[e] = b.value * cos(mtime * b.ftime); break;
-                default: bcOn[e] = 0; break;
-            }
+            bool acts;
+            bcValue[e] = style_value(b.style, b.value, b.ftime, mtime, acts);
+            bcOn[e] = acts;
         }
     }
 
@@ -1284,8 +1308,9 @@ struct Oracle {
     }
 
     // GridForcesTask::Execute (GridForcesTask.cpp:40-153) -> GetFintPlusFext
-    // (MatPoint3D.cpp:248-270, MatPoint2D.cpp:242-263); pFext is zero (no particle loads)
+    // (MatPoint3D.cpp:248-270, MatPoint2D.cpp:242-263); pFext is zero without particle loads
     void task_grid_forces() {
+        const bool fext = !loads.empty();
         scatter(ACC_FTOT, [&](long p, Acc &a) {
             ShapeOut o;
             shape(p, true, o);
@@ -1304,6 +1329,12 @@ struct Oracle {
                     fx = -mp * ((sxx - P) * o.dx[i] + sxy * o.dy[i]);
                     fy = -mp * (sxy * o.dx[i] + (syy - P) * o.dy[i]);
                     fz = 0.0;
+                }
+                if (fext) {                 // + fni * pFext
+                    const double *pe = &pfext[3 * p];
+                    fx = fx + o.fn[i] * pe[0];
+                    fy = fy + o.fn[i] * pe[1];
+                    if (g.dim == 3) fz = fz + o.fn[i] * pe[2];
                 }
                 a.ftot[3 * n] += fx; a.ftot[3 * n + 1] += fy; a.ftot[3 * n + 2] += fz;   // AddFtotTask3
             }
@@ -1555,6 +1586,20 @@ int oracle_set_velocity_bcs(void *h, int64_t n, const MpmVelocityBC *bcs) {
     }
     o->vbcs.assign(bcs, bcs + n);
     o->bc_values();                      // values at the current time until the next step's
+    return 0;
+}
+
+/* external particle loads in list order (n = 0 removes them) */
+int oracle_set_particle_loads(void *h, int64_t n, const MpmParticleLoad *loads) {
+    Oracle *o = (Oracle *)h;
+    for (int64_t i = 0; i < n; i++) {
+        if (loads[i].particle < 0 || loads[i].particle >= o->np) return 1;
+        if (loads[i].dir < 1 || loads[i].dir > o->g.dim) return 1;
+        if (loads[i].style < 1 || loads[i].style > 4) return 1;
+    }
+    o->loads.assign(loads, loads + n);
+    o->pfext.assign(3 * (size_t)o->np, 0.);
+    o->set_particle_fext();
     return 0;
 }
 

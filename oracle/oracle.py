@@ -40,6 +40,7 @@ def lib():
         _lib.oracle_run_task.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_set_threads.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_set_velocity_bcs.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.VelocityBC)]
+        _lib.oracle_set_particle_loads.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.ParticleLoad)]
         _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
         _lib.oracle_num_nodes.argtypes = [_P]
         _lib.oracle_num_nodes.restype = ctypes.c_int64
@@ -81,7 +82,17 @@ class Oracle:
         bcs = prob.velocity_bcs()
         if bcs:
             o.set_velocity_bcs(bcs)
+        loads = prob.particle_loads()
+        if loads:
+            o.set_particle_loads(loads)
         return o
+
+    def set_particle_loads(self, loads):
+        """external particle loads in list order (nairnmpm.ParticleLoad list)"""
+        import nairnmpm as nm
+        arr = (nm.ParticleLoad * max(len(loads), 1))(*loads)
+        if lib().oracle_set_particle_loads(self._h, len(loads), arr) != 0:
+            raise RuntimeError("bad particle load")
 
     def set_velocity_bcs(self, bcs):
         """grid velocity conditions in list order (nairnmpm.VelocityBC list)"""
