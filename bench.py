@@ -346,8 +346,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads of the CPU baseline (default: OMP_NUM_THREADS, else the cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["j2", "contact", "batch", "serve"], default="j2",
-                    help="j2: the 7.08M-particle J2 GIMP block (default); contact: the lCPDI contact pair; "
+    ap.add_argument("--workload", choices=["j2", "clamped", "contact", "batch", "serve"], default="j2",
+                    help="j2: the 7.08M-particle J2 GIMP block (default); clamped: the same block clamped on "
+                         "its bottom face and loaded on its top face; contact: the lCPDI contact pair; "
                          "batch: the 256-instance 2D sweep; serve: the served 3D stream")
     ap.add_argument("--requests", type=int, default=16, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
@@ -360,7 +361,7 @@ def main():
                     help="only run the plain copy used to scale the HBM byte counters")
     a = ap.parse_args()
     if a.steps is None:
-        a.steps = {"j2": 500, "contact": 200}.get(a.workload, 100)
+        a.steps = {"j2": 500, "clamped": 500, "contact": 200}.get(a.workload, 100)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
@@ -390,6 +391,19 @@ def main():
         xml = configs.j2_block_3d(cells=a.cells, lo=lo, hi=hi)
         workload = (f"3D J2 GIMP block, {a.cells}^3 cells of 1 mm, 2x2x2 particles per cell "
                     f"in [{lo},{hi}]^3 mm, halves colliding at +-50 m/s")
+        if a.workload == "clamped":
+            # grid velocity conditions on the bottom face's nodes (fixed in x, y, z) and a cosine
+            # x velocity on one side face; a net z load and a sine x load on the top face's particles
+            xml = configs.grid_bcs(xml, [
+                ("BCBox", {"xmin": lo, "xmax": hi, "ymin": lo, "ymax": hi, "zmin": lo - 0.1, "zmax": lo + 0.1},
+                 [{"dir": 1, "vel": 0.}, {"dir": 2, "vel": 0.}, {"dir": 3, "vel": 0.}]),
+                ("BCBox", {"xmin": lo - 0.1, "xmax": lo + 0.1, "ymin": lo, "ymax": hi, "zmin": lo + 1, "zmax": hi},
+                 [{"dir": 1, "style": 4, "vel": 1000., "time": 20.}])])
+            xml = configs.particle_bcs(xml, [
+                ("BCBox", {"xmin": lo, "xmax": hi, "ymin": lo, "ymax": hi, "zmin": hi - 0.5, "zmax": hi},
+                 ["net", {"dir": 3, "load": 2.0e5}, {"dir": 1, "style": 3, "load": 1.0e4, "ftime": 20.}])])
+            workload += ("; clamped on the bottom face's nodes, a cosine x velocity on one side face, "
+                         "a net z load and a sine x load on the top face's particles")
     prob = nm.Problem.from_xml(xml)
     grid, mats, run, npart, _ = prob.describe()
     dfields, ifields = prob.particles()
@@ -399,6 +413,13 @@ def main():
 
     sim = nm.Simulation(grid, mats, run)
     sim.upload(dfields, ifields)
+    bcs, loads = prob.velocity_bcs(), prob.particle_loads()
+    if bcs:
+        sim.set_velocity_bcs(bcs)
+    if loads:
+        sim.set_particle_loads(loads)
+    if bcs or loads:
+        print(f"# {len(bcs)} velocity conditions, {len(loads)} particle loads", file=sys.stderr, flush=True)
     sim.step(a.warmup)
     sim.sync()
     mass, _, _ = sim.grid_fields()
@@ -446,6 +467,10 @@ def main():
         import oracle as orc
         th = cpu_threads(a.cpu_threads)
         o = orc.Oracle(grid, mats, run, dfields, ifields)
+        if bcs:
+            o.set_velocity_bcs(bcs)
+        if loads:
+            o.set_particle_loads(loads)
         o.set_threads(th)
         # steps of the same block until the sample has run about --cpu-seconds (at least one)
         cpu_steps = cpu_s = 0
@@ -476,8 +501,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload,
                    "particles": npart, "grid_cells": [grid.nx, grid.ny, grid.nz], "active_nodes": active_nodes,
-                   "material": "IsoPlasticity J2, linear hardening" if a.workload == "j2" else "Neohookean x2 + CoulombFriction",
-                   "shape": "uGIMP" if a.workload == "j2" else "lCPDI", "method": "USAVG+ FLIP",
+                   "material": "Neohookean x2 + CoulombFriction" if a.workload == "contact" else "IsoPlasticity J2, linear hardening",
+                   "shape": "lCPDI" if a.workload == "contact" else "uGIMP", "method": "USAVG+ FLIP",
                    "graph_replay_value": graph_value},
         "roofline": roofline,
         "cpu_baseline": cpu,

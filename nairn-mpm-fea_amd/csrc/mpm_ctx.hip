@@ -84,7 +84,7 @@ static void enqueue_step(MpmCtx *c) {
     launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
     done(MPMGPU_KG_USF);
     if (c->kld.nl) launch_particle_loads(c->kg, c->kld, par, s);    // InitializationTask's pFext
-    launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, s);
+    launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, par, s);
     done(MPMGPU_KG_P2G_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
     vel_bcs(mm ? BC_PASS_GRID_FORCES : BC_PASS_FORCES_MOMENTA);
@@ -839,8 +839,9 @@ int mpmgpu_set_particle_loads(MpmCtx *c, int64_t n, const MpmParticleLoad *loads
     start.push_back((int)n);
     const size_t nl = start.size() - 1, ne = (size_t)n;
     if (ensure_time_slots(c)) return 1;
-    // doubles (value, ftime, fext), then ints (map, start, dir, style)
-    const size_t nd = 2 * ne + 3 * nl, ni = (size_t)np + nl + 1 + 2 * ne;
+    // doubles (value, ftime, fext), then ints (map, start, dir, style, two tile flag slots)
+    const size_t ntall = (size_t)c->kb.ntall;
+    const size_t nd = 2 * ne + 3 * nl, ni = (size_t)np + nl + 1 + 2 * ne + 2 * ntall;
     const size_t bytes = nd * sizeof(double) + ni * sizeof(int);
     HIPCHECK(c, hipMalloc(&c->ldbuf, bytes));
     std::vector<char> h(bytes, 0);
@@ -864,6 +865,8 @@ int mpmgpu_set_particle_loads(MpmCtx *c, int64_t n, const MpmParticleLoad *loads
     L.start = di + np;
     L.dir = di + np + nl + 1;
     L.style = di + np + nl + 1 + ne;
+    L.tflag[0] = di + np + nl + 1 + 2 * ne;     // zero from the host buffer
+    L.tflag[1] = L.tflag[0] + ntall;
     L.time = c->tbuf;
     return put_bc_time(c);
 }
@@ -890,7 +893,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_MASS_MOMENTUM, c->parity, s);
             break;
         case MPMGPU_TASK_UPDATE_STRAINS_FIRST: launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
-        case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, s); break;
+        case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, c->parity, s); break;
         case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
         case MPMGPU_TASK_UPDATE_MOMENTA:
             launch_node_force(c->dim, c->kg, c->kb, c->kn, s);

@@ -82,7 +82,7 @@ struct KBins {
 
 // launches of the tile kernels in one step, each with its own cursors (cleared when the list is built)
 enum { TILE_SLOT_MASS, TILE_SLOT_CONTACT_FIRST, TILE_SLOT_USF, TILE_SLOT_FORCE, TILE_SLOT_G2P,
-       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOTS };
+       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOT_FORCE_LOADS, TILE_SLOTS };
 #define TILE_SLOT_STRIDE 32
 
 struct KNodes {
@@ -122,8 +122,10 @@ void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, co
 void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                 const MpmMaterial *mats, hipStream_t s);
+// the force scatter; with loads, the tiles holding loaded particles are marked and scattered by a
+// second launch that adds N pFext, the others by the launch without the load term
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
-                      hipStream_t s);
+                      int par, hipStream_t s);
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                     hipStream_t s);
@@ -171,6 +173,7 @@ struct KLoads {
     const double *value, *ftime;
     double *fext;
     double *time;              // the two step start time slots shared with KBcs
+    int *tflag[2];             // per (field, tile): holds a loaded particle this step (slot par)
 };
 void launch_particle_loads(const KGrid &g, const KLoads &L, int par, hipStream_t s);
 

@@ -21,6 +21,7 @@
 //   k_usl_reset   UpdateStrainsFirstTask::FullStrainUpdate(last) + ResetElementsTask.cpp:44-265
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "mpm_device.h"
@@ -1084,7 +1085,8 @@ __global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNode
 // MatPoint3D::GetFintPlusFext, MatPoint3D.cpp:248-270; MatPoint2D.cpp:242-263) with the stress
 // just updated by k_usf; FX adds N * pFext of the loaded particles (pFext is zero otherwise)
 template <int DIM, int SH, bool FX>
-__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KNodes N, KLoads L) {
+__global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KNodes N, KLoads L,
+                                                  const int *loaded) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NCOR = DIM == 3 ? 8 : 4;
@@ -1109,6 +1111,8 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
         for (int c = 0; c < 3; c++) s_e[slot][c] = q >= 0 ? L.fext[3 * q + c] : 0.;
     };
     auto tile = [&](const int tall) {      // (field, context tile) pair
+        // with loads: the launch with the load term takes the marked tiles, the other the rest
+        if (loaded && (loaded[tall] != 0) != FX) return;
         const int fld = tall / g0.tilesTotal;
         const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
         load_tile_starts<DIM>(B, tall, s_cs);
@@ -2058,26 +2062,40 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
     }
 }
 
+// marks the (field, tile) of every loaded particle in slot par, clears slot par ^ 1 for the next step
+template <int DIM>
+__global__ void k_mark_load_tiles(KPart P, KBins B, KLoads L, int par) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < B.ntall) L.tflag[par ^ 1][p] = 0;
+    if (p >= P.n) return;
+    if (L.map[B.home[p]] >= 0) L.tflag[par][B.key[p] / TileGeom<DIM>::NC] = 1;
+}
+
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
-                      hipStream_t s) {
+                      int par, hipStream_t s) {
     const KBins tb = tile_bins(b, TILE_SLOT_FORCE);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    const int *loaded = nullptr;
     if (L.nl > 0) {
+        const long long n = std::max<long long>(p.n, b.ntall);
+        if (dim == 3) hipLaunchKernelGGL(k_mark_load_tiles<3>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, p, b, L, par);
+        else hipLaunchKernelGGL(k_mark_load_tiles<2>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, p, b, L, par);
+        loaded = L.tflag[par];
+        const KBins tl = tile_bins(b, TILE_SLOT_FORCE_LOADS);
         if (dim == 3) {
-            if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, L);
-            else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, L);
+            if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI, true>), tl, s, g, p, tl, nd, L, loaded);
+            else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP, true>), tl, s, g, p, tl, nd, L, loaded);
         } else {
-            if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, L);
-            else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, L);
+            if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI, true>), tl, s, g, p, tl, nd, L, loaded);
+            else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP, true>), tl, s, g, p, tl, nd, L, loaded);
         }
-        return;
     }
     if (dim == 3) {
-        if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, L);
-        else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, L);
+        if (cp) TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, L, loaded);
+        else TILE_LAUNCH((k_p2g_force<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, L, loaded);
     } else {
-        if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, L);
-        else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, L);
+        if (cp) TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, L, loaded);
+        else TILE_LAUNCH((k_p2g_force<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, L, loaded);
     }
 }
 
