@@ -110,6 +110,8 @@ typedef struct MpmRunParams {
     int32_t deleteLeaving;     /* fmobj->deleteLeavingParticles */
     int32_t maxLeaving;        /* warnings limit before abort (0 = never abort) */
     int32_t numFields;         /* material velocity fields (maxMaterialFields); 1 unless multimaterial */
+    double gridDamping;        /* <Damping>, 1/s: BodyForce::GetGridDamping (BodyForce.cpp:167-184) */
+    double particleDamping;    /* <PDamping>, 1/s: BodyForce::GetParticleDamping (:187-204) */
 } MpmRunParams;
 
 /* Counters reported back from the device (ResetElementsTask.cpp:71-151 warnings) */

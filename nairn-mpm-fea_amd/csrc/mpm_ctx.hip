@@ -372,6 +372,8 @@ static void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunPa
     g.xmin = grid->xmin; g.ymin = grid->ymin; g.zmin = grid->zmin;
     g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
     g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
+    g.gridDamping = run->gridDamping; g.particleDamping = run->particleDamping;
+    g.damped = run->gridDamping != 0. || run->particleDamping != 0.;
     g.nfields = 1;
     g.shape = run->shape;
     g.uniformLp = 0;
@@ -554,6 +556,7 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
     g = c->inst[0];
     g.nstride = nodeAcc;
     g.tilesTotal = tileAcc;
+    for (auto &gi : c->inst) g.damped |= gi.damped;     // the launch takes the damped instance
     if (alloc_context(c, mats, nmat)) { mpmgpu_destroy(c); return 1; }
     // instance tables on the device
     std::vector<int> tables;

@@ -31,6 +31,8 @@ struct KGrid {
     long long nnodes;
     double xmin, ymin, zmin, dx, dy, dz;
     double timestep, strainFirst, strainLast;
+    double gridDamping, particleDamping;   // constant damping of the FLIP update (1/s)
+    int damped;                // nonzero damping here (a batch: in any instance)
     int uniformLp;             // 1 when every particle has the same lp (then lp[] is used)
     int matType;               // the one material type of all particles, 0 when mixed
     double lp[3];

@@ -1302,7 +1302,7 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
 // with start-of-step shape functions (MatVelocityField::IncrementNodalVelAcc,
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
-template <int DIM, int SH>
+template <int DIM, int SH, bool DAMP>
 __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
@@ -1400,10 +1400,13 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
                         }
                 }
                 double vnew[3] = {0., 0., 0.};
+                // MatPoint3D::MoveParticle, FLIP (MatPoint3D.cpp:104-160): Adamp0 = alpha_p v +
+                // alpha_g vm (the DAMP instance; zero damping gives delV = Sacc dt)
 #pragma unroll
                 for (int c = 0; c < DIM; c++) {
                     double vm = Svk[c] - Sacc[c] * dt;
-                    double delV = Sacc[c] * dt;
+                    double delV = DAMP ? (Sacc[c] - (vold[c] * g.particleDamping + vm * g.gridDamping)) * dt
+                                       : Sacc[c] * dt;
                     vnew[c] = vold[c] + delV;
                     P.f[MPF_VEL_X + c][p] = vnew[c];
                     double delXRate = vm + 0.5 * delV;
@@ -2107,12 +2110,23 @@ void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
     const KBins tb = tile_bins(b, TILE_SLOT_G2P);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    // damping has its own instance, so the undamped update keeps its registers
+    if (g.damped) {
+        if (dim == 3) {
+            if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd);
+            else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd);
+        } else {
+            if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd);
+            else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd);
+        }
+        return;
+    }
     if (dim == 3) {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
-        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd);
+        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd);
     } else {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
-        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd);
+        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd);
     }
 }
 

@@ -346,6 +346,13 @@ struct Reader {
                 p.archiveTime = textnum(k) * time_units(k);
             } else if (k.name == "LeaveLimit") {
                 p.run.maxLeaving = (int)textnum(k);
+            } else if (k.name == "Damping" || k.name == "PDamping") {
+                // MPMReadHandler.cpp:311-350: a constant in 1/s (BodyForce damping, pdamping)
+                if (k.attr("function")) return fail("<" + k.name + " function=> (damping that varies in time) is not on this path");
+                if (k.attr("PIC")) return fail("PIC attribute on damping remove; use PeriodicXPIC task intead");
+                (k.name == "Damping" ? p.run.gridDamping : p.run.particleDamping) = textnum(k);
+            } else if (k.name == "FeedbackDamping" || k.name == "PFeedbackDamping" || k.name == "XPIC") {
+                return fail("<" + k.name + "> is not on this path");
             }
             // archive, global-archive and output-only settings do not change the step
         }
@@ -471,7 +478,8 @@ struct Reader {
             else if (n == "coeff") in.friction = v;
             else if (n == "coeffStatic") in.frictionStatic = v;
             else if (n == "UJOption") in.ujOption = (int)v;
-            else if (n == "Friction" || n == "SharedMaterialField" || n == "ArtificialVisc" || n == "ArtificialViscosity")
+            else if (n == "Friction" || n == "SharedMaterialField" || n == "ArtificialVisc" || n == "ArtificialViscosity" ||
+                     n == "PDamping")
                 return fail("<" + n + "> in <Material> is not on this path");
             else if (n == "Hardening") {
                 if (k.text.find("Linear") == std::string::npos && (int)v != 1 && !k.text.empty() && k.text.find_first_not_of(" \t\r\n") != std::string::npos)

@@ -72,7 +72,8 @@ class RunParams(ctypes.Structure):
                 ("strainLast", ctypes.c_double), ("shape", ctypes.c_int32), ("mpmMethod", ctypes.c_int32),
                 ("multiMaterial", ctypes.c_int32), ("contactLaw", ctypes.c_int32),
                 ("normalsMethod", ctypes.c_int32), ("deleteLeaving", ctypes.c_int32),
-                ("maxLeaving", ctypes.c_int32), ("numFields", ctypes.c_int32)]
+                ("maxLeaving", ctypes.c_int32), ("numFields", ctypes.c_int32),
+                ("gridDamping", ctypes.c_double), ("particleDamping", ctypes.c_double)]
 
 
 class InstanceDesc(ctypes.Structure):

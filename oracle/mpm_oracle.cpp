@@ -1378,7 +1378,9 @@ struct Oracle {
             int nc = g.dim == 3 ? 3 : 2;
             for (int c = 0; c < nc; c++) {
                 double vm = Svk[c] - Sacc[c] * dt;
-                double delV = Sacc[c] * dt;
+                // Adamp0 = particleAlpha vel + gridAlpha vm (MatPoint3D.cpp:117-120)
+                double adamp = F(MPF_VEL_X + c, p) * run.particleDamping + vm * run.gridDamping;
+                double delV = (Sacc[c] - adamp) * dt;
                 F(MPF_VEL_X + c, p) += delV;
                 double delXRate = vm + 0.5 * delV;
                 F(MPF_POS_X + c, p) += delXRate * dt;
