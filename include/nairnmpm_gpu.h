@@ -112,6 +112,7 @@ typedef struct MpmRunParams {
     int32_t numFields;         /* material velocity fields (maxMaterialFields); 1 unless multimaterial */
     double gridDamping;        /* <Damping>, 1/s: BodyForce::GetGridDamping (BodyForce.cpp:167-184) */
     double particleDamping;    /* <PDamping>, 1/s: BodyForce::GetParticleDamping (:187-204) */
+    double gravity[3];         /* <Gravity> BodyX/Y/ZForce, mm/s^2: BodyForce gforce (BodyForce.cpp:76-95) */
 } MpmRunParams;
 
 /* Counters reported back from the device (ResetElementsTask.cpp:71-151 warnings) */

@@ -374,6 +374,8 @@ static void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunPa
     g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
     g.gridDamping = run->gridDamping; g.particleDamping = run->particleDamping;
     g.damped = run->gridDamping != 0. || run->particleDamping != 0.;
+    for (int a = 0; a < 3; a++) g.gravity[a] = run->gravity[a];
+    g.hasGravity = run->gravity[0] != 0. || run->gravity[1] != 0. || run->gravity[2] != 0.;
     g.nfields = 1;
     g.shape = run->shape;
     g.uniformLp = 0;

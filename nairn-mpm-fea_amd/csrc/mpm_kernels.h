@@ -33,6 +33,8 @@ struct KGrid {
     double timestep, strainFirst, strainLast;
     double gridDamping, particleDamping;   // constant damping of the FLIP update (1/s)
     int damped;                // nonzero damping here (a batch: in any instance)
+    double gravity[3];         // constant body force per unit mass (mm/s^2)
+    int hasGravity;
     int uniformLp;             // 1 when every particle has the same lp (then lp[] is used)
     int matType;               // the one material type of all particles, 0 when mixed
     double lp[3];

@@ -1282,8 +1282,12 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
     gather_partials<DIM, NQ_FORCE>(g, B, N, fld, i, j, k, f);
     long long nn = g.nstride;
     if (DIM == 2) f[2] = 0.;
-    Nf.ftot[n] = f[0]; Nf.ftot[nn + n] = f[1]; Nf.ftot[2 * nn + n] = f[2];
     int cnt = Nf.cnt[n];
+    if (g.hasGravity && cnt > 0) {         // PostForcesTask.cpp:62-91 -> AddGravityAndBodyForceTask3
+        const double m = Nf.mass[n];       // (MatVelocityField.cpp:254-258): ftot += mass g
+        for (int c = 0; c < 3; c++) f[c] += m * g.gravity[c];
+    }
+    Nf.ftot[n] = f[0]; Nf.ftot[nn + n] = f[1]; Nf.ftot[2 * nn + n] = f[2];
     if (cnt <= 0) return;
     double dt = g.timestep;
     double px = Nf.pk[n] + f[0] * dt;

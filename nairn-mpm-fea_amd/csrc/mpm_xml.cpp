@@ -747,7 +747,18 @@ struct Reader {
                 if (!particlebcs(*k)) return false;
                 continue;
             }
-            if (n == "CrackList" || n == "Thermal" || n == "CustomTasks" || n == "Gravity")
+            if (n == "Gravity") {
+                // MPMReadHandler.cpp:1627-1660: constant body force components (mm/s^2)
+                for (auto &c : k->kids) {
+                    const std::string &cn = c->name;
+                    if (cn == "BodyXForce") p.run.gravity[0] = textnum(*c);
+                    else if (cn == "BodyYForce") p.run.gravity[1] = textnum(*c);
+                    else if (cn == "BodyZForce") p.run.gravity[2] = textnum(*c);
+                    else return fail("<" + cn + "> in <Gravity> (grid body force functions) is not on this path");
+                }
+                continue;
+            }
+            if (n == "CrackList" || n == "Thermal" || n == "CustomTasks")
                 if (!k->kids.empty() || !k->text.empty())
                     return fail("<" + n + "> is not on this path");
         }

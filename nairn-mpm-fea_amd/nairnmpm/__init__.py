@@ -73,7 +73,8 @@ class RunParams(ctypes.Structure):
                 ("multiMaterial", ctypes.c_int32), ("contactLaw", ctypes.c_int32),
                 ("normalsMethod", ctypes.c_int32), ("deleteLeaving", ctypes.c_int32),
                 ("maxLeaving", ctypes.c_int32), ("numFields", ctypes.c_int32),
-                ("gridDamping", ctypes.c_double), ("particleDamping", ctypes.c_double)]
+                ("gridDamping", ctypes.c_double), ("particleDamping", ctypes.c_double),
+                ("gravity", ctypes.c_double * 3)]
 
 
 class InstanceDesc(ctypes.Structure):

@@ -1346,6 +1346,11 @@ struct Oracle {
 #pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = pkc[3 * n + c];
+        // gravity on every active field (PostForcesTask.cpp:62-91, MatVelocityField.cpp:254-258)
+        const double *gv = run.gravity;
+        if (gv[0] != 0. || gv[1] != 0. || gv[2] != 0.)
+            for (long n = 0; n < (long)nf * g.nnodes; n++)
+                if (npts[n] > 0) for (int c = 0; c < 3; c++) ftot[3 * n + c] += mass[n] * gv[c];
         velocity_bcs(BC_GRID_FORCES);                    // PostForcesTask.cpp:94
     }
 
