@@ -35,6 +35,7 @@ FIELDS = [
 F = {name: i for i, name in enumerate(FIELDS)}
 NUM_FIELDS = len(FIELDS)
 IFIELDS = ["elem", "mat", "id", "left"]
+I = {name: i for i, name in enumerate(IFIELDS)}
 NUM_IFIELDS = len(IFIELDS)
 
 TASKS = [

@@ -1651,6 +1651,42 @@ int oracle_cpdi_shape(const MpmGridDesc *grid, int64_t elem, const double *state
     return so.n;
 }
 
+/* Lumped multimaterial contact on one node (unit tests): nf material velocity fields with mass,
+   point count, contact volume, mass-weighted displacement and volume gradient, and momentum,
+   force and momentum copy (vectors interleaved x,y,z) updated in place by one contact pass of
+   callType (0 mass-momentum, 1 update-momentum, 2 strains-last) with contact law `law`, normals
+   method `normals` and time step dt (MaterialContactNode::ContactOnKnownNodes,
+   MaterialContactNode.cpp:196-232). */
+int oracle_contact_node(const MpmMaterial *law, int32_t normals, int32_t callType, int32_t nf, double dt,
+                        const double *mass, const int32_t *npts, const double *cvol, const double *cdisp,
+                        const double *cgrad, double *pk, double *ftot, double *pkc) {
+    if (nf < 2 || nf > MPMGPU_MAX_MATERIALS || callType < 0 || callType > 2) return 1;
+    Oracle o;
+    o.g.nnodes = 1;
+    o.nf = nf;
+    o.mats.assign(law, law + 1);
+    memset(&o.run, 0, sizeof(o.run));
+    o.run.contactLaw = 0;
+    o.run.normalsMethod = normals;
+    o.run.timestep = dt;
+    o.run.multiMaterial = 1;
+    o.run.numFields = nf;
+    o.mass.assign(mass, mass + nf);
+    o.npts.assign(npts, npts + nf);
+    o.cvol.assign(cvol, cvol + nf);
+    o.cdisp.assign(cdisp, cdisp + 3 * nf);
+    o.cgrad.assign(cgrad, cgrad + 3 * nf);
+    o.pk.assign(pk, pk + 3 * nf);
+    o.ftot.assign(ftot, ftot + 3 * nf);
+    o.pkc.assign(pkc, pkc + 3 * nf);
+    o.contactNodes.assign(1, 0L);
+    o.material_contact(callType);
+    memcpy(pk, o.pk.data(), 3 * nf * sizeof(double));
+    memcpy(ftot, o.ftot.data(), 3 * nf * sizeof(double));
+    memcpy(pkc, o.pkc.data(), 3 * nf * sizeof(double));
+    return 0;
+}
+
 /* contact extrapolations of every field (interleaved vectors); 1 in single-material mode */
 int oracle_get_contact(void *h, double *vol, double *disp, double *grad) {
     Oracle *o = (Oracle *)h;

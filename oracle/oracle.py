@@ -54,6 +54,8 @@ def lib():
         _lib.oracle_constitutive.argtypes = [ctypes.c_int32, ctypes.POINTER(nm.Material), _PD, _PD]
         _lib.oracle_cpdi_shape.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.c_int64, _PD, ctypes.c_int32,
                                            _PL, _PD, _PD, _PD, _PD]
+        _lib.oracle_contact_node.argtypes = [ctypes.POINTER(nm.Material), ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_double, _PD, _PI, _PD, _PD, _PD, _PD, _PD, _PD]
     return _lib
 
 
@@ -192,3 +194,22 @@ def cpdi_shape(grid, elem, state, deriv=True):
     if k < 0:
         raise RuntimeError("a CPDI corner left the grid")
     return nodes[:k], fn[:k], dx[:k], dy[:k], dz[:k]
+
+
+def contact_node(law, normals, call_type, dt, mass, npts, vol, disp, grad, pk, ftot, pkc=None):
+    """One lumped contact pass on a single node of len(mass) material velocity fields. mass, npts
+    and vol are per field; disp (mass-weighted displacement), grad (volume gradient), pk, ftot and
+    pkc are [nf, 3]. Returns the updated (pk, ftot, pkc)."""
+    nf = len(mass)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    n = np.ascontiguousarray(npts, dtype=np.int32)
+    v = np.ascontiguousarray(vol, dtype=np.float64)
+    d = np.ascontiguousarray(disp, dtype=np.float64).reshape(nf * 3)
+    g = np.ascontiguousarray(grad, dtype=np.float64).reshape(nf * 3)
+    p = np.array(pk, dtype=np.float64).reshape(nf * 3)
+    f = np.array(ftot, dtype=np.float64).reshape(nf * 3)
+    c = np.zeros(nf * 3) if pkc is None else np.array(pkc, dtype=np.float64).reshape(nf * 3)
+    if lib().oracle_contact_node(ctypes.byref(law), normals, call_type, nf, dt, _d(m), n.ctypes.data_as(_PI),
+                                 _d(v), _d(d), _d(g), _d(p), _d(f), _d(c)) != 0:
+        raise RuntimeError("bad contact node input")
+    return p.reshape(nf, 3), f.reshape(nf, 3), c.reshape(nf, 3)
