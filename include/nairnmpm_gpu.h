@@ -107,8 +107,13 @@ typedef struct MpmRunParams {
     int32_t multiMaterial;     /* 0 single-material mode */
     int32_t contactLaw;        /* material index of the default contact law, -1 none */
     int32_t normalsMethod;     /* MultiMaterialMode Normals= (0 MAXG, 1 MAXV, 2 AVGG, 3 OWNG) */
-    int32_t deleteLeaving;     /* fmobj->deleteLeavingParticles */
-    int32_t maxLeaving;        /* warnings limit before abort (0 = never abort) */
+    int32_t deleteLeaving;     /* fmobj->deleteLeavingParticles: a particle that leaves the grid
+                                  would be deleted into the reservoir; not supported, so the
+                                  first one stops the run with an error */
+    int32_t maxLeaving;        /* pushed-back particles after which the run stops with "Too many
+                                  particles have left the grid" (0 = never); the XML reader
+                                  resolves <LeaveLimit> into these two as NairnMPM::CreateWarnings
+                                  does (NairnMPM.cpp:814-834). Batches ignore both. */
     int32_t numFields;         /* material velocity fields (maxMaterialFields); 1 unless multimaterial */
     double gridDamping;        /* <Damping>, 1/s: BodyForce::GetGridDamping (BodyForce.cpp:167-184) */
     double particleDamping;    /* <PDamping>, 1/s: BodyForce::GetParticleDamping (:187-204) */
@@ -123,6 +128,13 @@ typedef struct MpmStats {
     int64_t nanParticles;      /* particles with a NaN position */
     double time;               /* simulation time (s) */
     double lastStepMs;         /* device time of the last timed step (ms) */
+    int32_t stopped;           /* why the run stopped (ResetElementsTask's exceptions,
+                                  ResetElementsTask.cpp:71-151): 0 running, 1 too many particles
+                                  left the grid, 2 a NaN position, 3 a leaving particle would be
+                                  deleted; mpmgpu_sync and mpmgpu_step then fail with the message,
+                                  mpmgpu_download_particles gives the state the stopping step
+                                  left, and a new upload starts again */
+    int32_t pad;
 } MpmStats;
 
 /* Task ids for mpmgpu_run_task: one per hot-path MPMTask (NairnMPM.cpp:952-1108) */
@@ -170,7 +182,7 @@ int mpmgpu_problem_create(int32_t analysis, MpmProblem **out);
 int mpmgpu_problem_set_grid(MpmProblem *prob, const double *ext6, const double *cell3,
                             const int32_t *n3, double thickness);
 /* <MPMHeader> settings: "MatlPtsPerElement", "CFL", "TimeStep" (s), "MaxTime" (s),
-   "GIMP" (MPMGPU_SHAPE_*) */
+   "GIMP" (MPMGPU_SHAPE_*), "LeaveLimit" */
 int mpmgpu_problem_set(MpmProblem *prob, const char *key, double value);
 /* <Material Type= Name=> (Read_MPM/MaterialController.cpp:105-190); id is 1-based */
 int mpmgpu_problem_add_material(MpmProblem *prob, int32_t type, const char *name, int32_t *id);

@@ -8,7 +8,19 @@
 #include <utility>
 #include <vector>
 
+#include "mpm_events.h"
 #include "mpm_kernels.h"
+
+// the HIP events of the kernel-group timing
+struct HipEvents {
+    typedef hipEvent_t Event;
+    typedef hipStream_t Stream;
+    static std::string status(hipError_t e) { return e == hipSuccess ? std::string() : hipGetErrorString(e); }
+    static std::string create(Event *e) { return status(hipEventCreate(e)); }
+    static std::string record(Event e, Stream s) { return status(hipEventRecord(e, s)); }
+    static std::string elapsed(float *ms, Event a, Event b) { return status(hipEventElapsedTime(ms, a, b)); }
+    static void destroy(Event e) { (void)hipEventDestroy(e); }
+};
 
 // Particle rows: PROWS double rows (the MPF_* fields, then the second position buffer) and IROWS
 // int rows (the MPI_* fields, the cell key, the sorted slot -> particle order, the index at
@@ -19,6 +31,8 @@ constexpr int IROWS = MPI_NUM_FIELDS + 4;
 constexpr int IROW_KEY = MPI_NUM_FIELDS, IROW_ORDER = MPI_NUM_FIELDS + 1, IROW_HOME = MPI_NUM_FIELDS + 2,
               IROW_SHOME = MPI_NUM_FIELDS + 3;
 constexpr int64_t PARTICLE_BYTES = PROWS * sizeof(double) + IROWS * sizeof(int) + sizeof(double);
+// device counters: left grid, NaN positions, lCPDI corner faults, then the run stop (KBins::stop)
+constexpr int STATS_WORDS = 6;
 
 struct MpmCtx {
     int dim = 3;
@@ -53,6 +67,7 @@ struct MpmCtx {
     int nextTask = 0;                // task sequence check for mpmgpu_run_task
     int64_t steps = 0;
     double mtime = 0.;
+    bool stopped = false;            // the device stopped the run (ResetElementsTask exception)
     // grid velocity conditions (kbc.nb = 0: none) in one device buffer; bcTime is the step start
     // time as the device counts it (one addition of the time step per step)
     KBcs kbc{};
@@ -66,23 +81,8 @@ struct MpmCtx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // kernel timing: events around each kernel group of a step, on this context's stream
     bool profiling = false;
-    std::vector<hipEvent_t> evPool;
-    std::vector<std::pair<int, std::pair<int, int>>> evMarks;   // (group, (start ev, end ev))
-    size_t evUsed = 0;
-    double groupMs[MPMGPU_NUM_KERNEL_GROUPS] = {0};
-    int64_t groupLaunches[MPMGPU_NUM_KERNEL_GROUPS] = {0};
+    EventMarks<HipEvents, MPMGPU_NUM_KERNEL_GROUPS> ev;
     std::string err;
-
-    int mark() {
-        if (evUsed == evPool.size()) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return -1;
-            evPool.push_back(e);
-        }
-        int i = (int)evUsed++;
-        (void)hipEventRecord(evPool[i], stream);
-        return i;
-    }
 
     int fail(const std::string &m) {
         err = m;
@@ -109,4 +109,13 @@ int prepare_particles(MpmCtx *c);
 int sort_rows(MpmCtx *c);
 // current and next position rows by parity
 void set_positions(MpmCtx *c);
+// the kernels of one whole step on the context's stream (with profiling marks when on)
+void enqueue_step(MpmCtx *c);
+// input checks of mpmgpu_create (nullptr when the input is usable, else the message)
+const char *check_run(const MpmGridDesc *grid, const MpmRunParams *run);
+const char *check_materials(const MpmMaterial *mats, int nmat);
+// geometry, tiles, time step and run stops of one grid
+void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run);
+// ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
+size_t bin_words(long long nk, long long ntall);
 }  // namespace mpmi

@@ -29,6 +29,10 @@ void mpmi::set_positions(MpmCtx *c) {
     }
 }
 using mpmi::set_positions;
+using mpmi::check_run;
+using mpmi::check_materials;
+using mpmi::fill_grid;
+using mpmi::bin_words;
 
 static void destroy_graphs(MpmCtx *c) {
     for (int i = 0; i < 2; i++)
@@ -52,15 +56,12 @@ static int ensure_time_slots(MpmCtx *c) {
 
 // the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108); with
 // profiling on, an event is recorded between kernel groups
-static void enqueue_step(MpmCtx *c) {
+void mpmi::enqueue_step(MpmCtx *c) {
     hipStream_t s = c->stream;
     set_positions(c);
-    int e = c->profiling ? c->mark() : -1;
+    if (c->profiling) c->ev.begin(s);
     auto done = [&](int group) {
-        if (!c->profiling) return;
-        int e2 = c->mark();
-        c->evMarks.push_back({group, {e, e2}});
-        e = e2;
+        if (c->profiling) c->ev.end(group, s);
     };
     const bool mm = c->kg.nfields > 1;
     launch_p2g_mass(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
@@ -113,20 +114,110 @@ static void enqueue_step(MpmCtx *c) {
     c->parity ^= 1;
 }
 
-// events recorded between two collections of the kernel-group timings (profiling mode)
+// events recorded between two folds of the kernel-group timings (profiling mode)
 #define MAX_LIVE_EVENTS 1024
 
-// fold the recorded kernel-group events into per-group totals (after a stream sync)
-static void collect_marks(MpmCtx *c) {
-    for (auto &m : c->evMarks) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->evPool[m.second.first], c->evPool[m.second.second]) == hipSuccess) {
-            c->groupMs[m.first] += ms;
-            c->groupLaunches[m.first] += 1;
-        }
+// fold the recorded kernel-group events into per-group totals (after a stream sync); a failed
+// event call of profiling mode is returned as the context's error
+static int collect_marks(MpmCtx *c) {
+    c->ev.fold();
+    if (!c->ev.err.empty()) {
+        std::string m = "kernel-group timing: " + c->ev.err;
+        c->ev.clear_error();
+        return c->fail(m);
     }
-    c->evMarks.clear();
-    c->evUsed = 0;
+    return 0;
+}
+
+// The device's run stop (ResetElementsTask's exceptions), read after a wait for the stream. A
+// stopped run keeps the particle state of the step that stopped it: the positions that step wrote
+// are the current ones and the step count is that of the steps before it.
+static int read_stop(MpmCtx *c, unsigned long long *why) {
+    unsigned long long s[3] = {0, 0, 0};
+    HIPCHECK(c, hipMemcpy(s, c->kb.stop, sizeof(s), hipMemcpyDeviceToHost));
+    *why = s[0];
+    if (s[0] && !c->stopped) {
+        c->stopped = true;
+        c->parity = (int)((s[1] + 1) & 1ull);
+        c->steps = (int64_t)s[1];
+        c->mtime = (double)s[1] * c->run.timestep;
+        c->nextTask = 0;
+    }
+    return 0;
+}
+
+static const char *stop_message(unsigned long long why) {
+    switch (why) {
+        case STOP_TOO_MANY_LEFT: return "Too many particles have left the grid";
+        case STOP_NAN: return "Particle left grid with position nan.";
+        case STOP_DELETE:
+            return "A particle has left the grid and would be deleted into the reservoir, which is not supported "
+                   "(set <LeaveLimit> above 1 to push leaving particles back)";
+        default: return "the run stopped";
+    }
+}
+
+// the run's error once the device has stopped it
+static int check_stop(MpmCtx *c) {
+    unsigned long long why = 0;
+    if (read_stop(c, &why)) return 1;
+    if (why) return c->fail(std::string(stop_message(why)) + " (step " + std::to_string(c->steps + 1) + ")");
+    return 0;
+}
+
+// ---- input checks and grid geometry (shared with mpm_static.hip) ----------------------
+
+const char *mpmi::check_run(const MpmGridDesc *grid, const MpmRunParams *run) {
+    if (grid->analysis != MPMGPU_PLANE_STRAIN && grid->analysis != MPMGPU_THREED)
+        return "analysis type must be 10 (plane strain) or 12 (3D)";
+    if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI && run->shape != MPMGPU_SHAPE_LINEAR)
+        return "shape functions must be classic (0), uGIMP (1) or lCPDI (10) on the GPU path";
+    if (run->mpmMethod != 2) return "only the USAVG method is implemented";
+    return nullptr;
+}
+
+const char *mpmi::check_materials(const MpmMaterial *mats, int nmat) {
+    for (int i = 0; i < nmat; i++) {
+        int t = mats[i].type;
+        if (t != MPMGPU_MAT_ISOTROPIC && t != MPMGPU_MAT_ISOPLASTICITY && t != MPMGPU_MAT_NEOHOOKEAN &&
+            t != MPMGPU_MAT_MOONEY && t != MPMGPU_MAT_COULOMB && t != MPMGPU_MAT_CONTACT)
+            return "a material type is not implemented on the GPU path";
+    }
+    return nullptr;
+}
+
+// geometry, tiles and time step of one grid
+void mpmi::fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run) {
+    memset(&g, 0, sizeof(g));
+    g.dim = dim;
+    g.nx = grid->nx; g.ny = grid->ny; g.nz = dim == 3 ? grid->nz : 1;
+    g.nnx = g.nx + 1; g.nny = g.ny + 1; g.nnz = dim == 3 ? g.nz + 1 : 1;
+    int tx, ty, tz;
+    if (dim == 3) { tx = TileGeom<3>::TX; ty = TileGeom<3>::TY; tz = TileGeom<3>::TZ; }
+    else { tx = TileGeom<2>::TX; ty = TileGeom<2>::TY; tz = 1; }
+    g.ntx = (g.nx + tx - 1) / tx; g.nty = (g.ny + ty - 1) / ty; g.ntz = (g.nz + tz - 1) / tz;
+    g.ntiles = g.ntx * g.nty * g.ntz;
+    g.nnodes = (long long)g.nnx * g.nny * g.nnz;
+    g.nstride = g.nnodes;
+    g.tilesTotal = g.ntiles;
+    g.xmin = grid->xmin; g.ymin = grid->ymin; g.zmin = grid->zmin;
+    g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
+    g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
+    g.gridDamping = run->gridDamping; g.particleDamping = run->particleDamping;
+    g.damped = run->gridDamping != 0. || run->particleDamping != 0.;
+    for (int a = 0; a < 3; a++) g.gravity[a] = run->gravity[a];
+    g.hasGravity = run->gravity[0] != 0. || run->gravity[1] != 0. || run->gravity[2] != 0.;
+    g.nfields = 1;
+    g.shape = run->shape;
+    g.uniformLp = 0;
+    g.maxLeaving = run->maxLeaving > 0 ? run->maxLeaving : 0;
+    g.deleteLeaving = run->deleteLeaving ? 1 : 0;
+    g.stopOnNan = 1;
+}
+
+// ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
+size_t mpmi::bin_words(long long nk, long long ntall) {
+    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * 8 * TILE_SLOT_STRIDE);
 }
 
 extern "C" {
@@ -178,6 +269,7 @@ int mpmgpu_problem_set(MpmProblem *p, const char *key, double value) {
     } else if (k == "CFL") p->cfl = value;
     else if (k == "TimeStep") p->timestepIn = value;
     else if (k == "MaxTime") p->maxTime = value;
+    else if (k == "LeaveLimit") p->leaveLimit = (int)value;
     else if (k == "GIMP") p->run.shape = (int)value;
     else if (k == "MultiMaterialMode") {       // value = Normals method
         p->mmRequested = true;
@@ -336,56 +428,6 @@ void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 
 // ---- device context --------------------------------------------------------------------
 
-static const char *check_run(const MpmGridDesc *grid, const MpmRunParams *run) {
-    if (grid->analysis != MPMGPU_PLANE_STRAIN && grid->analysis != MPMGPU_THREED)
-        return "analysis type must be 10 (plane strain) or 12 (3D)";
-    if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI && run->shape != MPMGPU_SHAPE_LINEAR)
-        return "shape functions must be classic (0), uGIMP (1) or lCPDI (10) on the GPU path";
-    if (run->mpmMethod != 2) return "only the USAVG method is implemented";
-    return nullptr;
-}
-
-static const char *check_materials(const MpmMaterial *mats, int nmat) {
-    for (int i = 0; i < nmat; i++) {
-        int t = mats[i].type;
-        if (t != MPMGPU_MAT_ISOTROPIC && t != MPMGPU_MAT_ISOPLASTICITY && t != MPMGPU_MAT_NEOHOOKEAN &&
-            t != MPMGPU_MAT_MOONEY && t != MPMGPU_MAT_COULOMB && t != MPMGPU_MAT_CONTACT)
-            return "a material type is not implemented on the GPU path";
-    }
-    return nullptr;
-}
-
-// geometry, tiles and time step of one grid
-static void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run) {
-    memset(&g, 0, sizeof(g));
-    g.dim = dim;
-    g.nx = grid->nx; g.ny = grid->ny; g.nz = dim == 3 ? grid->nz : 1;
-    g.nnx = g.nx + 1; g.nny = g.ny + 1; g.nnz = dim == 3 ? g.nz + 1 : 1;
-    int tx, ty, tz;
-    if (dim == 3) { tx = TileGeom<3>::TX; ty = TileGeom<3>::TY; tz = TileGeom<3>::TZ; }
-    else { tx = TileGeom<2>::TX; ty = TileGeom<2>::TY; tz = 1; }
-    g.ntx = (g.nx + tx - 1) / tx; g.nty = (g.ny + ty - 1) / ty; g.ntz = (g.nz + tz - 1) / tz;
-    g.ntiles = g.ntx * g.nty * g.ntz;
-    g.nnodes = (long long)g.nnx * g.nny * g.nnz;
-    g.nstride = g.nnodes;
-    g.tilesTotal = g.ntiles;
-    g.xmin = grid->xmin; g.ymin = grid->ymin; g.zmin = grid->zmin;
-    g.dx = grid->dx; g.dy = grid->dy; g.dz = grid->dz;
-    g.timestep = run->timestep; g.strainFirst = run->strainFirst; g.strainLast = run->strainLast;
-    g.gridDamping = run->gridDamping; g.particleDamping = run->particleDamping;
-    g.damped = run->gridDamping != 0. || run->particleDamping != 0.;
-    for (int a = 0; a < 3; a++) g.gravity[a] = run->gravity[a];
-    g.hasGravity = run->gravity[0] != 0. || run->gravity[1] != 0. || run->gravity[2] != 0.;
-    g.nfields = 1;
-    g.shape = run->shape;
-    g.uniformLp = 0;
-}
-
-// ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
-static size_t bin_words(long long nk, long long ntall) {
-    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * 8 * TILE_SLOT_STRIDE);
-}
-
 // device buffers of the grid, bins and counters for c->kg (nstride nodes, tilesTotal tiles per field)
 static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
     KGrid &g = c->kg;
@@ -406,7 +448,7 @@ static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
     if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nf * nn * sizeof(int)) != hipSuccess ||
         hipMalloc(&partMem, (size_t)nf * g.tilesTotal * NQMAX * foot * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dmats, nmat * sizeof(MpmMaterial)) != hipSuccess ||
-        hipMalloc(&c->sbuf, 3 * sizeof(unsigned long long)) != hipSuccess)
+        hipMalloc(&c->sbuf, STATS_WORDS * sizeof(unsigned long long)) != hipSuccess)
         return fail_global("out of device memory allocating the grid");
     long long fnn = (long long)nf * nn;
     c->kn.mass = nodeMem;
@@ -424,9 +466,10 @@ static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
     c->ks.leftGrid = c->sbuf;
     c->ks.nanCount = c->sbuf + 1;
     c->ks.cpdiErr = c->sbuf + 2;
+    c->kb.stop = c->sbuf + 3;
     (void)hipMemsetAsync(nodeMem, 0, nodeBytes, c->stream);
     (void)hipMemsetAsync(cntMem, 0, (size_t)nf * nn * sizeof(int), c->stream);
-    (void)hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream);
+    (void)hipMemsetAsync(c->sbuf, 0, STATS_WORDS * sizeof(unsigned long long), c->stream);
     (void)hipMemcpyAsync(c->dmats, mats, nmat * sizeof(MpmMaterial), hipMemcpyHostToDevice, c->stream);
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
     // (nfields * tilesTotal) and its length and work counters
@@ -548,6 +591,7 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
     for (int i = 0; i < ninst; i++) {
         KGrid &gi = c->inst[i];
         fill_grid(gi, c->dim, &desc[i].grid, &desc[i].run);
+        gi.maxLeaving = gi.deleteLeaving = gi.stopOnNan = 0;     // a batch never stops
         gi.tileBase = tileAcc;
         gi.nodeBase = nodeAcc;
         tileAcc += gi.ntiles;
@@ -649,7 +693,8 @@ int mpmi::prepare_particles(MpmCtx *c) {
     // the second position buffer starts as a copy: in 2D its z row is never written by a step
     for (int a = 0; a < 3; a++)
         HIPCHECK(c, hipMemcpyAsync(c->posB[a], c->posA[a], nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, 3 * sizeof(unsigned long long), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->sbuf, 0, STATS_WORDS * sizeof(unsigned long long), c->stream));
+    c->stopped = false;
     set_positions(c);
     launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
@@ -877,6 +922,7 @@ int mpmgpu_set_particle_loads(MpmCtx *c, int64_t n, const MpmParticleLoad *loads
 }
 
 int mpmgpu_run_task(MpmCtx *c, int32_t task) {
+    if (c->stopped) return check_stop(c);
     if (task != c->nextTask)
         return c->fail("tasks must run in the step order (expected task " + std::to_string(c->nextTask) + ")");
     hipStream_t s = c->stream;
@@ -932,6 +978,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
 }
 
 int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
+    if (c->stopped) return check_stop(c);
     if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
     for (int32_t k = 0; k < nsteps; k++) c->bcTime += c->run.timestep;
     if (c->kp.n == 0) {
@@ -947,7 +994,7 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
         if (!c->graph[par]) {
             hipGraph_t gr;
             HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < G; k++) enqueue_step(c);
+            for (int k = 0; k < G; k++) mpmi::enqueue_step(c);
             HIPCHECK(c, hipStreamEndCapture(c->stream, &gr));
             hipError_t e = hipGraphInstantiate(&c->graph[par], gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
@@ -961,14 +1008,14 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
         }
     }
     for (; done < nsteps; done++) {
-        enqueue_step(c);
+        mpmi::enqueue_step(c);
         c->sinceSort++;
         if (c->sortInterval > 0 && c->sinceSort >= c->sortInterval && mpmi::sort_rows(c)) return 1;
-        if (c->evUsed >= MAX_LIVE_EVENTS) {
+        if (c->profiling && (!c->ev.err.empty() || c->ev.live() >= MAX_LIVE_EVENTS)) {
             // the group timings recorded so far are folded in, so the events can be recorded again:
-            // a long profiled run holds at most this many events
+            // a long profiled run holds at most this many events; a failed event call ends the call
             HIPCHECK(c, hipStreamSynchronize(c->stream));
-            collect_marks(c);
+            if (collect_marks(c)) return 1;
         }
     }
     HIPCHECK(c, hipGetLastError());
@@ -1001,11 +1048,11 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     size_t pb = PROWS * n * sizeof(double), ib = IROWS * n * sizeof(int);
     size_t bb = bin_words(c->kb.nkeys, c->kb.ntall) * sizeof(int);
     char *save = nullptr;
-    HIPCHECK(c, hipMalloc(&save, pb + ib + bb + 3 * sizeof(unsigned long long)));
+    HIPCHECK(c, hipMalloc(&save, pb + ib + bb + STATS_WORDS * sizeof(unsigned long long)));
     HIPCHECK(c, hipMemcpyAsync(save, c->pbuf, pb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(save + pb, c->ibuf, ib, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(save + pb + ib, c->bbuf, bb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+    HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, STATS_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                                c->stream));
     const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort;
     const int64_t steps = c->steps;
@@ -1029,7 +1076,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, save, pb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, save + pb, ib, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->bbuf, save + pb + ib, bb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(c->sbuf, save + pb + ib + bb, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+    HIPCHECK(c, hipMemcpyAsync(c->sbuf, save + pb + ib + bb, STATS_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                                c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(save);
@@ -1058,7 +1105,8 @@ static int check_cpdi(MpmCtx *c) {
 int mpmgpu_sync(MpmCtx *c) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     if (check_cpdi(c)) return 1;
-    collect_marks(c);
+    if (check_stop(c)) return 1;
+    if (collect_marks(c)) return 1;
     float ms = 0.f;
     // before any timed step (task by task) the events hold nothing; that error is not kept
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->lastStepMs = ms;
@@ -1082,6 +1130,8 @@ int mpmgpu_write_archive(MpmCtx *c, const char *path, const char *mpmOrder, doub
 int mpmgpu_download_particles(MpmCtx *c, double *dfields, int32_t *ifields) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     if (check_cpdi(c)) return 1;
+    unsigned long long why = 0;
+    if (read_stop(c, &why)) return 1;      // a stopped run's particles are still downloaded
     size_t nn = (size_t)c->kp.n;
     if (nn == 0) return 0;
     set_positions(c);
@@ -1125,8 +1175,11 @@ int mpmgpu_download_grid(MpmCtx *c, double *mass, double *pk, double *ftot) {
 int mpmgpu_query(MpmCtx *c, MpmStats *st) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     if (check_cpdi(c)) return 1;
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[2] = {0, 0}, why = 0;
     HIPCHECK(c, hipMemcpy(h, c->sbuf, sizeof(h), hipMemcpyDeviceToHost));
+    if (read_stop(c, &why)) return 1;
+    st->stopped = (int32_t)why;
+    st->pad = 0;
     st->steps = c->steps;
     st->nparticles = c->kp.n;
     st->leftGrid = (int64_t)h[0];
@@ -1138,18 +1191,18 @@ int mpmgpu_query(MpmCtx *c, MpmStats *st) {
 
 int mpmgpu_set_profiling(MpmCtx *c, int32_t on) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    collect_marks(c);
+    const int r = collect_marks(c);
     c->profiling = on != 0;
-    for (int g = 0; g < MPMGPU_NUM_KERNEL_GROUPS; g++) { c->groupMs[g] = 0.; c->groupLaunches[g] = 0; }
-    return 0;
+    c->ev.reset_totals();
+    return r;
 }
 
 int mpmgpu_kernel_times(MpmCtx *c, double *totalMs, int64_t *launches) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    collect_marks(c);
+    if (collect_marks(c)) return 1;
     for (int g = 0; g < MPMGPU_NUM_KERNEL_GROUPS; g++) {
-        if (totalMs) totalMs[g] = c->groupMs[g];
-        if (launches) launches[g] = c->groupLaunches[g];
+        if (totalMs) totalMs[g] = c->ev.totalMs[g];
+        if (launches) launches[g] = c->ev.launches[g];
     }
     return 0;
 }
@@ -1164,7 +1217,7 @@ const char *mpmgpu_kernel_group_name(int32_t g) {
 void mpmgpu_destroy(MpmCtx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto e : c->evPool) (void)hipEventDestroy(e);
+    c->ev.destroy();
     destroy_graphs(c);
     if (c->pbuf) (void)hipFree(c->pbuf);
     if (c->ibuf) (void)hipFree(c->ibuf);

@@ -46,6 +46,10 @@ struct KGrid {
     int ignoreContact;         // ContactLaw type 60: always move to the center-of-mass velocity
     double friction, frictionStatic;
     int shape;                 // MPMGPU_SHAPE_UGIMP or MPMGPU_SHAPE_LCPDI
+    // ResetElementsTask's stops (one simulation; a batch sets all three to 0): the run stops when
+    // maxLeaving particles have left the grid (0: never), at the first leaving particle when
+    // deleteLeaving is set, and at the first NaN position when stopOnNan is set
+    int maxLeaving, deleteLeaving, stopOnNan;
     // Several independent grids (a batch of instances) share the context's arrays: instance i
     // owns nodes [nodeBase, nodeBase+nnodes) and tiles [tileBase, tileBase+ntiles) of every field.
     // For one grid: nodeBase = tileBase = 0, nstride = nnodes, tilesTotal = ntiles, inst = null.
@@ -79,6 +83,12 @@ struct KBins {
     int *tiles;                // the non-empty (field, tile) pairs of the binning, in no fixed order
     int *ntiles;               // length of tiles
     int *work;                 // tile kernels' list cursors: 8 per launch of a step, 32 ints apart
+    // run stop (ResetElementsTask's exceptions): stop[0] the reason (STOP_*, 0 while running),
+    // stop[1] the steps completed before the step that stopped, stop[2] the steps completed since
+    // the upload (counted by the tile list). Once stop[0] is set, binning builds an empty tile
+    // list, so every later tile kernel has nothing to do and the particle state stays as the
+    // stopping step left it.
+    unsigned long long *stop;
     int nkeys;
     int ntall;                 // (field, tile) pairs: nfields * tilesTotal
     int slot;                  // the launch's cursor (set by the launch functions)
@@ -111,6 +121,16 @@ struct KStats {
 };
 
 enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
+
+// reasons a run stops on the device
+enum { STOP_NONE = 0, STOP_TOO_MANY_LEFT = 1, STOP_NAN = 2, STOP_DELETE = 3, STOP_BAD_INPUT = 4 };
+
+// the run stops (ResetElementsTask's exceptions, ResetElementsTask.cpp:71-151): the first reason
+// raised is kept with the number of steps completed before this one
+__device__ __forceinline__ void stop_run(const KBins &B, unsigned long long why) {
+    if (atomicCAS(&B.stop[0], 0ull, why) == 0ull) atomicExch(&B.stop[1], B.stop[2]);
+}
+
 
 // contact call types (UpdateMomentaTask::ContactAndMomentaBCs passType)
 enum { CONTACT_MASS_MOMENTUM = 0, CONTACT_UPDATE_MOMENTUM = 1, CONTACT_UPDATE_STRAINS_LAST = 2 };

@@ -357,6 +357,7 @@ template <int DIM>
 __global__ void k_bin_keys(KGrid g0, KPart P, KBins B) {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.n) return;
+    if (B.stop[0]) { B.key[p] = 0; return; }   // input refused on the device: nothing is binned
     const KGrid g = g0.inst ? g0.inst[g0.matInst[P.mat[p]]] : g0;
     int i, j, k;
     elem_ijk(g, P.elem[p], i, j, k);
@@ -430,6 +431,7 @@ __global__ void k_scan_add(int *out, const int *blockSums, int n) {
 // Slots by key: lanes of a wave with the same key (neighbours, since the rows are kept in tile
 // order) take consecutive slots with one atomic per key instead of one per particle.
 __global__ void k_bin_scatter(KPart P, KBins B) {
+    if (B.stop[0]) return;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < P.n;
     const int key = valid ? B.key[p] : -1;
@@ -457,6 +459,7 @@ __global__ void k_bin_sort_cells(KBins B) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= B.nkeys) return;
     int b = B.cellStart[c], e = B.cellStart[c + 1];
+    if (B.stop[0]) e = b;                    // stopped: nothing was scattered
     for (int i = b + 1; i < e; i++) {
         const int v = B.order[i];
         const int hv = B.shome[i];
@@ -477,7 +480,9 @@ __global__ void k_bin_sort_cells(KBins B) {
 __global__ void k_tile_list(KBins B, int nc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x < TILE_SLOTS * 8) B.work[threadIdx.x * TILE_SLOT_STRIDE] = 0;
-    const bool on = t < B.ntall && B.cellStart[(size_t)(t + 1) * nc] > B.cellStart[(size_t)t * nc];
+    const bool stopped = B.stop[0] != 0;
+    if (t == 0 && !stopped) B.stop[2] += 1;   // one more step completed
+    const bool on = !stopped && t < B.ntall && B.cellStart[(size_t)(t + 1) * nc] > B.cellStart[(size_t)t * nc];
     const unsigned long long m = __ballot(on);
     if (m == 0ull) return;
     const int lane = (int)__lane_id();
@@ -1567,6 +1572,7 @@ __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B,
             // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
             if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
                 atomicAdd(st.nanCount, 1ull);
+                if (g.stopOnNan) stop_run(B, STOP_NAN);
             } else if (!pt_in_elem<DIM>(g, ei, ej, ek, pos)) {
                 // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640) and EdgeElement2D/3D (:171-199)
                 bool left = false;
@@ -1609,7 +1615,13 @@ __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B,
                     P.elem[p] = e;
                     elem_ijk(g, e, ei, ej, ek);
                 } else {
-                    if (!P.left[p]) { P.left[p] = 1; atomicAdd(st.leftGrid, 1ull); }
+                    if (!P.left[p]) {
+                        P.left[p] = 1;
+                        const unsigned long long before = atomicAdd(st.leftGrid, 1ull);
+                        if (g.maxLeaving > 0 && before + 1ull >= (unsigned long long)g.maxLeaving)
+                            stop_run(B, STOP_TOO_MANY_LEFT);
+                    }
+                    if (g.deleteLeaving) stop_run(B, STOP_DELETE);
                     // ReturnToElement (ResetElementsTask.cpp:232-265)
                     double dtt = g.timestep;
                     double outside[3] = {pos[0], pos[1], pos[2]};

@@ -44,6 +44,8 @@ struct MpmProblem {
     double timestepIn = 1.;          // <TimeStep> (global timestep default)
     double maxTime = 1.;             // maxtime (NairnMPM.cpp:104)
     int ptsPerSide = 2;              // MatlPtsPerElement (NairnMPM.cpp:98-99)
+    int leaveLimit = 0;              // <LeaveLimit> (warnParticleLeftGrid input, NairnMPM.cpp:110)
+    int deleteLimit = -1;            // <DeleteLimit> (warnParticleDeleted input, NairnMPM.cpp:112)
     bool ptsSet = false;
     std::string archiveRoot;
     std::string archiveOrder = "iY";   // <MPMArchiveOrder>, <MPMArchive> items (ArchiveData defaults)

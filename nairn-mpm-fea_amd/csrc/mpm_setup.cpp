@@ -498,6 +498,21 @@ bool finalize(MpmProblem &p) {
     p.run.strainFirst = 0.5 * dt;               // fractionUSF = 0.5
     p.run.strainLast = dt - p.run.strainFirst;
     p.run.mpmMethod = 2;
+    // NairnMPM::CreateWarnings (NairnMPM.cpp:814-848): LeaveLimit 0 means 1% of the particles;
+    // then > 1 pushes leaving particles back and aborts once that many have left, 1 pushes back
+    // and never aborts (a warning limit of -1), and <= 0 deletes them into the reservoir
+    {
+        int lim = p.leaveLimit == 0 ? (int)(n / 100) : p.leaveLimit;
+        if (lim > 1) { p.run.deleteLeaving = 0; p.run.maxLeaving = lim; }
+        else if (lim == 1) { p.run.deleteLeaving = 0; p.run.maxLeaving = 0; }
+        else { p.run.deleteLeaving = 1; p.run.maxLeaving = -lim; }
+        // DeleteLimit > 1 deletes particles whose position is NaN (into the reservoir); otherwise
+        // the first NaN position stops the run
+        if (p.deleteLimit > 1) {
+            p.err = "<DeleteLimit> above 1 deletes particles into the reservoir, which is not on this path";
+            return false;
+        }
+    }
     p.finalized = true;
     return true;
 }

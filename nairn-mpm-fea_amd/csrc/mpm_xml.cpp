@@ -345,7 +345,9 @@ struct Reader {
             } else if (k.name == "ArchiveTime") {
                 p.archiveTime = textnum(k) * time_units(k);
             } else if (k.name == "LeaveLimit") {
-                p.run.maxLeaving = (int)textnum(k);
+                p.leaveLimit = (int)textnum(k);           // MPMReadHandler.cpp:837-843
+            } else if (k.name == "DeleteLimit") {
+                p.deleteLimit = (int)textnum(k);          // MPMReadHandler.cpp:844-850
             } else if (k.name == "Damping" || k.name == "PDamping") {
                 // MPMReadHandler.cpp:311-350: a constant in 1/s (BodyForce damping, pdamping)
                 if (k.attr("function")) return fail("<" + k.name + " function=> (damping that varies in time) is not on this path");

@@ -99,7 +99,12 @@ REQ_QUEUED, REQ_RESIDENT, REQ_DONE, REQ_COLLECTED = 0, 1, 2, 3
 
 class Stats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("nparticles", ctypes.c_int64), ("leftGrid", ctypes.c_int64),
-                ("nanParticles", ctypes.c_int64), ("time", ctypes.c_double), ("lastStepMs", ctypes.c_double)]
+                ("nanParticles", ctypes.c_int64), ("time", ctypes.c_double), ("lastStepMs", ctypes.c_double),
+                ("stopped", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+# why a run stopped (MpmStats.stopped)
+STOP_TOO_MANY_LEFT, STOP_NAN, STOP_DELETE = 1, 2, 3
 
 
 class VelocityBC(ctypes.Structure):

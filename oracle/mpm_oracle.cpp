@@ -1429,20 +1429,41 @@ struct Oracle {
     }
 
     // ResetElementsTask::Execute / ResetElement / ReturnToElement (ResetElementsTask.cpp:44-265),
-    // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640), EdgeElement2D/3D (MeshInfo.cpp:171-199)
+    // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640), EdgeElement2D/3D (MeshInfo.cpp:171-199).
+    // Stops the run as the reference's CommonException does (ResetElementsTask.cpp:71-151): at the
+    // first NaN position (the DeleteLimit default), when the pushed-back count reaches maxLeaving,
+    // or when a leaving particle would be deleted (deleteLeaving, not supported). Serially, the
+    // particles after the one that stops the run are not reset, as in the reference's loop.
+    std::string abortMsg;
     void task_reset_elements() {
+        abortMsg.clear();
+        if (nthreads <= 1) {
+            for (long p = 0; p < np; p++)
+                if (!reset_element(p)) break;
+        } else {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (long p = 0; p < np; p++) reset_element(p);
+        }
+        if (!abortMsg.empty()) throw std::runtime_error(abortMsg);
+    }
+    void stop_run(const char *m) {
+#pragma omp critical(oracle_abort)
+        if (abortMsg.empty()) abortMsg = m;
+    }
+    bool reset_element(long p) {
         const MpmGridDesc &d = g.d;
         long horiz = d.nx, vert = d.ny, depth = g.dim == 3 ? d.nz : 1;
         long totalElems = horiz * vert * depth;
-        particles([&](long p) {
+        {
             double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), F(MPF_POS_Z, p)};
             if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
 #pragma omp atomic
                 nanCount++;
-                return;
+                stop_run("Particle left grid with position nan.");
+                return false;
             }
             long e = I(MPI_ELEM, p);
-            if (pt_in_element(g, e, pos)) return;
+            if (pt_in_element(g, e, pos)) return true;
             bool left = false;
             long col = (long)(int)((pos[0] - d.xmin) / d.dx);
             if (col < 0 || col >= horiz) {
@@ -1471,11 +1492,21 @@ struct Oracle {
                     else if (num % horiz <= 1) left = true;
                 }
             }
-            if (!left) { I(MPI_ELEM, p) = (int32_t)(num - 1); return; }
+            if (!left) { I(MPI_ELEM, p) = (int32_t)(num - 1); return true; }
             if (!I(MPI_LEFT, p)) {
-#pragma omp atomic
-                leftGrid++;
+                int64_t count;
+#pragma omp atomic capture
+                count = ++leftGrid;
+                if (run.maxLeaving > 0 && count >= run.maxLeaving) {
+                    stop_run("Too many particles have left the grid");
+                    return false;
+                }
                 I(MPI_LEFT, p) = 1;
+            }
+            if (run.deleteLeaving) {
+                stop_run("A particle has left the grid and would be deleted into the reservoir, which is not "
+                         "supported (set <LeaveLimit> above 1 to push leaving particles back)");
+                return false;
             }
             // ReturnToElement
             double dt = run.timestep;
@@ -1496,7 +1527,8 @@ struct Oracle {
             }
             F(MPF_POS_X, p) = inside[0]; F(MPF_POS_Y, p) = inside[1];
             if (g.dim == 3) F(MPF_POS_Z, p) = inside[2];
-        });
+        }
+        return true;
     }
 
     // NairnMPM::MPMStep (NairnMPM.cpp:284-335) with the task list of CreateTasks (:870-1110)
@@ -1545,11 +1577,15 @@ int oracle_step(void *h, int32_t nsteps) {
     Oracle *o = (Oracle *)h;
     try {
         for (int s = 0; s < nsteps; s++) o->step();
-    } catch (const std::exception &) {
+    } catch (const std::exception &e) {
+        o->err = e.what();
         return 1;
     }
     return 0;
 }
+
+/* message of the last failed oracle_step / oracle_run_task */
+const char *oracle_error(void *h) { return ((Oracle *)h)->err.c_str(); }
 
 /* run one task of the step (same ids as MPMGPU_TASK_*) */
 int oracle_run_task(void *h, int32_t task) {
@@ -1569,7 +1605,8 @@ int oracle_run_task(void *h, int32_t task) {
             o->task_reset_elements(); o->steps++; o->mtime += o->run.timestep; break;
         default: return 1;
     }
-    } catch (const std::exception &) {
+    } catch (const std::exception &e) {
+        o->err = e.what();
         return 1;
     }
     return 0;

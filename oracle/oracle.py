@@ -37,6 +37,8 @@ def lib():
         _lib.oracle_create.restype = _P
         _lib.oracle_destroy.argtypes = [_P]
         _lib.oracle_step.argtypes = [_P, ctypes.c_int32]
+        _lib.oracle_error.argtypes = [_P]
+        _lib.oracle_error.restype = ctypes.c_char_p
         _lib.oracle_run_task.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_set_threads.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_set_velocity_bcs.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.VelocityBC)]
@@ -111,11 +113,11 @@ class Oracle:
 
     def step(self, n=1):
         if lib().oracle_step(self._h, n) != 0:
-            raise RuntimeError("oracle step failed (a CPDI particle domain node has left the grid)")
+            raise RuntimeError("oracle step failed: " + lib().oracle_error(self._h).decode())
 
     def run_task(self, t):
         if lib().oracle_run_task(self._h, t) != 0:
-            raise RuntimeError("oracle task failed")
+            raise RuntimeError("oracle task failed: " + lib().oracle_error(self._h).decode())
 
     def particles(self):
         import nairnmpm as nm

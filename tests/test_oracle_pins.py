@@ -386,6 +386,7 @@ def _leaver(frac, speed_cells=1.5, lp=0.5):
     prob.set_grid([0, 16, 0, 16, 0, 0], [1, 1, -1])
     mid = prob.add_material(nm.MAT_ISOTROPIC, "m", E=1000., nu=0.3, rho=1.0)
     prob.add_point(mid, [15.5, 8.5, 0.], vel=[0., 0., 0.], lp=(lp, lp, lp))
+    prob.set("LeaveLimit", 1000)        # push back (one particle alone would be deleted by default)
     g, mats, run, _, _ = prob.describe()
     d, i = prob.particles()
     dt = run.timestep
