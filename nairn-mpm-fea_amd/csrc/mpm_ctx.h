@@ -118,4 +118,18 @@ const char *check_materials(const MpmMaterial *mats, int nmat);
 void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run);
 // ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
 size_t bin_words(long long nk, long long ntall);
+// byte sizes of a context's grid, bin and counter arrays for its grid c->kg and nmat materials
+struct GridArrays {
+    size_t nodeBytes = 0, cntBytes = 0, partBytes = 0, matBytes = 0, statBytes = 0, binBytes = 0;
+    long long nkeys = 0;
+};
+GridArrays grid_arrays(const MpmCtx *c, int nmat);
+// the context's node, tile-partial, bin and counter pointers into arrays of those sizes
+void assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, int *cntMem, double *partMem, int *bbuf,
+                        unsigned long long *sbuf);
+// tile kernel grids sized for the current device's compute units
+void size_tile_grids();
+// the grid of one simulation (mpmgpu_create's checks and settings); nullptr or the message
+const char *single_grid(MpmCtx *c, const MpmGridDesc *grid, const MpmMaterial *mats, int nmat,
+                        const MpmRunParams *run);
 }  // namespace mpmi

@@ -135,7 +135,7 @@ static int collect_marks(MpmCtx *c) {
 static int read_stop(MpmCtx *c, unsigned long long *why) {
     unsigned long long s[3] = {0, 0, 0};
     HIPCHECK(c, hipMemcpy(s, c->kb.stop, sizeof(s), hipMemcpyDeviceToHost));
-    *why = s[0];
+    *why = stop_reason(s[0]);
     if (s[0] && !c->stopped) {
         c->stopped = true;
         c->parity = (int)((s[1] + 1) & 1ull);
@@ -428,29 +428,35 @@ void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 
 // ---- device context --------------------------------------------------------------------
 
-// device buffers of the grid, bins and counters for c->kg (nstride nodes, tilesTotal tiles per field)
-static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
-    KGrid &g = c->kg;
+}  // extern "C"
+
+// sizes of the grid, bin and counter arrays of a context for c->kg (nstride nodes, tilesTotal
+// tiles per field)
+mpmi::GridArrays mpmi::grid_arrays(const MpmCtx *c, int nmat) {
+    const KGrid &g = c->kg;
     const int nf = g.nfields;
-    int foot = c->dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
-    int nc = c->dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
-    long long nn = g.nstride;
-    c->kb.nkeys = nf * g.tilesTotal * nc;
-    hipError_t e;
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
-        return fail_global(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    const long long foot = c->dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
+    const long long nc = c->dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
+    const long long nn = g.nstride;
+    GridArrays a;
     // per field: mass, pk, vk, ftot (+ contact volume, displacement, gradient in multimaterial mode)
     const int perField = 1 + 3 + 3 + 3 + (nf > 1 ? 1 + 3 + 3 : 0);
-    size_t nodeBytes = (size_t)nf * nn * perField * sizeof(double);
-    double *nodeMem = nullptr;
-    int *cntMem = nullptr;
-    double *partMem = nullptr;
-    if (hipMalloc(&nodeMem, nodeBytes) != hipSuccess || hipMalloc(&cntMem, (size_t)nf * nn * sizeof(int)) != hipSuccess ||
-        hipMalloc(&partMem, (size_t)nf * g.tilesTotal * NQMAX * foot * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->dmats, nmat * sizeof(MpmMaterial)) != hipSuccess ||
-        hipMalloc(&c->sbuf, STATS_WORDS * sizeof(unsigned long long)) != hipSuccess)
-        return fail_global("out of device memory allocating the grid");
-    long long fnn = (long long)nf * nn;
+    a.nodeBytes = (size_t)nf * nn * perField * sizeof(double);
+    a.cntBytes = (size_t)nf * nn * sizeof(int);
+    a.partBytes = (size_t)nf * g.tilesTotal * NQMAX * foot * sizeof(double);
+    a.matBytes = (size_t)nmat * sizeof(MpmMaterial);
+    a.statBytes = STATS_WORDS * sizeof(unsigned long long);
+    a.nkeys = nf * g.tilesTotal * nc;
+    a.binBytes = bin_words(a.nkeys, (long long)nf * g.tilesTotal) * sizeof(int);
+    return a;
+}
+
+// the context's grid, bin and counter pointers into the given arrays (sized by grid_arrays)
+void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, int *cntMem, double *partMem,
+                              int *bbuf, unsigned long long *sbuf) {
+    const KGrid &g = c->kg;
+    const int nf = g.nfields;
+    const long long fnn = (long long)nf * g.nstride;
     c->kn.mass = nodeMem;
     c->kn.pk = nodeMem + fnn;
     c->kn.vk = nodeMem + 4 * fnn;
@@ -463,60 +469,90 @@ static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
         c->kn.cdisp = nodeMem + 11 * fnn;
         c->kn.cgrad = nodeMem + 14 * fnn;
     }
-    c->ks.leftGrid = c->sbuf;
-    c->ks.nanCount = c->sbuf + 1;
-    c->ks.cpdiErr = c->sbuf + 2;
-    c->kb.stop = c->sbuf + 3;
-    (void)hipMemsetAsync(nodeMem, 0, nodeBytes, c->stream);
-    (void)hipMemsetAsync(cntMem, 0, (size_t)nf * nn * sizeof(int), c->stream);
-    (void)hipMemsetAsync(c->sbuf, 0, STATS_WORDS * sizeof(unsigned long long), c->stream);
-    (void)hipMemcpyAsync(c->dmats, mats, nmat * sizeof(MpmMaterial), hipMemcpyHostToDevice, c->stream);
+    c->sbuf = sbuf;
+    c->ks.leftGrid = sbuf;
+    c->ks.nanCount = sbuf + 1;
+    c->ks.cpdiErr = sbuf + 2;
+    c->kb.stop = sbuf + 3;
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
     // (nfields * tilesTotal) and its length and work counters
-    long long nk = c->kb.nkeys;
-    long long nblk = (nk + 1 + 1023) / 1024 + 1;
+    const long long nk = a.nkeys;
+    const long long nblk = (nk + 1 + 1023) / 1024 + 1;
+    c->kb.nkeys = (int)nk;
     c->kb.ntall = nf * g.tilesTotal;
-    const size_t bwords = bin_words(nk, c->kb.ntall);
-    if (hipMalloc(&c->bbuf, bwords * sizeof(int)) != hipSuccess)
-        return fail_global("out of device memory allocating the bins");
-    c->kb.cellStart = c->bbuf;
-    c->kb.count = c->bbuf + nk + 1;
-    c->kb.cursor = c->bbuf + 2 * nk + 2;
-    c->kb.blockSums = c->bbuf + 3 * nk + 2;
+    c->bbuf = bbuf;
+    c->kb.cellStart = bbuf;
+    c->kb.count = bbuf + nk + 1;
+    c->kb.cursor = bbuf + 2 * nk + 2;
+    c->kb.blockSums = bbuf + 3 * nk + 2;
     c->kb.tiles = c->kb.blockSums + nblk;
     c->kb.ntiles = c->kb.tiles + c->kb.ntall;
     c->kb.work = c->kb.ntiles + 1;
     c->kb.slot = 0;
-    (void)hipMemsetAsync(c->bbuf, 0, bwords * sizeof(int), c->stream);
-    {
-        // the tile kernels' grids fill every CU at the kernels' occupancy
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
-            set_tile_cus(cus);
+}
+
+// the tile kernels' grids fill every CU of the current device at the kernels' occupancy
+void mpmi::size_tile_grids() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0)
+        set_tile_cus(cus);
+}
+
+// device buffers of the grid, bins and counters for c->kg (nstride nodes, tilesTotal tiles per field)
+static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail_global(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    const mpmi::GridArrays a = mpmi::grid_arrays(c, nmat);
+    double *nodeMem = nullptr;
+    int *cntMem = nullptr;
+    double *partMem = nullptr;
+    unsigned long long *sbuf = nullptr;
+    int *bbuf = nullptr;
+    if (hipMalloc(&nodeMem, a.nodeBytes) != hipSuccess || hipMalloc(&cntMem, a.cntBytes) != hipSuccess ||
+        hipMalloc(&partMem, a.partBytes) != hipSuccess || hipMalloc(&c->dmats, a.matBytes) != hipSuccess ||
+        hipMalloc(&sbuf, a.statBytes) != hipSuccess) {
+        if (nodeMem) (void)hipFree(nodeMem);
+        if (cntMem) (void)hipFree(cntMem);
+        if (partMem) (void)hipFree(partMem);
+        if (sbuf) (void)hipFree(sbuf);
+        return fail_global("out of device memory allocating the grid");
     }
+    if (hipMalloc(&bbuf, a.binBytes) != hipSuccess) {
+        (void)hipFree(nodeMem);
+        (void)hipFree(cntMem);
+        (void)hipFree(partMem);
+        (void)hipFree(sbuf);
+        return fail_global("out of device memory allocating the bins");
+    }
+    mpmi::assign_grid_arrays(c, a, nodeMem, cntMem, partMem, bbuf, sbuf);
+    (void)hipMemsetAsync(nodeMem, 0, a.nodeBytes, c->stream);
+    (void)hipMemsetAsync(cntMem, 0, a.cntBytes, c->stream);
+    (void)hipMemsetAsync(sbuf, 0, a.statBytes, c->stream);
+    (void)hipMemcpyAsync(c->dmats, mats, a.matBytes, hipMemcpyHostToDevice, c->stream);
+    (void)hipMemsetAsync(bbuf, 0, a.binBytes, c->stream);
+    mpmi::size_tile_grids();
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail_global("device initialisation failed");
     return 0;
 }
 
-int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat, const MpmRunParams *run,
-                  MpmCtx **out) {
-    *out = nullptr;
-    if (const char *m = check_run(grid, run)) return fail_global(m);
+const char *mpmi::single_grid(MpmCtx *c, const MpmGridDesc *grid, const MpmMaterial *mats, int nmat,
+                              const MpmRunParams *run) {
+    if (const char *m = check_run(grid, run)) return m;
     int nf = run->multiMaterial ? run->numFields : 1;
-    if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return fail_global("bad number of material velocity fields");
+    if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return "bad number of material velocity fields";
+    if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return "bad material count";
     if (nf > 1) {
         if (run->contactLaw < 0 || run->contactLaw >= nmat ||
             (mats[run->contactLaw].type != MPMGPU_MAT_COULOMB && mats[run->contactLaw].type != MPMGPU_MAT_CONTACT))
-            return fail_global("multimaterial mode needs a contact law material");
+            return "multimaterial mode needs a contact law material";
         if (run->normalsMethod < 0 || run->normalsMethod > 3)
-            return fail_global("contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG)");
+            return "contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG)";
     }
-    if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return fail_global("bad material count");
-    if (const char *m = check_materials(mats, nmat)) return fail_global(m);
-    MpmCtx *c = new MpmCtx();
+    if (const char *m = check_materials(mats, nmat)) return m;
     c->dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
     c->gd = *grid;
     c->run = *run;
@@ -532,6 +568,19 @@ int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat
         g.ignoreContact = law.type == MPMGPU_MAT_CONTACT ? 1 : 0;
         g.friction = law.friction;
         g.frictionStatic = law.frictionStatic;
+    }
+    return nullptr;
+}
+
+extern "C" {
+
+int mpmgpu_create(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat, const MpmRunParams *run,
+                  MpmCtx **out) {
+    *out = nullptr;
+    MpmCtx *c = new MpmCtx();
+    if (const char *m = mpmi::single_grid(c, grid, mats, nmat, run)) {
+        delete c;
+        return fail_global(m);
     }
     if (alloc_context(c, mats, nmat)) { mpmgpu_destroy(c); return 1; }
     *out = c;
@@ -698,6 +747,8 @@ int mpmi::prepare_particles(MpmCtx *c) {
     set_positions(c);
     launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
+    // the tile list counts a completed step at every binning; this one is the upload's, not a step's
+    HIPCHECK(c, hipMemsetAsync(c->kb.stop + 2, 0, sizeof(unsigned long long), c->stream));
     if (mpmi::sort_rows(c)) return 1;
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));

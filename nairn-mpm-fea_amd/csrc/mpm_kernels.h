@@ -83,7 +83,7 @@ struct KBins {
     int *tiles;                // the non-empty (field, tile) pairs of the binning, in no fixed order
     int *ntiles;               // length of tiles
     int *work;                 // tile kernels' list cursors: 8 per launch of a step, 32 ints apart
-    // run stop (ResetElementsTask's exceptions): stop[0] the reason (STOP_*, 0 while running),
+    // run stop (ResetElementsTask's exceptions): stop[0] the reason and its rank (stop_run; 0 while running),
     // stop[1] the steps completed before the step that stopped, stop[2] the steps completed since
     // the upload (counted by the tile list). Once stop[0] is set, binning builds an empty tile
     // list, so every later tile kernel has nothing to do and the particle state stays as the
@@ -125,10 +125,21 @@ enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
 // reasons a run stops on the device
 enum { STOP_NONE = 0, STOP_TOO_MANY_LEFT = 1, STOP_NAN = 2, STOP_DELETE = 3, STOP_BAD_INPUT = 4 };
 
-// the run stops (ResetElementsTask's exceptions, ResetElementsTask.cpp:71-151): the first reason
-// raised is kept with the number of steps completed before this one
-__device__ __forceinline__ void stop_run(const KBins &B, unsigned long long why) {
-    if (atomicCAS(&B.stop[0], 0ull, why) == 0ull) atomicExch(&B.stop[1], B.stop[2]);
+// The run stops (ResetElementsTask's exceptions, ResetElementsTask.cpp:71-151). The reference raises
+// the first exception in particle order; here the particles of a step run in parallel, so every
+// stop carries a rank and the lowest rank wins, whatever the timing: a NaN position ranks first
+// (rank -1), a leaving particle by its place among the step's new leavers (the value of the
+// left-grid count it incremented), so the decision falls to the leaver whose count is lowest, as
+// it does to the first leaver in particle order. stop[0] holds ~(((rank + 1) << 8) | why), nonzero
+// once stopped; stop[1] the steps completed before this one.
+__device__ __forceinline__ void stop_run(const KBins &B, unsigned long long why, long long rank) {
+    const unsigned long long key = ((unsigned long long)(rank + 1) << 8) | why;
+    atomicMax(&B.stop[0], ~key);
+    atomicExch(&B.stop[1], B.stop[2]);
+}
+// the reason of a stop word (0 while running)
+__host__ __device__ __forceinline__ unsigned long long stop_reason(unsigned long long word) {
+    return word ? (~word) & 0xffull : 0ull;
 }
 
 

@@ -1572,7 +1572,7 @@ __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B,
             // ResetElementsTask::ResetElement (ResetElementsTask.cpp:196-228)
             if (pos[0] != pos[0] || pos[1] != pos[1] || pos[2] != pos[2]) {
                 atomicAdd(st.nanCount, 1ull);
-                if (g.stopOnNan) stop_run(B, STOP_NAN);
+                if (g.stopOnNan) stop_run(B, STOP_NAN, -1);
             } else if (!pt_in_elem<DIM>(g, ei, ej, ek, pos)) {
                 // MeshInfo::FindElementFromPoint (MeshInfo.cpp:593-640) and EdgeElement2D/3D (:171-199)
                 bool left = false;
@@ -1615,13 +1615,17 @@ __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B,
                     P.elem[p] = e;
                     elem_ijk(g, e, ei, ej, ek);
                 } else {
+                    // the warning count comes first, then the deletion (ResetElementsTask.cpp:71-104)
                     if (!P.left[p]) {
                         P.left[p] = 1;
                         const unsigned long long before = atomicAdd(st.leftGrid, 1ull);
                         if (g.maxLeaving > 0 && before + 1ull >= (unsigned long long)g.maxLeaving)
-                            stop_run(B, STOP_TOO_MANY_LEFT);
+                            stop_run(B, STOP_TOO_MANY_LEFT, (long long)before);
+                        else if (g.deleteLeaving)
+                            stop_run(B, STOP_DELETE, (long long)before);
+                    } else if (g.deleteLeaving) {
+                        stop_run(B, STOP_DELETE, (long long)(*st.leftGrid));
                     }
-                    if (g.deleteLeaving) stop_run(B, STOP_DELETE);
                     // ReturnToElement (ResetElementsTask.cpp:232-265)
                     double dtt = g.timestep;
                     double outside[3] = {pos[0], pos[1], pos[2]};

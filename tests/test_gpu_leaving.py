@@ -17,13 +17,15 @@ F, I = nm.F, nm.I
 TOL = 1e-9
 
 
-def moving_block_3d(limit, speed=0.3):
-    xml = _header(configs.j2_block_3d(cells=12, lo=3, hi=9, vx=0.), f"    <LeaveLimit>{limit}</LeaveLimit>\n")
+def moving_block_3d(limit, speed=0.3, vx=20000.):
+    """The 3D J2 block's halves colliding at +-vx (mm/s) while the block drifts toward the +x face
+    of the grid by `speed` cells per step"""
+    xml = _header(configs.j2_block_3d(cells=12, lo=3, hi=9, vx=vx), f"    <LeaveLimit>{limit}</LeaveLimit>\n")
     prob = nm.Problem.from_xml(xml)
     g, mats, run, _, _ = prob.describe()
     d, i = prob.particles()
-    d[F["vel_x"]] = speed * g.dx / run.timestep
-    d[F["vel_z"]] = -0.1 * speed * g.dx / run.timestep
+    d[F["vel_x"]] += speed * g.dx / run.timestep
+    d[F["vel_z"]] += -0.1 * speed * g.dx / run.timestep
     return g, mats, run, d, i
 
 
@@ -35,7 +37,9 @@ def _both(g, mats, run, d, i):
 
 @pytest.mark.parametrize("dim", [2, 3])
 def test_pushed_back_particles_match_oracle(dim):
-    case = moving_block(limit=100000) if dim == 2 else moving_block_3d(100000)
+    # the blocks deform while they drift (a shearing profile in 2D, colliding halves in 3D): a rigid
+    # drift leaves only rounding noise in the stresses, which no relative bound can compare
+    case = moving_block(limit=100000, vmax=2000.) if dim == 2 else moving_block_3d(100000)
     sim, ora = _both(*case)
     print(f"leaving block {dim}D, {case[3].shape[1]} particles", flush=True)
     for n in (10, 20):
@@ -55,7 +59,7 @@ def test_pushed_back_particles_match_oracle(dim):
 
 
 def test_run_stops_when_the_limit_is_reached():
-    case = moving_block(limit=5)
+    case = moving_block(limit=5, vmax=2000.)
     sim, ora = _both(*case)
     print("leaving block, LeaveLimit 5", flush=True)
     for k in range(60):

@@ -136,3 +136,37 @@ def test_server_rejects_mismatched_requests():
     with pytest.raises(nm.MPMError):
         srv.submit_problem(prob3, 0)         # no steps
     srv.close()
+
+
+def test_served_results_match_oracle_at_50k_particles():
+    """A J2 and a neo-Hookean block of ~47k particles each (the served stream's smallest size),
+    served together with a third request that joins after a few steps, against the oracle."""
+    import oracle as orc
+    from helpers import compare
+    srv = nm.Server(analysis=nm.THREED, graph_steps=4)
+    cases = []
+    for k in (10, 0):                        # request 10 is J2, request 0 neo-Hookean
+        xml, _, j2 = configs.serve_request_3d(k, pad=3, nmin=50000, nmax=50000, smin=20, smax=20)
+        prob = nm.Problem.from_xml(xml)
+        cases.append((prob, 20, j2, srv.submit_problem(prob, 20)))
+    assert [c[2] for c in cases] == [True, False]
+    srv.advance(4)
+    late = _requests([3], nmin=5000, nmax=5000, smin=12, smax=12)[0]
+    late_id = srv.submit_problem(*late)
+    while any(srv.info(r)["state"] < nm.REQ_DONE for r in [c[3] for c in cases] + [late_id]):
+        srv.advance(4)
+    for prob, steps, j2, rid in cases:
+        d, _ = srv.collect(rid)
+        g, mats, run, n, _ = prob.describe()
+        dd, ii = prob.particles()
+        print(f"served {'J2' if j2 else 'neo-Hookean'} block, {n} particles, {steps} steps", flush=True)
+        ora = orc.Oracle(g, mats, run, dd, ii)
+        ora.step(steps)
+        r, _ = ora.particles()
+        errs = compare(d, r, 3)
+        print(errs, flush=True)
+        for key, v in errs.items():
+            assert v <= 1e-9, (j2, key, v)
+        if j2:
+            assert np.count_nonzero(r[nm.F["hist0"]]) > 0      # some particles yielded
+    srv.close()

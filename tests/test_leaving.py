@@ -16,17 +16,18 @@ def _header(xml, extra):
     return xml.replace("  </MPMHeader>", extra + "  </MPMHeader>", 1)
 
 
-def moving_block(limit=None, block=8, cells=16, speed=0.3, seed=5):
+def moving_block(limit=None, block=8, cells=16, speed=0.3, seed=5, vmax=0.):
     """A 2D elastic block (block x block cells, 4 particles per cell, so 4 block^2 particles)
-    moving rigidly toward the right edge of the grid by `speed` cells per step."""
-    xml = configs.elastic_block_2d(cells=cells, block=block, vmax=0., jitter=0.02, seed=seed)
+    moving toward the right edge of the grid by `speed` cells per step: rigidly when vmax is 0,
+    else with a shearing velocity profile of amplitude vmax (mm/s) on top of the drift."""
+    xml = configs.elastic_block_2d(cells=cells, block=block, vmax=vmax, jitter=0.02, seed=seed)
     if limit is not None:
         xml = _header(xml, f"    <LeaveLimit>{limit}</LeaveLimit>\n")
     prob = nm.Problem.from_xml(xml)
     g, mats, run, _, _ = prob.describe()
     d, i = prob.particles()
-    d[F["vel_x"]] = speed * g.dx / run.timestep
-    d[F["vel_y"]] = 0.1 * speed * g.dx / run.timestep
+    d[F["vel_x"]] += speed * g.dx / run.timestep
+    d[F["vel_y"]] += 0.1 * speed * g.dx / run.timestep
     return g, mats, run, d, i
 
 
