@@ -353,6 +353,25 @@ const char *mpmgpu_last_error(const MpmCtx *ctx);
 const char *mpmgpu_global_error(void);
 void mpmgpu_destroy(MpmCtx *ctx);
 
+/* ---- a solve that allocates nothing ---------------------------------------------------- */
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): advances n particles
+   held in the caller's device memory by nsteps explicit steps, as mpmgpu_create +
+   mpmgpu_upload_particles + mpmgpu_step + mpmgpu_download_particles would, and returns with the
+   stream waited for. dfields/ifields are device pointers to the row layout of
+   mpmgpu_problem_particles (MPF_NUM_FIELDS rows of n doubles, MPI_NUM_FIELDS rows of n int32),
+   read as the initial state and overwritten with the final one. The grid, tile partials, bins,
+   counters and second position buffer come from one static device array of
+   MPMGPU_STATIC_SCRATCH_BYTES per device, held until the process ends; a solve needing more is
+   refused with a message giving its size. No velocity conditions or particle loads; no row
+   sorting (results do not depend on it). stream: a hipStream_t, or null for the null stream.
+   Every call on a device shares its array: calls that overlap on one device race. Errors come
+   from mpmgpu_global_error(); a run stopped on the device (ResetElementsTask's exceptions)
+   returns its message with the stopping step's state in the rows. */
+#define MPMGPU_STATIC_SCRATCH_BYTES (256ull << 20)
+int mpmgpu_solve_static(const MpmGridDesc *grid, const MpmMaterial *mats, int32_t nmat, const MpmRunParams *run,
+                        int64_t n, double *dfields, int32_t *ifields, int32_t nsteps, void *stream);
+
 /* ---- solve server: continuous batching of independent solves -------------------------- */
 /* No counterpart in the reference (NairnMPM runs one analysis per process). A request is a whole
    solve: grid, materials, run parameters, initial particles and a number of steps. Requests wait

@@ -192,7 +192,23 @@ _SIGNATURES = {
     "mpmgpu_server_clock": ([_P], ctypes.c_double),
     "mpmgpu_server_error": ([_P], ctypes.c_char_p),
     "mpmgpu_server_destroy": ([_P], None),
+    "mpmgpu_solve_static": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Material), ctypes.c_int32,
+                             ctypes.POINTER(RunParams), ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
 }
+
+# bytes of the static device array behind mpmgpu_solve_static (MPMGPU_STATIC_SCRATCH_BYTES)
+STATIC_SCRATCH_BYTES = 256 << 20
+
+
+def solve_static(grid, mats, run, n: int, dfields_dev: int, ifields_dev: int, steps: int, stream: int = 0):
+    """mpmgpu_solve_static: advance n particles held in device memory (addresses of the
+    [NUM_FIELDS, n] float64 and [NUM_IFIELDS, n] int32 rows) by `steps` steps without allocating;
+    stream is a hipStream_t address (0: the null stream)."""
+    arr = (Material * max(len(mats), 1))(*mats)
+    _check(library().mpmgpu_solve_static(ctypes.byref(grid), arr, len(mats), ctypes.byref(run), int(n),
+                                         ctypes.c_void_p(int(dfields_dev)), ctypes.c_void_p(int(ifields_dev)),
+                                         int(steps), ctypes.c_void_p(int(stream)) if stream else None))
 
 KERNEL_GROUPS = ["k_p2g_mass", "k_node_mass", "k_usf", "k_p2g_force", "k_node_force", "k_g2p_p2g",
                  "k_node_mom2", "k_usl_reset", "binning", "contact"]

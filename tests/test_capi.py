@@ -85,3 +85,13 @@ def test_create_fails_loudly_without_a_device():
         return
     # a device is present: the simulation is a GPU one
     sim.close()
+
+
+def test_static_solve_refuses_sizes_beyond_its_scratch():
+    """mpmgpu_solve_static checks its size against the static array before any device call."""
+    prob = nm.Problem.from_xml(configs.j2_block_3d(cells=128, lo=16, hi=112))
+    g, mats, run, n, _ = prob.describe()
+    with pytest.raises(nm.MPMError, match="bytes of static scratch"):
+        nm.solve_static(g, mats, run, n, 0x1000, 0x1000, 1)
+    with pytest.raises(nm.MPMError, match="bad step count"):
+        nm.solve_static(g, mats, run, n, 0x1000, 0x1000, -1)
