@@ -65,18 +65,22 @@ TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_j2_block_128.md")
 
 
 def measured_traffic(kernel, cells):
-    """HBM bytes per launch of `kernel` on this workload, from the committed FETCH_SIZE/WRITE_SIZE
-    counter summary (rows '| kernel | cells | fetch KiB | write KiB | traffic bytes |'); None when
-    the kernel or size was not profiled."""
+    """(HBM bytes per launch of `kernel` on this workload, build the figure was measured on) from the
+    committed FETCH_SIZE/WRITE_SIZE counter summary (a 'Build: <commit>' line, then rows
+    '| kernel | cells | fetch KiB | write KiB | traffic bytes |'); (None, None) when the kernel or size
+    was not profiled. The figure is a table lookup, not a measurement of this run."""
+    build = None
     try:
         with open(TRAFFIC_TABLE) as f:
             for line in f:
+                if line.startswith("Build:"):
+                    build = line.split(":", 1)[1].strip()
                 cols = [c.strip() for c in line.strip().strip("|").split("|")]
                 if len(cols) >= 5 and cols[0] == kernel and cols[1] == str(cells):
-                    return float(cols[4])
+                    return float(cols[4]), build
     except OSError:
         pass
-    return None
+    return None, None
 
 
 def copy_calibration(nbytes, launches=5):
@@ -472,9 +476,12 @@ def main():
         bp += CPDI_EXTRA
     alg_bytes = bp * npart + bn * active_nodes
     achieved = alg_bytes / (dom_ms * 1e-3)
+    traffic, traffic_build = measured_traffic(dom, a.cells) if a.workload == "j2" else (None, None)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(dom, a.cells), "kernel": dom,
-                "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms}
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dom,
+                "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
+                "traffic_source": (f"table profiles/hbm_traffic_j2_block_128.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                   f"build {traffic_build})" if traffic is not None else None)}
 
     cpu = None
     if not a.no_cpu:
