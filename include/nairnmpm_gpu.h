@@ -106,7 +106,9 @@ typedef struct MpmRunParams {
     int32_t mpmMethod;         /* 2 = USAVG (only method of the hot path) */
     int32_t multiMaterial;     /* 0 single-material mode */
     int32_t contactLaw;        /* material index of the default contact law, -1 none */
-    int32_t normalsMethod;     /* MultiMaterialMode Normals= (0 MAXG, 1 MAXV, 2 AVGG, 3 OWNG) */
+    int32_t normalsMethod;     /* MultiMaterialMode Normals= (0 MAXG, 1 MAXV, 2 AVGG, 3 OWNG,
+                                  4 specified, 5 linear regression, 6 logistic regression, the
+                                  reference's default; MeshInfo.hpp:31-32) */
     int32_t deleteLeaving;     /* fmobj->deleteLeavingParticles: a particle that leaves the grid
                                   would be deleted into the reservoir; not supported, so the
                                   first one stops the run with an error */
@@ -118,6 +120,8 @@ typedef struct MpmRunParams {
     double gridDamping;        /* <Damping>, 1/s: BodyForce::GetGridDamping (BodyForce.cpp:167-184) */
     double particleDamping;    /* <PDamping>, 1/s: BodyForce::GetParticleDamping (:187-204) */
     double gravity[3];         /* <Gravity> BodyX/Y/ZForce, mm/s^2: BodyForce gforce (BodyForce.cpp:76-95) */
+    double contactNormal[3];   /* normals method 4: the unit normal from Polar/Azimuth
+                                  (MeshInfo::SetContactNormal, MeshInfo.cpp:1211-1227) */
 } MpmRunParams;
 
 /* Counters reported back from the device (ResetElementsTask.cpp:71-151 warnings) */

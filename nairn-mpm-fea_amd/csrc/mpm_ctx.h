@@ -31,8 +31,9 @@ constexpr int IROWS = MPI_NUM_FIELDS + 4;
 constexpr int IROW_KEY = MPI_NUM_FIELDS, IROW_ORDER = MPI_NUM_FIELDS + 1, IROW_HOME = MPI_NUM_FIELDS + 2,
               IROW_SHOME = MPI_NUM_FIELDS + 3;
 constexpr int64_t PARTICLE_BYTES = PROWS * sizeof(double) + IROWS * sizeof(int) + sizeof(double);
-// device counters: left grid, NaN positions, lCPDI corner faults, then the run stop (KBins::stop)
-constexpr int STATS_WORDS = 6;
+// device counters: left grid, NaN positions, lCPDI corner faults, the run stop (KBins::stop, three
+// words), regression-list overflows
+constexpr int STATS_WORDS = 7;
 
 struct MpmCtx {
     int dim = 3;
@@ -68,6 +69,8 @@ struct MpmCtx {
     int64_t steps = 0;
     double mtime = 0.;
     bool stopped = false;            // the device stopped the run (ResetElementsTask exception)
+    double *lrbuf = nullptr;         // regression normals: per-field normals and separations
+    int *lrlist = nullptr;           // regression normals: the step's contact nodes
     // grid velocity conditions (kbc.nb = 0: none) in one device buffer; bcTime is the step start
     // time as the device counts it (one addition of the time step per step)
     KBcs kbc{};
@@ -121,12 +124,13 @@ size_t bin_words(long long nk, long long ntall);
 // byte sizes of a context's grid, bin and counter arrays for its grid c->kg and nmat materials
 struct GridArrays {
     size_t nodeBytes = 0, cntBytes = 0, partBytes = 0, matBytes = 0, statBytes = 0, binBytes = 0;
+    size_t normBytes = 0, listBytes = 0;      // regression normals (methods 5 and 6) only
     long long nkeys = 0;
 };
 GridArrays grid_arrays(const MpmCtx *c, int nmat);
 // the context's node, tile-partial, bin and counter pointers into arrays of those sizes
 void assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, int *cntMem, double *partMem, int *bbuf,
-                        unsigned long long *sbuf);
+                        unsigned long long *sbuf, double *normMem, int *listMem);
 // tile kernel grids sized for the current device's compute units
 void size_tile_grids();
 // the grid of one simulation (mpmgpu_create's checks and settings); nullptr or the message

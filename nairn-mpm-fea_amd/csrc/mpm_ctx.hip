@@ -78,6 +78,7 @@ void mpmi::enqueue_step(MpmCtx *c) {
     if (mm) {      // PostExtrapolationTask: contact at MASS_MOMENTUM_CALL
         launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, false, s);
         launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+        launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_MASS_MOMENTUM, s);
         launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
         vel_bcs(BC_PASS_MASS_MOMENTUM);
         done(MPMGPU_KG_CONTACT);
@@ -91,6 +92,7 @@ void mpmi::enqueue_step(MpmCtx *c) {
     vel_bcs(mm ? BC_PASS_GRID_FORCES : BC_PASS_FORCES_MOMENTA);
     done(MPMGPU_KG_NODE_FORCE);
     if (mm) {      // UpdateMomentaTask: contact at UPDATE_MOMENTUM_CALL
+        launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_UPDATE_MOMENTUM, s);
         launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
         vel_bcs(BC_PASS_UPDATE_MOMENTUM);
         done(MPMGPU_KG_CONTACT);
@@ -103,6 +105,7 @@ void mpmi::enqueue_step(MpmCtx *c) {
     if (mm) {      // UpdateStrainsLastContactTask: contact at UPDATE_STRAINS_LAST_CALL
         launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, true, s);
         launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+        launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_UPDATE_STRAINS_LAST, s);
         launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
         vel_bcs(BC_PASS_UPDATE_STRAINS_LAST);
         done(MPMGPU_KG_CONTACT);
@@ -448,12 +451,16 @@ mpmi::GridArrays mpmi::grid_arrays(const MpmCtx *c, int nmat) {
     a.statBytes = STATS_WORDS * sizeof(unsigned long long);
     a.nkeys = nf * g.tilesTotal * nc;
     a.binBytes = bin_words(a.nkeys, (long long)nf * g.tilesTotal) * sizeof(int);
+    if (nf > 1 && g.normals >= 5) {
+        a.normBytes = (size_t)nf * 4 * nn * sizeof(double);
+        a.listBytes = (size_t)(nn + 1) * sizeof(int);
+    }
     return a;
 }
 
 // the context's grid, bin and counter pointers into the given arrays (sized by grid_arrays)
 void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, int *cntMem, double *partMem,
-                              int *bbuf, unsigned long long *sbuf) {
+                              int *bbuf, unsigned long long *sbuf, double *normMem, int *listMem) {
     const KGrid &g = c->kg;
     const int nf = g.nfields;
     const long long fnn = (long long)nf * g.nstride;
@@ -474,6 +481,9 @@ void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, i
     c->ks.nanCount = sbuf + 1;
     c->ks.cpdiErr = sbuf + 2;
     c->kb.stop = sbuf + 3;
+    c->ks.lrOverflow = sbuf + 6;
+    c->kn.cnorm = a.normBytes ? normMem : nullptr;
+    c->kn.clist = a.listBytes ? listMem : nullptr;
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
     // (nfields * tilesTotal) and its length and work counters
     const long long nk = a.nkeys;
@@ -526,7 +536,17 @@ static int alloc_context(MpmCtx *c, const MpmMaterial *mats, int nmat) {
         (void)hipFree(sbuf);
         return fail_global("out of device memory allocating the bins");
     }
-    mpmi::assign_grid_arrays(c, a, nodeMem, cntMem, partMem, bbuf, sbuf);
+    if (a.normBytes && (hipMalloc(&c->lrbuf, a.normBytes) != hipSuccess ||
+                        hipMalloc(&c->lrlist, a.listBytes) != hipSuccess)) {
+        (void)hipFree(nodeMem);
+        (void)hipFree(cntMem);
+        (void)hipFree(partMem);
+        (void)hipFree(sbuf);
+        (void)hipFree(bbuf);
+        return fail_global("out of device memory allocating the contact normals");
+    }
+    mpmi::assign_grid_arrays(c, a, nodeMem, cntMem, partMem, bbuf, sbuf, c->lrbuf, c->lrlist);
+    if (a.listBytes) (void)hipMemsetAsync(c->lrlist, 0, a.listBytes, c->stream);
     (void)hipMemsetAsync(nodeMem, 0, a.nodeBytes, c->stream);
     (void)hipMemsetAsync(cntMem, 0, a.cntBytes, c->stream);
     (void)hipMemsetAsync(sbuf, 0, a.statBytes, c->stream);
@@ -549,8 +569,9 @@ const char *mpmi::single_grid(MpmCtx *c, const MpmGridDesc *grid, const MpmMater
         if (run->contactLaw < 0 || run->contactLaw >= nmat ||
             (mats[run->contactLaw].type != MPMGPU_MAT_COULOMB && mats[run->contactLaw].type != MPMGPU_MAT_CONTACT))
             return "multimaterial mode needs a contact law material";
-        if (run->normalsMethod < 0 || run->normalsMethod > 3)
-            return "contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG)";
+        if (run->normalsMethod < 0 || run->normalsMethod > 6)
+            return "contact normals method must be 0 (MAXG), 1 (MAXV), 2 (AVGG), 3 (OWNG), 4 (specified), "
+                   "5 (linear regression) or 6 (logistic regression)";
     }
     if (const char *m = check_materials(mats, nmat)) return m;
     c->dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
@@ -568,6 +589,7 @@ const char *mpmi::single_grid(MpmCtx *c, const MpmGridDesc *grid, const MpmMater
         g.ignoreContact = law.type == MPMGPU_MAT_CONTACT ? 1 : 0;
         g.friction = law.friction;
         g.frictionStatic = law.frictionStatic;
+        for (int a = 0; a < 3; a++) g.contactNormal[a] = run->contactNormal[a];
     }
     return nullptr;
 }
@@ -990,6 +1012,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             if (c->kg.nfields > 1) {
                 launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, false, s);
                 launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+                launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_MASS_MOMENTUM, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_MASS_MOMENTUM, s);
             }
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_MASS_MOMENTUM, c->parity, s);
@@ -1000,7 +1023,10 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
         case MPMGPU_TASK_UPDATE_MOMENTA:
             launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_GRID_FORCES, c->parity, s);
-            if (c->kg.nfields > 1) launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
+            if (c->kg.nfields > 1) {
+                launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_UPDATE_MOMENTUM, s);
+                launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
+            }
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_MOMENTUM, c->parity, s);
             break;
         case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
@@ -1009,6 +1035,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             if (c->kg.nfields > 1) {
                 launch_p2g_contact(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, true, s);
                 launch_node_contact(c->dim, c->kg, c->kb, c->kn, s);
+                launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_UPDATE_STRAINS_LAST, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
             }
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_STRAINS_LAST, c->parity, s);
@@ -1143,8 +1170,15 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     return 0;
 }
 
-// lCPDI corner faults are fatal, as the reference's exception (MatPoint3D.cpp:610-615)
+// lCPDI corner faults are fatal, as the reference's exception (MatPoint3D.cpp:610-615); so is a
+// contact node seen by more particles than a regression list holds
 static int check_cpdi(MpmCtx *c) {
+    if (c->kn.cnorm) {
+        unsigned long long o = 0;
+        HIPCHECK(c, hipMemcpy(&o, c->ks.lrOverflow, sizeof(o), hipMemcpyDeviceToHost));
+        if (o) return c->fail("a contact node is seen by more than 1024 particles, more than the regression normals' "
+                              "list holds (" + std::to_string(o) + " node passes)");
+    }
     if (c->run.shape != MPMGPU_SHAPE_LCPDI) return 0;
     unsigned long long e = 0;
     HIPCHECK(c, hipMemcpy(&e, c->ks.cpdiErr, sizeof(e), hipMemcpyDeviceToHost));
@@ -1285,6 +1319,8 @@ void mpmgpu_destroy(MpmCtx *c) {
     if (c->bcbuf) (void)hipFree(c->bcbuf);
     if (c->ldbuf) (void)hipFree(c->ldbuf);
     if (c->tbuf) (void)hipFree(c->tbuf);
+    if (c->lrbuf) (void)hipFree(c->lrbuf);
+    if (c->lrlist) (void)hipFree(c->lrlist);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -45,6 +45,7 @@ struct KGrid {
     int normals;
     int ignoreContact;         // ContactLaw type 60: always move to the center-of-mass velocity
     double friction, frictionStatic;
+    double contactNormal[3];   // normals method 4 (specified)
     int shape;                 // MPMGPU_SHAPE_UGIMP or MPMGPU_SHAPE_LCPDI
     // ResetElementsTask's stops (one simulation; a batch sets all three to 0): the run stops when
     // maxLeaving particles have left the grid (0: never), at the first leaving particle when
@@ -110,6 +111,11 @@ struct KNodes {
     double *cvol;              // nnodes
     double *cdisp;             // 3*nnodes
     double *cgrad;             // 3*nnodes
+    // regression normals (methods 5 and 6): per field the unit normal and the separation of the
+    // field's material against the rest at each contact node, 4*nnodes (component-major); and
+    // the list of contact nodes of the step (clist[0] their count, the nodes from clist[1])
+    double *cnorm;
+    int *clist;
 };
 // Node arrays hold nfields consecutive copies (field f at offset f*nnodes for scalars and
 // f*3*nnodes for vectors); a kernel working on one field takes a view shifted to it.
@@ -118,6 +124,7 @@ struct KStats {
     unsigned long long *leftGrid;
     unsigned long long *nanCount;
     unsigned long long *cpdiErr;   // lCPDI corners outside the grid or beyond the neighbour cells
+    unsigned long long *lrOverflow;  // contact nodes seen by more particles than a regression list holds
 };
 
 enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
@@ -173,6 +180,11 @@ void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b,
                         const MpmMaterial *mats, bool newPositions, hipStream_t s);
 void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s);
+// regression normals (methods 5 and 6) before a contact pass: the contact-node list (at the
+// mass-momentum call), then each contact node's normals and separations from the particles that
+// see it (current positions: the new ones at the strains-last call)
+void launch_contact_normals(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                            const MpmMaterial *mats, const KStats &st, int callType, hipStream_t s);
 
 // Grid velocity conditions of one simulation (single grid), grouped by node: node b's conditions
 // are entries start[b]..start[b+1]-1, in the order of the condition list. One thread per

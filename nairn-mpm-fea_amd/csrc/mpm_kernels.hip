@@ -792,6 +792,7 @@ __global__ void k_node_mass(KGrid g0, KBins B, KNodes N) {
     const long long n = ng - g.nodeBase;
     if (n >= g.nnodes) return;                // padding between instances
     const KNodes Nf = field_view(g, N, fld);
+    if (nall == 0 && N.clist) N.clist[0] = 0;   // contact-node list of the step (regression normals)
     int i = (int)(n % g.nnx);
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
@@ -1960,10 +1961,14 @@ __global__ void k_contact(KGrid g, KNodes N, int callType) {
                 for (int c = 0; c < 3; c++) norm[c] += gradj[c];
                 double sc = 1. / sqrt(dot3(norm, norm));
                 for (int c = 0; c < 3; c++) norm[c] *= sc;
-            } else {
+            } else if (normals == 3) {
                 for (int c = 0; c < 3; c++) norm[c] = gradi[c];
                 double sc = 1. / sqrt(dot3(norm, norm));
                 for (int c = 0; c < 3; c++) norm[c] *= sc;
+            } else if (normals == 4) {            // SPECIFIED_NORMAL (no symmetry planes)
+                for (int c = 0; c < 3; c++) norm[c] = g.contactNormal[c];
+            } else {                              // regression: k_contact_normals found it
+                for (int c = 0; c < 3; c++) norm[c] = N.cnorm[(4 * i + c) * nn + n];
             }
             if (norm[0] != norm[0] || norm[1] != norm[1] || norm[2] != norm[2]) continue;
             double dotn = dot3(delPi, norm);
@@ -1976,7 +1981,8 @@ __global__ void k_contact(KGrid g, KNodes N, int callType) {
             double dispb[3];
             for (int c = 0; c < 3; c++) dispb[c] = delta[c];
             for (int c = 0; c < 3; c++) dispb[c] += dispa[c];
-            double deln = dot3(dispb, norm) - dot3(dispa, norm);
+            // the regression normals come with the separation of the point cloud
+            double deln = normals >= 5 ? N.cnorm[(4 * i + 3) * nn + n] : dot3(dispb, norm) - dot3(dispa, norm);
             mred *= massi;
             double delFi[3];
             if (postUpdate) {
@@ -1987,13 +1993,395 @@ __global__ void k_contact(KGrid g, KNodes N, int callType) {
                 for (int c = 0; c < 3; c++) delFi[c] = fs[c] * sc;
                 for (int c = 0; c < 3; c++) delFi[c] += N.ftot[(3 * i + c) * nn + n] * -1.;
             }
-            if (!coulomb_delta(g, delPi, norm, dotn, deln, mred, postUpdate ? delFi : nullptr)) continue;
+            // the separation's momentum-update term is skipped for regression normals (CoulombFriction.cpp:162)
+            if (!coulomb_delta(g, delPi, norm, dotn, deln, mred, postUpdate && normals < 5 ? delFi : nullptr)) continue;
         }
         change_momentum(g, N, i, n, delPi, callType);
         if (doingPairs) {
             for (int c = 0; c < 3; c++) delPi[c] *= -1.;
             change_momentum(g, N, j, n, delPi, callType);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Regression contact normals (normals methods 5 and 6): CrackVelocityFieldMulti::
+// LinearRegressionNormal and LogisticRegressionNormal (Nodes/MachineLearning.cpp:58-633) over the
+// particles that see each contact node (MaterialContactNode::GetMaterialContactData,
+// MaterialContactNode.cpp:253-350). A wave takes one contact node at a time: it lists the node's
+// particles in LDS (cells and fields in a fixed order, particles of a cell in upload order), then
+// the lanes split the list for every sum of the fit and a fixed reduction tree adds their parts,
+// so each node's result is the same on every run.
+
+// contact nodes of the step: two or more fields with particles (CalcTotalMassAndCount,
+// PostExtrapolationTask.cpp:62-80); the count is cleared by the mass node pass before
+template <int DIM>
+__global__ void k_contact_list(KGrid g, KNodes N) {
+    const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool on = false;
+    if (n < g.nnodes) {
+        int numMats = 0;
+        for (int f = 0; f < g.nfields; f++) numMats += N.cnt[f * g.nstride + n] > 0 ? 1 : 0;
+        on = numMats > 1;
+    }
+    const unsigned long long m = __ballot(on);
+    if (m == 0ull) return;
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&N.clist[0], __popcll(m));
+    base = __shfl(base, leader);
+    if (on) N.clist[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = (int)n;
+}
+
+// node (i,j,k) in particle p's shape-function node list (start-of-step shape functions):
+// uGIMP/classic: inside the support on every axis; lCPDI: a corner's function at the node is
+// at least 1e-15 (GetCPDIFunctions, MoreMPMElementBase.cpp:581-656)
+template <int DIM, int SH>
+__device__ __forceinline__ bool lr_member(const KGrid &g, const KPart &P, int p, int i, int j, int k) {
+    int ei, ej, ek;
+    elem_ijk(g, P.elem[p], ei, ej, ek);
+    const int o[3] = {i - ei + 1, j - ej + 1, DIM == 3 ? k - ek + 1 : 1};
+#pragma unroll
+    for (int a = 0; a < DIM; a++)
+        if (o[a] < 0 || o[a] > 3) return false;
+    if (SH == MPMGPU_SHAPE_LCPDI) {
+        CpdiCorners cc;
+        cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
+        constexpr int NCOR = DIM == 3 ? 8 : 4;
+        constexpr double thr = (DIM == 3 ? 8. : 4.) * 1e-15;
+        bool in = false;
+#pragma unroll
+        for (int c = 0; c < NCOR; c++) {
+            double T[DIM][4];
+            cpdi_table<DIM>(cc.xi[c], cc.off[c], T);
+            const double f = DIM == 3 ? T[0][o[0]] * T[1][o[1]] * T[2][o[2]] : T[0][o[0]] * T[1][o[1]];
+            in |= f >= thr;
+        }
+        return in;
+    }
+    double xi[3], invd[3], S[3][4], dS[3][4];
+    particle_xi<DIM>(g, ei, ej, ek, P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0., xi, invd);
+    particle_factors<DIM>(g, P, p, xi, S, dS);
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < DIM; a++) in &= S[a][o[a]] >= 0.;
+    return in;
+}
+
+// sum over the wave in a fixed tree, the total in every lane
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_down(v, off);
+    return __shfl(v, 0);
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_down(v, off));
+    return __shfl(v, 0);
+}
+
+// Matrix4::cofactor, determinant, Inverse and Times (Matrix4.cpp:96-103,133-140,206-231): the
+// inverse as signed cofactors scaled by 1/det, then times rhs
+__device__ __forceinline__ double lr_cofactor4(const double m[4][4], int row, int col) {
+    double c[3][3];
+    int cr = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (i == row) continue;
+        int cc = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (j == col) continue;
+            c[cr][cc++] = m[i][j];
+        }
+        cr++;
+    }
+    return c[0][0] * (c[1][1] * c[2][2] - c[2][1] * c[1][2]) - c[1][0] * (c[0][1] * c[2][2] - c[2][1] * c[0][2]) +
+           c[2][0] * (c[0][1] * c[1][2] - c[1][1] * c[0][2]);
+}
+__device__ __forceinline__ void lr_solve4(const double m[4][4], const double rhs[4], double out[4]) {
+    const double det = m[0][0] * lr_cofactor4(m, 0, 0) - m[1][0] * lr_cofactor4(m, 1, 0) +
+                       m[2][0] * lr_cofactor4(m, 2, 0) - m[3][0] * lr_cofactor4(m, 3, 0);
+    const double f = 1. / det;
+    double inv[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) inv[r][c] = ((r + c) & 1 ? -lr_cofactor4(m, c, r) : lr_cofactor4(m, c, r)) * f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) out[r] = inv[r][0] * rhs[0] + inv[r][1] * rhs[1] + inv[r][2] * rhs[2] + inv[r][3] * rhs[3];
+}
+// symmetric Matrix3 (Matrix3.cpp:107-121) Inverse (3D form, :446-460) times rhs (:394-408)
+__device__ __forceinline__ void lr_solve3(const double s[6], const double rhs[3], double out[3]) {
+    const double m[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], s[5]}};
+    double inv[3][3] = {{m[1][1] * m[2][2] - m[2][1] * m[1][2], -(m[0][1] * m[2][2] - m[2][1] * m[0][2]),
+                         m[0][1] * m[1][2] - m[1][1] * m[0][2]},
+                        {-(m[1][0] * m[2][2] - m[2][0] * m[1][2]), m[0][0] * m[2][2] - m[2][0] * m[0][2],
+                         -(m[0][0] * m[1][2] - m[1][0] * m[0][2])},
+                        {m[1][0] * m[2][1] - m[2][0] * m[1][1], -(m[0][0] * m[2][1] - m[2][0] * m[0][1]),
+                         m[0][0] * m[1][1] - m[1][0] * m[0][1]}};
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                       m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) + m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+    const double f = 1. / det;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) inv[r][c] *= f;
+#pragma unroll
+    for (int r = 0; r < 3; r++) out[r] = inv[r][0] * rhs[0] + inv[r][1] * rhs[1] + inv[r][2] * rhs[2];
+}
+
+// MatPoint3D/MatPoint2D::GetDeformedRadius, ellipse methods (MatPoint3D.cpp:497-510,
+// MatPoint2D.cpp:482-497): the distance from the particle's center to its deformed domain's
+// surface along the unit vector n (2D applies F^-T where 3D applies F^-1, as the reference does)
+template <int DIM>
+__device__ __forceinline__ double lr_radius(const KGrid &g, const KPart &P, int p, const double n[3]) {
+    double lpx, lpy, lpz;
+    if (g.uniformLp) { lpx = g.lp[0]; lpy = g.lp[1]; lpz = g.lp[2]; }
+    else { lpx = P.f[MPF_LP_X][p]; lpy = P.f[MPF_LP_Y][p]; lpz = P.f[MPF_LP_Z][p]; }
+    const double psx = g.dx * (0.5 * lpx), psy = g.dy * (0.5 * lpy);      // GetParticleSize
+    if (DIM == 2) {
+        const double exx = P.f[MPF_EP_XX][p], eyy = P.f[MPF_EP_YY][p], exy = P.f[MPF_EP_XY][p], wxy = P.f[MPF_WR_XY][p];
+        const double F00 = 1. + exx, F01 = 0.5 * (exy - wxy), F10 = 0.5 * (exy + wxy), F11 = 1. + eyy;
+        const double rdet2D = 1. / (F00 * F11 - F01 * F10);
+        const double n0x = (F11 * n[0] - F10 * n[1]) * rdet2D;
+        const double n0y = (-F01 * n[0] + F00 * n[1]) * rdet2D;
+        return 1. / sqrt(n0x * n0x / (psx * psx) + n0y * n0y / (psy * psy));
+    }
+    const double psz = g.dz * (0.5 * lpz);
+    double ep[6], wr[3], m[3][3];
+#pragma unroll
+    for (int c = 0; c < 6; c++) ep[c] = P.f[MPF_EP_XX + c][p];
+#pragma unroll
+    for (int c = 0; c < 3; c++) wr[c] = P.f[MPF_WR_YZ + c][p];
+    get_F3(ep, wr, m);
+    double inv[3][3] = {{m[1][1] * m[2][2] - m[2][1] * m[1][2], -(m[0][1] * m[2][2] - m[2][1] * m[0][2]),
+                         m[0][1] * m[1][2] - m[1][1] * m[0][2]},
+                        {-(m[1][0] * m[2][2] - m[2][0] * m[1][2]), m[0][0] * m[2][2] - m[2][0] * m[0][2],
+                         -(m[0][0] * m[1][2] - m[1][0] * m[0][2])},
+                        {m[1][0] * m[2][1] - m[2][0] * m[1][1], -(m[0][0] * m[2][1] - m[2][0] * m[0][1]),
+                         m[0][0] * m[1][1] - m[1][0] * m[0][1]}};
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                       m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) + m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+    const double f = 1. / det;
+    double n0[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) n0[r] = (inv[r][0] * f) * n[0] + (inv[r][1] * f) * n[1] + (inv[r][2] * f) * n[2];
+    return 1. / sqrt(n0[0] * n0[0] / (psx * psx) + n0[1] * n0[1] / (psy * psy) + n0[2] * n0[2] / (psz * psz));
+}
+
+#define LR_MAX 1024        // particles listed at one contact node (a wave's LDS list)
+
+template <int DIM, int SH>
+__global__ __launch_bounds__(NT) void k_contact_normals(KGrid g, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+                                                        KStats st, int newPos) {
+    __shared__ int s_list[NW][LR_MAX];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int *list = s_list[w];
+    const int count = N.clist[0];
+    const long long nn = g.nstride;
+    const double *px = newPos ? P.posn[0] : P.f[MPF_POS_X];
+    const double *py = newPos ? P.posn[1] : P.f[MPF_POS_Y];
+    const double *pz = DIM == 3 ? (newPos ? P.posn[2] : P.f[MPF_POS_Z]) : nullptr;
+    const bool logistic = g.normals == 6;
+    for (int e = blockIdx.x * NW + w; e < count; e += gridDim.x * NW) {
+        const long long n = N.clist[1 + e];
+        const int i = (int)(n % g.nnx), j = (int)((n / g.nnx) % g.nny), k = DIM == 3 ? (int)(n / ((long long)g.nnx * g.nny)) : 0;
+        const double nd[3] = {g.xmin + (double)i * g.dx, g.ymin + (double)j * g.dy, DIM == 3 ? g.zmin + (double)k * g.dz : 0.};
+        int act[MPMGPU_MAX_MATERIALS];
+        int numMats = 0;
+        for (int f = 0; f < g.nfields; f++)
+            if (N.cnt[f * nn + n] > 0) act[numMats++] = f;
+        // the particles seeing the node: cells i-2..i+1 (per axis) of every field
+        int nl = 0;
+        for (int f = 0; f < g.nfields; f++)
+            for (int cz = (DIM == 3 ? k - 2 : 0); cz <= (DIM == 3 ? k + 1 : 0); cz++) {
+                if (cz < 0 || cz >= g.nz) continue;
+                for (int cy = j - 2; cy <= j + 1; cy++) {
+                    if (cy < 0 || cy >= g.ny) continue;
+                    for (int cx = i - 2; cx <= i + 1; cx++) {
+                        if (cx < 0 || cx >= g.nx) continue;
+                        const int key = cell_key<DIM>(g, f, cx, cy, cz);
+                        const int b = B.cellStart[key], en = B.cellStart[key + 1];
+                        for (int s0 = b; s0 < en; s0 += 64) {
+                            const int s = s0 + lane;
+                            const int p = s < en ? B.order[s] : -1;
+                            const bool in = p >= 0 && lr_member<DIM, SH>(g, P, p, i, j, DIM == 3 ? k : 0);
+                            const unsigned long long m = __ballot(in);
+                            if (in) {
+                                const int idx = nl + __popcll(m & ((1ull << lane) - 1ull));
+                                if (idx < LR_MAX) list[idx] = p;
+                            }
+                            nl += __popcll(m);
+                        }
+                    }
+                }
+            }
+        if (nl > LR_MAX) {
+            if (lane == 0) atomicAdd(st.lrOverflow, 1ull);   // a fatal error, reported after the step
+            nl = LR_MAX;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int miMax = numMats == 2 ? 1 : numMats;
+        for (int mi = 0; mi < miMax; mi++) {
+            const int a = act[mi];
+            double norm[3] = {0., 0., 0.}, linNorm[3] = {0., 0., 0.};
+            const double cell = fmin(g.dx, fmin(g.dy, DIM == 3 ? g.dz : g.dy));     // GetMinCellDimension
+            const double normPenalty = 1.e-7 * (cell * cell);
+            if (DIM == 3) {
+                double mt[10], rhs[4], lambda[4];
+                if (logistic) { lambda[0] = lambda[1] = lambda[2] = normPenalty; lambda[3] = 0.; }
+                else { lambda[0] = lambda[1] = lambda[2] = 0.; lambda[3] = 0.1; }
+                for (int q = 0; q < 10; q++) mt[q] = 0.;
+                for (int q = 0; q < 4; q++) rhs[q] = 0.;
+                for (int t = lane; t < nl; t += 64) {
+                    const int p = list[t];
+                    const double mi_ = mats[P.mat[p]].field == a ? -1. : 1.;
+                    const double x = px[p] - nd[0], y = py[p] - nd[1], z = pz[p] - nd[2];
+                    mt[0] += x * x; mt[1] += x * y; mt[2] += x * z; mt[3] += x;
+                    mt[4] += y * y; mt[5] += y * z; mt[6] += y;
+                    mt[7] += z * z; mt[8] += z; mt[9] += 1.;
+                    rhs[0] += mi_ * x; rhs[1] += mi_ * y; rhs[2] += mi_ * z; rhs[3] += mi_;
+                }
+                for (int q = 0; q < 10; q++) mt[q] = wave_sum(mt[q]);
+                for (int q = 0; q < 4; q++) rhs[q] = wave_sum(rhs[q]);
+                auto mat4 = [&](double M[4][4]) {
+                    M[0][0] = mt[0] + lambda[0]; M[0][1] = mt[1]; M[0][2] = mt[2]; M[0][3] = mt[3];
+                    M[1][0] = mt[1]; M[1][1] = mt[4] + lambda[1]; M[1][2] = mt[5]; M[1][3] = mt[6];
+                    M[2][0] = mt[2]; M[2][1] = mt[5]; M[2][2] = mt[7] + lambda[2]; M[2][3] = mt[8];
+                    M[3][0] = mt[3]; M[3][1] = mt[6]; M[3][2] = mt[8]; M[3][3] = mt[9] + lambda[3];
+                };
+                double M[4][4], beta[4];
+                mat4(M);
+                lr_solve4(M, rhs, beta);
+                if (!logistic) {
+                    const double magi = sqrt(beta[0] * beta[0] + beta[1] * beta[1] + beta[2] * beta[2]);
+                    for (int c = 0; c < 3; c++) norm[c] = beta[c] * (1. / magi);
+                } else {
+                    for (int q = 0; q < 4; q++) beta[q] *= 2.;
+                    double prevNorm[3] = {beta[0], beta[1], beta[2]};
+                    double nmag = sqrt(dot3(prevNorm, prevNorm));
+                    for (int c = 0; c < 3; c++) prevNorm[c] *= 1. / nmag;
+                    for (int c = 0; c < 3; c++) linNorm[c] = prevNorm[c];
+                    double delta = 1.;
+                    int iter = 0;
+                    while (delta > 1.e-5 && iter < 15) {
+                        for (int q = 0; q < 10; q++) mt[q] = 0.;
+                        for (int q = 0; q < 4; q++) rhs[q] = 0.;
+                        for (int t = lane; t < nl; t += 64) {
+                            const int p = list[t];
+                            const double mi_ = mats[P.mat[p]].field == a ? -1. : 1.;
+                            const double x = px[p] - nd[0], y = py[p] - nd[1], z = pz[p] - nd[2];
+                            const double dotp = x * beta[0] + y * beta[1] + z * beta[2] + beta[3];
+                            const double arg = 1. + exp(-dotp);
+                            const double phi = 2. * (arg - 1.) / (arg * arg);
+                            const double phi2 = phi * phi;
+                            mt[0] += phi2 * x * x; mt[1] += phi2 * x * y; mt[2] += phi2 * x * z; mt[3] += phi2 * x;
+                            mt[4] += phi2 * y * y; mt[5] += phi2 * y * z; mt[6] += phi2 * y;
+                            mt[7] += phi2 * z * z; mt[8] += phi2 * z; mt[9] += phi2;
+                            const double mifi = (mi_ - 2. / arg + 1.) * phi;
+                            rhs[0] += mifi * x; rhs[1] += mifi * y; rhs[2] += mifi * z; rhs[3] += mifi;
+                        }
+                        for (int q = 0; q < 10; q++) mt[q] = wave_sum(mt[q]);
+                        // the penalty term starts the right-hand side (MachineLearning.cpp:327-328)
+                        for (int q = 0; q < 4; q++) rhs[q] = -lambda[q] * beta[q] + wave_sum(rhs[q]);
+                        mat4(M);
+                        double dbeta[4];
+                        lr_solve4(M, rhs, dbeta);
+                        for (int q = 0; q < 4; q++) beta[q] += dbeta[q];
+                        for (int c = 0; c < 3; c++) norm[c] = beta[c];
+                        nmag = sqrt(dot3(norm, norm));
+                        for (int c = 0; c < 3; c++) norm[c] *= 1. / nmag;
+                        delta = 1. - dot3(norm, prevNorm);
+                        iter++;
+                        for (int c = 0; c < 3; c++) prevNorm[c] = norm[c];
+                    }
+                }
+            } else {
+                double mt[6], rhs[3];
+                double lambda[3] = {0., 0., 0.1};
+                if (logistic) { lambda[0] = normPenalty; lambda[1] = normPenalty; lambda[2] = 0.; }
+                for (int q = 0; q < 6; q++) mt[q] = 0.;
+                for (int q = 0; q < 3; q++) rhs[q] = 0.;
+                for (int t = lane; t < nl; t += 64) {
+                    const int p = list[t];
+                    const double mi_ = mats[P.mat[p]].field == a ? -1. : 1.;
+                    const double x = px[p] - nd[0], y = py[p] - nd[1];
+                    mt[0] += x * x; mt[1] += x * y; mt[2] += x;
+                    mt[3] += y * y; mt[4] += y; mt[5] += 1.;
+                    rhs[0] += mi_ * x; rhs[1] += mi_ * y; rhs[2] += mi_;
+                }
+                for (int q = 0; q < 6; q++) mt[q] = wave_sum(mt[q]);
+                for (int q = 0; q < 3; q++) rhs[q] = wave_sum(rhs[q]);
+                double beta[3];
+                double S[6] = {mt[0] + lambda[0], mt[1], mt[2], mt[3] + lambda[1], mt[4], mt[5] + lambda[2]};
+                lr_solve3(S, rhs, beta);
+                if (!logistic) {
+                    norm[0] = beta[0]; norm[1] = beta[1];
+                    const double magi = sqrt(dot3(norm, norm));
+                    for (int c = 0; c < 3; c++) norm[c] *= 1. / magi;
+                } else {
+                    for (int q = 0; q < 3; q++) beta[q] *= 2.;
+                    double prevNorm[3] = {beta[0], beta[1], beta[2]};
+                    double nmag = sqrt(beta[0] * beta[0] + beta[1] * beta[1]);
+                    prevNorm[0] /= nmag; prevNorm[1] /= nmag;
+                    linNorm[0] = prevNorm[0]; linNorm[1] = prevNorm[1];
+                    double delta = 1., nv[3] = {0., 0., 0.};
+                    int iter = 0;
+                    while (delta > 1.e-5 && iter < 15) {
+                        for (int q = 0; q < 6; q++) mt[q] = 0.;
+                        for (int q = 0; q < 3; q++) rhs[q] = 0.;
+                        for (int t = lane; t < nl; t += 64) {
+                            const int p = list[t];
+                            const double mi_ = mats[P.mat[p]].field == a ? -1. : 1.;
+                            const double x = px[p] - nd[0], y = py[p] - nd[1], z = 1.;
+                            const double arg = 1. + exp(-(x * beta[0] + y * beta[1] + z * beta[2]));
+                            const double phi = 2. * (arg - 1.) / (arg * arg);
+                            const double phi2 = phi * phi;
+                            mt[0] += phi2 * x * x; mt[1] += phi2 * x * y; mt[2] += phi2 * x;
+                            mt[3] += phi2 * y * y; mt[4] += phi2 * y; mt[5] += phi2;
+                            const double mifi = (mi_ - 2. / arg + 1.) * phi;
+                            rhs[0] += mifi * x; rhs[1] += mifi * y; rhs[2] += mifi;
+                        }
+                        for (int q = 0; q < 6; q++) mt[q] = wave_sum(mt[q]);
+                        for (int q = 0; q < 3; q++) rhs[q] = -lambda[q] * beta[q] + wave_sum(rhs[q]);
+                        double S2[6] = {mt[0] + lambda[0], mt[1], mt[2], mt[3] + lambda[1], mt[4], mt[5] + lambda[2]};
+                        double dbeta[3];
+                        lr_solve3(S2, rhs, dbeta);
+                        for (int q = 0; q < 3; q++) beta[q] += dbeta[q];
+                        for (int q = 0; q < 3; q++) nv[q] = beta[q];
+                        nmag = sqrt(nv[0] * nv[0] + nv[1] * nv[1]);
+                        nv[0] /= nmag; nv[1] /= nmag;
+                        delta = 1. - nv[0] * prevNorm[0] - nv[1] * prevNorm[1];
+                        iter++;
+                        for (int q = 0; q < 3; q++) prevNorm[q] = nv[q];
+                    }
+                    norm[0] = nv[0]; norm[1] = nv[1]; norm[2] = 0.;
+                }
+            }
+            if (logistic && norm[0] != norm[0])
+                for (int c = 0; c < 3; c++) norm[c] = linNorm[c];
+            // separation from the point cloud (MachineLearning.cpp:195-226, 614-633)
+            double supA = 1.e30, supB = 1.e30;
+            for (int t = lane; t < nl; t += 64) {
+                const int p = list[t];
+                const double xp[3] = {px[p], py[p], DIM == 3 ? pz[p] : 0.};
+                const double r = lr_radius<DIM>(g, P, p, norm);
+                if (mats[P.mat[p]].field == a) supA = fmin(supA, -dot3(norm, xp) - r);
+                else supB = fmin(supB, dot3(norm, xp) - r);
+            }
+            supA = wave_min(supA);
+            supB = wave_min(supB);
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) N.cnorm[(4 * a + c) * nn + n] = norm[c];
+                N.cnorm[(4 * a + 3) * nn + n] = supA + supB;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -2344,6 +2732,25 @@ __global__ void k_particle_loads(KGrid g, KLoads L, int par) {
 
 void launch_particle_loads(const KGrid &g, const KLoads &L, int par, hipStream_t s) {
     hipLaunchKernelGGL(k_particle_loads, dim3(blocks_for(L.nl, NT)), dim3(NT), 0, s, g, L, par);
+}
+
+void launch_contact_normals(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                            const MpmMaterial *mats, const KStats &st, int callType, hipStream_t s) {
+    if (g.normals < 5 || !nd.cnorm) return;
+    if (callType == CONTACT_MASS_MOMENTUM) {
+        if (dim == 3) hipLaunchKernelGGL(k_contact_list<3>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd);
+        else hipLaunchKernelGGL(k_contact_list<2>, dim3(blocks_for(g.nnodes, NT)), dim3(NT), 0, s, g, nd);
+    }
+    const int np = callType == CONTACT_UPDATE_STRAINS_LAST ? 1 : 0;
+    const dim3 grid(4 * g_cus);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) hipLaunchKernelGGL((k_contact_normals<3, MPMGPU_SHAPE_LCPDI>), grid, dim3(NT), 0, s, g, p, b, nd, mats, st, np);
+        else hipLaunchKernelGGL((k_contact_normals<3, MPMGPU_SHAPE_UGIMP>), grid, dim3(NT), 0, s, g, p, b, nd, mats, st, np);
+    } else {
+        if (cp) hipLaunchKernelGGL((k_contact_normals<2, MPMGPU_SHAPE_LCPDI>), grid, dim3(NT), 0, s, g, p, b, nd, mats, st, np);
+        else hipLaunchKernelGGL((k_contact_normals<2, MPMGPU_SHAPE_UGIMP>), grid, dim3(NT), 0, s, g, p, b, nd, mats, st, np);
+    }
 }
 
 void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s) {

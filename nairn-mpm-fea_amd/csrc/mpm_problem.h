@@ -53,6 +53,7 @@ struct MpmProblem {
     // <MultiMaterialMode> contact settings
     bool mmRequested = false;
     bool normalsGiven = false;
+    double polarAngle = 0., azimuthAngle = 0.;    // <MultiMaterialMode Polar= Azimuth=> (degrees)
     std::string contactLawName;      // <Friction lawname=>
     int contactLawNum = 0;           // <Friction law=> (1-based), <0: automatic law -n-1, 0: none
     std::vector<MaterialInput> autoLaws;

@@ -423,10 +423,19 @@ bool finalize(MpmProblem &p) {
         if (p.run.multiMaterial) {
             // MeshInfo::MaterialOutput (MeshInfo.cpp:1126-1200): logistic regression is the default
             if (!p.normalsGiven) p.run.normalsMethod = 6;
-            if (p.run.normalsMethod > 3) {
-                p.err = "multimaterial contact normals other than 0 (MAXG), 1 (MAXV), 2 (AVGG) or 3 (OWNG) "
-                        "are not on this path; set <MultiMaterialMode Normals=...>";
-                return false;
+            if (p.run.normalsMethod == 4) {
+                // MeshInfo::SetContactNormal (MeshInfo.cpp:1211-1227)
+                const double pi = 3.141592653589793;
+                double sinp = 1.0;
+                p.run.contactNormal[2] = 0.;
+                if (p.dim == 3) {
+                    double polar = pi * p.polarAngle / 180.;
+                    p.run.contactNormal[2] = cos(polar);
+                    sinp = sin(polar);
+                }
+                double az = pi * p.azimuthAngle / 180.;
+                p.run.contactNormal[0] = cos(az) * sinp;
+                p.run.contactNormal[1] = sin(az) * sinp;
             }
             int law = -1;
             if (!p.contactLawName.empty()) {

@@ -111,6 +111,7 @@ struct Carve {
 size_t scratch_bytes(const mpmi::GridArrays &a, long long n, bool cpdi) {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     size_t s = up(a.nodeBytes) + up(a.cntBytes) + up(a.partBytes) + up(a.matBytes) + up(a.statBytes) + up(a.binBytes);
+    s += up(a.normBytes) + up(a.listBytes);
     s += up(CHK_WORDS * sizeof(unsigned long long));
     s += up(3 * (size_t)n * sizeof(double)) + up(4 * (size_t)n * sizeof(int));
     if (cpdi) s += up(9 * (size_t)n * sizeof(double));
@@ -157,7 +158,9 @@ int mpmgpu_solve_static(const MpmGridDesc *grid, const MpmMaterial *mats, int32_
     double *posB = cv.take<double>(3 * (size_t)n * sizeof(double));
     int *irows = cv.take<int>(4 * (size_t)n * sizeof(int));
     double *cbuf = cpdi ? cv.take<double>(9 * (size_t)n * sizeof(double)) : nullptr;
-    mpmi::assign_grid_arrays(&c, a, nodeMem, cntMem, partMem, bbuf, sbuf);
+    double *normMem = a.normBytes ? cv.take<double>(a.normBytes) : nullptr;
+    int *listMem = a.listBytes ? cv.take<int>(a.listBytes) : nullptr;
+    mpmi::assign_grid_arrays(&c, a, nodeMem, cntMem, partMem, bbuf, sbuf, normMem, listMem);
     c.sbuf = nullptr;
     c.bbuf = nullptr;                     // nothing here is the context's to free
     const long long words = (long long)(cv.used / sizeof(unsigned long long));
@@ -232,6 +235,9 @@ int mpmgpu_solve_static(const MpmGridDesc *grid, const MpmMaterial *mats, int32_
             hipLaunchKernelGGL(k_copy_rows, dim3(1024), dim3(256), 0, s, c.posB[k], c.posA[k], (long long)n);
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return hipfail("position copy", e);
     }
+    if (st[6])
+        return fail_global("a contact node is seen by more than 1024 particles, more than the regression normals' "
+                           "list holds");
     if (cpdi && st[2])
         return fail_global("A CPDI particle domain node has left the grid, or a particle domain reaches beyond the "
                            "neighbouring elements (" + std::to_string(st[2]) + " particle passes)");

@@ -278,12 +278,15 @@ struct Reader {
                 p.normalsGiven = false;
                 if (k.attr("Normals")) {
                     double v = num(k.attr("Normals"));
+                    if (!(v < 6.5)) return fail("Normals attribute on MultiMaterialMode must be 0 to 6.");
+                    // the reference's ranges, in which a value below -0.5 falls into 1
                     int m = v < 0.5 && v > -0.5 ? 0 : v < 1.5 ? 1 : v < 2.5 ? 2 : v < 3.5 ? 3 : v < 4.5 ? 4 : v < 5.5 ? 5 : 6;
                     p.run.normalsMethod = m;
                     p.normalsGiven = true;
                 }
-                if (k.attr("RigidBias") || k.attr("Polar") || k.attr("Azimuth"))
-                    return fail("rigid-material and specified-normal contact settings are not on this path");
+                // RigidBias only changes contact with rigid materials, which this path does not have
+                if (k.attr("Polar")) p.polarAngle = num(k.attr("Polar"));
+                if (k.attr("Azimuth")) p.azimuthAngle = num(k.attr("Azimuth"));
                 for (auto &c : k.kids) {
                     if (c->name == "ContactPosition") return fail("<ContactPosition> (contact by positions) is not on this path");
                     if (c->name != "Friction") continue;

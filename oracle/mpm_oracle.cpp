@@ -733,6 +733,7 @@ struct Oracle {
                 for (int f = 0; f < nf; f++) numberMaterials += npts[fnode(f, n)] >= 1 ? 1 : 0;
                 if (numberMaterials > 1) contactNodes.push_back(n);
             }
+            if (run.normalsMethod >= 5) contact_lists();
         }
         material_contact(CALL_MASS_MOMENTUM);
         // ContactAndMomentaBCs(MASS_MOMENTUM_CALL) (PostExtrapolationTask.cpp:153-159): the
@@ -820,7 +821,9 @@ struct Oracle {
         int style = coeff < 0. ? 0 : (coeff == 0. ? 1 : 2);            // STICK, FRICTIONLESS, FRICTIONAL
         if (coeffStatic > 0. && style == 1) style = 2;
         if (style == 0) return true;
-        if (delFi != nullptr) {
+        // skipped for the regression normals, whose separation comes from the particles
+        // (CoulombFriction.cpp:162)
+        if (delFi != nullptr && run.normalsMethod < 5) {
             double fn = dot(delFi, norm);
             deltaDotn += deltime * (dotn - 0.5 * fn * deltime) / mred;
         }
@@ -853,6 +856,265 @@ struct Oracle {
         return true;
     }
 
+    // ---- regression contact normals (normals methods 5 and 6) ----------------------------
+    // MaterialContactNode::GetMaterialContactData (MaterialContactNode.cpp:253-350), serial form:
+    // for each contact node, the particles whose shape-function node list holds it, in mpm[]
+    // order (start-of-step shape functions); built at the mass-momentum call, used by all three
+    std::vector<std::vector<long>> contactLists;
+    void contact_lists() {
+        contactLists.assign(contactNodes.size(), std::vector<long>());
+        std::vector<long> slot(g.nnodes, -1);
+        for (size_t c = 0; c < contactNodes.size(); c++) slot[contactNodes[c]] = (long)c;
+        for (long p = 0; p < np; p++) {
+            ShapeOut o;
+            shape(p, false, o);
+            for (int i = 0; i < o.n; i++)
+                if (slot[o.node[i]] >= 0) contactLists[slot[o.node[i]]].push_back(p);
+        }
+    }
+
+    // Matrix4::cofactor / determinant / Inverse / Times (Matrix4.cpp:96-103,133-140,206-231): the
+    // minor's determinant in Matrix3's 3D form (Matrix3.cpp:1195-1202)
+    static double cofactor4(const double m[4][4], int row, int col) {
+        double c[3][3];
+        int crow = 0;
+        for (int i = 0; i < 4; i++) {
+            if (i == row) continue;
+            int ccol = 0;
+            for (int j = 0; j < 4; j++) {
+                if (j == col) continue;
+                c[crow][ccol++] = m[i][j];
+            }
+            crow++;
+        }
+        return c[0][0] * (c[1][1] * c[2][2] - c[2][1] * c[1][2]) - c[1][0] * (c[0][1] * c[2][2] - c[2][1] * c[0][2]) +
+               c[2][0] * (c[0][1] * c[1][2] - c[1][1] * c[0][2]);
+    }
+    static void solve4(const double m[4][4], const double *rhs, double *out) {
+        double det = m[0][0] * cofactor4(m, 0, 0) - m[1][0] * cofactor4(m, 1, 0) + m[2][0] * cofactor4(m, 2, 0) -
+                     m[3][0] * cofactor4(m, 3, 0);
+        double f = 1. / det;
+        double inv[4][4];
+        for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) inv[r][c] = ((r + c) & 1 ? -cofactor4(m, c, r) : cofactor4(m, c, r)) * f;
+        for (int r = 0; r < 4; r++) out[r] = inv[r][0] * rhs[0] + inv[r][1] * rhs[1] + inv[r][2] * rhs[2] + inv[r][3] * rhs[3];
+    }
+    // Matrix3(xx,xy,xz,yy,yz,zz) symmetric 3D, Inverse and Times (Matrix3.cpp:107-121,394-408,446-460)
+    static void solve3(const double s[6], const double *rhs, double *out) {
+        double m[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], s[5]}};
+        double inv[3][3] = {{m[1][1] * m[2][2] - m[2][1] * m[1][2], -(m[0][1] * m[2][2] - m[2][1] * m[0][2]),
+                             m[0][1] * m[1][2] - m[1][1] * m[0][2]},
+                            {-(m[1][0] * m[2][2] - m[2][0] * m[1][2]), m[0][0] * m[2][2] - m[2][0] * m[0][2],
+                             -(m[0][0] * m[1][2] - m[1][0] * m[0][2])},
+                            {m[1][0] * m[2][1] - m[2][0] * m[1][1], -(m[0][0] * m[2][1] - m[2][0] * m[0][1]),
+                             m[0][0] * m[1][1] - m[1][0] * m[0][1]}};
+        double det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+                     m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+        double f = 1. / det;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) inv[r][c] *= f;
+        for (int r = 0; r < 3; r++) out[r] = inv[r][0] * rhs[0] + inv[r][1] * rhs[1] + inv[r][2] * rhs[2];
+    }
+
+    // MatPoint3D::GetDeformedRadius / MatPoint2D::GetDeformedRadius (MatPoint3D.cpp:497-510,
+    // MatPoint2D.cpp:482-497), the ellipse methods: the distance from the particle's center to
+    // the surface of its deformed domain along the unit vector n. In 2D the reference applies
+    // F^-T where 3D applies F^-1.
+    double deformed_radius(long p, const double *n) {
+        Mat3 Fm = get_F(p);
+        long ei, ej, ek;
+        g.elem_ijk(I(MPI_ELEM, p), ei, ej, ek);
+        double psx = (g.xpts[ei + 1] - g.xpts[ei]) * (0.5 * F(MPF_LP_X, p));     // GetParticleSize
+        double psy = (g.ypts[ej + 1] - g.ypts[ej]) * (0.5 * F(MPF_LP_Y, p));
+        if (g.dim == 2) {
+            const double (*f)[3] = Fm.m;
+            double rdet2D = 1. / (f[0][0] * f[1][1] - f[0][1] * f[1][0]);
+            double n0x = (f[1][1] * n[0] - f[1][0] * n[1]) * rdet2D;
+            double n0y = (-f[0][1] * n[0] + f[0][0] * n[1]) * rdet2D;
+            double rd = sqrt(n0x * n0x / (psx * psx) + n0y * n0y / (psy * psy));
+            return 1. / rd;
+        }
+        double psz = (g.zpts[ek + 1] - g.zpts[ek]) * (0.5 * F(MPF_LP_Z, p));
+        const double (*m)[3] = Fm.m;
+        double inv[3][3] = {{m[1][1] * m[2][2] - m[2][1] * m[1][2], -(m[0][1] * m[2][2] - m[2][1] * m[0][2]),
+                             m[0][1] * m[1][2] - m[1][1] * m[0][2]},
+                            {-(m[1][0] * m[2][2] - m[2][0] * m[1][2]), m[0][0] * m[2][2] - m[2][0] * m[0][2],
+                             -(m[0][0] * m[1][2] - m[1][0] * m[0][2])},
+                            {m[1][0] * m[2][1] - m[2][0] * m[1][1], -(m[0][0] * m[2][1] - m[2][0] * m[0][1]),
+                             m[0][0] * m[1][1] - m[1][0] * m[0][1]}};
+        double det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+                     m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+        double f = 1. / det;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) inv[r][c] *= f;
+        double n0[3];
+        for (int r = 0; r < 3; r++) n0[r] = inv[r][0] * n[0] + inv[r][1] * n[1] + inv[r][2] * n[2];
+        double rd = sqrt(n0[0] * n0[0] / (psx * psx) + n0[1] * n0[1] / (psy * psy) + n0[2] * n0[2] / (psz * psz));
+        return 1. / rd;
+    }
+
+    // the separation from the point cloud along norm (MachineLearning.cpp:195-226, 614-633): the
+    // gap between the material-a particles' farthest extent along norm and the others' nearest
+    double cloud_separation(const std::vector<long> &list, const int *mis, const double *norm) {
+        double supportForA = 1.e30, supportForB = 1.e30;
+        for (size_t i = 0; i < list.size(); i++) {
+            if (mis[i] == 0) continue;
+            long p = list[i];
+            double xp[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), g.dim == 3 ? F(MPF_POS_Z, p) : 0.};
+            if (mis[i] < 0) {
+                double t = -dot(norm, xp) - deformed_radius(p, norm);
+                if (t < supportForA) supportForA = t;
+            } else {
+                double t = dot(norm, xp) - deformed_radius(p, norm);
+                if (t < supportForB) supportForB = t;
+            }
+        }
+        return supportForA + supportForB;
+    }
+
+    // CrackVelocityFieldMulti::LinearRegressionNormal and LogisticRegressionNormal
+    // (Nodes/MachineLearning.cpp:58-228, 232-633) for material field a against all other
+    // materials of the node's list (b < 0); returns the unit normal and the separation
+    // (delMats.x), with the reference's penalties (offset 0.1 for the linear fit; 1e-7 cell^2 on
+    // the normal's components and 0 on the offset for the logistic fit), the linear fit's beta
+    // doubled as the logistic start, at most 15 Gauss-Newton iterations until
+    // 1 - cos(change) <= 1e-5, and the linear normal when the iteration ends in NaN
+    void regression_normal(long n, const std::vector<long> &list, int a, bool logistic, double norm[3],
+                           double &deln) {
+        const long nl = (long)list.size();
+        std::vector<int> mis(nl);
+        long ni = n % g.nnx, nj = (n / g.nnx) % g.nny, nk = g.dim == 3 ? n / (g.nnx * g.nny) : 0;
+        double nd[3] = {g.xpts[ni], g.ypts[nj], g.zpts[nk]};
+        for (long i = 0; i < nl; i++) mis[i] = mats[I(MPI_MAT, list[i])].field == a ? -1 : 1;
+        double linNorm[3] = {0., 0., 0.};
+        double cell = std::min(g.d.dx, std::min(g.d.dy, g.dim == 3 ? g.d.dz : g.d.dy));   // GetMinCellDimension
+        double cell2 = cell * cell;
+        if (g.dim == 3) {
+            double mt[10] = {0., 0., 0., 0., 0., 0., 0., 0., 0., 0.}, rhs[4] = {0., 0., 0., 0.}, lambda[4];
+            double normPenalty = 1.e-7 * cell2;
+            if (logistic) { for (int i = 0; i < 3; i++) lambda[i] = normPenalty; lambda[3] = 0.; }
+            else { lambda[0] = lambda[1] = lambda[2] = 0.; lambda[3] = 0.1; }
+            for (long i = 0; i < nl; i++) {
+                long p = list[i];
+                double x = F(MPF_POS_X, p) - nd[0], y = F(MPF_POS_Y, p) - nd[1], z = F(MPF_POS_Z, p) - nd[2];
+                mt[0] += x * x; mt[1] += x * y; mt[2] += x * z; mt[3] += x;
+                mt[4] += y * y; mt[5] += y * z; mt[6] += y;
+                mt[7] += z * z; mt[8] += z; mt[9] += 1.;
+                double mi = (double)mis[i];
+                rhs[0] += mi * x; rhs[1] += mi * y; rhs[2] += mi * z; rhs[3] += mi;
+            }
+            auto mat4 = [&](double out[4][4]) {
+                double m[4][4] = {{mt[0] + lambda[0], mt[1], mt[2], mt[3]},
+                                  {mt[1], mt[4] + lambda[1], mt[5], mt[6]},
+                                  {mt[2], mt[5], mt[7] + lambda[2], mt[8]},
+                                  {mt[3], mt[6], mt[8], mt[9] + lambda[3]}};
+                for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) out[r][c] = m[r][c];
+            };
+            double M[4][4], beta[4];
+            mat4(M);
+            solve4(M, rhs, beta);
+            if (!logistic) {
+                double magi = sqrt(beta[0] * beta[0] + beta[1] * beta[1] + beta[2] * beta[2]);
+                for (int c = 0; c < 3; c++) norm[c] = beta[c] * (1. / magi);
+            } else {
+                for (int i = 0; i < 4; i++) { beta[i] *= 2.; lambda[i] *= 1.; }
+                double prevNorm[3] = {beta[0], beta[1], beta[2]};
+                double nmag = sqrt(dot(prevNorm, prevNorm));
+                for (int c = 0; c < 3; c++) prevNorm[c] *= 1. / nmag;
+                for (int c = 0; c < 3; c++) linNorm[c] = prevNorm[c];
+                double deltaOneMinusCos = 1.;
+                int iter = 0;
+                while (deltaOneMinusCos > 1.e-5 && iter < 15) {
+                    for (int i = 0; i < 10; i++) mt[i] = 0.;
+                    for (int i = 0; i < 4; i++) rhs[i] = -lambda[i] * beta[i];
+                    for (long i = 0; i < nl; i++) {
+                        if (mis[i] == 0) continue;
+                        long p = list[i];
+                        double x = F(MPF_POS_X, p) - nd[0], y = F(MPF_POS_Y, p) - nd[1], z = F(MPF_POS_Z, p) - nd[2];
+                        double dotp = x * beta[0] + y * beta[1] + z * beta[2] + beta[3];
+                        double arg = 1. + exp(-dotp);
+                        double phi = 2. * (arg - 1.) / (arg * arg);
+                        double phi2 = phi * phi;
+                        mt[0] += phi2 * x * x; mt[1] += phi2 * x * y; mt[2] += phi2 * x * z; mt[3] += phi2 * x;
+                        mt[4] += phi2 * y * y; mt[5] += phi2 * y * z; mt[6] += phi2 * y;
+                        mt[7] += phi2 * z * z; mt[8] += phi2 * z; mt[9] += phi2;
+                        double mifi = ((double)mis[i] - 2. / arg + 1.) * phi;
+                        rhs[0] += mifi * x; rhs[1] += mifi * y; rhs[2] += mifi * z; rhs[3] += mifi;
+                    }
+                    mat4(M);
+                    double dbeta[4];
+                    solve4(M, rhs, dbeta);
+                    for (int i = 0; i < 4; i++) beta[i] += dbeta[i];
+                    for (int c = 0; c < 3; c++) norm[c] = beta[c];
+                    nmag = sqrt(dot(norm, norm));
+                    for (int c = 0; c < 3; c++) norm[c] *= 1. / nmag;
+                    deltaOneMinusCos = 1. - dot(norm, prevNorm);
+                    iter++;
+                    for (int c = 0; c < 3; c++) prevNorm[c] = norm[c];
+                }
+            }
+        } else {
+            double mt[6] = {0., 0., 0., 0., 0., 0.}, rhs[3] = {0., 0., 0.};
+            double lambda[3] = {0., 0., 0.1};
+            if (logistic) { double np2 = 1.e-7 * cell2; lambda[0] = np2; lambda[1] = np2; lambda[2] = 0.; }
+            for (long i = 0; i < nl; i++) {
+                long p = list[i];
+                double x = F(MPF_POS_X, p) - nd[0], y = F(MPF_POS_Y, p) - nd[1];
+                mt[0] += x * x; mt[1] += x * y; mt[2] += x;
+                mt[3] += y * y; mt[4] += y; mt[5] += 1.;
+                double mi = (double)mis[i];
+                if (logistic) { rhs[0] += x * mi; rhs[1] += y * mi; rhs[2] += 1. * mi; }
+                else { rhs[0] += mi * x; rhs[1] += mi * y; rhs[2] += mi; }
+            }
+            double beta[3];
+            double S[6] = {mt[0] + lambda[0], mt[1], mt[2], mt[3] + lambda[1], mt[4], mt[5] + lambda[2]};
+            solve3(S, rhs, beta);
+            if (!logistic) {
+                norm[0] = beta[0]; norm[1] = beta[1]; norm[2] = 0.;
+                double magi = sqrt(dot(norm, norm));
+                for (int c = 0; c < 3; c++) norm[c] *= 1. / magi;
+            } else {
+                for (int i = 0; i < 3; i++) { beta[i] *= 2.; lambda[i] *= 1.; }
+                double prevNorm[3] = {beta[0], beta[1], beta[2]};
+                double nmag = sqrt(beta[0] * beta[0] + beta[1] * beta[1]);
+                prevNorm[0] /= nmag; prevNorm[1] /= nmag;
+                linNorm[0] = prevNorm[0]; linNorm[1] = prevNorm[1]; linNorm[2] = 0.;
+                double deltaOneMinusCos = 1.;
+                int iter = 0;
+                double nv[3] = {0., 0., 0.};
+                while (deltaOneMinusCos > 1.e-5 && iter < 15) {
+                    for (int i = 0; i < 6; i++) mt[i] = 0.;
+                    rhs[0] = -lambda[0] * beta[0]; rhs[1] = -lambda[1] * beta[1]; rhs[2] = -lambda[2] * beta[2];
+                    for (long i = 0; i < nl; i++) {
+                        if (mis[i] == 0) continue;
+                        long p = list[i];
+                        double x = F(MPF_POS_X, p) - nd[0], y = F(MPF_POS_Y, p) - nd[1], z = 1.;
+                        double arg = 1. + exp(-(x * beta[0] + y * beta[1] + z * beta[2]));
+                        double phi = 2. * (arg - 1.) / (arg * arg);
+                        double phi2 = phi * phi;
+                        mt[0] += phi2 * x * x; mt[1] += phi2 * x * y; mt[2] += phi2 * x;
+                        mt[3] += phi2 * y * y; mt[4] += phi2 * y; mt[5] += phi2;
+                        double mifi = ((double)mis[i] - 2. / arg + 1.) * phi;
+                        rhs[0] += mifi * x; rhs[1] += mifi * y; rhs[2] += mifi;
+                    }
+                    double S2[6] = {mt[0] + lambda[0], mt[1], mt[2], mt[3] + lambda[1], mt[4], mt[5] + lambda[2]};
+                    double dbeta[3];
+                    solve3(S2, rhs, dbeta);
+                    for (int i = 0; i < 3; i++) beta[i] += dbeta[i];
+                    for (int i = 0; i < 3; i++) nv[i] = beta[i];
+                    nmag = sqrt(nv[0] * nv[0] + nv[1] * nv[1]);
+                    nv[0] /= nmag; nv[1] /= nmag;
+                    deltaOneMinusCos = 1. - nv[0] * prevNorm[0] - nv[1] * prevNorm[1];
+                    iter++;
+                    for (int i = 0; i < 3; i++) prevNorm[i] = nv[i];
+                }
+                norm[0] = nv[0]; norm[1] = nv[1]; norm[2] = 0.;
+            }
+        }
+        if (logistic && norm[0] != norm[0]) for (int c = 0; c < 3; c++) norm[c] = linNorm[c];
+        deln = cloud_separation(list, mis.data(), norm);
+    }
+
     // MaterialContactNode::ContactOnKnownNodes (MaterialContactNode.cpp:196-232) ->
     // CrackVelocityFieldMulti::MaterialContactOnCVF / MaterialContactOnCVFLumped
     // (CrackVelocityFieldMulti.cpp:302-671) for non-rigid materials, contact by displacements,
@@ -864,7 +1126,8 @@ struct Oracle {
         bool postUpdate = callType == CALL_UPDATE_MOMENTUM;
         double deltime = run.timestep;
         int normals = run.normalsMethod;
-        for (long n : contactNodes) {
+        for (size_t ci = 0; ci < contactNodes.size(); ci++) {
+            const long n = contactNodes[ci];
             int act[MPMGPU_MAX_MATERIALS];
             int numMats = 0;
             double Pc[3] = {0., 0., 0.}, Mc = 0.;
@@ -930,13 +1193,18 @@ struct Oracle {
                         double magi = dot(norm, norm);
                         double sc = 1. / sqrt(magi);
                         for (int c = 0; c < 3; c++) norm[c] *= sc;
-                    } else {                                              // EACH_MATERIALS_MASS_GRADIENT
+                    } else if (normals == 3) {                            // EACH_MATERIALS_MASS_GRADIENT
                         for (int c = 0; c < 3; c++) norm[c] = gradi[c];
                         double sc = 1. / sqrt(dot(norm, norm));
                         for (int c = 0; c < 3; c++) norm[c] *= sc;
+                    } else if (normals == 4) {                            // SPECIFIED_NORMAL
+                        for (int c = 0; c < 3; c++) norm[c] = run.contactNormal[c];
+                    } else {                                              // LINEAR/LOGISTIC_REGRESSION
+                        regression_normal(n, contactLists[ci], i, normals == 6, norm, deln);
                     }
                     if (norm[0] != norm[0] || norm[1] != norm[1] || norm[2] != norm[2]) continue;
                     dotn = dot(delPi, norm);
+                    const bool hasDeln = normals >= 5;     // the regression found the separation
                     double dispa[3], sa = 1. / massi;
                     for (int c = 0; c < 3; c++) dispa[c] = cdisp[3 * ki + c] * sa;
                     double delta[3], sr = 1. / mred;
@@ -947,7 +1215,7 @@ struct Oracle {
                     for (int c = 0; c < 3; c++) dispb[c] = delta[c];
                     for (int c = 0; c < 3; c++) dispb[c] += dispa[c];
                     // CrackSurfaceContact::MaterialSeparation by displacements (CrackSurfaceContact.cpp:281-287)
-                    deln = dot(dispb, norm) - dot(dispa, norm);
+                    if (!hasDeln) deln = dot(dispb, norm) - dot(dispa, norm);
                     mred *= massi;
                     double delFi[3], *delFiPtr = nullptr;
                     if (postUpdate) {
@@ -1734,6 +2002,23 @@ int oracle_get_contact(void *h, double *vol, double *disp, double *grad) {
     if (grad) memcpy(grad, o->cgrad.data(), 3 * nn * sizeof(double));
     return 0;
 }
+
+/* Regression normal of one node against the current particles (unit tests of normals methods 5
+   and 6): the particles whose start-of-step node lists hold `node` (in mpm[] order), material
+   field a against the rest; returns the list length, the unit normal and the separation. */
+int64_t oracle_regression_normal(void *h, int64_t node, int32_t a, int32_t logistic, double *norm, double *deln) {
+    Oracle *o = (Oracle *)h;
+    o->task_initialize();
+    o->contactNodes.assign(1, (long)node);
+    o->contact_lists();
+    double d = 0.;
+    o->regression_normal((long)node, o->contactLists[0], a, logistic != 0, norm, d);
+    *deln = d;
+    return (int64_t)o->contactLists[0].size();
+}
+
+/* MatPoint3D/MatPoint2D::GetDeformedRadius of particle p along the unit vector n (unit tests) */
+double oracle_deformed_radius(void *h, int64_t p, const double *n) { return ((Oracle *)h)->deformed_radius((long)p, n); }
 
 int oracle_counters(void *h, int64_t *out) {
     Oracle *o = (Oracle *)h;

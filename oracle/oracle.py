@@ -51,6 +51,10 @@ def lib():
         _lib.oracle_get_grid.argtypes = [_P, _PD, _PD, _PD, _PI]
         _lib.oracle_get_contact.argtypes = [_P, _PD, _PD, _PD]
         _lib.oracle_counters.argtypes = [_P, _PL]
+        _lib.oracle_regression_normal.argtypes = [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _PD, _PD]
+        _lib.oracle_regression_normal.restype = ctypes.c_int64
+        _lib.oracle_deformed_radius.argtypes = [_P, ctypes.c_int64, _PD]
+        _lib.oracle_deformed_radius.restype = ctypes.c_double
         _lib.oracle_shape.argtypes = [ctypes.POINTER(nm.GridDesc), ctypes.c_int32, ctypes.c_int64, _PD, _PD,
                                       ctypes.c_int32, _PL, _PD, _PD, _PD, _PD]
         _lib.oracle_constitutive.argtypes = [ctypes.c_int32, ctypes.POINTER(nm.Material), _PD, _PD]
@@ -158,6 +162,19 @@ class Oracle:
         c = np.zeros(3, dtype=np.int64)
         lib().oracle_counters(self._h, c.ctypes.data_as(_PL))
         return {"steps": int(c[0]), "leftGrid": int(c[1]), "nan": int(c[2])}
+
+    def regression_normal(self, node: int, field: int, logistic: bool):
+        """(unit normal, separation, list length) of a regression normal at `node` for material
+        field `field` against the other particles listed there (normals methods 5 and 6)"""
+        nv = np.zeros(3)
+        d = np.zeros(1)
+        k = lib().oracle_regression_normal(self._h, int(node), int(field), int(logistic), _d(nv), _d(d))
+        return nv, float(d[0]), int(k)
+
+    def deformed_radius(self, p: int, n):
+        """distance from particle p's center to its deformed domain's surface along unit vector n"""
+        v = np.ascontiguousarray(n, dtype=np.float64)
+        return float(lib().oracle_deformed_radius(self._h, int(p), _d(v)))
 
     def __del__(self):
         try:
