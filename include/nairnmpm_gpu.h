@@ -258,6 +258,35 @@ typedef struct MpmParticleLoad {
    (Read_MPM/Generators.cpp:1012-1023,1105-1188; MatPtLoadBC::SetBCValue, MatPtLoadBC.cpp:202-206).
    Count and copy as mpmgpu_problem_velocity_bcs. */
 int mpmgpu_problem_particle_loads(const MpmProblem *prob, MpmParticleLoad *loads, int64_t *n);
+/* One traction on one face of one particle (MatPtTractionBC, Boundary_Conditions/MatPtTractionBC.hpp):
+   particle index in mpm[] order (0-based); dir 1 x, 2 y, 3 z (3D), 11 the face's outward normal,
+   12 the face's first edge direction (2D); face 1-4 (2D: bottom, right, top, left) or 1-6 (3D:
+   -y, +x, +y, -x, -z, +z); style 1-4 as MpmVelocityBC; value in internal stress units
+   (g/(mm s^2) = Pa, Pa/s for linear); ftime as MpmVelocityBC; thickness the particle's thickness
+   (MatPoint2D::thickness, used in 2D only). Each step, after the grid forces and RestoreMomenta
+   and before gravity, every traction adds its value at the step's start time times the face
+   weight of GetSurfaceInfo (MatPoint2D.cpp:1138-1300, MatPoint3D.cpp:628-851: hat n times half
+   the face length times the thickness in 2D, a quarter of the face area in 3D) to the nodes of
+   the elements holding the face's corners, weighted by the elements' linear shape functions at
+   the corners, on the particle's material velocity field where that field has points
+   (MatPtTractionBC::AddMPFluxBC, MatPtTractionBC.cpp:64-227, without cracks or ExactTractions).
+   uGIMP and classic particles take the corners of the undeformed particle and the area of the
+   deformed face; lCPDI particles the deformed corners. */
+typedef struct MpmParticleTraction {
+    int64_t particle;
+    int32_t dir;
+    int32_t face;
+    int32_t style;
+    int32_t reserved;            /* 0 */
+    double value;
+    double ftime;
+    double thickness;
+} MpmParticleTraction;
+/* The <TractionBC> conditions of <ParticleBCs> (<BCLine> in 2D, <BCBox> in 3D), in list order:
+   condition by condition in document order, particles in mpm[] order (Read_MPM/Generators.cpp:
+   1105-1202; MatPtLoadBC::SetBCValue, MatPtLoadBC.cpp:202-206: MPa to Pa). Count and copy as
+   mpmgpu_problem_velocity_bcs. */
+int mpmgpu_problem_particle_tractions(const MpmProblem *prob, MpmParticleTraction *tractions, int64_t *n);
 void mpmgpu_problem_destroy(MpmProblem *prob);
 
 /* ---- archive files (ArchiveData, System/ArchiveData.cpp), host only ----------------- */
@@ -313,6 +342,12 @@ int mpmgpu_set_velocity_bcs(MpmCtx *ctx, int64_t n, const MpmVelocityBC *bcs);
    indices are upload indices (the order of mpmgpu_upload_particles); upload the particles first.
    Values are taken at the step's start time, as for the velocity conditions. */
 int mpmgpu_set_particle_loads(MpmCtx *ctx, int64_t n, const MpmParticleLoad *loads);
+/* Traction conditions of a single simulation (not a batch); n = 0 removes them. Particle indices
+   are upload indices; values are taken at the step's start time, as for the loads. A corner that
+   leaves the grid stops the run with the reference's error ("A Traction edge node has left the
+   grid"); so does an lCPDI corner beyond the particle's neighbour cells, which this
+   implementation does not reach. */
+int mpmgpu_set_particle_tractions(MpmCtx *ctx, int64_t n, const MpmParticleTraction *tractions);
 /* Run one hot-path MPMTask's Execute() on the device (MPMGPU_TASK_*), in reference order. */
 int mpmgpu_run_task(MpmCtx *ctx, int32_t task);
 /* Run nsteps whole steps (NairnMPM::MPMStep, NairnMPM.cpp:284-335), fused and graph-replayed. */

@@ -32,8 +32,8 @@ constexpr int IROW_KEY = MPI_NUM_FIELDS, IROW_ORDER = MPI_NUM_FIELDS + 1, IROW_H
               IROW_SHOME = MPI_NUM_FIELDS + 3;
 constexpr int64_t PARTICLE_BYTES = PROWS * sizeof(double) + IROWS * sizeof(int) + sizeof(double);
 // device counters: left grid, NaN positions, lCPDI corner faults, the run stop (KBins::stop, three
-// words), regression-list overflows
-constexpr int STATS_WORDS = 7;
+// words), regression-list overflows, traction corner faults
+constexpr int STATS_WORDS = 8;
 
 struct MpmCtx {
     int dim = 3;
@@ -76,9 +76,12 @@ struct MpmCtx {
     KBcs kbc{};
     char *bcbuf = nullptr;
     double bcTime = 0.;
-    // external particle loads (kld.nl = 0: none) in one device buffer
+    // external particle loads and tractions (kld.nl = 0: none) in one device buffer, built from
+    // the host copies of the two lists
     KLoads kld{};
     char *ldbuf = nullptr;
+    std::vector<MpmParticleLoad> loads;
+    std::vector<MpmParticleTraction> tractions;
     double *tbuf = nullptr;          // two step start time slots (by position parity)
     float lastStepMs = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

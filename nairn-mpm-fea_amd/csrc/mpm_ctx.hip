@@ -54,6 +54,8 @@ static int ensure_time_slots(MpmCtx *c) {
     return 0;
 }
 
+static int build_particle_bcs(MpmCtx *c);
+
 // the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108); with
 // profiling on, an event is recorded between kernel groups
 void mpmi::enqueue_step(MpmCtx *c) {
@@ -427,6 +429,19 @@ int mpmgpu_problem_particle_loads(const MpmProblem *pc, MpmParticleLoad *loads, 
     return 0;
 }
 
+int mpmgpu_problem_particle_tractions(const MpmProblem *pc, MpmParticleTraction *tr, int64_t *n) {
+    MpmProblem *p = const_cast<MpmProblem *>(pc);
+    if (!n) return fail_global("null count");
+    if (!mpmsetup::finalize(*p)) return fail_global(p->err);
+    const int64_t have = (int64_t)p->tractions.size();
+    if (tr) {
+        const int64_t k = std::min(*n, have);
+        for (int64_t i = 0; i < k; i++) tr[i] = p->tractions[i];
+    }
+    *n = have;
+    return 0;
+}
+
 void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 
 // ---- device context --------------------------------------------------------------------
@@ -482,6 +497,7 @@ void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, i
     c->ks.cpdiErr = sbuf + 2;
     c->kb.stop = sbuf + 3;
     c->ks.lrOverflow = sbuf + 6;
+    c->ks.tracErr = sbuf + 7;
     c->kn.cnorm = a.normBytes ? normMem : nullptr;
     c->kn.clist = a.listBytes ? listMem : nullptr;
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
@@ -848,8 +864,12 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
         HIPCHECK(c, hipMemcpyAsync(c->dinst, c->inst.data(), c->inst.size() * sizeof(KGrid), hipMemcpyHostToDevice,
                                    c->stream));
     }
-    // loads name particles of the previous upload
-    if (c->kld.nl && mpmgpu_set_particle_loads(c, 0, nullptr)) return 1;
+    // loads and tractions name particles of the previous upload
+    if (c->kld.nl) {
+        c->loads.clear();
+        c->tractions.clear();
+        if (build_particle_bcs(c)) return 1;
+    }
     if (mpmi::place_particles(c, n)) return 1;
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, dfields, MPF_NUM_FIELDS * nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, ifields, MPI_NUM_FIELDS * nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -927,6 +947,128 @@ int mpmgpu_set_velocity_bcs(MpmCtx *c, int64_t n, const MpmVelocityBC *bcs) {
     return put_bc_time(c);
 }
 
+}  // extern "C"
+
+// The device buffer of the loads and tractions (KLoads) from the context's two host lists. The
+// loaded particles (with a load, a traction or both) are numbered in upload order; each keeps its
+// loads, and its tractions, in list order.
+static int build_particle_bcs(MpmCtx *c) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    destroy_graphs(c);
+    if (c->ldbuf) { HIPCHECK(c, hipFree(c->ldbuf)); c->ldbuf = nullptr; }
+    c->kld = KLoads{};
+    const size_t ne = c->loads.size(), nte = c->tractions.size();
+    if (ne == 0 && nte == 0) return 0;
+    const long long np = c->kp.n;
+    auto by_particle = [](size_t n, auto part) {
+        std::vector<size_t> ord(n);
+        for (size_t i = 0; i < n; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return part(a) < part(b); });
+        return ord;
+    };
+    const std::vector<size_t> lo = by_particle(ne, [&](size_t i) { return c->loads[i].particle; });
+    const std::vector<size_t> to = by_particle(nte, [&](size_t i) { return c->tractions[i].particle; });
+    std::vector<int> map((size_t)np, -1), tof((size_t)np, -1);
+    for (const MpmParticleLoad &l : c->loads) map[(size_t)l.particle] = 0;
+    for (const MpmParticleTraction &t : c->tractions) map[(size_t)t.particle] = 0;
+    int nl = 0, nt = 0;
+    for (long long i = 0; i < np; i++)
+        if (map[(size_t)i] == 0) map[(size_t)i] = nl++;
+    for (size_t k = 0; k < nte; k++) {
+        const int64_t pi = c->tractions[to[k]].particle;
+        if (tof[(size_t)pi] < 0) tof[(size_t)pi] = nt++;
+    }
+    // loads: entries grouped by loaded particle
+    std::vector<int> start((size_t)nl + 1, 0), dir(ne), style(ne);
+    std::vector<double> value(ne), ftime(ne);
+    for (size_t k = 0; k < ne; k++) {
+        const MpmParticleLoad &l = c->loads[lo[k]];
+        start[(size_t)map[(size_t)l.particle] + 1]++;
+        dir[k] = l.dir - 1;
+        style[k] = l.style;
+        value[k] = l.value;
+        ftime[k] = l.ftime;
+    }
+    for (int q = 0; q < nl; q++) start[(size_t)q + 1] += start[(size_t)q];
+    // tractions: entries grouped by traction table, tables in upload order of their particles
+    std::vector<int> tslot((size_t)nl, -1), tstart((size_t)nt + 1, 0), tdir(nte), tface(nte), tstyle(nte);
+    std::vector<double> tvalue(nte), tftime(nte), tthick((size_t)nt, 0.);
+    for (size_t k = 0; k < nte; k++) {
+        const MpmParticleTraction &t = c->tractions[to[k]];
+        const int ts = tof[(size_t)t.particle];
+        if (tstart[(size_t)ts + 1]++ == 0) tthick[(size_t)ts] = t.thickness;
+        tslot[(size_t)map[(size_t)t.particle]] = ts;
+        tdir[k] = t.dir;
+        tface[k] = t.face;
+        tstyle[k] = t.style;
+        tvalue[k] = t.value;
+        tftime[k] = t.ftime;
+    }
+    for (int t = 0; t < nt; t++) tstart[(size_t)t + 1] += tstart[(size_t)t];
+    if (ensure_time_slots(c)) return 1;
+    // doubles: value, ftime, fext, tvalue, tftime, tthick, ttab; then ints: map, start, dir, style,
+    // two tile flag slots, tslot, tstart, tdir, tface, tstyle
+    const size_t ntall = (size_t)c->kb.ntall;
+    const size_t ntab = (size_t)3 * nt * TRAC_NODES;
+    const size_t nd = 2 * ne + 3 * (size_t)nl + 2 * nte + (size_t)nt + ntab;
+    const size_t ni = (size_t)np + (size_t)nl + 1 + 2 * ne + 2 * ntall + (size_t)nl + (size_t)nt + 1 + 3 * nte;
+    const size_t bytes = nd * sizeof(double) + ni * sizeof(int);
+    HIPCHECK(c, hipMalloc(&c->ldbuf, bytes));
+    std::vector<char> h(bytes, 0);
+    double *hd = (double *)h.data();
+    int *hi = (int *)(hd + nd);
+    auto putd = [&](size_t &at, const std::vector<double> &v) { memcpy(hd + at, v.data(), v.size() * sizeof(double)); at += v.size(); };
+    auto puti = [&](size_t &at, const std::vector<int> &v) { memcpy(hi + at, v.data(), v.size() * sizeof(int)); at += v.size(); };
+    size_t ad = 0, ai = 0;
+    putd(ad, value); putd(ad, ftime);
+    const size_t aFext = ad; ad += 3 * (size_t)nl;
+    const size_t aTv = ad; putd(ad, tvalue);
+    const size_t aTf = ad; putd(ad, tftime);
+    const size_t aTt = ad; putd(ad, tthick);
+    const size_t aTab = ad;
+    puti(ai, map);
+    const size_t aStart = ai; puti(ai, start);
+    const size_t aDir = ai; puti(ai, dir);
+    const size_t aStyle = ai; puti(ai, style);
+    const size_t aFlag = ai; ai += 2 * ntall;          // zero from the host buffer
+    const size_t aTslot = ai; puti(ai, tslot);
+    const size_t aTstart = ai; puti(ai, tstart);
+    const size_t aTdir = ai; puti(ai, tdir);
+    const size_t aTface = ai; puti(ai, tface);
+    const size_t aTstyle = ai; puti(ai, tstyle);
+    HIPCHECK(c, hipMemcpy(c->ldbuf, h.data(), bytes, hipMemcpyHostToDevice));
+    double *dd = (double *)c->ldbuf;
+    int *di = (int *)(dd + nd);
+    KLoads &L = c->kld;
+    L.nl = nl;
+    L.value = dd;
+    L.ftime = dd + ne;
+    L.fext = dd + aFext;
+    L.map = di;
+    L.start = di + aStart;
+    L.dir = di + aDir;
+    L.style = di + aStyle;
+    L.tflag[0] = di + aFlag;
+    L.tflag[1] = L.tflag[0] + ntall;
+    L.time = c->tbuf;
+    L.nt = nt;
+    if (nt > 0) {
+        L.tslot = di + aTslot;
+        L.tstart = di + aTstart;
+        L.tdir = di + aTdir;
+        L.tface = di + aTface;
+        L.tstyle = di + aTstyle;
+        L.tvalue = dd + aTv;
+        L.tftime = dd + aTf;
+        L.tthick = dd + aTt;
+        L.ttab = dd + aTab;
+    }
+    L.terr = c->ks.tracErr;
+    return put_bc_time(c);
+}
+
+extern "C" {
+
 int mpmgpu_set_particle_loads(MpmCtx *c, int64_t n, const MpmParticleLoad *loads) {
     if (n < 0 || n > 0x7fffffffLL) return c->fail("bad particle load count");
     if (n > 0 && !c->inst.empty()) return c->fail("particle loads are for single simulations, not batches");
@@ -937,61 +1079,32 @@ int mpmgpu_set_particle_loads(MpmCtx *c, int64_t n, const MpmParticleLoad *loads
         if (loads[i].dir < 1 || loads[i].dir > c->dim) return c->fail("particle load direction must be 1 to the dimension");
         if (loads[i].style < 1 || loads[i].style > 4) return c->fail("load styles other than 1 to 4 are not on this path");
     }
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    destroy_graphs(c);
-    if (c->ldbuf) { HIPCHECK(c, hipFree(c->ldbuf)); c->ldbuf = nullptr; }
-    c->kld = KLoads{};
-    if (n == 0) return 0;
-    // group by particle, list order kept within a particle
-    std::vector<int64_t> ord((size_t)n);
-    for (int64_t i = 0; i < n; i++) ord[(size_t)i] = i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return loads[a].particle < loads[b].particle; });
-    std::vector<int> map((size_t)np, -1), start, dir, style;
-    std::vector<double> value, ftime;
-    for (size_t k = 0; k < ord.size(); k++) {
-        const MpmParticleLoad &l = loads[ord[k]];
-        if (map[(size_t)l.particle] < 0) {
-            map[(size_t)l.particle] = (int)start.size();
-            start.push_back((int)k);
-        }
-        dir.push_back(l.dir - 1);
-        style.push_back(l.style);
-        value.push_back(l.value);
-        ftime.push_back(l.ftime);
+    c->loads.assign(loads, loads + n);
+    return build_particle_bcs(c);
+}
+
+int mpmgpu_set_particle_tractions(MpmCtx *c, int64_t n, const MpmParticleTraction *tr) {
+    if (n < 0 || n > 0x7fffffffLL) return c->fail("bad traction count");
+    if (n > 0 && !c->inst.empty()) return c->fail("tractions are for single simulations, not batches");
+    if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
+    const long long np = c->kp.n;
+    for (int64_t i = 0; i < n; i++) {
+        const MpmParticleTraction &t = tr[i];
+        if (t.particle < 0 || t.particle >= np) return c->fail("traction on a particle that was not uploaded");
+        if (c->dim == 3 && !((t.dir >= 1 && t.dir <= 3) || t.dir == 11))
+            return c->fail("'dir' in <TractionBC> must be 1-3 or 11 for 3D analyses.");
+        if (c->dim == 2 && !(t.dir == 1 || t.dir == 2 || t.dir == 11 || t.dir == 12))
+            return c->fail("'dir' in <LoadBC> or <ConcFluxBC> must be 1 or 2 and in <TractionBC> must be 1, 2, 11, or 12 for 2D analyses.");
+        if (c->dim == 3 && (t.face < 1 || t.face > 6))
+            return c->fail("'face' in <TractionBC>, <ConcFluxBC>, or <HeatFluxBC> element must be 1 to 6 for 3D analyses.");
+        if (c->dim == 2 && (t.face < 1 || t.face > 4))
+            return c->fail("'face' in <TractionBC>, <ConcFluxBC>, or <HeatFluxBC> element must be 1 to 4 for 2D analyses.");
+        if (t.style == 5) return c->fail("Tractions boundary conditions cannot use silent style.");
+        if (t.style < 1 || t.style > 4) return c->fail("traction styles other than 1 to 4 are not on this path");
+        if (c->dim == 2 && !(t.thickness > 0.)) return c->fail("a 2D traction needs the particle's thickness");
     }
-    start.push_back((int)n);
-    const size_t nl = start.size() - 1, ne = (size_t)n;
-    if (ensure_time_slots(c)) return 1;
-    // doubles (value, ftime, fext), then ints (map, start, dir, style, two tile flag slots)
-    const size_t ntall = (size_t)c->kb.ntall;
-    const size_t nd = 2 * ne + 3 * nl, ni = (size_t)np + nl + 1 + 2 * ne + 2 * ntall;
-    const size_t bytes = nd * sizeof(double) + ni * sizeof(int);
-    HIPCHECK(c, hipMalloc(&c->ldbuf, bytes));
-    std::vector<char> h(bytes, 0);
-    double *hd = (double *)h.data();
-    memcpy(hd, value.data(), ne * sizeof(double));
-    memcpy(hd + ne, ftime.data(), ne * sizeof(double));
-    int *hi = (int *)(hd + nd);
-    memcpy(hi, map.data(), (size_t)np * sizeof(int));
-    memcpy(hi + np, start.data(), (nl + 1) * sizeof(int));
-    memcpy(hi + np + nl + 1, dir.data(), ne * sizeof(int));
-    memcpy(hi + np + nl + 1 + ne, style.data(), ne * sizeof(int));
-    HIPCHECK(c, hipMemcpy(c->ldbuf, h.data(), bytes, hipMemcpyHostToDevice));
-    double *dd = (double *)c->ldbuf;
-    int *di = (int *)(dd + nd);
-    KLoads &L = c->kld;
-    L.nl = (int)nl;
-    L.value = dd;
-    L.ftime = dd + ne;
-    L.fext = dd + 2 * ne;
-    L.map = di;
-    L.start = di + np;
-    L.dir = di + np + nl + 1;
-    L.style = di + np + nl + 1 + ne;
-    L.tflag[0] = di + np + nl + 1 + 2 * ne;     // zero from the host buffer
-    L.tflag[1] = L.tflag[0] + ntall;
-    L.time = c->tbuf;
-    return put_bc_time(c);
+    c->tractions.assign(tr, tr + n);
+    return build_particle_bcs(c);
 }
 
 int mpmgpu_run_task(MpmCtx *c, int32_t task) {
@@ -1178,6 +1291,12 @@ static int check_cpdi(MpmCtx *c) {
         HIPCHECK(c, hipMemcpy(&o, c->ks.lrOverflow, sizeof(o), hipMemcpyDeviceToHost));
         if (o) return c->fail("a contact node is seen by more than 1024 particles, more than the regression normals' "
                               "list holds (" + std::to_string(o) + " node passes)");
+    }
+    if (c->kld.nt) {
+        unsigned long long t = 0;
+        HIPCHECK(c, hipMemcpy(&t, c->ks.tracErr, sizeof(t), hipMemcpyDeviceToHost));
+        if (t) return c->fail("A Traction edge node has left the grid, or an lCPDI traction corner lies beyond the "
+                              "particle's neighbouring elements (" + std::to_string(t) + " corners)");
     }
     if (c->run.shape != MPMGPU_SHAPE_LCPDI) return 0;
     unsigned long long e = 0;

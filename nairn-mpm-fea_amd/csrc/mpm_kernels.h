@@ -125,6 +125,7 @@ struct KStats {
     unsigned long long *nanCount;
     unsigned long long *cpdiErr;   // lCPDI corners outside the grid or beyond the neighbour cells
     unsigned long long *lrOverflow;  // contact nodes seen by more particles than a regression list holds
+    unsigned long long *tracErr;     // traction corners outside the grid or beyond the neighbour cells
 };
 
 enum { NQ_MASS = 4, NQ_FORCE = 3, NQ_MOM = 3, NQ_CONTACT = 7, NQMAX = 7 };
@@ -212,8 +213,15 @@ void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, 
 // External particle loads of one simulation: map[upload index] is -1 or the loaded particle's
 // slot q, whose loads are entries start[q]..start[q+1]-1 in list order; fext holds the step's
 // pFext of each slot (3 doubles), read by the force scatter through the particle's upload index.
+// Particles with external loads or tractions ("loaded" particles, nl of them): map takes a
+// particle's upload index to its loaded index q (-1: neither). Loads: entries [start[q],
+// start[q+1]) give pFext (fext[3q..3q+2], zero without loads). Tractions: tslot[q] is the
+// particle's traction table (-1: none); table t holds entries [tstart[t], tstart[t+1]) and, each
+// step, the nodal traction forces on the particle's 4^DIM candidate nodes (offsets -1..2 from its
+// cell), component-major: ttab[(3t + c) * TRAC_NODES + o], o = ((oz*4 + oy)*4 + ox) (2D: oy*4 + ox).
+#define TRAC_NODES 64
 struct KLoads {
-    int nl;                    // loaded particles (0: no loads)
+    int nl;                    // loaded particles (0: no loads and no tractions)
     const int *map;
     const int *start;
     const int *dir, *style;
@@ -221,6 +229,13 @@ struct KLoads {
     double *fext;
     double *time;              // the two step start time slots shared with KBcs
     int *tflag[2];             // per (field, tile): holds a loaded particle this step (slot par)
+    int nt;                    // traction tables (0: no tractions)
+    const int *tslot;
+    const int *tstart;
+    const int *tdir, *tface, *tstyle;
+    const double *tvalue, *tftime, *tthick;
+    double *ttab;
+    unsigned long long *terr;  // traction corners outside the grid or beyond the neighbour cells
 };
 void launch_particle_loads(const KGrid &g, const KLoads &L, int par, hipStream_t s);
 

@@ -578,9 +578,9 @@ __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
 struct RecMass { double wx, wy, wz, mp, vx, vy, vz; };
 struct RecForce { double wx, wy, wz, dx, dy, dz, s0, s1, s2, s3, s4, s5; };
 struct RecForce2 { double2 x, y, z; double a[6]; };
-// with external particle loads: pFext too
-struct RecForceX : RecForce { double e[3]; };
-struct RecForce2X : RecForce2 { double e[3]; };
+// with external particle loads or tractions: pFext and the particle's traction force at the node
+struct RecForceX : RecForce { double e[3], t[3]; };
+struct RecForce2X : RecForce2 { double e[3], t[3]; };
 
 template <int DIM, int NQ>
 __device__ __forceinline__ void zero_acc(double (*s_acc)[NQ][WaveFoot<DIM>::SIZE]) {
@@ -1107,14 +1107,24 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
     __shared__ double s_cwg[CP ? CH : 1][NCOR][3];
     __shared__ double s_st[CP ? CH : 1][NA];
     __shared__ double s_e[FX ? CH : 1][3];    // pFext of the staged particles
+    __shared__ int s_tt[FX ? CH : 1];         // their traction tables (-1: none)
     __shared__ double s_acc[NW][NQ_FORCE][WaveFoot<DIM>::SIZE];
     using R1 = std::conditional_t<FX, RecForceX, RecForce>;
     using R2 = std::conditional_t<FX, RecForce2X, RecForce2>;
-    // MatPtLoadBC's pFext of particle p (0 when it carries no load)
+    // MatPtLoadBC's pFext of particle p (0 when it carries no load) and its traction table
     auto stage_fext = [&](int slot, int p) {
         const int q = L.map[B.home[p]];
 #pragma unroll
         for (int c = 0; c < 3; c++) s_e[slot][c] = q >= 0 ? L.fext[3 * q + c] : 0.;
+        s_tt[slot] = q >= 0 && L.nt > 0 ? L.tslot[q] : -1;
+    };
+    // the staged particle's traction force at candidate node (ix, iy, iz) (zero without tractions)
+    auto trac = [&](int lq, int ix, int iy, int iz, double t[3]) {
+        const int tt = s_tt[lq];
+        const int o = DIM == 3 ? (iz * 4 + iy) * 4 + ix : iy * 4 + ix;
+        const double *tab = L.ttab + (size_t)3 * (tt < 0 ? 0 : tt) * TRAC_NODES + o;
+#pragma unroll
+        for (int c = 0; c < 3; c++) t[c] = tt >= 0 && c < DIM ? tab[c * TRAC_NODES] : 0.;
     };
     auto tile = [&](const int tall) {      // (field, context tile) pair
         // with loads: the launch with the load term takes the marked tiles, the other the rest
@@ -1201,16 +1211,19 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                         r.z = DIM == 3 ? s_F[lq][DIM - 1][iz] : make_double2(1., 0.);
 #pragma unroll
                         for (int q = 0; q < NA; q++) r.a[q] = s_a[lq][q];
-                        if constexpr (FX) { r.e[0] = s_e[lq][0]; r.e[1] = s_e[lq][1]; r.e[2] = s_e[lq][2]; }
+                        if constexpr (FX) {
+                            r.e[0] = s_e[lq][0]; r.e[1] = s_e[lq][1]; r.e[2] = s_e[lq][2];
+                            trac(lq, ix, iy, iz, r.t);
+                        }
                         return r;
                     },
                     [&](const R2 &r, double *acc, int, int, int) {
                         if constexpr (FX) {
-                            // + N pFext, N the product of the axis factors
+                            // + N pFext, N the product of the axis factors, + the traction force
                             const double w = DIM == 3 ? r.x.x * (r.y.x * r.z.x) : r.x.x * r.y.x;
-                            acc[0] = fma(r.e[0], w, acc[0]);
-                            acc[1] = fma(r.e[1], w, acc[1]);
-                            if (DIM == 3) acc[2] = fma(r.e[2], w, acc[2]);
+                            acc[0] = fma(r.e[0], w, acc[0]) + r.t[0];
+                            acc[1] = fma(r.e[1], w, acc[1]) + r.t[1];
+                            if (DIM == 3) acc[2] = fma(r.e[2], w, acc[2]) + r.t[2];
                         }
                         if (DIM == 3) {
                             const double gx = r.x.y * (r.y.x * r.z.x);
@@ -1242,14 +1255,17 @@ __global__ __launch_bounds__(NT) void k_p2g_force(KGrid g0, KPart P, KBins B, KN
                     r.s0 = s_st[lq][0]; r.s1 = s_st[lq][1]; r.s2 = s_st[lq][2];
                     if constexpr (DIM == 3) { r.s3 = s_st[lq][3]; r.s4 = s_st[lq][4]; r.s5 = s_st[lq][5]; }
                     else { r.s3 = r.s4 = r.s5 = 0.; }
-                    if constexpr (FX) { r.e[0] = s_e[lq][0]; r.e[1] = s_e[lq][1]; r.e[2] = s_e[lq][2]; }
+                    if constexpr (FX) {
+                        r.e[0] = s_e[lq][0]; r.e[1] = s_e[lq][1]; r.e[2] = s_e[lq][2];
+                        trac(lq, ix, iy, iz, r.t);
+                    }
                     return r;
                 },
                 [&](const R1 &r, double *acc, int, int, int) {
-                    if constexpr (FX) {              // + N pFext, N the merged corner weight
-                        acc[0] = fma(r.e[0], r.wx, acc[0]);
-                        acc[1] = fma(r.e[1], r.wx, acc[1]);
-                        if (DIM == 3) acc[2] = fma(r.e[2], r.wx, acc[2]);
+                    if constexpr (FX) {              // + N pFext, N the merged corner weight, + traction
+                        acc[0] = fma(r.e[0], r.wx, acc[0]) + r.t[0];
+                        acc[1] = fma(r.e[1], r.wx, acc[1]) + r.t[1];
+                        if (DIM == 3) acc[2] = fma(r.e[2], r.wx, acc[2]) + r.t[2];
                     }
                     // s: -mp times xx, yy, xy, zz, xz, yz of sigma - P; non-member nodes have zero gradient
                     if (DIM == 3) {
@@ -2473,13 +2489,177 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
     }
 }
 
-// marks the (field, tile) of every loaded particle in slot par, clears slot par ^ 1 for the next step
+// BoundaryCondition::GetNodeNum and BCValue (BoundaryCondition.cpp:72-140): 1 if the condition
+// acts at time t, its value in v
+__device__ __forceinline__ int style_value(int style, double val, double ft, double t, double &v) {
+    v = 0.;
+    switch (style) {
+        case 1: if (t >= ft) { v = val; return 1; } return 0;
+        case 2: if (t >= ft) { v = 0. + val * (t - ft); return 1; } return 0;
+        case 3: v = val * sin(t * ft); return 1;
+        case 4: v = val * cos(t * ft); return 1;
+        default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Traction conditions (MatPtTractionBC::AddMPFluxBC, MatPtTractionBC.cpp:64-227: planar and 3D,
+// no cracks, no ExactTractions). GetSurfaceInfo (MatPoint2D.cpp:1138-1300, MatPoint3D.cpp:628-851)
+// gives the corners of the loaded face and hat n times its weight: uGIMP and classic particles
+// take the corners of the undeformed particle and the size of the deformed face, lCPDI particles
+// the deformed corners and their own face. Each corner's element adds its linear shape functions
+// at the corner (GetShapeFunctionsForTractions, MoreMPMElementBase.cpp:91-107) times
+// tmag * weight to the particle's material field at the element's nodes where that field has
+// points (NodalPoint::AddTractionTask3, NodalPointMPM.cpp:644-650).
+
+// signs of r1, r2 (and r3) at the corners of each face, in the reference's corner order
+__constant__ signed char k_face_corners2[4][2][2] = {
+    {{-1, -1}, {1, -1}}, {{1, -1}, {1, 1}}, {{1, 1}, {-1, 1}}, {{-1, 1}, {-1, -1}}};
+__constant__ signed char k_face_corners3[6][4][3] = {
+    {{-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}},    // 1: -y, nodes 1, 2, 6, 5
+    {{1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {1, -1, 1}},        // 2: +x, nodes 2, 3, 7, 6
+    {{1, 1, -1}, {-1, 1, -1}, {-1, 1, 1}, {1, 1, 1}},        // 3: +y, nodes 3, 4, 8, 7
+    {{-1, 1, -1}, {-1, -1, -1}, {-1, -1, 1}, {-1, 1, 1}},    // 4: -x, nodes 4, 1, 5, 8
+    {{1, -1, -1}, {-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}},    // 5: -z, nodes 2, 1, 4, 3
+    {{-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}}};       // 6: +z, nodes 5, 6, 7, 8
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
+    c[0] = a[1] * b[2] - b[1] * a[2];          // CrossProduct (CommonUtilities.cpp:215-221)
+    c[1] = b[0] * a[2] - a[0] * b[2];
+    c[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+// corners (cx[k]) and weighted normal (wn) of one face; returns the corner count
 template <int DIM>
-__global__ void k_mark_load_tiles(KPart P, KBins B, KLoads L, int par) {
+__device__ int surface_info(const KGrid &g, const KPart &P, int p, int ei, int ej, int ek, int face, int dof,
+                            double thick, double cx[4][3], double wn[3]) {
+    double r[9];
+    cpdi_semi_sides<DIM>(g, P, p, ei, ej, ek, r);            // deformed semi sides F (psz)
+    double *r1 = r, *r2 = r + 3, *r3 = r + 6;
+    double uG = -1.;
+    if (g.shape != MPMGPU_SHAPE_LCPDI) {
+        if (DIM == 3) {
+            double ap[3];
+            if (face == 1 || face == 3) cross3(r1, r3, ap);
+            else if (face == 2 || face == 4) cross3(r2, r3, ap);
+            else cross3(r1, r2, ap);
+            uG = sqrt(ap[0] * ap[0] + ap[1] * ap[1] + ap[2] * ap[2]);
+        } else {
+            const double *rs = (face == 1 || face == 3) ? r1 : r2;
+            uG = sqrt(rs[0] * rs[0] + rs[1] * rs[1] + rs[2] * rs[2]);
+        }
+        // the undeformed semi sides (GetUndeformedSemiSides)
+        double lpx, lpy, lpz;
+        if (g.uniformLp) { lpx = g.lp[0]; lpy = g.lp[1]; lpz = g.lp[2]; }
+        else { lpx = P.f[MPF_LP_X][p]; lpy = P.f[MPF_LP_Y][p]; lpz = P.f[MPF_LP_Z][p]; }
+        const double ux = ((g.xmin + (double)(ei + 1) * g.dx) - (g.xmin + (double)ei * g.dx)) * (0.5 * lpx);
+        const double uy = ((g.ymin + (double)(ej + 1) * g.dy) - (g.ymin + (double)ej * g.dy)) * (0.5 * lpy);
+        const double uz = DIM == 3 ? ((g.zmin + (double)(ek + 1) * g.dz) - (g.zmin + (double)ek * g.dz)) * (0.5 * lpz) : 0.;
+#pragma unroll
+        for (int a = 0; a < 9; a++) r[a] = 0.;
+        r1[0] = ux; r2[1] = uy; r3[2] = uz;
+    }
+    const double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+    const int nc = DIM == 3 ? 4 : 2;
+    for (int k = 0; k < nc; k++) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (DIM == 3) {
+                const signed char *sg = k_face_corners3[face - 1][k];
+                cx[k][a] = ((pos[a] + (double)sg[0] * r1[a]) + (double)sg[1] * r2[a]) + (double)sg[2] * r3[a];
+            } else {
+                const signed char *sg = k_face_corners2[face - 1][k];
+                cx[k][a] = a < 2 ? (pos[a] + (double)sg[0] * r1[a]) + (double)sg[1] * r2[a] : 0.;
+            }
+        }
+    }
+    wn[0] = wn[1] = wn[2] = 0.;
+    double r0[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r0[a] = 0.5 * (cx[1][a] - cx[0][a]);
+    if (DIM == 3) {
+        double r1f[3], ap[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) r1f[a] = 0.5 * (cx[3][a] - cx[0][a]);
+        cross3(r0, r1f, ap);
+        const double am = sqrt(ap[0] * ap[0] + ap[1] * ap[1] + ap[2] * ap[2]);
+        const double faceWt = uG < 0. ? am : uG;
+        if (dof == 1) wn[0] = faceWt;
+        else if (dof == 2) wn[1] = faceWt;
+        else if (dof == 11) {
+            const double sc = uG > 0. ? uG / am : 1.;
+#pragma unroll
+            for (int a = 0; a < 3; a++) wn[a] = uG > 0. ? ap[a] * sc : ap[a];
+        } else wn[2] = faceWt;
+    } else {
+        double faceWt = uG < 0. ? sqrt(r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]) : uG;
+        faceWt *= thick;
+        if (dof == 1) wn[0] = faceWt;
+        else if (dof == 2) wn[1] = faceWt;
+        else if (dof == 11 || dof == 12) {
+            const double ex = dof == 11 ? r0[1] : r0[0], ey = dof == 11 ? -r0[0] : r0[1];
+            const double en = sqrt(ex * ex + ey * ey);
+            wn[0] = ex * faceWt / en;
+            wn[1] = ey * faceWt / en;
+        } else wn[2] = faceWt;
+    }
+    return nc;
+}
+
+// the nodal traction forces of traction table t (particle p) at step time tt
+template <int DIM>
+__device__ void particle_tractions(const KGrid &g, const KPart &P, const KNodes &N, const KLoads &L, int p, int t,
+                                   double tt) {
+    double *tab = L.ttab + (size_t)3 * t * TRAC_NODES;
+    for (int o = 0; o < 3 * TRAC_NODES; o++) tab[o] = 0.;
+    int ei, ej, ek;
+    elem_ijk(g, P.elem[p], ei, ej, ek);
+    const int fld = g.fieldOf[P.mat[p]];
+    const KNodes Nf = field_view(g, N, fld);
+    for (int e = L.tstart[t]; e < L.tstart[t + 1]; e++) {
+        double tmag;
+        style_value(L.tstyle[e], L.tvalue[e], L.tftime[e], tt, tmag);
+        double cx[4][3], wn[3];
+        const int nc = surface_info<DIM>(g, P, p, ei, ej, ek, L.tface[e], L.tdir[e], L.tthick[t], cx, wn);
+        for (int k = 0; k < nc; k++) {
+            bool ok = true;
+            const int ci = find_cell(cx[k][0], g.xmin, g.dx, g.nx, ok);
+            const int cj = find_cell(cx[k][1], g.ymin, g.dy, g.ny, ok);
+            const int ck = DIM == 3 ? find_cell(cx[k][2], g.zmin, g.dz, g.nz, ok) : 0;
+            const int ox = ci - ei, oy = cj - ej, oz = DIM == 3 ? ck - ek : 0;
+            if (!ok || ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1) {
+                atomicAdd(L.terr, 1ull);          // a fatal error, reported after the step
+                continue;
+            }
+            double xi[3], invd[3];
+            particle_xi<DIM>(g, ci, cj, ck, cx[k][0], cx[k][1], cx[k][2], xi, invd);
+            for (int jz = 0; jz < (DIM == 3 ? 2 : 1); jz++)
+                for (int jy = 0; jy < 2; jy++)
+                    for (int jx = 0; jx < 2; jx++) {
+                        const long long n = ((long long)(ck + jz) * g.nny + (cj + jy)) * g.nnx + (ci + jx);
+                        if (Nf.cnt[n] <= 0) continue;
+                        const double fn = cpdi_cfn<DIM>(xi, jx, jy, jz);
+                        const int o = DIM == 3 ? ((oz + jz + 1) * 4 + (oy + jy + 1)) * 4 + (ox + jx + 1)
+                                               : (oy + jy + 1) * 4 + (ox + jx + 1);
+#pragma unroll
+                        for (int c = 0; c < DIM; c++) tab[c * TRAC_NODES + o] += wn[c] * (tmag * fn);
+                    }
+        }
+    }
+}
+
+// marks the (field, tile) of every loaded particle in slot par, clears slot par ^ 1 for the next
+// step, and fills the traction tables of the step (PostForcesTask.cpp:51, after the grid forces'
+// strain update and the mass pass that sets the point counts)
+template <int DIM>
+__global__ void k_mark_load_tiles(KGrid g, KPart P, KBins B, KNodes N, KLoads L, int par) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < B.ntall) L.tflag[par ^ 1][p] = 0;
     if (p >= P.n) return;
-    if (L.map[B.home[p]] >= 0) L.tflag[par][B.key[p] / TileGeom<DIM>::NC] = 1;
+    const int q = L.map[B.home[p]];
+    if (q < 0) return;
+    L.tflag[par][B.key[p] / TileGeom<DIM>::NC] = 1;
+    if (L.nt > 0 && L.tslot[q] >= 0) particle_tractions<DIM>(g, P, N, L, (int)p, L.tslot[q], L.time[par]);
 }
 
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
@@ -2489,8 +2669,8 @@ void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, c
     const int *loaded = nullptr;
     if (L.nl > 0) {
         const long long n = std::max<long long>(p.n, b.ntall);
-        if (dim == 3) hipLaunchKernelGGL(k_mark_load_tiles<3>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, p, b, L, par);
-        else hipLaunchKernelGGL(k_mark_load_tiles<2>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, p, b, L, par);
+        if (dim == 3) hipLaunchKernelGGL(k_mark_load_tiles<3>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, g, p, b, nd, L, par);
+        else hipLaunchKernelGGL(k_mark_load_tiles<2>, dim3(blocks_for(n, NT)), dim3(NT), 0, s, g, p, b, nd, L, par);
         loaded = L.tflag[par];
         const KBins tl = tile_bins(b, TILE_SLOT_FORCE_LOADS);
         if (dim == 3) {
@@ -2599,18 +2779,6 @@ void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &
 // the list, then adds for every one; conditions on different nodes touch different values, so
 // taking each node's conditions in list order, zeroes first, gives the same numbers.
 
-// BoundaryCondition::GetNodeNum and BCValue (BoundaryCondition.cpp:72-140): 1 if the condition
-// acts at time t, its value in v
-__device__ __forceinline__ int style_value(int style, double val, double ft, double t, double &v) {
-    v = 0.;
-    switch (style) {
-        case 1: if (t >= ft) { v = val; return 1; } return 0;
-        case 2: if (t >= ft) { v = 0. + val * (t - ft); return 1; } return 0;
-        case 3: v = val * sin(t * ft); return 1;
-        case 4: v = val * cos(t * ft); return 1;
-        default: return 0;
-    }
-}
 __device__ __forceinline__ int bc_value(const KBcs &bc, int e, double t, double &v) {
     return style_value(bc.style[e], bc.value[e], bc.ftime[e], t, v);
 }

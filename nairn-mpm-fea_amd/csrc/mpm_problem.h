@@ -61,6 +61,8 @@ struct MpmProblem {
     std::vector<MpmVelocityBC> velBCs;
     // <ParticleBCs> loads in the reference's list order
     std::vector<MpmParticleLoad> loads;
+    // <ParticleBCs> tractions in the reference's list order
+    std::vector<MpmParticleTraction> tractions;
     // particles, in mpm[] order
     std::vector<double> d[MPF_NUM_FIELDS];
     std::vector<int32_t> iv[MPI_NUM_FIELDS];

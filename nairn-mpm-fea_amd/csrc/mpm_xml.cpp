@@ -356,6 +356,8 @@ struct Reader {
                 if (k.attr("function")) return fail("<" + k.name + " function=> (damping that varies in time) is not on this path");
                 if (k.attr("PIC")) return fail("PIC attribute on damping remove; use PeriodicXPIC task intead");
                 (k.name == "Damping" ? p.run.gridDamping : p.run.particleDamping) = textnum(k);
+            } else if (k.name == "ExactTractions") {
+                return fail("<ExactTractions> is not on this path");
             } else if (k.name == "FeedbackDamping" || k.name == "PFeedbackDamping" || k.name == "XPIC") {
                 return fail("<" + k.name + "> is not on this path");
             }
@@ -672,7 +674,7 @@ struct Reader {
         return true;
     }
 
-    // <ParticleBCs>: <LoadBC> external loads on the particles a <BCLine> (2D) or <BCBox> (3D)
+    // <ParticleBCs>: <LoadBC> external loads and <TractionBC> tractions on the particles a <BCLine> (2D) or <BCBox> (3D)
     // contains, particles in mpm[] order (ShapeController::nextParticle, ShapeController.cpp:277-290);
     // after <net/> the load is divided by the shape's particle count, <perParticle/> returns to
     // one load per particle (Generators.cpp:1012-1023, ShapeController::setNetBC, :292-309)
@@ -692,18 +694,48 @@ struct Reader {
                 const XNode &b = *bp;
                 if (b.name == "net") { net = true; continue; }
                 if (b.name == "perParticle") { net = false; continue; }
-                if (b.name != "LoadBC")
+                if (b.name != "LoadBC" && b.name != "TractionBC")
                     return fail("<" + b.name + "> in <" + s.name + "> is not on this path");
-                int dof = 0, style = 1;
+                int dof = 0, style = 1, face = 1;
                 double load = 0., ftime = 0.;
                 for (auto &a : b.attrs) {
                     const std::string &k = a.first;
                     const char *v = a.second.c_str();
                     if (k == "dir") dof = (int)strtol(v, nullptr, 10);
                     else if (k == "style") style = (int)strtol(v, nullptr, 10);
+                    else if (k == "face") face = (int)strtol(v, nullptr, 10);
                     else if (k == "load" || k == "value" || k == "stress") load = strtod(v, nullptr);
                     else if (k == "ftime" || k == "time" || k == "bath") ftime = strtod(v, nullptr);
-                    else if (k == "function") return fail("<LoadBC function=> is not on this path");
+                    else if (k == "function") return fail("<" + b.name + " function=> is not on this path");
+                }
+                if (b.name == "TractionBC") {
+                    // Generators.cpp:1139-1167,1189-1202: checks, then one condition per particle
+                    // (not divided by the particle count); the value in MPa becomes Pa
+                    // (MatPtLoadBC::SetBCValue, MatPtLoadBC.cpp:202-206)
+                    if (dim == 3 && !((dof >= 1 && dof <= 3) || dof == 11))
+                        return fail("'dir' in <TractionBC> must be 1-3 or 11 for 3D analyses.");
+                    if (dim == 3 && (face < 1 || face > 6))
+                        return fail("'face' in <TractionBC>, <ConcFluxBC>, or <HeatFluxBC> element must be 1 to 6 for 3D analyses.");
+                    if (dim == 2 && !(dof == 1 || dof == 2 || dof == 11 || dof == 12))
+                        return fail("'dir' in <LoadBC> or <ConcFluxBC> must be 1 or 2 and in <TractionBC> must be 1, 2, 11, or 12 for 2D analyses.");
+                    if (dim == 2 && (face < 1 || face > 4))
+                        return fail("'face' in <TractionBC>, <ConcFluxBC>, or <HeatFluxBC> element must be 1 to 4 for 2D analyses.");
+                    if (style == 5) return fail("Tractions boundary conditions cannot use silent style.");
+                    if (style < 1 || style > 4) return fail("traction styles other than 1 to 4 are not on this path");
+                    double v = 1.e6 * load;
+                    if (style == 2) v = v * 1.e3;
+                    MpmParticleTraction tr{};
+                    tr.dir = dof;
+                    tr.face = face;
+                    tr.style = style;
+                    tr.value = v;
+                    tr.ftime = first_time(style, ftime);
+                    for (int64_t i : inside) {
+                        tr.particle = i;
+                        tr.thickness = dim == 2 ? p.pthick[(size_t)i] : 1.;
+                        p.tractions.push_back(tr);
+                    }
+                    continue;
                 }
                 if (dim == 3 && (dof < 1 || dof > 3)) return fail("'dir' in <LoadBC> must be 1-3 for 3D analyses.");
                 if (dim == 2 && (dof < 1 || dof > 2))

@@ -121,6 +121,15 @@ class ParticleLoad(ctypes.Structure):
                 ("value", ctypes.c_double), ("ftime", ctypes.c_double)]
 
 
+class ParticleTraction(ctypes.Structure):
+    """one traction condition (MpmParticleTraction): particle (upload index), dir (1..3, 11 normal,
+    12 tangential in 2D), face (1..4 in 2D, 1..6 in 3D), style 1..4, value (Pa; linear: Pa/s),
+    ftime (s, or 1/s for the sine and cosine styles) and the particle's thickness (2D, mm)"""
+    _fields_ = [("particle", ctypes.c_int64), ("dir", ctypes.c_int32), ("face", ctypes.c_int32),
+                ("style", ctypes.c_int32), ("reserved", ctypes.c_int32), ("value", ctypes.c_double),
+                ("ftime", ctypes.c_double), ("thickness", ctypes.c_double)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -145,6 +154,8 @@ _SIGNATURES = {
     "mpmgpu_problem_velocity_bcs": ([_P, ctypes.POINTER(VelocityBC), ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "mpmgpu_problem_particle_loads": ([_P, ctypes.POINTER(ParticleLoad), ctypes.POINTER(ctypes.c_int64)],
                                       ctypes.c_int),
+    "mpmgpu_problem_particle_tractions": ([_P, ctypes.POINTER(ParticleTraction), ctypes.POINTER(ctypes.c_int64)],
+                                          ctypes.c_int),
     "mpmgpu_problem_destroy": ([_P], None),
     "mpmgpu_archive_order": ([ctypes.c_char_p, ctypes.c_char_p], ctypes.c_int),
     "mpmgpu_archive_record_size": ([ctypes.c_char_p, ctypes.c_int32], ctypes.c_int32),
@@ -159,6 +170,7 @@ _SIGNATURES = {
     "mpmgpu_upload_particles": ([_P, ctypes.c_int64, _PD, _PI], ctypes.c_int),
     "mpmgpu_set_velocity_bcs": ([_P, ctypes.c_int64, ctypes.POINTER(VelocityBC)], ctypes.c_int),
     "mpmgpu_set_particle_loads": ([_P, ctypes.c_int64, ctypes.POINTER(ParticleLoad)], ctypes.c_int),
+    "mpmgpu_set_particle_tractions": ([_P, ctypes.c_int64, ctypes.POINTER(ParticleTraction)], ctypes.c_int),
     "mpmgpu_batch_create": ([ctypes.c_int32, ctypes.POINTER(InstanceDesc), ctypes.POINTER(Material), ctypes.c_int32,
                              ctypes.POINTER(_P)], ctypes.c_int),
     "mpmgpu_run_task": ([_P, ctypes.c_int32], ctypes.c_int),
@@ -331,6 +343,14 @@ class Problem:
         _check(library().mpmgpu_problem_particle_loads(self._h, arr, ctypes.byref(n)))
         return [arr[k] for k in range(n.value)]
 
+    def particle_tractions(self):
+        """the <ParticleBCs> tractions in the reference's list order (ParticleTraction list)"""
+        n = ctypes.c_int64(0)
+        _check(library().mpmgpu_problem_particle_tractions(self._h, None, ctypes.byref(n)))
+        arr = (ParticleTraction * max(n.value, 1))()
+        _check(library().mpmgpu_problem_particle_tractions(self._h, arr, ctypes.byref(n)))
+        return [arr[k] for k in range(n.value)]
+
     def close(self):
         if self._h:
             library().mpmgpu_problem_destroy(self._h)
@@ -367,6 +387,9 @@ class Simulation:
         loads = prob.particle_loads()
         if loads:
             sim.set_particle_loads(loads)
+        tractions = prob.particle_tractions()
+        if tractions:
+            sim.set_particle_tractions(tractions)
         return sim
 
     def set_particle_loads(self, loads):
@@ -374,6 +397,12 @@ class Simulation:
         [] removes them; a later upload removes them too"""
         arr = (ParticleLoad * max(len(loads), 1))(*loads)
         _check(library().mpmgpu_set_particle_loads(self._h, len(loads), arr), self._h)
+
+    def set_particle_tractions(self, tractions):
+        """traction conditions (ParticleTraction list, reference list order, upload indices);
+        [] removes them; a later upload removes them too"""
+        arr = (ParticleTraction * max(len(tractions), 1))(*tractions)
+        _check(library().mpmgpu_set_particle_tractions(self._h, len(tractions), arr), self._h)
 
     def set_velocity_bcs(self, bcs):
         """grid velocity conditions (VelocityBC list, reference list order); [] removes them"""
@@ -489,8 +518,9 @@ class Batch(Simulation):
         replaces the problem's own initial particles (same count and order)"""
         descs, mats, ds, iss = [], [], [], []
         for k, prob in enumerate(problems):
-            if prob.velocity_bcs() or prob.particle_loads():
-                raise MPMError("grid velocity conditions and particle loads are for single simulations, not batches")
+            if prob.velocity_bcs() or prob.particle_loads() or prob.particle_tractions():
+                raise MPMError("grid velocity conditions, particle loads and tractions are for single simulations, "
+                               "not batches")
             g, m, run, n, _ = prob.describe()
             d, i = prob.particles() if particles is None or particles[k] is None else particles[k]
             i = i.copy()
@@ -567,8 +597,9 @@ class Server:
         return rid.value
 
     def submit_problem(self, prob: Problem, steps: int, particles=None) -> int:
-        if prob.velocity_bcs() or prob.particle_loads():
-            raise MPMError("grid velocity conditions and particle loads are for single simulations, not served solves")
+        if prob.velocity_bcs() or prob.particle_loads() or prob.particle_tractions():
+            raise MPMError("grid velocity conditions, particle loads and tractions are for single simulations, "
+                           "not served solves")
         g, mats, run, _, _ = prob.describe()
         d, i = prob.particles() if particles is None else particles
         return self.submit(g, mats, run, d, i, steps)

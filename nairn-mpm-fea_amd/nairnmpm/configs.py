@@ -150,6 +150,12 @@ def particle_bcs(xml: str, shapes) -> str:
     return grid_bcs(xml, shapes, block="ParticleBCs", child="LoadBC")
 
 
+def traction_bcs(xml: str, shapes) -> str:
+    """xml with a <ParticleBCs> block of <TractionBC> conditions, shapes as for grid_bcs, e.g.
+    ("BCLine", {...}, [{"dir": 2, "face": 3, "stress": 1.5, "style": 1}])"""
+    return grid_bcs(xml, shapes, block="ParticleBCs", child="TractionBC")
+
+
 def mooney_material_xml(G1: float = 8., G2: float = 2., K: float = 50., rho: float = 1.1,
                         name: str = "mooney") -> str:
     """<Material Type="8"> (Mooney-Rivlin) with G1, G2 and K in MPa."""

@@ -417,6 +417,152 @@ struct Oracle {
         }
     }
 
+    // traction conditions in MatPtTractionBC list order (firstTractionPt)
+    std::vector<MpmParticleTraction> tractions;
+
+    // MatPoint2D::GetSurfaceInfo (MatPoint2D.cpp:1138-1300, planar) and MatPoint3D::GetSurfaceInfo
+    // (MatPoint3D.cpp:628-851): the corners of the face (two in 2D, four in 3D, in the reference's
+    // order) and hat n times half the face length times the thickness (2D) or a quarter of the
+    // face area (3D). uGIMP and classic particles (not CPDI, not finite GIMP) take the corners of the
+    // undeformed particle (GetUndeformedSemiSides, MatPoint3D.cpp:528-534) and the size of the
+    // deformed face.
+    int surface_info(long p, int face, int dof, double thick, double cx[4][3], double wn[3]) {
+        Mat3 Fm = get_F(p);
+        const double (*pF)[3] = Fm.m;
+        long i, j, k; g.elem_ijk(I(MPI_ELEM, p), i, j, k);
+        // GetSemiSideVectors with GetParticleSize (MatPoint3D.cpp:413-434, MatPoint2D.cpp:423-440)
+        double psx = (g.xpts[i + 1] - g.xpts[i]) * (0.5 * F(MPF_LP_X, p));
+        double psy = (g.ypts[j + 1] - g.ypts[j]) * (0.5 * F(MPF_LP_Y, p));
+        double psz = g.dim == 3 ? (g.zpts[k + 1] - g.zpts[k]) * (0.5 * F(MPF_LP_Z, p)) : 0.;
+        double r1[3] = {pF[0][0] * psx, pF[1][0] * psx, g.dim == 3 ? pF[2][0] * psx : 0.};
+        double r2[3] = {pF[0][1] * psy, pF[1][1] * psy, g.dim == 3 ? pF[2][1] * psy : 0.};
+        double r3[3] = {pF[0][2] * psz, pF[1][2] * psz, pF[2][2] * psz};
+        if (g.dim == 2) r3[0] = r3[1] = r3[2] = 0.;
+        auto cross = [](const double *a, const double *b, double *c) {     // CrossProduct
+            c[0] = a[1] * b[2] - b[1] * a[2];
+            c[1] = b[0] * a[2] - a[0] * b[2];
+            c[2] = a[0] * b[1] - b[0] * a[1];
+        };
+        auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        double uG = -1.;
+        if (run.shape != MPMGPU_SHAPE_LCPDI) {
+            if (g.dim == 3) {
+                double ap[3];
+                switch (face) {
+                    case 1: case 3: cross(r1, r3, ap); break;
+                    case 4: case 2: cross(r2, r3, ap); break;
+                    default: cross(r1, r2, ap); break;
+                }
+                uG = sqrt(dot(ap, ap));
+            } else {
+                uG = (face == 1 || face == 3) ? sqrt(dot(r1, r1)) : sqrt(dot(r2, r2));
+            }
+            for (int a = 0; a < 3; a++) r1[a] = r2[a] = r3[a] = 0.;
+            r1[0] = psx; r2[1] = psy; r3[2] = psz;
+        }
+        const double pos[3] = {F(MPF_POS_X, p), F(MPF_POS_Y, p), g.dim == 3 ? F(MPF_POS_Z, p) : 0.};
+        // each corner as pos -+ r1 -+ r2 (-+ r3), evaluated left to right as the reference writes it
+        auto corner = [&](double *c, double s1, double s2, double s3) {
+            for (int a = 0; a < 3; a++) {
+                double v = pos[a];
+                v = s1 > 0. ? v + r1[a] : v - r1[a];
+                v = s2 > 0. ? v + r2[a] : v - r2[a];
+                if (g.dim == 3) v = s3 > 0. ? v + r3[a] : v - r3[a];
+                c[a] = g.dim == 3 || a < 2 ? v : 0.;
+            }
+        };
+        int nc;
+        if (g.dim == 3) {
+            nc = 4;
+            switch (face) {
+                case 1:  corner(cx[0], -1, -1, -1); corner(cx[1], 1, -1, -1); corner(cx[2], 1, -1, 1); corner(cx[3], -1, -1, 1); break;
+                case 2:  corner(cx[0], 1, -1, -1); corner(cx[1], 1, 1, -1); corner(cx[2], 1, 1, 1); corner(cx[3], 1, -1, 1); break;
+                case 3:  corner(cx[0], 1, 1, -1); corner(cx[1], -1, 1, -1); corner(cx[2], -1, 1, 1); corner(cx[3], 1, 1, 1); break;
+                case 4:  corner(cx[0], -1, 1, -1); corner(cx[1], -1, -1, -1); corner(cx[2], -1, -1, 1); corner(cx[3], -1, 1, 1); break;
+                case 5:  corner(cx[0], 1, -1, -1); corner(cx[1], -1, -1, -1); corner(cx[2], -1, 1, -1); corner(cx[3], 1, 1, -1); break;
+                default: corner(cx[0], -1, -1, 1); corner(cx[1], 1, -1, 1); corner(cx[2], 1, 1, 1); corner(cx[3], -1, 1, 1); break;
+            }
+        } else {
+            nc = 2;
+            switch (face) {
+                case 1:  corner(cx[0], -1, -1, 0); corner(cx[1], 1, -1, 0); break;
+                case 2:  corner(cx[0], 1, -1, 0); corner(cx[1], 1, 1, 0); break;
+                case 3:  corner(cx[0], 1, 1, 0); corner(cx[1], -1, 1, 0); break;
+                default: corner(cx[0], -1, 1, 0); corner(cx[1], -1, -1, 0); break;
+            }
+        }
+        double r0[3] = {0.5 * (cx[1][0] - cx[0][0]), 0.5 * (cx[1][1] - cx[0][1]), 0.5 * (cx[1][2] - cx[0][2])};
+        wn[0] = wn[1] = wn[2] = 0.;
+        if (g.dim == 3) {
+            double rb[3] = {0.5 * (cx[3][0] - cx[0][0]), 0.5 * (cx[3][1] - cx[0][1]), 0.5 * (cx[3][2] - cx[0][2])};
+            double ap[3];
+            cross(r0, rb, ap);
+            const double faceWt = uG < 0. ? sqrt(dot(ap, ap)) : uG;
+            switch (dof) {
+                case 1: wn[0] = faceWt; break;
+                case 2: wn[1] = faceWt; break;
+                case 11:
+                    if (uG > 0.) {
+                        const double sc = uG / sqrt(dot(ap, ap));
+                        for (int a = 0; a < 3; a++) ap[a] *= sc;
+                    }
+                    for (int a = 0; a < 3; a++) wn[a] = ap[a];
+                    break;
+                default: wn[2] = faceWt; break;
+            }
+        } else {
+            double faceWt = uG < 0. ? sqrt(dot(r0, r0)) : uG;
+            faceWt *= thick;
+            double ex, ey, en;
+            switch (dof) {
+                case 1: wn[0] = faceWt; break;
+                case 2: wn[1] = faceWt; break;
+                case 11:
+                    ex = r0[1]; ey = -r0[0]; en = sqrt(ex * ex + ey * ey);
+                    wn[0] = ex * faceWt / en; wn[1] = ey * faceWt / en;
+                    break;
+                case 12:
+                    ex = r0[0]; ey = r0[1]; en = sqrt(ex * ex + ey * ey);
+                    wn[0] = ex * faceWt / en; wn[1] = ey * faceWt / en;
+                    break;
+                default: wn[2] = faceWt; break;
+            }
+        }
+        return nc;
+    }
+
+    // MatPtTractionBC::SetParticleSurfaceTractions and AddMPFluxBC (MatPtTractionBC.cpp:64-240,
+    // no cracks, not ExactTractions, not axisymmetric), called from PostForcesTask.cpp:51: for each
+    // condition in list order and each face corner, the linear shape functions of the corner's
+    // element (GetShapeFunctionsForTractions, MoreMPMElementBase.cpp:91-107) times tmag times the
+    // weighted normal go to the particle's material field at the element's nodes where that field
+    // has points (NodalPoint::AddTractionTask3, NodalPointMPM.cpp:644-650); one rescaling pass
+    // only, so the net shape function is not renormalised.
+    void add_tractions() {
+        for (const MpmParticleTraction &t : tractions) {
+            const long p = (long)t.particle;
+            bool acts;
+            const double tmag = style_value(t.style, t.value, t.ftime, mtime, acts);
+            const int fld = field(p);
+            double cx[4][3], wn[3];
+            const int nc = surface_info(p, t.face, t.dir, t.thickness, cx, wn);
+            for (int c = 0; c < nc; c++) {
+                const long e = find_element(cx[c]);
+                if (e < 0) throw std::runtime_error("A Traction edge node has left the grid");
+                double xi[3];
+                get_xi(g, e, cx[c], xi);
+                ShapeOut o;
+                linear_shape(g, e, xi, false, o);
+                for (int i = 0; i < o.n; i++) {
+                    const long n = fnode(fld, o.node[i]);
+                    if (npts[n] <= 0) continue;
+                    const double s = tmag * o.fn[i];
+                    for (int a = 0; a < 3; a++) ftot[3 * n + a] += s * wn[a];
+                }
+            }
+        }
+    }
+
     // lCPDI corners of the step (MPMBase CPDIDomain: inElem, ncpos, wg; ws = 1/8 or 1/4)
     struct Cpdi { long elem[8]; double xi[8][3]; double wg[8][3]; };
     std::vector<Cpdi> cpdi;
@@ -1614,6 +1760,7 @@ struct Oracle {
 #pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] = pkc[3 * n + c];
+        add_tractions();                                 // PostForcesTask.cpp:51
         // gravity on every active field (PostForcesTask.cpp:62-91, MatVelocityField.cpp:254-258)
         const double *gv = run.gravity;
         if (gv[0] != 0. || gv[1] != 0. || gv[2] != 0.)
@@ -1899,6 +2046,32 @@ int oracle_set_velocity_bcs(void *h, int64_t n, const MpmVelocityBC *bcs) {
     o->vbcs.assign(bcs, bcs + n);
     o->bc_values();                      // values at the current time until the next step's
     return 0;
+}
+
+/* traction conditions in list order (n = 0 removes them) */
+int oracle_set_particle_tractions(void *h, int64_t n, const MpmParticleTraction *tr) {
+    Oracle *o = (Oracle *)h;
+    for (int64_t i = 0; i < n; i++) {
+        const MpmParticleTraction &t = tr[i];
+        if (t.particle < 0 || t.particle >= o->np) return 1;
+        if (o->g.dim == 3 && !((t.dir >= 1 && t.dir <= 3) || t.dir == 11)) return 1;
+        if (o->g.dim == 2 && !(t.dir == 1 || t.dir == 2 || t.dir == 11 || t.dir == 12)) return 1;
+        if (t.face < 1 || t.face > (o->g.dim == 3 ? 6 : 4)) return 1;
+        if (t.style < 1 || t.style > 4) return 1;
+    }
+    o->tractions.assign(tr, tr + n);
+    return 0;
+}
+
+/* the face corners and weighted normal of GetSurfaceInfo for particle p; returns the corner count */
+int oracle_surface_info(void *h, int64_t p, int32_t face, int32_t dir, double thickness, double *corners,
+                        double *wn) {
+    Oracle *o = (Oracle *)h;
+    double cx[4][3];
+    const int nc = o->surface_info((long)p, face, dir, thickness, cx, wn);
+    for (int c = 0; c < nc; c++)
+        for (int a = 0; a < 3; a++) corners[3 * c + a] = cx[c][a];
+    return nc;
 }
 
 /* external particle loads in list order (n = 0 removes them) */

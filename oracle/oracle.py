@@ -43,6 +43,9 @@ def lib():
         _lib.oracle_set_threads.argtypes = [_P, ctypes.c_int32]
         _lib.oracle_set_velocity_bcs.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.VelocityBC)]
         _lib.oracle_set_particle_loads.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.ParticleLoad)]
+        _lib.oracle_set_particle_tractions.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.ParticleTraction)]
+        _lib.oracle_surface_info.argtypes = [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                             _PD, _PD]
         _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
         _lib.oracle_num_nodes.argtypes = [_P]
         _lib.oracle_num_nodes.restype = ctypes.c_int64
@@ -93,7 +96,26 @@ class Oracle:
         loads = prob.particle_loads()
         if loads:
             o.set_particle_loads(loads)
+        tractions = prob.particle_tractions()
+        if tractions:
+            o.set_particle_tractions(tractions)
         return o
+
+    def set_particle_tractions(self, tractions):
+        """traction conditions in list order (nairnmpm.ParticleTraction list)"""
+        import nairnmpm as nm
+        arr = (nm.ParticleTraction * max(len(tractions), 1))(*tractions)
+        if lib().oracle_set_particle_tractions(self._h, len(tractions), arr) != 0:
+            raise RuntimeError("bad traction condition")
+
+    def surface_info(self, p, face, dir, thickness=1.0):
+        """GetSurfaceInfo of particle p in its current state: (corners, weighted normal), corners
+        an (n, 3) array in the reference's order"""
+        c = np.zeros(12)
+        wn = np.zeros(3)
+        nc = lib().oracle_surface_info(self._h, p, face, dir, thickness,
+                                       c.ctypes.data_as(_PD), wn.ctypes.data_as(_PD))
+        return c[:3 * nc].reshape(nc, 3), wn
 
     def set_particle_loads(self, loads):
         """external particle loads in list order (nairnmpm.ParticleLoad list)"""

@@ -67,14 +67,6 @@ def test_load_refusals(child, msg):
         nm.Problem.from_xml(_block2d([("BCLine", {"x1": 4, "y1": 4, "x2": 12, "y2": 4}, [child])]))
 
 
-def test_traction_refused():
-    xml = configs.grid_bcs(configs.elastic_block_2d(cells=16, block=8),
-                           [("BCLine", {"x1": 4, "y1": 4, "x2": 12, "y2": 4}, [{"dir": 1, "stress": 1., "face": 1}])],
-                           block="ParticleBCs", child="TractionBC")
-    with pytest.raises(nm.MPMError, match="TractionBC"):
-        nm.Problem.from_xml(xml)
-
-
 def _momentum(o, dim):
     d, _ = o.particles()
     return np.array([np.sum(d[F["mass"]] * d[F["vel_" + a]]) for a in "xyz"[:dim]])
