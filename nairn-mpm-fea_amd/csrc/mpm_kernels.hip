@@ -428,49 +428,67 @@ __global__ void k_scan_add(int *out, const int *blockSums, int n) {
     if (i < n) out[i] += blockSums[i / SCAN_BLOCK];
 }
 
-// Slots by key: lanes of a wave with the same key (neighbours, since the rows are kept in tile
-// order) take consecutive slots with one atomic per key instead of one per particle.
+// Slots by key. The rows are kept in tile order, so the keys of a wave's 64 consecutive rows come
+// in runs; the first lane of each run takes the run's slots with one atomic, all runs of the wave
+// at once (one round trip per wave, not one per key). The order inside a cell is then fixed by
+// k_bin_sort_cells, so the slots' first placement does not change any result.
 __global__ void k_bin_scatter(KPart P, KBins B) {
     if (B.stop[0]) return;
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < P.n;
     const int key = valid ? B.key[p] : -1;
     const int lane = (int)__lane_id();
-    unsigned long long active = __ballot(valid);
-    while (active) {
-        const int leader = __ffsll((long long)active) - 1;
-        const int lkey = __shfl(key, leader);
-        const unsigned long long same = __ballot(key == lkey) & active;
-        int base = 0;
-        if (lane == leader) base = B.cellStart[lkey] + atomicAdd(&B.cursor[lkey], __popcll(same));
-        base = __shfl(base, leader);
-        if ((same >> lane) & 1ull) {
-            const int slot = base + __popcll(same & ((1ull << lane) - 1ull));
-            B.order[slot] = (int)p;
-            B.shome[slot] = B.home[p];
-        }
-        active &= ~same;
+    const int prev = __shfl_up(key, 1);
+    const bool start = valid && (lane == 0 || prev != key);
+    const unsigned long long starts = __ballot(start);
+    const int nvalid = 64 - __clzll(__ballot(valid));                 // valid lanes are a prefix
+    const unsigned long long upto = (2ull << lane) - 1ull;           // lanes 0..lane
+    const int leader = 63 - __clzll(starts & upto);                   // this lane's run start
+    int base = 0;
+    if (start) {
+        const unsigned long long later = starts & ~upto;
+        const int end = later ? __ffsll((long long)later) - 1 : nvalid;
+        base = B.cellStart[key] + atomicAdd(&B.cursor[key], end - lane);
+    }
+    base = __shfl(base, leader & 63);
+    if (valid) {
+        const int slot = base + (lane - leader);
+        B.order[slot] = (int)p;
+        B.shome[slot] = B.home[p];
     }
 }
 
 // the particles of a cell in upload order (their index at upload, which the memory order of the
-// rows does not change), so every sum over a cell's particles has the same order in any layout
+// rows does not change), so every sum over a cell's particles has the same order in any layout.
+// Most cells arrive in order already (their particles stayed put, in rows sorted by upload index):
+// a cell of up to 16 is checked with independent loads first and only sorted when it is not.
 __global__ void k_bin_sort_cells(KBins B) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= B.nkeys) return;
     int b = B.cellStart[c], e = B.cellStart[c + 1];
     if (B.stop[0]) e = b;                    // stopped: nothing was scattered
-    for (int i = b + 1; i < e; i++) {
-        const int v = B.order[i];
-        const int hv = B.shome[i];
-        int j = i - 1;
-        while (j >= b && B.shome[j] > hv) {
-            B.order[j + 1] = B.order[j];
-            B.shome[j + 1] = B.shome[j];
-            j--;
+    bool sorted = false;
+    if (e - b <= 16) {
+        int h[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) h[t] = b + t < e ? B.shome[b + t] : 0x7fffffff;
+        sorted = true;
+#pragma unroll
+        for (int t = 1; t < 16; t++) sorted = sorted && h[t - 1] <= h[t];
+    }
+    if (!sorted) {
+        for (int i = b + 1; i < e; i++) {
+            const int v = B.order[i];
+            const int hv = B.shome[i];
+            int j = i - 1;
+            while (j >= b && B.shome[j] > hv) {
+                B.order[j + 1] = B.order[j];
+                B.shome[j + 1] = B.shome[j];
+                j--;
+            }
+            B.order[j + 1] = v;
+            B.shome[j + 1] = hv;
         }
-        B.order[j + 1] = v;
-        B.shome[j + 1] = hv;
     }
     B.cursor[c] = 0;
     B.count[c] = 0;
