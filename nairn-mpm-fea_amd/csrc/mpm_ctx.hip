@@ -56,6 +56,12 @@ static int ensure_time_slots(MpmCtx *c) {
 
 static int build_particle_bcs(MpmCtx *c);
 
+// every (field, tile) pair's nodes are visited by the next step's node passes (after an upload or a
+// restored state, when nodes may hold values of particles that are no longer there)
+static hipError_t mark_all_needed(MpmCtx *c) {
+    return hipMemsetAsync(c->kb.needAny, 1, (size_t)c->kb.ntall * sizeof(int), c->stream);
+}
+
 // the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108); with
 // profiling on, an event is recorded between kernel groups
 void mpmi::enqueue_step(MpmCtx *c) {
@@ -222,7 +228,7 @@ void mpmi::fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunPar
 
 // ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
 size_t mpmi::bin_words(long long nk, long long ntall) {
-    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * 8 * TILE_SLOT_STRIDE);
+    return (size_t)(3 * nk + 2 + (nk + 1 + 1023) / 1024 + 1 + ntall + 1 + TILE_SLOTS * 8 * TILE_SLOT_STRIDE + 3 * ntall);
 }
 
 extern "C" {
@@ -501,7 +507,7 @@ void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, i
     c->kn.cnorm = a.normBytes ? normMem : nullptr;
     c->kn.clist = a.listBytes ? listMem : nullptr;
     // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
-    // (nfields * tilesTotal) and its length and work counters
+    // (nfields * tilesTotal) and its length, work counters, and the node passes' pair flags
     const long long nk = a.nkeys;
     const long long nblk = (nk + 1 + 1023) / 1024 + 1;
     c->kb.nkeys = (int)nk;
@@ -514,6 +520,9 @@ void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, i
     c->kb.tiles = c->kb.blockSums + nblk;
     c->kb.ntiles = c->kb.tiles + c->kb.ntall;
     c->kb.work = c->kb.ntiles + 1;
+    c->kb.need = c->kb.work + TILE_SLOTS * 8 * TILE_SLOT_STRIDE;
+    c->kb.needAny = c->kb.need + c->kb.ntall;
+    c->kb.tileOn = c->kb.needAny + c->kb.ntall;
     c->kb.slot = 0;
 }
 
@@ -787,6 +796,8 @@ int mpmi::prepare_particles(MpmCtx *c) {
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
     // the tile list counts a completed step at every binning; this one is the upload's, not a step's
     HIPCHECK(c, hipMemsetAsync(c->kb.stop + 2, 0, sizeof(unsigned long long), c->stream));
+    // the first step visits every node, clearing what an earlier upload left
+    HIPCHECK(c, mark_all_needed(c));
     if (mpmi::sort_rows(c)) return 1;
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -1267,6 +1278,7 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     HIPCHECK(c, hipMemcpyAsync(c->pbuf, save, pb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->ibuf, save + pb, ib, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->bbuf, save + pb + ib, bb, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(c, mark_all_needed(c));      // the next step clears the nodes the trials touched
     HIPCHECK(c, hipMemcpyAsync(c->sbuf, save + pb + ib + bb, STATS_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                                c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));

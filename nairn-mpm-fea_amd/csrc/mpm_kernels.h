@@ -84,6 +84,11 @@ struct KBins {
     int *tiles;                // the non-empty (field, tile) pairs of the binning, in no fixed order
     int *ntiles;               // length of tiles
     int *work;                 // tile kernels' list cursors: 8 per launch of a step, 32 ints apart
+    // per (field, tile): a non-empty pair's footprint reaches the pair's nodes at this binning
+    // (need), at this or the previous one (needAny: the pairs whose nodes the node passes visit)
+    int *need;
+    int *needAny;
+    int *tileOn;               // per (field, tile): non-empty at this binning (k_tile_list)
     // run stop (ResetElementsTask's exceptions): stop[0] the reason and its rank (stop_run; 0 while running),
     // stop[1] the steps completed before the step that stopped, stop[2] the steps completed since
     // the upload (counted by the tile list). Once stop[0] is set, binning builds an empty tile

@@ -501,6 +501,7 @@ __global__ void k_tile_list(KBins B, int nc) {
     const bool stopped = B.stop[0] != 0;
     if (t == 0 && !stopped) B.stop[2] += 1;   // one more step completed
     const bool on = !stopped && t < B.ntall && B.cellStart[(size_t)(t + 1) * nc] > B.cellStart[(size_t)t * nc];
+    if (t < B.ntall) B.tileOn[t] = on ? 1 : 0;
     const unsigned long long m = __ballot(on);
     if (m == 0ull) return;
     const int lane = (int)__lane_id();
@@ -509,6 +510,32 @@ __global__ void k_tile_list(KBins B, int nc) {
     if (lane == leader) base = atomicAdd(B.ntiles, __popcll(m));
     base = __shfl(base, leader);
     if (on) B.tiles[base + __popcll(m & ((1ull << lane) - 1ull))] = t;
+}
+
+// the (field, tile) pairs the next node passes visit (node_needed): needed = the pair or one of its
+// neighbours is non-empty; needAny = needed now or at the previous binning. A stopped run needs none.
+template <int DIM>
+__global__ void k_tile_need(KGrid g0, KBins B) {
+    using G = TileGeom<DIM>;
+    const int tall = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tall >= B.ntall) return;
+    const int fld = tall / g0.tilesTotal;
+    const int tg = tall - fld * g0.tilesTotal;
+    const KGrid g = tile_grid(g0, tg);
+    const int t = tg - g.tileBase;
+    const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = DIM == 3 ? t / (g.ntx * g.nty) : 0;
+    int need = 0;
+    if (!B.stop[0]) {
+        const int zl = DIM == 3 ? max(tz - 1, 0) : 0, zh = DIM == 3 ? min(tz + 1, g.ntz - 1) : 0;
+        for (int z = zl; z <= zh && !need; z++)
+            for (int y = max(ty - 1, 0); y <= min(ty + 1, g.nty - 1) && !need; y++)
+                for (int x = max(tx - 1, 0); x <= min(tx + 1, g.ntx - 1); x++) {
+                    const long long u = (long long)fld * g0.tilesTotal + g.tileBase + (z * g.nty + y) * g.ntx + x;
+                    if (B.tileOn[u]) { need = 1; break; }
+                }
+    }
+    B.needAny[tall] = need | B.need[tall];
+    B.need[tall] = need;
 }
 
 // A tile kernel's blocks walk the list of non-empty tiles, so no workgroup is spent on an empty
@@ -790,13 +817,26 @@ __device__ __forceinline__ void gather_partials(const KGrid &g, const KBins &B, 
         for (int ty = tyl; ty <= tyh; ty++)
             for (int tx = txl; tx <= txh; tx++) {
                 int t = (tz * g.nty + ty) * g.ntx + tx + g.tileBase + fld * g.tilesTotal;
-                if (B.cellStart[(t + 1) * G::NC] == B.cellStart[t * G::NC]) continue;
+                if (!B.tileOn[t]) continue;
                 int fx = i - (tx * G::TX - 1), fy = j - (ty * G::TY - 1);   // slot in the footprint
                 int s = fy * G::FX + fx;
                 if (DIM == 3) s += (k - (tz * G::TZ - 1)) * G::FX * G::FY;
 #pragma unroll
                 for (int q = 0; q < NQ; q++) out[q] += N.part[((size_t)t * NQMAX + q) * G::FOOT + s];
             }
+}
+
+// Node passes visit the nodes of the (field, tile) pairs that a non-empty pair's footprint reached at
+// this binning or the one before (k_tile_need): a node of any other pair was cleared when its pair
+// last was visited, or was never written, so it is skipped and stays zero. A node's own pair is the
+// tile holding it (the grid's last node row belongs to the last tile); the pairs whose footprints
+// can reach it are that tile and its neighbours.
+template <int DIM>
+__device__ __forceinline__ bool node_needed(const KGrid &g, const KBins &B, int fld, int i, int j, int k) {
+    using G = TileGeom<DIM>;
+    const int tx = min(i / G::TX, g.ntx - 1), ty = min(j / G::TY, g.nty - 1);
+    const int tz = DIM == 3 ? min(k / G::TZ, g.ntz - 1) : 0;
+    return B.needAny[fld * g.tilesTotal + g.tileBase + (tz * g.nty + ty) * g.ntx + tx] != 0;
 }
 
 // node pass after P2G #1: mass, momentum, count and v = p/m (MatVelocityField.cpp:158-164,239-251)
@@ -815,6 +855,7 @@ __global__ void k_node_mass(KGrid g0, KBins B, KNodes N) {
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
+    if (!node_needed<DIM>(g, B, fld, i, j, k)) return;
     double a[NQ_MASS];
     gather_partials<DIM, NQ_MASS>(g, B, N, fld, i, j, k, a);
     long long nn = g.nstride;
@@ -1318,6 +1359,7 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
+    if (!node_needed<DIM>(g, B, fld, i, j, k)) return;
     double f[NQ_FORCE];
     gather_partials<DIM, NQ_FORCE>(g, B, N, fld, i, j, k, f);
     long long nn = g.nstride;
@@ -1536,6 +1578,7 @@ __global__ void k_node_mom2(KGrid g0, KBins B, KNodes N) {
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
+    if (!node_needed<DIM>(g, B, fld, i, j, k)) return;
     double a[NQ_MOM];
     gather_partials<DIM, NQ_MOM>(g, B, N, fld, i, j, k, a);
     long long nn = g.nstride;
@@ -1857,6 +1900,7 @@ __global__ void k_node_contact(KGrid g0, KBins B, KNodes N) {
     long long r = n / g.nnx;
     int j = (int)(r % g.nny);
     int k = (int)(r / g.nny);
+    if (!node_needed<DIM>(g, B, fld, i, j, k)) return;
     double a[NQ_CONTACT];
     gather_partials<DIM, NQ_CONTACT>(g, B, N, fld, i, j, k, a);
     long long nn = g.nstride;
@@ -2430,7 +2474,6 @@ void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hi
 }
 
 void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s) {
-    (void)g;
     int n = b.nkeys + 1;          // count[nkeys] stays zero, so cellStart[nkeys] = total
     int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(NT), 0, s, b.count, b.cellStart, b.blockSums, n);
@@ -2440,6 +2483,8 @@ void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, 
     hipLaunchKernelGGL(k_bin_sort_cells, dim3(blocks_for(b.nkeys, NT)), dim3(NT), 0, s, b);
     hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(b.ntall, NT)), dim3(NT), 0, s, b,
                        dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC);
+    if (dim == 3) hipLaunchKernelGGL(k_tile_need<3>, dim3(blocks_for(b.ntall, NT)), dim3(NT), 0, s, g, b);
+    else hipLaunchKernelGGL(k_tile_need<2>, dim3(blocks_for(b.ntall, NT)), dim3(NT), 0, s, g, b);
 }
 
 static int g_cus = 256;
