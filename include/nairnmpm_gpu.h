@@ -77,6 +77,7 @@ typedef struct MpmGridDesc {
 } MpmGridDesc;
 
 #define MPMGPU_MAX_MATERIALS 16
+#define MPMGPU_MAX_XPIC_ORDER 32
 
 /* Material constants after VerifyAndLoadProperties, in the reference's specific units. */
 typedef struct MpmMaterial {
@@ -122,6 +123,19 @@ typedef struct MpmRunParams {
     double gravity[3];         /* <Gravity> BodyX/Y/ZForce, mm/s^2: BodyForce gforce (BodyForce.cpp:76-95) */
     double contactNormal[3];   /* normals method 4: the unit normal from Polar/Azimuth
                                   (MeshInfo::SetContactNormal, MeshInfo.cpp:1211-1227) */
+    /* The PeriodicXPIC custom task (Custom_Tasks/PeriodicXPIC.cpp): in the steps it selects the
+       particle update is XPIC(k) (xpicFMPM = 0: XPICExtrapolationTask.cpp, MatPoint3D.cpp:152-170)
+       or FMPM(k) (xpicFMPM = 1, which also takes the strain updates' grid velocities from the
+       extrapolation, UpdateStrainsFirstTask.cpp:106-114), k = xpicOrder; 0 = FLIP only. The task
+       selects every xpicPeriodSteps-th step when that is > 0, else steps by time, one each
+       xpicPeriodTime seconds (periodicCFL already multiplied by the CFL time step), as
+       PeriodicXPIC::PrepareForStep does (PeriodicXPIC.cpp:283-312). Single material; not in
+       batches, the solve server or mpmgpu_solve_static; order at most MPMGPU_MAX_XPIC_ORDER. */
+    int32_t xpicOrder;
+    int32_t xpicFMPM;
+    int32_t xpicPeriodSteps;
+    int32_t xpicReserved;      /* 0 */
+    double xpicPeriodTime;
 } MpmRunParams;
 
 /* Counters reported back from the device (ResetElementsTask.cpp:71-151 warnings) */

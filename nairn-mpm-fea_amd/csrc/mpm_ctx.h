@@ -83,6 +83,12 @@ struct MpmCtx {
     std::vector<MpmParticleLoad> loads;
     std::vector<MpmParticleTraction> tractions;
     double *tbuf = nullptr;          // two step start time slots (by position parity)
+    // PeriodicXPIC schedule, kept on the host: the order of the step being enqueued (0: FLIP), the
+    // next step number and time that run with run.xpicOrder, and the start time of the next step
+    // to enqueue (one addition of the time step per step)
+    int xpicStep = 0;
+    int64_t xpicNextStep = 0;
+    double xpicNextTime = 0., xpicClock = 0.;
     float lastStepMs = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // kernel timing: events around each kernel group of a step, on this context's stream

@@ -62,8 +62,38 @@ static hipError_t mark_all_needed(MpmCtx *c) {
     return hipMemsetAsync(c->kb.needAny, 1, (size_t)c->kb.ntall * sizeof(int), c->stream);
 }
 
-// the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108); with
-// profiling on, an event is recorded between kernel groups
+// PeriodicXPIC::PrepareForStep and StepCalculation (PeriodicXPIC.cpp:283-312,364-373): the order
+// of step number st (1 for the first step). The reference decides a step's order before the first
+// step (step 0, time 0) and at the end of the step before it, so the step tested is st and the
+// time tested is dt for the first step and the step's start time after that; a step that gets
+// the order moves the next one on by the period (steps), or to the tested time plus the period.
+static void xpic_schedule(MpmCtx *c, int64_t st) {
+    const MpmRunParams &r = c->run;
+    c->xpicStep = 0;
+    if (r.xpicOrder <= 0) return;
+    if (st == 1) {
+        c->xpicNextStep = r.xpicPeriodSteps;
+        c->xpicNextTime = r.xpicPeriodTime;
+    }
+    if (r.xpicPeriodSteps > 0) {
+        if (st >= c->xpicNextStep) {
+            c->xpicStep = r.xpicOrder;
+            c->xpicNextStep += r.xpicPeriodSteps;
+        }
+    } else if (r.xpicPeriodTime > 0.) {
+        const double t = st == 1 ? 0. + r.timestep : c->xpicClock;
+        if (t >= c->xpicNextTime) {
+            c->xpicStep = r.xpicOrder;
+            c->xpicNextTime = t + r.xpicPeriodTime;
+        }
+    }
+}
+
+// every step of the run has the same order, so whole steps can be captured in a graph
+static bool xpic_uniform(const MpmCtx *c) { return c->run.xpicOrder <= 0 || c->run.xpicPeriodSteps == 1; }
+
+// the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108) with the
+// XPIC order in c->xpicStep; with profiling on, an event is recorded between kernel groups
 void mpmi::enqueue_step(MpmCtx *c) {
     hipStream_t s = c->stream;
     set_positions(c);
@@ -91,13 +121,21 @@ void mpmi::enqueue_step(MpmCtx *c) {
         vel_bcs(BC_PASS_MASS_MOMENTUM);
         done(MPMGPU_KG_CONTACT);
     }
+    // FMPM(k > 1) strain updates take the extrapolated grid velocities (UpdateStrainsFirstTask.cpp:106-114)
+    const int xo = c->xpicStep;
+    const bool fmpm = c->run.xpicFMPM != 0;
+    auto xpic = [&](bool forcePass) {
+        launch_xpic(c->dim, c->kg, c->kp, c->kb, c->kn, c->kbc, xo, fmpm, forcePass, par, s);
+    };
+    if (fmpm && xo > 1) xpic(false);
     launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
     done(MPMGPU_KG_USF);
     if (c->kld.nl) launch_particle_loads(c->kg, c->kld, par, s);    // InitializationTask's pFext
     launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, par, s);
     done(MPMGPU_KG_P2G_FORCE);
     launch_node_force(c->dim, c->kg, c->kb, c->kn, s);
-    vel_bcs(mm ? BC_PASS_GRID_FORCES : BC_PASS_FORCES_MOMENTA);
+    // the UPDATE_MOMENTUM pass is left out in XPIC and FMPM steps of order > 1 (NodalVelBC.cpp:365-370)
+    vel_bcs(mm || xo > 1 ? BC_PASS_GRID_FORCES : BC_PASS_FORCES_MOMENTA);
     done(MPMGPU_KG_NODE_FORCE);
     if (mm) {      // UpdateMomentaTask: contact at UPDATE_MOMENTUM_CALL
         launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_UPDATE_MOMENTUM, s);
@@ -105,7 +143,10 @@ void mpmi::enqueue_step(MpmCtx *c) {
         vel_bcs(BC_PASS_UPDATE_MOMENTUM);
         done(MPMGPU_KG_CONTACT);
     }
-    launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s);
+    // the particle update of XPIC(k > 1) and FMPM(k > 1) takes the extrapolated grid velocities
+    // (UpdateParticlesTask.cpp:66-79)
+    if (xo > 1) xpic(true);
+    launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, xo == 0 ? 0 : (fmpm ? xo : -xo), s);
     done(MPMGPU_KG_G2P_P2G);
     launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
     if (!mm) vel_bcs(BC_PASS_UPDATE_STRAINS_LAST);
@@ -118,6 +159,7 @@ void mpmi::enqueue_step(MpmCtx *c) {
         vel_bcs(BC_PASS_UPDATE_STRAINS_LAST);
         done(MPMGPU_KG_CONTACT);
     }
+    if (fmpm && xo > 1) xpic(false);
     launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
     done(MPMGPU_KG_USL_RESET);
     launch_bin_finish(c->dim, c->kg, c->kp, c->kb, s);
@@ -184,6 +226,10 @@ const char *mpmi::check_run(const MpmGridDesc *grid, const MpmRunParams *run) {
     if (run->shape != MPMGPU_SHAPE_UGIMP && run->shape != MPMGPU_SHAPE_LCPDI && run->shape != MPMGPU_SHAPE_LINEAR)
         return "shape functions must be classic (0), uGIMP (1) or lCPDI (10) on the GPU path";
     if (run->mpmMethod != 2) return "only the USAVG method is implemented";
+    if (run->xpicOrder < 0 || run->xpicOrder > MPMGPU_MAX_XPIC_ORDER)
+        return "the XPIC or FMPM order must be 0 (FLIP) to 32";
+    if (run->xpicOrder > 0 && run->xpicPeriodSteps <= 0 && !(run->xpicPeriodTime > 0.))
+        return "an XPIC or FMPM order needs a period in steps or a period time above 0";
     return nullptr;
 }
 
@@ -463,8 +509,9 @@ mpmi::GridArrays mpmi::grid_arrays(const MpmCtx *c, int nmat) {
     const long long nc = c->dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
     const long long nn = g.nstride;
     GridArrays a;
-    // per field: mass, pk, vk, ftot (+ contact volume, displacement, gradient in multimaterial mode)
-    const int perField = 1 + 3 + 3 + 3 + (nf > 1 ? 1 + 3 + 3 : 0);
+    // per field: mass, pk, vk, ftot (+ contact volume, displacement, gradient in multimaterial mode;
+    // + the XPIC increment with XPIC or FMPM of order > 1)
+    const int perField = 1 + 3 + 3 + 3 + (nf > 1 ? 1 + 3 + 3 : 0) + (c->run.xpicOrder > 1 ? 3 : 0);
     a.nodeBytes = (size_t)nf * nn * perField * sizeof(double);
     a.cntBytes = (size_t)nf * nn * sizeof(int);
     a.partBytes = (size_t)nf * g.tilesTotal * NQMAX * foot * sizeof(double);
@@ -497,6 +544,7 @@ void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, i
         c->kn.cdisp = nodeMem + 11 * fnn;
         c->kn.cgrad = nodeMem + 14 * fnn;
     }
+    c->kn.vprev = c->run.xpicOrder > 1 ? nodeMem + (nf > 1 ? 17 : 10) * fnn : nullptr;
     c->sbuf = sbuf;
     c->ks.leftGrid = sbuf;
     c->ks.nanCount = sbuf + 1;
@@ -589,6 +637,7 @@ const char *mpmi::single_grid(MpmCtx *c, const MpmGridDesc *grid, const MpmMater
     if (const char *m = check_run(grid, run)) return m;
     int nf = run->multiMaterial ? run->numFields : 1;
     if (nf < 1 || nf > MPMGPU_MAX_MATERIALS) return "bad number of material velocity fields";
+    if (nf > 1 && run->xpicOrder > 0) return "XPIC and FMPM are not available in multimaterial mode on this path";
     if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return "bad material count";
     if (nf > 1) {
         if (run->contactLaw < 0 || run->contactLaw >= nmat ||
@@ -644,6 +693,7 @@ int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64
     const long long nodes = (g.nnodes + 255) / 256 * 256;        // as laid out in a batch
     const long long nk = (long long)g.ntiles * nc;
     int64_t bytes = nodes * (10 * sizeof(double) + sizeof(int));   // mass, pk, vk, ftot, cnt
+    if (run->xpicOrder > 1) bytes += nodes * 3 * (int64_t)sizeof(double);   // the XPIC increment
     bytes += (int64_t)g.ntiles * NQMAX * foot * sizeof(double);    // tile partials
     bytes += (int64_t)bin_words(nk, g.ntiles) * sizeof(int);            // bins
     bytes += n * PARTICLE_BYTES;
@@ -668,6 +718,7 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
         if (d.grid.analysis != analysis || d.run.shape != shape)
             return fail_global("all instances of a batch need the same analysis type and shape functions");
         if (d.run.multiMaterial) return fail_global("multimaterial instances cannot be batched");
+        if (d.run.xpicOrder > 0) return fail_global("instances with XPIC or FMPM cannot be batched");
         if (d.firstMaterial < 0 || d.numMaterials < 1 || d.firstMaterial + d.numMaterials > nmat)
             return fail_global("instance material range outside the material table");
         for (int m = d.firstMaterial; m < d.firstMaterial + d.numMaterials; m++) {
@@ -805,6 +856,7 @@ int mpmi::prepare_particles(MpmCtx *c) {
     c->steps = 0;
     c->mtime = 0.;
     c->bcTime = 0.;
+    c->xpicClock = 0.;
     return put_bc_time(c);
 }
 
@@ -1124,8 +1176,10 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
         return c->fail("tasks must run in the step order (expected task " + std::to_string(c->nextTask) + ")");
     hipStream_t s = c->stream;
     set_positions(c);
+    const bool fmpm = c->run.xpicFMPM != 0;
     switch (task) {
         case MPMGPU_TASK_INITIALIZATION:          // natural coordinates are found in each pass
+            xpic_schedule(c, c->steps + 1);
             if (c->kld.nl) launch_particle_loads(c->kg, c->kld, c->parity, s);
             break;
         case MPMGPU_TASK_MASS_MOMENTUM:
@@ -1141,7 +1195,11 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             }
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_MASS_MOMENTUM, c->parity, s);
             break;
-        case MPMGPU_TASK_UPDATE_STRAINS_FIRST: launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s); break;
+        case MPMGPU_TASK_UPDATE_STRAINS_FIRST:
+            if (fmpm && c->xpicStep > 1)
+                launch_xpic(c->dim, c->kg, c->kp, c->kb, c->kn, c->kbc, c->xpicStep, true, false, c->parity, s);
+            launch_usf(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, s);
+            break;
         case MPMGPU_TASK_GRID_FORCES: launch_p2g_force(c->dim, c->kg, c->kp, c->kb, c->kn, c->kld, c->parity, s); break;
         case MPMGPU_TASK_POST_FORCES: break;          // restore and update in one node pass
         case MPMGPU_TASK_UPDATE_MOMENTA:
@@ -1151,9 +1209,15 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
                 launch_contact_normals(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, CONTACT_UPDATE_MOMENTUM, s);
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_MOMENTUM, s);
             }
-            if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_MOMENTUM, c->parity, s);
+            if (c->kbc.nb && c->xpicStep <= 1)
+                launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_MOMENTUM, c->parity, s);
             break;
-        case MPMGPU_TASK_UPDATE_PARTICLES: launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, s); break;
+        case MPMGPU_TASK_UPDATE_PARTICLES: {
+            const int xo = c->xpicStep;
+            if (xo > 1) launch_xpic(c->dim, c->kg, c->kp, c->kb, c->kn, c->kbc, xo, fmpm, true, c->parity, s);
+            launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, xo == 0 ? 0 : (fmpm ? xo : -xo), s);
+            break;
+        }
         case MPMGPU_TASK_UPDATE_STRAINS_LAST:
             launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
             if (c->kg.nfields > 1) {
@@ -1163,6 +1227,8 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
                 launch_contact(c->dim, c->kg, c->kn, CONTACT_UPDATE_STRAINS_LAST, s);
             }
             if (c->kbc.nb) launch_vel_bcs(c->kg, c->kn, c->kbc, BC_PASS_UPDATE_STRAINS_LAST, c->parity, s);
+            if (fmpm && c->xpicStep > 1)
+                launch_xpic(c->dim, c->kg, c->kp, c->kb, c->kn, c->kbc, c->xpicStep, true, false, c->parity, s);
             launch_usl_reset(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, c->ks, s);
             break;
         case MPMGPU_TASK_RESET_ELEMENTS:
@@ -1171,6 +1237,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
             c->steps++;
             c->mtime += c->run.timestep;
             c->bcTime += c->run.timestep;
+            c->xpicClock += c->run.timestep;
             break;
         default: return c->fail("unknown task id");
     }
@@ -1191,10 +1258,12 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
     int done = 0;
     const int G = c->graphSteps;
-    if (nsteps >= G && !c->profiling) {
+    // a schedule that gives some steps XPIC and others not runs step by step
+    if (nsteps >= G && !c->profiling && xpic_uniform(c)) {
         int par = c->parity;
         if (!c->graph[par]) {
             hipGraph_t gr;
+            c->xpicStep = c->run.xpicOrder > 0 ? c->run.xpicOrder : 0;
             HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < G; k++) mpmi::enqueue_step(c);
             HIPCHECK(c, hipStreamEndCapture(c->stream, &gr));
@@ -1205,12 +1274,18 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
         }
         for (; done + G <= nsteps; done += G) {
             HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
+            for (int k = 1; k <= G; k++) {      // the schedule of the graph's steps, all of one order
+                xpic_schedule(c, c->steps + done + k);
+                c->xpicClock += c->run.timestep;
+            }
             c->sinceSort += G;
             if (c->sortInterval > 0 && c->sinceSort >= c->sortInterval && mpmi::sort_rows(c)) return 1;
         }
     }
     for (; done < nsteps; done++) {
+        xpic_schedule(c, c->steps + done + 1);
         mpmi::enqueue_step(c);
+        c->xpicClock += c->run.timestep;
         c->sinceSort++;
         if (c->sortInterval > 0 && c->sinceSort >= c->sortInterval && mpmi::sort_rows(c)) return 1;
         if (c->profiling && (!c->ev.err.empty() || c->ev.live() >= MAX_LIVE_EVENTS)) {
@@ -1257,8 +1332,8 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, STATS_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                                c->stream));
     const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort;
-    const int64_t steps = c->steps;
-    const double mtime = c->mtime, bcTime = c->bcTime;
+    const int64_t steps = c->steps, xpicNextStep = c->xpicNextStep;
+    const double mtime = c->mtime, bcTime = c->bcTime, xpicNextTime = c->xpicNextTime, xpicClock = c->xpicClock;
     int best = oldG;
     double bestMs = 1e300;
     static const int cand[] = {2, 4, 8, 16, 32};
@@ -1288,6 +1363,9 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     c->steps = steps;
     c->mtime = mtime;
     c->bcTime = bcTime;
+    c->xpicNextStep = xpicNextStep;
+    c->xpicNextTime = xpicNextTime;
+    c->xpicClock = xpicClock;
     if (put_bc_time(c)) return 1;
     mpmgpu_set_graph_steps(c, best);
     if (graphSteps) *graphSteps = best;

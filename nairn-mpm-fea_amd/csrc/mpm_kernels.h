@@ -101,8 +101,9 @@ struct KBins {
 };
 
 // launches of the tile kernels in one step, each with its own cursors (cleared when the list is built)
+// (the XPIC iterations share one slot, whose cursors each iteration's node pass clears again)
 enum { TILE_SLOT_MASS, TILE_SLOT_CONTACT_FIRST, TILE_SLOT_USF, TILE_SLOT_FORCE, TILE_SLOT_G2P,
-       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOT_FORCE_LOADS, TILE_SLOTS };
+       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOT_FORCE_LOADS, TILE_SLOT_XPIC, TILE_SLOTS };
 #define TILE_SLOT_STRIDE 32
 
 struct KNodes {
@@ -110,6 +111,7 @@ struct KNodes {
     double *pk;                // 3*nnodes (component-major)
     double *vk;                // 3*nnodes
     double *ftot;              // 3*nnodes
+    double *vprev;             // 3*nnodes: the XPIC series' latest increment (null without XPIC)
     int *cnt;                  // nnodes: particle-node pairs (numberPoints)
     double *part;              // per-(field, tile) partial sums, nfields * ntiles * NQMAX * FOOT
     // multimaterial contact extrapolations: volume, mass-weighted displacement, volume gradient
@@ -175,7 +177,9 @@ void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const K
 void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
                       int par, hipStream_t s);
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
-void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+// the particle update with the USL momentum scatter; mode 0 moves the particles by FLIP, mode k > 0
+// by FMPM(k) and mode -k by XPIC(k) (k > 1 takes the grid velocities of launch_xpic)
+void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, int mode,
                     hipStream_t s);
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
 void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
@@ -211,9 +215,18 @@ struct KBcs {
 // passes of NodalVelBC::GridVelocityConditions; GRID_FORCES runs the force pass and the momentum
 // update of the nodes with conditions, FORCES_MOMENTA adds the UPDATE_MOMENTUM pass in the same
 // launch (single-material mode, where no contact pass comes between)
+// The XPIC passes zero the held components of the series increment vprev (and take them back
+// out of vk); XPIC_FORCE, the particle-update pass of XPIC, also moves the lumped reaction into ftot.
 enum { BC_PASS_MASS_MOMENTUM = 0, BC_PASS_GRID_FORCES, BC_PASS_UPDATE_MOMENTUM, BC_PASS_FORCES_MOMENTA,
-       BC_PASS_UPDATE_STRAINS_LAST };
+       BC_PASS_UPDATE_STRAINS_LAST, BC_PASS_XPIC, BC_PASS_XPIC_FORCE };
 void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, int par, hipStream_t s);
+// XPIC(order) or FMPM(order) grid velocities into vk (order > 1): the series started from the
+// lumped velocities (FMPM: pk/m; XPIC: pk/m at the start of the momentum update, plus ftot dt/m),
+// then order - 1 increments, each a particle gather and scatter, a node pass and, with velocity
+// conditions (bc.nb > 0), the XPIC pass of the conditions (forcePass: the particle-update pass of
+// XPIC, which keeps the reactions in ftot)
+void launch_xpic(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KBcs &bc,
+                 int order, bool fmpm, bool forcePass, int par, hipStream_t s);
 
 // External particle loads of one simulation: map[upload index] is -1 or the loaded particle's
 // slot q, whose loads are entries start[q]..start[q+1]-1 in list order; fext holds the step's

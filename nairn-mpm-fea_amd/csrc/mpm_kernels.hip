@@ -58,6 +58,7 @@ __device__ __forceinline__ KNodes field_view(const KGrid &g, const KNodes &N, in
     v.pk = N.pk + 3 * f * nn + b;
     v.vk = N.vk + 3 * f * nn + b;
     v.ftot = N.ftot + 3 * f * nn + b;
+    if (N.vprev) v.vprev = N.vprev + 3 * f * nn + b;
     v.cnt = N.cnt + f * nn + b;
     if (N.cvol) {
         v.cvol = N.cvol + f * nn + b;
@@ -516,7 +517,6 @@ __global__ void k_tile_list(KBins B, int nc) {
 // neighbours is non-empty; needAny = needed now or at the previous binning. A stopped run needs none.
 template <int DIM>
 __global__ void k_tile_need(KGrid g0, KBins B) {
-    using G = TileGeom<DIM>;
     const int tall = blockIdx.x * blockDim.x + threadIdx.x;
     if (tall >= B.ntall) return;
     const int fld = tall / g0.tilesTotal;
@@ -1389,7 +1389,8 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
 template <int DIM, int SH, bool DAMP>
-__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
+__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N,
+                                                                                  int mode) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NCOR = DIM == 3 ? 8 : 4;
@@ -1486,17 +1487,40 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2
                         }
                 }
                 double vnew[3] = {0., 0., 0.};
-                // MatPoint3D::MoveParticle, FLIP (MatPoint3D.cpp:104-160): Adamp0 = alpha_p v +
-                // alpha_g vm (the DAMP instance; zero damping gives delV = Sacc dt)
+                // MatPoint3D::MoveParticle (MatPoint3D.cpp:104-194): Adamp0 = alpha_p v + alpha_g vm
+                // (the DAMP instance; zero damping gives delV = Sacc dt for FLIP)
+                if (mode == 0) {
+                    // FLIP: vm = S(v - a dt), v += (Sa - Adamp0) dt, x += (vm + delV/2) dt (:104-160)
 #pragma unroll
-                for (int c = 0; c < DIM; c++) {
-                    double vm = Svk[c] - Sacc[c] * dt;
-                    double delV = DAMP ? (Sacc[c] - (vold[c] * g.particleDamping + vm * g.gridDamping)) * dt
-                                       : Sacc[c] * dt;
-                    vnew[c] = vold[c] + delV;
-                    P.f[MPF_VEL_X + c][p] = vnew[c];
-                    double delXRate = vm + 0.5 * delV;
-                    P.posn[c][p] = pos[c] + delXRate * dt;
+                    for (int c = 0; c < DIM; c++) {
+                        double vm = Svk[c] - Sacc[c] * dt;
+                        double delV = DAMP ? (Sacc[c] - (vold[c] * g.particleDamping + vm * g.gridDamping)) * dt
+                                           : Sacc[c] * dt;
+                        vnew[c] = vold[c] + delV;
+                        P.f[MPF_VEL_X + c][p] = vnew[c];
+                        double delXRate = vm + 0.5 * delV;
+                        P.posn[c][p] = pos[c] + delXRate * dt;
+                    }
+                } else {
+                    // FMPM(k) (mode k): vm = S v+(k), v = vm - Adamp0 dt, x += (v(n) + v(n+1)) dt/2
+                    // (:122-137); XPIC(k) (mode -k): vm = S(v(k) + a dt), v = vm - Adamp0 dt,
+                    // x += (vm + delV/2) dt (:152-170)
+#pragma unroll
+                    for (int c = 0; c < DIM; c++) {
+                        const double vm = Svk[c];
+                        const double adamp = DAMP ? vold[c] * g.particleDamping + vm * g.gridDamping : 0.;
+                        if (mode > 0) {
+                            vnew[c] = vm + adamp * -dt;
+                            const double delXRate = vold[c] + vnew[c];
+                            P.posn[c][p] = pos[c] + delXRate * (0.5 * dt);
+                        } else {
+                            vnew[c] = vm - adamp * dt;
+                            const double delV = vnew[c] - vold[c];
+                            const double delXRate = vm + 0.5 * delV;
+                            P.posn[c][p] = pos[c] + delXRate * dt;
+                        }
+                        P.f[MPF_VEL_X + c][p] = vnew[c];
+                    }
                 }
                 q4[0] = mp;
                 q4[1] = vnew[0]; q4[2] = vnew[1]; q4[3] = vnew[2];
@@ -1587,6 +1611,225 @@ __global__ void k_node_mom2(KGrid g0, KBins B, KNodes N) {
     if (m != 0.) {
         double s = 1. / m;
         Nf.vk[n] = s * a[0]; Nf.vk[nn + n] = s * a[1]; Nf.vk[2 * nn + n] = s * a[2];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// XPIC(k) and FMPM(k) grid velocities (XPICExtrapolationTask::Execute, XPICExtrapolationTask.cpp:50-164)
+// for one material velocity field per material: v(k) = sum_{r<k} (P(I - A))^r v0 with
+// A = diag(1/m) S^T Mp S, built as vk = v0 + Delta v(2) + ... + Delta v(k), each increment
+// Delta v(r) = (I - A) Delta v(r-1) with the held components of the conditions zeroed.
+
+// InitializeXPICData -> MatVelocityField::XPICSupport(INITIALIZE_XPIC) (MatVelocityField.cpp:321-345):
+// FMPM starts from v0 = pk/m (the momenta the caller's pass left), XPIC from the velocities
+// before the momentum update, v0 = (pk - ftot dt)/m, with ftot dt/m added to vk so the result
+// is v(k) + a dt. vprev = v0 on every node with particles.
+template <int DIM>
+__global__ void k_xpic_init(KGrid g0, KBins B, KNodes N, int fmpm) {
+    long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nall >= g0.nstride * g0.nfields) return;
+    const int fld = (int)(nall / g0.nstride);
+    const long long ng = nall - (long long)fld * g0.nstride;
+    const KGrid g = node_grid(g0, ng);
+    const long long n = ng - g.nodeBase;
+    if (n >= g.nnodes) return;
+    const KNodes Nf = field_view(g, N, fld);
+    if (Nf.cnt[n] <= 0) return;
+    const long long nn = g.nstride;
+    const double m = Nf.mass[n], dt = g.timestep, sc = 1. / m;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double p = Nf.pk[c * nn + n];
+        double vp, vs;
+        if (fmpm) {
+            vp = sc * p;
+            vs = vp;
+        } else {
+            const double f = Nf.ftot[c * nn + n];
+            vp = (p + f * -dt) * sc;
+            vs = vp + f * (dt / m);
+        }
+        Nf.vprev[c * nn + n] = vp;
+        Nf.vk[c * nn + n] = vs;
+    }
+}
+
+// Particle loop of one increment (XPICExtrapolationTask::XPICDoubleLoop, XPICExtrapolationTask.cpp:182-211):
+// node i gains sum_p mp S_ip (sum_j S_jp vprev_j), the particle's gather of the last increment
+// scattered as a momentum; the node pass divides by m_i. Shape functions are those of the step's
+// start, as in every pass of the step.
+template <int DIM, int SH>
+__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_xpic_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
+    using G = TileGeom<DIM>;
+    constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
+    constexpr int NCOR = DIM == 3 ? 8 : 4;
+    constexpr int SUB = CP ? NT / NCOR : NT / 2;
+    __shared__ int s_cs[G::NC + 1];
+    __shared__ double s_n[3][G::FOOT];         // vprev
+    __shared__ double s_S[CP ? 1 : DIM * 4][SUB + 1];
+    __shared__ double s_ct[CP ? SUB : 1][NCOR][DIM][4];
+    __shared__ double s_q[4][SUB + 1];
+    __shared__ double s_acc[NW][NQ_MOM][WaveFoot<DIM>::SIZE];
+    auto tile = [&](const int tall) {
+        const int fld = tall / g0.tilesTotal;
+        const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
+        const int t = tall - fld * g0.tilesTotal - g.tileBase;
+        const KNodes Nf = field_view(g, N, fld);
+        load_tile_starts<DIM>(B, tall, s_cs);
+        const int tb = s_cs[0], te = s_cs[G::NC];
+        if (tb == te) return;
+        int ox, oy, oz;
+        tile_origin<DIM>(g, t, ox, oy, oz);
+        {
+            // footprint increments (zero outside the grid and on nodes without particles)
+            const long long nn = g.nstride;
+            for (int s = threadIdx.x; s < G::FOOT; s += NT) {
+                const long long n = foot_node<DIM>(g, s, ox, oy, oz);
+                const bool on = n >= 0 && Nf.cnt[n] > 0;
+#pragma unroll
+                for (int c = 0; c < 3; c++) s_n[c][s] = on ? Nf.vprev[c * nn + n] : 0.;
+            }
+        }
+        zero_acc<DIM, NQ_MOM>(s_acc);
+        __syncthreads();
+        for (int c0 = tb; c0 < te; c0 += NT) {
+            const int c1 = min(c0 + NT, te);
+            const int k = c0 + threadIdx.x;
+            double S[3][4], q4[4];
+            CpdiCorners cc;
+            if (k < c1) {
+                const int p = B.order[k];
+                const int e0 = P.elem[p];
+                const double pos[3] = {P.f[MPF_POS_X][p], P.f[MPF_POS_Y][p], DIM == 3 ? P.f[MPF_POS_Z][p] : 0.};
+                const double mp = P.f[MPF_MASS][p];
+                int ei, ej, ek;
+                elem_ijk(g, e0, ei, ej, ek);
+                const int lx = ei - ox, ly = ej - oy, lz = DIM == 3 ? ek - oz : 0;
+                double u[3] = {0., 0., 0.};
+                if (CP) {
+                    cpdi_particle<DIM>(g, P, p, ei, ej, ek, cc);
+                    constexpr double ws = DIM == 3 ? 0.125 : 0.25;
+                    double sum[3] = {0., 0., 0.};
+#pragma unroll
+                    for (int c = 0; c < NCOR; c++) {
+                        double vc[3];
+                        cpdi_corner_sum<DIM, 3>(cc.xi[c], cc.off[c], lx, ly, lz, s_n, vc);
+#pragma unroll
+                        for (int v = 0; v < 3; v++) sum[v] += vc[v];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 3; a++) u[a] = ws * sum[a];
+                } else {
+                    double xi[3], invd[3], dS[3][4];
+                    particle_xi<DIM>(g, ei, ej, ek, pos[0], pos[1], pos[2], xi, invd);
+                    particle_factors<DIM>(g, P, p, xi, S, dS);
+                    int w[3];
+                    double Sw[3][3], Dw[3][3];
+                    support_window<DIM>(xi, S, dS, invd, w, Sw, Dw);
+                    constexpr int NZ = DIM == 3 ? 3 : 1;
+#pragma unroll
+                    for (int jz = 0; jz < NZ; jz++)
+#pragma unroll
+                        for (int jy = 0; jy < 3; jy++)
+#pragma unroll
+                            for (int jx = 0; jx < 3; jx++) {
+                                const double fi = DIM == 3 ? Sw[0][jx] * (Sw[1][jy] * Sw[2][jz]) : Sw[0][jx] * Sw[1][jy];
+                                const int s = foot_slot<DIM>(lx + w[0] + jx - 1, ly + w[1] + jy - 1,
+                                                             DIM == 3 ? lz + w[2] + jz - 1 : 0);
+                                u[0] = fma(s_n[0][s], fi, u[0]);
+                                u[1] = fma(s_n[1][s], fi, u[1]);
+                                u[2] = fma(s_n[2][s], fi, u[2]);
+                            }
+                }
+                q4[0] = mp;
+                q4[1] = u[0]; q4[2] = u[1]; q4[3] = u[2];
+            }
+            for (int h = 0; h < NT / SUB; h++) {
+                const int r0 = c0 + h * SUB, r1 = min(r0 + SUB, c1);
+                if (r0 >= r1) break;
+                if ((int)(threadIdx.x / SUB) == h && k < c1) {
+                    const int i = threadIdx.x - h * SUB;
+                    if (CP) {
+#pragma unroll
+                        for (int c = 0; c < NCOR; c++) {
+                            double xo[3];
+#pragma unroll
+                            for (int a = 0; a < 3; a++) {
+                                xo[a] = cc.xi[c][a];
+                                asm volatile("" : "+v"(xo[a]));
+                            }
+                            cpdi_table<DIM>(xo, cc.off[c], s_ct[i][c]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < DIM; a++)
+#pragma unroll
+                            for (int o = 0; o < 4; o++) {
+                                const double v = S[a][o] >= 0. ? S[a][o] : 0.;
+                                s_S[a * 4 + o][i] = a == 0 ? v * q4[0] : v;
+                            }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) s_q[q][i] = q4[q];
+                }
+                __syncthreads();
+                wave_cells<DIM, NQ_MOM>(s_cs, r0, r1, s_acc,
+                    [&](int lq, int ix, int iy, int iz) {
+                        RecMass r;
+                        if (CP) {
+                            double gr[3];
+                            cpdi_table_node<DIM, false>(s_ct[lq], nullptr, ix, iy, iz, r.wx, gr);
+                            r.wy = 1.; r.wz = 1.;
+                        } else {
+                            r.wx = s_S[ix][lq]; r.wy = s_S[4 + iy][lq]; r.wz = DIM == 3 ? s_S[(DIM - 1) * 4 + iz][lq] : 1.;
+                        }
+                        r.mp = s_q[0][lq]; r.vx = s_q[1][lq]; r.vy = s_q[2][lq]; r.vz = s_q[3][lq];
+                        return r;
+                    },
+                    [&](const RecMass &r, double *acc, int, int, int) {
+                        const double fnmp = CP ? r.wx * r.mp : (DIM == 3 ? r.wx * (r.wy * r.wz) : r.wx * r.wy);
+                        acc[0] = fma(r.vx, fnmp, acc[0]);
+                        acc[1] = fma(r.vy, fnmp, acc[1]);
+                        acc[2] = fma(r.vz, fnmp, acc[2]);
+                    });
+                __syncthreads();
+            }
+        }
+        write_partials<DIM, NQ_MOM>(N, tall, s_acc);
+    };
+    for_each_tile(B, tile);
+}
+
+// Node pass of one increment (GetDeltaV and UpdateXStar, XPICExtrapolationTask.cpp:222-240;
+// MatVelocityField::XPICSupport GET_DELTAV and UPDATE_VSTAR, MatVelocityField.cpp:347-378):
+// vprev -= (the scatter's sum)/m, vk += vprev. The XPIC pass of the velocity conditions then zeroes
+// the held components of vprev and takes them back out of vk. Block 0 clears the cursors of the
+// XPIC tile slot for the next increment's scatter (the scatter before this pass has finished).
+template <int DIM>
+__global__ void k_xpic_node(KGrid g0, KBins B, KNodes N) {
+    if (blockIdx.x == 0 && threadIdx.x < 8) B.work[(TILE_SLOT_XPIC * 8 + threadIdx.x) * TILE_SLOT_STRIDE] = 0;
+    long long nall = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nall >= g0.nstride * g0.nfields) return;
+    const int fld = (int)(nall / g0.nstride);
+    const long long ng = nall - (long long)fld * g0.nstride;
+    const KGrid g = node_grid(g0, ng);
+    const long long n = ng - g.nodeBase;
+    if (n >= g.nnodes) return;
+    const KNodes Nf = field_view(g, N, fld);
+    if (Nf.cnt[n] <= 0) return;
+    int i = (int)(n % g.nnx);
+    long long r = n / g.nnx;
+    int j = (int)(r % g.nny);
+    int k = (int)(r / g.nny);
+    double a[NQ_MOM];
+    gather_partials<DIM, NQ_MOM>(g, B, N, fld, i, j, k, a);
+    const long long nn = g.nstride;
+    const double m = Nf.mass[n];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double d = Nf.vprev[c * nn + n] - a[c] / m;
+        Nf.vprev[c * nn + n] = d;
+        Nf.vk[c * nn + n] += d;
     }
 }
 
@@ -2758,26 +3001,26 @@ void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd
     else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
-void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, hipStream_t s) {
+void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, int mode, hipStream_t s) {
     const KBins tb = tile_bins(b, TILE_SLOT_G2P);
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     // damping has its own instance, so the undamped update keeps its registers
     if (g.damped) {
         if (dim == 3) {
-            if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd);
-            else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd);
+            if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, mode);
+            else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, mode);
         } else {
-            if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd);
-            else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd);
+            if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, mode);
+            else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, mode);
         }
         return;
     }
     if (dim == 3) {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd);
-        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, mode);
+        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, mode);
     } else {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd);
-        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, mode);
+        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, mode);
     }
 }
 
@@ -2908,6 +3151,19 @@ __global__ void k_vel_bcs(KGrid g, KNodes N, KBcs bc, int par) {
     const long long n = bc.node[b], nn = g.nstride;
     if (Nf.cnt[n] <= 0) return;
     const double m = Nf.mass[n], dt = g.timestep;
+    if (PASS == BC_PASS_XPIC || PASS == BC_PASS_XPIC_FORCE) {
+        // NodalVelBC::GridVelocityConditions for the XPIC passes (NodalVelBC.cpp:327-335) ->
+        // MatVelocityField::ZeroVelocityBC (MatVelocityField.cpp:523-534) on the increment
+        for (int e = e0; e < e1; e++) {
+            if (!bc.act[e]) continue;
+            const int a = bc.dir[e];
+            const double dotn = Nf.vprev[a * nn + n];
+            Nf.vprev[a * nn + n] = 0.;
+            Nf.vk[a * nn + n] -= dotn;
+            if (PASS == BC_PASS_XPIC_FORCE) Nf.ftot[a * nn + n] += -m * dotn / dt;
+        }
+        return;
+    }
     double p[3] = {Nf.pk[n], Nf.pk[nn + n], Nf.pk[2 * nn + n]};
     double fo[3] = {Nf.ftot[n], Nf.ftot[nn + n], Nf.ftot[2 * nn + n]};
     double *sv = bc.save + ((long long)f * bc.nb + b) * 3;
@@ -2939,7 +3195,30 @@ void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, 
         case BC_PASS_GRID_FORCES: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_GRID_FORCES>, grid, blk, 0, s, g, nd, bc, par); break;
         case BC_PASS_UPDATE_MOMENTUM: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_MOMENTUM>, grid, blk, 0, s, g, nd, bc, par); break;
         case BC_PASS_FORCES_MOMENTA: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_FORCES_MOMENTA>, grid, blk, 0, s, g, nd, bc, par); break;
+        case BC_PASS_XPIC: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_XPIC>, grid, blk, 0, s, g, nd, bc, par); break;
+        case BC_PASS_XPIC_FORCE: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_XPIC_FORCE>, grid, blk, 0, s, g, nd, bc, par); break;
         default: hipLaunchKernelGGL(k_vel_bcs<BC_PASS_UPDATE_STRAINS_LAST>, grid, blk, 0, s, g, nd, bc, par); break;
+    }
+}
+
+void launch_xpic(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KBcs &bc,
+                 int order, bool fmpm, bool forcePass, int par, hipStream_t s) {
+    const dim3 ng(blocks_for(g.nstride * g.nfields, NT)), blk(NT);
+    const KBins tb = tile_bins(b, TILE_SLOT_XPIC);
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) hipLaunchKernelGGL(k_xpic_init<3>, ng, blk, 0, s, g, b, nd, fmpm ? 1 : 0);
+    else hipLaunchKernelGGL(k_xpic_init<2>, ng, blk, 0, s, g, b, nd, fmpm ? 1 : 0);
+    for (int it = 2; it <= order; it++) {
+        if (dim == 3) {
+            if (cp) TILE_LAUNCH((k_xpic_p2g<3, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
+            else TILE_LAUNCH((k_xpic_p2g<3, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
+            hipLaunchKernelGGL(k_xpic_node<3>, ng, blk, 0, s, g, tb, nd);
+        } else {
+            if (cp) TILE_LAUNCH((k_xpic_p2g<2, MPMGPU_SHAPE_LCPDI>), tb, s, g, p, tb, nd);
+            else TILE_LAUNCH((k_xpic_p2g<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
+            hipLaunchKernelGGL(k_xpic_node<2>, ng, blk, 0, s, g, tb, nd);
+        }
+        if (bc.nb) launch_vel_bcs(g, nd, bc, forcePass && !fmpm ? BC_PASS_XPIC_FORCE : BC_PASS_XPIC, par, s);
     }
 }
 

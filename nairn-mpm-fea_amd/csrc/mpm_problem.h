@@ -63,6 +63,10 @@ struct MpmProblem {
     std::vector<MpmParticleLoad> loads;
     // <ParticleBCs> tractions in the reference's list order
     std::vector<MpmParticleTraction> tractions;
+    // <CustomTasks><Schedule name="PeriodicXPIC"> (PeriodicXPIC.cpp:23-55 defaults)
+    bool xpicTask = false;
+    int xpicOrderIn = 0, fmpmOrderIn = 0, xpicStepsIn = -1;
+    double xpicTimeIn = -1., xpicCFLIn = -1.;     // s, CFL factor
     // particles, in mpm[] order
     std::vector<double> d[MPF_NUM_FIELDS];
     std::vector<int32_t> iv[MPI_NUM_FIELDS];

@@ -398,6 +398,7 @@ int mpmgpu_server_submit(MpmServer *s, const MpmGridDesc *grid, const MpmMateria
     if (grid->analysis != s->p.analysis || run->shape != s->p.shape)
         return submit_fail(s, "request analysis type or shape functions differ from the server's");
     if (run->multiMaterial) return submit_fail(s, "multimaterial requests cannot be served");
+    if (run->xpicOrder > 0) return submit_fail(s, "XPIC and FMPM requests cannot be served");
     if (nmat < 1 || nmat > MPMGPU_MAX_MATERIALS) return submit_fail(s, "bad material count");
     if (steps < 1) return submit_fail(s, "a request needs at least one step");
     if (n < 1 || n > 0x7fffffffLL) return submit_fail(s, "bad particle count");

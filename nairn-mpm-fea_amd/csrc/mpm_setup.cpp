@@ -522,6 +522,35 @@ bool finalize(MpmProblem &p) {
             return false;
         }
     }
+    // PeriodicXPIC::Finalize and Initialize (PeriodicXPIC.cpp:160-252): the order, FMPM or XPIC,
+    // and the mechanics schedule; steps override times, a CFL factor sets the time from the
+    // mechanics time step
+    if (p.xpicTask) {
+        if (p.xpicOrderIn < 1 && p.fmpmOrderIn < 1) {
+            p.err = "PeriodicXPIC custom task must set XPIC or FMPM order to 1 or higher";
+            return false;
+        }
+        if (p.xpicOrderIn >= 1 && p.fmpmOrderIn >= 1) {
+            p.err = "PeriodicXPIC custom task must set XPIC or FMPM order, but not both";
+            return false;
+        }
+        p.run.xpicFMPM = p.fmpmOrderIn > 0 ? 1 : 0;
+        p.run.xpicOrder = p.fmpmOrderIn > 0 ? p.fmpmOrderIn : p.xpicOrderIn;
+        double ptime = p.xpicTimeIn, pcfl = p.xpicCFLIn;
+        int psteps = p.xpicStepsIn;
+        if (psteps > 0) ptime = pcfl = -1.;
+        else if (!(ptime > 0. || pcfl > 0.)) {
+            p.err = "PeriodicXPIC custom task did not activate any XPIC or FMPM calculations";
+            return false;
+        }
+        if (pcfl > 0.) ptime = pcfl * timeStepMinMechanics;
+        p.run.xpicPeriodSteps = psteps > 0 ? psteps : 0;
+        p.run.xpicPeriodTime = psteps > 0 ? 0. : ptime;
+        if (p.run.multiMaterial) {
+            p.err = "PeriodicXPIC in multimaterial mode (incremental contact) is not on this path";
+            return false;
+        }
+    }
     p.finalized = true;
     return true;
 }

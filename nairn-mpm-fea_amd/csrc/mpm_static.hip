@@ -129,6 +129,7 @@ int mpmgpu_solve_static(const MpmGridDesc *grid, const MpmMaterial *mats, int32_
     if (n < 0 || n > 0x7fffffffLL) return fail_global("bad particle count");
     if (nsteps < 0) return fail_global("bad step count");
     if (n > 0 && (!dfields || !ifields)) return fail_global("null particle rows");
+    if (run->xpicOrder > 0) return fail_global("XPIC and FMPM are not available in the static-scratch solve");
     MpmCtx c;
     if (const char *m = mpmi::single_grid(&c, grid, mats, nmat, run)) return fail_global(m);
     const mpmi::GridArrays a = mpmi::grid_arrays(&c, nmat);

@@ -760,6 +760,44 @@ struct Reader {
         return true;
     }
 
+    // <CustomTasks>: <Schedule name="PeriodicXPIC"> with <Parameter name="...">value</Parameter>
+    // (MPMReadHandler.cpp:1682-1690, SetCustomTasks.cpp:27-160, PeriodicXPIC::InputParam
+    // PeriodicXPIC.cpp:60-157: periodicTime in ms); other tasks are not on this path
+    bool customtasks(const XNode &ct) {
+        for (auto &sp : ct.kids) {
+            const XNode &sc = *sp;
+            if (sc.name != "Schedule") continue;
+            const char *nm = sc.attr("name");
+            const std::string task = nm ? nm : "";
+            if (task != "PeriodicXPIC") return fail("custom task " + task + " is not on this path");
+            if (p.xpicTask) return fail("more than one PeriodicXPIC task is not on this path");
+            p.xpicTask = true;
+            for (auto &pp : sc.kids) {
+                const XNode &pr = *pp;
+                if (pr.name != "Parameter") continue;
+                const char *an = pr.attr("name");
+                if (!an) return fail("Custom task parameter has no name.");
+                std::string key = an;
+                if (!key.empty() && key.back() >= '0' && key.back() <= '9') key.pop_back();   // parameter number
+                const char *v = pr.text.c_str();
+                if (key == "XPICOrder") p.xpicOrderIn = (int)strtol(v, nullptr, 10);
+                else if (key == "FMPMOrder") p.fmpmOrderIn = (int)strtol(v, nullptr, 10);
+                else if (key == "periodicSteps") p.xpicStepsIn = (int)strtol(v, nullptr, 10);
+                else if (key == "periodicTime") p.xpicTimeIn = strtod(v, nullptr) * 1.e-3;
+                else if (key == "periodicCFL") p.xpicCFLIn = strtod(v, nullptr);
+                else if (key == "verbose") continue;
+                else if (key == "DynamicConverge") {
+                    if (strtod(v, nullptr) > 0.) return fail("PeriodicXPIC DynamicConverge is not on this path");
+                } else if (key == "periodicTimeConduction" || key == "periodicCFLConduction" ||
+                           key == "periodicStepsConduction" || key == "periodicTimeDiffusion" ||
+                           key == "periodicCFLDiffusion" || key == "periodicStepsDiffusion")
+                    return fail("PeriodicXPIC for conduction and diffusion is not on this path");
+                else return fail("Unrecognized custom task parameter: '" + std::string(an) + "'");
+            }
+        }
+        return true;
+    }
+
     bool run(const XNode &root) {
         if (root.name != "JANFEAInput") return fail("root element must be <JANFEAInput>");
         p.run.shape = MPMGPU_SHAPE_LINEAR;       // ElementBase::useGimp default until <GIMP>
@@ -795,7 +833,11 @@ struct Reader {
                 }
                 continue;
             }
-            if (n == "CrackList" || n == "Thermal" || n == "CustomTasks")
+            if (n == "CustomTasks") {
+                if (!customtasks(*k)) return false;
+                continue;
+            }
+            if (n == "CrackList" || n == "Thermal")
                 if (!k->kids.empty() || !k->text.empty())
                     return fail("<" + n + "> is not on this path");
         }

@@ -75,7 +75,9 @@ class RunParams(ctypes.Structure):
                 ("normalsMethod", ctypes.c_int32), ("deleteLeaving", ctypes.c_int32),
                 ("maxLeaving", ctypes.c_int32), ("numFields", ctypes.c_int32),
                 ("gridDamping", ctypes.c_double), ("particleDamping", ctypes.c_double),
-                ("gravity", ctypes.c_double * 3), ("contactNormal", ctypes.c_double * 3)]
+                ("gravity", ctypes.c_double * 3), ("contactNormal", ctypes.c_double * 3),
+                ("xpicOrder", ctypes.c_int32), ("xpicFMPM", ctypes.c_int32), ("xpicPeriodSteps", ctypes.c_int32),
+                ("xpicReserved", ctypes.c_int32), ("xpicPeriodTime", ctypes.c_double)]
 
 
 class InstanceDesc(ctypes.Structure):

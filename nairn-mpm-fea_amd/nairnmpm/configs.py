@@ -156,6 +156,24 @@ def traction_bcs(xml: str, shapes) -> str:
     return grid_bcs(xml, shapes, block="ParticleBCs", child="TractionBC")
 
 
+def periodic_xpic(xml: str, order: int = 2, fmpm: bool = False, steps: int | None = 1,
+                  time_ms: float | None = None, cfl: float | None = None) -> str:
+    """xml with a PeriodicXPIC custom task: XPIC(order) or FMPM(order) every `steps` steps, or by
+    time (periodicTime in ms) or by a CFL factor"""
+    params = [("FMPMOrder" if fmpm else "XPICOrder", order)]
+    if steps is not None:
+        params.append(("periodicSteps", steps))
+    if time_ms is not None:
+        params.append(("periodicTime", time_ms))
+    if cfl is not None:
+        params.append(("periodicCFL", cfl))
+    body = "  <CustomTasks>\n    <Schedule name=\"PeriodicXPIC\">\n"
+    for k, v in params:
+        body += f'      <Parameter name="{k}">{v!r}</Parameter>\n'
+    body += "    </Schedule>\n  </CustomTasks>\n"
+    return xml.replace("</JANFEAInput>", body + "</JANFEAInput>", 1)
+
+
 def mooney_material_xml(G1: float = 8., G2: float = 2., K: float = 50., rho: float = 1.1,
                         name: str = "mooney") -> str:
     """<Material Type="8"> (Mooney-Rivlin) with G1, G2 and K in MPa."""

@@ -1384,6 +1384,117 @@ struct Oracle {
         }
     }
 
+    // ---- XPIC(k) and FMPM(k): the PeriodicXPIC custom task ------------------------------------
+    // the order of the running step: run.xpicOrder in the steps the task's schedule selects, else 0
+    int xpicStepOrder = 0;
+    int64_t xpicNextStep = 0;            // nextPeriodicStep (PeriodicXPIC.cpp:261)
+    double xpicNextTime = 0.;            // nextPeriodicTime (:254)
+    std::vector<double> vprev, vnext;    // vk[VSTARPREV_VEC], vk[VSTARNEXT_VEC] (MatVelocityField.hpp:25-27)
+    enum { XPIC_STRAIN_UPDATE = 32, XPIC_PARTICLE_UPDATE = 64 };
+
+    // PeriodicXPIC::PrepareForStep and StepCalculation (PeriodicXPIC.cpp:283-312,364-373): the
+    // decision for a step is made by Initialize before the first step (mstep = 0, mtime = 0) and by
+    // the custom-task pass at the end of the step before (CMAnalysis advances mtime after MPMStep),
+    // so the step number tested is this step's and the time tested is mtime + timestep of the
+    // decision: dt for the first step, this step's start time for later ones
+    void xpic_schedule() {
+        xpicStepOrder = 0;
+        if (run.xpicOrder <= 0) return;
+        const int64_t s = steps + 1;
+        if (s == 1) {
+            xpicNextStep = run.xpicPeriodSteps;
+            xpicNextTime = run.xpicPeriodTime;
+        }
+        if (run.xpicPeriodSteps > 0) {
+            if (s >= xpicNextStep) {
+                xpicStepOrder = run.xpicOrder;
+                xpicNextStep += run.xpicPeriodSteps;
+            }
+        } else if (run.xpicPeriodTime > 0.) {
+            const double t = s == 1 ? 0. + run.timestep : mtime;
+            if (t >= xpicNextTime) {
+                xpicStepOrder = run.xpicOrder;
+                xpicNextTime = t + run.xpicPeriodTime;
+            }
+        }
+    }
+
+    // XPICExtrapolationTask::Execute (XPICExtrapolationTask.cpp:50-164) for mechanics, one material
+    // velocity field per material, no cracks, no contact: v(k) in vk, plus a*dt for XPIC
+    void xpic_extrapolation(int option) {
+        const int m = xpicStepOrder;
+        if (m <= 1) return;
+        const double dt = run.timestep;
+        const bool fmpm = run.xpicFMPM != 0;
+        const long nn = (long)nf * g.nnodes;
+        vprev.assign(3 * (size_t)nn, 0.);
+        vnext.assign(3 * (size_t)nn, 0.);
+        // InitializeXPICData -> MatVelocityField::XPICSupport(INITIALIZE_XPIC) (MatVelocityField.cpp:321-345)
+        for (long k = 0; k < nn; k++) {
+            if (npts[k] == 0) continue;
+            double *vp = &vprev[3 * k], *vs = &vk[3 * k];
+            const double *p = &pk[3 * k], *f = &ftot[3 * k];
+            if (fmpm) {
+                const double sc = 1. / mass[k];
+                for (int c = 0; c < 3; c++) vp[c] = sc * p[c];
+                for (int c = 0; c < 3; c++) vs[c] = vp[c];
+            } else {
+                for (int c = 0; c < 3; c++) vp[c] = p[c];
+                for (int c = 0; c < 3; c++) vp[c] += f[c] * -dt;
+                const double sc = 1. / mass[k];
+                for (int c = 0; c < 3; c++) vp[c] *= sc;
+                for (int c = 0; c < 3; c++) vs[c] = vp[c];
+                const double sa = dt / mass[k];
+                for (int c = 0; c < 3; c++) vs[c] += f[c] * sa;
+            }
+        }
+        for (int it = 2; it <= m; it++) {
+            // XPICDoubleLoop (:182-211): node i gains mp Si Sj / mi * vprev_j over the particle's nodes
+            particles([&](long p) {
+                ShapeOut o;
+                shape(p, false, o);
+                const long off = fnode(field(p), 0);
+                const double mp = F(MPF_MASS, p);
+                for (int i = 0; i < o.n; i++) {
+                    const long ni = off + o.node[i];
+                    const double mi = mass[ni];
+                    for (int j = 0; j < o.n; j++) {
+                        const long nj = off + o.node[j];
+                        const double weight = o.fn[i] * o.fn[j];
+                        const double w = mp * weight / mi;
+                        for (int c = 0; c < 3; c++) vnext[3 * ni + c] += w * vprev[3 * nj + c];
+                    }
+                }
+            });
+            // GetDeltaV (:222-235): vprev -= vnext (XPICSupport GET_DELTAV, MatVelocityField.cpp:347-363)
+            for (long k = 0; k < nn; k++)
+                if (npts[k] != 0)
+                    for (int c = 0; c < 3; c++) vprev[3 * k + c] -= vnext[3 * k + c];
+            // NodalVelBC::GridVelocityConditions for XPIC passes (NodalVelBC.cpp:327-335) ->
+            // MatVelocityField::ZeroVelocityBC (MatVelocityField.cpp:523-534): zero the axis
+            // component of the increment; the particle pass of XPIC keeps the lumped-mass reaction
+            // in the material force
+            for (size_t e = 0; e < vbcs.size(); e++) {
+                if (!bcOn[e]) continue;
+                const int a = vbcs[e].dir - 1;
+                for (int f = 0; f < nf; f++) {
+                    const long k = fnode(f, (long)vbcs[e].node);
+                    if (npts[k] <= 0) continue;
+                    const double dotn = vprev[3 * k + a];
+                    vprev[3 * k + a] += -dotn;
+                    if (option == XPIC_PARTICLE_UPDATE && !fmpm) ftot[3 * k + a] += -mass[k] * dotn / dt;
+                }
+            }
+            // UpdateXStar (XPICSupport UPDATE_VSTAR, :365-378): v* += Delta v, vnext = 0
+            for (long k = 0; k < nn; k++)
+                if (npts[k] != 0)
+                    for (int c = 0; c < 3; c++) {
+                        vk[3 * k + c] += vprev[3 * k + c];
+                        vnext[3 * k + c] = 0.;
+                    }
+        }
+    }
+
     // MatVelocityField::GridValueCalculation (MatVelocityField.cpp:239-251)
     void grid_velocity() {
 #pragma omp parallel for num_threads(nthreads) schedule(static) if (nthreads > 1)
@@ -1394,9 +1505,11 @@ struct Oracle {
         }
     }
 
-    // UpdateStrainsFirstTask::FullStrainUpdate (UpdateStrainsFirstTask.cpp:101-168)
+    // UpdateStrainsFirstTask::FullStrainUpdate (UpdateStrainsFirstTask.cpp:101-168); FMPM(k > 1)
+    // takes its grid velocities from the XPIC extrapolation (:106-114), every other mode lumped
     void full_strain_update(double strainTime) {
-        grid_velocity();
+        if (run.xpicFMPM && xpicStepOrder > 1) xpic_extrapolation(XPIC_STRAIN_UPDATE);
+        else grid_velocity();
         particles([&](long p) {
             ShapeOut o;
             shape(p, true, o);
@@ -1776,13 +1889,19 @@ struct Oracle {
         for (long n = 0; n < (long)nf * g.nnodes; n++)
             if (npts[n] > 0) for (int c = 0; c < 3; c++) pk[3 * n + c] += ftot[3 * n + c] * dt;
         material_contact(CALL_UPDATE_MOMENTUM);          // ContactAndMomentaBCs (UpdateMomentaTask.cpp:58)
-        velocity_bcs(BC_UPDATE_MOMENTUM);                // (UpdateMomentaTask.cpp:93)
+        // (UpdateMomentaTask.cpp:93); skipped in XPIC and FMPM steps of order > 1 (NodalVelBC.cpp:365-370)
+        if (xpicStepOrder <= 1) velocity_bcs(BC_UPDATE_MOMENTUM);
     }
 
-    // UpdateParticlesTask::Execute (UpdateParticlesTask.cpp:44-298), FLIP, no damping ->
-    // IncrementNodalVelAcc (MatVelocityField.cpp:410-426) -> MoveParticle (MatPoint3D.cpp:104-194)
+    // UpdateParticlesTask::Execute (UpdateParticlesTask.cpp:44-298) -> IncrementNodalVelAcc
+    // (MatVelocityField.cpp:410-426) -> MoveParticle (MatPoint3D.cpp:104-194): FLIP (m = 0), FMPM(k)
+    // (m = k > 0) or XPIC(k) (m = -k < 0); for k > 1 the grid velocities come from the XPIC
+    // extrapolation (UpdateParticlesTask.cpp:66-79)
     void task_update_particles() {
-        grid_velocity();
+        const int order = xpicStepOrder;
+        if (order > 1) xpic_extrapolation(XPIC_PARTICLE_UPDATE);
+        else grid_velocity();
+        const int m = run.xpicFMPM ? order : -order;
         double dt = run.timestep;
         particles([&](long p) {
             ShapeOut o;
@@ -1792,18 +1911,37 @@ struct Oracle {
             for (int i = 0; i < o.n; i++) {
                 long n = off + o.node[i];
                 for (int c = 0; c < 3; c++) Svk[c] += vk[3 * n + c] * o.fn[i];
-                double mnode = o.fn[i] / mass[n];
-                for (int c = 0; c < 3; c++) Sacc[c] += ftot[3 * n + c] * mnode;
+                if (m <= 0) {
+                    double mnode = o.fn[i] / mass[n];
+                    for (int c = 0; c < 3; c++) Sacc[c] += ftot[3 * n + c] * mnode;
+                }
             }
             int nc = g.dim == 3 ? 3 : 2;
             for (int c = 0; c < nc; c++) {
-                double vm = Svk[c] - Sacc[c] * dt;
+                const double vold = F(MPF_VEL_X + c, p);
+                // vm: S(v - a dt) for FLIP, Sv+(k) for FMPM, S(v(k) + a dt) for XPIC (:106-116)
+                double vm = m == 0 ? Svk[c] - Sacc[c] * dt : Svk[c];
                 // Adamp0 = particleAlpha vel + gridAlpha vm (MatPoint3D.cpp:117-120)
-                double adamp = F(MPF_VEL_X + c, p) * run.particleDamping + vm * run.gridDamping;
-                double delV = (Sacc[c] - adamp) * dt;
-                F(MPF_VEL_X + c, p) += delV;
-                double delXRate = vm + 0.5 * delV;
-                F(MPF_POS_X + c, p) += delXRate * dt;
+                double adamp = vold * run.particleDamping + vm * run.gridDamping;
+                if (m > 0) {
+                    // FMPM: v = vm - Adamp0 dt, x += 0.5 (v(n+1) + v(n)) dt (:122-137)
+                    double vnew = vm + adamp * -dt;
+                    F(MPF_VEL_X + c, p) = vnew;
+                    double delXRate = vold + vnew;
+                    F(MPF_POS_X + c, p) += delXRate * (0.5 * dt);
+                } else if (m == 0) {
+                    double delV = (Sacc[c] - adamp) * dt;
+                    F(MPF_VEL_X + c, p) += delV;
+                    double delXRate = vm + 0.5 * delV;
+                    F(MPF_POS_X + c, p) += delXRate * dt;
+                } else {
+                    // XPIC: v = Svk - Adamp0 dt, x += (vm + 0.5 delV) dt (:152-170)
+                    double vnew = Svk[c] - adamp * dt;
+                    double delV = vnew - vold;
+                    F(MPF_VEL_X + c, p) = vnew;
+                    double delXRate = vm + 0.5 * delV;
+                    F(MPF_POS_X + c, p) += delXRate * dt;
+                }
             }
         });
     }
@@ -1948,6 +2086,7 @@ struct Oracle {
 
     // NairnMPM::MPMStep (NairnMPM.cpp:284-335) with the task list of CreateTasks (:870-1110)
     void step() {
+        xpic_schedule();
         task_initialize();
         task_mass_momentum();
         task_post_extrapolation();
@@ -2007,7 +2146,7 @@ int oracle_run_task(void *h, int32_t task) {
     Oracle *o = (Oracle *)h;
     try {
     switch (task) {
-        case MPMGPU_TASK_INITIALIZATION: o->task_initialize(); break;
+        case MPMGPU_TASK_INITIALIZATION: o->xpic_schedule(); o->task_initialize(); break;
         case MPMGPU_TASK_MASS_MOMENTUM: o->task_mass_momentum(); break;
         case MPMGPU_TASK_POST_EXTRAPOLATION: o->task_post_extrapolation(); break;
         case MPMGPU_TASK_UPDATE_STRAINS_FIRST: o->full_strain_update(o->run.strainFirst); break;
@@ -2047,6 +2186,9 @@ int oracle_set_velocity_bcs(void *h, int64_t n, const MpmVelocityBC *bcs) {
     o->bc_values();                      // values at the current time until the next step's
     return 0;
 }
+
+/* the XPIC or FMPM order the last step ran with (0: FLIP) */
+int32_t oracle_xpic_order(void *h) { return ((Oracle *)h)->xpicStepOrder; }
 
 /* traction conditions in list order (n = 0 removes them) */
 int oracle_set_particle_tractions(void *h, int64_t n, const MpmParticleTraction *tr) {
@@ -2104,6 +2246,13 @@ int oracle_get_grid(void *h, double *mass, double *pk, double *ftot, int32_t *np
     if (pk) memcpy(pk, o->pk.data(), 3 * nn * sizeof(double));
     if (ftot) memcpy(ftot, o->ftot.data(), 3 * nn * sizeof(double));
     if (npts) memcpy(npts, o->npts.data(), nn * sizeof(int32_t));
+    return 0;
+}
+
+/* the grid velocities vk of every field (x,y,z per field node) */
+int oracle_get_velocities(void *h, double *vk) {
+    Oracle *o = (Oracle *)h;
+    memcpy(vk, o->vk.data(), 3 * (size_t)o->nf * o->g.nnodes * sizeof(double));
     return 0;
 }
 

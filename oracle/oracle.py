@@ -44,6 +44,8 @@ def lib():
         _lib.oracle_set_velocity_bcs.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.VelocityBC)]
         _lib.oracle_set_particle_loads.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.ParticleLoad)]
         _lib.oracle_set_particle_tractions.argtypes = [_P, ctypes.c_int64, ctypes.POINTER(nm.ParticleTraction)]
+        _lib.oracle_xpic_order.argtypes = [_P]
+        _lib.oracle_get_velocities.argtypes = [_P, _PD]
         _lib.oracle_surface_info.argtypes = [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                              _PD, _PD]
         _lib.oracle_get_particles.argtypes = [_P, _PD, _PI]
@@ -107,6 +109,18 @@ class Oracle:
         arr = (nm.ParticleTraction * max(len(tractions), 1))(*tractions)
         if lib().oracle_set_particle_tractions(self._h, len(tractions), arr) != 0:
             raise RuntimeError("bad traction condition")
+
+    def grid_velocities(self):
+        """grid velocities vk [nf, 3, nn]"""
+        nn = lib().oracle_num_nodes(self._h)
+        nf = lib().oracle_num_fields(self._h)
+        v = np.zeros(3 * nf * nn)
+        lib().oracle_get_velocities(self._h, _d(v))
+        return v.reshape(nf, nn, 3).transpose(0, 2, 1)
+
+    def xpic_order(self):
+        """the XPIC or FMPM order the last step ran with (0: FLIP)"""
+        return int(lib().oracle_xpic_order(self._h))
 
     def surface_info(self, p, face, dir, thickness=1.0):
         """GetSurfaceInfo of particle p in its current state: (corners, weighted normal), corners
