@@ -162,7 +162,9 @@ def bench_batch(a):
     batch.step(a.warmup)
     batch.sync()
     gsteps, tune_ms = batch.autotune(trial_steps=32)
-    print(f"# autotuner: graphs of {gsteps} steps ({tune_ms:.3f} ms/step in trials)", file=sys.stderr, flush=True)
+    tshape = batch.tile_shape
+    print(f"# autotuner: tile shape {tshape}, graphs of {gsteps} steps ({tune_ms:.3f} ms/step in trials)",
+          file=sys.stderr, flush=True)
 
     # timed region: K batch steps replayed from the tuned graph
     batch.sync()
@@ -227,7 +229,7 @@ def bench_batch(a):
                                "log-uniform, E 0.5-5 GPa, nu 0.25-0.4, yield 5-50 MPa, uGIMP, one step of "
                                "each instance per batch step",
                    "particles": npart, "instances": a.instances, "active_nodes": active_nodes,
-                   "graph_steps": gsteps, "one_at_a_time_value": serial,
+                   "graph_steps": gsteps, "tile_shape": tshape, "one_at_a_time_value": serial,
                    "one_at_a_time_sample": f"{a.serial_steps} graph-replayed steps of every instance alone",
                    "speedup_vs_one_at_a_time": value / serial},
         "roofline": roofline,
@@ -373,6 +375,8 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--graph-steps", type=int, default=8, help="serve: batch steps per graph / admission check")
     ap.add_argument("--max-resident", type=int, default=0, help="serve: most solves resident at once (0: no limit)")
+    ap.add_argument("--tile-shape", type=int, choices=[0, 1], default=0,
+                    help="j2/clamped/contact: tile shape (0: 4x4x4 cells per tile, 1: 8x4x4); the batch tunes its own")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of sweep instances")
     ap.add_argument("--serial-steps", type=int, default=20, help="batch: steps of each instance run alone")
     ap.add_argument("--cpu-seconds", type=float, default=10., help="CPU baseline sample length (wall seconds)")
@@ -432,6 +436,7 @@ def main():
 
     sim = nm.Simulation(grid, mats, run)
     sim.upload(dfields, ifields)
+    sim.set_tile_shape(a.tile_shape)
     bcs, loads = prob.velocity_bcs(), prob.particle_loads()
     if bcs:
         sim.set_velocity_bcs(bcs)
@@ -525,7 +530,7 @@ def main():
                    "particles": npart, "grid_cells": [grid.nx, grid.ny, grid.nz], "active_nodes": active_nodes,
                    "material": "Neohookean x2 + CoulombFriction" if a.workload == "contact" else "IsoPlasticity J2, linear hardening",
                    "shape": "lCPDI" if a.workload == "contact" else "uGIMP", "method": "USAVG+ FLIP",
-                   "graph_replay_value": graph_value},
+                   "tile_shape": a.tile_shape, "graph_replay_value": graph_value},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -372,9 +372,17 @@ int mpmgpu_set_graph_steps(MpmCtx *ctx, int32_t steps);
    only at upload). A memory-layout choice with no counterpart in the reference: results are the
    same for any interval, since particles inside a cell are summed in upload order. */
 int mpmgpu_set_sort_interval(MpmCtx *ctx, int32_t steps);
-/* Launch-shape tuner (no counterpart in the reference): times graph lengths 2..32 over
-   trialSteps steps each, keeps the fastest, and restores the simulation state it started from. */
+/* Tuner (no counterpart in the reference): for each tile shape (the cells one bin tile holds:
+   shape 0 is 4x4x4 cells in 3D and 8x8 in 2D, shape 1 is 8x4x4 and 16x8) times graph lengths
+   2..32 over trialSteps steps each, keeps the fastest pair, and restores the simulation state it
+   started from in the chosen shape (mpmgpu_tile_shape reports it). Results of one shape are
+   deterministic; the two shapes sum grid values in different orders, so they agree to rounding. */
 int mpmgpu_autotune(MpmCtx *ctx, int32_t trialSteps, int32_t *graphSteps, double *msPerStep);
+/* Set the tile shape (0 or 1, as above; default 0) between whole steps: the particles are binned
+   again. No counterpart in the reference. */
+int mpmgpu_set_tile_shape(MpmCtx *ctx, int32_t shape);
+/* The context's tile shape (-1 for a null context). */
+int32_t mpmgpu_tile_shape(const MpmCtx *ctx);
 /* Wait for all queued device work of this context. */
 int mpmgpu_sync(MpmCtx *ctx);
 /* Download particles in creation (mpm[]) order. */

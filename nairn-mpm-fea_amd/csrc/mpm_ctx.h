@@ -57,7 +57,9 @@ struct MpmCtx {
     std::vector<int> matInstH;        // material -> instance
     KGrid *dinst = nullptr;
     int *dtables = nullptr;           // tileInst, nodeBlockInst, matInst
-    int *bbuf = nullptr;              // binning arrays
+    int *bbuf = nullptr;              // binning arrays (bbufBytes: enough for either tile shape)
+    size_t bbufBytes = 0;
+    int tileCap = 0;                  // batch: entries of the tile -> instance table dtables holds
     unsigned long long *sbuf = nullptr;
     long long ncap = 0;
     hipStream_t stream = nullptr;
@@ -126,20 +128,28 @@ void enqueue_step(MpmCtx *c);
 // input checks of mpmgpu_create (nullptr when the input is usable, else the message)
 const char *check_run(const MpmGridDesc *grid, const MpmRunParams *run);
 const char *check_materials(const MpmMaterial *mats, int nmat);
-// geometry, tiles, time step and run stops of one grid
+// geometry, tiles (shape 0), time step and run stops of one grid
 void fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run);
+// the tile counts of grid g for a tile shape
+void set_tiles(KGrid &g, int shape);
+// tiles per field of a context (summed over a batch's instances) for a tile shape
+long long tiles_total(const MpmCtx *c, int shape);
+// switch the context to another tile shape: tile layout, bins and instance tables, then the
+// particles binned again and their rows sorted (results do not depend on the row order)
+int set_tile_shape(MpmCtx *c, int shape);
 // ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
 size_t bin_words(long long nk, long long ntall);
 // byte sizes of a context's grid, bin and counter arrays for its grid c->kg and nmat materials
 struct GridArrays {
     size_t nodeBytes = 0, cntBytes = 0, partBytes = 0, matBytes = 0, statBytes = 0, binBytes = 0;
     size_t normBytes = 0, listBytes = 0;      // regression normals (methods 5 and 6) only
-    long long nkeys = 0;
 };
 GridArrays grid_arrays(const MpmCtx *c, int nmat);
 // the context's node, tile-partial, bin and counter pointers into arrays of those sizes
 void assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, int *cntMem, double *partMem, int *bbuf,
                         unsigned long long *sbuf, double *normMem, int *listMem);
+// the bin array pointers into c->bbuf for the current tile shape
+void assign_bins(MpmCtx *c);
 // tile kernel grids sized for the current device's compute units
 void size_tile_grids();
 // the grid of one simulation (mpmgpu_create's checks and settings); nullptr or the message

@@ -32,6 +32,7 @@ using mpmi::set_positions;
 using mpmi::check_run;
 using mpmi::check_materials;
 using mpmi::fill_grid;
+using mpmi::set_tiles;
 using mpmi::bin_words;
 
 static void destroy_graphs(MpmCtx *c) {
@@ -243,17 +244,21 @@ const char *mpmi::check_materials(const MpmMaterial *mats, int nmat) {
     return nullptr;
 }
 
-// geometry, tiles and time step of one grid
+// the tiles of grid g (cells set) for tile shape `shape`
+void mpmi::set_tiles(KGrid &g, int shape) {
+    const TileSize t = tile_size(g.dim, shape);
+    g.tileShape = shape;
+    g.ntx = (g.nx + t.tx - 1) / t.tx; g.nty = (g.ny + t.ty - 1) / t.ty; g.ntz = (g.nz + t.tz - 1) / t.tz;
+    g.ntiles = g.ntx * g.nty * g.ntz;
+}
+
+// geometry, tiles (shape 0) and time step of one grid
 void mpmi::fill_grid(KGrid &g, int dim, const MpmGridDesc *grid, const MpmRunParams *run) {
     memset(&g, 0, sizeof(g));
     g.dim = dim;
     g.nx = grid->nx; g.ny = grid->ny; g.nz = dim == 3 ? grid->nz : 1;
     g.nnx = g.nx + 1; g.nny = g.ny + 1; g.nnz = dim == 3 ? g.nz + 1 : 1;
-    int tx, ty, tz;
-    if (dim == 3) { tx = TileGeom<3>::TX; ty = TileGeom<3>::TY; tz = TileGeom<3>::TZ; }
-    else { tx = TileGeom<2>::TX; ty = TileGeom<2>::TY; tz = 1; }
-    g.ntx = (g.nx + tx - 1) / tx; g.nty = (g.ny + ty - 1) / ty; g.ntz = (g.nz + tz - 1) / tz;
-    g.ntiles = g.ntx * g.nty * g.ntz;
+    set_tiles(g, 0);
     g.nnodes = (long long)g.nnx * g.nny * g.nnz;
     g.nstride = g.nnodes;
     g.tilesTotal = g.ntiles;
@@ -500,13 +505,26 @@ void mpmgpu_problem_destroy(MpmProblem *p) { delete p; }
 
 }  // extern "C"
 
-// sizes of the grid, bin and counter arrays of a context for c->kg (nstride nodes, tilesTotal
-// tiles per field)
+// tiles per field of the context (the sum over a batch's instances) for tile shape `shape`
+long long mpmi::tiles_total(const MpmCtx *c, int shape) {
+    if (c->inst.empty()) {
+        KGrid g = c->kg;
+        set_tiles(g, shape);
+        return g.ntiles;
+    }
+    long long t = 0;
+    for (KGrid g : c->inst) {
+        set_tiles(g, shape);
+        t += g.ntiles;
+    }
+    return t;
+}
+
+// sizes of the grid, bin and counter arrays of a context for c->kg (nstride nodes; partials and
+// bins for the larger of the tile shapes)
 mpmi::GridArrays mpmi::grid_arrays(const MpmCtx *c, int nmat) {
     const KGrid &g = c->kg;
     const int nf = g.nfields;
-    const long long foot = c->dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
-    const long long nc = c->dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
     const long long nn = g.nstride;
     GridArrays a;
     // per field: mass, pk, vk, ftot (+ contact volume, displacement, gradient in multimaterial mode;
@@ -514,11 +532,15 @@ mpmi::GridArrays mpmi::grid_arrays(const MpmCtx *c, int nmat) {
     const int perField = 1 + 3 + 3 + 3 + (nf > 1 ? 1 + 3 + 3 : 0) + (c->run.xpicOrder > 1 ? 3 : 0);
     a.nodeBytes = (size_t)nf * nn * perField * sizeof(double);
     a.cntBytes = (size_t)nf * nn * sizeof(int);
-    a.partBytes = (size_t)nf * g.tilesTotal * NQMAX * foot * sizeof(double);
     a.matBytes = (size_t)nmat * sizeof(MpmMaterial);
     a.statBytes = STATS_WORDS * sizeof(unsigned long long);
-    a.nkeys = nf * g.tilesTotal * nc;
-    a.binBytes = bin_words(a.nkeys, (long long)nf * g.tilesTotal) * sizeof(int);
+    // tile partials and bins: enough for either tile shape, so the shape can change in place
+    for (int shape = 0; shape < MPM_TILE_SHAPES; shape++) {
+        const TileSize t = tile_size(c->dim, shape);
+        const long long tiles = tiles_total(c, shape);
+        a.partBytes = std::max<size_t>(a.partBytes, (size_t)nf * tiles * NQMAX * t.foot * sizeof(double));
+        a.binBytes = std::max<size_t>(a.binBytes, bin_words((long long)nf * tiles * t.nc, (long long)nf * tiles) * sizeof(int));
+    }
     if (nf > 1 && g.normals >= 5) {
         a.normBytes = (size_t)nf * 4 * nn * sizeof(double);
         a.listBytes = (size_t)(nn + 1) * sizeof(int);
@@ -554,13 +576,22 @@ void mpmi::assign_grid_arrays(MpmCtx *c, const GridArrays &a, double *nodeMem, i
     c->ks.tracErr = sbuf + 7;
     c->kn.cnorm = a.normBytes ? normMem : nullptr;
     c->kn.clist = a.listBytes ? listMem : nullptr;
-    // binning arrays: cellStart(nkeys+1), count(nkeys+1), cursor(nkeys), blockSums, tile list
-    // (nfields * tilesTotal) and its length, work counters, and the node passes' pair flags
-    const long long nk = a.nkeys;
+    c->bbuf = bbuf;
+    c->bbufBytes = a.binBytes;
+    assign_bins(c);
+}
+
+// the binning arrays in c->bbuf for the current tile shape: cellStart(nkeys+1), count(nkeys+1),
+// cursor(nkeys), blockSums, tile list (nfields * tilesTotal) and its length, work counters, and
+// the node passes' pair flags
+void mpmi::assign_bins(MpmCtx *c) {
+    const KGrid &g = c->kg;
+    const int nf = g.nfields;
+    const long long nk = (long long)nf * g.tilesTotal * tile_size(c->dim, g.tileShape).nc;
     const long long nblk = (nk + 1 + 1023) / 1024 + 1;
+    int *bbuf = c->bbuf;
     c->kb.nkeys = (int)nk;
     c->kb.ntall = nf * g.tilesTotal;
-    c->bbuf = bbuf;
     c->kb.cellStart = bbuf;
     c->kb.count = bbuf + nk + 1;
     c->kb.cursor = bbuf + 2 * nk + 2;
@@ -688,18 +719,98 @@ int64_t mpmgpu_footprint(const MpmGridDesc *grid, const MpmRunParams *run, int64
     const int dim = grid->analysis == MPMGPU_THREED ? 3 : 2;
     KGrid g;
     fill_grid(g, dim, grid, run);
-    const long long foot = dim == 3 ? TileGeom<3>::FOOT : TileGeom<2>::FOOT;
-    const long long nc = dim == 3 ? TileGeom<3>::NC : TileGeom<2>::NC;
     const long long nodes = (g.nnodes + 255) / 256 * 256;        // as laid out in a batch
-    const long long nk = (long long)g.ntiles * nc;
     int64_t bytes = nodes * (10 * sizeof(double) + sizeof(int));   // mass, pk, vk, ftot, cnt
     if (run->xpicOrder > 1) bytes += nodes * 3 * (int64_t)sizeof(double);   // the XPIC increment
-    bytes += (int64_t)g.ntiles * NQMAX * foot * sizeof(double);    // tile partials
-    bytes += (int64_t)bin_words(nk, g.ntiles) * sizeof(int);            // bins
+    int64_t tileBytes = 0;          // tile partials and bins, for the larger of the tile shapes
+    for (int shape = 0; shape < MPM_TILE_SHAPES; shape++) {
+        set_tiles(g, shape);
+        const TileSize t = tile_size(dim, shape);
+        tileBytes = std::max<int64_t>(tileBytes, (int64_t)g.ntiles * NQMAX * t.foot * (int64_t)sizeof(double) +
+                                                     (int64_t)bin_words((long long)g.ntiles * t.nc, g.ntiles) * (int64_t)sizeof(int));
+    }
+    bytes += tileBytes;
     bytes += n * PARTICLE_BYTES;
     if (run->shape == MPMGPU_SHAPE_LCPDI) bytes += n * 9 * (int64_t)sizeof(double);
     return bytes;
 }
+
+}  // extern "C"
+
+// A batch's instance grids and its tables (tile -> instance for the current tile layout, node
+// block -> instance, material -> instance) on the device, and the context grid's pointers to them.
+static int upload_batch_tables(MpmCtx *c) {
+    const int ninst = (int)c->inst.size();
+    KGrid &g = c->kg;
+    const int tileAcc = g.tilesTotal;
+    const long long nodeAcc = g.nstride;
+    if (tileAcc > c->tileCap) return c->fail("batch tile table too small for the tile layout");
+    std::vector<int> tables;
+    tables.reserve(tileAcc + nodeAcc / 256 + c->matInstH.size());
+    for (int i = 0; i < ninst; i++) tables.insert(tables.end(), c->inst[i].ntiles, i);
+    for (int i = 0; i < ninst; i++) tables.insert(tables.end(), (c->inst[i].nnodes + 255) / 256, i);
+    tables.insert(tables.end(), c->matInstH.begin(), c->matInstH.end());
+    HIPCHECK(c, hipMemcpy(c->dtables, tables.data(), tables.size() * sizeof(int), hipMemcpyHostToDevice));
+    g.inst = c->dinst;
+    g.tileInst = c->dtables;
+    g.nodeBlockInst = c->dtables + tileAcc;
+    g.matInst = c->dtables + tileAcc + nodeAcc / 256;
+    HIPCHECK(c, hipMemcpy(c->dinst, c->inst.data(), ninst * sizeof(KGrid), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// The tile layout of the context for another tile shape: the grid's (or each instance's) tiles,
+// a batch's tables, the bin pointers and the loaded particles' per-tile flags. With rebin the
+// particles are binned for it, the bins' completed-step count kept, every node visited by the
+// next step and the rows sorted into the new tile order.
+static int apply_tile_shape(MpmCtx *c, int shape, bool rebin) {
+    if (shape != c->kg.tileShape) {
+        destroy_graphs(c);
+        if (c->inst.empty()) {
+            mpmi::set_tiles(c->kg, shape);
+            c->kg.tilesTotal = c->kg.ntiles;
+        } else {
+            int acc = 0;
+            for (KGrid &gi : c->inst) {
+                mpmi::set_tiles(gi, shape);
+                gi.tileBase = acc;
+                acc += gi.ntiles;
+            }
+            for (KGrid &gi : c->inst) gi.tilesTotal = acc;
+            KGrid &g = c->kg;
+            const KGrid &g0 = c->inst[0];
+            g.tileShape = shape;
+            g.ntx = g0.ntx; g.nty = g0.nty; g.ntz = g0.ntz; g.ntiles = g0.ntiles;
+            g.tilesTotal = acc;
+            if (upload_batch_tables(c)) return 1;
+        }
+        // the histogram and cursors of the new layout sit where the old one kept other arrays:
+        // every bin array starts from zero, as at creation
+        mpmi::assign_bins(c);
+        HIPCHECK(c, hipMemsetAsync(c->bbuf, 0, c->bbufBytes, c->stream));
+        if (c->kld.nl && build_particle_bcs(c)) return 1;
+    }
+    if (!rebin || c->kp.n == 0) return 0;
+    unsigned long long done = 0;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(&done, c->kb.stop + 2, sizeof(done), hipMemcpyDeviceToHost));
+    set_positions(c);
+    launch_bin_keys(c->dim, c->kg, c->kp, c->kb, c->stream);
+    launch_bin_finish(c->dim, c->kg, c->kp, c->kb, c->stream);
+    HIPCHECK(c, hipMemcpyAsync(c->kb.stop + 2, &done, sizeof(done), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, mark_all_needed(c));
+    if (mpmi::sort_rows(c)) return 1;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpmi::set_tile_shape(MpmCtx *c, int shape) {
+    if (shape < 0 || shape >= MPM_TILE_SHAPES) return c->fail("tile shape must be 0 or 1");
+    if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
+    return apply_tile_shape(c, shape, true);
+}
+
+extern "C" {
 
 // Independent instances (each its own grid, time step, materials and particles) stepped together:
 // their tiles and nodes are laid out one after another in one context, every kernel looks up the
@@ -751,23 +862,18 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
     g.tilesTotal = tileAcc;
     for (auto &gi : c->inst) g.damped |= gi.damped;     // the launch takes the damped instance
     if (alloc_context(c, mats, nmat)) { mpmgpu_destroy(c); return 1; }
-    // instance tables on the device
-    std::vector<int> tables;
-    tables.reserve(tileAcc + nodeAcc / 256 + nmat);
-    for (int i = 0; i < ninst; i++) tables.insert(tables.end(), c->inst[i].ntiles, i);
-    for (int i = 0; i < ninst; i++) tables.insert(tables.end(), (c->inst[i].nnodes + 255) / 256, i);
-    tables.insert(tables.end(), matInst.begin(), matInst.end());
+    // instance tables on the device, with room for the tile count of either tile shape
+    for (int sh = 0; sh < MPM_TILE_SHAPES; sh++) c->tileCap = std::max(c->tileCap, (int)mpmi::tiles_total(c, sh));
     if (hipMalloc(&c->dinst, ninst * sizeof(KGrid)) != hipSuccess ||
-        hipMalloc(&c->dtables, tables.size() * sizeof(int)) != hipSuccess) {
+        hipMalloc(&c->dtables, ((size_t)c->tileCap + nodeAcc / 256 + nmat) * sizeof(int)) != hipSuccess) {
         mpmgpu_destroy(c);
         return fail_global("out of device memory allocating the batch tables");
     }
-    (void)hipMemcpy(c->dtables, tables.data(), tables.size() * sizeof(int), hipMemcpyHostToDevice);
-    g.inst = c->dinst;
-    g.tileInst = c->dtables;
-    g.nodeBlockInst = c->dtables + tileAcc;
-    g.matInst = c->dtables + tileAcc + nodeAcc / 256;
-    (void)hipMemcpy(c->dinst, c->inst.data(), ninst * sizeof(KGrid), hipMemcpyHostToDevice);
+    if (upload_batch_tables(c)) {
+        std::string m = c->err;
+        mpmgpu_destroy(c);
+        return fail_global(m);
+    }
     *out = c;
     return 0;
 }
@@ -1315,63 +1421,89 @@ int mpmgpu_set_graph_steps(MpmCtx *c, int32_t steps) {
     return 0;
 }
 
-// Launch-shape tuner: times whole steps captured in graphs of 2 to 32 steps and keeps the fastest.
-// The particle state, bins and counters are saved before the trials and restored after them, so
-// tuning leaves the simulation where it was.
+// Tuner: for each tile shape (TileShape: the cells a tile holds, so the bins, the tile partials
+// and the workgroups of every tile kernel), times whole steps captured in graphs of 2 to 32 steps
+// and keeps the fastest pair. The particle state, bins and counters are saved before the trials and
+// restored before each shape's trials and after them, so tuning leaves the simulation where it was
+// (in the chosen shape; its particles are binned again when that differs from the shape before).
 int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *msPerStep) {
     if (c->nextTask != 0) return c->fail("a step is half done through mpmgpu_run_task");
     if (c->kp.n == 0 || c->profiling) return c->fail("nothing to tune");
     size_t n = (size_t)c->ncap;
     size_t pb = PROWS * n * sizeof(double), ib = IROWS * n * sizeof(int);
-    size_t bb = bin_words(c->kb.nkeys, c->kb.ntall) * sizeof(int);
+    const size_t bb = c->bbufBytes, sb = STATS_WORDS * sizeof(unsigned long long);
     char *save = nullptr;
-    HIPCHECK(c, hipMalloc(&save, pb + ib + bb + STATS_WORDS * sizeof(unsigned long long)));
+    HIPCHECK(c, hipMalloc(&save, pb + ib + bb + sb));
     HIPCHECK(c, hipMemcpyAsync(save, c->pbuf, pb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(save + pb, c->ibuf, ib, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(c, hipMemcpyAsync(save + pb + ib, c->bbuf, bb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, STATS_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
-                               c->stream));
-    const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort;
+    HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, sb, hipMemcpyDeviceToDevice, c->stream));
+    const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort, shape0 = c->kg.tileShape;
     const int64_t steps = c->steps, xpicNextStep = c->xpicNextStep;
     const double mtime = c->mtime, bcTime = c->bcTime, xpicNextTime = c->xpicNextTime, xpicClock = c->xpicClock;
-    int best = oldG;
+    // the state from before the trials in tile shape `shape`: the saved bins in the shape the
+    // context had, the particles binned again in the other
+    auto restore = [&](int shape) -> int {
+        HIPCHECK(c, hipMemcpyAsync(c->pbuf, save, pb, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(c->ibuf, save + pb, ib, hipMemcpyDeviceToDevice, c->stream));
+        // counters and run stop first: a binning of a stopped run would leave the bins empty
+        HIPCHECK(c, hipMemcpyAsync(c->sbuf, save + pb + ib + bb, sb, hipMemcpyDeviceToDevice, c->stream));
+        c->parity = parity;
+        if (shape == shape0) {
+            if (apply_tile_shape(c, shape, false)) return 1;
+            HIPCHECK(c, hipMemcpyAsync(c->bbuf, save + pb + ib, bb, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHECK(c, mark_all_needed(c));      // the next step clears the nodes the trials touched
+        } else {
+            // binned from the saved rows; every node is visited by the next step
+            if (apply_tile_shape(c, shape, true)) return 1;
+        }
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->sinceSort = sinceSort;
+        c->steps = steps;
+        c->mtime = mtime;
+        c->bcTime = bcTime;
+        c->xpicNextStep = xpicNextStep;
+        c->xpicNextTime = xpicNextTime;
+        c->xpicClock = xpicClock;
+        return put_bc_time(c);
+    };
+    int best = oldG, bestShape = shape0;
     double bestMs = 1e300;
     static const int cand[] = {2, 4, 8, 16, 32};
-    for (int G : cand) {
-        int k = trialSteps < G ? G : (trialSteps / G) * G;
-        mpmgpu_set_graph_steps(c, G);
-        if (mpmgpu_step(c, G)) break;                      // capture and warm up
-        HIPCHECK(c, hipStreamSynchronize(c->stream));
-        if (mpmgpu_step(c, k)) break;
-        HIPCHECK(c, hipStreamSynchronize(c->stream));
-        float ms = 0.f;
-        HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        double per = ms / k;
-        if (per < bestMs) { bestMs = per; best = G; }
+    bool failed = false;
+    for (int sh = 0; sh < MPM_TILE_SHAPES && !failed; sh++) {
+        if (restore(sh)) { failed = true; break; }
+        for (int G : cand) {
+            int k = trialSteps < G ? G : (trialSteps / G) * G;
+            mpmgpu_set_graph_steps(c, G);
+            if (mpmgpu_step(c, G) || hipStreamSynchronize(c->stream) != hipSuccess) { failed = true; break; }
+            if (mpmgpu_step(c, k) || hipStreamSynchronize(c->stream) != hipSuccess) { failed = true; break; }
+            float ms = 0.f;
+            hipError_t e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            if (e != hipSuccess) {
+                c->fail(std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+                failed = true;
+                break;
+            }
+            double per = ms / k;
+            if (per < bestMs) { bestMs = per; best = G; bestShape = sh; }
+        }
     }
-    // restore the state from before the trials
-    HIPCHECK(c, hipMemcpyAsync(c->pbuf, save, pb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(c->ibuf, save + pb, ib, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(c->bbuf, save + pb + ib, bb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, mark_all_needed(c));      // the next step clears the nodes the trials touched
-    HIPCHECK(c, hipMemcpyAsync(c->sbuf, save + pb + ib + bb, STATS_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
-                               c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (failed) { best = oldG; bestShape = shape0; }
+    std::string trialErr = failed ? c->err : std::string();
+    const int rc = restore(bestShape);
     (void)hipFree(save);
-    c->parity = parity;
-    c->sinceSort = sinceSort;
-    c->steps = steps;
-    c->mtime = mtime;
-    c->bcTime = bcTime;
-    c->xpicNextStep = xpicNextStep;
-    c->xpicNextTime = xpicNextTime;
-    c->xpicClock = xpicClock;
-    if (put_bc_time(c)) return 1;
+    if (rc) return 1;
     mpmgpu_set_graph_steps(c, best);
+    if (failed) return c->fail("autotune trial failed: " + trialErr);
     if (graphSteps) *graphSteps = best;
     if (msPerStep) *msPerStep = bestMs;
     return 0;
 }
+
+int mpmgpu_set_tile_shape(MpmCtx *c, int32_t shape) { return mpmi::set_tile_shape(c, shape); }
+
+int32_t mpmgpu_tile_shape(const MpmCtx *c) { return c ? c->kg.tileShape : -1; }
 
 // lCPDI corner faults are fatal, as the reference's exception (MatPoint3D.cpp:610-615); so is a
 // contact node seen by more particles than a regression list holds

@@ -5,21 +5,32 @@
 
 #include "../../include/nairnmpm_gpu.h"
 
-// Tile geometry: cells per tile on each axis. A particle in a cell touches nodes at offsets
-// -1..2, so a tile's node footprint is (T+3) nodes per axis.
-template <int DIM> struct TileGeom;
-template <> struct TileGeom<3> {
-    static constexpr int TX = 4, TY = 4, TZ = 4;
+// Tile shapes: cells per tile on each axis. A particle in a cell touches nodes at offsets -1..2,
+// so a tile's node footprint is (T+3) nodes per axis. Shape 0 is the default; shape 1 doubles the
+// tile along x (fewer footprint nodes per cell, more LDS per workgroup, half the tiles). The step
+// kernels are compiled once per shape; a context runs one shape, chosen by the autotuner or set.
+template <int TX_, int TY_, int TZ_> struct TileDims {
+    static constexpr int TX = TX_, TY = TY_, TZ = TZ_;
     static constexpr int NC = TX * TY * TZ;
-    static constexpr int FX = TX + 3, FY = TY + 3, FZ = TZ + 3;
+    static constexpr int FX = TX + 3, FY = TY + 3, FZ = TZ == 1 ? 1 : TZ + 3;
     static constexpr int FOOT = FX * FY * FZ;
 };
-template <> struct TileGeom<2> {
-    static constexpr int TX = 8, TY = 8, TZ = 1;
-    static constexpr int NC = TX * TY;
-    static constexpr int FX = TX + 3, FY = TY + 3, FZ = 1;
-    static constexpr int FOOT = FX * FY;
+template <int DIM, int S> struct TileShape;
+template <> struct TileShape<3, 0> : TileDims<4, 4, 4> {};
+template <> struct TileShape<3, 1> : TileDims<8, 4, 4> {};
+template <> struct TileShape<2, 0> : TileDims<8, 8, 1> {};
+template <> struct TileShape<2, 1> : TileDims<16, 8, 1> {};
+#define MPM_TILE_SHAPES 2
+
+// the same numbers for host code
+struct TileSize {
+    int tx, ty, tz, nc, foot;
 };
+template <class T> constexpr TileSize tile_size_of() { return TileSize{T::TX, T::TY, T::TZ, T::NC, T::FOOT}; }
+inline TileSize tile_size(int dim, int shape) {
+    if (dim == 3) return shape ? tile_size_of<TileShape<3, 1>>() : tile_size_of<TileShape<3, 0>>();
+    return shape ? tile_size_of<TileShape<2, 1>>() : tile_size_of<TileShape<2, 0>>();
+}
 
 // kernel arguments (passed by value)
 struct KGrid {
@@ -28,6 +39,7 @@ struct KGrid {
     int nnx, nny, nnz;         // nodes (nnz = 1 in 2D)
     int ntx, nty, ntz;         // tiles
     int ntiles;
+    int tileShape;             // 0 or 1 (TileShape): which compiled instance of the kernels runs
     long long nnodes;
     double xmin, ymin, zmin, dx, dy, dz;
     double timestep, strainFirst, strainLast;
@@ -161,40 +173,6 @@ __host__ __device__ __forceinline__ unsigned long long stop_reason(unsigned long
 // contact call types (UpdateMomentaTask::ContactAndMomentaBCs passType)
 enum { CONTACT_MASS_MOMENTUM = 0, CONTACT_UPDATE_MOMENTUM = 1, CONTACT_UPDATE_STRAINS_LAST = 2 };
 
-struct KLoads;
-
-// host-side launchers (mpm_kernels.hip); all take the context's stream
-void set_tile_cus(int cus);        // compute units the tile kernels' grids are sized for
-void launch_bin_keys(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
-void launch_bin_finish(int dim, const KGrid &g, const KPart &p, const KBins &b, hipStream_t s);
-void launch_p2g_mass(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                     const MpmMaterial *mats, const KStats &st, hipStream_t s);
-void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
-void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                const MpmMaterial *mats, hipStream_t s);
-// the force scatter; with loads, the tiles holding loaded particles are marked and scattered by a
-// second launch that adds N pFext, the others by the launch without the load term
-void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KLoads &L,
-                      int par, hipStream_t s);
-void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
-// the particle update with the USL momentum scatter; mode 0 moves the particles by FLIP, mode k > 0
-// by FMPM(k) and mode -k by XPIC(k) (k > 1 takes the grid velocities of launch_xpic)
-void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, int mode,
-                    hipStream_t s);
-void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
-void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                      const MpmMaterial *mats, const KStats &st, hipStream_t s);
-// multimaterial mode: contact extrapolations (displacements from the new positions when
-// newPositions is set) and the contact pass on nodes with two or more materials
-void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                        const MpmMaterial *mats, bool newPositions, hipStream_t s);
-void launch_node_contact(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s);
-void launch_contact(int dim, const KGrid &g, const KNodes &nd, int callType, hipStream_t s);
-// regression normals (methods 5 and 6) before a contact pass: the contact-node list (at the
-// mass-momentum call), then each contact node's normals and separations from the particles that
-// see it (current positions: the new ones at the strains-last call)
-void launch_contact_normals(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                            const MpmMaterial *mats, const KStats &st, int callType, hipStream_t s);
 
 // Grid velocity conditions of one simulation (single grid), grouped by node: node b's conditions
 // are entries start[b]..start[b+1]-1, in the order of the condition list. One thread per
@@ -219,14 +197,6 @@ struct KBcs {
 // out of vk); XPIC_FORCE, the particle-update pass of XPIC, also moves the lumped reaction into ftot.
 enum { BC_PASS_MASS_MOMENTUM = 0, BC_PASS_GRID_FORCES, BC_PASS_UPDATE_MOMENTUM, BC_PASS_FORCES_MOMENTA,
        BC_PASS_UPDATE_STRAINS_LAST, BC_PASS_XPIC, BC_PASS_XPIC_FORCE };
-void launch_vel_bcs(const KGrid &g, const KNodes &nd, const KBcs &bc, int pass, int par, hipStream_t s);
-// XPIC(order) or FMPM(order) grid velocities into vk (order > 1): the series started from the
-// lumped velocities (FMPM: pk/m; XPIC: pk/m at the start of the momentum update, plus ftot dt/m),
-// then order - 1 increments, each a particle gather and scatter, a node pass and, with velocity
-// conditions (bc.nb > 0), the XPIC pass of the conditions (forcePass: the particle-update pass of
-// XPIC, which keeps the reactions in ftot)
-void launch_xpic(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, const KBcs &bc,
-                 int order, bool fmpm, bool forcePass, int par, hipStream_t s);
 
 // External particle loads of one simulation: map[upload index] is -1 or the loaded particle's
 // slot q, whose loads are entries start[q]..start[q+1]-1 in list order; fext holds the step's
@@ -255,7 +225,6 @@ struct KLoads {
     double *ttab;
     unsigned long long *terr;  // traction corners outside the grid or beyond the neighbour cells
 };
-void launch_particle_loads(const KGrid &g, const KLoads &L, int par, hipStream_t s);
 
 // A run of particles copied between the particle buffers of two contexts (the solve server moves
 // a resident solve into the next batch layout). Double rows are read from sd (row stride sn)
@@ -273,12 +242,14 @@ struct KMove {
     int matShift;
     int homeShift;             // added to the upload-index row (int row MPI_NUM_FIELDS + 2)
 };
-void launch_move(const KMove &m, hipStream_t s);
 
-// Particle rows in memory order: dst[k] = src[order[k]] (gather into sorted order) and
-// dst[home[k]] = src[k] (back to upload order); n entries of 8 (double) or 4 (int) bytes.
-void launch_gather_row(const double *src, double *dst, const int *order, long long n, hipStream_t s);
-void launch_gather_irow(const int *src, int *dst, const int *order, long long n, hipStream_t s);
-void launch_scatter_row(const double *src, double *dst, const int *home, long long n, hipStream_t s);
-void launch_scatter_irow(const int *src, int *dst, const int *home, long long n, hipStream_t s);
-void launch_iota(int *a, long long n, int base, hipStream_t s);
+// Launchers (mpm_launch.inc), all on the context's stream. The step kernels are compiled once per
+// tile shape, into namespaces mpmk_s0 and mpmk_s1; the launchers outside them run the instance of
+// g.tileShape (those without a grid argument do not depend on the shape).
+namespace mpmk_s0 {
+#include "mpm_launch.inc"
+}
+namespace mpmk_s1 {
+#include "mpm_launch.inc"
+}
+#include "mpm_launch.inc"

@@ -29,6 +29,23 @@
 
 #define NT 256
 
+// The kernels of one tile shape: this file builds shape 0, and mpm_kernels_s1.hip builds shape 1
+// by defining MPM_TILE_SHAPE before including it. Each goes into its own namespace.
+#ifndef MPM_TILE_SHAPE
+#define MPM_TILE_SHAPE 0
+#endif
+#if MPM_TILE_SHAPE == 0
+#define MPM_KNS mpmk_s0
+#else
+#define MPM_KNS mpmk_s1
+#endif
+namespace MPM_KNS {
+
+template <int DIM> using TileGeom = TileShape<DIM, MPM_TILE_SHAPE>;
+// waves per SIMD the uGIMP particle-update scatters are built for: shape 1's larger footprint
+// leaves LDS for two workgroups per CU
+#define G2P_WAVES (MPM_TILE_SHAPE == 0 ? 3 : 2)
+
 template <int DIM>
 __device__ __forceinline__ void tile_origin(const KGrid &g, int t, int &ox, int &oy, int &oz) {
     using G = TileGeom<DIM>;
@@ -1389,7 +1406,7 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
 template <int DIM, int SH, bool DAMP>
-__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N,
+__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N,
                                                                                   int mode) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
@@ -1659,7 +1676,7 @@ __global__ void k_xpic_init(KGrid g0, KBins B, KNodes N, int fmpm) {
 // scattered as a momentum; the node pass divides by m_i. Shape functions are those of the step's
 // start, as in every pass of the step.
 template <int DIM, int SH>
-__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : 3) void k_xpic_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
+__global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void k_xpic_p2g(KGrid g0, KPart P, KBins B, KNodes N) {
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NCOR = DIM == 3 ? 8 : 4;
@@ -3218,7 +3235,7 @@ void launch_xpic(int dim, const KGrid &g, const KPart &p, const KBins &b, const 
             else TILE_LAUNCH((k_xpic_p2g<2, MPMGPU_SHAPE_UGIMP>), tb, s, g, p, tb, nd);
             hipLaunchKernelGGL(k_xpic_node<2>, ng, blk, 0, s, g, tb, nd);
         }
-        if (bc.nb) launch_vel_bcs(g, nd, bc, forcePass && !fmpm ? BC_PASS_XPIC_FORCE : BC_PASS_XPIC, par, s);
+        if (bc.nb) MPM_KNS::launch_vel_bcs(g, nd, bc, forcePass && !fmpm ? BC_PASS_XPIC_FORCE : BC_PASS_XPIC, par, s);
     }
 }
 
@@ -3326,3 +3343,5 @@ void launch_scatter_irow(const int *src, int *dst, const int *home, long long n,
 void launch_iota(int *a, long long n, int base, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, NT)), dim3(NT), 0, s, a, n, base);
 }
+
+}  // namespace mpmk_s0 / mpmk_s1

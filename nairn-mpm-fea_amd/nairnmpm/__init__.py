@@ -181,6 +181,8 @@ _SIGNATURES = {
     "mpmgpu_set_sort_interval": ([_P, ctypes.c_int32], ctypes.c_int),
     "mpmgpu_autotune": ([_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
+    "mpmgpu_set_tile_shape": ([_P, ctypes.c_int32], ctypes.c_int),
+    "mpmgpu_tile_shape": ([_P], ctypes.c_int32),
     "mpmgpu_sync": ([_P], ctypes.c_int),
     "mpmgpu_download_particles": ([_P, _PD, _PI], ctypes.c_int),
     "mpmgpu_query": ([_P, ctypes.POINTER(Stats)], ctypes.c_int),
@@ -428,8 +430,17 @@ class Simulation:
         """steps between sorts of the particle rows into tile order (0: only at upload)"""
         _check(library().mpmgpu_set_sort_interval(self._h, int(steps)), self._h)
 
+    def set_tile_shape(self, shape: int):
+        """tile shape 0 (4x4x4 cells in 3D, 8x8 in 2D) or 1 (8x4x4, 16x8); the particles are binned again"""
+        _check(library().mpmgpu_set_tile_shape(self._h, int(shape)), self._h)
+
+    @property
+    def tile_shape(self) -> int:
+        return int(library().mpmgpu_tile_shape(self._h))
+
     def autotune(self, trial_steps: int = 32):
-        """pick the graph length with the fastest steps; the state is left unchanged"""
+        """pick the tile shape and graph length with the fastest steps (the shape stays in
+        tile_shape); the state is left unchanged"""
         g, ms = ctypes.c_int32(), ctypes.c_double()
         _check(library().mpmgpu_autotune(self._h, int(trial_steps), ctypes.byref(g), ctypes.byref(ms)), self._h)
         return g.value, ms.value

@@ -196,6 +196,7 @@ def test_autotune_leaves_state_unchanged():
     assert g in (2, 4, 8, 16, 32) and ms > 0
     assert a.stats().steps == 5
     assert np.array_equal(a.particles()[0], b.particles()[0])
+    b.set_tile_shape(a.tile_shape)          # the tuner's tile shape sums in its own order
     a.step(30)
     b.step(30)
     assert np.array_equal(a.particles()[0], b.particles()[0])
@@ -508,6 +509,7 @@ def test_batch_sweep_members_equal_alone_and_grid():
         g_, mats, run, _, _ = prob.describe()
         sim = nm.Simulation(g_, mats, run)
         sim.upload(d0, i0)
+        sim.set_tile_shape(batch.tile_shape)
         sim.step(30)
         d, _ = batch.instance_particles(k)
         assert np.array_equal(d, sim.particles()[0]), f"member {ks[k]} differs from its solo run"
