@@ -1405,9 +1405,14 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
 // with start-of-step shape functions (MatVelocityField::IncrementNodalVelAcc,
 // MatVelocityField.cpp:410-426; MatPoint3D::MoveParticle, MatPoint3D.cpp:104-194;
 // NodalPoint::AddMassMomentumLast, NodalPointMPM.cpp:470-486)
-template <int DIM, int SH, bool DAMP>
+// MOVE: G2P_FLIP (no damping), G2P_FLIP_DAMPED, or G2P_SERIES, which moves by FMPM(mode) for
+// mode > 0 and by XPIC(-mode) for mode < 0 with the damping of the grid (zero damping gives the
+// same numbers as none). The FLIP instances keep their registers for the common update.
+enum { G2P_FLIP = 0, G2P_FLIP_DAMPED = 1, G2P_SERIES = 2 };
+template <int DIM, int SH, int MOVE>
 __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N,
                                                                                   int mode) {
+    constexpr bool DAMP = MOVE != G2P_FLIP;
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
     constexpr int NCOR = DIM == 3 ? 8 : 4;
@@ -1506,7 +1511,7 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void 
                 double vnew[3] = {0., 0., 0.};
                 // MatPoint3D::MoveParticle (MatPoint3D.cpp:104-194): Adamp0 = alpha_p v + alpha_g vm
                 // (the DAMP instance; zero damping gives delV = Sacc dt for FLIP)
-                if (mode == 0) {
+                if (MOVE != G2P_SERIES) {
                     // FLIP: vm = S(v - a dt), v += (Sa - Adamp0) dt, x += (vm + delV/2) dt (:104-160)
 #pragma unroll
                     for (int c = 0; c < DIM; c++) {
@@ -3018,27 +3023,24 @@ void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd
     else hipLaunchKernelGGL(k_node_force<2>, dim3(blocks_for(g.nstride * g.nfields, NT)), dim3(NT), 0, s, g, b, nd);
 }
 
+template <int MOVE>
+static void g2p_move(int dim, const KGrid &g, const KPart &p, const KBins &tb, const KNodes &nd, int mode, hipStream_t s) {
+    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
+    if (dim == 3) {
+        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, MOVE>), tb, s, g, p, tb, nd, mode);
+        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, MOVE>), tb, s, g, p, tb, nd, mode);
+    } else {
+        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, MOVE>), tb, s, g, p, tb, nd, mode);
+        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, MOVE>), tb, s, g, p, tb, nd, mode);
+    }
+}
+
 void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, int mode, hipStream_t s) {
     const KBins tb = tile_bins(b, TILE_SLOT_G2P);
-    const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
-    // damping has its own instance, so the undamped update keeps its registers
-    if (g.damped) {
-        if (dim == 3) {
-            if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, mode);
-            else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, mode);
-        } else {
-            if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, true>), tb, s, g, p, tb, nd, mode);
-            else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, true>), tb, s, g, p, tb, nd, mode);
-        }
-        return;
-    }
-    if (dim == 3) {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, mode);
-        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, mode);
-    } else {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, false>), tb, s, g, p, tb, nd, mode);
-        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, false>), tb, s, g, p, tb, nd, mode);
-    }
+    // damping and the FMPM/XPIC moves have their own instances, so the plain FLIP update keeps its registers
+    if (mode != 0) g2p_move<G2P_SERIES>(dim, g, p, tb, nd, mode, s);
+    else if (g.damped) g2p_move<G2P_FLIP_DAMPED>(dim, g, p, tb, nd, mode, s);
+    else g2p_move<G2P_FLIP>(dim, g, p, tb, nd, mode, s);
 }
 
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
