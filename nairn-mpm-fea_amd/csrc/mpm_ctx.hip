@@ -1434,10 +1434,15 @@ int mpmgpu_autotune(MpmCtx *c, int32_t trialSteps, int32_t *graphSteps, double *
     const size_t bb = c->bbufBytes, sb = STATS_WORDS * sizeof(unsigned long long);
     char *save = nullptr;
     HIPCHECK(c, hipMalloc(&save, pb + ib + bb + sb));
-    HIPCHECK(c, hipMemcpyAsync(save, c->pbuf, pb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(save + pb, c->ibuf, ib, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(save + pb + ib, c->bbuf, bb, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(save + pb + ib + bb, c->sbuf, sb, hipMemcpyDeviceToDevice, c->stream));
+    hipError_t e0 = hipMemcpyAsync(save, c->pbuf, pb, hipMemcpyDeviceToDevice, c->stream);
+    if (e0 == hipSuccess) e0 = hipMemcpyAsync(save + pb, c->ibuf, ib, hipMemcpyDeviceToDevice, c->stream);
+    if (e0 == hipSuccess) e0 = hipMemcpyAsync(save + pb + ib, c->bbuf, bb, hipMemcpyDeviceToDevice, c->stream);
+    if (e0 == hipSuccess) e0 = hipMemcpyAsync(save + pb + ib + bb, c->sbuf, sb, hipMemcpyDeviceToDevice, c->stream);
+    if (e0 != hipSuccess) {       // nothing has changed yet: the state is where it was
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(save);
+        return c->fail(std::string("autotune: saving the state: ") + hipGetErrorString(e0));
+    }
     const int parity = c->parity, oldG = c->graphSteps, sinceSort = c->sinceSort, shape0 = c->kg.tileShape;
     const int64_t steps = c->steps, xpicNextStep = c->xpicNextStep;
     const double mtime = c->mtime, bcTime = c->bcTime, xpicNextTime = c->xpicNextTime, xpicClock = c->xpicClock;

@@ -602,7 +602,51 @@ __device__ __forceinline__ void wave_cells(const int *s_cs, int r0, int r1,
     using G = TileGeom<DIM>;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int o1x = l & 3, o1y = (l >> 2) & 3, o1z = DIM == 3 ? (l >> 4) : 1;   // offset + 1
-    const bool lane_on = DIM == 3 || l < 16;
+    if constexpr (DIM == 2) {
+        // A 2D cell has 16 candidate nodes, so a wave takes four cells per pass: lane group
+        // l / 16 one cell, lane l % 16 its candidate. The wave's cells sit at x = w + NW a
+        // (a < A) on every row; the four cells of a pass are 4 cells apart in x or y, so they
+        // touch different nodes and their sums go to different slots of the wave's copy. Each
+        // node's slot takes its cells in a fixed pass order.
+        constexpr int A = G::TX / NW;
+        static_assert(A == 4 || (A == 2 && G::TY == 8), "2D tiles 4*NW cells wide, or 2*NW wide and 8 high");
+        constexpr int PASSES = A * G::TY / 4;
+        const int grp = l >> 4;
+        const int lx = w + NW * (A == 4 ? grp : (grp & 1));
+#pragma unroll 1
+        for (int it = 0; it < PASSES; it++) {
+            const int ly = A == 4 ? it : it + 4 * (grp >> 1);
+            const int c = ly * G::TX + lx;
+            const int b = max(s_cs[c], r0), e = min(s_cs[c + 1], r1);
+            if (__ballot(b < e) == 0ull) continue;      // none of the four cells has particles here
+            double acc[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc[q] = 0.;
+            int k = b;
+            for (; k + 4 <= e; k += 4) {
+                auto a0 = load(k - r0, o1x, o1y, o1z);
+                auto a1 = load(k + 1 - r0, o1x, o1y, o1z);
+                auto a2 = load(k + 2 - r0, o1x, o1y, o1z);
+                auto a3 = load(k + 3 - r0, o1x, o1y, o1z);
+                add(a0, acc, lx, ly, 0);
+                add(a1, acc, lx, ly, 0);
+                add(a2, acc, lx, ly, 0);
+                add(a3, acc, lx, ly, 0);
+            }
+            for (; k < e; k++) {
+                auto a0 = load(k - r0, o1x, o1y, o1z);
+                add(a0, acc, lx, ly, 0);
+            }
+            if (b < e) {
+                const int slot = (ly + o1y) * WaveFoot<DIM>::WX + (lx + o1x - w);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) s_acc[w][q][slot] += acc[q];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        return;
+    }
+    const bool lane_on = true;
     for (int c = w; c < G::NC; c += NW) {
         int b = max(s_cs[c], r0), e = min(s_cs[c + 1], r1);
         if (b >= e) continue;
