@@ -375,6 +375,13 @@ def main():
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--graph-steps", type=int, default=8, help="serve: batch steps per graph / admission check")
     ap.add_argument("--max-resident", type=int, default=0, help="serve: most solves resident at once (0: no limit)")
+    ap.add_argument("--xpic-order", type=int, default=0,
+                    help="j2/clamped: XPIC(k) in every step through a PeriodicXPIC task (0: FLIP)")
+    ap.add_argument("--fmpm", action="store_true", help="with --xpic-order: FMPM(k) instead of XPIC(k)")
+    ap.add_argument("--normals", type=int, default=2,
+                    help="contact: MultiMaterialMode normals method (2 AVGG, 5 linear and 6 logistic regression)")
+    ap.add_argument("--archive", action="store_true",
+                    help="j2/clamped/contact: time writing one archive of the final state (to a temporary file)")
     ap.add_argument("--tile-shape", type=int, choices=[0, 1], default=0,
                     help="j2/clamped/contact: tile shape (0: 4x4x4 cells per tile, 1: 8x4x4); the batch tunes its own")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of sweep instances")
@@ -406,9 +413,11 @@ def main():
         # two neo-Hookean cylinders (radius 20, length 60 cells) closing at 20 m/s, lCPDI,
         # multimaterial contact with Coulomb friction 0.3, AVGG normals
         cells = a.cells if a.cells != 128 else 160
-        xml = configs.contact_cylinders_3d(cells=cells, radius=cells / 8., length=cells * 3. / 8., gap=4. * cells / 160.)
+        xml = configs.contact_cylinders_3d(cells=cells, radius=cells / 8., length=cells * 3. / 8., gap=4. * cells / 160.,
+                                           normals=a.normals)
+        nname = {0: "MAXG", 1: "MAXV", 2: "AVGG", 3: "OWNG", 5: "linear regression", 6: "logistic regression"}
         workload = (f"3D two-body contact: neo-Hookean cylinders r={cells / 8:g} L={cells * 3 / 8:g} cells on a "
-                    f"{cells}^3 grid, lCPDI, Coulomb mu=0.3, closing at 20 m/s")
+                    f"{cells}^3 grid, lCPDI, Coulomb mu=0.3, {nname.get(a.normals, a.normals)} normals, closing at 20 m/s")
         a.cells = cells
     else:
         xml = configs.j2_block_3d(cells=a.cells, lo=lo, hi=hi)
@@ -427,6 +436,9 @@ def main():
                  ["net", {"dir": 3, "load": 2.0e5}, {"dir": 1, "style": 3, "load": 1.0e4, "ftime": 20.}])])
             workload += ("; clamped on the bottom face's nodes, a cosine x velocity on one side face, "
                          "a net z load and a sine x load on the top face's particles")
+    if a.xpic_order > 0 and a.workload != "contact":
+        xml = configs.periodic_xpic(xml, order=a.xpic_order, fmpm=a.fmpm, steps=1)
+        workload += f"; {'FMPM' if a.fmpm else 'XPIC'}({a.xpic_order}) in every step (PeriodicXPIC)"
     prob = nm.Problem.from_xml(xml)
     grid, mats, run, npart, _ = prob.describe()
     dfields, ifields = prob.particles()
@@ -471,6 +483,22 @@ def main():
     st = sim.stats()
     print(f"# eager+events {value:.4e} particle-steps/s, graph replay {graph_value:.4e}; "
           f"left grid {st.leftGrid}, nan {st.nanParticles}", file=sys.stderr, flush=True)
+    archive = None
+    if a.archive:
+        # one archive of the final state (every particle row the device carries), written to a
+        # temporary file: device download, record packing and the file write
+        import tempfile
+        order = "iYYYYYNNNNNYNY"
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "bench.000001.mpm")
+            sim.sync()
+            ta = time.perf_counter()
+            sim.write_archive(path, order, time_ms=0.)
+            tsec = time.perf_counter() - ta
+            nbytes = os.path.getsize(path)
+        archive = {"order": order, "bytes": nbytes, "seconds": tsec, "MB_per_s": nbytes / tsec / 1e6}
+        print(f"# archive: {nbytes / 1e6:.1f} MB in {tsec:.3f} s ({nbytes / tsec / 1e9:.2f} GB/s)",
+              file=sys.stderr, flush=True)
     for g, (ms, n) in times.items():
         print(f"#   {g:14s} {ms / max(n, 1):9.4f} ms/launch ({n} launches)", file=sys.stderr)
 
@@ -530,7 +558,10 @@ def main():
                    "particles": npart, "grid_cells": [grid.nx, grid.ny, grid.nz], "active_nodes": active_nodes,
                    "material": "Neohookean x2 + CoulombFriction" if a.workload == "contact" else "IsoPlasticity J2, linear hardening",
                    "shape": "lCPDI" if a.workload == "contact" else "uGIMP", "method": "USAVG+ FLIP",
-                   "tile_shape": a.tile_shape, "graph_replay_value": graph_value},
+                   "tile_shape": a.tile_shape, "graph_replay_value": graph_value,
+                   **({"xpic": {"order": a.xpic_order, "fmpm": a.fmpm}} if a.xpic_order > 0 else {}),
+                   **({"normals": a.normals} if a.workload == "contact" else {}),
+                   **({"archive": archive} if archive else {})},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
