@@ -2580,31 +2580,51 @@ __global__ __launch_bounds__(NT) void k_contact_normals(KGrid g, KPart P, KBins 
         int numMats = 0;
         for (int f = 0; f < g.nfields; f++)
             if (N.cnt[f * nn + n] > 0) act[numMats++] = f;
-        // the particles seeing the node: cells i-2..i+1 (per axis) of every field
+        // the particles seeing the node: cells i-2..i+1 (per axis) of every field, in (field, z, y, x)
+        // cell order and sorted order inside a cell. Lane q < 4^DIM holds cell q's slot range; a wave
+        // prefix sum of the counts lays the field's particles out one after another, so every pass
+        // tests 64 of them whatever the cells hold (the list comes out in the same order).
+        constexpr int NCELL = DIM == 3 ? 64 : 16;
         int nl = 0;
-        for (int f = 0; f < g.nfields; f++)
-            for (int cz = (DIM == 3 ? k - 2 : 0); cz <= (DIM == 3 ? k + 1 : 0); cz++) {
-                if (cz < 0 || cz >= g.nz) continue;
-                for (int cy = j - 2; cy <= j + 1; cy++) {
-                    if (cy < 0 || cy >= g.ny) continue;
-                    for (int cx = i - 2; cx <= i + 1; cx++) {
-                        if (cx < 0 || cx >= g.nx) continue;
-                        const int key = cell_key<DIM>(g, f, cx, cy, cz);
-                        const int b = B.cellStart[key], en = B.cellStart[key + 1];
-                        for (int s0 = b; s0 < en; s0 += 64) {
-                            const int s = s0 + lane;
-                            const int p = s < en ? B.order[s] : -1;
-                            const bool in = p >= 0 && lr_member<DIM, SH>(g, P, p, i, j, DIM == 3 ? k : 0);
-                            const unsigned long long m = __ballot(in);
-                            if (in) {
-                                const int idx = nl + __popcll(m & ((1ull << lane) - 1ull));
-                                if (idx < LR_MAX) list[idx] = p;
-                            }
-                            nl += __popcll(m);
-                        }
-                    }
+        for (int f = 0; f < g.nfields; f++) {
+            int cb = 0, cn = 0;
+            if (lane < NCELL) {
+                const int cx = i - 2 + (lane & 3), cy = j - 2 + ((lane >> 2) & 3);
+                const int cz = DIM == 3 ? k - 2 + (lane >> 4) : 0;
+                if (cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny && cz >= 0 && cz < g.nz) {
+                    const int key = cell_key<DIM>(g, f, cx, cy, cz);
+                    cb = B.cellStart[key];
+                    cn = B.cellStart[key + 1] - cb;
                 }
             }
+            int incl = cn;                                   // inclusive prefix over the lanes
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int u = __shfl_up(incl, d);
+                if (lane >= d) incl += u;
+            }
+            const int total = __shfl(incl, 63);
+            for (int t0 = 0; t0 < total; t0 += 64) {
+                const int t = t0 + lane;
+                // the cell of flattened index t: the first lane whose inclusive prefix exceeds t
+                int lo = 0, hi = NCELL - 1;
+#pragma unroll
+                for (int it = 0; it < (DIM == 3 ? 6 : 4); it++) {
+                    const int mid = (lo + hi) >> 1;
+                    if (__shfl(incl, mid) > t) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int qb = __shfl(cb, lo), qe = __shfl(incl - cn, lo);
+                const int p = t < total ? B.order[qb + (t - qe)] : -1;
+                const bool in = p >= 0 && lr_member<DIM, SH>(g, P, p, i, j, DIM == 3 ? k : 0);
+                const unsigned long long m = __ballot(in);
+                if (in) {
+                    const int idx = nl + __popcll(m & ((1ull << lane) - 1ull));
+                    if (idx < LR_MAX) list[idx] = p;
+                }
+                nl += __popcll(m);
+            }
+        }
         if (nl > LR_MAX) {
             if (lane == 0) atomicAdd(st.lrOverflow, 1ull);   // a fatal error, reported after the step
             nl = LR_MAX;
