@@ -374,7 +374,8 @@ def main():
                          "object (0: none)")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--graph-steps", type=int, default=8, help="serve: batch steps per graph / admission check")
-    ap.add_argument("--max-resident", type=int, default=0, help="serve: most solves resident at once (0: no limit)")
+    ap.add_argument("--max-resident", type=int, default=3,
+                    help="served stream: most solves resident at once (0: no limit; 3 keeps p99 below one at a time)")
     ap.add_argument("--xpic-order", type=int, default=0,
                     help="j2/clamped: XPIC(k) in every step through a PeriodicXPIC task (0: FLIP)")
     ap.add_argument("--fmpm", action="store_true", help="with --xpic-order: FMPM(k) instead of XPIC(k)")
