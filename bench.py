@@ -374,8 +374,9 @@ def main():
                          "object (0: none)")
     ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--graph-steps", type=int, default=8, help="serve: batch steps per graph / admission check")
-    ap.add_argument("--max-resident", type=int, default=3,
-                    help="served stream: most solves resident at once (0: no limit; 3 keeps p99 below one at a time)")
+    ap.add_argument("--max-resident", type=int, default=None,
+                    help="served stream: most solves resident at once (0: no limit; default 3 for --workload serve, "
+                         "where it keeps p99 below one at a time, and no limit for the 8-request stream of the j2 line)")
     ap.add_argument("--xpic-order", type=int, default=0,
                     help="j2/clamped: XPIC(k) in every step through a PeriodicXPIC task (0: FLIP)")
     ap.add_argument("--fmpm", action="store_true", help="with --xpic-order: FMPM(k) instead of XPIC(k)")
@@ -391,6 +392,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy used to scale the HBM byte counters")
     a = ap.parse_args()
+    if a.max_resident is None:
+        a.max_resident = 3 if a.workload == "serve" else 0
     if a.steps is None:
         a.steps = {"j2": 500, "clamped": 500, "contact": 200}.get(a.workload, 100)
     if a.copy_calibration:
