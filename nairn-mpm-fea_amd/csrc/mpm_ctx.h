@@ -64,6 +64,7 @@ struct MpmCtx {
     long long ncap = 0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // graphs of graphSteps steps, by position parity
+    int graphPhase[2] = {0, 0};      // the first step's number mod the XPIC period each was captured for
     int graphSteps = 2;              // even: a graph ends on the parity it started from
     int parity = 0;                  // which position buffer holds the current positions
     double *posA[3] = {nullptr, nullptr, nullptr}, *posB[3] = {nullptr, nullptr, nullptr};

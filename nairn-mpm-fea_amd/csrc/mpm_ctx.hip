@@ -90,8 +90,26 @@ static void xpic_schedule(MpmCtx *c, int64_t st) {
     }
 }
 
-// every step of the run has the same order, so whole steps can be captured in a graph
-static bool xpic_uniform(const MpmCtx *c) { return c->run.xpicOrder <= 0 || c->run.xpicPeriodSteps == 1; }
+// Graphs hold whole steps with their XPIC orders fixed at capture. A schedule by step count
+// selects exactly the steps s with s % P == 0 (the next step starts at P and moves on by P), so a
+// graph of G steps, G a multiple of P, has the same orders at every launch that starts at the same
+// s % P: a graph is kept with that phase and captured again when a call starts at another one. A
+// schedule by time runs step by step.
+static bool xpic_graphable(const MpmCtx *c) {
+    const MpmRunParams &r = c->run;
+    if (r.xpicOrder <= 0 || r.xpicPeriodSteps == 1) return true;
+    return r.xpicPeriodSteps > 1 && c->graphSteps % r.xpicPeriodSteps == 0;
+}
+static int xpic_phase(const MpmCtx *c, int64_t st) {
+    const int P = c->run.xpicPeriodSteps;
+    return c->run.xpicOrder > 0 && P > 1 ? (int)(st % P) : 0;
+}
+// the order of step st under a schedule that xpic_graphable accepts
+static int xpic_order_at(const MpmCtx *c, int64_t st) {
+    const int P = c->run.xpicPeriodSteps;
+    if (c->run.xpicOrder <= 0) return 0;
+    return P == 1 || st % P == 0 ? c->run.xpicOrder : 0;
+}
 
 // the kernels of one whole step (NairnMPM::MPMStep task list, NairnMPM.cpp:952-1108) with the
 // XPIC order in c->xpicStep; with profiling on, an event is recorded between kernel groups
@@ -1364,23 +1382,30 @@ int mpmgpu_step(MpmCtx *c, int32_t nsteps) {
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
     int done = 0;
     const int G = c->graphSteps;
-    // a schedule that gives some steps XPIC and others not runs step by step
-    if (nsteps >= G && !c->profiling && xpic_uniform(c)) {
+    if (nsteps >= G && !c->profiling && xpic_graphable(c)) {
         int par = c->parity;
+        const int phase = xpic_phase(c, c->steps + 1);
+        if (c->graph[par] && c->graphPhase[par] != phase) {
+            (void)hipGraphExecDestroy(c->graph[par]);
+            c->graph[par] = nullptr;
+        }
         if (!c->graph[par]) {
             hipGraph_t gr;
-            c->xpicStep = c->run.xpicOrder > 0 ? c->run.xpicOrder : 0;
             HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < G; k++) mpmi::enqueue_step(c);
+            for (int k = 0; k < G; k++) {
+                c->xpicStep = xpic_order_at(c, c->steps + 1 + k);
+                mpmi::enqueue_step(c);
+            }
             HIPCHECK(c, hipStreamEndCapture(c->stream, &gr));
             hipError_t e = hipGraphInstantiate(&c->graph[par], gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
             if (e != hipSuccess) return c->fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            c->graphPhase[par] = phase;
             c->parity = par;      // capture does not run the work
         }
         for (; done + G <= nsteps; done += G) {
             HIPCHECK(c, hipGraphLaunch(c->graph[par], c->stream));
-            for (int k = 1; k <= G; k++) {      // the schedule of the graph's steps, all of one order
+            for (int k = 1; k <= G; k++) {      // the schedule's state through the graph's steps
                 xpic_schedule(c, c->steps + done + k);
                 c->xpicClock += c->run.timestep;
             }

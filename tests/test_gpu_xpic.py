@@ -2,8 +2,9 @@
 L2 <= 1e-9 of particle positions, displacements, velocities and stresses; grid forces after the
 particle update to 1e-9): orders 1 to 5, uGIMP, classic and lCPDI particles in 2D and 3D, grid
 velocity conditions (each increment's held components zeroed; XPIC's particle pass keeps the
-reactions in the forces), damping, schedules by steps and by time (steps of different orders run
-step by step, one order for every step runs in graphs), rows sorted into tile order, and the
+reactions in the forces), damping, schedules by steps and by time (a schedule by steps runs in
+graphs whose length is a multiple of its period, captured for the phase of each call's first
+step; by time, step by step), rows sorted into tile order, and the
 task-by-task path. Each case also checks that the run differs from FLIP, so the order was used."""
 import numpy as np
 import pytest
@@ -123,6 +124,26 @@ def test_schedules_by_steps_and_time():
         sim.step(31)
         ora.step(31)
         _check(sim, ora, 2, f"schedule {orders} +37")
+
+
+@pytest.mark.parametrize("order,fmpm,period,graph", [(3, False, 3, 6), (2, True, 2, 4), (2, False, 3, 8)])
+def test_periodic_schedule_in_graphs(order, fmpm, period, graph):
+    # every period-th step in graphs of `graph` steps: with graph a multiple of the period the steps
+    # run in graphs captured for the phase of each call's first step (calls below start at several
+    # phases); otherwise step by step
+    name = f"{'FMPM' if fmpm else 'XPIC'}({order}) every {period} steps, {graph}-step graphs"
+    print("case:", name, flush=True)
+    base = configs.grid_bcs(configs.elastic_block_2d(cells=16, block=8, vmax=3000.), HELD_2D)
+    prob = nm.Problem.from_xml(configs.periodic_xpic(base, order=order, fmpm=fmpm, steps=period))
+    sim = nm.Simulation.from_problem(prob)
+    ora = orc.Oracle.from_problem(prob)
+    sim.set_graph_steps(graph)
+    done = 0
+    for n in (1, 13, 30, 7, 25):
+        sim.step(n)
+        ora.step(n)
+        done += n
+        _check(sim, ora, 2, f"{name} step {done}")
 
 
 @pytest.mark.parametrize("fmpm", [False, True])
