@@ -95,7 +95,10 @@ typedef struct MpmMaterial {
     double friction;           /* CoulombFriction frictionCoeff (contact laws only; <0 stick) */
     double frictionStatic;     /* CoulombFriction frictionCoeffStatic (ignored if <= 0) */
     int32_t ujOption;          /* HyperElastic UofJOption */
-    int32_t pad;
+    int32_t usePDamping;       /* <PDamping> in <Material>: the material's particles take pdamping in
+                                  place of the run's particle damping (MaterialBase::GetMaterialDamping,
+                                  MaterialBaseMPM.cpp:731-744; UpdateParticlesTask.cpp:118) */
+    double pdamping;           /* 1/s */
 } MpmMaterial;
 
 /* Step parameters (NairnMPM.hpp:23 globals timestep, strainTimestepFirst/Last) */

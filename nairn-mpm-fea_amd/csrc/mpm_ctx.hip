@@ -165,7 +165,7 @@ void mpmi::enqueue_step(MpmCtx *c) {
     // the particle update of XPIC(k > 1) and FMPM(k > 1) takes the extrapolated grid velocities
     // (UpdateParticlesTask.cpp:66-79)
     if (xo > 1) xpic(true);
-    launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, xo == 0 ? 0 : (fmpm ? xo : -xo), s);
+    launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, xo == 0 ? 0 : (fmpm ? xo : -xo), s);
     done(MPMGPU_KG_G2P_P2G);
     launch_node_mom2(c->dim, c->kg, c->kb, c->kn, s);
     if (!mm) vel_bcs(BC_PASS_UPDATE_STRAINS_LAST);
@@ -703,6 +703,7 @@ const char *mpmi::single_grid(MpmCtx *c, const MpmGridDesc *grid, const MpmMater
     c->mats.assign(mats, mats + nmat);
     KGrid &g = c->kg;
     fill_grid(g, c->dim, grid, run);
+    for (int m = 0; m < nmat; m++) g.damped |= mats[m].usePDamping;   // a material's own particle damping
     g.nfields = nf;
     for (int m = 0; m < MPMGPU_MAX_MATERIALS; m++)
         g.fieldOf[m] = (nf > 1 && m < nmat && mats[m].field >= 0 && mats[m].field < nf) ? mats[m].field : 0;
@@ -879,6 +880,7 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
     g.nstride = nodeAcc;
     g.tilesTotal = tileAcc;
     for (auto &gi : c->inst) g.damped |= gi.damped;     // the launch takes the damped instance
+    for (int m = 0; m < nmat; m++) g.damped |= mats[m].usePDamping;
     if (alloc_context(c, mats, nmat)) { mpmgpu_destroy(c); return 1; }
     // instance tables on the device, with room for the tile count of either tile shape
     for (int sh = 0; sh < MPM_TILE_SHAPES; sh++) c->tileCap = std::max(c->tileCap, (int)mpmi::tiles_total(c, sh));
@@ -1339,7 +1341,7 @@ int mpmgpu_run_task(MpmCtx *c, int32_t task) {
         case MPMGPU_TASK_UPDATE_PARTICLES: {
             const int xo = c->xpicStep;
             if (xo > 1) launch_xpic(c->dim, c->kg, c->kp, c->kb, c->kn, c->kbc, xo, fmpm, true, c->parity, s);
-            launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, xo == 0 ? 0 : (fmpm ? xo : -xo), s);
+            launch_g2p_p2g(c->dim, c->kg, c->kp, c->kb, c->kn, c->dmats, xo == 0 ? 0 : (fmpm ? xo : -xo), s);
             break;
         }
         case MPMGPU_TASK_UPDATE_STRAINS_LAST:

@@ -1455,7 +1455,7 @@ __global__ void k_node_force(KGrid g0, KBins B, KNodes N) {
 enum { G2P_FLIP = 0, G2P_FLIP_DAMPED = 1, G2P_SERIES = 2 };
 template <int DIM, int SH, int MOVE>
 __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void k_g2p_p2g(KGrid g0, KPart P, KBins B, KNodes N,
-                                                                                  int mode) {
+                                                                                  const MpmMaterial *mats, int mode) {
     constexpr bool DAMP = MOVE != G2P_FLIP;
     using G = TileGeom<DIM>;
     constexpr bool CP = SH == MPMGPU_SHAPE_LCPDI;
@@ -1553,6 +1553,13 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void 
                         }
                 }
                 double vnew[3] = {0., 0., 0.};
+                // particle damping: the material's own <PDamping> in place of the run's
+                // (MaterialBase::GetMaterialDamping, MaterialBaseMPM.cpp:742-744)
+                double alphaP = 0.;
+                if (DAMP) {
+                    const MpmMaterial &mt = mats[P.mat[p]];
+                    alphaP = mt.usePDamping ? mt.pdamping : g.particleDamping;
+                }
                 // MatPoint3D::MoveParticle (MatPoint3D.cpp:104-194): Adamp0 = alpha_p v + alpha_g vm
                 // (the DAMP instance; zero damping gives delV = Sacc dt for FLIP)
                 if (MOVE != G2P_SERIES) {
@@ -1560,7 +1567,7 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void 
 #pragma unroll
                     for (int c = 0; c < DIM; c++) {
                         double vm = Svk[c] - Sacc[c] * dt;
-                        double delV = DAMP ? (Sacc[c] - (vold[c] * g.particleDamping + vm * g.gridDamping)) * dt
+                        double delV = DAMP ? (Sacc[c] - (vold[c] * alphaP + vm * g.gridDamping)) * dt
                                            : Sacc[c] * dt;
                         vnew[c] = vold[c] + delV;
                         P.f[MPF_VEL_X + c][p] = vnew[c];
@@ -1574,7 +1581,7 @@ __global__ __launch_bounds__(NT, SH == MPMGPU_SHAPE_LCPDI ? 1 : G2P_WAVES) void 
 #pragma unroll
                     for (int c = 0; c < DIM; c++) {
                         const double vm = Svk[c];
-                        const double adamp = DAMP ? vold[c] * g.particleDamping + vm * g.gridDamping : 0.;
+                        const double adamp = DAMP ? vold[c] * alphaP + vm * g.gridDamping : 0.;
                         if (mode > 0) {
                             vnew[c] = vm + adamp * -dt;
                             const double delXRate = vold[c] + vnew[c];
@@ -3088,23 +3095,25 @@ void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd
 }
 
 template <int MOVE>
-static void g2p_move(int dim, const KGrid &g, const KPart &p, const KBins &tb, const KNodes &nd, int mode, hipStream_t s) {
+static void g2p_move(int dim, const KGrid &g, const KPart &p, const KBins &tb, const KNodes &nd,
+                     const MpmMaterial *mats, int mode, hipStream_t s) {
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
     if (dim == 3) {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, MOVE>), tb, s, g, p, tb, nd, mode);
-        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, MOVE>), tb, s, g, p, tb, nd, mode);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_LCPDI, MOVE>), tb, s, g, p, tb, nd, mats, mode);
+        else TILE_LAUNCH((k_g2p_p2g<3, MPMGPU_SHAPE_UGIMP, MOVE>), tb, s, g, p, tb, nd, mats, mode);
     } else {
-        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, MOVE>), tb, s, g, p, tb, nd, mode);
-        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, MOVE>), tb, s, g, p, tb, nd, mode);
+        if (cp) TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_LCPDI, MOVE>), tb, s, g, p, tb, nd, mats, mode);
+        else TILE_LAUNCH((k_g2p_p2g<2, MPMGPU_SHAPE_UGIMP, MOVE>), tb, s, g, p, tb, nd, mats, mode);
     }
 }
 
-void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, int mode, hipStream_t s) {
+void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                    const MpmMaterial *mats, int mode, hipStream_t s) {
     const KBins tb = tile_bins(b, TILE_SLOT_G2P);
     // damping and the FMPM/XPIC moves have their own instances, so the plain FLIP update keeps its registers
-    if (mode != 0) g2p_move<G2P_SERIES>(dim, g, p, tb, nd, mode, s);
-    else if (g.damped) g2p_move<G2P_FLIP_DAMPED>(dim, g, p, tb, nd, mode, s);
-    else g2p_move<G2P_FLIP>(dim, g, p, tb, nd, mode, s);
+    if (mode != 0) g2p_move<G2P_SERIES>(dim, g, p, tb, nd, mats, mode, s);
+    else if (g.damped) g2p_move<G2P_FLIP_DAMPED>(dim, g, p, tb, nd, mats, mode, s);
+    else g2p_move<G2P_FLIP>(dim, g, p, tb, nd, mats, mode, s);
 }
 
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {

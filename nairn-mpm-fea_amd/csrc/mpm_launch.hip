@@ -35,9 +35,9 @@ void launch_p2g_force(int dim, const KGrid &g, const KPart &p, const KBins &b, c
 void launch_node_force(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
     BY_SHAPE(g, launch_node_force(dim, g, b, nd, s));
 }
-void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd, int mode,
-                    hipStream_t s) {
-    BY_SHAPE(g, launch_g2p_p2g(dim, g, p, b, nd, mode, s));
+void launch_g2p_p2g(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
+                    const MpmMaterial *mats, int mode, hipStream_t s) {
+    BY_SHAPE(g, launch_g2p_p2g(dim, g, p, b, nd, mats, mode, s));
 }
 void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd, hipStream_t s) {
     BY_SHAPE(g, launch_node_mom2(dim, g, b, nd, s));

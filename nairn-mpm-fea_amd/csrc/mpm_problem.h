@@ -20,6 +20,7 @@ struct MaterialInput {
     double friction = 0.;            // CoulombFriction::CoulombFriction defaults
     double frictionStatic = -1.;
     int ujOption = 0;
+    double pdamping = -1.e12;        // <PDamping> in <Material> (MPMReadHandler.cpp:319-324): none below -1e12
     bool readE = false, readNu = false, readG = false;
 };
 

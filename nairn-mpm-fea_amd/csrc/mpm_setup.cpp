@@ -117,6 +117,10 @@ bool material_verify(int analysis, const MaterialInput &in, MpmMaterial &o, std:
     o.type = in.type;
     o.rho = in.rho * 0.001;                      // MaterialBaseMPM.cpp:56-58 legacy scaling
     o.ujOption = in.ujOption;
+    if (in.pdamping > -1.e12) {                  // MaterialBase::SetDamping (MaterialBaseMPM.cpp:731-738)
+        o.usePDamping = 1;
+        o.pdamping = in.pdamping;
+    }
     switch (in.type) {
         case MPMGPU_MAT_ISOTROPIC:
         case MPMGPU_MAT_ISOPLASTICITY: {

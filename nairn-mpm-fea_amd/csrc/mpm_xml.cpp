@@ -485,8 +485,13 @@ struct Reader {
             else if (n == "coeff") in.friction = v;
             else if (n == "coeffStatic") in.frictionStatic = v;
             else if (n == "UJOption") in.ujOption = (int)v;
-            else if (n == "Friction" || n == "SharedMaterialField" || n == "ArtificialVisc" || n == "ArtificialViscosity" ||
-                     n == "PDamping")
+            else if (n == "PDamping") {
+                // MPMReadHandler.cpp:311-343: a constant in 1/s; no function of time for a material
+                if (k.attr("function")) return fail("Material damping function of time is not allowed");
+                if (k.attr("PIC")) return fail("PIC attribute on damping remove; use PeriodicXPIC task intead");
+                in.pdamping = v;
+            }
+            else if (n == "Friction" || n == "SharedMaterialField" || n == "ArtificialVisc" || n == "ArtificialViscosity")
                 return fail("<" + n + "> in <Material> is not on this path");
             else if (n == "Hardening") {
                 if (k.text.find("Linear") == std::string::npos && (int)v != 1 && !k.text.empty() && k.text.find_first_not_of(" \t\r\n") != std::string::npos)

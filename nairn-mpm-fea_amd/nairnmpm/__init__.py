@@ -65,7 +65,8 @@ class Material(ctypes.Structure):
                 ("alphaMax", ctypes.c_double), ("Gsp", ctypes.c_double), ("Lamesp", ctypes.c_double),
                 ("Ksp", ctypes.c_double), ("G1sp", ctypes.c_double), ("G2sp", ctypes.c_double),
                 ("waveSpeed", ctypes.c_double), ("friction", ctypes.c_double),
-                ("frictionStatic", ctypes.c_double), ("ujOption", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("frictionStatic", ctypes.c_double), ("ujOption", ctypes.c_int32), ("usePDamping", ctypes.c_int32),
+                ("pdamping", ctypes.c_double)]
 
 
 class RunParams(ctypes.Structure):

@@ -1917,12 +1917,16 @@ struct Oracle {
                 }
             }
             int nc = g.dim == 3 ? 3 : 2;
+            // the material may replace the particle damping (MaterialBase::GetMaterialDamping,
+            // MaterialBaseMPM.cpp:742-744, called at UpdateParticlesTask.cpp:118)
+            const MpmMaterial &pm = mats[I(MPI_MAT, p)];
+            const double particleAlpha = pm.usePDamping ? pm.pdamping : run.particleDamping;
             for (int c = 0; c < nc; c++) {
                 const double vold = F(MPF_VEL_X + c, p);
                 // vm: S(v - a dt) for FLIP, Sv+(k) for FMPM, S(v(k) + a dt) for XPIC (:106-116)
                 double vm = m == 0 ? Svk[c] - Sacc[c] * dt : Svk[c];
                 // Adamp0 = particleAlpha vel + gridAlpha vm (MatPoint3D.cpp:117-120)
-                double adamp = vold * run.particleDamping + vm * run.gridDamping;
+                double adamp = vold * particleAlpha + vm * run.gridDamping;
                 if (m > 0) {
                     // FMPM: v = vm - Adamp0 dt, x += 0.5 (v(n+1) + v(n)) dt (:122-137)
                     double vnew = vm + adamp * -dt;

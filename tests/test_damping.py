@@ -11,6 +11,7 @@ from nairnmpm import configs
 import oracle as orc
 
 F = nm.F
+I = nm.I
 
 
 def _with_header(xml, extra):
@@ -34,10 +35,58 @@ def test_header_refusals(extra, msg):
         nm.Problem.from_xml(_with_header(configs.elastic_block_2d(cells=16, block=8), extra))
 
 
-def test_material_damping_refused():
-    xml = configs.elastic_block_2d(cells=16, block=8).replace("<nu>", "<PDamping>10</PDamping>\n    <nu>", 1)
-    with pytest.raises(nm.MPMError, match="PDamping"):
-        nm.Problem.from_xml(xml)
+def _material_damping(xml, value, block=1):
+    """xml with <PDamping> in the material named 'block <block>' (or the first material)"""
+    tag = f'Name="block {block}">' if f'Name="block {block}">' in xml else "<Material "
+    i = xml.index(tag)
+    j = xml.index("<nu>", i)
+    return xml[:j] + f"<PDamping>{value!r}</PDamping>\n    " + xml[j:]
+
+
+def test_material_damping_settings():
+    # <PDamping> in <Material> (MPMReadHandler.cpp:319-324, MaterialBase::SetDamping): a constant
+    # in 1/s that the material's particles take in place of the run's particle damping
+    xml = _material_damping(configs.elastic_block_2d(cells=16, block=8), 125.5)
+    _, mats, run, _, _ = nm.Problem.from_xml(xml).describe()
+    assert mats[0].usePDamping == 1 and mats[0].pdamping == 125.5 and run.particleDamping == 0.
+    _, mats, _, _, _ = nm.Problem.from_xml(_material_damping(configs.elastic_block_2d(cells=16, block=8), 0.)).describe()
+    assert mats[0].usePDamping == 1 and mats[0].pdamping == 0.
+    _, mats, _, _, _ = nm.Problem.from_xml(configs.elastic_block_2d(cells=16, block=8)).describe()
+    assert mats[0].usePDamping == 0
+    bad = configs.elastic_block_2d(cells=16, block=8).replace("<nu>", '<PDamping function="t">10</PDamping>\n    <nu>', 1)
+    with pytest.raises(nm.MPMError, match="Material damping function of time is not allowed"):
+        nm.Problem.from_xml(bad)
+
+
+def _two_blocks(ga, pa, ma):
+    """the contact blocks in single-material mode (one velocity field), translating rigidly toward
+    each other; block 1's material has its own particle damping ma"""
+    xml = configs.contact_blocks_2d(cells=40, gap=6., vx=3000., vy=-1000.)   # no shared nodes
+    i, j = xml.index("    <MultiMaterialMode"), xml.index("</MultiMaterialMode>\n") + len("</MultiMaterialMode>\n")
+    xml = _with_header(xml[:i] + xml[j:], f"    <Damping>{ga!r}</Damping>\n    <PDamping>{pa!r}</PDamping>\n")
+    return _material_damping(xml, ma, block=1)
+
+
+def test_material_damping_replaces_the_run_particle_damping():
+    # two rigidly translating blocks: block 1 (material PDamping ma) decays as (1 - (ma + ga) dt)^k,
+    # block 2 as (1 - (pa + ga) dt)^k with the run's particle damping
+    ga, pa, ma = 700., 2500., 300.
+    prob = nm.Problem.from_xml(_two_blocks(ga, pa, ma))
+    g, mats, run, _, _ = prob.describe()
+    assert run.multiMaterial == 0 and mats[0].usePDamping == 1 and mats[1].usePDamping == 0
+    o = orc.Oracle.from_problem(prob)
+    d0, i0 = o.particles()
+    mat = i0[I["mat"]]
+    dt = run.timestep
+    for k in range(1, 16):
+        o.step(1)
+        r, _ = o.particles()
+        for m, a in ((0, ma), (1, pa)):
+            sel = mat == m
+            f = (1. - (a + ga) * dt) ** k
+            # block 1 has no y velocity: rounding of order 1e-13 mm/s there
+            np.testing.assert_allclose(r[F["vel_x"]][sel], d0[F["vel_x"]][sel] * f, rtol=1e-11, atol=1e-9)
+            np.testing.assert_allclose(r[F["vel_y"]][sel], d0[F["vel_y"]][sel] * f, rtol=1e-11, atol=1e-9)
 
 
 @pytest.mark.parametrize("ga, pa", [(3000., 0.), (0., 4000.), (1500., 2500.)])

@@ -63,3 +63,36 @@ def test_batch_with_damped_and_undamped_instances():
         ora.step(30)
         errs = compare(d, ora.particles()[0], 2)
         assert max(errs.values()) <= TOL, (k, errs)
+
+
+def _material_damping(xml, value, block=1):
+    """xml with <PDamping> in the material named 'block <block>' (or the first material)"""
+    tag = f'Name="block {block}">' if f'Name="block {block}">' in xml else "<Material "
+    i = xml.index(tag)
+    j = xml.index("<nu>", i)
+    return xml[:j] + f"<PDamping>{value!r}</PDamping>\n    " + xml[j:]
+
+
+@pytest.mark.parametrize("label, make, checks", [
+    ("2D block, material damping only", lambda: _material_damping(configs.elastic_block_2d(cells=16, block=8), 900.),
+     (1, 20, 60)),
+    ("2D block, material damping over the run's", lambda: _material_damping(
+        _with_header(configs.elastic_block_2d(cells=16, block=8, shape="lCPDI"),
+                     "    <Damping>2000</Damping>\n    <PDamping>1500</PDamping>\n"), 250.), (1, 20, 60)),
+    ("2D contact blocks, block 1 damped", lambda: _material_damping(
+        _with_header(configs.contact_blocks_2d(normals=2, law_type=61, friction=0., vy=3000.),
+                     "    <PDamping>800</PDamping>\n"), 100.), (60, 200)),
+])
+def test_material_damping_parity(label, make, checks):
+    # <PDamping> in <Material> replaces the run's particle damping for that material's particles
+    print(f"case: {label}", flush=True)
+    prob = nm.Problem.from_xml(make())
+    assert any(m.usePDamping for m in prob.describe()[1])
+    sim = nm.Simulation.from_problem(prob)
+    ora = orc.Oracle.from_problem(prob)
+    done = 0
+    for target in checks:
+        sim.step(target - done)
+        ora.step(target - done)
+        done = target
+        _check(sim, ora, 2, f"{label} step {target}")
