@@ -3,7 +3,8 @@ at +-50 m/s): two steps against the oracle in its threaded timing mode (relative
 positions, displacements, velocities and stresses), and size-independent properties over longer
 runs: total momentum conserved (no conditions or body forces act), grid mass equal to particle
 mass, and the same bits from a second context that sorts its rows every step and steps without
-graphs."""
+graphs. The contact cylinders (160^3), the 256-instance batched sweep and served requests of the 3D
+stream at their own sizes follow below."""
 import os
 
 import numpy as np
@@ -122,3 +123,44 @@ def test_full_size_sweep_members_equal_alone():
     print("member 0 vs oracle", {k: f"{v:.3e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v <= 1e-9, f"member 0: {k} relative L2 {v:.3e}"
+
+
+def test_full_size_served_requests_equal_alone():
+    # four requests of the served 3D stream at its own sizes (50k-2M particles, 200-2000 steps,
+    # J2 and neo-Hookean), two at once and two while those run: every served result equals the
+    # same solve alone bit for bit, and the smallest matches the oracle over its first steps
+    print("case: served 3D stream, full-size requests", flush=True)
+    reqs = []
+    for k in (3, 4, 10, 12):        # 55k and 1.9M neo-Hookean, 475k and 111k J2 particles
+        xml, steps, _ = configs.serve_request_3d(k)
+        reqs.append((nm.Problem.from_xml(xml), steps))
+    print("requests", [(p.describe()[3], s) for p, s in reqs], flush=True)
+    srv = nm.Server(analysis=nm.THREED, graph_steps=8)
+    ids = [srv.submit_problem(*reqs[0]), srv.submit_problem(*reqs[1])]
+    srv.advance(40)
+    ids += [srv.submit_problem(*reqs[2]), srv.submit_problem(*reqs[3])]
+    rounds = 0
+    while any(srv.info(r)["state"] < nm.REQ_DONE for r in ids):
+        srv.advance(8)
+        rounds += 1
+        assert rounds < 2000
+    for (prob, steps), rid in zip(reqs, ids):
+        d, i = srv.collect(rid)
+        g, mats, run, _, _ = prob.describe()
+        sim = nm.Simulation(g, mats, run)
+        sim.upload(*prob.particles())
+        sim.step(steps)
+        assert np.array_equal(d, sim.particles()[0]), f"request of {prob.describe()[3]} particles differs alone"
+        sim.close()
+    srv.close()
+    k = int(np.argmin([p.describe()[3] for p, _ in reqs]))
+    prob, _ = reqs[k]
+    sim = nm.Simulation.from_problem(prob)
+    ora = orc.Oracle.from_problem(prob)
+    ora.set_threads(min(16, os.cpu_count() or 1))
+    sim.step(20)
+    ora.step(20)
+    errs = compare(sim.particles()[0], ora.particles()[0], 3)
+    print(f"request of {prob.describe()[3]} particles, 20 steps vs oracle", {k: f"{v:.3e}" for k, v in errs.items()}, flush=True)
+    for key, v in errs.items():
+        assert v <= 1e-9, f"{key} relative L2 {v:.3e}"
