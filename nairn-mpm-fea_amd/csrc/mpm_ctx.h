@@ -60,6 +60,8 @@ struct MpmCtx {
     int *bbuf = nullptr;              // binning arrays (bbufBytes: enough for either tile shape)
     size_t bbufBytes = 0;
     int tileCap = 0;                  // batch: entries of the tile -> instance table dtables holds
+    std::vector<int> instType;        // batch: each instance's material type (0 mixed, -1 no particles)
+    int *dtileType = nullptr;         // batch: tile -> its instance's type (tileCap entries)
     unsigned long long *sbuf = nullptr;
     long long ncap = 0;
     hipStream_t stream = nullptr;
@@ -138,6 +140,9 @@ long long tiles_total(const MpmCtx *c, int shape);
 // switch the context to another tile shape: tile layout, bins and instance tables, then the
 // particles binned again and their rows sorted (results do not depend on the row order)
 int set_tile_shape(MpmCtx *c, int shape);
+// a batch's instance material types (0: the instance mixes types, -1: no particles): the types the
+// strain updates launch for and the tile -> type table of the current layout
+int set_instance_types(MpmCtx *c, const std::vector<int> &types);
 // ints of the binning arrays for nk cell keys and ntall (field, tile) pairs
 size_t bin_words(long long nk, long long ntall);
 // byte sizes of a context's grid, bin and counter arrays for its grid c->kg and nmat materials

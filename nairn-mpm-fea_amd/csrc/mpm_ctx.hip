@@ -778,6 +778,40 @@ static int upload_batch_tables(MpmCtx *c) {
     return 0;
 }
 
+// A batch's tile -> material type table for the current layout (kg.tileType), from the instance
+// types of the last upload
+static int upload_tile_types(MpmCtx *c) {
+    if (!c->dtileType || c->instType.size() != c->inst.size()) return 0;
+    std::vector<int> tt((size_t)c->kg.tilesTotal, -1);
+    for (size_t k = 0; k < c->inst.size(); k++)
+        for (int t = 0; t < c->inst[k].ntiles; t++) tt[(size_t)c->inst[k].tileBase + t] = c->instType[k];
+    HIPCHECK(c, hipMemcpy(c->dtileType, tt.data(), tt.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->kg.tileType = c->dtileType;
+    return 0;
+}
+
+int mpmi::set_instance_types(MpmCtx *c, const std::vector<int> &types) {
+    c->instType = types;
+    KGrid &g = c->kg;
+    g.ntypes = 0;
+    g.anyMixed = 0;
+    for (int it : c->instType) {
+        if (it == 0) { g.anyMixed = 1; continue; }
+        if (it < 0) continue;
+        bool have = false;
+        for (int j = 0; j < g.ntypes; j++) have |= g.types[j] == it;
+        if (!have && g.ntypes < 4) g.types[g.ntypes++] = it;
+        else if (!have) g.anyMixed = 1;
+    }
+    if (g.anyMixed)          // a type beyond the four listed (none on this path) takes the all-laws launch
+        for (int &it : c->instType) {
+            bool listed = false;
+            for (int j = 0; j < g.ntypes; j++) listed |= g.types[j] == it;
+            if (it > 0 && !listed) it = 0;
+        }
+    return upload_tile_types(c);
+}
+
 // The tile layout of the context for another tile shape: the grid's (or each instance's) tiles,
 // a batch's tables, the bin pointers and the loaded particles' per-tile flags. With rebin the
 // particles are binned for it, the bins' completed-step count kept, every node visited by the
@@ -801,7 +835,7 @@ static int apply_tile_shape(MpmCtx *c, int shape, bool rebin) {
             g.tileShape = shape;
             g.ntx = g0.ntx; g.nty = g0.nty; g.ntz = g0.ntz; g.ntiles = g0.ntiles;
             g.tilesTotal = acc;
-            if (upload_batch_tables(c)) return 1;
+            if (upload_batch_tables(c) || upload_tile_types(c)) return 1;
         }
         // the histogram and cursors of the new layout sit where the old one kept other arrays:
         // every bin array starts from zero, as at creation
@@ -885,7 +919,8 @@ int mpmgpu_batch_create(int32_t ninst, const MpmInstanceDesc *desc, const MpmMat
     // instance tables on the device, with room for the tile count of either tile shape
     for (int sh = 0; sh < MPM_TILE_SHAPES; sh++) c->tileCap = std::max(c->tileCap, (int)mpmi::tiles_total(c, sh));
     if (hipMalloc(&c->dinst, ninst * sizeof(KGrid)) != hipSuccess ||
-        hipMalloc(&c->dtables, ((size_t)c->tileCap + nodeAcc / 256 + nmat) * sizeof(int)) != hipSuccess) {
+        hipMalloc(&c->dtables, ((size_t)c->tileCap + nodeAcc / 256 + nmat) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&c->dtileType, (size_t)c->tileCap * sizeof(int)) != hipSuccess) {
         mpmgpu_destroy(c);
         return fail_global("out of device memory allocating the batch tables");
     }
@@ -1050,6 +1085,15 @@ int mpmgpu_upload_particles(MpmCtx *c, int64_t n, const double *dfields, const i
             }
         }
         for (auto &gi : c->inst) gi.matType = c->kg.matType;
+        // each instance's material type, for the strain updates' per-type launches
+        std::vector<int> types(c->inst.size(), -1);
+        for (int64_t i = 0; i < n; i++) {
+            const int m = ifields[MPI_MAT * nn + i];
+            int &it = types[(size_t)c->matInstH[m]];
+            const int t = c->mats[m].type;
+            it = it == -1 || it == t ? t : 0;
+        }
+        if (mpmi::set_instance_types(c, types)) return 1;
         HIPCHECK(c, hipMemcpyAsync(c->dinst, c->inst.data(), c->inst.size() * sizeof(KGrid), hipMemcpyHostToDevice,
                                    c->stream));
     }
@@ -1683,6 +1727,7 @@ void mpmgpu_destroy(MpmCtx *c) {
     if (c->cbuf) (void)hipFree(c->cbuf);
     if (c->dinst) (void)hipFree(c->dinst);
     if (c->dtables) (void)hipFree(c->dtables);
+    if (c->dtileType) (void)hipFree(c->dtileType);
     if (c->bbuf) (void)hipFree(c->bbuf);
     if (c->kn.mass) (void)hipFree(c->kn.mass);
     if (c->kn.cnt) (void)hipFree(c->kn.cnt);

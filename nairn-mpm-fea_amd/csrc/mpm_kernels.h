@@ -49,6 +49,11 @@ struct KGrid {
     int hasGravity;
     int uniformLp;             // 1 when every particle has the same lp (then lp[] is used)
     int matType;               // the one material type of all particles, 0 when mixed
+    // a batch whose instances differ in material type: each tile's type (its instance's, 0 when the
+    // instance mixes types, -1 when it has no particles) and the types present, so the strain
+    // updates run one launch per type with that law's registers (null / 0 otherwise)
+    const int *tileType;
+    int ntypes, types[4], anyMixed;
     double lp[3];
     // material velocity fields (multimaterial mode when nfields > 1)
     int nfields;
@@ -114,8 +119,11 @@ struct KBins {
 
 // launches of the tile kernels in one step, each with its own cursors (cleared when the list is built)
 // (the XPIC iterations share one slot, whose cursors each iteration's node pass clears again)
+// (per-type strain launches of a batch take TILE_SLOT_TYPED + j: j < 4 the first half step's,
+// 4 + j the second's)
 enum { TILE_SLOT_MASS, TILE_SLOT_CONTACT_FIRST, TILE_SLOT_USF, TILE_SLOT_FORCE, TILE_SLOT_G2P,
-       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOT_FORCE_LOADS, TILE_SLOT_XPIC, TILE_SLOTS };
+       TILE_SLOT_CONTACT_LAST, TILE_SLOT_USL, TILE_SLOT_FORCE_LOADS, TILE_SLOT_XPIC, TILE_SLOT_TYPED,
+       TILE_SLOTS = TILE_SLOT_TYPED + 8 };
 #define TILE_SLOT_STRIDE 32
 
 struct KNodes {

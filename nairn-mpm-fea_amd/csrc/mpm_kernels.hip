@@ -1169,11 +1169,13 @@ __device__ __forceinline__ void store_state(const KPart &P, int p, int type, con
 // USF strain update, first half step (UpdateStrainsFirstTask::FullStrainUpdate,
 // UpdateStrainsFirstTask.cpp:101-168 -> MatPoint3D::UpdateStrain, MatPoint3D.cpp:45-60)
 template <int DIM, int MAT, int SH>
-__global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats) {
+__global__ __launch_bounds__(NT, 2) void k_usf(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
+                                               int only) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
     auto tile = [&](const int tall) {      // (field, context tile) pair
+        if (only >= 0 && g0.tileType[tall % g0.tilesTotal] != only) return;   // another launch's type
         const int fld = tall / g0.tilesTotal;
         const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
         const int t = tall - fld * g0.tilesTotal - g.tileBase;
@@ -1925,11 +1927,12 @@ __device__ __forceinline__ bool pt_in_elem(const KGrid &g, int ei, int ej, int e
 // of the next binning
 template <int DIM, int MAT, int SH>
 __global__ __launch_bounds__(NT, 2) void k_usl_reset(KGrid g0, KPart P, KBins B, KNodes N, const MpmMaterial *mats,
-                                                  KStats st) {
+                                                  KStats st, int only) {
     using G = TileGeom<DIM>;
     __shared__ int s_cs[G::NC + 1];
     __shared__ double s_v[3][G::FOOT];
     auto tile = [&](const int tall) {      // (field, context tile) pair
+        if (only >= 0 && g0.tileType[tall % g0.tilesTotal] != only) return;   // another launch's type
         const int fld = tall / g0.tilesTotal;
         const KGrid g = tile_grid(g0, tall - fld * g0.tilesTotal);
         const int t = tall - fld * g0.tilesTotal - g.tileBase;
@@ -2860,32 +2863,47 @@ void launch_node_mass(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
 
 template <int DIM, int SH>
 static void usf_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                    const MpmMaterial *mats, hipStream_t s) {
-    const KBins tb = tile_bins(b, TILE_SLOT_USF);
+                    const MpmMaterial *mats, int only, int slot, hipStream_t s) {
+    const KBins tb = tile_bins(b, slot);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_ISOTROPIC, SH>), tb, s, g, p, tb, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_ISOTROPIC, SH>), tb, s, g, p, tb, nd, mats, only); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), tb, s, g, p, tb, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), tb, s, g, p, tb, nd, mats, only); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), tb, s, g, p, tb, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), tb, s, g, p, tb, nd, mats, only); break;
         case MPMGPU_MAT_MOONEY:
-            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_MOONEY, SH>), tb, s, g, p, tb, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, MPMGPU_MAT_MOONEY, SH>), tb, s, g, p, tb, nd, mats, only); break;
         default:
-            TILE_LAUNCH((k_usf<DIM, 0, SH>), tb, s, g, p, tb, nd, mats); break;
+            TILE_LAUNCH((k_usf<DIM, 0, SH>), tb, s, g, p, tb, nd, mats, only); break;
     }
+}
+
+// the strain updates of a launch: one launch with the law of every particle when they share a
+// type (or the grid has no per-tile types), else one launch per type present, each taking only the
+// tiles of its type, and one with every law for tiles whose instance mixes types
+template <typename L>
+static void by_type(const KGrid &g, int firstTypedSlot, int plainSlot, L launch) {
+    if (g.matType != 0 || !g.tileType) {
+        launch(g.matType, -1, plainSlot);
+        return;
+    }
+    for (int j = 0; j < g.ntypes; j++) launch(g.types[j], g.types[j], firstTypedSlot + j);
+    if (g.anyMixed) launch(0, 0, plainSlot);
 }
 
 void launch_usf(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                 const MpmMaterial *mats, hipStream_t s) {
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
-    if (dim == 3) {
-        if (cp) usf_mat<3, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, s);
-        else usf_mat<3, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, s);
-    } else {
-        if (cp) usf_mat<2, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, s);
-        else usf_mat<2, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, s);
-    }
+    by_type(g, TILE_SLOT_TYPED, TILE_SLOT_USF, [&](int mat, int only, int slot) {
+        if (dim == 3) {
+            if (cp) usf_mat<3, MPMGPU_SHAPE_LCPDI>(mat, g, p, b, nd, mats, only, slot, s);
+            else usf_mat<3, MPMGPU_SHAPE_UGIMP>(mat, g, p, b, nd, mats, only, slot, s);
+        } else {
+            if (cp) usf_mat<2, MPMGPU_SHAPE_LCPDI>(mat, g, p, b, nd, mats, only, slot, s);
+            else usf_mat<2, MPMGPU_SHAPE_UGIMP>(mat, g, p, b, nd, mats, only, slot, s);
+        }
+    });
 }
 
 // BoundaryCondition::GetNodeNum and BCValue (BoundaryCondition.cpp:72-140): 1 if the condition
@@ -3123,32 +3141,34 @@ void launch_node_mom2(int dim, const KGrid &g, const KBins &b, const KNodes &nd,
 
 template <int DIM, int SH>
 static void usl_reset_mat(int mat, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
-                          const MpmMaterial *mats, const KStats &st, hipStream_t s) {
-    const KBins tb = tile_bins(b, TILE_SLOT_USL);
+                          const MpmMaterial *mats, const KStats &st, int only, int slot, hipStream_t s) {
+    const KBins tb = tile_bins(b, slot);
     switch (mat) {
         case MPMGPU_MAT_ISOTROPIC:
-            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC, SH>), tb, s, g, p, tb, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_ISOTROPIC, SH>), tb, s, g, p, tb, nd, mats, st, only); break;
         case MPMGPU_MAT_ISOPLASTICITY:
-            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), tb, s, g, p, tb, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_ISOPLASTICITY, SH>), tb, s, g, p, tb, nd, mats, st, only); break;
         case MPMGPU_MAT_NEOHOOKEAN:
-            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), tb, s, g, p, tb, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_NEOHOOKEAN, SH>), tb, s, g, p, tb, nd, mats, st, only); break;
         case MPMGPU_MAT_MOONEY:
-            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_MOONEY, SH>), tb, s, g, p, tb, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, MPMGPU_MAT_MOONEY, SH>), tb, s, g, p, tb, nd, mats, st, only); break;
         default:
-            TILE_LAUNCH((k_usl_reset<DIM, 0, SH>), tb, s, g, p, tb, nd, mats, st); break;
+            TILE_LAUNCH((k_usl_reset<DIM, 0, SH>), tb, s, g, p, tb, nd, mats, st, only); break;
     }
 }
 
 void launch_usl_reset(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,
                       const MpmMaterial *mats, const KStats &st, hipStream_t s) {
     const bool cp = g.shape == MPMGPU_SHAPE_LCPDI;
-    if (dim == 3) {
-        if (cp) usl_reset_mat<3, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, st, s);
-        else usl_reset_mat<3, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, st, s);
-    } else {
-        if (cp) usl_reset_mat<2, MPMGPU_SHAPE_LCPDI>(g.matType, g, p, b, nd, mats, st, s);
-        else usl_reset_mat<2, MPMGPU_SHAPE_UGIMP>(g.matType, g, p, b, nd, mats, st, s);
-    }
+    by_type(g, TILE_SLOT_TYPED + 4, TILE_SLOT_USL, [&](int mat, int only, int slot) {
+        if (dim == 3) {
+            if (cp) usl_reset_mat<3, MPMGPU_SHAPE_LCPDI>(mat, g, p, b, nd, mats, st, only, slot, s);
+            else usl_reset_mat<3, MPMGPU_SHAPE_UGIMP>(mat, g, p, b, nd, mats, st, only, slot, s);
+        } else {
+            if (cp) usl_reset_mat<2, MPMGPU_SHAPE_LCPDI>(mat, g, p, b, nd, mats, st, only, slot, s);
+            else usl_reset_mat<2, MPMGPU_SHAPE_UGIMP>(mat, g, p, b, nd, mats, st, only, slot, s);
+        }
+    });
 }
 
 void launch_p2g_contact(int dim, const KGrid &g, const KPart &p, const KBins &b, const KNodes &nd,

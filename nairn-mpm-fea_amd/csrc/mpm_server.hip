@@ -164,6 +164,11 @@ int relayout(MpmServer *s, const std::vector<Request *> &members) {
     if (hipMemcpyAsync(nc->dinst, nc->inst.data(), ni * sizeof(KGrid), hipMemcpyHostToDevice, nc->stream) !=
         hipSuccess)
         return bail("instance table");
+    {
+        std::vector<int> types(ni);
+        for (size_t k = 0; k < ni; k++) types[k] = members[k]->matType;
+        if (mpmi::set_instance_types(nc, types)) return bail("instance types");
+    }
     if (old) mpmi::set_positions(old);
     std::vector<std::vector<int32_t>> staged;   // newcomers' int rows with batch material numbers
     staged.reserve(ni);
